@@ -1,0 +1,500 @@
+// cdx_api.hip -- context, state upload/download, Moreau position update and profiling.
+//
+// The state set here is what DynamicsAssembler stacks for a solver
+// (src/physics/assembly/dynamics_assembler.cpp:41-120,211-332): bodies in body order and a
+// contact buffer in Contact[] order.  Everything is copied once into HBM and stays there.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "cdx_common.hpp"
+
+namespace cdx {
+
+int set_err(cdx_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_check(cdx_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return CDX_OK;
+    return set_err(ctx, CDX_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+void prof_begin(cdx_ctx* ctx, const char* /*name*/, hipEvent_t* start) {
+    hipEvent_t e;
+    if (!ctx->event_pool.empty()) {
+        e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        hipEventCreate(&e);
+    }
+    hipEventRecord(e, ctx->stream);
+    *start = e;
+}
+
+void prof_end(cdx_ctx* ctx, const char* name, hipEvent_t start) {
+    hipEvent_t e;
+    if (!ctx->event_pool.empty()) {
+        e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        hipEventCreate(&e);
+    }
+    hipEventRecord(e, ctx->stream);
+    ctx->pending.push_back(PendingEvent{name, start, e});
+    if (ctx->pending.size() >= 4096) harvest_events(ctx);
+}
+
+int harvest_events(cdx_ctx* ctx) {
+    if (ctx->pending.empty()) return CDX_OK;
+    hipEventSynchronize(ctx->pending.back().stop);
+    for (auto& p : ctx->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.start, p.stop);
+        auto& s = ctx->stats[p.name];
+        s.calls += 1;
+        s.ms += ms;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+    return CDX_OK;
+}
+
+// Body -> incident contact-side CSR, sides listed in contact order so the per-body sum has a
+// fixed order.  Entry value = 2*c + side.
+int build_incidence(cdx_ctx* ctx) {
+    if (ctx->incidence_valid) return CDX_OK;
+    const int nb = ctx->nb, nc = ctx->nc;
+    std::vector<int> ptr(nb + 1, 0), idx(2 * (size_t)nc + 1);
+    for (int c = 0; c < nc; ++c) {
+        const int a = ctx->h_a[c], b = ctx->h_b[c];
+        if (a >= 0) ptr[a + 1]++;
+        if (b >= 0) ptr[b + 1]++;
+    }
+    for (int i = 0; i < nb; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int c = 0; c < nc; ++c) {
+        const int a = ctx->h_a[c], b = ctx->h_b[c];
+        if (a >= 0) idx[fill[a]++] = 2 * c;
+        if (b >= 0) idx[fill[b]++] = 2 * c + 1;
+    }
+    // d_incptr is sized with the body capacity, d_incidx with the contact capacity.
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incptr, ptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (nc > 0)
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incidx, idx.data(), sizeof(int) * 2 * (size_t)nc, hipMemcpyHostToDevice, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->incidence_valid = true;
+    return CDX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------------
+
+// f_gyro = -omega x (I omega), I = diag from the inverse inertia (world.cpp:152-173,
+// M = 1/Minv as rebuildMass_ does, dynamics_assembler.cpp:56-60).
+__global__ void k_gyro(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                       const double* __restrict__ v, double* __restrict__ fg) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double* o = fg + (size_t)b * kBodyStride;
+    for (int k = 0; k < kBodyStride; ++k) o[k] = 0.0;
+    if (dof[b] != 6) return;
+    const double* w = v + (size_t)b * kBodyStride + 3;
+    const double* mi = minv + (size_t)b * kMinvStride;
+    double I[3], Iw[3];
+    for (int k = 0; k < 3; ++k) I[k] = mi[1 + k] > 0 ? 1.0 / mi[1 + k] : 0.0;
+    for (int k = 0; k < 3; ++k) Iw[k] = I[k] * w[k];
+    o[3] = -(w[1] * Iw[2] - w[2] * Iw[1]);
+    o[4] = -(w[2] * Iw[0] - w[0] * Iw[2]);
+    o[5] = -(w[0] * Iw[1] - w[1] * Iw[0]);
+}
+
+// Explicit position update (integration_base.cpp:6-30) with alignQuaternionTo
+// (math_helper.hpp:11-17).  When `vsrc` differs from v, the solved velocities are first
+// written back as the body velocities (writeVelocityToSystem, dynamics_assembler.cpp:145-188).
+__global__ void k_position_update(int nb, const int* __restrict__ dof, double* __restrict__ q,
+                                  double* __restrict__ v, const double* __restrict__ vsrc, double h, int write_v) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double* u = v + (size_t)b * kBodyStride;
+    if (write_v) {
+        const double* s = vsrc + (size_t)b * kBodyStride;
+        for (int k = 0; k < 6; ++k) u[k] = s[k];
+    }
+    double* x = q + (size_t)b * kBodyStride;
+    x[0] += h * u[0];
+    x[1] += h * u[1];
+    x[2] += h * u[2];
+    if (dof[b] != 6) return;
+    const double w0 = x[6];
+    const double P0 = x[3], P1 = x[4], P2 = x[5];
+    const double wx = u[3], wy = u[4], wz = u[5];
+    const double hh = h * 0.5;
+    double nw = w0 - hh * (P0 * wx + P1 * wy + P2 * wz);
+    const double c0 = P1 * wz - P2 * wy;
+    const double c1 = P2 * wx - P0 * wz;
+    const double c2 = P0 * wy - P1 * wx;
+    double n0 = P0 + hh * (w0 * wx + c0);
+    double n1 = P1 + hh * (w0 * wy + c1);
+    double n2 = P2 + hh * (w0 * wz + c2);
+    if (!(isfinite(n0) && isfinite(n1) && isfinite(n2) && isfinite(nw))) return;  // keep q_prev
+    const double s2 = n0 * n0 + n1 * n1 + n2 * n2 + nw * nw;
+    if (s2 > 0) {
+        const double nn = sqrt(s2);
+        n0 /= nn;
+        n1 /= nn;
+        n2 /= nn;
+        nw /= nn;
+    }
+    if (P0 * n0 + P1 * n1 + P2 * n2 + w0 * nw < 0) {
+        n0 = -n0;
+        n1 = -n1;
+        n2 = -n2;
+        nw = -nw;
+    }
+    x[3] = n0;
+    x[4] = n1;
+    x[5] = n2;
+    x[6] = nw;
+}
+
+}  // namespace cdx
+
+using namespace cdx;
+
+extern "C" {
+
+void cdx_default_params(cdx_solver_params* p) {
+    if (!p) return;
+    p->alpha = 0.3;
+    p->relaxation = 1.0;
+    p->max_iter = 200000;
+    p->tol_abs = 1e-5;
+    p->tol_rel = 1e-5;
+    p->check_every = 10;
+    p->warmstart = 1;
+    p->accel = CDX_ACCEL_NONE;
+    p->nesterov_beta_threshold = 1.0;
+    p->nesterov_restart_limit = 10000;
+}
+
+int cdx_ctx_create(int device, cdx_ctx** out) {
+    if (!out) return CDX_BAD_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CDX_HIP_ERROR;
+    if (device < 0 || device >= n) return CDX_BAD_ARG;
+    cdx_ctx* ctx = new cdx_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CDX_HIP_ERROR;
+    }
+    if (hipMalloc((void**)&ctx->d_ctrl, 4 * sizeof(int)) != hipSuccess || hipMalloc((void**)&ctx->d_res, 4 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_flag, 4 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&ctx->h_flag_dev, ctx->h_flag, 0) != hipSuccess) {
+        cdx_ctx_destroy(ctx);
+        return CDX_HIP_ERROR;
+    }
+    *out = ctx;
+    return CDX_OK;
+}
+
+void cdx_ctx_destroy(cdx_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    void* bufs[] = {ctx->d_x, ctx->d_v, ctx->d_minv, ctx->d_fext, ctx->d_fgyro, ctx->d_q, ctx->d_uprev, ctx->d_ufree, ctx->d_dof,
+                    ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_p, ctx->d_R, ctx->d_bias,
+                    ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    if (ctx->h_flag) hipHostFree(ctx->h_flag);
+    for (auto& p : ctx->pending) {
+        hipEventDestroy(p.start);
+        hipEventDestroy(p.stop);
+    }
+    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* cdx_last_error(const cdx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cdx_synchronize(cdx_ctx* ctx) {
+    if (!ctx) return CDX_BAD_ARG;
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CDX_OK;
+}
+
+int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, const double* v, const double* fext,
+                   const double* fgyro, const double* q) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (nb < 0 || (nb > 0 && (!dof || !minv || !v || !fext))) return set_err(ctx, CDX_BAD_ARG, "cdx_set_bodies: null array");
+    hipSetDevice(ctx->device);
+    std::vector<int> voff(nb + 1), qoff(nb + 1);
+    int nv = 0, nq = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (dof[b] != 3 && dof[b] != 6) return set_err(ctx, CDX_BAD_ARG, "cdx_set_bodies: dof must be 3 or 6");
+        voff[b] = nv;
+        qoff[b] = nq;
+        nv += dof[b];
+        nq += dof[b] == 6 ? 7 : 3;
+    }
+    voff[nb] = nv;
+    qoff[nb] = nq;
+    std::vector<double> hx((size_t)nb * kBodyStride, 0.0), hf((size_t)nb * kBodyStride, 0.0),
+        hg((size_t)nb * kBodyStride, 0.0), hq((size_t)nb * kBodyStride, 0.0), hm((size_t)nb * kMinvStride, 0.0);
+    bool singular = false;
+    for (int b = 0; b < nb; ++b) {
+        const int o = voff[b], d = dof[b];
+        // The reference's mass diagonal has one mass on all translational dofs (world.cpp:49-64).
+        if (minv[o] != minv[o + 1] || minv[o] != minv[o + 2])
+            return set_err(ctx, CDX_BAD_ARG, "cdx_set_bodies: translational inverse mass must be isotropic");
+        hm[(size_t)b * kMinvStride] = minv[o];
+        for (int k = 0; k < d; ++k) {
+            if (!std::isfinite(minv[o + k]) || !std::isfinite(v[o + k]) || !std::isfinite(fext[o + k]) ||
+                (fgyro && !std::isfinite(fgyro[o + k])))
+                return set_err(ctx, CDX_NONFINITE, "cdx_set_bodies: non-finite body input");
+            if (!(minv[o + k] > 0)) singular = true;
+            hx[(size_t)b * kBodyStride + k] = v[o + k];
+            hf[(size_t)b * kBodyStride + k] = fext[o + k];
+            if (fgyro) hg[(size_t)b * kBodyStride + k] = fgyro[o + k];
+        }
+        if (d == 6)
+            for (int k = 0; k < 3; ++k) hm[(size_t)b * kMinvStride + 1 + k] = minv[o + 3 + k];
+        if (q) {
+            const int qo = qoff[b], nqb = d == 6 ? 7 : 3;
+            for (int k = 0; k < nqb; ++k) hq[(size_t)b * kBodyStride + k] = q[qo + k];
+        }
+    }
+    const bool resized = nb != ctx->nb;
+    ctx->nb = nb;
+    ctx->nv = nv;
+    ctx->dof.assign(dof, dof + nb);
+    ctx->voff = voff;
+    ctx->qoff = qoff;
+    ctx->have_q = q != nullptr;
+    ctx->singular_mass = singular;
+    const size_t n8 = (size_t)nb * kBodyStride;
+    size_t cap = ctx->body_cap;
+    // all body buffers share one capacity
+    if ((size_t)nb > ctx->body_cap || ctx->d_x == nullptr) {
+        double** bufs8[] = {&ctx->d_x, &ctx->d_v, &ctx->d_fext, &ctx->d_fgyro, &ctx->d_q, &ctx->d_uprev, &ctx->d_ufree};
+        for (double** p : bufs8) {
+            if (*p) hipFree(*p);
+            *p = nullptr;
+            CDX_TRY(ensure_alloc(ctx, p, n8, nullptr, "body state"));
+        }
+        if (ctx->d_minv) hipFree(ctx->d_minv);
+        ctx->d_minv = nullptr;
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_minv, (size_t)nb * kMinvStride, nullptr, "minv"));
+        if (ctx->d_dof) hipFree(ctx->d_dof);
+        ctx->d_dof = nullptr;
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_dof, (size_t)nb, nullptr, "dof"));
+        if (ctx->d_incptr) hipFree(ctx->d_incptr);
+        ctx->d_incptr = nullptr;
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_incptr, (size_t)nb + 1, nullptr, "incidence ptr"));
+        cap = nb;
+    }
+    ctx->body_cap = cap;
+    if (nb > 0) {
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_v, hx.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_x, hx.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_fext, hf.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_q, hq.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_minv, hm.data(), (size_t)nb * kMinvStride * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_dof, dof, (size_t)nb * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        if (fgyro) {
+            CDX_HIP(ctx, hipMemcpyAsync(ctx->d_fgyro, hg.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            CDX_LAUNCH(ctx, "gyro", k_gyro, grid_for(nb), kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fgyro);
+        }
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (resized) {
+        ctx->incidence_valid = false;
+        ctx->coloring_valid = false;
+    }
+    ctx->solved = false;
+    return CDX_OK;
+}
+
+int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b, const double* mu, const double* frame,
+                     const double* angA, const double* angB, const double* rest, const double* vbias, const double* warm) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (nc < 0 || (nc > 0 && (!body_a || !body_b || !mu || !frame || !angA || !angB || !rest || !vbias || !warm)))
+        return set_err(ctx, CDX_BAD_ARG, "cdx_set_contacts: null array");
+    hipSetDevice(ctx->device);
+    int nrows = 0;
+    for (int c = 0; c < nc; ++c) {
+        const int a = body_a[c], b = body_b[c];
+        if (a < -1 || a >= ctx->nb || b < -1 || b >= ctx->nb || (a >= 0 && a == b))
+            return set_err(ctx, CDX_BAD_ARG, "cdx_set_contacts: body index out of range (bodies must be set first)");
+        if (!std::isfinite(mu[c])) return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite mu");
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(frame[9 * (size_t)c + k]) || !std::isfinite(angA[9 * (size_t)c + k]) || !std::isfinite(angB[9 * (size_t)c + k]))
+                return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite contact frame");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(rest[3 * (size_t)c + k]) || !std::isfinite(vbias[3 * (size_t)c + k]) || !std::isfinite(warm[3 * (size_t)c + k]))
+                return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite contact data");
+        if (a >= 0 || b >= 0) nrows += mu[c] > 0 ? 3 : 1;
+    }
+    // Component-major SoA staging.  Static-static contacts carry no rows (rebuildW_, :259-260):
+    // both ids become -1 and the kernels skip them.
+    std::vector<double> J((size_t)kJ * nc), m(nc), r(3 * (size_t)nc), vb(3 * (size_t)nc), w(3 * (size_t)nc);
+    std::vector<int> ia(body_a, body_a + nc), ib(body_b, body_b + nc);
+    bool any_rest = false;
+    for (int c = 0; c < nc; ++c) {
+        for (int k = 0; k < 9; ++k) {
+            J[(size_t)k * nc + c] = frame[9 * (size_t)c + k];
+            J[(size_t)(9 + k) * nc + c] = ia[c] >= 0 ? angA[9 * (size_t)c + k] : 0.0;
+            J[(size_t)(18 + k) * nc + c] = ib[c] >= 0 ? angB[9 * (size_t)c + k] : 0.0;
+        }
+        // point masses have no angular rows (buildContactRowByDof, dof 3)
+        if (ia[c] >= 0 && ctx->dof[ia[c]] == 3)
+            for (int k = 0; k < 9; ++k) J[(size_t)(9 + k) * nc + c] = 0.0;
+        if (ib[c] >= 0 && ctx->dof[ib[c]] == 3)
+            for (int k = 0; k < 9; ++k) J[(size_t)(18 + k) * nc + c] = 0.0;
+        const bool fr = mu[c] > 0;
+        m[c] = fr ? mu[c] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const bool row = k == 0 || fr;
+            r[(size_t)k * nc + c] = row ? rest[3 * (size_t)c + k] : 0.0;
+            if (row && rest[3 * (size_t)c + k] != 0.0 && (ia[c] >= 0 || ib[c] >= 0)) any_rest = true;
+            vb[(size_t)k * nc + c] = row ? vbias[3 * (size_t)c + k] : 0.0;
+            w[(size_t)k * nc + c] = row ? warm[3 * (size_t)c + k] : 0.0;
+        }
+    }
+    if ((size_t)nc > ctx->contact_cap || !ctx->d_ida) {
+        int** ibufs[] = {&ctx->d_ida, &ctx->d_idb};
+        for (int** p : ibufs) {
+            if (*p) hipFree(*p);
+            *p = nullptr;
+            CDX_TRY(ensure_alloc(ctx, p, (size_t)nc, nullptr, "contact ids"));
+        }
+        struct {
+            double** p;
+            size_t n;
+        } dbufs[] = {{&ctx->d_J, (size_t)kJ * nc}, {&ctx->d_mu, (size_t)nc},        {&ctx->d_rest, 3 * (size_t)nc},
+                     {&ctx->d_vb, 3 * (size_t)nc}, {&ctx->d_p, 3 * (size_t)nc},   {&ctx->d_R, 3 * (size_t)nc},
+                     {&ctx->d_bias, 3 * (size_t)nc}, {&ctx->d_imp, 12 * (size_t)nc}};
+        for (auto& d : dbufs) {
+            if (*d.p) hipFree(*d.p);
+            *d.p = nullptr;
+            CDX_TRY(ensure_alloc(ctx, d.p, d.n, nullptr, "contact buffer"));
+        }
+        if (ctx->d_incidx) hipFree(ctx->d_incidx);
+        ctx->d_incidx = nullptr;
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_incidx, 2 * (size_t)nc, nullptr, "incidence"));
+        ctx->contact_cap = nc;
+    }
+    for (int c = 0; c < nc; ++c)
+        if (ia[c] < 0 && ib[c] < 0) ia[c] = ib[c] = -1;
+    ctx->nc = nc;
+    ctx->nrows = nrows;
+    ctx->any_rest = any_rest;
+    ctx->h_a = ia;
+    ctx->h_b = ib;
+    ctx->h_mu.assign(m.begin(), m.end());
+    if (nc > 0) {
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_ida, ia.data(), sizeof(int) * nc, hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_idb, ib.data(), sizeof(int) * nc, hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_J, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_mu, m.data(), sizeof(double) * nc, hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rest, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_vb, vb.data(), sizeof(double) * vb.size(), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->incidence_valid = false;
+    ctx->coloring_valid = false;
+    ctx->solved = false;
+    return build_incidence(ctx);
+}
+
+int cdx_get_velocities(cdx_ctx* ctx, double* v_out) {
+    if (!ctx || !v_out) return CDX_BAD_ARG;
+    if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_velocities: no solve yet");
+    std::vector<double> hx((size_t)ctx->nb * kBodyStride);
+    if (ctx->nb)
+        CDX_HIP(ctx, hipMemcpyAsync(hx.data(), ctx->d_x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < ctx->nb; ++b)
+        for (int k = 0; k < ctx->dof[b]; ++k) v_out[ctx->voff[b] + k] = hx[(size_t)b * kBodyStride + k];
+    return CDX_OK;
+}
+
+int cdx_get_impulses(cdx_ctx* ctx, double* imp_out) {
+    if (!ctx || !imp_out) return CDX_BAD_ARG;
+    if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_impulses: no solve yet");
+    const int nc = ctx->nc;
+    std::vector<double> hp(3 * (size_t)nc);
+    if (nc) CDX_HIP(ctx, hipMemcpyAsync(hp.data(), ctx->d_p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 3; ++k) imp_out[3 * (size_t)c + k] = hp[(size_t)k * nc + c];
+    return CDX_OK;
+}
+
+int cdx_get_positions(cdx_ctx* ctx, double* q_out) {
+    if (!ctx || !q_out) return CDX_BAD_ARG;
+    if (!ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_get_positions: no positions were set");
+    std::vector<double> hq((size_t)ctx->nb * kBodyStride);
+    if (ctx->nb)
+        CDX_HIP(ctx, hipMemcpyAsync(hq.data(), ctx->d_q, hq.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < ctx->nb; ++b) {
+        const int nq = ctx->dof[b] == 6 ? 7 : 3;
+        for (int k = 0; k < nq; ++k) q_out[ctx->qoff[b] + k] = hq[(size_t)b * kBodyStride + k];
+    }
+    return CDX_OK;
+}
+
+int cdx_moreau_predrift(cdx_ctx* ctx, double dt, double theta) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (!ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_moreau_predrift: no positions were set");
+    const double h = (1.0 - theta) * dt;
+    if (h != 0.0 && ctx->nb > 0)
+        CDX_LAUNCH(ctx, "moreau_drift", k_position_update, grid_for(ctx->nb), kBlock, ctx->nb, ctx->d_dof, ctx->d_q, ctx->d_v,
+                   ctx->d_v, h, 0);
+    CDX_HIP(ctx, hipGetLastError());
+    return CDX_OK;
+}
+
+int cdx_moreau_update(cdx_ctx* ctx, double dt, double theta) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (!ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_moreau_update: no positions were set");
+    if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_moreau_update: no solve yet");
+    if (ctx->nb > 0) {
+        CDX_LAUNCH(ctx, "moreau_update", k_position_update, grid_for(ctx->nb), kBlock, ctx->nb, ctx->d_dof, ctx->d_q, ctx->d_v,
+                   ctx->d_x, theta * dt, 1);
+        CDX_LAUNCH(ctx, "gyro", k_gyro, grid_for(ctx->nb), kBlock, ctx->nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fgyro);
+    }
+    CDX_HIP(ctx, hipGetLastError());
+    return CDX_OK;
+}
+
+int cdx_set_profiling(cdx_ctx* ctx, int enable) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (!enable) harvest_events(ctx);
+    ctx->profiling = enable != 0;
+    return CDX_OK;
+}
+
+int cdx_kernel_time(cdx_ctx* ctx, const char* kernel, int* calls, double* total_ms) {
+    if (!ctx || !kernel) return CDX_BAD_ARG;
+    harvest_events(ctx);
+    auto it = ctx->stats.find(kernel);
+    if (calls) *calls = it == ctx->stats.end() ? 0 : it->second.calls;
+    if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.ms;
+    if (it != ctx->stats.end()) ctx->stats.erase(it);
+    return CDX_OK;
+}
+
+}  // extern "C"

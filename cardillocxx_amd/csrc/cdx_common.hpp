@@ -1,0 +1,195 @@
+// cdx_common.hpp -- internal context and helpers of the gfx950 contact solve library.
+//
+// Device data layout (all FP64, HBM-resident for the life of a context):
+//   bodies   : x/v/fext/fgyro/q padded to 8 doubles (64 B) per body so that a contact gathers
+//              one body twist with three 16-byte loads from one 64-byte segment;
+//              minv packed as 4 doubles (m^-1, Ixx^-1, Iyy^-1, Izz^-1).
+//   contacts : structure of arrays, component-major ([k * nc + c]) so that every per-contact
+//              scalar is one fully coalesced 512-byte wave load:
+//              J[27][nc] (frame 9 | angA 9 | angB 9), R[3][nc], bias[3][nc], mu[nc], p[3][nc].
+//   scatter  : imp[(2c + side) * 6 + k], the generalised impulse J_side^T dp of one contact side,
+//              written by the contact pass and gathered per body through a CSR incidence list
+//              (inc_ptr/inc_idx) so the body pass needs no atomics and sums in a fixed order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cdx/cdx.h"
+
+namespace cdx {
+
+constexpr int kBodyStride = 8;  // doubles per padded body vector
+constexpr int kMinvStride = 4;  // doubles per body inverse mass
+constexpr int kJ = 27;          // Jacobian doubles per contact
+constexpr int kBlock = 256;     // threads per workgroup (4 wave64)
+
+struct KernelStat {
+    int calls = 0;
+    double ms = 0;
+};
+
+struct PendingEvent {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace cdx
+
+struct cdx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- bodies
+    int nb = 0, nv = 0;
+    std::vector<int> dof, voff, qoff;
+    bool have_q = false;
+    bool singular_mass = false;  // a dynamic dof with zero mass: S not factorizable
+    double* d_x = nullptr;       // solver velocity state [nb*8]
+    double* d_v = nullptr;       // body velocity v_n [nb*8]
+    double* d_minv = nullptr;    // [nb*4]
+    double* d_fext = nullptr;    // [nb*8]
+    double* d_fgyro = nullptr;   // [nb*8]
+    double* d_q = nullptr;       // [nb*8]
+    double* d_uprev = nullptr;   // [nb*8] previous iterate (GS residual)
+    double* d_ufree = nullptr;   // [nb*8]
+    int* d_dof = nullptr;        // [nb]
+    size_t body_cap = 0;
+
+    // ---- contacts
+    int nc = 0;
+    std::vector<int> h_a, h_b;
+    std::vector<double> h_mu;
+    int nrows = 0;
+    bool any_rest = false;  // any non-zero restitution row (PgsAssembler::rhs, pgs_assembler.cpp:64-67)
+    int* d_ida = nullptr;
+    int* d_idb = nullptr;
+    double* d_J = nullptr;     // [27*nc]
+    double* d_mu = nullptr;    // [nc]
+    double* d_rest = nullptr;  // [3*nc]
+    double* d_vb = nullptr;    // [3*nc]
+    double* d_p = nullptr;     // [3*nc] impulses (PJ: normal >= 0, GS: lambda <= 0 during the solve)
+    double* d_R = nullptr;     // [3*nc]
+    double* d_bias = nullptr;  // [3*nc]
+    double* d_imp = nullptr;   // [12*nc]
+    int* d_incptr = nullptr;   // [nb+1]
+    int* d_incidx = nullptr;   // [2*nc]
+    size_t contact_cap = 0;
+    bool incidence_valid = false;
+
+    // ---- colored GS topology (rebuilt when contacts change)
+    bool coloring_valid = false;
+    int ncolors = 0;
+    std::vector<int> color_ptr;  // [ncolors+1] into the colour-sorted block list
+    int* d_color_blocks = nullptr;  // [nblocks] contact index per colour-sorted block
+    size_t color_cap = 0;
+
+    // ---- solve control
+    int* d_ctrl = nullptr;         // [0] done flag, [1] iterations, [2] status
+    double* d_res = nullptr;       // [0] last residual
+    double* d_partial = nullptr;   // per-block residual partial sums
+    size_t partial_cap = 0;
+    int* h_flag = nullptr;         // pinned mirror of the done flag
+    int* h_flag_dev = nullptr;
+    bool solved = false;
+
+    // ---- profiling
+    bool profiling = false;
+    std::vector<cdx::PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, cdx::KernelStat> stats;
+};
+
+namespace cdx {
+
+int set_err(cdx_ctx* ctx, int code, const std::string& msg);
+int hip_check(cdx_ctx* ctx, hipError_t e, const char* what);
+template <typename T>
+int ensure_alloc(cdx_ctx* ctx, T** ptr, size_t count, size_t* cap_or_null, const char* what);
+void prof_begin(cdx_ctx* ctx, const char* name, hipEvent_t* start);
+void prof_end(cdx_ctx* ctx, const char* name, hipEvent_t start);
+int harvest_events(cdx_ctx* ctx);
+int build_incidence(cdx_ctx* ctx);
+int grid_for(int n);
+int reset_control(cdx_ctx* ctx);
+int read_residual(cdx_ctx* ctx, double* r);
+int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
+// Reduces ctx->d_partial[0:nparts) into the residual; mode 0 = PJ test (res <= 1),
+// mode 1 = Gauss-Seidel test (res < 1, NaN/Inf -> CDX_DIVERGED).
+int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode);
+
+// Launch helper: brackets the launch with profiling events when enabled.
+#define CDX_LAUNCH(ctx, name, kernel, grid, block, ...)                                  \
+    do {                                                                                  \
+        hipEvent_t cdx_ev_start__ = nullptr;                                              \
+        if ((ctx)->profiling) ::cdx::prof_begin((ctx), (name), &cdx_ev_start__);          \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__); \
+        if ((ctx)->profiling) ::cdx::prof_end((ctx), (name), cdx_ev_start__);             \
+    } while (0)
+
+#define CDX_TRY(expr)               \
+    do {                            \
+        int cdx_st__ = (expr);      \
+        if (cdx_st__) return cdx_st__; \
+    } while (0)
+
+#define CDX_HIP(ctx, expr) CDX_TRY(::cdx::hip_check((ctx), (expr), #expr))
+
+template <typename T>
+int ensure_alloc(cdx_ctx* ctx, T** ptr, size_t count, size_t* cap, const char* what) {
+    if (count == 0) count = 1;
+    if (*ptr && cap && *cap >= count) return CDX_OK;
+    if (*ptr && !cap) return CDX_OK;
+    if (*ptr) {
+        hipFree(*ptr);
+        *ptr = nullptr;
+    }
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+    if (e != hipSuccess) return set_err(ctx, CDX_HIP_ERROR, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+    if (cap) *cap = count;
+    return CDX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device helpers
+// ------------------------------------------------------------------------------------------
+
+// Wave64 + workgroup sum of one double per thread.  Result valid in thread 0.  Fixed tree
+// order, so the value is deterministic for a given launch shape.
+__device__ inline double block_sum(double v, double* lds /* >= blockDim.x/64 */) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) s += lds[w];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Maps the hardware workgroup id to a logical tile so that each of the 8 XCDs (which receive
+// workgroups round-robin) walks one contiguous range of the spatially sorted contact list,
+// keeping neighbouring contacts, and the bodies they share, in one XCD's L2.
+__device__ inline int xcd_tile(int nblocks) {
+    const int hw = blockIdx.x;
+    const int per = (nblocks + 7) >> 3;
+    const int xcd = hw & 7, slot = hw >> 3;
+    const int full = nblocks - (per - 1) * 8;  // number of XCDs that get `per` tiles
+    int logical;
+    if (xcd < full || per * 8 == nblocks) {
+        logical = xcd * per + slot;
+    } else {
+        logical = full * per + (xcd - full) * (per - 1) + slot;
+    }
+    return logical;
+}
+
+}  // namespace cdx

@@ -1,0 +1,119 @@
+// cdx_device.hpp -- device helpers shared by the solve kernels, and the host sweep driver.
+#pragma once
+
+#include <algorithm>
+#include <deque>
+
+#include "cdx_common.hpp"
+
+namespace cdx {
+
+__device__ inline void load_body(const double* __restrict__ x, int b, double u[6]) {
+    if (b < 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) u[k] = 0.0;
+        return;
+    }
+    const double2* p = reinterpret_cast<const double2*>(x + (size_t)b * kBodyStride);
+    const double2 a0 = p[0], a1 = p[1], a2 = p[2];
+    u[0] = a0.x;
+    u[1] = a0.y;
+    u[2] = a1.x;
+    u[3] = a1.y;
+    u[4] = a2.x;
+    u[5] = a2.y;
+}
+
+__device__ inline void store6(double* __restrict__ dst, const double v[6]) {
+    double2* p = reinterpret_cast<double2*>(dst);
+    p[0] = make_double2(v[0], v[1]);
+    p[1] = make_double2(v[2], v[3]);
+    p[2] = make_double2(v[4], v[5]);
+}
+
+// Row k of W_c times the stacked twist: side A carries sign -1, side B +1
+// (dynamics_assembler.cpp:295-296; contact_jacobian.hpp:17-23).
+__device__ inline double row_dot(const double d[3], const double aA[3], const double aB[3], const double ua[6],
+                                 const double ub[6]) {
+    const double sA = d[0] * ua[0] + d[1] * ua[1] + d[2] * ua[2] + aA[0] * ua[3] + aA[1] * ua[4] + aA[2] * ua[5];
+    const double sB = d[0] * ub[0] + d[1] * ub[1] + d[2] * ub[2] + aB[0] * ub[3] + aB[1] * ub[4] + aB[2] * ub[5];
+    return sB - sA;
+}
+
+struct ContactJ {
+    double d[3][3], aA[3][3], aB[3][3];
+};
+
+__device__ inline void load_J(const double* __restrict__ J, int nc, int c, int nr, ContactJ& j) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            j.d[k][m] = J[(size_t)(3 * k + m) * nc + c];
+            j.aA[k][m] = J[(size_t)(9 + 3 * k + m) * nc + c];
+            j.aB[k][m] = J[(size_t)(18 + 3 * k + m) * nc + c];
+        }
+    }
+}
+
+// Side impulses J_A^T dp = -[sum d_k dp_k ; sum angA_k dp_k], J_B^T dp = +[... ; sum angB_k dp_k].
+__device__ inline void write_side_impulses(double* __restrict__ imp, int c, int a, int b, int nr, const ContactJ& j,
+                                           const double dp[3]) {
+    double lin[3] = {0, 0, 0}, angA[3] = {0, 0, 0}, angB[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            lin[m] += j.d[k][m] * dp[k];
+            angA[m] += j.aA[k][m] * dp[k];
+            angB[m] += j.aB[k][m] * dp[k];
+        }
+    }
+    if (a >= 0) {
+        const double va[6] = {-lin[0], -lin[1], -lin[2], -angA[0], -angA[1], -angA[2]};
+        store6(imp + (size_t)(2 * c) * 6, va);
+    }
+    if (b >= 0) {
+        const double vb[6] = {lin[0], lin[1], lin[2], angB[0], angB[1], angB[2]};
+        store6(imp + (size_t)(2 * c + 1) * 6, vb);
+    }
+}
+
+// Runs the sweep loop of a solve: enqueues sweeps in chunks (aligned to the residual cadence) and
+// keeps at most `window` chunks in flight, polling the pinned done flag.  Enqueued sweeps after
+// convergence return on their first instruction.
+template <typename SweepFn>
+int run_sweeps(cdx_ctx* ctx, int max_iter, int chunk, int window, SweepFn&& sweep, int* iters_out) {
+    std::deque<hipEvent_t> inflight;
+    int it = 0;
+    bool done = false;
+    while (it < max_iter && !done) {
+        const int end = std::min(max_iter, it + chunk);
+        for (; it < end; ++it) CDX_TRY(sweep(it));
+        hipEvent_t ev;
+        CDX_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        CDX_HIP(ctx, hipEventRecord(ev, ctx->stream));
+        inflight.push_back(ev);
+        if ((int)inflight.size() >= window) {
+            hipEvent_t e = inflight.front();
+            inflight.pop_front();
+            CDX_HIP(ctx, hipEventSynchronize(e));
+            hipEventDestroy(e);
+            if (__atomic_load_n(&ctx->h_flag[0], __ATOMIC_ACQUIRE)) done = true;
+        }
+    }
+    for (hipEvent_t e : inflight) {
+        hipEventSynchronize(e);
+        hipEventDestroy(e);
+    }
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int ctrl[3] = {0, 0, 0};
+    CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    *iters_out = ctrl[0] ? ctrl[1] : max_iter;
+    return ctrl[0] ? ctrl[2] : CDX_OK;
+}
+
+
+}  // namespace cdx

@@ -1,0 +1,65 @@
+/*
+ * cdx_scene.h -- CPU contact generation feeding the device contact buffer.
+ *
+ * Contact generation stays on the CPU (the reference runs it through Coal in
+ * CollisionCoal::detectAll, src/collision/collision_coal.cpp:405-548).  These functions
+ * produce contacts with the reference's conventions:
+ *   - normal from A to B, contact point at the midpoint of the two witness points, tangents
+ *     from tangentFrameFromNormal (tangent_frame.hpp:23-50), body-frame copies through R^T
+ *     (makeContact, collision_coal.cpp:112-163);
+ *   - friction combined per pair with "min" (combineFrictionMu, :94-106);
+ *   - angular Jacobian rows r_body x d_body per side, the input of buildContactRowByDof
+ *     (contact_jacobian.hpp:17-23);
+ *   - warm start by greedy nearest-point matching per body pair within a maximum distance
+ *     (ContactTracker::matchContactsForPair, contact_tracker.cpp:43-72).
+ * Output arrays follow the layout of include/cdx/cdx.h.
+ */
+#ifndef CDX_SCENE_H
+#define CDX_SCENE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Right-handed orthonormal (t1, t2) from a contact normal (tangent_frame.hpp:23-50). */
+void cdx_tangent_frame(const double n[3], double t1[3], double t2[3]);
+
+/* Sphere / static half-space contacts.
+ *   q[7*nb]       : x,y,z, qx,qy,qz,qw per sphere (rigid bodies);
+ *   radius[nb], mu_body[nb];
+ *   walls         : nw half-spaces {x : n.x >= d} (n[3*nw] unit inward normal, d[nw]), mu_wall[nw];
+ *   margin        : report pairs with gap <= margin (the collision security margin,
+ *                   config.hpp:31);
+ * Contacts are emitted per body a (ascending): walls first, then spheres b > a (ascending).
+ * Sphere-wall contacts have A = the wall (-1), B = the sphere.
+ * Writes at most cap contacts; returns the number found (which may exceed cap) or -1.
+ * point[3*c] and depth[c] (penetration >= 0) are written for warm-start matching. */
+int cdx_sphere_contacts(int nb, const double* q, const double* radius, const double* mu_body, int nw,
+                        const double* wall_n, const double* wall_d, const double* mu_wall, double margin, int cap,
+                        int* body_a, int* body_b, double* mu, double* frame, double* angA, double* angB,
+                        double* point, double* depth);
+
+/* Warm start: for each current contact, the impulse of the nearest unused previous contact of
+ * the same body pair within max_dist, greedily by increasing distance
+ * (contact_tracker.cpp:43-72); zero if none. */
+int cdx_match_warmstart(int nprev, const int* prev_a, const int* prev_b, const double* prev_point, const double* prev_imp,
+                        int ncur, const int* cur_a, const int* cur_b, const double* cur_point, double max_dist,
+                        double* warm_out);
+
+/* Jittered-grid sphere drop (1000 spheres in a 1x1x2 box): positions on an nx x ny x nz
+ * lattice of pitch `pitch`, first layer centre at z0, uniform jitter +-jitter in x and y;
+ * random orientations.  Returns the count placed. */
+int cdx_scene_sphere_grid(int n, int nx, int ny, double pitch, double z0, double jitter, uint64_t seed, double* q);
+
+/* Settled polydisperse sphere pile: radii log-uniform in [rmin, rmax], random sequential
+ * placement in the box [0,Lx]x[0,Ly]x[0,Lz], then a position-based gravity settle of
+ * `settle_steps` steps.  Bodies are returned in Morton (spatial) order.  Returns n or -1. */
+int cdx_scene_sphere_pile(int n, double rmin, double rmax, double Lx, double Ly, double Lz, uint64_t seed, int settle_steps,
+                          double* q, double* radius);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,264 @@
+"""ctypes bindings of the C ABI (include/cdx/cdx.h) and the CPU contact front end.
+
+The device library must be built in-tree (``python -m cardillocxx_amd.build``); loading fails
+loudly when it is missing -- there is no CPU fallback for the solve.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcdx.so")
+SCENE_PATH = os.path.join(PKG, "libcdx_scene.so")
+
+CDX_OK = 0
+CDX_NOT_FACTORIZED = 1
+CDX_NONFINITE = 2
+CDX_DIVERGED = 3
+CDX_BAD_ARG = 4
+CDX_HIP_ERROR = 5
+CDX_NO_STATE = 6
+STATUS_NAMES = {
+    0: "CDX_OK",
+    1: "CDX_NOT_FACTORIZED",
+    2: "CDX_NONFINITE",
+    3: "CDX_DIVERGED",
+    4: "CDX_BAD_ARG",
+    5: "CDX_HIP_ERROR",
+    6: "CDX_NO_STATE",
+}
+CDX_ACCEL_NONE = 0
+CDX_ACCEL_NESTEROV = 1
+
+# Every symbol include/cdx/cdx.h declares.
+EXPORTED = [
+    "cdx_default_params",
+    "cdx_ctx_create",
+    "cdx_ctx_destroy",
+    "cdx_last_error",
+    "cdx_synchronize",
+    "cdx_set_bodies",
+    "cdx_set_contacts",
+    "cdx_pj_solve",
+    "cdx_gs_colored_solve",
+    "cdx_get_velocities",
+    "cdx_get_impulses",
+    "cdx_get_positions",
+    "cdx_moreau_predrift",
+    "cdx_moreau_update",
+    "cdx_set_profiling",
+    "cdx_kernel_time",
+]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_double),
+        ("relaxation", C.c_double),
+        ("max_iter", C.c_int),
+        ("tol_abs", C.c_double),
+        ("tol_rel", C.c_double),
+        ("check_every", C.c_int),
+        ("warmstart", C.c_int),
+        ("accel", C.c_int),
+        ("nesterov_beta_threshold", C.c_double),
+        ("nesterov_restart_limit", C.c_int),
+    ]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("residual", C.c_double), ("nrows", C.c_int), ("ncolors", C.c_int)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+_lib = None
+_scene = None
+
+
+def _declare(lib):
+    sig = {
+        "cdx_default_params": (None, [C.POINTER(SolverParams)]),
+        "cdx_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "cdx_ctx_destroy": (None, [_vp]),
+        "cdx_last_error": (C.c_char_p, [_vp]),
+        "cdx_synchronize": (C.c_int, [_vp]),
+        "cdx_set_bodies": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
+        "cdx_set_contacts": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "cdx_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), C.POINTER(SolveInfo)]),
+        "cdx_gs_colored_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), C.POINTER(SolveInfo)]),
+        "cdx_get_velocities": (C.c_int, [_vp, _dp]),
+        "cdx_get_impulses": (C.c_int, [_vp, _dp]),
+        "cdx_get_positions": (C.c_int, [_vp, _dp]),
+        "cdx_moreau_predrift": (C.c_int, [_vp, C.c_double, C.c_double]),
+        "cdx_moreau_update": (C.c_int, [_vp, C.c_double, C.c_double]),
+        "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
+        "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """The HIP device library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m cardillocxx_amd.build`")
+        _lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        _declare(_lib)
+    return _lib
+
+
+def scene_lib():
+    global _scene
+    if _scene is None:
+        if not os.path.exists(SCENE_PATH):
+            raise RuntimeError(f"{SCENE_PATH} is missing: build it with `python -m cardillocxx_amd.build`")
+        _scene = C.CDLL(SCENE_PATH)
+        _scene.cdx_tangent_frame.restype = None
+        _scene.cdx_tangent_frame.argtypes = [_dp, _dp, _dp]
+        _scene.cdx_sphere_contacts.restype = C.c_int
+        _scene.cdx_sphere_contacts.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                                               _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        _scene.cdx_match_warmstart.restype = C.c_int
+        _scene.cdx_match_warmstart.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _dp, C.c_double, _dp]
+        _scene.cdx_scene_sphere_grid.restype = C.c_int
+        _scene.cdx_scene_sphere_grid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                 C.c_uint64, _dp]
+        _scene.cdx_scene_sphere_pile.restype = C.c_int
+        _scene.cdx_scene_sphere_pile.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                 C.c_uint64, C.c_int, _dp, _dp]
+    return _scene
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous array required"
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "int32 C-contiguous array required"
+    return a.ctypes.data_as(_ip)
+
+
+def default_params() -> SolverParams:
+    p = SolverParams()
+    lib().cdx_default_params(C.byref(p))
+    return p
+
+
+class CdxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Context:
+    """One device context (one HIP stream) -- a thin owner of ``cdx_ctx*``."""
+
+    def __init__(self, device: int = 0):
+        self._l = lib()
+        h = _vp()
+        st = self._l.cdx_ctx_create(device, C.byref(h))
+        if st != CDX_OK:
+            raise CdxError(st, f"cdx_ctx_create(device={device}) failed")
+        self.h = h
+        self.nb = 0
+        self.nv = 0
+        self.nq = 0
+        self.nc = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._l.cdx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int, what: str):
+        if st != CDX_OK:
+            raise CdxError(st, f"{what}: {self._l.cdx_last_error(self.h).decode()}")
+
+    def set_bodies(self, dof, minv, v, fext, fgyro=None, q=None):
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (minv, v, fext)]
+        fg = None if fgyro is None else np.ascontiguousarray(fgyro, dtype=np.float64)
+        qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+        self._check(self._l.cdx_set_bodies(self.h, len(dof), iptr(dof), *[dptr(a) for a in arr], dptr(fg), dptr(qq)),
+                    "cdx_set_bodies")
+        self.nb = len(dof)
+        self.nv = int(dof.sum())
+        self.nq = int(np.where(dof == 6, 7, 3).sum())
+
+    def set_contacts(self, body_a, body_b, mu, frame, angA, angB, rest, vbias, warm):
+        ia = np.ascontiguousarray(body_a, dtype=np.int32)
+        ib = np.ascontiguousarray(body_b, dtype=np.int32)
+        d = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (mu, frame, angA, angB, rest, vbias, warm)]
+        self._check(self._l.cdx_set_contacts(self.h, len(ia), iptr(ia), iptr(ib), *[dptr(x) for x in d]),
+                    "cdx_set_contacts")
+        self.nc = len(ia)
+
+    def _solve(self, fn, dt, theta, params):
+        info = SolveInfo()
+        st = fn(self.h, dt, theta, C.byref(params), C.byref(info))
+        return st, info
+
+    def pj_solve(self, dt, theta, params):
+        st, info = self._solve(self._l.cdx_pj_solve, dt, theta, params)
+        self._check(st, "cdx_pj_solve")
+        return info
+
+    def gs_colored_solve(self, dt, theta, params, raise_on_divergence=True):
+        st, info = self._solve(self._l.cdx_gs_colored_solve, dt, theta, params)
+        if st != CDX_OK and (raise_on_divergence or st != CDX_DIVERGED):
+            self._check(st, "cdx_gs_colored_solve")
+        return info
+
+    def velocities(self):
+        out = np.empty(self.nv, dtype=np.float64)
+        self._check(self._l.cdx_get_velocities(self.h, dptr(out)), "cdx_get_velocities")
+        return out
+
+    def impulses(self):
+        out = np.empty((self.nc, 3), dtype=np.float64)
+        self._check(self._l.cdx_get_impulses(self.h, dptr(out)), "cdx_get_impulses")
+        return out
+
+    def positions(self):
+        out = np.empty(self.nq, dtype=np.float64)
+        self._check(self._l.cdx_get_positions(self.h, dptr(out)), "cdx_get_positions")
+        return out
+
+    def moreau_predrift(self, dt, theta):
+        self._check(self._l.cdx_moreau_predrift(self.h, dt, theta), "cdx_moreau_predrift")
+
+    def moreau_update(self, dt, theta):
+        self._check(self._l.cdx_moreau_update(self.h, dt, theta), "cdx_moreau_update")
+
+    def synchronize(self):
+        self._check(self._l.cdx_synchronize(self.h), "cdx_synchronize")
+
+    def set_profiling(self, on: bool):
+        self._check(self._l.cdx_set_profiling(self.h, 1 if on else 0), "cdx_set_profiling")
+
+    def kernel_time(self, name: str):
+        calls = C.c_int(0)
+        ms = C.c_double(0)
+        self._check(self._l.cdx_kernel_time(self.h, name.encode(), C.byref(calls), C.byref(ms)), "cdx_kernel_time")
+        return calls.value, ms.value

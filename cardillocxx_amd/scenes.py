@@ -1,0 +1,286 @@
+"""Scene builders and the contact problem container.
+
+A :class:`ContactProblem` holds exactly what the reference's DynamicsAssembler hands a solver
+(src/physics/assembly/dynamics_assembler.hpp:24-103): stacked body state and the contact buffer
+in the layout of include/cdx/cdx.h.  Contact generation runs on the CPU (libcdx_scene.so).
+
+Scenes (seeded, deterministic):
+  * ``sphere_drop_grid``  -- 1000 spheres r=0.05 m, m=1 kg on a jittered 8x8 grid (pitch 0.12 m,
+    16 layers) dropped into a 1x1x2 m box of static half-spaces (the plain CPU-runnable case).
+  * ``sphere_pile``       -- 100k spheres, radius log-uniform in [0.02, 0.08] m, 1000 kg/m^3,
+    random sequential placement in a 4x4x8 m box then a gravity settle; mu ~ U[0.3, 0.8] per body.
+  * ``random_problem``    -- small random contact problems (test inputs).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import native
+
+GRAVITY = np.array([0.0, 0.0, -9.81])
+
+
+@dataclasses.dataclass
+class ContactProblem:
+    dof: np.ndarray  # int32 [nb]
+    minv: np.ndarray  # [nv]
+    v: np.ndarray  # [nv]
+    fext: np.ndarray  # [nv]
+    fgyro: np.ndarray  # [nv]
+    q: np.ndarray  # [nq]
+    body_a: np.ndarray  # int32 [nc]
+    body_b: np.ndarray
+    mu: np.ndarray  # [nc]
+    frame: np.ndarray  # [nc, 9]
+    angA: np.ndarray  # [nc, 9]
+    angB: np.ndarray
+    rest: np.ndarray  # [nc, 3]
+    vbias: np.ndarray  # [nc, 3]
+    warm: np.ndarray  # [nc, 3]
+    point: np.ndarray | None = None  # [nc, 3] world contact points (warm-start matching)
+
+    @property
+    def nb(self):
+        return len(self.dof)
+
+    @property
+    def nv(self):
+        return int(self.dof.sum())
+
+    @property
+    def nc(self):
+        return len(self.body_a)
+
+    def contact_arrays(self):
+        return (self.body_a, self.body_b, self.mu, self.frame, self.angA, self.angB, self.rest, self.vbias, self.warm)
+
+    def body_arrays(self):
+        return (self.dof, self.minv, self.v, self.fext, self.fgyro, self.q)
+
+    def copy(self):
+        return dataclasses.replace(self, **{f.name: (None if getattr(self, f.name) is None else getattr(self, f.name).copy())
+                                            for f in dataclasses.fields(self)})
+
+
+# --------------------------------------------------------------------------------------------
+# bodies
+# --------------------------------------------------------------------------------------------
+
+def sphere_bodies(radius, mass, q, v=None):
+    """Rigid spheres: M = [m,m,m, I,I,I] with I = 2/5 m r^2 (rigid_body_factory.cpp:67-69),
+    f_ext = m g (world.cpp:113-135), v = [v_world; omega_body]."""
+    radius = np.asarray(radius, dtype=np.float64)
+    mass = np.broadcast_to(np.asarray(mass, dtype=np.float64), radius.shape)
+    nb = len(radius)
+    inertia = 0.4 * mass * radius * radius
+    minv = np.empty((nb, 6))
+    minv[:, :3] = (1.0 / mass)[:, None]
+    minv[:, 3:] = (1.0 / inertia)[:, None]
+    fext = np.zeros((nb, 6))
+    fext[:, :3] = mass[:, None] * GRAVITY[None, :]
+    vv = np.zeros((nb, 6)) if v is None else np.asarray(v, dtype=np.float64).reshape(nb, 6)
+    return dict(dof=np.full(nb, 6, dtype=np.int32), minv=minv.reshape(-1), v=vv.reshape(-1).copy(),
+                fext=fext.reshape(-1), fgyro=np.zeros(nb * 6), q=np.ascontiguousarray(q, dtype=np.float64).reshape(-1))
+
+
+def gyro_force(dof, minv, v):
+    """f_gyro = -omega x (I omega) (world.cpp:152-173)."""
+    out = np.zeros_like(v)
+    off = 0
+    for d in dof:
+        if d == 6:
+            w = v[off + 3: off + 6]
+            I = np.where(minv[off + 3: off + 6] > 0, 1.0 / np.where(minv[off + 3: off + 6] > 0, minv[off + 3: off + 6], 1), 0)
+            out[off + 3: off + 6] = -np.cross(w, I * w)
+        off += d
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# contacts
+# --------------------------------------------------------------------------------------------
+
+def box_walls(lo, hi, mu_wall, top=False):
+    """Static half-spaces bounding [lo, hi] (inward normals), open at the top unless `top`."""
+    n, d = [], []
+    n.append((0, 0, 1)); d.append(lo[2])
+    n.append((1, 0, 0)); d.append(lo[0])
+    n.append((-1, 0, 0)); d.append(-hi[0])
+    n.append((0, 1, 0)); d.append(lo[1])
+    n.append((0, -1, 0)); d.append(-hi[1])
+    if top:
+        n.append((0, 0, -1)); d.append(-hi[2])
+    n = np.asarray(n, dtype=np.float64)
+    return n, np.asarray(d, dtype=np.float64), np.full(len(d), float(mu_wall))
+
+
+def sphere_contacts(q, radius, mu_body, walls, margin=0.0, friction=True):
+    """CPU broad+narrow phase for spheres and static half-spaces (libcdx_scene)."""
+    L = native.scene_lib()
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    mu_body = np.ascontiguousarray(mu_body, dtype=np.float64)
+    wn, wd, wmu = (np.ascontiguousarray(x, dtype=np.float64) for x in walls)
+    nb = len(radius)
+    cap = max(16, 8 * nb)
+    while True:
+        ia = np.empty(cap, np.int32); ib = np.empty(cap, np.int32)
+        mu = np.empty(cap); frame = np.empty((cap, 9)); aA = np.empty((cap, 9)); aB = np.empty((cap, 9))
+        pt = np.empty((cap, 3)); dep = np.empty(cap)
+        n = L.cdx_sphere_contacts(nb, native.dptr(q), native.dptr(radius), native.dptr(mu_body), len(wd),
+                                  native.dptr(wn.reshape(-1)), native.dptr(wd), native.dptr(wmu), margin, cap,
+                                  native.iptr(ia), native.iptr(ib), native.dptr(mu), native.dptr(frame),
+                                  native.dptr(aA), native.dptr(aB), native.dptr(pt), native.dptr(dep))
+        if n < 0:
+            raise RuntimeError("cdx_sphere_contacts failed")
+        if n <= cap:
+            break
+        cap = n
+    if not friction:
+        mu = np.zeros(n)
+    return dict(body_a=ia[:n].copy(), body_b=ib[:n].copy(), mu=mu[:n].copy(), frame=frame[:n].copy(),
+                angA=aA[:n].copy(), angB=aB[:n].copy(), rest=np.zeros((n, 3)), vbias=np.zeros((n, 3)),
+                warm=np.zeros((n, 3)), point=pt[:n].copy(), depth=dep[:n].copy())
+
+
+def match_warmstart(prev, cur, max_dist=0.02):
+    """ContactTracker greedy matching (contact_tracker.cpp:43-72); `prev` must carry `imp`."""
+    L = native.scene_lib()
+    warm = np.zeros((len(cur["body_a"]), 3))
+    if len(prev["body_a"]) == 0 or len(cur["body_a"]) == 0:
+        return warm
+    args = [np.ascontiguousarray(prev["body_a"], np.int32), np.ascontiguousarray(prev["body_b"], np.int32),
+            np.ascontiguousarray(prev["point"], np.float64), np.ascontiguousarray(prev["imp"], np.float64),
+            np.ascontiguousarray(cur["body_a"], np.int32), np.ascontiguousarray(cur["body_b"], np.int32),
+            np.ascontiguousarray(cur["point"], np.float64)]
+    st = L.cdx_match_warmstart(len(args[0]), native.iptr(args[0]), native.iptr(args[1]), native.dptr(args[2]),
+                               native.dptr(args[3]), len(args[4]), native.iptr(args[4]), native.iptr(args[5]),
+                               native.dptr(args[6]), max_dist, native.dptr(warm))
+    if st != 0:
+        raise RuntimeError("cdx_match_warmstart failed")
+    return warm
+
+
+def make_problem(bodies: dict, contacts: dict) -> ContactProblem:
+    keys = ["body_a", "body_b", "mu", "frame", "angA", "angB", "rest", "vbias", "warm"]
+    return ContactProblem(**{k: bodies[k] for k in ["dof", "minv", "v", "fext", "fgyro", "q"]},
+                          **{k: contacts[k] for k in keys}, point=contacts.get("point"))
+
+
+# --------------------------------------------------------------------------------------------
+# scenes
+# --------------------------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class SphereScene:
+    q: np.ndarray  # [nb, 7]
+    radius: np.ndarray
+    mass: np.ndarray
+    mu_body: np.ndarray
+    walls: tuple
+    margin: float = 0.0
+
+    def bodies(self, v=None):
+        return sphere_bodies(self.radius, self.mass, self.q, v)
+
+    def contacts(self, q=None, friction=True):
+        qq = self.q if q is None else np.asarray(q).reshape(-1, 7)
+        return sphere_contacts(qq, self.radius, self.mu_body, self.walls, self.margin, friction)
+
+    def problem(self, friction=True):
+        return make_problem(self.bodies(), self.contacts(friction=friction))
+
+
+def sphere_drop_grid(n=1000, seed=0, mu=0.5):
+    """1000 spheres (r=0.05, m=1) seeded on a jittered grid, dropped into a 1x1x2 m box."""
+    L = native.scene_lib()
+    q = np.empty((n, 7))
+    L.cdx_scene_sphere_grid(n, 8, 8, 0.12, 0.06, 0.01, seed, native.dptr(q))
+    radius = np.full(n, 0.05)
+    return SphereScene(q=q, radius=radius, mass=np.ones(n), mu_body=np.full(n, mu),
+                       walls=box_walls((-0.5, -0.5, 0.0), (0.5, 0.5, 2.0), mu))
+
+
+def sphere_pile(n=100_000, seed=0, settle_steps=300, rmin=0.02, rmax=0.08, box=(4.0, 4.0, 8.0), density=1000.0,
+                margin=None):
+    """Settled polydisperse sphere pile (radii log-uniform, density 1000, mu ~ U[0.3, 0.8])."""
+    L = native.scene_lib()
+    q = np.empty((n, 7))
+    r = np.empty(n)
+    st = L.cdx_scene_sphere_pile(n, rmin, rmax, box[0], box[1], box[2], seed, settle_steps, native.dptr(q),
+                                 native.dptr(r))
+    if st != n:
+        raise RuntimeError("cdx_scene_sphere_pile failed")
+    rng = np.random.default_rng(seed + 1)
+    mu_body = rng.uniform(0.3, 0.8, n)
+    mass = density * 4.0 / 3.0 * math.pi * r ** 3
+    return SphereScene(q=q, radius=r, mass=mass, mu_body=mu_body,
+                       walls=box_walls((0.0, 0.0, 0.0), box, 0.8), margin=0.05 * rmin if margin is None else margin)
+
+
+def _rand_rot(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]]), q
+
+
+def random_problem(nb=20, nc=60, seed=0, p_static=0.15, p_frictionless=0.2, p_point=0.1, rest=0.0, warm=True,
+                   vscale=1.0):
+    """Random well-posed contact problem: random masses/inertias, orthonormal contact frames,
+    angular rows r_body x d_body of random lever arms, static sides, point masses, warm starts."""
+    rng = np.random.default_rng(seed)
+    dof = np.where(rng.random(nb) < p_point, 3, 6).astype(np.int32)
+    minv, v, fext, q = [], [], [], []
+    rots = []
+    for d in dof:
+        m = rng.uniform(0.5, 2.0)
+        minv += [1 / m] * 3
+        v += list(rng.normal(scale=vscale, size=3))
+        fext += list(m * GRAVITY)
+        R, qq = _rand_rot(rng)
+        rots.append(R)
+        if d == 6:
+            I = rng.uniform(0.01, 0.1, 3)
+            minv += list(1 / I)
+            v += list(rng.normal(scale=vscale, size=3))
+            fext += [0.0, 0.0, 0.0]
+            q += list(rng.normal(size=3)) + list(qq)
+        else:
+            q += list(rng.normal(size=3))
+    minv = np.array(minv); v = np.array(v); fext = np.array(fext); q = np.array(q)
+    fgyro = gyro_force(dof, minv, v)
+    ia = np.empty(nc, np.int32); ib = np.empty(nc, np.int32)
+    frame = np.empty((nc, 9)); aA = np.zeros((nc, 9)); aB = np.zeros((nc, 9))
+    for c in range(nc):
+        a, b = rng.choice(nb, 2, replace=False)
+        if rng.random() < p_static:
+            if rng.random() < 0.5:
+                a = -1
+            else:
+                b = -1
+        ia[c], ib[c] = a, b
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        t1 = np.cross(n, rng.normal(size=3)); t1 /= np.linalg.norm(t1)
+        t2 = np.cross(n, t1)
+        frame[c] = np.concatenate([n, t1, t2])
+        for side, body, out in ((0, a, aA), (1, b, aB)):
+            if body < 0 or dof[body] == 3:
+                continue
+            R = rots[body]
+            r = rng.normal(scale=0.3, size=3)
+            out[c] = np.concatenate([np.cross(r, R.T @ n), np.cross(r, R.T @ t1), np.cross(r, R.T @ t2)])
+    mu = np.where(rng.random(nc) < p_frictionless, 0.0, rng.uniform(0.2, 0.9, nc))
+    restitution = np.full((nc, 3), float(rest))
+    vbias = np.where((ia < 0) | (ib < 0), 1, 0)[:, None] * rng.normal(scale=0.05, size=(nc, 3))
+    w = np.zeros((nc, 3))
+    if warm:
+        w[:, 0] = np.abs(rng.normal(scale=0.1, size=nc))
+        w[:, 1:] = rng.normal(scale=0.02, size=(nc, 2))
+    return ContactProblem(dof=dof, minv=minv, v=v, fext=fext, fgyro=fgyro, q=q, body_a=ia, body_b=ib, mu=mu,
+                          frame=frame, angA=aA, angB=aB, rest=restitution, vbias=vbias, warm=w)

@@ -1,0 +1,144 @@
+/*
+ * cdx.h -- C ABI of the MI355X (gfx950) Moreau contact solve.
+ *
+ * This is the device boundary that replaces CardilloCxx's solver backends
+ * behind cardillo::solver::SolverBase (src/physics/solver/solver_base.hpp:14-36):
+ *
+ *   scene config -> contact list -> impulses -> velocity/position update
+ *
+ * All arithmetic is FP64 (the reference's real_t is double: src/misc/types.hpp:16).
+ * Every entry point returns an int status (CDX_OK == 0); cdx_last_error() gives
+ * the message of the most recent failure on a context.  The reference signals
+ * failures with exceptions (pj_assembler.cpp:175,178-180; projected_gauss_seidel.cpp:137);
+ * the C++ adapter in cardillocxx_amd/csrc/host/cdx_solvers.hpp turns non-zero
+ * statuses back into std::runtime_error.
+ *
+ * Threading: a context owns one HIP stream (default priority) and is not
+ * thread-safe; use one context per host thread.  Host pointers passed in are
+ * only read for the duration of the call (device copies are owned by the context).
+ *
+ * Layouts (the same flat SoA layout the reference's DynamicsAssembler getters
+ * expose, src/physics/assembly/dynamics_assembler.hpp:24-56):
+ *   bodies  : nb bodies, dof[b] in {3 (point mass), 6 (rigid)}; stacked arrays of
+ *             length nv = sum(dof) in body order (bodyVelOffsets):
+ *             velocity [v_world(3); omega_body(3)], minv diag [m^-1 x3; I^-1 diag],
+ *             fext [f_world; tau_body], fgyro [0; -omega x I omega].
+ *             Positions q are stacked 3 (point mass) or 7 (x,y,z, qx,qy,qz,qw) per body.
+ *   contacts: nc contacts in the reference's Contact[] order (src/collision/types.hpp:12-46):
+ *             body_a/body_b body index or -1 for a static side; mu > 0 marks a
+ *             frictional contact (3 rows n,t1,t2), otherwise 1 normal row
+ *             (dynamics_assembler.cpp:236-324);
+ *             frame[9*c]  = n, t1, t2 (world);
+ *             angA[9*c]   = r_A,body x d_A,body for d in {n,t1,t2} (unsigned; the
+ *                           side sign -1 for A, +1 for B is applied inside, as
+ *                           buildContactRowByDof does, contact_jacobian.hpp:17-23);
+ *             angB[9*c]   = same for side B;
+ *             rest[3*c]   = restitution per row; vbias[3*c] = static-side velocity
+ *                           term per row (DynamicsAssembler::contactVVec);
+ *             warm[3*c]   = previous impulse (canonical: normal >= 0, warmstart.hpp:17-24).
+ */
+#ifndef CDX_CDX_H
+#define CDX_CDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cdx_ctx cdx_ctx;
+
+enum cdx_status {
+    CDX_OK = 0,
+    CDX_NOT_FACTORIZED = 1, /* S = diag(M) singular: zero mass on a dynamic dof (pj_assembler.cpp:98-113,175) */
+    CDX_NONFINITE = 2,      /* non-finite input */
+    CDX_DIVERGED = 3,       /* Gauss-Seidel NaN/Inf residual (projected_gauss_seidel.cpp:133-138) */
+    CDX_BAD_ARG = 4,
+    CDX_HIP_ERROR = 5,
+    CDX_NO_STATE = 6 /* called before bodies / contacts / a solve were set */
+};
+
+enum cdx_accel {
+    CDX_ACCEL_NONE = 0,    /* standard_loop (projected_jacobi.cpp:371-377) */
+    CDX_ACCEL_NESTEROV = 1 /* nesterov_loop (projected_jacobi.cpp:379-449) */
+};
+
+/* Solver parameters; the Config keys they stand for (src/config/config.hpp:61-113). */
+typedef struct {
+    double alpha;      /* pj.alpha (default 0.3) */
+    double relaxation; /* pj.relaxation (default 1.0; Gauss-Seidel only) */
+    int max_iter;      /* pj.max_iterations (default 200000) */
+    double tol_abs;    /* pj.tol_abs (default 1e-5) */
+    double tol_rel;    /* pj.tol_rel (default 1e-5) */
+    int check_every;   /* PJ residual cadence (reference: every 10 sweeps, projected_jacobi.cpp:375) */
+    int warmstart;     /* pj.warmstart (default 1) */
+    int accel;         /* enum cdx_accel (pj.nesterov) */
+    double nesterov_beta_threshold; /* pj.nesterov_beta_threshold (default 1.0) */
+    int nesterov_restart_limit;     /* pj.nesterov_restart_limit (default 10000) */
+} cdx_solver_params;
+
+typedef struct {
+    int iters;       /* sweeps performed (SolverBase::lastIterations, solver_base.hpp:29) */
+    double residual; /* last residual evaluated (NaN if none) */
+    int nrows;       /* contact rows (frictionless + 3 x frictional) */
+    int ncolors;     /* colored Gauss-Seidel: colour classes used */
+} cdx_solve_info;
+
+/* Fills *p with the reference Config defaults (config.hpp:61-113). */
+void cdx_default_params(cdx_solver_params* p);
+
+/* Context lifetime.  device: HIP ordinal. */
+int cdx_ctx_create(int device, cdx_ctx** out);
+void cdx_ctx_destroy(cdx_ctx* ctx);
+const char* cdx_last_error(const cdx_ctx* ctx);
+int cdx_synchronize(cdx_ctx* ctx);
+
+/* Body state: stacks what DynamicsAssembler::refreshState builds
+ * (dynamics_assembler.cpp:41-120, 469-513).  fgyro may be NULL, in which case it is
+ * computed on the device as -omega x (I omega) (world.cpp:152-173).  q may be NULL
+ * when no position update is needed. */
+int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, const double* v, const double* fext,
+                   const double* fgyro, const double* q);
+
+/* Contact buffer: the input of DynamicsAssembler::rebuildW_ (dynamics_assembler.cpp:211-332),
+ * filled by CPU contact generation and kept device-resident. */
+int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b, const double* mu, const double* frame,
+                     const double* angA, const double* angB, const double* rest, const double* vbias,
+                     const double* warm);
+
+/* Projected Jacobi: replaces ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500).
+ * Results stay on the device: fetch with cdx_get_velocities / cdx_get_impulses.
+ * The new impulses also become the next warm start (storeImpulse, warmstart.hpp:41-56). */
+int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info);
+
+/* Graph-coloured projected Gauss-Seidel: the PGS block update
+ * (projected_gauss_seidel.cpp:144-169) run colour by colour as in
+ * CondensedSolver's coloredSweep (condensed_solver.cpp:220-251), Welsh-Powell colouring
+ * (graph_coloring.cpp:8-41).  Residual every sweep, CDX_DIVERGED on NaN/Inf. */
+int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info);
+
+/* Outputs of the last solve.  v_out: nv stacked velocities (the SolverBase::solve return
+ * value); imp_out: 3*nc impulses per contact, canonical normal >= 0, zero tangents for
+ * frictionless contacts (Contact::last_impulse). */
+int cdx_get_velocities(cdx_ctx* ctx, double* v_out);
+int cdx_get_impulses(cdx_ctx* ctx, double* imp_out);
+int cdx_get_positions(cdx_ctx* ctx, double* q_out);
+
+/* Moreau theta-method around the solve (moreau.cpp:6-13, integration_base.cpp:6-30):
+ *   cdx_moreau_predrift : q += (1-theta) h B(q) v_n
+ *   cdx_moreau_update   : v_n <- solved velocities (writeVelocityToSystem,
+ *                         dynamics_assembler.cpp:145-188), q += theta h B(q) v_{n+1},
+ *                         quaternion normalised and sign-aligned (math_helper.hpp:11-17),
+ *                         f_gyro recomputed from the new velocities (world.cpp:152-173). */
+int cdx_moreau_predrift(cdx_ctx* ctx, double dt, double theta);
+int cdx_moreau_update(cdx_ctx* ctx, double dt, double theta);
+
+/* Kernel timing with HIP events on the context's stream (the analogue of the reference's
+ * TimingManager scopes, src/misc/timings/TimingManager.hpp:16-58).  When enabled every
+ * launch is bracketed by events; cdx_kernel_time reads back the call count and total ms for
+ * one kernel name ("pj_contacts", "pj_bodies", "gs_color", ...) and resets it. */
+int cdx_set_profiling(cdx_ctx* ctx, int enable);
+int cdx_kernel_time(cdx_ctx* ctx, const char* kernel, int* calls, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,752 @@
+/*
+ * cdx_oracle.c -- CPU restatement of CardilloCxx's Moreau contact solve.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Written from the reference source as text (no code copied); each function
+ * cites the file:line it follows.  Arithmetic order mirrors the reference
+ * where it is cheap to do so (row-major SpMV summed in column order, W^T
+ * products accumulated row by row) so that the restatement is as close as
+ * possible to Eigen's evaluation; bit identity is not claimed (Eigen's
+ * SparseLU and vectorised reductions are not reproduced).
+ *
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fopenmp).
+ */
+#include "cdx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+void orc_cross(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Frisvad-style tangent frame with south-pole branch.
+ * Follows src/collision/tangent_frame.hpp:23-50. */
+void orc_tangent_frame(const double n_in[3], double t1[3], double t2[3]) {
+    double n[3] = {n_in[0], n_in[1], n_in[2]};
+    const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (len <= 2.220446049250313e-16) { /* tangent_frame.hpp:26-31 */
+        t1[0] = 1; t1[1] = 0; t1[2] = 0;
+        t2[0] = 0; t2[1] = 1; t2[2] = 0;
+        return;
+    }
+    n[0] /= len; n[1] /= len; n[2] /= len;
+    if (n[2] < -0.9999999) { /* tangent_frame.hpp:34-41 */
+        double t[3] = {0, -1, 0};
+        const double d = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
+        t[0] -= n[0] * d; t[1] -= n[1] * d; t[2] -= n[2] * d;
+        const double tl = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (tl > 0) { t[0] /= tl; t[1] /= tl; t[2] /= tl; }
+        t1[0] = t[0]; t1[1] = t[1]; t1[2] = t[2];
+        orc_cross(n, t1, t2);
+        return;
+    }
+    const double a = 1.0 / (1.0 + n[2]); /* tangent_frame.hpp:43-49 */
+    const double b = -n[0] * n[1] * a;
+    double t[3] = {1.0 - n[0] * n[0] * a, b, -n[0]};
+    const double tl = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    if (tl > 0) { t[0] /= tl; t[1] /= tl; t[2] /= tl; }
+    t1[0] = t[0]; t1[1] = t[1]; t1[2] = t[2];
+    orc_cross(n, t1, t2);
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* W assembly: src/physics/assembly/dynamics_assembler.cpp:211-332 and                     */
+/* src/physics/assembly/contact_jacobian.hpp:17-23.                                         */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int row, col;
+    double val;
+} trip;
+
+static int trip_cmp(const void* pa, const void* pb) {
+    const trip* a = (const trip*)pa;
+    const trip* b = (const trip*)pb;
+    if (a->row != b->row) return a->row < b->row ? -1 : 1;
+    if (a->col != b->col) return a->col < b->col ? -1 : 1;
+    return 0;
+}
+
+int orc_build_W(const orc_bodies* B, const orc_contacts* C, orc_W* W) {
+    memset(W, 0, sizeof(*W));
+    const int nb = B->nb, nc = C->nc;
+    W->voff = (int*)malloc(sizeof(int) * (size_t)(nb + 1));
+    int nv = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (B->dof[b] != 3 && B->dof[b] != 6) return ORC_BAD_ARG;
+        W->voff[b] = nv;
+        nv += B->dof[b];
+    }
+    W->voff[nb] = nv;
+    W->nv = nv;
+    W->row_base = (int*)malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+    W->row_size = (int*)malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+    const size_t cap = (size_t)nc * 3 + 1;
+    W->mu = (double*)calloc(cap, sizeof(double));
+    W->rest = (double*)calloc(cap, sizeof(double));
+    W->cvec = (double*)calloc(cap, sizeof(double));
+    trip* trips = (trip*)malloc(sizeof(trip) * ((size_t)nc * 36 + 1));
+    size_t nt = 0;
+    int row = 0, nnfc = 0;
+    for (int ci = 0; ci < nc; ++ci) { W->row_base[ci] = -1; W->row_size[ci] = 0; }
+
+    /* Two passes: frictionless rows first, then frictional (dynamic_assembler.cpp:236-244). */
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int ci = 0; ci < nc; ++ci) {
+            const int frictional = C->mu[ci] > 0;
+            if (pass == 0 && frictional) continue;
+            if (pass == 1 && !frictional) continue;
+            const int a = C->body_a[ci], bb = C->body_b[ci];
+            if (a < -1 || a >= nb || bb < -1 || bb >= nb || (a >= 0 && a == bb)) { free(trips); return ORC_BAD_ARG; }
+            if (a < 0 && bb < 0) continue; /* static-static skipped (:259-260) */
+            const int nr = frictional ? 3 : 1;
+            W->row_base[ci] = row;
+            W->row_size[ci] = nr;
+            for (int k = 0; k < nr; ++k) {
+                const int r = row + k;
+                const double* d = C->frame + 9 * (size_t)ci + 3 * k;
+                for (int side = 0; side < 2; ++side) {
+                    const int body = side == 0 ? a : bb;
+                    const double s = side == 0 ? -1.0 : 1.0; /* A: -1, B: +1 (:295-296) */
+                    if (body < 0) continue; /* static side goes to cvec via vbias */
+                    const int col0 = W->voff[body];
+                    const int dof = B->dof[body];
+                    const double* ang = (side == 0 ? C->angA : C->angB) + 9 * (size_t)ci + 3 * k;
+                    double rowv[6];
+                    /* buildContactRowByDof: [s*d_world ; r_body x (s*d_body)] (contact_jacobian.hpp:17-23) */
+                    for (int j = 0; j < 3; ++j) rowv[j] = s * d[j];
+                    for (int j = 0; j < 3; ++j) rowv[3 + j] = s * ang[j];
+                    for (int j = 0; j < dof; ++j) {
+                        if (rowv[j] != 0.0) { /* zero entries dropped (:274-277) */
+                            trips[nt].row = r; trips[nt].col = col0 + j; trips[nt].val = rowv[j];
+                            ++nt;
+                        }
+                    }
+                }
+                W->cvec[r] = C->vbias[3 * (size_t)ci + k];
+                W->rest[r] = C->rest[3 * (size_t)ci + k];
+                W->mu[r] = frictional ? C->mu[ci] : 0.0; /* :301-312 */
+            }
+            row += nr;
+            if (!frictional) ++nnfc;
+        }
+    }
+    W->nrows = row;
+    W->nnfc = nnfc;
+    /* Compress like Eigen's setFromTriplets: sort by (row, col), sum duplicates. */
+    qsort(trips, nt, sizeof(trip), trip_cmp);
+    W->rowptr = (int*)calloc((size_t)row + 1, sizeof(int));
+    W->col = (int*)malloc(sizeof(int) * (nt + 1));
+    W->val = (double*)malloc(sizeof(double) * (nt + 1));
+    size_t nnz = 0;
+    for (size_t t = 0; t < nt; ++t) {
+        if (nnz > 0 && t > 0 && trips[t].row == trips[t - 1].row && trips[t].col == trips[t - 1].col) {
+            W->val[nnz - 1] += trips[t].val;
+            continue;
+        }
+        W->col[nnz] = trips[t].col;
+        W->val[nnz] = trips[t].val;
+        W->rowptr[trips[t].row + 1]++;
+        ++nnz;
+    }
+    for (int r = 0; r < row; ++r) W->rowptr[r + 1] += W->rowptr[r];
+    free(trips);
+    return ORC_OK;
+}
+
+void orc_free_W(orc_W* w) {
+    free(w->rowptr); free(w->col); free(w->val); free(w->mu); free(w->rest); free(w->cvec);
+    free(w->row_base); free(w->row_size); free(w->voff);
+    memset(w, 0, sizeof(*w));
+}
+
+/* y = W x (row-wise, column order) */
+static void spmv(const orc_W* W, const double* x, double* y) {
+    for (int r = 0; r < W->nrows; ++r) {
+        double s = 0;
+        for (int k = W->rowptr[r]; k < W->rowptr[r + 1]; ++k) s += W->val[k] * x[W->col[k]];
+        y[r] = s;
+    }
+}
+
+/* y += W^T p, accumulated row by row (Eigen row-major transpose product). */
+static void spmv_t_acc(const orc_W* W, const double* p, double* y) {
+    for (int r = 0; r < W->nrows; ++r) {
+        const double pr = p[r];
+        for (int k = W->rowptr[r]; k < W->rowptr[r + 1]; ++k) y[W->col[k]] += W->val[k] * pr;
+    }
+}
+
+/* D_i = sum_j W_ij^2 Minv_j (projected_jacobi.cpp:57-69; pgs_assembler.cpp:226-238). */
+static void delassus_diag(const orc_W* W, const double* minv, double* D) {
+    for (int r = 0; r < W->nrows; ++r) {
+        double s = 0;
+        for (int k = W->rowptr[r]; k < W->rowptr[r + 1]; ++k) {
+            const double w = W->val[k];
+            s += w * w * minv[W->col[k]];
+        }
+        D[r] = s;
+    }
+}
+
+/* Residual (projected_jacobi.cpp:140-144). */
+static double pj_residual(int nv, const double* v, const double* vp, double tol_rel, double tol_abs) {
+    double s = 0;
+    for (int j = 0; j < nv; ++j) {
+        const double q = dmax(fabs(v[j]), fabs(vp[j])) * tol_rel + tol_abs;
+        const double e = (v[j] - vp[j]) / q;
+        s += e * e;
+    }
+    return 1.0 / sqrt((double)nv) * sqrt(s);
+}
+
+/* Projection (projected_jacobi.cpp:100-119): normals >= 0, tangents into the Coulomb disk. */
+static void pj_project(const orc_W* W, double* p, int nthreads) {
+    for (int i = 0; i < W->nnfc; ++i) p[i] = dmax(p[i], 0.0);
+    const int nc = W->nrows;
+    const int nnfc = W->nnfc;
+    const double* mu = W->mu;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (int i = nnfc; i < nc; i += 3) {
+        p[i] = dmax(p[i], 0.0);
+        const double t1 = p[i + 1], t2 = p[i + 2];
+        const double tn2 = t1 * t1 + t2 * t2;
+        if (tn2 <= 0) continue;
+        const double s = dmin(1.0, (mu[i] * p[i]) / sqrt(tn2));
+        p[i + 1] = s * t1;
+        p[i + 2] = s * t2;
+    }
+}
+
+static void fill_mdiag(int nv, const double* minv, double* M) {
+    /* rebuildMass_: M = 1/Minv where Minv > 0 else 0 (dynamics_assembler.cpp:56-60) */
+    for (int j = 0; j < nv; ++j) M[j] = minv[j] > 0 ? 1.0 / minv[j] : 0.0;
+}
+
+/* PJ solve: ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500) with build_workspace
+ * (:316-369), standard_loop (:371-377) / nesterov_loop (:379-449), pj_sweep (:121-138),
+ * rhs (pj_assembler.cpp:127-170, contact-only), solveS on S = diag(M) (pj_assembler.cpp:26-30,173-182),
+ * warm start / store (warmstart.hpp:26-56). */
+int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double theta, const orc_params* prm,
+                 int nthreads, double* v_out, double* imp_out, orc_result* res) {
+    (void)theta; /* theta only enters spring/damper rows and S for bilateral scenes */
+    orc_W W;
+    int st = orc_build_W(B, C, &W);
+    if (st) { orc_free_W(&W); return st; }
+    const int nv = W.nv, nr = W.nrows;
+    res->iters = 0;
+    res->residual = NAN;
+    res->nrows = nr;
+    res->ncolors = 0;
+    double* M = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
+    fill_mdiag(nv, B->minv, M);
+    for (int j = 0; j < nv; ++j)
+        if (M[j] == 0.0) { free(M); orc_free_W(&W); return ORC_NOT_FACTORIZED; } /* singular diagonal S */
+
+    /* rhs_free = M v + dt f_ext (+ dt f_gyro) (pj_assembler.cpp:152-153) */
+    double* x = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
+    for (int j = 0; j < nv; ++j) x[j] = M[j] * B->v[j] + dt * B->fext[j];
+    for (int j = 0; j < nv; ++j) x[j] += dt * B->fgyro[j];
+
+    if (nr == 0) { /* no contacts: free solution (:460-467) */
+        for (int j = 0; j < nv; ++j) v_out[j] = x[j] / M[j];
+        for (int c = 0; c < C->nc; ++c) { imp_out[3 * c] = imp_out[3 * c + 1] = imp_out[3 * c + 2] = 0; }
+        free(x); free(M); orc_free_W(&W);
+        return ORC_OK;
+    }
+
+    double* R = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* bias = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* p = (double*)calloc((size_t)nr, sizeof(double));
+    double* pp = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* dl = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* xp = (double*)malloc(sizeof(double) * (size_t)nv);
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)nv);
+    double* RWv = (double*)malloc(sizeof(double) * (W.rowptr[nr] + 1));
+
+    /* v_prev = W v + c; biasImpulse = c + e .* v_prev; R = alpha / D; bias = R .* bias (:337-347) */
+    spmv(&W, B->v, bias);
+    for (int r = 0; r < nr; ++r) bias[r] += W.cvec[r];
+    for (int r = 0; r < nr; ++r) bias[r] = W.cvec[r] + W.rest[r] * bias[r];
+    delassus_diag(&W, B->minv, R);
+    for (int r = 0; r < nr; ++r) R[r] = (R[r] > 0) ? (prm->alpha / R[r]) : 0.0;
+    for (int r = 0; r < nr; ++r) bias[r] = R[r] * bias[r];
+    for (int r = 0; r < nr; ++r)
+        for (int k = W.rowptr[r]; k < W.rowptr[r + 1]; ++k) RWv[k] = R[r] * W.val[k];
+
+    /* warm start (:358-359, warmstart.hpp:26-37) */
+    if (prm->warmstart)
+        for (int ci = 0; ci < C->nc; ++ci) {
+            const int base = W.row_base[ci];
+            if (base < 0) continue;
+            p[base] = C->warm[3 * ci];
+            if (W.row_size[ci] > 1) { p[base + 1] = C->warm[3 * ci + 1]; p[base + 2] = C->warm[3 * ci + 2]; }
+        }
+    /* x0 = S^-1 (rhs_free + W^T p) (:361-363) */
+    spmv_t_acc(&W, p, x);
+    for (int j = 0; j < nv; ++j) x[j] = x[j] / M[j];
+
+    orc_W RW = W;
+    RW.val = RWv;
+
+#define PJ_SWEEP()                                                          \
+    do {                                                                    \
+        memcpy(xp, x, sizeof(double) * (size_t)nv);                         \
+        memcpy(pp, p, sizeof(double) * (size_t)nr);                         \
+        spmv(&RW, x, dl);                                                   \
+        for (int r = 0; r < nr; ++r) dl[r] += bias[r];                      \
+        for (int r = 0; r < nr; ++r) p[r] -= dl[r];                         \
+        pj_project(&W, p, nthreads);                                        \
+        for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];                  \
+        memset(rhs, 0, sizeof(double) * (size_t)nv);                        \
+        spmv_t_acc(&W, dl, rhs);                                            \
+        for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];                 \
+    } while (0)
+
+    if (!prm->nesterov) {
+        /* standard_loop (:371-377): residual only when iter % 10 == 0 */
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            PJ_SWEEP();
+            res->iters = iter + 1;
+            if (iter % 10 == 0) {
+                const double e = pj_residual(nv, x, xp, prm->tol_rel, prm->tol_abs);
+                res->residual = e;
+                if (e <= 1) break;
+            }
+        }
+    } else {
+        /* nesterov_loop (:379-449) */
+        double* xk = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* pk = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xy = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* py = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xk1 = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* pk1 = (double*)malloc(sizeof(double) * (size_t)nr);
+        memcpy(xk, x, sizeof(double) * nv); memcpy(xy, x, sizeof(double) * nv); memcpy(xk1, x, sizeof(double) * nv);
+        memcpy(pk, p, sizeof(double) * nr); memcpy(py, p, sizeof(double) * nr); memcpy(pk1, p, sizeof(double) * nr);
+        double thk = 1.0, err_prev = INFINITY;
+        int restarts = 0, momentum_disabled = 0;
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(x, xy, sizeof(double) * nv);
+            memcpy(p, py, sizeof(double) * nr);
+            PJ_SWEEP();
+            memcpy(xk1, x, sizeof(double) * nv);
+            memcpy(pk1, p, sizeof(double) * nr);
+            res->iters = iter + 1;
+            const double err = pj_residual(nv, xk1, xy, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (err <= 1) break;
+            const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+            const double betak1 = (thk - 1.0) / thk1;
+            int restart = 0;
+            if (!isfinite(err)) restart = 1;
+            else if (err_prev < INFINITY && err > 1.05 * err_prev) restart = 1;
+            else if (betak1 > prm->nesterov_beta_threshold) restart = 1;
+            else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = 1;
+            else {
+                double dot = 0;
+                for (int j = 0; j < nv; ++j) dot += (xy[j] - xk1[j]) * (xk1[j] - xk[j]);
+                if (dot > 0.0) restart = 1;
+            }
+            if (restart) {
+                memcpy(xy, xk1, sizeof(double) * nv); memcpy(py, pk1, sizeof(double) * nr);
+                memcpy(xk, xk1, sizeof(double) * nv); memcpy(pk, pk1, sizeof(double) * nr);
+                thk = 1.0;
+                if (++restarts >= prm->nesterov_restart_limit) momentum_disabled = 1;
+            } else {
+                if (momentum_disabled) {
+                    memcpy(xy, xk1, sizeof(double) * nv); memcpy(py, pk1, sizeof(double) * nr);
+                    thk = 1.0;
+                } else {
+                    for (int j = 0; j < nv; ++j) xy[j] = xk1[j] + betak1 * (xk1[j] - xk[j]);
+                    for (int r = 0; r < nr; ++r) py[r] = pk1[r] + betak1 * (pk1[r] - pk[r]);
+                    thk = thk1;
+                }
+                memcpy(xk, xk1, sizeof(double) * nv); memcpy(pk, pk1, sizeof(double) * nr);
+            }
+            err_prev = err;
+        }
+        memcpy(x, xk1, sizeof(double) * nv);
+        memcpy(p, pk1, sizeof(double) * nr);
+        free(xk); free(pk); free(xy); free(py); free(xk1); free(pk1);
+    }
+#undef PJ_SWEEP
+
+    /* storeImpulse (warmstart.hpp:41-56) */
+    for (int ci = 0; ci < C->nc; ++ci) {
+        double* o = imp_out + 3 * (size_t)ci;
+        o[0] = o[1] = o[2] = 0;
+        const int base = W.row_base[ci];
+        if (base < 0) continue;
+        o[0] = p[base];
+        if (W.row_size[ci] > 1) { o[1] = p[base + 1]; o[2] = p[base + 2]; }
+    }
+    memcpy(v_out, x, sizeof(double) * (size_t)nv);
+    free(R); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(M);
+    orc_free_W(&W);
+    return ORC_OK;
+}
+
+/* u_free = v + Minv .* (dt f_ext) (+ Minv .* (dt f_gyro)) (pgs_assembler.cpp:75-82) */
+static void u_free_of(int nv, const orc_bodies* B, double dt, double* uf) {
+    for (int j = 0; j < nv; ++j) uf[j] = B->v[j] + B->minv[j] * (dt * B->fext[j]);
+    for (int j = 0; j < nv; ++j) uf[j] += B->minv[j] * (dt * B->fgyro[j]);
+}
+
+/* Residual on velocities u_free - u_corr (projected_gauss_seidel.cpp:124-131) */
+static double gs_residual(int nv, const double* uf, const double* uk, const double* uk1, double tol_rel, double tol_abs) {
+    double s = 0;
+    for (int j = 0; j < nv; ++j) {
+        const double d = uk1[j] - uk[j];
+        const double v0 = uf[j] - uk[j], v1 = uf[j] - uk1[j];
+        const double q = dmax(fabs(v0), fabs(v1)) * tol_rel + tol_abs;
+        const double e = d / q;
+        s += e * e;
+    }
+    return 1.0 / sqrt((double)nv) * sqrt(s);
+}
+
+/* PGS projection on one block, lambda <= 0 convention, correcting dlambda
+ * (projected_gauss_seidel.cpp:11-26). */
+static void pgs_project(double* lam, double* dl, double mu, int off, int n) {
+    const double ln = lam[off];
+    lam[off] = dmin(ln, 0.0);
+    dl[off] += lam[off] - ln;
+    if (n <= 1) return;
+    const double t1 = lam[off + 1], t2 = lam[off + 2];
+    const double tn = sqrt(t1 * t1 + t2 * t2);
+    if (tn <= 0) return;
+    const double s = dmin(1.0, -(mu * lam[off]) / tn);
+    const double p1 = s * t1, p2 = s * t2;
+    dl[off + 1] += p1 - t1;
+    dl[off + 2] += p2 - t2;
+    lam[off + 1] = p1;
+    lam[off + 2] = p2;
+}
+
+/* PGS: ProjectedGaussSeidelSolver::solve (projected_gauss_seidel.cpp:56-268), standard loop,
+ * with PgsAssembler::ufree/rhs/DinvDiag (pgs_assembler.cpp:12-82,220-288) for contact rows.
+ * `colored` visits the blocks colour class by colour class (condensed_solver.cpp:220-251)
+ * instead of in row order; the block update is unchanged. */
+static int pgs_core(const orc_bodies* B, const orc_contacts* C, double dt, const orc_params* prm, int colored,
+                    double* v_out, double* imp_out, orc_result* res) {
+    orc_W W;
+    int st = orc_build_W(B, C, &W);
+    if (st) { orc_free_W(&W); return st; }
+    const int nv = W.nv, nr = W.nrows;
+    res->iters = 0; res->residual = NAN; res->nrows = nr; res->ncolors = 0;
+    double* uf = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
+    u_free_of(nv, B, dt, uf);
+    if (nr == 0) {
+        memcpy(v_out, uf, sizeof(double) * (size_t)nv);
+        for (int c = 0; c < C->nc; ++c) imp_out[3 * c] = imp_out[3 * c + 1] = imp_out[3 * c + 2] = 0;
+        free(uf); orc_free_W(&W);
+        return ORC_OK;
+    }
+    double* D = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* Dinv = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* lam = (double*)calloc((size_t)nr, sizeof(double));
+    double* dl = (double*)calloc((size_t)nr, sizeof(double));
+    double* uc = (double*)calloc((size_t)nv, sizeof(double));
+    double* uk = (double*)malloc(sizeof(double) * (size_t)nv);
+    delassus_diag(&W, B->minv, D);
+    for (int r = 0; r < W.nnfc; ++r) Dinv[r] = D[r] > 0 ? 1.0 / D[r] : 0.0; /* :277-280 */
+    for (int r = W.nnfc; r < nr; ++r) Dinv[r] = 1.0 / D[r];                   /* :282-285 (no guard) */
+    /* rhs = W u_free + c (+ e .* (W v + c) when any e != 0) (pgs_assembler.cpp:61-70) */
+    int any_rest = 0;
+    for (int r = 0; r < nr; ++r) if (W.rest[r] != 0.0) any_rest = 1;
+    double* bi = (double*)malloc(sizeof(double) * (size_t)nr);
+    for (int r = 0; r < nr; ++r) bi[r] = W.cvec[r];
+    if (any_rest) {
+        spmv(&W, B->v, rhs);
+        for (int r = 0; r < nr; ++r) rhs[r] += W.cvec[r];
+        for (int r = 0; r < nr; ++r) bi[r] += W.rest[r] * rhs[r];
+    }
+    spmv(&W, uf, rhs);
+    for (int r = 0; r < nr; ++r) rhs[r] += bi[r];
+    free(bi);
+    /* warm start with inverted normal (:95-101) */
+    if (prm->warmstart)
+        for (int ci = 0; ci < C->nc; ++ci) {
+            const int base = W.row_base[ci];
+            if (base < 0) continue;
+            lam[base] = -C->warm[3 * ci];
+            if (W.row_size[ci] > 1) { lam[base + 1] = C->warm[3 * ci + 1]; lam[base + 2] = C->warm[3 * ci + 2]; }
+        }
+    /* u_corr = (Minv W^T) lambda (:81,108) */
+    for (int r = 0; r < nr; ++r)
+        for (int k = W.rowptr[r]; k < W.rowptr[r + 1]; ++k) uc[W.col[k]] += (B->minv[W.col[k]] * W.val[k]) * lam[r];
+    memcpy(uk, uc, sizeof(double) * (size_t)nv);
+
+    /* block list: row offsets in row order (1-row frictionless, 3-row frictional) */
+    const int nblk = W.nnfc + (nr - W.nnfc) / 3;
+    int* boff = (int*)malloc(sizeof(int) * ((size_t)nblk + 1));
+    for (int k = 0, off = 0; k < nblk; ++k) { boff[k] = off; off += off < W.nnfc ? 1 : 3; }
+    int* visit = (int*)malloc(sizeof(int) * ((size_t)nblk + 1));
+    for (int k = 0; k < nblk; ++k) visit[k] = k;
+    if (colored) {
+        /* blocks in row order are the contacts with rows, frictionless first */
+        int* ba = (int*)malloc(sizeof(int) * ((size_t)nblk + 1));
+        int* bb = (int*)malloc(sizeof(int) * ((size_t)nblk + 1));
+        for (int ci = 0; ci < C->nc; ++ci) {
+            const int base = W.row_base[ci];
+            if (base < 0) continue;
+            const int k = base < W.nnfc ? base : W.nnfc + (base - W.nnfc) / 3;
+            ba[k] = C->body_a[ci]; bb[k] = C->body_b[ci];
+        }
+        int* col = (int*)malloc(sizeof(int) * ((size_t)nblk + 1));
+        const int ncol = orc_color_blocks(B->nb, nblk, ba, bb, col);
+        res->ncolors = ncol;
+        int m = 0;
+        for (int c = 0; c < ncol; ++c)
+            for (int k = 0; k < nblk; ++k)
+                if (col[k] == c) visit[m++] = k;
+        free(ba); free(bb); free(col);
+    }
+
+    for (int iter = 0; iter < prm->max_iter; ++iter) {
+        /* pgs_sweep (:144-169) */
+        for (int v = 0; v < nblk; ++v) {
+            const int off = boff[visit[v]];
+            const int bs = off < W.nnfc ? 1 : 3;
+            for (int k = 0; k < bs; ++k) {
+                const int r = off + k;
+                double s = 0;
+                for (int q = W.rowptr[r]; q < W.rowptr[r + 1]; ++q) s += W.val[q] * uc[W.col[q]];
+                const double rr = rhs[r] - s; /* C = 0 for contact rows */
+                dl[r] = (Dinv[r] * rr) * prm->relaxation;
+            }
+            for (int k = 0; k < bs; ++k) dl[off + k] *= prm->alpha;
+            for (int k = 0; k < bs; ++k) lam[off + k] += dl[off + k];
+            pgs_project(lam, dl, W.mu[off], off, bs);
+            for (int k = 0; k < bs; ++k) {
+                const int r = off + k;
+                for (int q = W.rowptr[r]; q < W.rowptr[r + 1]; ++q) uc[W.col[q]] += (B->minv[W.col[q]] * W.val[q]) * dl[r];
+            }
+        }
+        res->iters = iter + 1;
+        const double e = gs_residual(nv, uf, uc, uk, prm->tol_rel, prm->tol_abs);
+        res->residual = e;
+        if (isnan(e) || isinf(e)) { st = ORC_DIVERGED; break; } /* :133-138 */
+        if (e < 1.0) break;
+        memcpy(uk, uc, sizeof(double) * (size_t)nv);
+    }
+    for (int ci = 0; ci < C->nc; ++ci) {
+        double* o = imp_out + 3 * (size_t)ci;
+        o[0] = o[1] = o[2] = 0;
+        const int base = W.row_base[ci];
+        if (base < 0) continue;
+        o[0] = -lam[base];
+        if (W.row_size[ci] > 1) { o[1] = lam[base + 1]; o[2] = lam[base + 2]; }
+    }
+    for (int j = 0; j < nv; ++j) v_out[j] = uf[j] - uc[j]; /* :267 */
+    free(uf); free(D); free(Dinv); free(rhs); free(lam); free(dl); free(uc); free(uk); free(boff); free(visit);
+    orc_free_W(&W);
+    return st;
+}
+
+int orc_pgs_solve(const orc_bodies* B, const orc_contacts* C, double dt, double theta, const orc_params* prm,
+                  double* v_out, double* imp_out, orc_result* res) {
+    (void)theta;
+    return pgs_core(B, C, dt, prm, 0, v_out, imp_out, res);
+}
+
+int orc_colored_gs_solve(const orc_bodies* B, const orc_contacts* C, double dt, double theta, const orc_params* prm,
+                         double* v_out, double* imp_out, orc_result* res) {
+    (void)theta;
+    return pgs_core(B, C, dt, prm, 1, v_out, imp_out, res);
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* Colouring for the coloured GS: condensed_solver.cpp:195-215, graph_coloring.cpp:8-41.  */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int deg, idx;
+} degidx;
+
+static int degidx_cmp(const void* pa, const void* pb) {
+    const degidx* a = (const degidx*)pa;
+    const degidx* b = (const degidx*)pb;
+    if (a->deg != b->deg) return a->deg > b->deg ? -1 : 1; /* descending degree */
+    return a->idx < b->idx ? -1 : (a->idx > b->idx); /* ties: ascending index (stable) */
+}
+
+static int int_cmp(const void* a, const void* b) {
+    const int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* Block adjacency = blocks sharing a body (buildBlockAdjacency, condensed_solver.cpp:195-215),
+ * greedy colouring in descending-degree order (graph_coloring.cpp:8-41).  The reference sorts
+ * with std::sort (unstable); ties here are broken by block index. */
+int orc_color_blocks(int nb, int nc, const int* body_a, const int* body_b, int* color_of) {
+    /* incidence lists */
+    int* cnt = (int*)calloc((size_t)nb + 1, sizeof(int));
+    for (int c = 0; c < nc; ++c) {
+        if (body_a[c] >= 0) cnt[body_a[c] + 1]++;
+        if (body_b[c] >= 0) cnt[body_b[c] + 1]++;
+    }
+    for (int b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    int* inc = (int*)malloc(sizeof(int) * ((size_t)cnt[nb] + 1));
+    int* fill = (int*)calloc((size_t)nb + 1, sizeof(int));
+    for (int c = 0; c < nc; ++c) {
+        if (body_a[c] >= 0) inc[cnt[body_a[c]] + fill[body_a[c]]++] = c;
+        if (body_b[c] >= 0) inc[cnt[body_b[c]] + fill[body_b[c]]++] = c;
+    }
+    /* adjacency as sorted unique lists */
+    int* adjcnt = (int*)calloc((size_t)nc + 1, sizeof(int));
+    for (int b = 0; b < nb; ++b) {
+        const int d = cnt[b + 1] - cnt[b];
+        for (int i = 0; i < d; ++i) adjcnt[inc[cnt[b] + i] + 1] += d - 1;
+    }
+    for (int c = 0; c < nc; ++c) adjcnt[c + 1] += adjcnt[c];
+    int* adj = (int*)malloc(sizeof(int) * ((size_t)adjcnt[nc] + 1));
+    int* afill = (int*)calloc((size_t)nc + 1, sizeof(int));
+    for (int b = 0; b < nb; ++b) {
+        const int s = cnt[b], d = cnt[b + 1] - cnt[b];
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j)
+                if (i != j) { const int u = inc[s + i]; adj[adjcnt[u] + afill[u]++] = inc[s + j]; }
+    }
+    int* deg = (int*)calloc((size_t)nc + 1, sizeof(int));
+    for (int c = 0; c < nc; ++c) {
+        int* a = adj + adjcnt[c];
+        const int n = afill[c];
+        qsort(a, (size_t)n, sizeof(int), int_cmp);
+        int u = 0;
+        for (int i = 0; i < n; ++i)
+            if (u == 0 || a[u - 1] != a[i]) a[u++] = a[i];
+        deg[c] = u;
+    }
+    degidx* order = (degidx*)malloc(sizeof(degidx) * ((size_t)nc + 1));
+    for (int c = 0; c < nc; ++c) { order[c].deg = deg[c]; order[c].idx = c; color_of[c] = -1; }
+    qsort(order, (size_t)nc, sizeof(degidx), degidx_cmp);
+    int* stamp = (int*)malloc(sizeof(int) * ((size_t)nc + 2));
+    for (int i = 0; i < nc + 2; ++i) stamp[i] = -1;
+    int ncolors = 0;
+    for (int k = 0; k < nc; ++k) {
+        const int node = order[k].idx;
+        for (int i = 0; i < deg[node]; ++i) {
+            const int cc = color_of[adj[adjcnt[node] + i]];
+            if (cc >= 0) stamp[cc] = k;
+        }
+        int chosen = 0;
+        while (stamp[chosen] == k) ++chosen;
+        color_of[node] = chosen;
+        if (chosen + 1 > ncolors) ncolors = chosen + 1;
+    }
+    free(cnt); free(inc); free(fill); free(adjcnt); free(adj); free(afill); free(deg); free(order); free(stamp);
+    return ncolors;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* Moreau drift: integration_base.cpp:6-30, math_helper.hpp:11-17.                          */
+/* ---------------------------------------------------------------------------------------- */
+void orc_position_update(int nb, const int* dof, double* q, const double* v, double h) {
+    int qo = 0, vo = 0;
+    for (int b = 0; b < nb; ++b) {
+        double* x = q + qo;
+        const double* u = v + vo;
+        x[0] += h * u[0]; x[1] += h * u[1]; x[2] += h * u[2];
+        if (dof[b] == 6) {
+            double* P = x + 3; /* (x, y, z, w) coefficient order */
+            const double qp[4] = {P[0], P[1], P[2], P[3]};
+            const double* w = u + 3;
+            const double w0 = P[3];
+            P[3] -= h * 0.5 * (P[0] * w[0] + P[1] * w[1] + P[2] * w[2]);
+            double cr[3];
+            orc_cross(P, w, cr);
+            const double n0 = P[0] + h * 0.5 * (w0 * w[0] + cr[0]);
+            const double n1 = P[1] + h * 0.5 * (w0 * w[1] + cr[1]);
+            const double n2 = P[2] + h * 0.5 * (w0 * w[2] + cr[2]);
+            P[0] = n0; P[1] = n1; P[2] = n2;
+            /* alignQuaternionTo: non-finite -> previous; normalize; flip to q_prev hemisphere */
+            if (!(isfinite(P[0]) && isfinite(P[1]) && isfinite(P[2]) && isfinite(P[3]))) {
+                P[0] = qp[0]; P[1] = qp[1]; P[2] = qp[2]; P[3] = qp[3];
+            } else {
+                const double n2s = P[0] * P[0] + P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+                if (n2s > 0) {
+                    const double nn = sqrt(n2s);
+                    P[0] /= nn; P[1] /= nn; P[2] /= nn; P[3] /= nn;
+                }
+                const double dt = qp[0] * P[0] + qp[1] * P[1] + qp[2] * P[2] + qp[3] * P[3];
+                if (dt < 0) { P[0] = -P[0]; P[1] = -P[1]; P[2] = -P[2]; P[3] = -P[3]; }
+            }
+            qo += 7;
+        } else {
+            qo += 3;
+        }
+        vo += dof[b];
+    }
+}
+
+/* f_gyro = -omega x (I omega) with body-frame omega, I = diag (world.cpp:152-173). */
+void orc_gyro_force(int nb, const int* dof, const double* minv, const double* v, double* f) {
+    int vo = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int j = 0; j < dof[b]; ++j) f[vo + j] = 0.0;
+        if (dof[b] == 6) {
+            const double* w = v + vo + 3;
+            double I[3], Iw[3], c[3];
+            for (int j = 0; j < 3; ++j) I[j] = minv[vo + 3 + j] > 0 ? 1.0 / minv[vo + 3 + j] : 0.0;
+            for (int j = 0; j < 3; ++j) Iw[j] = I[j] * w[j];
+            orc_cross(w, Iw, c);
+            for (int j = 0; j < 3; ++j) f[vo + 3 + j] = -c[j];
+        }
+        vo += dof[b];
+    }
+}
+
+/* Natural-map complementarity residual with diagonal Delassus scaling:
+ * ||p - Pi_K(p - D^-1 (W v + c + e .* (W v_n + c)))|| / max(||p||, tiny).  Fixed points of PJ,
+ * PGS and colored GS (all of which use a positive multiple of D^-1) are its zeros. */
+double orc_complementarity(const orc_bodies* B, const orc_contacts* C, const double* v, const double* imp) {
+    orc_W W;
+    if (orc_build_W(B, C, &W)) { orc_free_W(&W); return NAN; }
+    const int nr = W.nrows;
+    double* g = (double*)malloc(sizeof(double) * ((size_t)nr + 1));
+    double* g0 = (double*)malloc(sizeof(double) * ((size_t)nr + 1));
+    double* D = (double*)malloc(sizeof(double) * ((size_t)nr + 1));
+    spmv(&W, v, g);
+    spmv(&W, B->v, g0);
+    delassus_diag(&W, B->minv, D);
+    double num = 0, den = 0;
+    for (int ci = 0; ci < C->nc; ++ci) {
+        const int base = W.row_base[ci];
+        if (base < 0) continue;
+        const int n = W.row_size[ci];
+        double p[3] = {imp[3 * ci], n > 1 ? imp[3 * ci + 1] : 0, n > 1 ? imp[3 * ci + 2] : 0};
+        double z[3];
+        for (int k = 0; k < n; ++k) {
+            const int r = base + k;
+            const double gam = g[r] + W.cvec[r] + W.rest[r] * (g0[r] + W.cvec[r]);
+            z[k] = p[k] - (D[r] > 0 ? gam / D[r] : 0.0);
+        }
+        z[0] = dmax(z[0], 0.0);
+        if (n > 1) {
+            const double tn = sqrt(z[1] * z[1] + z[2] * z[2]);
+            if (tn > 0) {
+                const double s = dmin(1.0, W.mu[base] * z[0] / tn);
+                z[1] *= s; z[2] *= s;
+            }
+        }
+        for (int k = 0; k < n; ++k) { num += (p[k] - z[k]) * (p[k] - z[k]); den += p[k] * p[k]; }
+    }
+    free(g); free(g0); free(D);
+    orc_free_W(&W);
+    return sqrt(num) / dmax(sqrt(den), 1e-300);
+}

@@ -1,0 +1,361 @@
+"""Pins the CPU oracle (oracle/cdx_oracle.c) against what the mathematics fixes.
+
+The reference ships no tests or golden vectors for this path, so each restated function is
+checked against closed forms, brute force on tiny inputs, invariants and textbook routines --
+chosen so that a dropped term, a sign flip or a transposed operand fails at least one check.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from cardillocxx_amd import scenes
+
+G = 9.81
+
+
+def plane_scene(r=0.05, m=1.0, mu=0.5, pos=(0, 0, 0.05)):
+    q = np.array([[*pos, 0, 0, 0, 1.0]])
+    return scenes.SphereScene(q=q, radius=np.array([r]), mass=np.array([m]), mu_body=np.array([mu]),
+                              walls=(np.array([[0, 0, 1.0]]), np.array([0.0]), np.array([mu])))
+
+
+def point_mass_on_plane(m=2.0, v=(0.0, 0.0, -1.0), mu=0.4, rest=0.0):
+    """One point mass (3 dof) touching a static plane: normal +z, A = plane, B = mass."""
+    return scenes.ContactProblem(
+        dof=np.array([3], np.int32), minv=np.full(3, 1 / m), v=np.array(v, float), fext=np.array([0, 0, -m * G]),
+        fgyro=np.zeros(3), q=np.zeros(3), body_a=np.array([-1], np.int32), body_b=np.array([0], np.int32),
+        mu=np.array([mu]), frame=np.array([[0, 0, 1, 1, 0, 0, 0, 1, 0.0]]), angA=np.zeros((1, 9)),
+        angB=np.zeros((1, 9)), rest=np.full((1, 3), rest), vbias=np.zeros((1, 3)), warm=np.zeros((1, 3)))
+
+
+def test_pj_single_contact_closed_form():
+    """S = M: each sweep scales the normal gap velocity by (1 - alpha):
+    p_k = m_eff * gamma0 * (1 - (1 - alpha)^k).  Pins R, the sweep, the S-solve and iters."""
+    prob = plane_scene().problem()
+    prob.v[2] = -1.0
+    dt = 1e-3
+    for alpha in (0.3, 0.7):
+        for k in (1, 2, 7, 23):
+            st, v, imp, res = oracle.pj_solve(prob, dt, oracle.params(alpha=alpha, max_iter=k, tol_abs=0, tol_rel=0,
+                                                                      warmstart=0))
+            assert st == 0 and res.iters == k
+            g0 = -1.0 - G * dt
+            expect = -1.0 * g0 * (1 - (1 - alpha) ** k)
+            assert imp[0, 0] == pytest.approx(expect, rel=1e-13)
+            assert v[2] == pytest.approx(g0 + expect, rel=1e-12, abs=1e-15)
+
+
+def test_free_fall_and_empty_contact_set():
+    prob = plane_scene(pos=(0, 0, 1.0)).problem()  # no contact (sphere far above the plane)
+    assert prob.nc == 0
+    prob.v[:] = [0.1, -0.2, 0.3, 1.0, 2.0, 3.0]
+    dt = 1e-3
+    for solve in (oracle.pj_solve, oracle.pgs_solve, oracle.colored_gs_solve):
+        st, v, imp, res = solve(prob, dt, oracle.params())
+        assert st == 0 and res.iters == 0
+        np.testing.assert_allclose(v[:3], [0.1, -0.2, 0.3 - G * dt], rtol=1e-14)
+        np.testing.assert_allclose(v[3:], [1, 2, 3], rtol=1e-14)
+
+
+@pytest.mark.parametrize("vt,expect_stick", [((0.1, 0.0), True), ((3.0, -2.0), False)])
+def test_coulomb_stick_and_slip(vt, expect_stick):
+    """Point mass on a plane: converged tangential impulse is -m v_t (stick) when
+    |m v_t| <= mu p_n, otherwise mu p_n opposite to the slip (slip).
+
+    PGS stores its tangential impulses with the sign of its internal lambda, i.e. opposite to
+    the physical impulse: storeImpulse(..., invertNormalSign=true) flips only the normal
+    (projected_gauss_seidel.cpp:265, warmstart.hpp:80-95) while u = u_free - M^-1 W^T lambda."""
+    m, mu, dt = 2.0, 0.4, 1e-3
+    prob = point_mass_on_plane(m=m, v=(vt[0], vt[1], -1.0), mu=mu)
+    for solve, tsign in ((oracle.pj_solve, 1.0), (oracle.pgs_solve, -1.0)):
+        st, v, imp, res = solve(prob, dt, oracle.params(tol_abs=1e-14, tol_rel=1e-14, max_iter=100000))
+        assert st == 0
+        pn = imp[0, 0]
+        assert pn == pytest.approx(m * (1.0 + G * dt), rel=1e-9)
+        assert v[2] == pytest.approx(0.0, abs=1e-9)
+        # frame: t1 = +x, t2 = +y
+        pt = tsign * imp[0, 1:]
+        if expect_stick:
+            np.testing.assert_allclose(pt, -m * np.array(vt), rtol=1e-8)
+            np.testing.assert_allclose(v[:2], 0.0, atol=1e-9)
+        else:
+            assert np.linalg.norm(pt) == pytest.approx(mu * pn, rel=1e-9)
+            d = np.array(vt) / np.linalg.norm(vt)
+            np.testing.assert_allclose(pt / np.linalg.norm(pt), -d, atol=1e-7)
+
+
+@pytest.mark.parametrize("e", [0.0, 0.5, 1.0])
+def test_restitution_target(e):
+    """bias = R .* (c + e .* (W v_n + c)) makes the converged normal velocity -e * v_n."""
+    prob = point_mass_on_plane(v=(0, 0, -1.0), mu=0.0, rest=e)
+    dt = 1e-4
+    st, v, imp, res = oracle.pj_solve(prob, dt, oracle.params(tol_abs=1e-14, tol_rel=1e-14))
+    assert st == 0
+    assert v[2] == pytest.approx(e * 1.0, abs=1e-9)
+    st, v2, imp2, res = oracle.pgs_solve(prob, dt, oracle.params(tol_abs=1e-14, tol_rel=1e-14))
+    assert v2[2] == pytest.approx(e * 1.0, abs=1e-9)
+
+
+def test_head_on_collision_momentum():
+    """Two equal free spheres approaching head-on (frictionless, e=0, no gravity):
+    common velocity after the solve, momentum conserved exactly."""
+    r = 0.05
+    sc = scenes.SphereScene(q=np.array([[0, 0, 0, 0, 0, 0, 1.0], [2 * r, 0, 0, 0, 0, 0, 1.0]]),
+                            radius=np.array([r, r]), mass=np.array([1.0, 1.0]), mu_body=np.array([0.5, 0.5]),
+                            walls=(np.zeros((0, 3)), np.zeros(0), np.zeros(0)))
+    prob = sc.problem(friction=False)
+    prob.fext[:] = 0
+    prob.v[:] = 0
+    prob.v[0], prob.v[6] = 1.0, -3.0
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=1e-15, tol_rel=1e-15))
+    assert st == 0
+    assert v[0] == pytest.approx(-1.0, abs=1e-12) and v[6] == pytest.approx(-1.0, abs=1e-12)
+    assert v[0] + v[6] == pytest.approx(-2.0, abs=1e-13)
+    assert imp[0, 0] == pytest.approx(2.0, rel=1e-12)
+
+
+def delassus(prob):
+    """Dense W, M^-1 for tiny problems (independent of the oracle's sparse assembly)."""
+    nv = prob.nv
+    voff = np.concatenate([[0], np.cumsum(prob.dof)])
+    rows, c_rows, mu_rows = [], [], []
+    order = [c for c in range(prob.nc) if prob.mu[c] <= 0] + [c for c in range(prob.nc) if prob.mu[c] > 0]
+    for c in order:
+        nr = 3 if prob.mu[c] > 0 else 1
+        for k in range(nr):
+            w = np.zeros(nv)
+            d = prob.frame[c, 3 * k:3 * k + 3]
+            for body, s, ang in ((prob.body_a[c], -1, prob.angA[c, 3 * k:3 * k + 3]),
+                                 (prob.body_b[c], 1, prob.angB[c, 3 * k:3 * k + 3])):
+                if body < 0:
+                    continue
+                o = voff[body]
+                w[o:o + 3] = s * d
+                if prob.dof[body] == 6:
+                    w[o + 3:o + 6] = s * ang
+            rows.append(w)
+            c_rows.append(prob.vbias[c, k])
+    return np.array(rows), np.array(c_rows), order
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_frictionless_lcp_brute_force(seed):
+    """For 2-4 frictionless contacts, enumerate the active sets of
+    0 <= p  _|_  W M^-1 W^T p + (W u_free + c) >= 0 and compare with the converged PJ, PGS and
+    coloured GS."""
+    rng = np.random.default_rng(seed)
+    nc = 2 + seed % 3
+    prob = scenes.random_problem(nb=3, nc=nc, seed=100 + seed, p_static=0.3, p_frictionless=1.0, p_point=0.0,
+                                 warm=False)
+    prob.v[:] = rng.normal(size=prob.nv)
+    dt = 1e-3
+    W, cvec, order = delassus(prob)
+    uf = prob.v + prob.minv * (dt * prob.fext) + prob.minv * (dt * prob.fgyro)
+    A = W @ np.diag(prob.minv) @ W.T
+    b = W @ uf + cvec
+    best = None
+    for act in itertools.product([0, 1], repeat=len(b)):
+        S = [i for i in range(len(b)) if act[i]]
+        p = np.zeros(len(b))
+        if S:
+            try:
+                p[S] = np.linalg.solve(A[np.ix_(S, S)], -b[S])
+            except np.linalg.LinAlgError:
+                continue
+        w = A @ p + b
+        if (p >= -1e-12).all() and (w >= -1e-10).all() and abs(p @ w) < 1e-10:
+            best = p
+            break
+    assert best is not None
+    v_exact = uf + prob.minv * (W.T @ best)
+    tight = oracle.params(tol_abs=1e-15, tol_rel=1e-15, max_iter=2_000_000)
+    for solve in (oracle.pj_solve, oracle.pgs_solve, oracle.colored_gs_solve):
+        st, v, imp, res = solve(prob, dt, tight)
+        assert st == 0
+        p_solver = imp[order, 0]
+        np.testing.assert_allclose(p_solver, best, atol=1e-8 * max(1, np.abs(best).max()))
+        np.testing.assert_allclose(v, v_exact, atol=1e-8 * max(1, np.abs(v_exact).max()))
+
+
+def test_single_frictional_contact_dense_projection():
+    """One frictional contact between two rigid bodies: the PJ fixed point satisfies
+    p = Pi_K(p - D^-1 (W v + c)) -- checked with a dense evaluation of W and the disk projection."""
+    prob = scenes.random_problem(nb=2, nc=1, seed=7, p_static=0.0, p_frictionless=0.0, p_point=0.0, warm=False)
+    prob.body_a[:] = 0
+    prob.body_b[:] = 1
+    W, cvec, _ = delassus(prob)
+    # make the contact closing so the normal impulse is active
+    prob.v[:] = 0
+    prob.v[:3] = 2.0 * prob.frame[0, :3]
+    dt = 1e-3
+    st, v, imp, res = oracle.pj_solve(prob, dt, oracle.params(tol_abs=1e-15, tol_rel=1e-15, max_iter=500000))
+    assert st == 0
+    p = imp[0]
+    D = np.einsum("ij,j,ij->i", W, prob.minv, W)
+    z = p - (W @ v + cvec) / D
+    z[0] = max(z[0], 0)
+    tn = np.linalg.norm(z[1:])
+    if tn > 0:
+        z[1:] *= min(1, prob.mu[0] * z[0] / tn)
+    np.testing.assert_allclose(p, z, atol=1e-9 * np.abs(p).max())
+    assert p[0] > 0
+    assert oracle.complementarity(prob, v, imp) < 1e-9
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_invariants_and_momentum(seed):
+    """After every solve: p_n >= 0, |p_t| <= mu p_n; internal contacts conserve momentum."""
+    prob = scenes.random_problem(nb=10, nc=30, seed=seed, p_static=0.0, p_point=0.2)
+    prob.fext[:] = 0
+    prob.fgyro[:] = 0
+    for k in (1, 3, 17):
+        st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(max_iter=k, tol_abs=0, tol_rel=0))
+        assert (imp[:, 0] >= 0).all()
+        tn = np.linalg.norm(imp[:, 1:], axis=1)
+        assert (tn <= prob.mu * imp[:, 0] * (1 + 1e-12) + 1e-300).all()
+        assert (imp[prob.mu <= 0, 1:] == 0).all()
+        voff = np.concatenate([[0], np.cumsum(prob.dof)])
+        mom = lambda vel: sum(vel[voff[b]:voff[b] + 3] / prob.minv[voff[b]] for b in range(prob.nb))
+        np.testing.assert_allclose(mom(v), mom(prob.v), atol=1e-12 * np.abs(prob.v).max() * 10)
+
+
+def test_residual_cadence_and_tolerance():
+    """The PJ loop checks the residual only when iter % 10 == 0, so a converged solve ends at
+    1, 11, 21, ... sweeps; with tol 0 it runs to max_iter.  PGS checks every sweep."""
+    prob = scenes.random_problem(nb=12, nc=40, seed=3, warm=True)
+    for tol in (1e-3, 1e-6, 1e-9):
+        st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=tol, tol_rel=tol))
+        assert res.iters % 10 == 1 and res.residual <= 1
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=0, tol_rel=0, max_iter=37))
+    assert res.iters == 37
+    st, v, imp, res = oracle.pgs_solve(prob, 1e-3, oracle.params(tol_abs=1e-6, tol_rel=1e-6))
+    assert res.residual < 1
+
+
+def test_pj_pgs_same_fixed_point_frictionless():
+    """With a frictionless, strictly convex problem the solution is unique: PJ, PGS and
+    coloured GS agree when run to tight tolerance."""
+    prob = scenes.random_problem(nb=8, nc=12, seed=11, p_frictionless=1.0, p_point=0.0)
+    tight = oracle.params(tol_abs=1e-14, tol_rel=1e-14, max_iter=5_000_000)
+    out = [s(prob, 1e-3, tight) for s in (oracle.pj_solve, oracle.pgs_solve, oracle.colored_gs_solve)]
+    for st, v, imp, res in out:
+        assert st == 0
+    for st, v, imp, res in out[1:]:
+        np.testing.assert_allclose(v, out[0][1], atol=1e-9)
+
+
+def test_colored_equals_pgs_when_single_colour():
+    """If no two contacts share a body there is one colour and the coloured sweep visits the
+    blocks in row order: identical to PGS bit for bit."""
+    prob = scenes.random_problem(nb=20, nc=10, seed=5)
+    prob.body_a[:] = np.arange(0, 20, 2)
+    prob.body_b[:] = np.arange(1, 20, 2)
+    prm = oracle.params(tol_abs=1e-10, tol_rel=1e-10)
+    a = oracle.pgs_solve(prob, 1e-3, prm)
+    b = oracle.colored_gs_solve(prob, 1e-3, prm)
+    assert b[3].ncolors == 1
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]) and a[3].iters == b[3].iters
+
+
+def test_error_paths():
+    prob = scenes.random_problem(nb=4, nc=6, seed=1)
+    bad = prob.copy()
+    bad.minv[0:3] = 0.0  # zero mass on a dynamic body: S singular
+    assert oracle.pj_solve(bad, 1e-3, oracle.params())[0] == oracle.ORC_NOT_FACTORIZED
+    # alpha >> 2 makes Gauss-Seidel blow up: NaN/Inf residual -> diverged
+    st = oracle.pgs_solve(prob, 1e-3, oracle.params(alpha=1e6, max_iter=5000, tol_abs=1e-12, tol_rel=1e-12))[0]
+    assert st == oracle.ORC_DIVERGED
+    bad = prob.copy()
+    bad.body_b[0] = 99
+    assert oracle.pj_solve(bad, 1e-3, oracle.params())[0] == oracle.ORC_BAD_ARG
+
+
+def test_tangent_frame_textbook():
+    rng = np.random.default_rng(0)
+    ns = [rng.normal(size=3) for _ in range(200)] + [np.array([0, 0, -1.0]), np.array([1e-9, 0, -1.0]),
+                                                    np.array([0, 0, 1.0]), np.array([0, 0, 5.0])]
+    for n in ns:
+        t1, t2 = oracle.tangent_frame(n)
+        nn = n / np.linalg.norm(n)
+        B = np.stack([t1, t2, nn])
+        np.testing.assert_allclose(B @ B.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(B) == pytest.approx(1.0, abs=1e-12)  # (t1, t2, n) right-handed
+    t1, t2 = oracle.tangent_frame(np.array([0, 0, -1.0]))
+    np.testing.assert_allclose(t1, [0, -1, 0])  # south-pole branch
+    t1, t2 = oracle.tangent_frame(np.array([0, 0, 1.0]))
+    np.testing.assert_allclose(t1, [1, 0, 0])
+    t1, t2 = oracle.tangent_frame(np.zeros(3))
+    np.testing.assert_allclose(t1, [1, 0, 0]) and np.testing.assert_allclose(t2, [0, 1, 0])
+
+
+def test_welsh_powell_valid_and_greedy():
+    """Valid colouring (no shared body within a colour) and exactly the greedy result of a
+    brute-force restatement on small random graphs."""
+    rng = np.random.default_rng(2)
+    for trial in range(20):
+        nb, nc = 6, int(rng.integers(1, 14))
+        a = rng.integers(-1, nb, nc).astype(np.int32)
+        b = rng.integers(0, nb, nc).astype(np.int32)
+        b = np.where(a == b, (b + 1) % nb, b).astype(np.int32)
+        ncol, col = oracle.color_blocks(nb, a, b)
+        adj = [set() for _ in range(nc)]
+        for i in range(nc):
+            for j in range(nc):
+                if i != j and ({a[i], b[i]} - {-1}) & ({a[j], b[j]} - {-1}):
+                    adj[i].add(j)
+        for i in range(nc):
+            for j in adj[i]:
+                assert col[i] != col[j]
+        order = sorted(range(nc), key=lambda i: (-len(adj[i]), i))
+        ref = [-1] * nc
+        for i in order:
+            used = {ref[j] for j in adj[i] if ref[j] >= 0}
+            c = 0
+            while c in used:
+                c += 1
+            ref[i] = c
+        assert list(col) == ref and ncol == max(ref) + 1
+
+
+def quat_mul(p, q):
+    px, py, pz, pw = p
+    qx, qy, qz, qw = q
+    return np.array([pw * qx + px * qw + py * qz - pz * qy, pw * qy - px * qz + py * qw + pz * qx,
+                     pw * qz + px * qy - py * qx + pz * qw, pw * qw - px * qx - py * qy - pz * qz])
+
+
+def test_quaternion_drift():
+    """q += h/2 q (x) (0, omega_body), normalised, sign-aligned; agrees with the exponential map
+    to O(h^2); point masses only translate."""
+    rng = np.random.default_rng(4)
+    q0 = rng.normal(size=4)
+    q0 /= np.linalg.norm(q0)
+    w = rng.normal(size=3)
+    v = np.concatenate([[1.0, 2.0, 3.0], w])
+    dof = np.array([6, 3], np.int32)
+    q = np.concatenate([[0, 0, 0], q0, [5, 5, 5]])
+    vv = np.concatenate([v, [0.5, 0, -1]])
+    for h in (1e-2, 1e-3):
+        qn = oracle.position_update(dof, q, vv, h)
+        np.testing.assert_allclose(qn[:3], h * v[:3], rtol=1e-14)
+        np.testing.assert_allclose(qn[7:], [5 + 0.5 * h, 5, 5 - h], rtol=1e-14)
+        th = np.linalg.norm(w) * h
+        dq = np.concatenate([np.sin(th / 2) * w / np.linalg.norm(w), [np.cos(th / 2)]])
+        exact = quat_mul(q0, dq)  # body-frame omega: right multiplication
+        assert np.linalg.norm(qn[3:7]) == pytest.approx(1.0, abs=1e-15)
+        assert np.linalg.norm(qn[3:7] - exact) < 2 * (np.linalg.norm(w) * h) ** 2
+        assert np.dot(qn[3:7], q0) > 0
+
+
+def test_gyroscopic_force():
+    dof = np.array([6], np.int32)
+    minv = np.array([1, 1, 1, 1 / 2.0, 1 / 3.0, 1 / 5.0])
+    v = np.array([0, 0, 0, 0.3, -0.7, 1.1])
+    f = oracle.gyro_force(dof, minv, v)
+    I = np.array([2.0, 3.0, 5.0])
+    np.testing.assert_allclose(f[3:], -np.cross(v[3:], I * v[3:]), rtol=1e-15)
+    # isotropic inertia: omega x (I omega) = 0
+    f = oracle.gyro_force(dof, np.array([1, 1, 1, 0.5, 0.5, 0.5]), v)
+    np.testing.assert_allclose(f, 0, atol=1e-15)
