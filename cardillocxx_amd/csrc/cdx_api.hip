@@ -363,10 +363,11 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
             for (int k = 0; k < 9; ++k) J[(size_t)(18 + k) * nc + c] = 0.0;
         const bool fr = mu[c] > 0;
         m[c] = fr ? mu[c] : 0.0;
+        const bool has_rows = ia[c] >= 0 || ib[c] >= 0;
         for (int k = 0; k < 3; ++k) {
-            const bool row = k == 0 || fr;
+            const bool row = has_rows && (k == 0 || fr);
             r[(size_t)k * nc + c] = row ? rest[3 * (size_t)c + k] : 0.0;
-            if (row && rest[3 * (size_t)c + k] != 0.0 && (ia[c] >= 0 || ib[c] >= 0)) any_rest = true;
+            if (row && rest[3 * (size_t)c + k] != 0.0) any_rest = true;
             vb[(size_t)k * nc + c] = row ? vbias[3 * (size_t)c + k] : 0.0;
             w[(size_t)k * nc + c] = row ? warm[3 * (size_t)c + k] : 0.0;
         }
