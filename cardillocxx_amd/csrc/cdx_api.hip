@@ -211,7 +211,8 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void* bufs[] = {ctx->d_x, ctx->d_v, ctx->d_minv, ctx->d_fext, ctx->d_fgyro, ctx->d_q, ctx->d_uprev, ctx->d_ufree, ctx->d_dof,
                     ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_p, ctx->d_R, ctx->d_bias,
-                    ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial};
+                    ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial,
+                    ctx->d_partial2, ctx->d_pk, ctx->d_nest, ctx->d_scene_i, ctx->d_scene_res};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);
@@ -321,6 +322,7 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
         ctx->incidence_valid = false;
         ctx->coloring_valid = false;
     }
+    ctx->nscenes = 0;
     ctx->solved = false;
     return CDX_OK;
 }
@@ -415,6 +417,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->incidence_valid = false;
     ctx->coloring_valid = false;
+    ctx->nscenes = 0;
     ctx->solved = false;
     return build_incidence(ctx);
 }

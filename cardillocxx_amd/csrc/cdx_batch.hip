@@ -1,0 +1,243 @@
+// cdx_batch.hip -- thousands of independent scenes solved by projected Jacobi in one launch.
+//
+// The reference runs one PhysicsEngine per scene (physics_engine.hpp:215-221), so a batch is a
+// sequence of independent ProjectedJacobiSolver::solve calls.  Here every scene is a contiguous
+// range of the context's body and contact arrays (cdx_set_scenes) and one launch solves them all:
+// a workgroup of B threads hosts G scenes (B/G threads each, a multiple of the wave size) and
+// runs each scene's whole solve -- setup, sweeps, residual every `check_every` sweeps, own
+// convergence test -- with workgroup barriers between the contact and body phases.  The
+// per-contact and per-body arithmetic is the same device code as the grid-wide solve
+// (cdx_pj_device.hpp), so every scene's result equals a standalone solve of that scene.
+// Scenes are launched largest first (longest-processing-time order) so the hardware's dynamic
+// workgroup dispatch balances uneven scene sizes.  (B, G) comes from the autotuner below.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "cdx_pj_device.hpp"
+
+namespace cdx {
+
+constexpr int kMaxBatchBlock = 512;
+
+__global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
+    int ns, int groups, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
+    const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
+    const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ mu,
+    const double* __restrict__ rest, const double* __restrict__ vb, double* __restrict__ R, double* __restrict__ bias,
+    double* __restrict__ p, double* __restrict__ imp, const int* __restrict__ dof, const double* __restrict__ minv,
+    const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
+    const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
+    int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
+    double* __restrict__ res_out) {
+    __shared__ double wsum[kMaxBatchBlock / 64];
+    __shared__ int gdone[kMaxBatchBlock / 64];
+    __shared__ int all_done;
+    const int gsize = blockDim.x / groups;
+    const int g = threadIdx.x / gsize, lt = threadIdx.x - g * gsize;
+    const int slot = blockIdx.x * groups + g;
+    const int s = slot < ns ? order[slot] : -1;
+    const int b0 = s >= 0 ? boff[s] : 0, b1 = s >= 0 ? boff[s + 1] : 0;
+    const int c0 = s >= 0 ? coff[s] : 0, c1 = s >= 0 ? coff[s + 1] : 0;
+    const bool rows = s >= 0 && srows[s] > 0;
+    // setup: R, b, p0 and the side impulses of W^T p0, then x0
+    if (rows)
+        for (int c = c0 + lt; c < c1; c += gsize)
+            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+    __syncthreads();
+    for (int b = b0 + lt; b < b1; b += gsize) pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
+    if (lt == 0) gdone[g] = rows ? 0 : 1;
+    bool done = !rows;
+    int iters = 0;
+    double res = rows ? NAN : NAN;
+    __syncthreads();
+    const int waves_per_group = gsize >> 6;
+    for (int it = 0; it < max_iter; ++it) {
+        if (!done)
+            for (int c = c0 + lt; c < c1; c += gsize) pj_contact_sweep(c, nc, ida, idb, J, R, bias, mu, p, x, imp);
+        __syncthreads();
+        const int check = (it % check_every) == 0;
+        double e2 = 0;
+        if (!done)
+            for (int b = b0 + lt; b < b1; b += gsize)
+                e2 += pj_body_sweep(b, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs);
+        if (check) {
+            for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+            if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
+        }
+        __syncthreads();
+        if (check && !done) {
+            double t = 0;
+            for (int w = 0; w < waves_per_group; ++w) t += wsum[g * waves_per_group + w];
+            res = 1.0 / sqrt((double)snv[s]) * sqrt(t);
+            if (res <= 1.0) {
+                done = true;
+                iters = it + 1;
+            }
+        }
+        if (!done) iters = it + 1;
+        if (lt == 0) gdone[g] = done ? 1 : 0;
+        if (threadIdx.x == 0) all_done = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int a = 1;
+            for (int k = 0; k < groups; ++k) a &= gdone[k];
+            all_done = a;
+        }
+        __syncthreads();
+        if (all_done) break;
+    }
+    if (s >= 0 && lt == 0) {
+        iters_out[s] = rows ? iters : 0;
+        res_out[s] = res;
+    }
+}
+
+int upload_scenes(cdx_ctx* ctx) {
+    const int ns = ctx->nscenes;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_scene_i, 6 * (size_t)ns + 2, &ctx->scene_cap, "scene tables"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_scene_res, (size_t)ns, &ctx->scene_res_cap, "scene residuals"));
+    std::vector<int> buf(6 * (size_t)ns + 2, 0);
+    // [order ns | boff ns+1 | coff ns+1 | nv ns | rows ns | iters ns]
+    std::copy(ctx->scene_order.begin(), ctx->scene_order.end(), buf.begin());
+    std::copy(ctx->scene_boff.begin(), ctx->scene_boff.end(), buf.begin() + ns);
+    std::copy(ctx->scene_coff.begin(), ctx->scene_coff.end(), buf.begin() + 2 * ns + 1);
+    std::copy(ctx->scene_nv.begin(), ctx->scene_nv.end(), buf.begin() + 3 * ns + 2);
+    std::copy(ctx->scene_rows.begin(), ctx->scene_rows.end(), buf.begin() + 4 * ns + 2);
+    CDX_HIP(ctx, hipMemcpy(ctx->d_scene_i, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice));
+    return CDX_OK;
+}
+
+static int launch_batch(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, int block, int groups) {
+    const int ns = ctx->nscenes;
+    const int* d = ctx->d_scene_i;
+    const int grid = (ns + groups - 1) / groups;
+    CDX_LAUNCH(ctx, "batch_pj", k_batch_pj, grid, block, ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2,
+               d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R,
+               ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,
+               ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha, prm->warmstart, prm->max_iter, prm->check_every,
+               prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+    return hip_check(ctx, hipGetLastError(), "batch launch");
+}
+
+static bool valid_launch(int block, int groups) {
+    return (block == 64 || block == 128 || block == 256 || block == 512) && groups >= 1 && block % groups == 0 &&
+           (block / groups) % 64 == 0;
+}
+
+}  // namespace cdx
+
+using namespace cdx;
+
+extern "C" {
+
+int cdx_set_scenes(cdx_ctx* ctx, int nscenes, const int* body_off, const int* contact_off) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (nscenes < 1 || !body_off || !contact_off) return set_err(ctx, CDX_BAD_ARG, "cdx_set_scenes: bad arguments");
+    if (body_off[0] != 0 || contact_off[0] != 0 || body_off[nscenes] != ctx->nb || contact_off[nscenes] != ctx->nc)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_set_scenes: offsets must start at 0 and cover all bodies and contacts");
+    std::vector<int> nv(nscenes, 0), rows(nscenes, 0), cost(nscenes, 0);
+    for (int s = 0; s < nscenes; ++s) {
+        if (body_off[s + 1] < body_off[s] || contact_off[s + 1] < contact_off[s])
+            return set_err(ctx, CDX_BAD_ARG, "cdx_set_scenes: offsets must be non-decreasing");
+        for (int b = body_off[s]; b < body_off[s + 1]; ++b) nv[s] += ctx->dof[b];
+        for (int c = contact_off[s]; c < contact_off[s + 1]; ++c) {
+            const int a = ctx->h_a[c], b = ctx->h_b[c];
+            if ((a >= 0 && (a < body_off[s] || a >= body_off[s + 1])) || (b >= 0 && (b < body_off[s] || b >= body_off[s + 1])))
+                return set_err(ctx, CDX_BAD_ARG, "cdx_set_scenes: a contact references a body of another scene");
+            if (a >= 0 || b >= 0) rows[s] += ctx->h_mu[c] > 0 ? 3 : 1;
+        }
+        cost[s] = 4 * (contact_off[s + 1] - contact_off[s]) + (body_off[s + 1] - body_off[s]);
+    }
+    ctx->nscenes = nscenes;
+    ctx->scene_boff.assign(body_off, body_off + nscenes + 1);
+    ctx->scene_coff.assign(contact_off, contact_off + nscenes + 1);
+    ctx->scene_nv = nv;
+    ctx->scene_rows = rows;
+    ctx->scene_order.resize(nscenes);
+    std::iota(ctx->scene_order.begin(), ctx->scene_order.end(), 0);
+    std::stable_sort(ctx->scene_order.begin(), ctx->scene_order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    return upload_scenes(ctx);
+}
+
+int cdx_batch_set_launch(cdx_ctx* ctx, int block, int groups) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (!valid_launch(block, groups)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_set_launch: invalid (block, groups)");
+    ctx->batch_block = block;
+    ctx->batch_groups = groups;
+    return CDX_OK;
+}
+
+int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* iters, double* residuals) {
+    (void)theta;
+    if (!ctx) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    CDX_TRY(check_solve_args(ctx, prm));
+    if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: only the standard loop is batched");
+    if (ctx->nscenes < 1 || ctx->scene_boff.back() != ctx->nb || ctx->scene_coff.back() != ctx->nc)
+        return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
+    ctx->solved = false;
+    CDX_TRY(build_incidence(ctx));
+    CDX_TRY(launch_batch(ctx, dt, prm, ctx->batch_block, ctx->batch_groups));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int ns = ctx->nscenes;
+    if (iters) CDX_HIP(ctx, hipMemcpy(iters, ctx->d_scene_i + 5 * ns + 2, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    if (residuals) CDX_HIP(ctx, hipMemcpy(residuals, ctx->d_scene_res, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    ctx->solved = true;
+    return CDX_OK;
+}
+
+// Times every valid (block, groups) on the current batch and keeps the fastest.  The warm-start
+// impulses are restored before each trial so every candidate solves the same problem.
+int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* block_out, int* groups_out,
+                       double* best_ms) {
+    (void)theta;
+    if (!ctx) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    CDX_TRY(check_solve_args(ctx, prm));
+    if (ctx->nscenes < 1) return set_err(ctx, CDX_NO_STATE, "cdx_batch_autotune: scenes not set");
+    CDX_TRY(build_incidence(ctx));
+    const size_t np = 3 * (size_t)ctx->nc;
+    double* saved = nullptr;
+    CDX_HIP(ctx, hipMalloc((void**)&saved, sizeof(double) * (np ? np : 1)));
+    if (np) CDX_HIP(ctx, hipMemcpyAsync(saved, ctx->d_p, sizeof(double) * np, hipMemcpyDeviceToDevice, ctx->stream));
+    const int cands[][2] = {{64, 1}, {128, 1}, {128, 2}, {256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {512, 4}, {512, 8}};
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double best = 1e300;
+    int bb = ctx->batch_block, bg = ctx->batch_groups;
+    int st = CDX_OK;
+    for (auto& cd : cands) {
+        float ms = 0;
+        for (int rep = 0; rep < 2 && st == CDX_OK; ++rep) {  // first run warms caches and code
+            if (np) hipMemcpyAsync(ctx->d_p, saved, sizeof(double) * np, hipMemcpyDeviceToDevice, ctx->stream);
+            hipEventRecord(e0, ctx->stream);
+            st = launch_batch(ctx, dt, prm, cd[0], cd[1]);
+            hipEventRecord(e1, ctx->stream);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (st) break;
+        if (ms < best) {
+            best = ms;
+            bb = cd[0];
+            bg = cd[1];
+        }
+    }
+    if (np) hipMemcpyAsync(ctx->d_p, saved, sizeof(double) * np, hipMemcpyDeviceToDevice, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(saved);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (st) return st;
+    ctx->batch_block = bb;
+    ctx->batch_groups = bg;
+    if (block_out) *block_out = bb;
+    if (groups_out) *groups_out = bg;
+    if (best_ms) *best_ms = best;
+    return CDX_OK;
+}
+
+}  // extern "C"

@@ -89,11 +89,25 @@ struct cdx_ctx {
     int* d_color_blocks = nullptr;  // [nblocks] contact index per colour-sorted block
     size_t color_cap = 0;
 
+    // ---- batched scenes (cdx_set_scenes); cleared whenever bodies or contacts are set
+    int nscenes = 0;
+    std::vector<int> scene_boff, scene_coff, scene_nv, scene_rows, scene_order;
+    int* d_scene_i = nullptr;  // [order | boff | coff | nv | rows | iters]
+    size_t scene_cap = 0;
+    double* d_scene_res = nullptr;
+    size_t scene_res_cap = 0;
+    int batch_block = 256, batch_groups = 1;
+
     // ---- solve control
     int* d_ctrl = nullptr;         // [0] done flag, [1] iterations, [2] status
     double* d_res = nullptr;       // [0] last residual
     double* d_partial = nullptr;   // per-block residual partial sums
     size_t partial_cap = 0;
+    double* d_partial2 = nullptr;  // Nesterov: residual and restart-dot partials
+    size_t partial2_cap = 0;
+    double* d_pk = nullptr;        // Nesterov: previous impulse iterate [3*nc]
+    size_t pk_cap = 0;
+    double* d_nest = nullptr;      // Nesterov scalars: beta, thk, err_prev, restarts, disabled
     int* h_flag = nullptr;         // pinned mirror of the done flag
     int* h_flag_dev = nullptr;
     bool solved = false;

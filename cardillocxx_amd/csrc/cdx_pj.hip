@@ -18,7 +18,7 @@
 #include <cmath>
 #include <deque>
 
-#include "cdx_device.hpp"
+#include "cdx_pj_device.hpp"
 
 namespace cdx {
 
@@ -36,51 +36,7 @@ __global__ void k_pj_setup_contacts(int nc, const int* __restrict__ ida, const i
                                     double* __restrict__ p, double* __restrict__ imp) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const int a = ida[c], b = idb[c];
-    if (a < 0 && b < 0) {
-        for (int k = 0; k < 3; ++k) {
-            R[(size_t)k * nc + c] = 0;
-            bias[(size_t)k * nc + c] = 0;
-            p[(size_t)k * nc + c] = 0;
-        }
-        return;
-    }
-    const int nr = mu[c] > 0 ? 3 : 1;
-    ContactJ j;
-    load_J(J, nc, c, nr, j);
-    double ua[6], ub[6];
-    load_body(v, a, ua);
-    load_body(v, b, ub);
-    double mA[4] = {0, 0, 0, 0}, mB[4] = {0, 0, 0, 0};
-    if (a >= 0)
-        for (int m = 0; m < 4; ++m) mA[m] = minv[(size_t)a * kMinvStride + m];
-    if (b >= 0)
-        for (int m = 0; m < 4; ++m) mB[m] = minv[(size_t)b * kMinvStride + m];
-    double pw[3] = {0, 0, 0};
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) {
-            R[(size_t)k * nc + c] = 0;
-            bias[(size_t)k * nc + c] = 0;
-            p[(size_t)k * nc + c] = 0;
-            continue;
-        }
-        // D summed side by side in column order (lower body index first), as the row-major
-        // compressed W stores the row (rdiag_sparse iterates the row's entries in column order).
-        double DA = 0, DB = 0;
-        for (int m = 0; m < 3; ++m) DA += j.d[k][m] * j.d[k][m] * mA[0];
-        for (int m = 0; m < 3; ++m) DA += j.aA[k][m] * j.aA[k][m] * mA[1 + m];
-        for (int m = 0; m < 3; ++m) DB += j.d[k][m] * j.d[k][m] * mB[0];
-        for (int m = 0; m < 3; ++m) DB += j.aB[k][m] * j.aB[k][m] * mB[1 + m];
-        const double D = (a >= 0 && b >= 0 && b < a) ? DB + DA : DA + DB;
-        const double Rk = D > 0 ? alpha / D : 0.0;
-        const double ck = vb[(size_t)k * nc + c];
-        const double vprev = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub) + ck;
-        R[(size_t)k * nc + c] = Rk;
-        bias[(size_t)k * nc + c] = Rk * (ck + rest[(size_t)k * nc + c] * vprev);
-        pw[k] = warmstart ? p[(size_t)k * nc + c] : 0.0;
-        p[(size_t)k * nc + c] = pw[k];
-    }
-    write_side_impulses(imp, c, a, b, nr, j, pw);
+    pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
 }
 
 // x0 = S^-1 (M v + dt f_ext + dt f_gyro + W^T p0) (pj_assembler.cpp:150-153,173-182).
@@ -92,34 +48,7 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
                                   int have_contacts, double* __restrict__ x) {
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= nb) return;
-    const int nd = dof[bi];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (have_contacts) {
-        for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
-            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
-            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
-            acc[0] += q0.x;
-            acc[1] += q0.y;
-            acc[2] += q1.x;
-            acc[3] += q1.y;
-            acc[4] += q2.x;
-            acc[5] += q2.y;
-        }
-    }
-    const double* vv = v + (size_t)bi * kBodyStride;
-    const double* fe = fext + (size_t)bi * kBodyStride;
-    const double* fg = fgyro + (size_t)bi * kBodyStride;
-    const double* mi = minv + (size_t)bi * kMinvStride;
-    double out[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < nd; ++k) {
-        const double mk = k < 3 ? mi[0] : mi[k - 2];
-        const double M = mk > 0 ? 1.0 / mk : 0.0;
-        double r = M * vv[k] + dt * fe[k];
-        r += dt * fg[k];
-        r += acc[k];
-        out[k] = r / M;
-    }
-    store6(x + (size_t)bi * kBodyStride, out);
+    pj_setup_body(bi, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, have_contacts, x);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -135,42 +64,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts(int nc, int nblocks, con
     if (ctrl[0]) return;  // converged: the rest of the enqueued sweeps are no-ops
     const int c = xcd_tile(nblocks) * kBlock + threadIdx.x;
     if (c >= nc) return;
-    const int a = ida[c], b = idb[c];
-    if (a < 0 && b < 0) return;
-    const double muc = mu[c];
-    const int nr = muc > 0 ? 3 : 1;
-    ContactJ j;
-    load_J(J, nc, c, nr, j);
-    double ua[6], ub[6];
-    load_body(x, a, ua);
-    load_body(x, b, ub);
-    double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) break;
-        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
-        pold[k] = p[(size_t)k * nc + c];
-        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
-    }
-    // project() (projected_jacobi.cpp:100-119)
-    pn[0] = fmax(pn[0], 0.0);
-    if (nr == 3) {
-        const double t1 = pn[1], t2 = pn[2];
-        const double tn2 = t1 * t1 + t2 * t2;
-        if (tn2 > 0) {
-            const double s = fmin(1.0, (muc * pn[0]) / sqrt(tn2));
-            pn[1] = s * t1;
-            pn[2] = s * t2;
-        }
-    }
-    double dp[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) break;
-        dp[k] = pn[k] - pold[k];
-        p[(size_t)k * nc + c] = pn[k];
-    }
-    write_side_impulses(imp, c, a, b, nr, j, dp);
+    pj_contact_sweep(c, nc, ida, idb, J, R, bias, mu, p, x, imp);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pj_bodies(int nb, const int* __restrict__ dof,
@@ -182,42 +76,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies(int nb, const int* __restr
     __shared__ double red[kBlock / 64];
     const int bi = blockIdx.x * kBlock + threadIdx.x;
     double e2 = 0;
-    if (bi < nb) {
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        const int i0 = incptr[bi], i1 = incptr[bi + 1];
-        for (int i = i0; i < i1; ++i) {
-            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
-            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
-            acc[0] += q0.x;
-            acc[1] += q0.y;
-            acc[2] += q1.x;
-            acc[3] += q1.y;
-            acc[4] += q2.x;
-            acc[5] += q2.y;
-        }
-        double u[6];
-        load_body(x, bi, u);
-        const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
-        const int nd = dof[bi];
-        double un[6];
-        const double Ml = m4.x > 0 ? 1.0 / m4.x : 0.0;
-        const double Ma[3] = {m4.y > 0 ? 1.0 / m4.y : 0.0, m4.z > 0 ? 1.0 / m4.z : 0.0, m4.w > 0 ? 1.0 / m4.w : 0.0};
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double M = k < 3 ? Ml : Ma[k - 3];
-            un[k] = (k < nd) ? u[k] + acc[k] / M : u[k];
-        }
-        if (i1 > i0) store6(x + (size_t)bi * kBodyStride, un);
-        if (check) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                if (k >= nd) break;
-                const double q = fmax(fabs(un[k]), fabs(u[k])) * tol_rel + tol_abs;
-                const double e = (un[k] - u[k]) / q;
-                e2 += e * e;
-            }
-        }
-    }
+    if (bi < nb) e2 = pj_body_sweep(bi, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs);
     if (check) {
         const double s = block_sum(e2, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -288,16 +147,198 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm) {
     return CDX_OK;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Nesterov-accelerated loop (nesterov_loop, projected_jacobi.cpp:379-449), fused into the sweep.
+//
+// Buffers after iteration i: X = xk1_i, XK = xk_i, P = pk1_i, PK = pk_i, and beta_i from the
+// finalize kernel (0 on a restart or once momentum is disabled).  Iteration i+1 forms
+// xy = X + beta (X - XK) and py = P + beta (P - PK) on the fly, sweeps from (xy, py), and
+// rotates XK <- X, PK <- P.  The residual compares xk1 with xy and the restart test uses
+// (xy - xk1) . (xk1 - xk) over the velocity entries, exactly as the reference loop does.
+// ---------------------------------------------------------------------------------------------
+
+// nest[]: 0 beta, 1 thk, 2 err_prev, 3 restarts, 4 momentum_disabled
+__device__ inline void extrapolate6(const double u[6], const double uk[6], double beta, double out[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = beta == 0.0 ? u[k] : u[k] + beta * (u[k] - uk[k]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pjn_contacts(int nc, int nblocks, const int* __restrict__ ida,
+                                                         const int* __restrict__ idb, const double* __restrict__ J,
+                                                         const double* __restrict__ R, const double* __restrict__ bias,
+                                                         const double* __restrict__ mu, double* __restrict__ p,
+                                                         double* __restrict__ pk, const double* __restrict__ x,
+                                                         const double* __restrict__ xk, double* __restrict__ imp,
+                                                         const int* __restrict__ ctrl, const double* __restrict__ nest) {
+    if (ctrl[0]) return;
+    const double beta = nest[0];
+    const int c = xcd_tile(nblocks) * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6], t0[6], t1[6];
+    load_body(x, a, t0);
+    load_body(xk, a, t1);
+    extrapolate6(t0, t1, beta, ua);
+    load_body(x, b, t0);
+    load_body(xk, b, t1);
+    extrapolate6(t0, t1, beta, ub);
+    double py[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const double pc = p[(size_t)k * nc + c], pp = pk[(size_t)k * nc + c];
+        py[k] = beta == 0.0 ? pc : pc + beta * (pc - pp);
+        pk[(size_t)k * nc + c] = pc;
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        pn[k] = py[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+    }
+    pn[0] = fmax(pn[0], 0.0);
+    if (nr == 3) {
+        const double a1 = pn[1], a2 = pn[2];
+        const double tn2 = a1 * a1 + a2 * a2;
+        if (tn2 > 0) {
+            const double s = fmin(1.0, (muc * pn[0]) / sqrt(tn2));
+            pn[1] = s * a1;
+            pn[2] = s * a2;
+        }
+    }
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = pn[k] - py[k];
+        p[(size_t)k * nc + c] = pn[k];
+    }
+    write_side_impulses(imp, c, a, b, nr, j, dp);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pjn_bodies(int nb, const int* __restrict__ dof,
+                                                       const double* __restrict__ minv, const int* __restrict__ incptr,
+                                                       const int* __restrict__ incidx, const double* __restrict__ imp,
+                                                       double* __restrict__ x, double* __restrict__ xk, double tol_rel,
+                                                       double tol_abs, double* __restrict__ partial,
+                                                       const int* __restrict__ ctrl, const double* __restrict__ nest) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const double beta = nest[0];
+    const int bi = blockIdx.x * kBlock + threadIdx.x;
+    double e2 = 0, dt = 0;
+    if (bi < nb) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
+            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
+            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+            acc[0] += q0.x;
+            acc[1] += q0.y;
+            acc[2] += q1.x;
+            acc[3] += q1.y;
+            acc[4] += q2.x;
+            acc[5] += q2.y;
+        }
+        double u[6], uk[6], y[6], un[6];
+        load_body(x, bi, u);
+        load_body(xk, bi, uk);
+        extrapolate6(u, uk, beta, y);
+        const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
+        const int nd = dof[bi];
+        const double Ml = m4.x > 0 ? 1.0 / m4.x : 0.0;
+        const double Ma[3] = {m4.y > 0 ? 1.0 / m4.y : 0.0, m4.z > 0 ? 1.0 / m4.z : 0.0, m4.w > 0 ? 1.0 / m4.w : 0.0};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double M = k < 3 ? Ml : Ma[k - 3];
+            un[k] = (k < nd) ? y[k] + acc[k] / M : y[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= nd) break;
+            const double q = fmax(fabs(un[k]), fabs(y[k])) * tol_rel + tol_abs;
+            const double e = (un[k] - y[k]) / q;
+            e2 += e * e;
+            dt += (y[k] - un[k]) * (un[k] - u[k]);
+        }
+        store6(xk + (size_t)bi * kBodyStride, u);
+        store6(x + (size_t)bi * kBodyStride, un);
+    }
+    const double s = block_sum(e2, red);
+    const double d = block_sum(dt, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = d;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const double* __restrict__ partial, int nv, int it,
+                                                         double beta_threshold, int restart_limit, int* __restrict__ ctrl,
+                                                         double* __restrict__ res, int* __restrict__ hflag,
+                                                         double* __restrict__ nest) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    double s = 0, d = 0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        s += partial[i];
+        d += partial[nparts + i];
+    }
+    s = block_sum(s, red);
+    d = block_sum(d, red);
+    if (threadIdx.x != 0) return;
+    const double err = 1.0 / sqrt((double)nv) * sqrt(s);
+    res[0] = err;
+    if (err <= 1.0) {
+        ctrl[1] = it + 1;
+        ctrl[2] = 0;
+        ctrl[0] = 1;
+        hflag[0] = 1;
+        return;
+    }
+    double thk = nest[1];
+    const double err_prev = nest[2];
+    double restarts = nest[3];
+    int disabled = nest[4] != 0.0;
+    const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+    const double betak1 = (thk - 1.0) / thk1;
+    bool restart = false;
+    if (!isfinite(err)) restart = true;
+    else if (err_prev < INFINITY && err > 1.05 * err_prev) restart = true;
+    else if (betak1 > beta_threshold) restart = true;
+    else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = true;
+    else if (d > 0.0) restart = true;
+    double beta = 0.0;
+    if (restart) {
+        thk = 1.0;
+        restarts += 1.0;
+        if (restarts >= (double)restart_limit) disabled = 1;
+    } else if (disabled) {
+        thk = 1.0;
+    } else {
+        beta = betak1;
+        thk = thk1;
+    }
+    nest[0] = beta;
+    nest[1] = thk;
+    nest[2] = err;
+    nest[3] = restarts;
+    nest[4] = disabled ? 1.0 : 0.0;
+}
+
 }  // namespace cdx
 
 using namespace cdx;
+
+static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
 
 extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info) {
     (void)theta;  // theta only enters bilateral rows and S of spring/damper scenes
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
-    if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: acceleration mode not supported on the device");
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: unknown acceleration mode");
     if (info) {
         info->iters = 0;
         info->residual = std::nan("");
@@ -327,6 +368,47 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     }
     const int gb = grid_for(nb), gc = grid_for(nc);
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb, &ctx->partial_cap, "residual partials"));
+    int iters = 0;
+    int st = CDX_OK;
+    if (prm->accel == CDX_ACCEL_NESTEROV) {
+        // xk = x0, pk = p0, beta = 0, thk = 1, err_prev = inf, restarts = 0, momentum on
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)nc, &ctx->pk_cap, "nesterov pk"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_nest, 8, nullptr, "nesterov scalars"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)gb, &ctx->partial2_cap, "nesterov partials"));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_uprev, ctx->d_x, sizeof(double) * (size_t)nb * kBodyStride, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
+        const double n0[8] = {0.0, 1.0, INFINITY, 0.0, 0.0, 0.0, 0.0, 0.0};
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_nest, n0, sizeof(n0), hipMemcpyHostToDevice, ctx->stream));
+        auto nsweep = [&](int it) -> int {
+            CDX_LAUNCH(ctx, "pj_contacts", k_pjn_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                       ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_x, ctx->d_uprev, ctx->d_imp, ctx->d_ctrl,
+                       ctx->d_nest);
+            CDX_LAUNCH(ctx, "pj_bodies", k_pjn_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                       ctx->d_imp, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
+                       ctx->d_nest);
+            CDX_LAUNCH(ctx, "pj_residual", k_pjn_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it,
+                       prm->nesterov_beta_threshold, prm->nesterov_restart_limit, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev,
+                       ctx->d_nest);
+            return hip_check(ctx, hipGetLastError(), "pj nesterov sweep launch");
+        };
+        st = run_sweeps(ctx, prm->max_iter, 10, 4, nsweep, &iters);
+    } else {
+        st = run_plain_sweeps(ctx, prm, gb, gc, &iters);
+    }
+    if (st) return set_err(ctx, st, "cdx_pj_solve failed");
+    double r = 0;
+    CDX_TRY(read_residual(ctx, &r));
+    if (info) {
+        info->iters = iters;
+        info->residual = r;
+    }
+    ctx->solved = true;
+    return CDX_OK;
+}
+
+static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out) {
+    const int nb = ctx->nb, nc = ctx->nc;
     const int check_every = prm->check_every;
     auto sweep = [&](int it) -> int {
         const int check = (it % check_every) == 0;
@@ -337,15 +419,5 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         if (check) CDX_TRY(launch_residual_finalize(ctx, gb, it, 0));
         return hip_check(ctx, hipGetLastError(), "pj sweep launch");
     };
-    int iters = 0;
-    const int st = run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, &iters);
-    if (st) return set_err(ctx, st, "cdx_pj_solve failed");
-    double r = 0;
-    CDX_TRY(read_residual(ctx, &r));
-    if (info) {
-        info->iters = iters;
-        info->residual = r;
-    }
-    ctx->solved = true;
-    return CDX_OK;
+    return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);
 }

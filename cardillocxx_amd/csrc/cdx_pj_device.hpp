@@ -1,0 +1,188 @@
+// cdx_pj_device.hpp -- per-contact and per-body work of one projected-Jacobi solve, shared by the
+// grid-wide kernels (cdx_pj.hip) and the batched persistent kernel (cdx_batch.hip) so both run
+// the same arithmetic.
+#pragma once
+
+#include "cdx_device.hpp"
+
+namespace cdx {
+
+// Per-contact setup of one PJ solve (see k_pj_setup_contacts).
+__device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                        const double* __restrict__ J, const double* __restrict__ mu,
+                                        const double* __restrict__ rest, const double* __restrict__ vb,
+                                        const double* __restrict__ minv, const double* __restrict__ v, double alpha,
+                                        int warmstart, double* __restrict__ R, double* __restrict__ bias,
+                                        double* __restrict__ p, double* __restrict__ imp) {
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) {
+        for (int k = 0; k < 3; ++k) {
+            R[(size_t)k * nc + c] = 0;
+            bias[(size_t)k * nc + c] = 0;
+            p[(size_t)k * nc + c] = 0;
+        }
+        return;
+    }
+    const int nr = mu[c] > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6];
+    load_body(v, a, ua);
+    load_body(v, b, ub);
+    double mA[4] = {0, 0, 0, 0}, mB[4] = {0, 0, 0, 0};
+    if (a >= 0)
+        for (int m = 0; m < 4; ++m) mA[m] = minv[(size_t)a * kMinvStride + m];
+    if (b >= 0)
+        for (int m = 0; m < 4; ++m) mB[m] = minv[(size_t)b * kMinvStride + m];
+    double pw[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) {
+            R[(size_t)k * nc + c] = 0;
+            bias[(size_t)k * nc + c] = 0;
+            p[(size_t)k * nc + c] = 0;
+            continue;
+        }
+        // D summed side by side in column order (lower body index first), as the row-major
+        // compressed W stores the row (rdiag_sparse iterates the row's entries in column order).
+        double DA = 0, DB = 0;
+        for (int m = 0; m < 3; ++m) DA += j.d[k][m] * j.d[k][m] * mA[0];
+        for (int m = 0; m < 3; ++m) DA += j.aA[k][m] * j.aA[k][m] * mA[1 + m];
+        for (int m = 0; m < 3; ++m) DB += j.d[k][m] * j.d[k][m] * mB[0];
+        for (int m = 0; m < 3; ++m) DB += j.aB[k][m] * j.aB[k][m] * mB[1 + m];
+        const double D = (a >= 0 && b >= 0 && b < a) ? DB + DA : DA + DB;
+        const double Rk = D > 0 ? alpha / D : 0.0;
+        const double ck = vb[(size_t)k * nc + c];
+        const double vprev = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub) + ck;
+        R[(size_t)k * nc + c] = Rk;
+        bias[(size_t)k * nc + c] = Rk * (ck + rest[(size_t)k * nc + c] * vprev);
+        pw[k] = warmstart ? p[(size_t)k * nc + c] : 0.0;
+        p[(size_t)k * nc + c] = pw[k];
+    }
+    write_side_impulses(imp, c, a, b, nr, j, pw);
+}
+
+// Per-body x0 (see k_pj_setup_bodies).
+__device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const double* __restrict__ minv,
+                                     const double* __restrict__ v, const double* __restrict__ fext,
+                                     const double* __restrict__ fgyro, const int* __restrict__ incptr,
+                                     const int* __restrict__ incidx, const double* __restrict__ imp, double dt,
+                                     int have_contacts, double* __restrict__ x) {
+    const int nd = dof[bi];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (have_contacts) {
+        for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
+            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
+            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+            acc[0] += q0.x;
+            acc[1] += q0.y;
+            acc[2] += q1.x;
+            acc[3] += q1.y;
+            acc[4] += q2.x;
+            acc[5] += q2.y;
+        }
+    }
+    const double* vv = v + (size_t)bi * kBodyStride;
+    const double* fe = fext + (size_t)bi * kBodyStride;
+    const double* fg = fgyro + (size_t)bi * kBodyStride;
+    const double* mi = minv + (size_t)bi * kMinvStride;
+    double out[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < nd; ++k) {
+        const double mk = k < 3 ? mi[0] : mi[k - 2];
+        const double M = mk > 0 ? 1.0 / mk : 0.0;
+        double r = M * vv[k] + dt * fe[k];
+        r += dt * fg[k];
+        r += acc[k];
+        out[k] = r / M;
+    }
+    store6(x + (size_t)bi * kBodyStride, out);
+}
+
+// One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).
+__device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                        const double* __restrict__ J, const double* __restrict__ R,
+                                        const double* __restrict__ bias, const double* __restrict__ mu,
+                                        double* __restrict__ p, const double* __restrict__ x,
+                                        double* __restrict__ imp) {
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6];
+    load_body(x, a, ua);
+    load_body(x, b, ub);
+    double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        pold[k] = p[(size_t)k * nc + c];
+        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+    }
+    // project() (projected_jacobi.cpp:100-119)
+    pn[0] = fmax(pn[0], 0.0);
+    if (nr == 3) {
+        const double t1 = pn[1], t2 = pn[2];
+        const double tn2 = t1 * t1 + t2 * t2;
+        if (tn2 > 0) {
+            const double s = fmin(1.0, (muc * pn[0]) / sqrt(tn2));
+            pn[1] = s * t1;
+            pn[2] = s * t2;
+        }
+    }
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = pn[k] - pold[k];
+        p[(size_t)k * nc + c] = pn[k];
+    }
+    write_side_impulses(imp, c, a, b, nr, j, dp);
+}
+
+// One body of a PJ sweep: x += (W^T dp) / M over its incidences in a fixed order; returns the
+// body's part of the residual sum (projected_jacobi.cpp:140-144) when `check`.
+__device__ inline double pj_body_sweep(int bi, const int* __restrict__ dof, const double* __restrict__ minv,
+                                       const int* __restrict__ incptr, const int* __restrict__ incidx,
+                                       const double* __restrict__ imp, double* __restrict__ x, int check,
+                                       double tol_rel, double tol_abs) {
+    double e2 = 0;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int i0 = incptr[bi], i1 = incptr[bi + 1];
+    for (int i = i0; i < i1; ++i) {
+        const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
+        const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+        acc[0] += q0.x;
+        acc[1] += q0.y;
+        acc[2] += q1.x;
+        acc[3] += q1.y;
+        acc[4] += q2.x;
+        acc[5] += q2.y;
+    }
+    double u[6];
+    load_body(x, bi, u);
+    const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
+    const int nd = dof[bi];
+    double un[6];
+    const double Ml = m4.x > 0 ? 1.0 / m4.x : 0.0;
+    const double Ma[3] = {m4.y > 0 ? 1.0 / m4.y : 0.0, m4.z > 0 ? 1.0 / m4.z : 0.0, m4.w > 0 ? 1.0 / m4.w : 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double M = k < 3 ? Ml : Ma[k - 3];
+        un[k] = (k < nd) ? u[k] + acc[k] / M : u[k];
+    }
+    if (i1 > i0) store6(x + (size_t)bi * kBodyStride, un);
+    if (check) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= nd) break;
+            const double q = fmax(fabs(un[k]), fabs(u[k])) * tol_rel + tol_abs;
+            const double e = (un[k] - u[k]) / q;
+            e2 += e * e;
+        }
+    }
+    return e2;
+}
+
+}  // namespace cdx

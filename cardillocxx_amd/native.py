@@ -51,6 +51,10 @@ EXPORTED = [
     "cdx_moreau_update",
     "cdx_set_profiling",
     "cdx_kernel_time",
+    "cdx_set_scenes",
+    "cdx_batch_pj_solve",
+    "cdx_batch_set_launch",
+    "cdx_batch_autotune",
 ]
 
 
@@ -99,6 +103,10 @@ def _declare(lib):
         "cdx_moreau_update": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
         "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
+        "cdx_set_scenes": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+        "cdx_batch_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _dp]),
+        "cdx_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "cdx_batch_autotune": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _ip, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -229,6 +237,28 @@ class Context:
         if st != CDX_OK and (raise_on_divergence or st != CDX_DIVERGED):
             self._check(st, "cdx_gs_colored_solve")
         return info
+
+    def set_scenes(self, body_off, contact_off):
+        bo = np.ascontiguousarray(body_off, dtype=np.int32)
+        co = np.ascontiguousarray(contact_off, dtype=np.int32)
+        self._check(self._l.cdx_set_scenes(self.h, len(bo) - 1, iptr(bo), iptr(co)), "cdx_set_scenes")
+        self.nscenes = len(bo) - 1
+
+    def batch_pj_solve(self, dt, theta, params):
+        it = np.zeros(self.nscenes, np.int32)
+        res = np.zeros(self.nscenes)
+        self._check(self._l.cdx_batch_pj_solve(self.h, dt, theta, C.byref(params), iptr(it), dptr(res)),
+                    "cdx_batch_pj_solve")
+        return it, res
+
+    def batch_set_launch(self, block, groups):
+        self._check(self._l.cdx_batch_set_launch(self.h, block, groups), "cdx_batch_set_launch")
+
+    def batch_autotune(self, dt, theta, params):
+        b, g, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._check(self._l.cdx_batch_autotune(self.h, dt, theta, C.byref(params), C.byref(b), C.byref(g), C.byref(ms)),
+                    "cdx_batch_autotune")
+        return b.value, g.value, ms.value
 
     def velocities(self):
         out = np.empty(self.nv, dtype=np.float64)

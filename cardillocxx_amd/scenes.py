@@ -284,3 +284,34 @@ def random_problem(nb=20, nc=60, seed=0, p_static=0.15, p_frictionless=0.2, p_po
         w[:, 1:] = rng.normal(scale=0.02, size=(nc, 2))
     return ContactProblem(dof=dof, minv=minv, v=v, fext=fext, fgyro=fgyro, q=q, body_a=ia, body_b=ib, mu=mu,
                           frame=frame, angA=aA, angB=aB, rest=restitution, vbias=vbias, warm=w)
+
+
+def concat_problems(problems):
+    """Concatenate independent contact problems into one batch: body ids of scene s are shifted
+    by the bodies before it; returns (problem, body_off, contact_off)."""
+    boff, coff = [0], [0]
+    parts = {k: [] for k in ("dof", "minv", "v", "fext", "fgyro", "q", "body_a", "body_b", "mu", "frame", "angA",
+                             "angB", "rest", "vbias", "warm")}
+    for p in problems:
+        shift = boff[-1]
+        for k in ("dof", "minv", "v", "fext", "fgyro", "q", "mu", "frame", "angA", "angB", "rest", "vbias", "warm"):
+            parts[k].append(getattr(p, k))
+        parts["body_a"].append(np.where(p.body_a >= 0, p.body_a + shift, -1).astype(np.int32))
+        parts["body_b"].append(np.where(p.body_b >= 0, p.body_b + shift, -1).astype(np.int32))
+        boff.append(boff[-1] + p.nb)
+        coff.append(coff[-1] + p.nc)
+    cat = {k: np.concatenate(v) if v else np.zeros(0) for k, v in parts.items()}
+    cat["dof"] = cat["dof"].astype(np.int32)
+    for k in ("frame", "angA", "angB"):
+        cat[k] = cat[k].reshape(-1, 9)
+    for k in ("rest", "vbias", "warm"):
+        cat[k] = cat[k].reshape(-1, 3)
+    return ContactProblem(**cat), np.array(boff, np.int32), np.array(coff, np.int32)
+
+
+def split_velocities(problems, v):
+    out, o = [], 0
+    for p in problems:
+        out.append(v[o:o + p.nv])
+        o += p.nv
+    return out

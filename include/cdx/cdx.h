@@ -131,6 +131,28 @@ int cdx_get_positions(cdx_ctx* ctx, double* q_out);
 int cdx_moreau_predrift(cdx_ctx* ctx, double dt, double theta);
 int cdx_moreau_update(cdx_ctx* ctx, double dt, double theta);
 
+/* ---- Batched scenes (no counterpart in the reference, which runs one PhysicsEngine per scene,
+ * physics_engine.hpp:215-221).  The context's bodies and contacts are partitioned into nscenes
+ * independent scenes: scene s owns bodies [body_off[s], body_off[s+1]) and contacts
+ * [contact_off[s], contact_off[s+1]); contacts may only reference bodies of their own scene.
+ * Must be called after cdx_set_bodies / cdx_set_contacts (which clear the partition). */
+int cdx_set_scenes(cdx_ctx* ctx, int nscenes, const int* body_off, const int* contact_off);
+
+/* Solves every scene with the projected-Jacobi standard loop in one persistent launch; each scene
+ * has its own residual test and iteration count (iters[s], residuals[s]; either may be NULL).
+ * Results are read with cdx_get_velocities / cdx_get_impulses like a single solve. */
+int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* iters,
+                       double* residuals);
+
+/* Launch shape of the batched solve: `block` threads per workgroup (64..512) hosting `groups`
+ * scenes (block/groups must be a multiple of 64). */
+int cdx_batch_set_launch(cdx_ctx* ctx, int block, int groups);
+
+/* Autotuner: times every launch shape on the current batch (warm start restored before each
+ * trial), keeps the fastest for later cdx_batch_pj_solve calls and reports it. */
+int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* block,
+                       int* groups, double* best_ms);
+
 /* Kernel timing with HIP events on the context's stream (the analogue of the reference's
  * TimingManager scopes, src/misc/timings/TimingManager.hpp:16-58).  When enabled every
  * launch is bracketed by events; cdx_kernel_time reads back the call count and total ms for

@@ -1,0 +1,100 @@
+"""Batched solve: every scene of a one-launch batch equals a standalone solve of that scene."""
+import numpy as np
+import pytest
+
+import oracle
+from cardillocxx_amd import native, scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel(a, b):
+    den = max(np.abs(b).max(initial=0.0), 1e-300)
+    return np.abs(np.asarray(a) - np.asarray(b)).max(initial=0.0) / den
+
+
+def make_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    probs = []
+    for s in range(n):
+        nb = int(np.exp(rng.uniform(np.log(4), np.log(120))))
+        nc = int(rng.integers(0, 4 * nb))
+        probs.append(scenes.random_problem(nb=nb, nc=nc, seed=1000 * seed + s, warm=bool(s % 2)))
+    return probs
+
+
+def upload_batch(ctx, probs):
+    cat, boff, coff = scenes.concat_problems(probs)
+    ctx.set_bodies(cat.dof, cat.minv, cat.v, cat.fext, cat.fgyro, cat.q)
+    ctx.set_contacts(*cat.contact_arrays())
+    ctx.set_scenes(boff, coff)
+    return cat, boff, coff
+
+
+@pytest.mark.parametrize("launch", [(256, 1), (64, 1), (512, 4), (128, 2)])
+def test_batch_matches_standalone_oracle(ctx, launch):
+    probs = make_batch(40, seed=1)
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = 1e-8
+    ctx.batch_set_launch(*launch)
+    iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    v = ctx.velocities()
+    imp = ctx.impulses()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=1e-8, tol_rel=1e-8))
+        assert iters[s] == r.iters, s
+        vs = v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()]
+        assert rel(vs, v_ref) <= TOL
+        if prob.nc:
+            assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL
+
+
+def test_batch_fixed_sweeps_equals_grid_solve(ctx):
+    """Same arithmetic as the grid-wide solve: a one-scene batch is bitwise identical to it."""
+    prob = scenes.random_problem(nb=300, nc=1200, seed=5)
+    p = native.default_params()
+    p.max_iter, p.tol_abs, p.tol_rel = 77, 0.0, 0.0
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.pj_solve(1e-3, 1.0, p)
+    v1, i1 = ctx.velocities(), ctx.impulses()
+    upload_batch(ctx, [prob])
+    iters, _ = ctx.batch_pj_solve(1e-3, 1.0, p)
+    assert iters[0] == 77
+    assert np.array_equal(ctx.velocities(), v1) and np.array_equal(ctx.impulses(), i1)
+
+
+def test_batch_autotune_and_sphere_scenes(ctx):
+    probs = []
+    for s in range(16):
+        pile = scenes.sphere_pile(n=60 + 20 * s, settle_steps=60, box=(0.5, 0.5, 1.0), seed=s)
+        probs.append(pile.problem())
+    upload_batch(ctx, probs)
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = 1e-6
+    block, groups, ms = ctx.batch_autotune(1e-3, 1.0, p)
+    assert block in (64, 128, 256, 512) and ms > 0
+    iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    v = ctx.velocities()
+    o = 0
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=1e-6, tol_rel=1e-6))
+        assert iters[s] == r.iters
+        assert rel(v[o:o + prob.nv], v_ref) <= TOL
+        o += prob.nv
+
+
+def test_scene_validation(ctx):
+    probs = make_batch(3, seed=2)
+    cat, boff, coff = scenes.concat_problems(probs)
+    ctx.set_bodies(cat.dof, cat.minv, cat.v, cat.fext, cat.fgyro, cat.q)
+    ctx.set_contacts(*cat.contact_arrays())
+    bad = boff.copy()
+    bad[1] = bad[1] - 1 if bad[1] > 0 else 1
+    with pytest.raises(native.CdxError):
+        ctx.set_scenes(bad, coff)
+    with pytest.raises(native.CdxError):
+        ctx.batch_set_launch(256, 3)

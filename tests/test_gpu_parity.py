@@ -122,7 +122,9 @@ def test_error_statuses(ctx):
 def test_colored_gs_matches_oracle(ctx, case):
     prob = scenes.random_problem(**case)
     dt = 1e-3
-    for sweeps, tol in ((7, 0.0), (200000, 1e-7)):
+    # fixed sweep count: a tolerance that cannot be met (zero would make 0/0 = NaN, which GS
+    # treats as divergence, projected_gauss_seidel.cpp:133-138)
+    for sweeps, tol in ((7, 1e-30), (200000, 1e-7)):
         st, v_ref, imp_ref, res = oracle.colored_gs_solve(prob, dt, oracle.params(max_iter=sweeps, tol_abs=tol,
                                                                                   tol_rel=tol))
         assert st == 0
@@ -187,3 +189,19 @@ def test_moreau_update_matches_oracle(ctx):
     ctx.moreau_predrift(1e-3, 0.5)
     np.testing.assert_allclose(ctx.positions(), oracle.position_update(prob.dof, prob.q, prob.v, 0.5e-3), rtol=0,
                                atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_pj_nesterov_matches_oracle(ctx, seed):
+    """Nesterov loop (projected_jacobi.cpp:379-449): residual every sweep, restarts on residual
+    growth / beta threshold / direction reversal, momentum disabled after the restart limit."""
+    prob = scenes.random_problem(nb=120, nc=400, seed=30 + seed)
+    dt = 1e-3
+    for kw in (dict(max_iter=40, tol_abs=0.0, tol_rel=0.0), dict(tol_abs=1e-8, tol_rel=1e-8),
+               dict(tol_abs=1e-8, tol_rel=1e-8, nesterov_restart_limit=3), dict(tol_abs=1e-8, tol_rel=1e-8,
+                                                                                 nesterov_beta_threshold=0.5)):
+        okw = {k: v for k, v in kw.items()}
+        st, v_ref, imp_ref, res = oracle.pj_solve(prob, dt, oracle.params(nesterov=1, **okw))
+        v, imp, info = gpu_pj(ctx, prob, dt, accel=native.CDX_ACCEL_NESTEROV, **kw)
+        assert info.iters == res.iters
+        assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
