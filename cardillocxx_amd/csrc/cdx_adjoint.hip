@@ -1,0 +1,630 @@
+// cdx_adjoint.hip -- vector-Jacobian product of the FP64 projected-Jacobi solve.
+//
+// Forward (cdx_pj.hip, projected_jacobi.cpp:121-138,316-377): u_free = v + h M^-1 (f_ext + f_gyro),
+// p_0 = warm start, x_0 = u_free + M^-1 W^T p_0, and per sweep
+//     z_k = p_k - R (W x_k + c''),  p_{k+1} = Pi_mu(z_k),  x_{k+1} = x_k + M^-1 W^T (p_{k+1} - p_k)
+// with R = alpha / diag(W M^-1 W^T) and c'' = c + e .* (W v + c) (so b = R c'').  The geometry
+// (W, c, e) is held fixed.  Two backward modes share one final stage:
+//
+//  implicit  At the fixed point p* = Pi(p* - R(W x* + c'')), x* = u_free + M^-1 W^T p*,
+//            the adjoint lambda solves (I - P (I - R A))^T lambda = g_p with g_p = W M^-1 L_x + L_p,
+//            A = W M^-1 W^T and P = dPi/dz per contact.  It is iterated as
+//            lambda <- g_p + xi - W M^-1 W^T (R xi),  xi = P^T lambda,
+//            which is the transposed linearised sweep: one contact pass and one body pass per
+//            iteration, the same memory pattern as the forward.  Then with s = R xi:
+//              x_bar = L_x - W^T s,  c''_bar = -s,  R_bar = -xi .* (W x* + c'').
+//  unrolled  Reverse mode through the K recorded sweeps (x_k, p_k).
+//
+// Final stage: u_free_bar = x_bar_0, M^-1_bar from x = ... + M^-1 W^T p and R(M^-1), v_bar from
+// u_free (with the gyroscopic Jacobian) and the restitution term of c'', f_ext_bar = h M^-1 u_bar.
+#include <cmath>
+
+#include "cdx_pj_device.hpp"
+
+namespace cdx {
+
+// contact scratch layout, component-major [k * nc + c]
+enum {
+    A_P = 0,       // 9: dPi/dz, row-major (row = output n,t1,t2; column = input)
+    A_Y = 9,       // 3: W x* + c''
+    A_DMU = 12,    // 3: dp/dmu
+    A_GP = 15,     // 3: g_p
+    A_LAM = 18,    // 3: lambda
+    A_XI = 21,     // 3: xi = P^T lambda
+    A_RBAR = 24,   // 3: accumulated R_bar (unrolled)
+    A_CBAR = 27,   // 3: accumulated c''_bar (unrolled)
+    A_MUBAR = 30,  // 1: accumulated mu_bar (unrolled)
+    A_COUNT = 31
+};
+
+// Generalised Jacobian of project() (projected_jacobi.cpp:100-119) at z, following the same
+// branches: normal clamp, then the disk scaling s = min(1, mu p_n / |t|) when |t| > 0.
+__device__ inline void projection_jacobian(const double z[3], int nr, double mu, double P[9], double dmu[3]) {
+    for (int k = 0; k < 9; ++k) P[k] = 0.0;
+    dmu[0] = dmu[1] = dmu[2] = 0.0;
+    const double pn = fmax(z[0], 0.0);
+    const double dn = z[0] > 0.0 ? 1.0 : 0.0;
+    P[0] = dn;
+    if (nr == 1) return;
+    const double t1 = z[1], t2 = z[2];
+    const double tn2 = t1 * t1 + t2 * t2;
+    if (!(tn2 > 0)) {  // no scaling applied: p_t = t
+        P[4] = 1.0;
+        P[8] = 1.0;
+        return;
+    }
+    const double tn = sqrt(tn2);
+    const double sr = (mu * pn) / tn;
+    if (sr >= 1.0) {  // inside the disk (stick)
+        P[4] = 1.0;
+        P[8] = 1.0;
+        return;
+    }
+    // slip: p_t = mu p_n t / |t|
+    const double u1 = t1 / tn, u2 = t2 / tn;
+    P[4] = sr * (1.0 - u1 * u1);
+    P[5] = sr * (-u1 * u2);
+    P[7] = sr * (-u2 * u1);
+    P[8] = sr * (1.0 - u2 * u2);
+    P[3] = mu * u1 * dn;
+    P[6] = mu * u2 * dn;
+    dmu[1] = pn * u1;
+    dmu[2] = pn * u2;
+}
+
+__device__ inline void mat_t_vec(const double P[9], const double v[3], double o[3]) {
+    for (int c = 0; c < 3; ++c) o[c] = P[c] * v[0] + P[3 + c] * v[1] + P[6 + c] * v[2];
+}
+
+__device__ inline void minv_body(const double* __restrict__ minv, int b, double m[6]) {
+    if (b < 0) {
+        for (int k = 0; k < 6; ++k) m[k] = 0;
+        return;
+    }
+    const double* mi = minv + (size_t)b * kMinvStride;
+    m[0] = m[1] = m[2] = mi[0];
+    m[3] = mi[1];
+    m[4] = mi[2];
+    m[5] = mi[3];
+}
+
+// unscaled bias c'' = c + e .* (W v + c) of row k
+__device__ inline double cbias(const ContactJ& j, int k, int nc, int c, const double* __restrict__ rest,
+                               const double* __restrict__ vb, const double va[6], const double vbb[6]) {
+    const double ck = vb[(size_t)k * nc + c];
+    return ck + rest[(size_t)k * nc + c] * (row_dot(j.d[k], j.aA[k], j.aB[k], va, vbb) + ck);
+}
+
+// Implicit setup: P, y, dmu at the fixed point, g_p, lambda_0 = g_p, xi_0, and the side impulses
+// of W^T (R xi_0).
+__global__ void k_adj_setup(int nc, const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
+                            const double* __restrict__ R, const double* __restrict__ bias, const double* __restrict__ mu,
+                            const double* __restrict__ rest, const double* __restrict__ vb, const double* __restrict__ minv,
+                            const double* __restrict__ v, const double* __restrict__ x, const double* __restrict__ p,
+                            const double* __restrict__ lx, const double* __restrict__ lp, double* __restrict__ S,
+                            double* __restrict__ imp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) {
+        for (int k = 0; k < A_COUNT; ++k) S[(size_t)k * nc + c] = 0;
+        return;
+    }
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double xa[6], xb[6], va[6], vbb[6], la[6], lb[6], ma[6], mb[6];
+    load_body(x, a, xa);
+    load_body(x, b, xb);
+    load_body(v, a, va);
+    load_body(v, b, vbb);
+    load_body(lx, a, la);
+    load_body(lx, b, lb);
+    minv_body(minv, a, ma);
+    minv_body(minv, b, mb);
+    for (int k = 0; k < 6; ++k) {
+        la[k] *= ma[k];
+        lb[k] *= mb[k];
+    }
+    double z[3] = {0, 0, 0}, y[3] = {0, 0, 0}, gp[3] = {0, 0, 0};
+    for (int k = 0; k < nr; ++k) {
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], xa, xb);
+        const double pk = p[(size_t)k * nc + c];
+        z[k] = pk - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+        y[k] = g + cbias(j, k, nc, c, rest, vb, va, vbb);
+        gp[k] = row_dot(j.d[k], j.aA[k], j.aB[k], la, lb) + (lp ? lp[3 * (size_t)c + k] : 0.0);
+    }
+    double P[9], dmu[3], xi[3];
+    projection_jacobian(z, nr, muc, P, dmu);
+    mat_t_vec(P, gp, xi);
+    for (int k = 0; k < 9; ++k) S[(size_t)(A_P + k) * nc + c] = P[k];
+    double s[3];
+    for (int k = 0; k < 3; ++k) {
+        S[(size_t)(A_Y + k) * nc + c] = y[k];
+        S[(size_t)(A_DMU + k) * nc + c] = dmu[k];
+        S[(size_t)(A_GP + k) * nc + c] = gp[k];
+        S[(size_t)(A_LAM + k) * nc + c] = gp[k];
+        S[(size_t)(A_XI + k) * nc + c] = xi[k];
+        S[(size_t)(A_RBAR + k) * nc + c] = 0.0;
+        S[(size_t)(A_CBAR + k) * nc + c] = 0.0;
+        s[k] = k < nr ? R[(size_t)k * nc + c] * xi[k] : 0.0;
+    }
+    S[(size_t)A_MUBAR * nc + c] = 0.0;
+    write_side_impulses(imp, c, a, b, nr, j, s);
+}
+
+// t = M^-1 (sum of side impulses)
+__global__ void k_adj_body_t(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                             const int* __restrict__ incptr, const int* __restrict__ incidx, const double* __restrict__ imp,
+                             double* __restrict__ t, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= nb) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
+        const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
+        const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+        acc[0] += q0.x;
+        acc[1] += q0.y;
+        acc[2] += q1.x;
+        acc[3] += q1.y;
+        acc[4] += q2.x;
+        acc[5] += q2.y;
+    }
+    double m[6];
+    minv_body(minv, bi, m);
+    double o[6];
+    for (int k = 0; k < 6; ++k) o[k] = k < dof[bi] ? m[k] * acc[k] : 0.0;
+    store6(t + (size_t)bi * kBodyStride, o);
+}
+
+// lambda <- g_p + xi - W t; xi <- P^T lambda; side impulses of W^T (R xi); convergence partials.
+__global__ void __launch_bounds__(kBlock) k_adj_iter(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                                     const double* __restrict__ J, const double* __restrict__ R,
+                                                     const double* __restrict__ mu, const double* __restrict__ t,
+                                                     double* __restrict__ S, double* __restrict__ imp, int check,
+                                                     double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double d2 = 0, n2 = 0;
+    if (c < nc) {
+        const int a = ida[c], b = idb[c];
+        if (a >= 0 || b >= 0) {
+            const int nr = mu[c] > 0 ? 3 : 1;
+            ContactJ j;
+            load_J(J, nc, c, nr, j);
+            double ta[6], tb[6];
+            load_body(t, a, ta);
+            load_body(t, b, tb);
+            double lam[3] = {0, 0, 0}, P[9], xi[3], s[3] = {0, 0, 0};
+            for (int k = 0; k < nr; ++k) {
+                const double old = S[(size_t)(A_LAM + k) * nc + c];
+                lam[k] = S[(size_t)(A_GP + k) * nc + c] + S[(size_t)(A_XI + k) * nc + c] -
+                         row_dot(j.d[k], j.aA[k], j.aB[k], ta, tb);
+                d2 += (lam[k] - old) * (lam[k] - old);
+                n2 += lam[k] * lam[k];
+            }
+            for (int k = 0; k < 9; ++k) P[k] = S[(size_t)(A_P + k) * nc + c];
+            mat_t_vec(P, lam, xi);
+            for (int k = 0; k < nr; ++k) {
+                S[(size_t)(A_LAM + k) * nc + c] = lam[k];
+                S[(size_t)(A_XI + k) * nc + c] = xi[k];
+                s[k] = R[(size_t)k * nc + c] * xi[k];
+            }
+            write_side_impulses(imp, c, a, b, nr, j, s);
+        }
+    }
+    if (check) {
+        const double sd = block_sum(d2, red);
+        const double sn = block_sum(n2, red);
+        if (threadIdx.x == 0) {
+            partial[2 * blockIdx.x] = sd;
+            partial[2 * blockIdx.x + 1] = sn;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_finalize(int nparts, const double* __restrict__ partial, int it, double tol,
+                                                         int* __restrict__ ctrl, double* __restrict__ res,
+                                                         int* __restrict__ hflag) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    double d = 0, n = 0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        d += partial[2 * i];
+        n += partial[2 * i + 1];
+    }
+    d = block_sum(d, red);
+    n = block_sum(n, red);
+    if (threadIdx.x == 0) {
+        const double r = n > 0 ? sqrt(d / n) : sqrt(d);
+        res[0] = r;
+        if (r <= tol) {
+            ctrl[1] = it + 1;
+            ctrl[0] = 1;
+            hflag[0] = 1;
+        }
+    }
+}
+
+// ---- unrolled reverse sweep k: contact pass
+// Inputs: recorded x_k (xk), p_k (pk), p_{k+1} (pk1), x_bar_{k+1} (xbar, body), p_bar_{k+1} (pbar,
+// contact, 3 per contact in S[A_LAM]).  Outputs: p_bar_k in S[A_LAM], side impulses of R z_bar (imp)
+// and of dp_k = p_{k+1} - p_k (imp2), accumulated R_bar, c''_bar, mu_bar.
+__global__ void __launch_bounds__(kBlock) k_unr_contacts(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                                         const double* __restrict__ J, const double* __restrict__ R,
+                                                         const double* __restrict__ bias, const double* __restrict__ mu,
+                                                         const double* __restrict__ rest, const double* __restrict__ vb,
+                                                         const double* __restrict__ minv, const double* __restrict__ v,
+                                                         const double* __restrict__ xk, const double* __restrict__ pk,
+                                                         const double* __restrict__ pk1, const double* __restrict__ xbar,
+                                                         double* __restrict__ S, double* __restrict__ imp,
+                                                         double* __restrict__ imp2) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6], ga[6], gb[6], ma[6], mb[6], va[6], vbb[6];
+    load_body(xk, a, ua);
+    load_body(xk, b, ub);
+    load_body(xbar, a, ga);
+    load_body(xbar, b, gb);
+    load_body(v, a, va);
+    load_body(v, b, vbb);
+    minv_body(minv, a, ma);
+    minv_body(minv, b, mb);
+    for (int k = 0; k < 6; ++k) {
+        ga[k] *= ma[k];
+        gb[k] *= mb[k];
+    }
+    double z[3] = {0, 0, 0}, y[3] = {0, 0, 0}, ptot[3] = {0, 0, 0}, w[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+    for (int k = 0; k < nr; ++k) {
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        const double p0 = pk[(size_t)k * nc + c];
+        z[k] = p0 - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+        y[k] = g + cbias(j, k, nc, c, rest, vb, va, vbb);
+        w[k] = row_dot(j.d[k], j.aA[k], j.aB[k], ga, gb);  // W M^-1 x_bar_{k+1}
+        ptot[k] = S[(size_t)(A_LAM + k) * nc + c] + w[k];
+        dp[k] = pk1[(size_t)k * nc + c] - p0;
+    }
+    double P[9], dmu[3], zb[3], s[3] = {0, 0, 0};
+    projection_jacobian(z, nr, muc, P, dmu);
+    mat_t_vec(P, ptot, zb);
+    double mub = 0;
+    for (int k = 0; k < nr; ++k) mub += dmu[k] * ptot[k];
+    S[(size_t)A_MUBAR * nc + c] += mub;
+    for (int k = 0; k < nr; ++k) {
+        const double Rk = R[(size_t)k * nc + c];
+        S[(size_t)(A_LAM + k) * nc + c] = zb[k] - w[k];  // p_bar_k
+        S[(size_t)(A_RBAR + k) * nc + c] += -zb[k] * y[k];
+        S[(size_t)(A_CBAR + k) * nc + c] += -Rk * zb[k];
+        s[k] = Rk * zb[k];
+    }
+    write_side_impulses(imp, c, a, b, nr, j, s);
+    write_side_impulses(imp2, c, a, b, nr, j, dp);
+}
+
+// unrolled reverse sweep k: body pass.  M^-1_bar += x_bar_{k+1} .* W^T dp_k ; x_bar_k = x_bar_{k+1} - W^T (R z_bar)
+__global__ void k_unr_bodies(int nb, const int* __restrict__ dof, const int* __restrict__ incptr,
+                             const int* __restrict__ incidx, const double* __restrict__ imp, const double* __restrict__ imp2,
+                             double* __restrict__ xbar, double* __restrict__ gminv) {
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= nb) return;
+    double a1[6] = {0, 0, 0, 0, 0, 0}, a2[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
+        const double* q = imp + (size_t)incidx[i] * 6;
+        const double* r = imp2 + (size_t)incidx[i] * 6;
+        for (int k = 0; k < 6; ++k) {
+            a1[k] += q[k];
+            a2[k] += r[k];
+        }
+    }
+    double g[6], gm[6];
+    load_body(xbar, bi, g);
+    load_body(gminv, bi, gm);
+    for (int k = 0; k < dof[bi]; ++k) {
+        gm[k] += g[k] * a2[k];
+        g[k] -= a1[k];
+    }
+    store6(xbar + (size_t)bi * kBodyStride, g);
+    store6(gminv + (size_t)bi * kBodyStride, gm);
+}
+
+// Final contact pass (both modes): side vectors of (a) W^T s, (b) W^T pv, (c) sum_i rho_i W_ij^2,
+// (d) W^T (e .* c''_bar); and mu_bar per contact.
+__global__ void k_adj_final_contacts(int nc, int implicit, double alpha, const int* __restrict__ ida,
+                                     const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ R,
+                                     const double* __restrict__ mu, const double* __restrict__ rest,
+                                     const double* __restrict__ pv, const double* __restrict__ S, double* __restrict__ ia,
+                                     double* __restrict__ ib, double* __restrict__ ic, double* __restrict__ id,
+                                     double* __restrict__ gmu) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) {
+        if (gmu) gmu[c] = 0.0;
+        return;
+    }
+    const int nr = mu[c] > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double s[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, rho[3] = {0, 0, 0}, ec[3] = {0, 0, 0};
+    double mub = 0;
+    for (int k = 0; k < nr; ++k) {
+        const double Rk = R[(size_t)k * nc + c];
+        pp[k] = pv[(size_t)k * nc + c];
+        double cbar;
+        if (implicit) {
+            const double xi = S[(size_t)(A_XI + k) * nc + c];
+            s[k] = Rk * xi;
+            rho[k] = xi * S[(size_t)(A_Y + k) * nc + c] * Rk * Rk / alpha;
+            cbar = -s[k];
+            mub += S[(size_t)(A_DMU + k) * nc + c] * S[(size_t)(A_LAM + k) * nc + c];
+        } else {
+            rho[k] = -S[(size_t)(A_RBAR + k) * nc + c] * Rk * Rk / alpha;
+            cbar = S[(size_t)(A_CBAR + k) * nc + c];
+        }
+        ec[k] = rest[(size_t)k * nc + c] * cbar;
+    }
+    if (!implicit) mub = S[(size_t)A_MUBAR * nc + c];
+    if (gmu) gmu[c] = mub;
+    write_side_impulses(ia, c, a, b, nr, j, s);
+    write_side_impulses(ib, c, a, b, nr, j, pp);
+    write_side_impulses(id, c, a, b, nr, j, ec);
+    // (c): per side, sum_k rho_k W_kj^2 (signs square away)
+    double la[6] = {0, 0, 0, 0, 0, 0}, lbv[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < nr; ++k)
+        for (int m = 0; m < 3; ++m) {
+            const double d2 = j.d[k][m] * j.d[k][m] * rho[k];
+            la[m] += d2;
+            lbv[m] += d2;
+            la[3 + m] += j.aA[k][m] * j.aA[k][m] * rho[k];
+            lbv[3 + m] += j.aB[k][m] * j.aB[k][m] * rho[k];
+        }
+    if (a >= 0) store6(ic + (size_t)(2 * c) * 6, la);
+    if (b >= 0) store6(ic + (size_t)(2 * c + 1) * 6, lbv);
+}
+
+// Final body pass: x_bar = base - (a); M^-1_bar = acc + x_bar .* (b) + (c); u_free backward.
+__global__ void k_adj_final_bodies(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                   const double* __restrict__ v, const double* __restrict__ fext,
+                                   const double* __restrict__ fgyro, const int* __restrict__ incptr,
+                                   const int* __restrict__ incidx, const double* __restrict__ ia,
+                                   const double* __restrict__ ib, const double* __restrict__ ic,
+                                   const double* __restrict__ id, const double* __restrict__ base,
+                                   const double* __restrict__ gacc, double h, double* __restrict__ gv,
+                                   double* __restrict__ gm, double* __restrict__ gf) {
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= nb) return;
+    double sa[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0}, sc[6] = {0, 0, 0, 0, 0, 0}, sd[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
+        const size_t o = (size_t)incidx[i] * 6;
+        for (int k = 0; k < 6; ++k) {
+            sa[k] += ia[o + k];
+            sb[k] += ib[o + k];
+            sc[k] += ic[o + k];
+            sd[k] += id[o + k];
+        }
+    }
+    const int nd = dof[bi];
+    double g[6], acc[6], m[6], vv[6], fe[6], fg[6];
+    load_body(base, bi, g);
+    load_body(gacc, bi, acc);
+    load_body(v, bi, vv);
+    load_body(fext, bi, fe);
+    load_body(fgyro, bi, fg);
+    minv_body(minv, bi, m);
+    double xb[6], gmo[6] = {0, 0, 0, 0, 0, 0}, gvo[6] = {0, 0, 0, 0, 0, 0}, gfo[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < nd; ++k) {
+        xb[k] = g[k] - sa[k];  // u_free_bar
+        gmo[k] = acc[k] + xb[k] * sb[k] + sc[k] + h * xb[k] * (fe[k] + fg[k]);
+        gfo[k] = h * m[k] * xb[k];
+        gvo[k] = xb[k] + sd[k];
+    }
+    if (nd == 6) {
+        // u_free_ang = w + h I^-1 (tau + f_gyro(w)), f_gyro = -w x (I w):
+        // v_bar_w += -h ( -I (w x g') + (I w) x g' ),  g' = I^-1 u_bar;  I^-1_bar += h I_j^2 w_j [g' x w]_j
+        const double w[3] = {vv[3], vv[4], vv[5]};
+        double I[3], Iw[3], gp[3];
+        for (int k = 0; k < 3; ++k) {
+            I[k] = m[3 + k] > 0 ? 1.0 / m[3 + k] : 0.0;
+            Iw[k] = I[k] * w[k];
+            gp[k] = m[3 + k] * xb[3 + k];
+        }
+        const double wxg[3] = {w[1] * gp[2] - w[2] * gp[1], w[2] * gp[0] - w[0] * gp[2], w[0] * gp[1] - w[1] * gp[0]};
+        const double Iwxg[3] = {Iw[1] * gp[2] - Iw[2] * gp[1], Iw[2] * gp[0] - Iw[0] * gp[2], Iw[0] * gp[1] - Iw[1] * gp[0]};
+        const double gxw[3] = {-wxg[0], -wxg[1], -wxg[2]};
+        for (int k = 0; k < 3; ++k) {
+            gvo[3 + k] += -h * (-I[k] * wxg[k] + Iwxg[k]);
+            gmo[3 + k] += h * I[k] * I[k] * w[k] * gxw[k];
+        }
+    }
+    store6(gv + (size_t)bi * kBodyStride, gvo);
+    store6(gm + (size_t)bi * kBodyStride, gmo);
+    store6(gf + (size_t)bi * kBodyStride, gfo);
+}
+
+}  // namespace cdx
+
+using namespace cdx;
+
+static int adj_alloc(cdx_ctx* ctx) {
+    const size_t nc = (size_t)ctx->nc, nb = (size_t)ctx->nb;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj, (size_t)A_COUNT * nc, &ctx->adj_cap, "adjoint contact scratch"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_imp, 48 * nc, &ctx->adj_imp_cap, "adjoint side vectors"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_body, 6 * kBodyStride * nb, &ctx->adj_body_cap, "adjoint body scratch"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_partial, 2 * (size_t)grid_for((int)nc) + 2, &ctx->adj_partial_cap, "adjoint partials"));
+    return CDX_OK;
+}
+
+static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
+                       const double* g_v, const double* g_p, double* gv_out, double* gminv_out, double* gfext_out,
+                       double* gmu_out, double** dlp_out, double** dgmu_out, int* iters_out, double* res_out, int* mode_io) {
+    const int nb = ctx->nb, nc = ctx->nc;
+    double* B = ctx->d_adj_body;  // [lx | t (later gf) | xbar | gminv_acc | gv | gm]
+    double* lx = B;
+    double* t = B + (size_t)kBodyStride * nb;
+    double* xbar = B + 2 * (size_t)kBodyStride * nb;
+    double* gacc = B + 3 * (size_t)kBodyStride * nb;
+    double* gvd = B + 4 * (size_t)kBodyStride * nb;
+    double* gmd = B + 5 * (size_t)kBodyStride * nb;
+    double* gfd = t;  // t is free again by the final stage
+    {
+        std::vector<double> h((size_t)kBodyStride * nb, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < ctx->dof[b]; ++k) h[(size_t)b * kBodyStride + k] = g_v[ctx->voff[b] + k];
+        if (nb) CDX_HIP(ctx, hipMemcpyAsync(lx, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        if (nb) CDX_HIP(ctx, hipMemsetAsync(gacc, 0, sizeof(double) * kBodyStride * (size_t)nb, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (g_p && nc) {
+        CDX_HIP(ctx, hipMalloc((void**)dlp_out, sizeof(double) * 3 * (size_t)nc));
+        CDX_HIP(ctx, hipMemcpy(*dlp_out, g_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyHostToDevice));
+    }
+    double* dlp = *dlp_out;
+    CDX_HIP(ctx, hipMalloc((void**)dgmu_out, sizeof(double) * (size_t)(nc ? nc : 1)));
+    double* dgmu = *dgmu_out;
+    double* S = ctx->d_adj;
+    double* I0 = ctx->d_adj_imp;  // 4 side-vector buffers of 12 doubles per contact
+    double* I1 = I0 + 12 * (size_t)nc;
+    double* I2 = I0 + 24 * (size_t)nc;
+    double* I3 = I0 + 36 * (size_t)nc;
+    const int gc = grid_for(nc), gb = grid_for(nb);
+    const bool rows = nc > 0 && ctx->nrows > 0;
+    int& mode = *mode_io;
+    if (rows && mode == 0) {
+        CDX_TRY(reset_control(ctx));
+        CDX_LAUNCH(ctx, "adj_setup", k_adj_setup, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias,
+                   ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, ctx->d_x, ctx->d_p, lx, dlp, S, I0);
+        CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx, I0, t,
+                   nullptr);
+        auto iter = [&](int it) -> int {
+            const int check = (it % 10) == 9;
+            CDX_LAUNCH(ctx, "adj_iter", k_adj_iter, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, t, S,
+                       I0, check, ctx->d_adj_partial, ctx->d_ctrl);
+            CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                       I0, t, ctx->d_ctrl);
+            if (check)
+                CDX_LAUNCH(ctx, "adj_residual", k_adj_finalize, 1, kBlock, gc, ctx->d_adj_partial, it, aprm->tol, ctx->d_ctrl,
+                           ctx->d_res, ctx->h_flag_dev);
+            return hip_check(ctx, hipGetLastError(), "adjoint iteration launch");
+        };
+        int it_done = 0;
+        CDX_TRY(run_sweeps(ctx, aprm->max_iter, 10, 4, iter, &it_done));
+        int ctrl[3];
+        CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+        CDX_TRY(read_residual(ctx, res_out));
+        *iters_out = it_done;
+        if (!ctrl[0]) {  // not converged within max_iter
+            if (aprm->mode != CDX_ADJ_AUTO)
+                return set_err(ctx, CDX_DIVERGED, "cdx_pj_adjoint: implicit adjoint iteration did not converge");
+            mode = 1;
+        }
+    }
+    if (rows && mode == 1) {
+        if (!ctx->record_valid)
+            return set_err(ctx, CDX_BAD_ARG, "cdx_pj_adjoint: unrolled mode needs the forward solve recorded (cdx_set_recording)");
+        const int K = ctx->recorded;
+        CDX_HIP(ctx, hipMemsetAsync(S, 0, sizeof(double) * A_COUNT * (size_t)nc, ctx->stream));
+        if (dlp) {  // p_bar_K = L_p (component-major)
+            std::vector<double> hl(3 * (size_t)nc), cm(3 * (size_t)nc);
+            for (int c = 0; c < nc; ++c)
+                for (int k = 0; k < 3; ++k) cm[(size_t)k * nc + c] = g_p[3 * (size_t)c + k];
+            CDX_HIP(ctx, hipMemcpyAsync(S + (size_t)A_LAM * nc, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice,
+                                        ctx->stream));
+            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        CDX_HIP(ctx, hipMemcpyAsync(xbar, lx, sizeof(double) * kBodyStride * (size_t)nb, hipMemcpyDeviceToDevice, ctx->stream));
+        for (int k = K - 1; k >= 0; --k) {
+            const double* xk = ctx->d_rec_x + (size_t)k * kBodyStride * nb;
+            const double* pk = ctx->d_rec_p + (size_t)k * 3 * nc;
+            const double* pk1 = (k + 1 < K) ? ctx->d_rec_p + (size_t)(k + 1) * 3 * nc : ctx->d_p;
+            CDX_LAUNCH(ctx, "unr_contacts", k_unr_contacts, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                       ctx->d_bias, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, xk, pk, pk1, xbar, S, I0, I1);
+            CDX_LAUNCH(ctx, "unr_bodies", k_unr_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_incptr, ctx->d_incidx, I0, I1, xbar,
+                       gacc);
+        }
+        CDX_HIP(ctx, hipGetLastError());
+        *iters_out = K;
+        *res_out = 0.0;
+    }
+    // final stage (shared by both modes)
+    const double* pv = mode == 0 ? ctx->d_p : ctx->d_rec_p;  // p* (implicit) or p_0 (unrolled)
+    const double* base = mode == 0 ? lx : xbar;
+    if (rows) {
+        CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, mode == 0 ? 1 : 0, prm->alpha, ctx->d_ida, ctx->d_idb,
+                   ctx->d_J, ctx->d_R, ctx->d_mu, ctx->d_rest, pv, S, I0, I1, I2, I3, dgmu);
+    } else {
+        CDX_HIP(ctx, hipMemsetAsync(I0, 0, sizeof(double) * 48 * (size_t)(nc ? nc : 1), ctx->stream));
+        CDX_HIP(ctx, hipMemsetAsync(dgmu, 0, sizeof(double) * (size_t)(nc ? nc : 1), ctx->stream));
+    }
+    if (nb)
+        CDX_LAUNCH(ctx, "adj_final_bodies", k_adj_final_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext,
+                   ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, I0, I1, I2, I3, base, gacc, dt, gvd, gmd, gfd);
+    CDX_HIP(ctx, hipGetLastError());
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> hv((size_t)kBodyStride * nb), hm((size_t)kBodyStride * nb), hf((size_t)kBodyStride * nb);
+    if (nb) {
+        CDX_HIP(ctx, hipMemcpy(hv.data(), gvd, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+        CDX_HIP(ctx, hipMemcpy(hm.data(), gmd, hm.size() * sizeof(double), hipMemcpyDeviceToHost));
+        CDX_HIP(ctx, hipMemcpy(hf.data(), gfd, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < ctx->dof[b]; ++k) {
+            const int o = ctx->voff[b] + k;
+            if (gv_out) gv_out[o] = hv[(size_t)b * kBodyStride + k];
+            if (gminv_out) gminv_out[o] = hm[(size_t)b * kBodyStride + k];
+            if (gfext_out) gfext_out[o] = hf[(size_t)b * kBodyStride + k];
+        }
+    if (gmu_out && nc) CDX_HIP(ctx, hipMemcpy(gmu_out, dgmu, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost));
+    return CDX_OK;
+}
+
+extern "C" {
+
+int cdx_set_recording(cdx_ctx* ctx, int enable, int max_sweeps) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (enable && max_sweeps < 1) return set_err(ctx, CDX_BAD_ARG, "cdx_set_recording: max_sweeps must be >= 1");
+    ctx->record = enable != 0;
+    ctx->record_max = enable ? max_sweeps : 0;
+    ctx->recorded = 0;
+    return CDX_OK;
+}
+
+int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm, const double* g_v,
+                   const double* g_p, double* gv_out, double* gminv_out, double* gfext_out, double* gmu_out,
+                   cdx_adjoint_info* info) {
+    if (!ctx || !prm || !aprm || !g_v) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    if (!ctx->solved || !ctx->pj_last) return set_err(ctx, CDX_NO_STATE, "cdx_pj_adjoint: needs a preceding cdx_pj_solve");
+    if (aprm->mode < 0 || aprm->mode > 2 || aprm->max_iter < 1) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_adjoint: bad params");
+    if (info) {
+        info->iters = 0;
+        info->residual = std::nan("");
+        info->mode_used = aprm->mode;
+    }
+    CDX_TRY(adj_alloc(ctx));
+    double* dlp = nullptr;
+    double* dgmu = nullptr;
+    int iters = 0;
+    double res = std::nan("");
+    int mode = aprm->mode == CDX_ADJ_UNROLLED ? 1 : 0;
+    const int st = adjoint_run(ctx, dt, prm, aprm, g_v, g_p, gv_out, gminv_out, gfext_out, gmu_out, &dlp, &dgmu, &iters,
+                               &res, &mode);
+    if (dlp) hipFree(dlp);
+    if (dgmu) hipFree(dgmu);
+    if (info) {
+        info->iters = iters;
+        info->residual = res;
+        info->mode_used = mode == 0 ? CDX_ADJ_IMPLICIT : CDX_ADJ_UNROLLED;
+    }
+    return st;
+}
+
+}  // extern "C"

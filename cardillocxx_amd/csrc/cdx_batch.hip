@@ -185,6 +185,7 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (iters) CDX_HIP(ctx, hipMemcpy(iters, ctx->d_scene_i + 5 * ns + 2, sizeof(int) * ns, hipMemcpyDeviceToHost));
     if (residuals) CDX_HIP(ctx, hipMemcpy(residuals, ctx->d_scene_res, sizeof(double) * ns, hipMemcpyDeviceToHost));
     ctx->solved = true;
+    ctx->pj_last = false;
     return CDX_OK;
 }
 

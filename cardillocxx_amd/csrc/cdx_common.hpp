@@ -98,6 +98,22 @@ struct cdx_ctx {
     size_t scene_res_cap = 0;
     int batch_block = 256, batch_groups = 1;
 
+    // ---- differentiation
+    bool pj_last = false;  // the last solve was cdx_pj_solve (adjoint available)
+    bool record = false, record_valid = false;
+    int record_max = 0, recorded = 0;
+    double* d_rec_x = nullptr;  // [record_max][nb*8]
+    double* d_rec_p = nullptr;  // [record_max][3*nc]
+    size_t rec_x_cap = 0, rec_p_cap = 0;
+    double* d_adj = nullptr;
+    size_t adj_cap = 0;
+    double* d_adj_imp = nullptr;
+    size_t adj_imp_cap = 0;
+    double* d_adj_body = nullptr;
+    size_t adj_body_cap = 0;
+    double* d_adj_partial = nullptr;
+    size_t adj_partial_cap = 0;
+
     // ---- solve control
     int* d_ctrl = nullptr;         // [0] done flag, [1] iterations, [2] status
     double* d_res = nullptr;       // [0] last residual

@@ -341,6 +341,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     const int nb = ctx->nb, nc = ctx->nc;
     if (nb == 0) {
         ctx->solved = true;
+    ctx->pj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_incidence(ctx));
@@ -353,6 +354,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
                                     ctx->stream));
         CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->solved = true;
+    ctx->pj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_coloring(ctx));
@@ -389,6 +391,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
         info->ncolors = ctx->ncolors;
     }
     ctx->solved = true;
+    ctx->pj_last = false;
     if (st) return set_err(ctx, st, "Projected Gauss-Seidel diverged");
     return CDX_OK;
 }

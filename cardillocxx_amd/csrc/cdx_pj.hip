@@ -83,6 +83,69 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies(int nb, const int* __restr
     }
 }
 
+// Body pass with 8 lanes per body: lane l sums incidences l, l+8, ... of its body, the 8 partial
+// twists are combined by a fixed xor-shuffle tree (deterministic), and lanes 0..5 each update one
+// velocity component.  Eight times the lanes of one-lane-per-body keeps enough waves in flight to
+// hide the latency of the index -> side-impulse gather chain.
+constexpr int kLanesPerBody = 8;
+
+__global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                                       const int* __restrict__ incptr, const int* __restrict__ incidx,
+                                                       const double* __restrict__ imp, double* __restrict__ x, int check,
+                                                       double tol_rel, double tol_abs, double* __restrict__ partial,
+                                                       const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    const bool live = bi < nb;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int i0 = 0, i1 = 0;
+    if (live) {
+        i0 = incptr[bi];
+        i1 = incptr[bi + 1];
+        for (int i = i0 + l; i < i1; i += kLanesPerBody) {
+            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
+            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+            acc[0] += q0.x;
+            acc[1] += q0.y;
+            acc[2] += q1.x;
+            acc[3] += q1.y;
+            acc[4] += q2.x;
+            acc[5] += q2.y;
+        }
+    }
+#pragma unroll
+    for (int off = kLanesPerBody / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += __shfl_xor(acc[k], off, kLanesPerBody);
+    double e2 = 0;
+    if (live && l < 6) {
+        const int nd = dof[bi];
+        double a = acc[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k)
+            if (l == k) a = acc[k];
+        if (l < nd) {
+            const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+            const double M = mk > 0 ? 1.0 / mk : 0.0;
+            double* xp = x + (size_t)bi * kBodyStride + l;
+            const double u = *xp;
+            const double un = u + a / M;
+            if (i1 > i0) *xp = un;
+            if (check) {
+                const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                const double e = (un - u) / q;
+                e2 = e * e;
+            }
+        }
+    }
+    if (check) {
+        const double s = block_sum(e2, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
 // mode 0: PJ (break when res <= 1); mode 1: GS (break when res < 1, NaN/Inf -> diverged).
 __global__ void __launch_bounds__(kBlock) k_residual_finalize(int nparts, const double* __restrict__ partial, int nv,
                                                               int it, int mode, int* __restrict__ ctrl,
@@ -346,6 +409,9 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         info->ncolors = 0;
     }
     ctx->solved = false;
+    ctx->pj_last = false;
+    ctx->record_valid = false;
+    ctx->recorded = 0;
     const int nb = ctx->nb, nc = ctx->nc;
     if (nb == 0) {
         ctx->solved = true;
@@ -364,6 +430,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     if (!have_contacts) {  // no contacts: free solution only (projected_jacobi.cpp:460-467)
         CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->solved = true;
+        ctx->pj_last = true;
+        ctx->record_valid = ctx->record;
         return CDX_OK;
     }
     const int gb = grid_for(nb), gc = grid_for(nc);
@@ -404,19 +472,35 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         info->residual = r;
     }
     ctx->solved = true;
+    ctx->pj_last = true;
+    ctx->recorded = iters;
+    ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && iters <= ctx->record_max;
     return CDX_OK;
 }
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out) {
     const int nb = ctx->nb, nc = ctx->nc;
     const int check_every = prm->check_every;
+    const int gb8 = grid_for(nb * kLanesPerBody);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
+    (void)gb;
+    if (ctx->record) {
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_rec_x, (size_t)ctx->record_max * kBodyStride * nb, &ctx->rec_x_cap, "recorded x"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_rec_p, (size_t)ctx->record_max * 3 * nc, &ctx->rec_p_cap, "recorded p"));
+    }
     auto sweep = [&](int it) -> int {
+        if (ctx->record && it < ctx->record_max) {  // (x_k, p_k) before sweep k, for the unrolled backward
+            CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rec_x + (size_t)it * kBodyStride * nb, ctx->d_x,
+                                        sizeof(double) * kBodyStride * (size_t)nb, hipMemcpyDeviceToDevice, ctx->stream));
+            CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rec_p + (size_t)it * 3 * nc, ctx->d_p, sizeof(double) * 3 * (size_t)nc,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        }
         const int check = (it % check_every) == 0;
         CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
                    ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
-        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
-        if (check) CDX_TRY(launch_residual_finalize(ctx, gb, it, 0));
+        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0));
         return hip_check(ctx, hipGetLastError(), "pj sweep launch");
     };
     return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);

@@ -210,6 +210,316 @@ int cdx_sphere_contacts(int nb, const double* q, const double* radius, const dou
     return total;
 }
 
+
+}  // extern "C"
+
+namespace {
+
+struct Body {
+    int type;  // 0 sphere, 1 box
+    V3 c;
+    double R[3][3];  // columns are the body axes in world coordinates
+    V3 h;            // sphere: h.x = radius; box: half extents
+    double bound;    // bounding-sphere radius
+};
+
+inline V3 col(const double R[3][3], int k) { return {R[0][k], R[1][k], R[2][k]}; }
+
+void rot_of(const double* q, double R[3][3]) {
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0][0] = 1 - 2 * (y * y + z * z), R[0][1] = 2 * (x * y - z * w), R[0][2] = 2 * (x * z + y * w);
+    R[1][0] = 2 * (x * y + z * w), R[1][1] = 1 - 2 * (x * x + z * z), R[1][2] = 2 * (y * z - x * w);
+    R[2][0] = 2 * (x * z - y * w), R[2][1] = 2 * (y * z + x * w), R[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+Body make_body(const double* q, int type, const double* dims) {
+    Body b;
+    b.type = type;
+    b.c = {q[0], q[1], q[2]};
+    rot_of(q + 3, b.R);
+    if (type == 0) {
+        b.h = {dims[0], 0, 0};
+        b.bound = dims[0];
+    } else {
+        b.h = {dims[0], dims[1], dims[2]};
+        b.bound = std::sqrt(dims[0] * dims[0] + dims[1] * dims[1] + dims[2] * dims[2]);
+    }
+    return b;
+}
+
+double hcomp(const V3& h, int k) { return k == 0 ? h.x : (k == 1 ? h.y : h.z); }
+
+// Body (dynamic, index `bi`) against static half-space n.x >= d; the wall is side A.
+void wall_contacts(int bi, int w, const Body& B, V3 n, double d, double margin, std::vector<RawContact>& out) {
+    if (B.type == 0) {
+        const double hgt = dot(n, B.c) - d;
+        const double gap = hgt - B.h.x;
+        if (gap > margin) return;
+        const V3 p1 = B.c - hgt * n, p2 = B.c - B.h.x * n;
+        out.push_back(RawContact{-1, bi, w, n, 0.5 * (p1 + p2), std::max(0.0, -gap)});
+        return;
+    }
+    for (int k = 0; k < 8; ++k) {
+        const double sx = (k & 1) ? 1 : -1, sy = (k & 2) ? 1 : -1, sz = (k & 4) ? 1 : -1;
+        const V3 corner = B.c + (sx * B.h.x) * col(B.R, 0) + (sy * B.h.y) * col(B.R, 1) + (sz * B.h.z) * col(B.R, 2);
+        const double dist = dot(n, corner) - d;
+        if (dist > margin) continue;
+        out.push_back(RawContact{-1, bi, w, n, corner - (0.5 * dist) * n, std::max(0.0, -dist)});
+    }
+}
+
+// Sphere S against box X; returns normal from the box to the sphere, gap and the two witnesses.
+bool sphere_box(const Body& S, const Body& X, double margin, V3& n, V3& pbox, V3& psph, double& gap) {
+    const V3 dl = S.c - X.c;
+    double loc[3] = {dot(dl, col(X.R, 0)), dot(dl, col(X.R, 1)), dot(dl, col(X.R, 2))};
+    double cp[3];
+    bool inside = true;
+    for (int k = 0; k < 3; ++k) {
+        const double hk = hcomp(X.h, k);
+        cp[k] = std::max(-hk, std::min(hk, loc[k]));
+        if (cp[k] != loc[k]) inside = false;
+    }
+    V3 nl;
+    if (!inside) {
+        V3 diff{loc[0] - cp[0], loc[1] - cp[1], loc[2] - cp[2]};
+        const double dist = std::sqrt(dot(diff, diff));
+        gap = dist - S.h.x;
+        if (gap > margin) return false;
+        nl = (1.0 / dist) * diff;
+    } else {
+        int kb = 0;
+        double pen = 1e300;
+        for (int k = 0; k < 3; ++k) {
+            const double pk = hcomp(X.h, k) - std::fabs(loc[k]);
+            if (pk < pen) pen = pk, kb = k;
+        }
+        const double sg = loc[kb] >= 0 ? 1.0 : -1.0;
+        nl = {kb == 0 ? sg : 0.0, kb == 1 ? sg : 0.0, kb == 2 ? sg : 0.0};
+        cp[kb] = sg * hcomp(X.h, kb);
+        gap = -(pen + S.h.x);
+    }
+    n = nl.x * col(X.R, 0) + nl.y * col(X.R, 1) + nl.z * col(X.R, 2);
+    pbox = X.c + cp[0] * col(X.R, 0) + cp[1] * col(X.R, 1) + cp[2] * col(X.R, 2);
+    psph = S.c - S.h.x * n;
+    return true;
+}
+
+// Box-box by the separating-axis test; contacts from clipping the incident face against the
+// reference face (face axes) or one closest-point contact (edge-edge axes).  Normal from A to B.
+void box_box(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {
+    const V3 d = B.c - A.c;
+    double best = -1e300;
+    int kind = -1, ai = -1, bj = -1;
+    V3 axis_best{0, 0, 1};
+    auto proj = [](const Body& X, V3 ax) {
+        return X.h.x * std::fabs(dot(col(X.R, 0), ax)) + X.h.y * std::fabs(dot(col(X.R, 1), ax)) +
+               X.h.z * std::fabs(dot(col(X.R, 2), ax));
+    };
+    auto test = [&](V3 ax, int k, int i, int j, double bias) {
+        const double sep = std::fabs(dot(d, ax)) - proj(A, ax) - proj(B, ax);
+        if (sep > margin) return false;
+        if (sep > best + bias) {
+            best = sep;
+            kind = k;
+            ai = i;
+            bj = j;
+            axis_best = ax;
+        }
+        return true;
+    };
+    for (int i = 0; i < 3; ++i)
+        if (!test(col(A.R, i), 0, i, -1, 0.0)) return;
+    for (int j = 0; j < 3; ++j)
+        if (!test(col(B.R, j), 1, -1, j, 0.0)) return;
+    const double edge_bias = 1e-9 + 0.05 * std::fabs(best);  // prefer face contacts when nearly tied
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            V3 ax = cross(col(A.R, i), col(B.R, j));
+            const double len = std::sqrt(dot(ax, ax));
+            if (len < 1e-9) continue;
+            if (!test((1.0 / len) * ax, 2, i, j, edge_bias)) return;
+        }
+    V3 n = dot(d, axis_best) < 0 ? -1.0 * axis_best : axis_best;  // A -> B
+    if (kind == 2) {
+        // edges of A along axis ai and of B along bj closest to each other
+        V3 ea = A.c, eb = B.c;
+        for (int k = 0; k < 3; ++k) {
+            if (k != ai) ea = ea + ((dot(col(A.R, k), n) > 0 ? 1.0 : -1.0) * hcomp(A.h, k)) * col(A.R, k);
+            if (k != bj) eb = eb - ((dot(col(B.R, k), n) > 0 ? 1.0 : -1.0) * hcomp(B.h, k)) * col(B.R, k);
+        }
+        const V3 ua = col(A.R, ai), ub = col(B.R, bj);
+        const V3 r = ea - eb;
+        const double b = dot(ua, ub), dd = dot(ua, r), e = dot(ub, r);
+        const double den = 1 - b * b;
+        double s = den > 1e-12 ? (b * e - dd) / den : 0.0;
+        s = std::max(-hcomp(A.h, ai), std::min(hcomp(A.h, ai), s));
+        double t = std::max(-hcomp(B.h, bj), std::min(hcomp(B.h, bj), b * s + e));
+        const V3 pa = ea + s * ua, pb = eb + t * ub;
+        const double sep = dot(pb - pa, n);
+        if (sep > margin) return;
+        out.push_back(RawContact{ia, ib, -1, n, 0.5 * (pa + pb), std::max(0.0, -sep)});
+        return;
+    }
+    // face contact: reference face on R (outward normal nr), incident face on I
+    const Body& Rb = kind == 0 ? A : B;
+    const Body& Ib = kind == 0 ? B : A;
+    const int rf = kind == 0 ? ai : bj;
+    const V3 nr = kind == 0 ? n : -1.0 * n;
+    int jf = 0;
+    double bd = -1;
+    for (int k = 0; k < 3; ++k) {
+        const double v = std::fabs(dot(col(Ib.R, k), nr));
+        if (v > bd) bd = v, jf = k;
+    }
+    const double sg = dot(col(Ib.R, jf), nr) > 0 ? -1.0 : 1.0;
+    const V3 fc = Ib.c + (sg * hcomp(Ib.h, jf)) * col(Ib.R, jf);
+    const int u1 = (jf + 1) % 3, u2 = (jf + 2) % 3;
+    std::vector<V3> poly = {fc + hcomp(Ib.h, u1) * col(Ib.R, u1) + hcomp(Ib.h, u2) * col(Ib.R, u2),
+                            fc - hcomp(Ib.h, u1) * col(Ib.R, u1) + hcomp(Ib.h, u2) * col(Ib.R, u2),
+                            fc - hcomp(Ib.h, u1) * col(Ib.R, u1) - hcomp(Ib.h, u2) * col(Ib.R, u2),
+                            fc + hcomp(Ib.h, u1) * col(Ib.R, u1) - hcomp(Ib.h, u2) * col(Ib.R, u2)};
+    // Sutherland-Hodgman against the four side planes of the reference face
+    const int r1 = (rf + 1) % 3, r2 = (rf + 2) % 3;
+    const int sides[4][2] = {{r1, 1}, {r1, -1}, {r2, 1}, {r2, -1}};
+    for (auto& sd : sides) {
+        const V3 ax = ((double)sd[1]) * col(Rb.R, sd[0]);
+        const double lim = dot(ax, Rb.c) + hcomp(Rb.h, sd[0]);
+        std::vector<V3> np;
+        for (size_t k = 0; k < poly.size(); ++k) {
+            const V3 P = poly[k], Q = poly[(k + 1) % poly.size()];
+            const double dp = dot(ax, P) - lim, dq = dot(ax, Q) - lim;
+            if (dp <= 0) np.push_back(P);
+            if ((dp <= 0) != (dq <= 0)) np.push_back(P + (dp / (dp - dq)) * (Q - P));
+        }
+        poly.swap(np);
+        if (poly.empty()) return;
+    }
+    const double plane = dot(nr, Rb.c) + hcomp(Rb.h, rf);
+    for (const V3& P : poly) {
+        const double sep = dot(nr, P) - plane;
+        if (sep > margin) continue;
+        out.push_back(RawContact{ia, ib, -1, n, P - (0.5 * sep) * nr, std::max(0.0, -sep)});
+    }
+}
+
+void pair_contacts(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {
+    const V3 d = B.c - A.c;
+    const double reach = A.bound + B.bound + margin;
+    if (dot(d, d) > reach * reach) return;
+    if (A.type == 0 && B.type == 0) {
+        const double dist = std::sqrt(dot(d, d));
+        const double gap = dist - A.h.x - B.h.x;
+        if (gap > margin) return;
+        const V3 n = dist > 0 ? (1.0 / dist) * d : V3{0, 0, 1};
+        out.push_back(RawContact{ia, ib, -1, n, 0.5 * ((A.c + A.h.x * n) + (B.c - B.h.x * n)), std::max(0.0, -gap)});
+    } else if (A.type == 1 && B.type == 1) {
+        box_box(ia, ib, A, B, margin, out);
+    } else {
+        const bool a_sphere = A.type == 0;
+        V3 n, pbox, psph;
+        double gap;
+        if (!sphere_box(a_sphere ? A : B, a_sphere ? B : A, margin, n, pbox, psph, gap)) return;
+        // n points from the box to the sphere; contact normals run from A to B
+        const V3 nab = a_sphere ? -1.0 * n : n;
+        out.push_back(RawContact{ia, ib, -1, nab, 0.5 * (pbox + psph), std::max(0.0, -gap)});
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const int* shape, const double* dims,
+                       const double* mu_body, int nw, const double* wall_n, const double* wall_d, const double* mu_wall,
+                       double margin, int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA,
+                       double* angB, double* point, double* depth, int* contact_off) {
+    if (nscenes < 1 || !body_off || nw < 0) return -1;
+    const int nb = body_off[nscenes];
+    std::vector<Body> bodies(nb);
+    for (int i = 0; i < nb; ++i) {
+        if (shape[i] != 0 && shape[i] != 1) return -1;
+        bodies[i] = make_body(q + 7 * (size_t)i, shape[i], dims + 3 * (size_t)i);
+    }
+    std::vector<std::vector<RawContact>> per(nb);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nscenes; ++s) {
+        const int b0 = body_off[s], b1 = body_off[s + 1];
+        if (b1 <= b0) continue;
+        double bmax = 0;
+        std::vector<double> cen(3 * (size_t)(b1 - b0));
+        for (int i = b0; i < b1; ++i) {
+            bmax = std::max(bmax, bodies[i].bound);
+            cen[3 * (i - b0)] = bodies[i].c.x, cen[3 * (i - b0) + 1] = bodies[i].c.y, cen[3 * (i - b0) + 2] = bodies[i].c.z;
+        }
+        Grid g;
+        g.build(b1 - b0, cen.data(), 3, std::max(2 * bmax + margin, 1e-9));
+        for (int a = b0; a < b1; ++a) {
+            auto& out = per[a];
+            for (int w = 0; w < nw; ++w)
+                wall_contacts(a, w, bodies[a], V3{wall_n[3 * w], wall_n[3 * w + 1], wall_n[3 * w + 2]}, wall_d[w], margin, out);
+            std::vector<int> nbrs;
+            g.around(&cen[3 * (a - b0)], [&](int j) {
+                if (j + b0 > a) nbrs.push_back(j + b0);
+            });
+            std::sort(nbrs.begin(), nbrs.end());
+            for (int b : nbrs) pair_contacts(a, b, bodies[a], bodies[b], margin, out);
+        }
+    }
+    std::vector<int> off(nb + 1, 0);
+    for (int a = 0; a < nb; ++a) off[a + 1] = off[a] + (int)per[a].size();
+    const int total = off[nb];
+    if (contact_off)
+        for (int s = 0; s <= nscenes; ++s) contact_off[s] = off[body_off[s]];
+    if (total > cap) return total;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int a = 0; a < nb; ++a)
+        for (size_t k = 0; k < per[a].size(); ++k)
+            fill_contact(per[a][k], q, mu_body, mu_wall, off[a] + (int)k, body_a, body_b, mu, frame, angA, angB, point, depth);
+    return total;
+}
+
+
+int cdx_scene_rsa(int nscenes, const int* body_off, const double* bound, const double* lo, const double* hi, uint64_t seed,
+                  double* pos) {
+    if (nscenes < 1 || !body_off) return -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nscenes; ++s) {
+        std::mt19937_64 rng(seed * 1000003ULL + (uint64_t)s);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        const int b0 = body_off[s], b1 = body_off[s + 1];
+        std::vector<int> order(b1 - b0);
+        std::iota(order.begin(), order.end(), b0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bound[a] > bound[b]; });
+        std::vector<int> placed;
+        double top = hi[2];
+        for (int i : order) {
+            const double r = bound[i];
+            int tries = 0;
+            for (;;) {
+                const double p[3] = {lo[0] + r + U(rng) * std::max(0.0, hi[0] - lo[0] - 2 * r),
+                                     lo[1] + r + U(rng) * std::max(0.0, hi[1] - lo[1] - 2 * r),
+                                     lo[2] + r + U(rng) * std::max(0.0, top - lo[2] - 2 * r)};
+                bool ok = true;
+                for (int j : placed) {
+                    const double dx = p[0] - pos[3 * j], dy = p[1] - pos[3 * j + 1], dz = p[2] - pos[3 * j + 2];
+                    const double rr = r + bound[j];
+                    if (dx * dx + dy * dy + dz * dz < rr * rr) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (ok) {
+                    for (int k = 0; k < 3; ++k) pos[3 * i + k] = p[k];
+                    placed.push_back(i);
+                    break;
+                }
+                if (++tries % 200 == 0) top *= 1.1;  // column grows when the box is too full
+            }
+        }
+    }
+    return 0;
+}
+
 int cdx_match_warmstart(int nprev, const int* prev_a, const int* prev_b, const double* prev_point, const double* prev_imp,
                         int ncur, const int* cur_a, const int* cur_b, const double* cur_point, double max_dist,
                         double* warm_out) {

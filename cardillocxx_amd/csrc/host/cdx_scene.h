@@ -41,6 +41,25 @@ int cdx_sphere_contacts(int nb, const double* q, const double* radius, const dou
                         int* body_a, int* body_b, double* mu, double* frame, double* angA, double* angB,
                         double* point, double* depth);
 
+/* Spheres and boxes in nscenes independent scenes (bodies of scene s: [body_off[s], body_off[s+1]);
+ * only bodies of the same scene collide; every scene sees the same static half-spaces).
+ * shape[b]: 0 sphere (dims[3b] = radius), 1 box (dims[3b..3b+2] = half extents).
+ * Box-halfspace: every corner within the margin; box-sphere: closest point; box-box:
+ * separating-axis test, then the incident face clipped against the reference face (face axes)
+ * or one closest-point contact (edge-edge axes).  Contacts of scene s are
+ * [contact_off[s], contact_off[s+1]) when contact_off is given.  Return value as
+ * cdx_sphere_contacts. */
+int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const int* shape, const double* dims,
+                       const double* mu_body, int nw, const double* wall_n, const double* wall_d, const double* mu_wall,
+                       double margin, int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA,
+                       double* angB, double* point, double* depth, int* contact_off);
+
+/* Random sequential placement of bounding spheres (radius bound[b]) without overlap inside the
+ * box [lo, hi] of each scene, largest first; a scene's column grows upward when the box is too
+ * full.  pos[3*nb]. */
+int cdx_scene_rsa(int nscenes, const int* body_off, const double* bound, const double* lo, const double* hi, uint64_t seed,
+                  double* pos);
+
 /* Warm start: for each current contact, the impulse of the nearest unused previous contact of
  * the same body pair within max_dist, greedily by increasing distance
  * (contact_tracker.cpp:43-72); zero if none. */

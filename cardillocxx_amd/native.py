@@ -55,7 +55,12 @@ EXPORTED = [
     "cdx_batch_pj_solve",
     "cdx_batch_set_launch",
     "cdx_batch_autotune",
+    "cdx_set_recording",
+    "cdx_pj_adjoint",
 ]
+CDX_ADJ_IMPLICIT = 0
+CDX_ADJ_UNROLLED = 1
+CDX_ADJ_AUTO = 2
 
 
 class SolverParams(C.Structure):
@@ -71,6 +76,14 @@ class SolverParams(C.Structure):
         ("nesterov_beta_threshold", C.c_double),
         ("nesterov_restart_limit", C.c_int),
     ]
+
+
+class AdjointParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double)]
+
+
+class AdjointInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("residual", C.c_double), ("mode_used", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -104,6 +117,9 @@ def _declare(lib):
         "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
         "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
         "cdx_set_scenes": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+        "cdx_set_recording": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "cdx_pj_adjoint": (C.c_int, [_vp, C.c_double, C.POINTER(SolverParams), C.POINTER(AdjointParams), _dp, _dp, _dp, _dp,
+                                     _dp, _dp, C.POINTER(AdjointInfo)]),
         "cdx_batch_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _dp]),
         "cdx_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
         "cdx_batch_autotune": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _ip, _dp]),
@@ -136,6 +152,11 @@ def scene_lib():
         _scene.cdx_sphere_contacts.restype = C.c_int
         _scene.cdx_sphere_contacts.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
                                                _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        _scene.cdx_shape_contacts.restype = C.c_int
+        _scene.cdx_shape_contacts.argtypes = [C.c_int, _ip, _dp, _ip, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                                              _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        _scene.cdx_scene_rsa.restype = C.c_int
+        _scene.cdx_scene_rsa.argtypes = [C.c_int, _ip, _dp, _dp, _dp, C.c_uint64, _dp]
         _scene.cdx_match_warmstart.restype = C.c_int
         _scene.cdx_match_warmstart.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _dp, C.c_double, _dp]
         _scene.cdx_scene_sphere_grid.restype = C.c_int
@@ -259,6 +280,21 @@ class Context:
         self._check(self._l.cdx_batch_autotune(self.h, dt, theta, C.byref(params), C.byref(b), C.byref(g), C.byref(ms)),
                     "cdx_batch_autotune")
         return b.value, g.value, ms.value
+
+    def set_recording(self, on, max_sweeps=0):
+        self._check(self._l.cdx_set_recording(self.h, 1 if on else 0, max_sweeps), "cdx_set_recording")
+
+    def pj_adjoint(self, dt, params, g_v, g_p=None, mode=0, max_iter=100000, tol=1e-12):
+        """dL/dv_n, dL/dM^-1, dL/df_ext (stacked, nv) and dL/dmu (per contact) of the last PJ solve."""
+        a = AdjointParams(mode, max_iter, tol)
+        info = AdjointInfo()
+        gv = np.ascontiguousarray(g_v, dtype=np.float64)
+        gp = None if g_p is None else np.ascontiguousarray(g_p, dtype=np.float64).reshape(-1)
+        ov, om, of = np.zeros(self.nv), np.zeros(self.nv), np.zeros(self.nv)
+        omu = np.zeros(max(self.nc, 1))
+        self._check(self._l.cdx_pj_adjoint(self.h, dt, C.byref(params), C.byref(a), dptr(gv), dptr(gp), dptr(ov), dptr(om),
+                                           dptr(of), dptr(omu), C.byref(info)), "cdx_pj_adjoint")
+        return dict(v=ov, minv=om, fext=of, mu=omu[:self.nc], info=info)
 
     def velocities(self):
         out = np.empty(self.nv, dtype=np.float64)

@@ -258,7 +258,11 @@ def random_problem(nb=20, nc=60, seed=0, p_static=0.15, p_frictionless=0.2, p_po
     ia = np.empty(nc, np.int32); ib = np.empty(nc, np.int32)
     frame = np.empty((nc, 9)); aA = np.zeros((nc, 9)); aB = np.zeros((nc, 9))
     for c in range(nc):
-        a, b = rng.choice(nb, 2, replace=False)
+        # neighbours in index order keep the contact graph sparse (bounded degree), as in a pile
+        a = int(rng.integers(0, nb))
+        b = (a + 1 + int(rng.integers(0, min(3, nb - 1)))) % nb
+        if a > b:
+            a, b = b, a
         if rng.random() < p_static:
             if rng.random() < 0.5:
                 a = -1
@@ -315,3 +319,153 @@ def split_velocities(problems, v):
         out.append(v[o:o + p.nv])
         o += p.nv
     return out
+
+
+# --------------------------------------------------------------------------------------------
+# spheres and boxes in many independent scenes
+# --------------------------------------------------------------------------------------------
+
+SPHERE, BOX = 0, 1
+
+
+def shape_bodies(shape, dims, mass, q, v=None):
+    """Rigid spheres/boxes: sphere I = 2/5 m r^2, box I = m (h_j^2 + h_k^2) / 3
+    (computeUnitInertia, rigid_body_factory.cpp:59-91)."""
+    shape = np.asarray(shape)
+    dims = np.asarray(dims, dtype=np.float64).reshape(-1, 3)
+    mass = np.asarray(mass, dtype=np.float64)
+    nb = len(shape)
+    I = np.empty((nb, 3))
+    sph = shape == SPHERE
+    I[sph] = (0.4 * mass[sph] * dims[sph, 0] ** 2)[:, None]
+    h2 = dims ** 2
+    I[~sph, 0] = mass[~sph] * (h2[~sph, 1] + h2[~sph, 2]) / 3.0
+    I[~sph, 1] = mass[~sph] * (h2[~sph, 0] + h2[~sph, 2]) / 3.0
+    I[~sph, 2] = mass[~sph] * (h2[~sph, 0] + h2[~sph, 1]) / 3.0
+    minv = np.concatenate([np.repeat((1.0 / mass)[:, None], 3, axis=1), 1.0 / I], axis=1)
+    fext = np.zeros((nb, 6))
+    fext[:, :3] = mass[:, None] * GRAVITY[None, :]
+    vv = np.zeros((nb, 6)) if v is None else np.asarray(v, dtype=np.float64).reshape(nb, 6)
+    dof = np.full(nb, 6, dtype=np.int32)
+    fgyro = gyro_force(dof, minv.reshape(-1), vv.reshape(-1))
+    return dict(dof=dof, minv=minv.reshape(-1), v=vv.reshape(-1).copy(), fext=fext.reshape(-1), fgyro=fgyro,
+                q=np.ascontiguousarray(q, dtype=np.float64).reshape(-1))
+
+
+def shape_contacts(body_off, q, shape, dims, mu_body, walls, margin=0.0, friction=True):
+    """CPU contacts for spheres/boxes in independent scenes (libcdx_scene cdx_shape_contacts)."""
+    L = native.scene_lib()
+    body_off = np.ascontiguousarray(body_off, np.int32)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    shape = np.ascontiguousarray(shape, np.int32)
+    dims = np.ascontiguousarray(dims, dtype=np.float64).reshape(-1)
+    mu_body = np.ascontiguousarray(mu_body, dtype=np.float64)
+    wn, wd, wmu = (np.ascontiguousarray(x, dtype=np.float64) for x in walls)
+    ns = len(body_off) - 1
+    cap = max(64, 8 * len(shape))
+    coff = np.zeros(ns + 1, np.int32)
+    while True:
+        ia = np.empty(cap, np.int32); ib = np.empty(cap, np.int32)
+        mu = np.empty(cap); frame = np.empty((cap, 9)); aA = np.empty((cap, 9)); aB = np.empty((cap, 9))
+        pt = np.empty((cap, 3)); dep = np.empty(cap)
+        n = L.cdx_shape_contacts(ns, native.iptr(body_off), native.dptr(q), native.iptr(shape), native.dptr(dims),
+                                 native.dptr(mu_body), len(wd), native.dptr(wn.reshape(-1)), native.dptr(wd),
+                                 native.dptr(wmu), margin, cap, native.iptr(ia), native.iptr(ib), native.dptr(mu),
+                                 native.dptr(frame), native.dptr(aA), native.dptr(aB), native.dptr(pt), native.dptr(dep),
+                                 native.iptr(coff))
+        if n < 0:
+            raise RuntimeError("cdx_shape_contacts failed")
+        if n <= cap:
+            break
+        cap = n
+    if not friction:
+        mu = np.zeros(n)
+    return dict(body_a=ia[:n].copy(), body_b=ib[:n].copy(), mu=mu[:n].copy(), frame=frame[:n].copy(),
+                angA=aA[:n].copy(), angB=aB[:n].copy(), rest=np.zeros((n, 3)), vbias=np.zeros((n, 3)),
+                warm=np.zeros((n, 3)), point=pt[:n].copy(), depth=dep[:n].copy(), contact_off=coff)
+
+
+@dataclasses.dataclass
+class ShapeBatch:
+    """Independent scenes of spheres and boxes; scene s owns bodies [body_off[s], body_off[s+1])."""
+    body_off: np.ndarray
+    shape: np.ndarray
+    dims: np.ndarray  # [nb, 3] sphere radius in column 0, box half extents
+    mass: np.ndarray
+    mu_body: np.ndarray
+    q: np.ndarray  # [nb, 7]
+    v: np.ndarray  # [nb, 6]
+    walls: tuple
+    margin: float = 0.0
+
+    @property
+    def nscenes(self):
+        return len(self.body_off) - 1
+
+    def bodies(self):
+        return shape_bodies(self.shape, self.dims, self.mass, self.q, self.v)
+
+    def contacts(self, q=None, friction=True):
+        return shape_contacts(self.body_off, self.q if q is None else q, self.shape, self.dims, self.mu_body, self.walls,
+                              self.margin, friction)
+
+    def problem(self, friction=True):
+        c = self.contacts(friction=friction)
+        return make_problem(self.bodies(), c), c["contact_off"]
+
+
+def rsa(body_off, bound, lo, hi, seed):
+    """Random sequential placement of bounding spheres per scene (libcdx_scene cdx_scene_rsa)."""
+    body_off = np.ascontiguousarray(body_off, np.int32)
+    bound = np.ascontiguousarray(bound, np.float64)
+    pos = np.zeros((len(bound), 3))
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    native.scene_lib().cdx_scene_rsa(len(body_off) - 1, native.iptr(body_off), native.dptr(bound), native.dptr(lo),
+                                     native.dptr(hi), seed, native.dptr(pos))
+    return pos
+
+
+def batched_drop(nscenes=4096, seed=0, nmin=8, nmax=512, density=1000.0, mu=0.5):
+    """Independent scenes, bodies per scene log-uniform in [nmin, nmax], spheres and boxes 50/50
+    (box edges U[0.05, 0.2] m, sphere radius of the equal-volume sphere), placed at random without
+    overlap in a unit box and released from rest."""
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax + 1), nscenes))).astype(int)
+    counts = np.clip(counts, nmin, nmax)
+    body_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    nb = int(body_off[-1])
+    shape = np.where(rng.random(nb) < 0.5, SPHERE, BOX).astype(np.int32)
+    edges = rng.uniform(0.05, 0.2, (nb, 3))
+    vol = edges.prod(axis=1)
+    dims = 0.5 * edges
+    rad = (3.0 * vol / (4.0 * math.pi)) ** (1.0 / 3.0)
+    dims[shape == SPHERE] = rad[shape == SPHERE, None] * np.array([1.0, 0.0, 0.0])
+    mass = density * vol
+    bound = np.where(shape == SPHERE, rad, np.linalg.norm(dims, axis=1))
+    q = np.zeros((nb, 7))
+    q[:, :3] = rsa(body_off, bound, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), seed)
+    qq = rng.normal(size=(nb, 4))
+    q[:, 3:] = qq / np.linalg.norm(qq, axis=1, keepdims=True)
+    return ShapeBatch(body_off=body_off, shape=shape, dims=dims, mass=mass, mu_body=np.full(nb, mu), q=q,
+                      v=np.zeros((nb, 6)), walls=box_walls((0, 0, 0), (1.0, 1.0, 1.0), mu), margin=1e-3)
+
+
+def boxes_on_plane(nscenes=256, per_scene=64, seed=0, edge=0.1, mass=1.0, vstd=0.5, spacing=0.25):
+    """Boxes (edge 0.1 m, 1 kg) resting on a plane on an 8x8 grid, horizontal initial velocities
+    N(0, vstd^2), friction mu ~ U[0.2, 0.9] per box."""
+    rng = np.random.default_rng(seed)
+    nb = nscenes * per_scene
+    side = int(math.ceil(math.sqrt(per_scene)))
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    grid = np.stack([gx.ravel(), gy.ravel()], 1)[:per_scene] * spacing
+    q = np.zeros((nb, 7))
+    q[:, 0:2] = np.tile(grid, (nscenes, 1))
+    q[:, 2] = 0.5 * edge
+    q[:, 6] = 1.0
+    v = np.zeros((nb, 6))
+    v[:, :2] = rng.normal(scale=vstd, size=(nb, 2))
+    walls = (np.array([[0.0, 0.0, 1.0]]), np.array([0.0]), np.array([1.0]))
+    return ShapeBatch(body_off=(np.arange(nscenes + 1) * per_scene).astype(np.int32), shape=np.full(nb, BOX, np.int32),
+                      dims=np.full((nb, 3), 0.5 * edge), mass=np.full(nb, float(mass)), mu_body=rng.uniform(0.2, 0.9, nb),
+                      q=q, v=v, walls=walls, margin=1e-4)

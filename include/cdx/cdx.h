@@ -153,6 +153,46 @@ int cdx_batch_set_launch(cdx_ctx* ctx, int block, int groups);
 int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* block,
                        int* groups, double* best_ms);
 
+/* ---- Differentiation of the projected-Jacobi solve (no counterpart in the reference).
+ * The solve maps (v_n, M^-1, f_ext, mu) to (v_{n+1} = x*, p*) through the fixed point
+ * p* = Pi_mu(p* - R (W x* + b)), x* = u_free + M^-1 W^T p* (projected_jacobi.cpp:121-138).
+ * The backward of the last cdx_pj_solve on a context is computed with the contact geometry
+ * (W, c, e) held fixed:
+ *   CDX_ADJ_IMPLICIT : implicit differentiation of the fixed point -- one linear adjoint solve
+ *                      lambda = g_p + (I - A R) P^T lambda (A = W M^-1 W^T, P the generalized
+ *                      Jacobian of the Coulomb projection per active-set case), iterated with the
+ *                      same contact/body passes as the forward sweep;
+ *   CDX_ADJ_UNROLLED : reverse mode through the recorded forward sweeps (cdx_set_recording must
+ *                      have been on during the solve);
+ *   CDX_ADJ_AUTO     : implicit, falling back to unrolled when the adjoint iteration does not
+ *                      converge within max_iter (requires recording). */
+enum cdx_adjoint_mode { CDX_ADJ_IMPLICIT = 0, CDX_ADJ_UNROLLED = 1, CDX_ADJ_AUTO = 2 };
+
+typedef struct {
+    int mode;     /* enum cdx_adjoint_mode */
+    int max_iter; /* implicit: iteration cap */
+    double tol;   /* implicit: stop when ||lambda_k+1 - lambda_k|| <= tol ||lambda_k+1|| (checked every 10) */
+} cdx_adjoint_params;
+
+typedef struct {
+    int iters;       /* implicit iterations, or unrolled sweeps */
+    double residual; /* last relative change of the implicit iterate */
+    int mode_used;
+} cdx_adjoint_info;
+
+/* Record every sweep's (x, p) of subsequent cdx_pj_solve calls (standard loop) for the unrolled
+ * backward; `max_sweeps` bounds the memory (sweeps beyond it are not recorded and make the
+ * unrolled backward fail with CDX_BAD_ARG). */
+int cdx_set_recording(cdx_ctx* ctx, int enable, int max_sweeps);
+
+/* Vector-Jacobian product of the last cdx_pj_solve.  Inputs: g_v = dL/dv_{n+1} (nv, required),
+ * g_p = dL/dp (3*nc canonical impulses, or NULL).  Outputs (each may be NULL):
+ * gv_out = dL/dv_n (nv), gminv_out = dL/dM^-1 (nv, f_ext held fixed), gfext_out = dL/df_ext (nv),
+ * gmu_out = dL/dmu per contact (nc). */
+int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
+                   const double* g_v, const double* g_p, double* gv_out, double* gminv_out, double* gfext_out,
+                   double* gmu_out, cdx_adjoint_info* info);
+
 /* Kernel timing with HIP events on the context's stream (the analogue of the reference's
  * TimingManager scopes, src/misc/timings/TimingManager.hpp:16-58).  When enabled every
  * launch is bracketed by events; cdx_kernel_time reads back the call count and total ms for

@@ -46,9 +46,9 @@ CASES = [
     dict(nb=2, nc=1, seed=1, p_static=0.0, p_frictionless=1.0),
     dict(nb=20, nc=60, seed=2),
     dict(nb=50, nc=200, seed=3, p_point=0.3),
-    dict(nb=200, nc=900, seed=4, rest=0.4),
-    dict(nb=300, nc=1200, seed=5, p_frictionless=0.0, p_static=0.3),
-    dict(nb=1000, nc=3000, seed=6, warm=False),
+    dict(nb=200, nc=300, seed=4, rest=0.4),
+    dict(nb=300, nc=450, seed=5, p_frictionless=0.0, p_static=0.3),
+    dict(nb=1000, nc=1500, seed=6, warm=False),
 ]
 
 
@@ -69,7 +69,7 @@ def test_pj_fixed_sweeps(ctx, case, sweeps):
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("tol", [1e-4, 1e-8])
 def test_pj_to_tolerance_same_iterations(ctx, seed, tol):
-    prob = scenes.random_problem(nb=80, nc=300, seed=10 + seed)
+    prob = scenes.random_problem(nb=80, nc=120, seed=10 + seed)
     dt = 1e-3
     st, v_ref, imp_ref, res = oracle.pj_solve(prob, dt, oracle.params(tol_abs=tol, tol_rel=tol))
     v, imp, info = gpu_pj(ctx, prob, dt, tol_abs=tol, tol_rel=tol)
@@ -79,7 +79,7 @@ def test_pj_to_tolerance_same_iterations(ctx, seed, tol):
 
 
 def test_pj_warmstart_off_and_alpha(ctx):
-    prob = scenes.random_problem(nb=60, nc=200, seed=21)
+    prob = scenes.random_problem(nb=60, nc=90, seed=21)
     for alpha, warm in ((0.7, 0), (0.1, 1)):
         st, v_ref, imp_ref, res = oracle.pj_solve(prob, 1e-3, oracle.params(alpha=alpha, warmstart=warm, max_iter=50,
                                                                             tol_abs=0, tol_rel=0))
@@ -169,7 +169,7 @@ def test_pj_pile_parity(ctx):
 
 
 def test_pj_deterministic(ctx):
-    prob = scenes.random_problem(nb=500, nc=2000, seed=9)
+    prob = scenes.random_problem(nb=500, nc=750, seed=9)
     a = gpu_pj(ctx, prob, 1e-3, max_iter=64, tol_abs=0.0, tol_rel=0.0)
     b = gpu_pj(ctx, prob, 1e-3, max_iter=64, tol_abs=0.0, tol_rel=0.0)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
@@ -195,7 +195,7 @@ def test_moreau_update_matches_oracle(ctx):
 def test_pj_nesterov_matches_oracle(ctx, seed):
     """Nesterov loop (projected_jacobi.cpp:379-449): residual every sweep, restarts on residual
     growth / beta threshold / direction reversal, momentum disabled after the restart limit."""
-    prob = scenes.random_problem(nb=120, nc=400, seed=30 + seed)
+    prob = scenes.random_problem(nb=120, nc=180, seed=30 + seed)
     dt = 1e-3
     for kw in (dict(max_iter=40, tol_abs=0.0, tol_rel=0.0), dict(tol_abs=1e-8, tol_rel=1e-8),
                dict(tol_abs=1e-8, tol_rel=1e-8, nesterov_restart_limit=3), dict(tol_abs=1e-8, tol_rel=1e-8,
