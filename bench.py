@@ -7,9 +7,14 @@ problem from a cold start (no warm start) to the reference residual test with
 tol_abs = tol_rel = 1e-8 (checked every 10 sweeps), i.e. the hot path over one batch of input
 with everything resident in HBM.  contact-updates/s = contacts x sweeps / time.
 
-Prints one JSON line.  `roofline` is the dominant kernel (pj_contacts) from HIP events recorded
-on the solve's own stream during the timed region; `cpu_baseline` is the CPU oracle (a
+Prints one JSON line.  `value` comes from the timed steps (production path: chunks of sweeps
+replayed as HIP graphs).  `roofline` is the dominant kernel (pj_contacts): its average launch
+duration from HIP events recorded around every launch on the solve's own stream during one more
+step of the same workload right after the timed ones.  `cpu_baseline` is the CPU oracle (a
 restatement of the reference algorithm) timed on a bounded sample of the same workload.
+
+--copy-mb N also runs one device-to-device copy of N MiB, the calibration kernel for the
+FETCH_SIZE / WRITE_SIZE counters collected under rocprofv3 --pmc.
 """
 from __future__ import annotations
 
@@ -47,6 +52,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200_000)
     ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "
@@ -54,6 +60,21 @@ def main():
         sys.exit(2)
 
     from cardillocxx_amd import native, scenes
+
+    if args.copy_mb:
+        import ctypes
+
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = args.copy_mb << 20
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes))
+        hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes))
+        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # device to device
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        print(f"# calibration copy: {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
 
     t0 = time.time()
     pile = scenes.sphere_pile(n=args.spheres, settle_steps=args.settle, seed=0)
@@ -75,9 +96,6 @@ def main():
         info = ctx.pj_solve(dt, 1.0, p)
         print(f"# warmup solve: {info.iters} sweeps, residual {info.residual:.3e}", file=sys.stderr, flush=True)
 
-    ctx.set_profiling(True)
-    ctx.kernel_time("pj_contacts")
-    ctx.kernel_time("pj_bodies")
     ctx.synchronize()
     sweeps = 0
     residuals = []
@@ -88,15 +106,26 @@ def main():
         residuals.append(info.residual)
     ctx.synchronize()
     t2 = time.perf_counter()
+    elapsed = t2 - t1
+    cups = prob.nc * sweeps / elapsed
+
+    # Kernel durations: one more step of the same workload with HIP events bracketing every
+    # launch on the solve's stream (graph replay is off while events are recorded).
+    ctx.set_profiling(True)
+    for k in ("pj_contacts", "pj_bodies", "pj_residual"):
+        ctx.kernel_time(k)
+    tp0 = time.perf_counter()
+    pinfo = ctx.pj_solve(dt, 1.0, p)
+    ctx.synchronize()
+    tp = time.perf_counter() - tp0
     ncall_c, ms_c = ctx.kernel_time("pj_contacts")
     ncall_b, ms_b = ctx.kernel_time("pj_bodies")
     ctx.set_profiling(False)
-    elapsed = t2 - t1
-    cups = prob.nc * sweeps / elapsed
-    # average duration over the sweeps that did work (launches after convergence return at once
+    psweeps = max(pinfo.iters, 1)
+    # average over the sweeps that did work (launches enqueued after convergence return at once
     # and are included in the total, so this slightly overstates the duration)
-    avg_c = ms_c / 1e3 / max(sweeps, 1)
-    avg_b = ms_b / 1e3 / max(sweeps, 1)
+    avg_c = ms_c / 1e3 / psweeps
+    avg_b = ms_b / 1e3 / psweeps
     ach_c = bc / avg_c / 1e9
     ach_sweep = (bc + bb) * sweeps / elapsed / 1e9
 
@@ -140,7 +169,10 @@ def main():
                      "frac": ach_c / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bc, "avg_launch_us": avg_c * 1e6,
                      "pj_bodies_avg_us": avg_b * 1e6, "pj_bodies_GBs": bb / avg_b / 1e9 if avg_b > 0 else None,
-                     "whole_sweep_GBs": ach_sweep},
+                     "whole_sweep_GBs": ach_sweep, "whole_sweep_frac": ach_sweep / HBM_PEAK_GBS,
+                     "timing": "HIP events around every launch on the solve stream, one profiled step of the same "
+                               "workload run right after the timed steps",
+                     "profiled_step_ms": tp * 1e3, "profiled_sweeps": pinfo.iters},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

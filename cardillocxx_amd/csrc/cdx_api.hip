@@ -424,7 +424,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
 }
 
 int cdx_get_velocities(cdx_ctx* ctx, double* v_out) {
-    if (!ctx || !v_out) return CDX_BAD_ARG;
+    if (!ctx || (!v_out && ctx->nv > 0)) return CDX_BAD_ARG;
     if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_velocities: no solve yet");
     std::vector<double> hx((size_t)ctx->nb * kBodyStride);
     if (ctx->nb)
@@ -436,7 +436,7 @@ int cdx_get_velocities(cdx_ctx* ctx, double* v_out) {
 }
 
 int cdx_get_impulses(cdx_ctx* ctx, double* imp_out) {
-    if (!ctx || !imp_out) return CDX_BAD_ARG;
+    if (!ctx || (!imp_out && ctx->nc > 0)) return CDX_BAD_ARG;
     if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_impulses: no solve yet");
     const int nc = ctx->nc;
     std::vector<double> hp(3 * (size_t)nc);
@@ -448,7 +448,7 @@ int cdx_get_impulses(cdx_ctx* ctx, double* imp_out) {
 }
 
 int cdx_get_positions(cdx_ctx* ctx, double* q_out) {
-    if (!ctx || !q_out) return CDX_BAD_ARG;
+    if (!ctx || (!q_out && ctx->nb > 0)) return CDX_BAD_ARG;
     if (!ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_get_positions: no positions were set");
     std::vector<double> hq((size_t)ctx->nb * kBodyStride);
     if (ctx->nb)

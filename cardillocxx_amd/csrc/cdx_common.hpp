@@ -128,6 +128,9 @@ struct cdx_ctx {
     int* h_flag_dev = nullptr;
     bool solved = false;
 
+    // ---- replay whole chunks of sweeps as HIP graphs (off while profiling or recording)
+    bool graphs = true;
+
     // ---- profiling
     bool profiling = false;
     std::vector<cdx::PendingEvent> pending;
@@ -151,7 +154,7 @@ int read_residual(cdx_ctx* ctx, double* r);
 int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
 // Reduces ctx->d_partial[0:nparts) into the residual; mode 0 = PJ test (res <= 1),
 // mode 1 = Gauss-Seidel test (res < 1, NaN/Inf -> CDX_DIVERGED).
-int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode);
+int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence = 1);
 
 // Launch helper: brackets the launch with profiling events when enabled.
 #define CDX_LAUNCH(ctx, name, kernel, grid, block, ...)                                  \

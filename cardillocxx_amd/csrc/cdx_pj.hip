@@ -83,6 +83,99 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies(int nb, const int* __restr
     }
 }
 
+// Contact pass with 4 lanes per contact (lane k = row k of n, t1, t2; lane 3 idle): each lane
+// loads only its row's 9 Jacobian entries, so the kernel needs about half the registers of the
+// one-lane-per-contact form and runs at twice the occupancy.  The normal and tangential values
+// meet through shuffles inside the group of 4 for the Coulomb projection (every lane evaluates the
+// same scale factor), and the side impulses J^T dp are summed over the rows by an xor tree; lane
+// j of the group then writes 3 of the contact's 12 side-impulse doubles (96 contiguous bytes).
+constexpr int kLanesPerContact = 4;
+
+__global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, const int* __restrict__ ida,
+                                                         const int* __restrict__ idb, const double* __restrict__ J,
+                                                         const double* __restrict__ R, const double* __restrict__ bias,
+                                                         const double* __restrict__ mu, double* __restrict__ p,
+                                                         const double* __restrict__ x, double* __restrict__ imp,
+                                                         const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const int tid = xcd_tile(nblocks) * kBlock + threadIdx.x;
+    const int c = tid / kLanesPerContact, k = tid % kLanesPerContact;
+    const bool live = c < nc;
+    int a = -1, b = -1;
+    double muc = 0;
+    if (live) {
+        a = ida[c];
+        b = idb[c];
+        muc = mu[c];
+    }
+    const int nr = muc > 0 ? 3 : 1;
+    const bool row = live && (a >= 0 || b >= 0) && k < nr;
+    double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
+    double pold = 0, pt = 0;
+    if (row) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            d[m] = J[(size_t)(3 * k + m) * nc + c];
+            aA[m] = J[(size_t)(9 + 3 * k + m) * nc + c];
+            aB[m] = J[(size_t)(18 + 3 * k + m) * nc + c];
+        }
+        double ua[6], ub[6];
+        load_body(x, a, ua);
+        load_body(x, b, ub);
+        const double g = row_dot(d, aA, aB, ua, ub);
+        pold = p[(size_t)k * nc + c];
+        pt = pold - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+    }
+    // project() (projected_jacobi.cpp:100-119) on the group's three values
+    const int base = threadIdx.x & ~(kLanesPerContact - 1);
+    const double zn = __shfl(pt, base, 64);
+    const double z1 = __shfl(pt, base + 1, 64);
+    const double z2 = __shfl(pt, base + 2, 64);
+    const double pn = fmax(zn, 0.0);
+    double pnew = pn;
+    if (k > 0) {
+        pnew = k == 1 ? z1 : z2;
+        const double tn2 = z1 * z1 + z2 * z2;
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (muc * pn) / sqrt(tn2));
+            pnew = sc * pnew;
+        }
+    }
+    double dp = 0;
+    if (row) {
+        dp = pnew - pold;
+        p[(size_t)k * nc + c] = pnew;
+    }
+    // side impulses: sum over rows of d_k dp_k, aA_k dp_k, aB_k dp_k
+    double v9[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        v9[m] = d[m] * dp;
+        v9[3 + m] = aA[m] * dp;
+        v9[6 + m] = aB[m] * dp;
+    }
+#pragma unroll
+    for (int off = 1; off < kLanesPerContact; off <<= 1)
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
+    if (live && (a >= 0 || b >= 0)) {
+        // lane 0: -lin (side A), 1: -angA, 2: +lin (side B), 3: +angB
+        const bool sideB = k >= 2;
+        const int side_body = sideB ? b : a;
+        if (side_body >= 0) {
+            double o0, o1, o2;
+            if (k == 0) o0 = -v9[0], o1 = -v9[1], o2 = -v9[2];
+            else if (k == 1) o0 = -v9[3], o1 = -v9[4], o2 = -v9[5];
+            else if (k == 2) o0 = v9[0], o1 = v9[1], o2 = v9[2];
+            else o0 = v9[6], o1 = v9[7], o2 = v9[8];
+            double* dst = imp + (size_t)(2 * c + (sideB ? 1 : 0)) * 6 + (k & 1) * 3;
+            dst[0] = o0;
+            dst[1] = o1;
+            dst[2] = o2;
+        }
+    }
+}
+
 // Body pass with 8 lanes per body: lane l sums incidences l, l+8, ... of its body, the 8 partial
 // twists are combined by a fixed xor-shuffle tree (deterministic), and lanes 0..5 each update one
 // velocity component.  Eight times the lanes of one-lane-per-body keeps enough waves in flight to
@@ -147,8 +240,11 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
 }
 
 // mode 0: PJ (break when res <= 1); mode 1: GS (break when res < 1, NaN/Inf -> diverged).
+// it < 0: the sweep index comes from the device chunk counter ctrl[3] (graph-replayed chunks of
+// `cadence` sweeps whose residual check is the chunk's first sweep); the counter advances on
+// every call.
 __global__ void __launch_bounds__(kBlock) k_residual_finalize(int nparts, const double* __restrict__ partial, int nv,
-                                                              int it, int mode, int* __restrict__ ctrl,
+                                                              int it, int cadence, int mode, int* __restrict__ ctrl,
                                                               double* __restrict__ res, int* __restrict__ hflag) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
@@ -156,6 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_residual_finalize(int nparts, const 
     for (int i = threadIdx.x; i < nparts; i += kBlock) s += partial[i];
     s = block_sum(s, red);
     if (threadIdx.x == 0) {
+        if (it < 0) it = ctrl[3] * cadence;
+        ctrl[3] = ctrl[3] + 1;
         const double r = 1.0 / sqrt((double)nv) * sqrt(s);
         res[0] = r;
         int done = 0, status = 0;
@@ -187,9 +285,9 @@ int reset_control(cdx_ctx* ctx) {
     return CDX_OK;
 }
 
-int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode) {
+int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence) {
     CDX_LAUNCH(ctx, mode == 0 ? "pj_residual" : "gs_residual", k_residual_finalize, 1, kBlock, nparts, ctx->d_partial, ctx->nv,
-               it, mode, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev);
+               it, cadence, mode, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev);
     return hip_check(ctx, hipGetLastError(), "residual finalize launch");
 }
 
@@ -482,12 +580,23 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     const int nb = ctx->nb, nc = ctx->nc;
     const int check_every = prm->check_every;
     const int gb8 = grid_for(nb * kLanesPerBody);
+    const int gc4 = grid_for(nc * kLanesPerContact);
+    (void)gc;
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
     (void)gb;
     if (ctx->record) {
         CDX_TRY(ensure_alloc(ctx, &ctx->d_rec_x, (size_t)ctx->record_max * kBodyStride * nb, &ctx->rec_x_cap, "recorded x"));
         CDX_TRY(ensure_alloc(ctx, &ctx->d_rec_p, (size_t)ctx->record_max * 3 * nc, &ctx->rec_p_cap, "recorded p"));
     }
+    // one sweep; it < 0 inside a captured chunk (the residual kernel then reads the chunk counter)
+    auto launch_sweep = [&](int it, int check) -> int {
+        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                   ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
+        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));
+        return hip_check(ctx, hipGetLastError(), "pj sweep launch");
+    };
     auto sweep = [&](int it) -> int {
         if (ctx->record && it < ctx->record_max) {  // (x_k, p_k) before sweep k, for the unrolled backward
             CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rec_x + (size_t)it * kBodyStride * nb, ctx->d_x,
@@ -495,13 +604,35 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
             CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rec_p + (size_t)it * 3 * nc, ctx->d_p, sizeof(double) * 3 * (size_t)nc,
                                         hipMemcpyDeviceToDevice, ctx->stream));
         }
-        const int check = (it % check_every) == 0;
-        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                   ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
-        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                   ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
-        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0));
-        return hip_check(ctx, hipGetLastError(), "pj sweep launch");
+        return launch_sweep(it, (it % check_every) == 0);
     };
-    return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);
+    const int nchunks = prm->max_iter / check_every;
+    if (!ctx->graphs || ctx->profiling || ctx->record || nchunks < 2)
+        return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);
+    // Graph path: one chunk of `check_every` sweeps captured once and replayed, so the host
+    // submits one graph per chunk instead of two kernels per sweep.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int cst = CDX_OK;
+    for (int i = 0; i < check_every && cst == CDX_OK; ++i) cst = launch_sweep(-1, i == 0);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (cst) {
+        if (graph) hipGraphDestroy(graph);
+        return cst;
+    }
+    CDX_HIP(ctx, ce);
+    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    CDX_HIP(ctx, ce);
+    auto chunk = [&](int it) -> int {
+        if (it / check_every < nchunks) {
+            if (it % check_every != 0) return CDX_OK;  // the whole chunk goes with its first sweep
+            return hip_check(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+        }
+        return launch_sweep(it, (it % check_every) == 0);  // tail shorter than a chunk
+    };
+    const int st = run_sweeps(ctx, prm->max_iter, check_every, 4, chunk, iters_out);
+    hipGraphExecDestroy(exec);
+    return st;
 }

@@ -53,7 +53,8 @@ def test_batch_matches_standalone_oracle(ctx, launch):
 
 
 def test_batch_fixed_sweeps_equals_grid_solve(ctx):
-    """Same arithmetic as the grid-wide solve: a one-scene batch is bitwise identical to it."""
+    """Same per-contact arithmetic as the grid-wide solve (which sums the side impulses in a
+    different order): a one-scene batch agrees with it to rounding."""
     prob = scenes.random_problem(nb=300, nc=450, seed=5)
     p = native.default_params()
     p.max_iter, p.tol_abs, p.tol_rel = 77, 0.0, 0.0
@@ -64,7 +65,8 @@ def test_batch_fixed_sweeps_equals_grid_solve(ctx):
     upload_batch(ctx, [prob])
     iters, _ = ctx.batch_pj_solve(1e-3, 1.0, p)
     assert iters[0] == 77
-    assert np.array_equal(ctx.velocities(), v1) and np.array_equal(ctx.impulses(), i1)
+    assert np.abs(ctx.velocities() - v1).max() <= 1e-12 * np.abs(v1).max()
+    assert np.abs(ctx.impulses() - i1).max() <= 1e-12 * np.abs(i1).max()
 
 
 def test_batch_autotune_and_sphere_scenes(ctx):
@@ -93,8 +95,16 @@ def test_scene_validation(ctx):
     ctx.set_bodies(cat.dof, cat.minv, cat.v, cat.fext, cat.fgyro, cat.q)
     ctx.set_contacts(*cat.contact_arrays())
     bad = boff.copy()
-    bad[1] = bad[1] - 1 if bad[1] > 0 else 1
+    bad[-1] -= 1  # does not cover every body
     with pytest.raises(native.CdxError):
         ctx.set_scenes(bad, coff)
+    # a contact of scene 0 moved into scene 1's range
+    c0 = int(np.argmax(cat.body_a >= 0))
+    s0 = int(np.searchsorted(coff, c0, side="right") - 1)
+    moved = coff.copy()
+    moved[s0 + 1] = c0  # contact c0 now belongs to scene s0+1 but references a body of scene s0
+    if 0 < c0 < coff[-1]:
+        with pytest.raises(native.CdxError):
+            ctx.set_scenes(boff, np.sort(moved))
     with pytest.raises(native.CdxError):
         ctx.batch_set_launch(256, 3)

@@ -1,0 +1,75 @@
+"""Step-level backward with checkpointing: gradients of sum_s ||COM_s(final) - target_s||^2
+w.r.t. initial velocities, friction coefficients and masses, against central finite differences
+of the same rollout with the contact geometry of every step held at the nominal trajectory's, and
+implicit vs unrolled (FP64) per-step adjoints."""
+import numpy as np
+import pytest
+
+from cardillocxx_amd import diff, native, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(steps=12, k=5, mode=native.CDX_ADJ_IMPLICIT):
+    batch = scenes.boxes_on_plane(nscenes=2, per_scene=4, seed=3, vstd=0.5)
+    r = diff.Rollout(batch, steps=steps, dt=1e-3, checkpoint_every=k, tol=1e-12, adjoint_mode=mode, adjoint_tol=1e-13)
+    rng = np.random.default_rng(0)
+    target = rng.normal(scale=0.1, size=(batch.nscenes, 3))
+    return batch, r, target
+
+
+def loss_of(r, batch, target, v0, mu, mass):
+    pos = r.forward(v0, mu, mass)
+    return diff.com_loss(pos, mass, batch.body_off, target)
+
+
+def test_rollout_gradients_vs_finite_differences():
+    batch, r, target = setup()
+    v0, mu, mass = batch.v.reshape(-1).copy(), batch.mu_body.copy(), batch.mass.copy()
+    L, gpos, gm_direct = loss_of(r, batch, target, v0, mu, mass)
+    g = r.backward(gpos)
+    r.frozen = r.contacts_log  # FD of the same map: geometry of every step held fixed
+    rng = np.random.default_rng(1)
+    for name, base, grad in (("v0", v0, g.v0.reshape(-1)), ("mu", mu, g.mu), ("mass", mass, g.mass + gm_direct)):
+        for _ in range(2):
+            d = rng.normal(size=base.shape)
+            if name == "v0":
+                d.reshape(-1, 6)[:, 2:] = 0.0  # planar perturbation keeps the boxes on the plane
+            eps = 1e-6
+
+            def f(x):
+                args = dict(v0=v0, mu=mu, mass=mass)
+                args[name] = x
+                return loss_of(r, batch, target, args["v0"], args["mu"], args["mass"])[0]
+
+            fd = (f(base + eps * d) - f(base - eps * d)) / (2 * eps)
+            an = float(grad @ d)
+            assert an == pytest.approx(fd, rel=1e-4, abs=1e-9), (name, an, fd)
+
+
+def test_implicit_vs_unrolled_rollout_gradient():
+    batch, ri, target = setup(steps=8, k=4)
+    _, ru, _ = setup(steps=8, k=4, mode=native.CDX_ADJ_UNROLLED)
+    v0, mu, mass = batch.v.reshape(-1).copy(), batch.mu_body.copy(), batch.mass.copy()
+    _, gpos, _ = loss_of(ri, batch, target, v0, mu, mass)
+    gi = ri.backward(gpos)
+    loss_of(ru, batch, target, v0, mu, mass)
+    gu = ru.backward(gpos)
+    for a, b in ((gi.v0, gu.v0), (gi.mu, gu.mu), (gi.mass, gu.mass)):
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9 * max(1.0, np.abs(a).max()))
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    batch, r, target = setup(steps=6, k=3)
+    v0 = torch.tensor(batch.v.reshape(-1, 6), dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(batch.mu_body, dtype=torch.float64, requires_grad=True)
+    mass = torch.tensor(batch.mass, dtype=torch.float64, requires_grad=True)
+    L = diff.rollout_loss(r, v0, mu, mass, torch.tensor(target))
+    L.backward()
+    L2, gpos, gm_direct = loss_of(r, batch, target, batch.v.reshape(-1), batch.mu_body, batch.mass)
+    g = r.backward(gpos)
+    assert float(L) == pytest.approx(L2, rel=1e-12)
+    np.testing.assert_allclose(v0.grad.numpy(), g.v0, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(mu.grad.numpy(), g.mu, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(mass.grad.numpy(), g.mass + gm_direct, rtol=1e-10, atol=1e-14)
