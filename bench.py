@@ -9,8 +9,9 @@ with everything resident in HBM.  contact-updates/s = contacts x sweeps / time.
 
 Prints one JSON line.  `value` comes from the timed steps (production path: chunks of sweeps
 replayed as HIP graphs).  `roofline` is the dominant kernel (pj_contacts): its average launch
-duration from HIP events recorded around every launch on the solve's own stream during one more
-step of the same workload right after the timed ones.  `cpu_baseline` is the CPU oracle (a
+duration from start/stop HIP events carried in every dispatch on the solve's own stream during one
+more step of the same workload right after the timed ones; `traffic` is the HBM bytes per launch
+from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when it was taken on this workload.  `cpu_baseline` is the CPU oracle (a
 restatement of the reference algorithm) timed on a bounded sample of the same workload.
 
 --copy-mb N also runs one device-to-device copy of N MiB, the calibration kernel for the
@@ -33,6 +34,24 @@ HBM_PEAK_GBS = 8000.0
 B_CONTACT_DYN = 328  # algorithmic bytes per frictional contact-update, both sides dynamic
 B_CONTACT_STATIC = 256  # ... with a static side
 B_BODY = 128  # per body per sweep (6 u read + 6 u write + 4 M^-1)
+
+
+def committed_traffic(kernel, contacts):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if it was
+    taken on a problem with the same contact count (FETCH_SIZE/WRITE_SIZE passes scaled by a
+    256 MiB copy calibration); None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pj_contacts4_pile100k_hbm_traffic.txt")
+    try:
+        vals = {}
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] in ("traffic_bytes_per_launch", "traffic_contacts") and f[1] == kernel:
+                vals[f[0]] = int(f[2])
+        if vals.get("traffic_contacts") == contacts:
+            return vals["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
+    except (OSError, ValueError, KeyError):
+        pass
+    return None, None
 
 
 def algorithmic_bytes(prob):
@@ -109,8 +128,8 @@ def main():
     elapsed = t2 - t1
     cups = prob.nc * sweeps / elapsed
 
-    # Kernel durations: one more step of the same workload with HIP events bracketing every
-    # launch on the solve's stream (graph replay is off while events are recorded).
+    # Kernel durations: one more step of the same workload with start/stop HIP events carried in
+    # every dispatch on the solve's stream (graph replay is off while events are recorded).
     ctx.set_profiling(True)
     for k in ("pj_contacts", "pj_bodies", "pj_residual"):
         ctx.kernel_time(k)
@@ -128,6 +147,8 @@ def main():
     avg_b = ms_b / 1e3 / psweeps
     ach_c = bc / avg_c / 1e9
     ach_sweep = (bc + bb) * sweeps / elapsed / 1e9
+
+    traffic, traffic_src = committed_traffic("k_pj_contacts4", prob.nc)
 
     v = ctx.velocities()
     imp = ctx.impulses()
@@ -166,12 +187,12 @@ def main():
                    "spheres": prob.nb, "contacts": prob.nc, "rows": int(info.nrows), "sweeps_per_step": sweeps / args.steps,
                    "residual": float(np.max(residuals)), "alpha": 0.3, "cone_invariant": cone_ok},
         "roofline": {"bound": "hbm", "kernel": "pj_contacts", "achieved": ach_c, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_c / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_c / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": bc, "avg_launch_us": avg_c * 1e6,
                      "pj_bodies_avg_us": avg_b * 1e6, "pj_bodies_GBs": bb / avg_b / 1e9 if avg_b > 0 else None,
                      "whole_sweep_GBs": ach_sweep, "whole_sweep_frac": ach_sweep / HBM_PEAK_GBS,
-                     "timing": "HIP events around every launch on the solve stream, one profiled step of the same "
-                               "workload run right after the timed steps",
+                     "timing": "start/stop HIP events carried in every kernel dispatch (hipExtLaunchKernelGGL) on the "
+                               "solve stream, one profiled step of the same workload run right after the timed steps",
                      "profiled_step_ms": tp * 1e3, "profiled_sweeps": pinfo.iters},
         "cpu_baseline": cpu,
     }

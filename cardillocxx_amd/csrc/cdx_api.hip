@@ -23,7 +23,7 @@ int hip_check(cdx_ctx* ctx, hipError_t e, const char* what) {
 
 int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
-void prof_begin(cdx_ctx* ctx, const char* /*name*/, hipEvent_t* start) {
+static hipEvent_t pool_event(cdx_ctx* ctx) {
     hipEvent_t e;
     if (!ctx->event_pool.empty()) {
         e = ctx->event_pool.back();
@@ -31,20 +31,16 @@ void prof_begin(cdx_ctx* ctx, const char* /*name*/, hipEvent_t* start) {
     } else {
         hipEventCreate(&e);
     }
-    hipEventRecord(e, ctx->stream);
-    *start = e;
+    return e;
 }
 
-void prof_end(cdx_ctx* ctx, const char* name, hipEvent_t start) {
-    hipEvent_t e;
-    if (!ctx->event_pool.empty()) {
-        e = ctx->event_pool.back();
-        ctx->event_pool.pop_back();
-    } else {
-        hipEventCreate(&e);
-    }
-    hipEventRecord(e, ctx->stream);
-    ctx->pending.push_back(PendingEvent{name, start, e});
+void prof_events(cdx_ctx* ctx, hipEvent_t* start, hipEvent_t* stop) {
+    *start = pool_event(ctx);
+    *stop = pool_event(ctx);
+}
+
+void prof_push(cdx_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t stop) {
+    ctx->pending.push_back(PendingEvent{name, start, stop});
     if (ctx->pending.size() >= 4096) harvest_events(ctx);
 }
 

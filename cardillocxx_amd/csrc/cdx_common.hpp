@@ -12,6 +12,7 @@
 //              (inc_ptr/inc_idx) so the body pass needs no atomics and sums in a fixed order.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -144,8 +145,8 @@ int set_err(cdx_ctx* ctx, int code, const std::string& msg);
 int hip_check(cdx_ctx* ctx, hipError_t e, const char* what);
 template <typename T>
 int ensure_alloc(cdx_ctx* ctx, T** ptr, size_t count, size_t* cap_or_null, const char* what);
-void prof_begin(cdx_ctx* ctx, const char* name, hipEvent_t* start);
-void prof_end(cdx_ctx* ctx, const char* name, hipEvent_t start);
+void prof_events(cdx_ctx* ctx, hipEvent_t* start, hipEvent_t* stop);
+void prof_push(cdx_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t stop);
 int harvest_events(cdx_ctx* ctx);
 int build_incidence(cdx_ctx* ctx);
 int grid_for(int n);
@@ -157,12 +158,20 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
 int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence = 1);
 
 // Launch helper: brackets the launch with profiling events when enabled.
-#define CDX_LAUNCH(ctx, name, kernel, grid, block, ...)                                  \
-    do {                                                                                  \
-        hipEvent_t cdx_ev_start__ = nullptr;                                              \
-        if ((ctx)->profiling) ::cdx::prof_begin((ctx), (name), &cdx_ev_start__);          \
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__); \
-        if ((ctx)->profiling) ::cdx::prof_end((ctx), (name), cdx_ev_start__);             \
+// With profiling on, the start/stop events travel in the kernel's own dispatch
+// (hipExtLaunchKernelGGL), so the measured interval is the kernel's execution, not the gap
+// between separate marker packets.
+#define CDX_LAUNCH(ctx, name, kernel, grid, block, ...)                                                  \
+    do {                                                                                                  \
+        if ((ctx)->profiling) {                                                                           \
+            hipEvent_t cdx_ev_start__, cdx_ev_stop__;                                                     \
+            ::cdx::prof_events((ctx), &cdx_ev_start__, &cdx_ev_stop__);                                   \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, cdx_ev_start__,     \
+                                  cdx_ev_stop__, 0, __VA_ARGS__);                                         \
+            ::cdx::prof_push((ctx), (name), cdx_ev_start__, cdx_ev_stop__);                               \
+        } else {                                                                                          \
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__);           \
+        }                                                                                                 \
     } while (0)
 
 #define CDX_TRY(expr)               \
