@@ -1,18 +1,29 @@
-"""Benchmark: FP64 projected-Jacobi contact updates per second on one MI355X.
+"""Benchmark: FP64 projected-Jacobi contact updates per second on one MI355X, plus the batched and
+differentiable forms.
 
-Workload (the large single scene): 100k spheres, radius log-uniform in [0.02, 0.08] m, density
-1000 kg/m^3, random packing settled in a 4x4x8 m box, mu ~ U[0.3, 0.8] per body (min-combined),
-~4e5 frictional contacts, h = 5e-4 s.  One step = one full projected-Jacobi solve of that contact
-problem from a cold start (no warm start) to the reference residual test with
-tol_abs = tol_rel = 1e-8 (checked every 10 sweeps), i.e. the hot path over one batch of input
-with everything resident in HBM.  contact-updates/s = contacts x sweeps / time.
+Headline (`value`): the large single scene -- 100k spheres, radius log-uniform in [0.02, 0.08] m,
+density 1000 kg/m^3, random packing settled in a 4x4x8 m box, mu ~ U[0.3, 0.8] per body
+(min-combined), ~4e5 frictional contacts, h = 5e-4 s.  One step = one full projected-Jacobi solve
+of that contact problem from a cold start (no warm start) to the reference residual test with
+tol_abs = tol_rel = 1e-8 (checked every 10 sweeps), everything resident in HBM.
+contact-updates/s = contacts x sweeps / time.  Timed with chunks of sweeps replayed as HIP graphs
+(the production path).
 
-Prints one JSON line.  `value` comes from the timed steps (production path: chunks of sweeps
-replayed as HIP graphs).  `roofline` is the dominant kernel (pj_contacts): its average launch
-duration from start/stop HIP events carried in every dispatch on the solve's own stream during one
-more step of the same workload right after the timed ones; `traffic` is the HBM bytes per launch
-from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when it was taken on this workload.  `cpu_baseline` is the CPU oracle (a
-restatement of the reference algorithm) timed on a bounded sample of the same workload.
+`roofline`: the dominant kernel (pj_contacts).  Its average duration comes from start/stop HIP
+events carried in every dispatch on the solve's own stream during one more step of the same
+workload right after the timed ones; `traffic` is the HBM bytes per launch from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary when it was taken on this workload; `copy_GBs` is a
+device-to-device copy measured in the same run (a second denominator).
+
+`cpu_baseline`: the CPU oracle (a C restatement of the reference PJ sweep: single-thread SpMV as
+the reference pins Eigen to one thread, OpenMP projection) on a bounded sample of the same
+workload, and a fully threaded variant as an upper bound.
+
+`colored_gs`: the coloured Gauss-Seidel solve of the same pile to the same tolerance (residual
+every sweep).  `batched`: 4096 independent sphere/box scenes dropped into a unit box, one batched
+solve (one launch) per Moreau step, against the same scenes solved one by one through the plain
+solver on sampled steps.  `differentiable`: 256 scenes of 64 boxes on a plane, forward and
+checkpointed backward of a 100-step rollout, against the same scenes rolled out one by one.
 
 --copy-mb N also runs one device-to-device copy of N MiB, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters collected under rocprofv3 --pmc.
@@ -20,6 +31,7 @@ FETCH_SIZE / WRITE_SIZE counters collected under rocprofv3 --pmc.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -34,6 +46,10 @@ HBM_PEAK_GBS = 8000.0
 B_CONTACT_DYN = 328  # algorithmic bytes per frictional contact-update, both sides dynamic
 B_CONTACT_STATIC = 256  # ... with a static side
 B_BODY = 128  # per body per sweep (6 u read + 6 u write + 4 M^-1)
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
 def committed_traffic(kernel, contacts):
@@ -60,46 +76,69 @@ def algorithmic_bytes(prob):
     return both * B_CONTACT_DYN + one * B_CONTACT_STATIC, prob.nb * B_BODY
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spheres", type=int, default=100_000)
-    ap.add_argument("--settle", type=int, default=600)
-    ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--max-iter", type=int, default=200_000)
-    ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "
-              "shard across GPUs here -- see DESIGN.md for how it would.", file=sys.stderr)
-        sys.exit(2)
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    from cardillocxx_amd import native, scenes
 
-    if args.copy_mb:
-        import ctypes
+class Hip:
+    """The few HIP runtime calls the copy measurements need."""
 
-        hip = ctypes.CDLL("libamdhip64.so")
-        nbytes = args.copy_mb << 20
-        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-        hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes))
-        hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes))
-        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
-        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # device to device
-        hip.hipDeviceSynchronize()
-        hip.hipFree(src)
-        hip.hipFree(dst)
-        print(f"# calibration copy: {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
+    def __init__(self):
+        self.l = ctypes.CDLL("libamdhip64.so")
 
+    def malloc(self, n):
+        p = ctypes.c_void_p()
+        if self.l.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p
+
+    def copy_once(self, mb):
+        n = mb << 20
+        src, dst = self.malloc(n), self.malloc(n)
+        self.l.hipMemset(src, 1, ctypes.c_size_t(n))
+        self.l.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)  # device to device
+        self.l.hipDeviceSynchronize()
+        self.l.hipFree(src)
+        self.l.hipFree(dst)
+
+    def copy_bandwidth(self, mb=1024, reps=10):
+        """Device-to-device copy rate in GB/s, counting the bytes read and the bytes written."""
+        n = mb << 20
+        src, dst = self.malloc(n), self.malloc(n)
+        self.l.hipMemset(src, 1, ctypes.c_size_t(n))
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        self.l.hipEventCreate(ctypes.byref(e0))
+        self.l.hipEventCreate(ctypes.byref(e1))
+        self.l.hipMemcpyAsync(dst, src, ctypes.c_size_t(n), 3, None)
+        self.l.hipEventRecord(e0, None)
+        for _ in range(reps):
+            self.l.hipMemcpyAsync(dst, src, ctypes.c_size_t(n), 3, None)
+        self.l.hipEventRecord(e1, None)
+        self.l.hipEventSynchronize(e1)
+        ms = ctypes.c_float(0)
+        self.l.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        self.l.hipEventDestroy(e0)
+        self.l.hipEventDestroy(e1)
+        self.l.hipFree(src)
+        self.l.hipFree(dst)
+        return 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+
+
+# --------------------------------------------------------------------------------------------
+# large single scene (headline)
+# --------------------------------------------------------------------------------------------
+
+def pile_leg(args, native, scenes, metrics):
     t0 = time.time()
     pile = scenes.sphere_pile(n=args.spheres, settle_steps=args.settle, seed=0)
     prob = pile.problem()
-    t_scene = time.time() - t0
-    print(f"# scene: {prob.nb} bodies, {prob.nc} contacts, generated in {t_scene:.1f}s", file=sys.stderr, flush=True)
+    log(f"scene: {prob.nb} bodies, {prob.nc} contacts, generated in {time.time() - t0:.1f}s")
     dt = 5e-4
     bc, bb = algorithmic_bytes(prob)
 
@@ -113,7 +152,7 @@ def main():
 
     for _ in range(args.warmup):
         info = ctx.pj_solve(dt, 1.0, p)
-        print(f"# warmup solve: {info.iters} sweeps, residual {info.residual:.3e}", file=sys.stderr, flush=True)
+        log(f"warmup solve: {info.iters} sweeps, residual {info.residual:.3e}")
 
     ctx.synchronize()
     sweeps = 0
@@ -124,9 +163,13 @@ def main():
         sweeps += info.iters
         residuals.append(info.residual)
     ctx.synchronize()
-    t2 = time.perf_counter()
-    elapsed = t2 - t1
+    elapsed = time.perf_counter() - t1
     cups = prob.nc * sweeps / elapsed
+    log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s, {cups:.3e} contact-updates/s")
+    v = ctx.velocities()
+    imp = ctx.impulses()
+    cone_ok = bool((imp[:, 0] >= 0).all() and (np.linalg.norm(imp[:, 1:], axis=1) <= prob.mu * imp[:, 0] * (1 + 1e-12) + 1e-300).all())
+    comp_pj = metrics.complementarity(prob, v, imp)
 
     # Kernel durations: one more step of the same workload with start/stop HIP events carried in
     # every dispatch on the solve's stream (graph replay is off while events are recorded).
@@ -140,19 +183,32 @@ def main():
     ncall_c, ms_c = ctx.kernel_time("pj_contacts")
     ncall_b, ms_b = ctx.kernel_time("pj_bodies")
     ctx.set_profiling(False)
+    # launches enqueued after convergence return at once; count the sweeps that did work
     psweeps = max(pinfo.iters, 1)
-    # average over the sweeps that did work (launches enqueued after convergence return at once
-    # and are included in the total, so this slightly overstates the duration)
     avg_c = ms_c / 1e3 / psweeps
     avg_b = ms_b / 1e3 / psweeps
     ach_c = bc / avg_c / 1e9
     ach_sweep = (bc + bb) * sweeps / elapsed / 1e9
-
     traffic, traffic_src = committed_traffic("k_pj_contacts4", prob.nc)
 
-    v = ctx.velocities()
-    imp = ctx.impulses()
-    cone_ok = bool((imp[:, 0] >= 0).all() and (np.linalg.norm(imp[:, 1:], axis=1) <= prob.mu * imp[:, 0] * (1 + 1e-12) + 1e-300).all())
+    # coloured Gauss-Seidel on the same problem and tolerance (residual every sweep)
+    pg = native.default_params()
+    pg.tol_abs = pg.tol_rel = args.tol
+    pg.max_iter = args.gs_max_iter
+    pg.warmstart = 0
+    ctx.synchronize()
+    tg0 = time.perf_counter()
+    ginfo = ctx.gs_colored_solve(dt, 1.0, pg, raise_on_divergence=False)
+    ctx.synchronize()
+    tg = time.perf_counter() - tg0
+    vg = ctx.velocities()
+    ig = ctx.impulses()
+    ig[:, 1:] *= -1.0  # the Gauss-Seidel solver stores tangents with the opposite sign
+    gs = {"value": prob.nc * ginfo.iters / tg, "unit": "contact-updates/s", "sweeps": ginfo.iters, "colors": ginfo.ncolors,
+          "residual": ginfo.residual, "seconds_to_solution": tg, "pj_seconds_to_solution": elapsed / args.steps,
+          "complementarity": metrics.complementarity(prob, vg, ig), "pj_complementarity": comp_pj,
+          "kinetic_energy": float(0.5 * (vg ** 2 / prob.minv).sum()), "pj_kinetic_energy": float(0.5 * (v ** 2 / prob.minv).sum())}
+    log(f"colored GS: {ginfo.iters} sweeps, {ginfo.ncolors} colours, {tg:.3f}s")
 
     cpu = None
     if not args.no_cpu:
@@ -164,10 +220,19 @@ def main():
         tc = time.perf_counter()
         st, v_cpu, imp_cpu, res = oracle.pj_solve(prob, dt, prm, nthreads=threads)
         tcpu = time.perf_counter() - tc
+        tc = time.perf_counter()
+        st2, _, _, res2 = oracle.pj_solve(prob, dt, prm, nthreads=-threads)
+        tcpu2 = time.perf_counter() - tc
         cpu = {"value": prob.nc * res.iters / tcpu, "unit": "contact-updates/s", "cores": threads, "kind": "port",
-               "sample": f"oracle PJ (C restatement of projected_jacobi.cpp, OpenMP projection, single-thread "
-                         f"SpMV) on the same {prob.nc}-contact problem, {res.iters} sweeps incl. W assembly, "
-                         f"{tcpu:.1f}s"}
+               "cpu": cpu_model(),
+               "sample": f"oracle PJ (C restatement of projected_jacobi.cpp: single-thread SpMV as the reference pins "
+                         f"Eigen to one thread, OpenMP projection on {threads} threads) on the same {prob.nc}-contact "
+                         f"problem, {res.iters} cold-start sweeps incl. W assembly, {tcpu:.1f}s",
+               "fully_threaded": {"value": prob.nc * res2.iters / tcpu2, "cores": threads,
+                                  "sample": f"same sweeps with both SpMVs and the vector updates on {threads} OpenMP "
+                                            f"threads (upper bound, not the reference's structure), {tcpu2:.1f}s"},
+               "gpu_over_cpu": cups / (prob.nc * res.iters / tcpu)}
+        log(f"cpu baseline: {cpu['value']:.3e} (threaded {cpu['fully_threaded']['value']:.3e}) contact-updates/s")
 
     out = {
         "metric": "FP64 contact-updates/s at res<=1e-8 (projected Jacobi)",
@@ -185,7 +250,8 @@ def main():
         "config": {"workload": "sphere pile: 100k spheres r~logU[0.02,0.08] m, rho=1000, 4x4x8 m box, "
                                "mu~U[0.3,0.8], h=5e-4 s, cold-start PJ to tol 1e-8",
                    "spheres": prob.nb, "contacts": prob.nc, "rows": int(info.nrows), "sweeps_per_step": sweeps / args.steps,
-                   "residual": float(np.max(residuals)), "alpha": 0.3, "cone_invariant": cone_ok},
+                   "residual": float(np.max(residuals)), "alpha": 0.3, "cone_invariant": cone_ok,
+                   "complementarity": comp_pj},
         "roofline": {"bound": "hbm", "kernel": "pj_contacts", "achieved": ach_c, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_c / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": bc, "avg_launch_us": avg_c * 1e6,
@@ -195,7 +261,241 @@ def main():
                                "solve stream, one profiled step of the same workload run right after the timed steps",
                      "profiled_step_ms": tp * 1e3, "profiled_sweeps": pinfo.iters},
         "cpu_baseline": cpu,
+        "colored_gs": gs,
     }
+    del ctx
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# batched: thousands of independent scenes, one solve launch per step
+# --------------------------------------------------------------------------------------------
+
+def _scene_problem(bd, con, warm, v, q, b0, b1, c0, c1):
+    """Scene [b0, b1) / contacts [c0, c1) of the stacked state as a standalone problem."""
+    def rebase(ids):
+        ids = ids[c0:c1].copy()
+        ids[ids >= 0] -= b0
+        return ids
+    bodies = (np.full(b1 - b0, 6, np.int32), bd["minv"][6 * b0:6 * b1], v[6 * b0:6 * b1], bd["fext"][6 * b0:6 * b1],
+              None, q[7 * b0:7 * b1])
+    contacts = (rebase(con["body_a"]), rebase(con["body_b"]), con["mu"][c0:c1], con["frame"][c0:c1], con["angA"][c0:c1],
+                con["angB"][c0:c1], con["rest"][c0:c1], con["vbias"][c0:c1], warm[c0:c1])
+    return bodies, contacts
+
+
+def batched_leg(args, native, scenes):
+    t0 = time.time()
+    b = scenes.batched_drop(nscenes=args.batch_scenes, seed=0)
+    bd = b.bodies()
+    ns, nb = b.nscenes, len(b.mass)
+    log(f"batched: {ns} scenes, {nb} bodies, generated in {time.time() - t0:.1f}s")
+    dt = 1e-3
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = args.tol
+    p.max_iter = args.batch_max_iter
+    p.warmstart = 1
+    ctx = native.Context(0)
+    ctx.set_bodies(bd["dof"], bd["minv"], bd["v"], bd["fext"], None, bd["q"])
+    one = native.Context(0)
+    rng = np.random.default_rng(1)
+    steps = args.batch_steps
+    sample_steps = set(int(s) for s in np.linspace(steps // 5, steps - 1, args.obo_steps).round()) if args.obo_steps else set()
+    q = bd["q"].copy()
+    v = bd["v"].copy()
+    prev = None
+    t_cpu = t_dev = t_solve = 0.0
+    cu = 0  # contact-updates (sum over scenes of contacts x sweeps)
+    contacts_total = 0
+    capped = 0
+    tune = None
+    obo_time = batch_time_sampled = 0.0
+    obo_solves = 0
+    worst_dv = 0.0
+    per_scene_max = 0
+    for s in range(steps):
+        ta = time.perf_counter()
+        con = b.contacts(q=q.reshape(-1, 7))
+        nc = len(con["body_a"])
+        warm = scenes.match_warmstart(prev, con) if prev is not None and nc else np.zeros((nc, 3))
+        tb = time.perf_counter()
+        ctx.set_contacts(con["body_a"], con["body_b"], con["mu"], con["frame"], con["angA"], con["angB"], con["rest"],
+                         con["vbias"], warm)
+        ctx.set_scenes(b.body_off, con["contact_off"])
+        if s == args.tune_step:
+            tune = ctx.batch_autotune(dt, 1.0, p)
+            log(f"batched: autotuned (block, groups) = ({tune[0]}, {tune[1]}), {tune[2]:.3f} ms at step {s}")
+        ctx.synchronize()
+        ts = time.perf_counter()
+        iters, res = ctx.batch_pj_solve(dt, 1.0, p)
+        t_step_solve = time.perf_counter() - ts
+        t_solve += t_step_solve
+        coff = con["contact_off"]
+        per = np.diff(coff)
+        per_scene_max = max(per_scene_max, int(per.max()) if len(per) else 0)
+        cu += int((per.astype(np.int64) * iters).sum())
+        contacts_total += nc
+        capped += int((iters >= p.max_iter).sum())
+        if s in sample_steps:
+            # the same scenes one by one through the plain solver, on a sample of scenes
+            vb = ctx.velocities()
+            pick = rng.choice(ns, size=min(args.obo_scenes, ns), replace=False)
+            t_one = 0.0
+            for k in pick:
+                b0, b1 = int(b.body_off[k]), int(b.body_off[k + 1])
+                c0, c1 = int(coff[k]), int(coff[k + 1])
+                bodies, contacts = _scene_problem(bd, con, warm, v, q, b0, b1, c0, c1)
+                one.set_bodies(*bodies)
+                one.set_contacts(*contacts)
+                one.synchronize()
+                t1 = time.perf_counter()
+                one.pj_solve(dt, 1.0, p)
+                t_one += time.perf_counter() - t1
+                v1 = one.velocities()
+                scale = max(np.abs(vb[6 * b0:6 * b1]).max(), 1e-30)
+                worst_dv = max(worst_dv, float(np.abs(v1 - vb[6 * b0:6 * b1]).max() / scale))
+            obo_time += t_one * ns / len(pick)
+            batch_time_sampled += t_step_solve
+            obo_solves += len(pick)
+        ctx.moreau_update(dt, 1.0)
+        q = ctx.positions()
+        v = ctx.velocities()
+        imp = ctx.impulses()
+        tc = time.perf_counter()
+        prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
+        t_cpu += tb - ta
+        t_dev += tc - tb
+        if s % 50 == 0 or s == steps - 1:
+            log(f"batched step {s}: {nc} contacts (max {int(per.max()) if len(per) else 0}/scene), "
+                f"max sweeps {int(iters.max())}, solve {t_step_solve * 1e3:.2f} ms")
+    out = {
+        "metric": "batched env-steps/s (scenes x Moreau steps per second)",
+        "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box, "
+                    f"h=1e-3 s, {steps} steps, warm-started PJ to tol {args.tol:g} per scene, one batched solve launch per step",
+        "steps": steps, "scenes": ns, "bodies": nb,
+        "value": ns * steps / t_dev, "unit": "env-steps/s",
+        "definition": "device path per step: contact upload, batched solve, position update, state download",
+        "solve_only_env_steps_per_s": ns * steps / t_solve,
+        "end_to_end_env_steps_per_s": ns * steps / (t_cpu + t_dev),
+        "cpu_contact_generation_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
+        "contact_updates_per_s": cu / t_solve, "mean_contacts_per_step": contacts_total / steps,
+        "max_contacts_per_scene": per_scene_max, "scene_solves_at_iteration_cap": capped,
+        "launch": {"block": tune[0], "groups": tune[1]} if tune else None,
+        "one_by_one": None,
+    }
+    if obo_solves:
+        out["one_by_one"] = {
+            "sampled_steps": sorted(sample_steps), "sampled_scene_solves": obo_solves,
+            "estimated_solve_s_per_step": obo_time / len(sample_steps),
+            "batched_solve_s_per_step": batch_time_sampled / len(sample_steps),
+            "speedup": obo_time / max(batch_time_sampled, 1e-12),
+            "max_rel_velocity_difference": worst_dv,
+            "method": "at each sampled step a random sample of scenes is solved alone with cdx_pj_solve (same inputs, "
+                      "same per-scene tolerance); solve time scaled to all scenes, compared with the batched solve of "
+                      "that step"}
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# differentiable: forward + checkpointed backward of batched rollouts
+# --------------------------------------------------------------------------------------------
+
+def diff_leg(args, native, scenes):
+    from cardillocxx_amd import diff
+
+    batch = scenes.boxes_on_plane(nscenes=args.diff_scenes, per_scene=64, seed=0)
+    rng = np.random.default_rng(0)
+    target = rng.normal(scale=0.1, size=(batch.nscenes, 3)) + np.array([0.875, 0.875, 0.05])
+    kw = dict(steps=args.diff_steps, dt=1e-3, checkpoint_every=10, tol=args.tol, adjoint_tol=args.adjoint_tol,
+              max_iter=args.batch_max_iter)
+
+    def run(b, tgt, batched):
+        r = diff.Rollout(b, batched=batched, **kw)
+        v0, mu, mass = b.v.reshape(-1).copy(), b.mu_body.copy(), b.mass.copy()
+        t0 = time.perf_counter()
+        pos = r.forward(v0, mu, mass)
+        t1 = time.perf_counter()
+        loss, gpos, gm = diff.com_loss(pos, mass, b.body_off, tgt)
+        g = r.backward(gpos)
+        t2 = time.perf_counter()
+        return r, loss, g, gm, t1 - t0, t2 - t1
+
+    r, loss, g, gm, tf, tb = run(batch, target, True)
+    ns, steps = batch.nscenes, args.diff_steps
+    log(f"differentiable: {ns} scenes, forward {tf:.2f}s, backward {tb:.2f}s, "
+        f"{r.stats['forward_solves']} solves, {r.stats['adjoint_iters']} adjoint iterations")
+    pick = rng.choice(ns, size=min(args.diff_obo, ns), replace=False)
+    t_one = 0.0
+    worst = 0.0
+    for k in pick:
+        k = int(k)
+        _, _, g1, gm1, tf1, tb1 = run(batch.scene(k), target[k][None, :], False)
+        t_one += tf1 + tb1
+        b0, b1 = batch.body_off[k], batch.body_off[k + 1]
+        for a, c in ((g1.v0, g.v0[b0:b1]), (g1.mu, g.mu[b0:b1]), (g1.mass + gm1, (g.mass + gm)[b0:b1])):
+            worst = max(worst, float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-30)))
+    t_obo = t_one * ns / len(pick)
+    log(f"differentiable one by one: {len(pick)} scenes in {t_one:.2f}s")
+    return {
+        "metric": "fwd+bwd env-steps/s (scenes x steps through forward and checkpointed backward per second)",
+        "workload": f"{ns} scenes x 64 boxes (edge 0.1 m, 1 kg) on a plane, v0~N(0,0.5^2), {steps} steps at h=1e-3 s, "
+                    f"loss sum_s ||COM_s(final) - target_s||^2, grads wrt v0, mu, mass; implicit FP64 adjoint, "
+                    f"checkpoint every 10 steps, PJ tol {args.tol:g} per scene, adjoint tol {args.adjoint_tol:g}",
+        "value": ns * steps / (tf + tb), "unit": "env-steps/s",
+        "forward_s": tf, "backward_s": tb, "loss": loss,
+        "forward_solves": r.stats["forward_solves"], "adjoint_iterations": r.stats["adjoint_iters"],
+        "one_by_one": {"sampled_scenes": [int(k) for k in pick], "estimated_s": t_obo, "speedup": t_obo / (tf + tb),
+                       "max_rel_gradient_difference": worst,
+                       "method": "sampled scenes rolled out alone (forward + backward through the grid-wide solve and "
+                                 "adjoint), time scaled to all scenes"},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spheres", type=int, default=100_000)
+    ap.add_argument("--settle", type=int, default=600)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iter", type=int, default=200_000)
+    ap.add_argument("--gs-max-iter", type=int, default=50_000)
+    ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
+    ap.add_argument("--pile-only", action="store_true", help="skip the batched and differentiable legs")
+    ap.add_argument("--batch-scenes", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-max-iter", type=int, default=20_000)
+    ap.add_argument("--tune-step", type=int, default=100)
+    ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
+    ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")
+    ap.add_argument("--diff-scenes", type=int, default=256)
+    ap.add_argument("--diff-steps", type=int, default=100)
+    ap.add_argument("--diff-obo", type=int, default=4, help="scenes rolled out one by one")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "
+              "shard across GPUs here -- see DESIGN.md for how it would.", file=sys.stderr)
+        sys.exit(2)
+
+    from cardillocxx_amd import metrics, native, scenes
+
+    hip = Hip()
+    if args.copy_mb:
+        hip.copy_once(args.copy_mb)
+        log(f"calibration copy: {args.copy_mb} MiB device to device")
+    copy_gbs = hip.copy_bandwidth()
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+
+    out = pile_leg(args, native, scenes, metrics)
+    out["roofline"]["copy_GBs"] = copy_gbs
+    out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
+    if not args.pile_only:
+        out["batched"] = batched_leg(args, native, scenes)
+        out["differentiable"] = diff_leg(args, native, scenes)
     print(json.dumps(out), flush=True)
 
 

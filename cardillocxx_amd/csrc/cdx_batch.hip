@@ -41,8 +41,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     const int b0 = s >= 0 ? boff[s] : 0, b1 = s >= 0 ? boff[s + 1] : 0;
     const int c0 = s >= 0 ? coff[s] : 0, c1 = s >= 0 ? coff[s + 1] : 0;
     const bool rows = s >= 0 && srows[s] > 0;
-    // setup: R, b, p0 and the side impulses of W^T p0, then x0
-    if (rows)
+    // setup: R, b, p0 and the side impulses of W^T p0, then x0 (also for a scene without rows,
+    // whose static-static contacts get R = 0 and p = 0, as in the grid-wide setup)
+    if (s >= 0)
         for (int c = c0 + lt; c < c1; c += gsize)
             pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
     __syncthreads();
@@ -178,6 +179,8 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (ctx->nscenes < 1 || ctx->scene_boff.back() != ctx->nb || ctx->scene_coff.back() != ctx->nc)
         return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
     ctx->solved = false;
+    ctx->pj_last = false;
+    ctx->record_valid = false;
     CDX_TRY(build_incidence(ctx));
     CDX_TRY(launch_batch(ctx, dt, prm, ctx->batch_block, ctx->batch_groups));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -185,7 +188,9 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (iters) CDX_HIP(ctx, hipMemcpy(iters, ctx->d_scene_i + 5 * ns + 2, sizeof(int) * ns, hipMemcpyDeviceToHost));
     if (residuals) CDX_HIP(ctx, hipMemcpy(residuals, ctx->d_scene_res, sizeof(double) * ns, hipMemcpyDeviceToHost));
     ctx->solved = true;
-    ctx->pj_last = false;
+    ctx->pj_last = true;  // same converged state as cdx_pj_solve: the implicit adjoint applies
+    ctx->record_valid = false;
+    ctx->recorded = 0;
     return CDX_OK;
 }
 

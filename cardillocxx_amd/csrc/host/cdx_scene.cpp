@@ -435,46 +435,53 @@ int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const 
                        double* angB, double* point, double* depth, int* contact_off) {
     if (nscenes < 1 || !body_off || nw < 0) return -1;
     const int nb = body_off[nscenes];
-    std::vector<Body> bodies(nb);
-    for (int i = 0; i < nb; ++i) {
+    for (int i = 0; i < nb; ++i)
         if (shape[i] != 0 && shape[i] != 1) return -1;
-        bodies[i] = make_body(q + 7 * (size_t)i, shape[i], dims + 3 * (size_t)i);
-    }
-    std::vector<std::vector<RawContact>> per(nb);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int s = 0; s < nscenes; ++s) {
-        const int b0 = body_off[s], b1 = body_off[s + 1];
-        if (b1 <= b0) continue;
-        double bmax = 0;
-        std::vector<double> cen(3 * (size_t)(b1 - b0));
-        for (int i = b0; i < b1; ++i) {
-            bmax = std::max(bmax, bodies[i].bound);
-            cen[3 * (i - b0)] = bodies[i].c.x, cen[3 * (i - b0) + 1] = bodies[i].c.y, cen[3 * (i - b0) + 2] = bodies[i].c.z;
+    std::vector<Body> bodies(nb);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nb; ++i) bodies[i] = make_body(q + 7 * (size_t)i, shape[i], dims + 3 * (size_t)i);
+    // contacts of scene s, in body order (walls of body a, then its pairs (a, b > a) by b)
+    std::vector<std::vector<RawContact>> per(nscenes);
+#pragma omp parallel
+    {
+        std::vector<int> nbrs;
+        std::vector<double> cen;
+#pragma omp for schedule(dynamic, 1)
+        for (int s = 0; s < nscenes; ++s) {
+            const int b0 = body_off[s], b1 = body_off[s + 1];
+            if (b1 <= b0) continue;
+            double bmax = 0;
+            cen.resize(3 * (size_t)(b1 - b0));
+            for (int i = b0; i < b1; ++i) {
+                bmax = std::max(bmax, bodies[i].bound);
+                cen[3 * (i - b0)] = bodies[i].c.x, cen[3 * (i - b0) + 1] = bodies[i].c.y, cen[3 * (i - b0) + 2] = bodies[i].c.z;
+            }
+            Grid g;
+            g.build(b1 - b0, cen.data(), 3, std::max(2 * bmax + margin, 1e-9));
+            auto& out = per[s];
+            for (int a = b0; a < b1; ++a) {
+                for (int w = 0; w < nw; ++w)
+                    wall_contacts(a, w, bodies[a], V3{wall_n[3 * w], wall_n[3 * w + 1], wall_n[3 * w + 2]}, wall_d[w], margin,
+                                  out);
+                nbrs.clear();
+                g.around(&cen[3 * (a - b0)], [&](int j) {
+                    if (j + b0 > a) nbrs.push_back(j + b0);
+                });
+                std::sort(nbrs.begin(), nbrs.end());
+                for (int b : nbrs) pair_contacts(a, b, bodies[a], bodies[b], margin, out);
+            }
         }
-        Grid g;
-        g.build(b1 - b0, cen.data(), 3, std::max(2 * bmax + margin, 1e-9));
-        for (int a = b0; a < b1; ++a) {
-            auto& out = per[a];
-            for (int w = 0; w < nw; ++w)
-                wall_contacts(a, w, bodies[a], V3{wall_n[3 * w], wall_n[3 * w + 1], wall_n[3 * w + 2]}, wall_d[w], margin, out);
-            std::vector<int> nbrs;
-            g.around(&cen[3 * (a - b0)], [&](int j) {
-                if (j + b0 > a) nbrs.push_back(j + b0);
-            });
-            std::sort(nbrs.begin(), nbrs.end());
-            for (int b : nbrs) pair_contacts(a, b, bodies[a], bodies[b], margin, out);
-        }
     }
-    std::vector<int> off(nb + 1, 0);
-    for (int a = 0; a < nb; ++a) off[a + 1] = off[a] + (int)per[a].size();
-    const int total = off[nb];
+    std::vector<int> off(nscenes + 1, 0);
+    for (int s = 0; s < nscenes; ++s) off[s + 1] = off[s] + (int)per[s].size();
+    const int total = off[nscenes];
     if (contact_off)
-        for (int s = 0; s <= nscenes; ++s) contact_off[s] = off[body_off[s]];
+        for (int s = 0; s <= nscenes; ++s) contact_off[s] = off[s];
     if (total > cap) return total;
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int a = 0; a < nb; ++a)
-        for (size_t k = 0; k < per[a].size(); ++k)
-            fill_contact(per[a][k], q, mu_body, mu_wall, off[a] + (int)k, body_a, body_b, mu, frame, angA, angB, point, depth);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < nscenes; ++s)
+        for (size_t k = 0; k < per[s].size(); ++k)
+            fill_contact(per[s][k], q, mu_body, mu_wall, off[s] + (int)k, body_a, body_b, mu, frame, angA, angB, point, depth);
     return total;
 }
 

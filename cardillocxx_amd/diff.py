@@ -34,7 +34,7 @@ class Grads:
 class Rollout:
     def __init__(self, batch: ShapeBatch, steps: int, dt: float = 1e-3, checkpoint_every: int = 10, tol: float = 1e-10,
                  adjoint_mode: int = native.CDX_ADJ_IMPLICIT, adjoint_tol: float = 1e-11, device: int = 0, ctx=None,
-                 max_iter: int = 200000):
+                 max_iter: int = 200000, batched: bool = False):
         self.batch = batch
         self.steps = steps
         self.dt = dt
@@ -46,6 +46,12 @@ class Rollout:
         self.prm.warmstart = 1
         self.adjoint_mode = adjoint_mode
         self.adjoint_tol = adjoint_tol
+        # batched: every forward solve is one cdx_batch_pj_solve over the batch's scenes, each
+        # scene converging on its own residual exactly as a standalone solve of that scene would
+        # (implicit adjoints only: the batched solve records no sweeps)
+        self.batched = batched and batch.nscenes > 1
+        if self.batched and adjoint_mode != native.CDX_ADJ_IMPLICIT:
+            raise ValueError("Rollout: batched solves support the implicit adjoint only")
         self.frozen = None  # optional per-step contact dicts to reuse (geometry held fixed)
         self.stats = dict(forward_solves=0, sweeps=0, adjoint_iters=0)
 
@@ -76,8 +82,13 @@ class Rollout:
         ctx.set_bodies(np.full(len(q) // 7, 6, np.int32), minv, v, fext, fgyro, q)
         ctx.set_contacts(con["body_a"], con["body_b"], mu, con["frame"], con["angA"], con["angB"], con["rest"],
                          con["vbias"], con["warm"] if warm is None else warm)
-        info = ctx.pj_solve(self.dt, 1.0, prm or self.prm)
         self.stats["forward_solves"] += 1
+        if self.batched and prm is None:
+            ctx.set_scenes(self.batch.body_off, con["contact_off"])
+            iters, _ = ctx.batch_pj_solve(self.dt, 1.0, self.prm)
+            self.stats["sweeps"] += int(iters.max()) if len(iters) else 0
+            return None
+        info = ctx.pj_solve(self.dt, 1.0, prm or self.prm)
         self.stats["sweeps"] += info.iters
         return info
 

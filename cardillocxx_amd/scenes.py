@@ -87,15 +87,19 @@ def sphere_bodies(radius, mass, q, v=None):
 
 
 def gyro_force(dof, minv, v):
-    """f_gyro = -omega x (I omega) (world.cpp:152-173)."""
+    """f_gyro = -omega x (I omega) (world.cpp:152-173) for every 6-dof body; zero elsewhere."""
+    dof = np.asarray(dof)
+    v = np.asarray(v, dtype=np.float64)
+    minv = np.asarray(minv, dtype=np.float64)
     out = np.zeros_like(v)
-    off = 0
-    for d in dof:
-        if d == 6:
-            w = v[off + 3: off + 6]
-            I = np.where(minv[off + 3: off + 6] > 0, 1.0 / np.where(minv[off + 3: off + 6] > 0, minv[off + 3: off + 6], 1), 0)
-            out[off + 3: off + 6] = -np.cross(w, I * w)
-        off += d
+    off = np.concatenate([[0], np.cumsum(dof)[:-1]]).astype(np.int64)
+    rb = off[dof == 6]
+    if len(rb):
+        idx = rb[:, None] + np.arange(3, 6)[None, :]
+        w = v[idx]
+        mi = minv[idx]
+        I = np.where(mi > 0, 1.0 / np.where(mi > 0, mi, 1.0), 0.0)
+        out[idx] = -np.cross(w, I * w)
     return out
 
 
@@ -404,6 +408,13 @@ class ShapeBatch:
 
     def bodies(self):
         return shape_bodies(self.shape, self.dims, self.mass, self.q, self.v)
+
+    def scene(self, s):
+        """Scene s on its own (a one-scene batch with the same walls and margin)."""
+        b0, b1 = int(self.body_off[s]), int(self.body_off[s + 1])
+        return ShapeBatch(body_off=np.array([0, b1 - b0], np.int32), shape=self.shape[b0:b1].copy(),
+                          dims=self.dims[b0:b1].copy(), mass=self.mass[b0:b1].copy(), mu_body=self.mu_body[b0:b1].copy(),
+                          q=self.q[b0:b1].copy(), v=self.v[b0:b1].copy(), walls=self.walls, margin=self.margin)
 
     def contacts(self, q=None, friction=True):
         return shape_contacts(self.body_off, self.q if q is None else q, self.shape, self.dims, self.mu_body, self.walls,

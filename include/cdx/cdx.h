@@ -185,7 +185,10 @@ typedef struct {
  * unrolled backward fail with CDX_BAD_ARG). */
 int cdx_set_recording(cdx_ctx* ctx, int enable, int max_sweeps);
 
-/* Vector-Jacobian product of the last cdx_pj_solve.  Inputs: g_v = dL/dv_{n+1} (nv, required),
+/* Vector-Jacobian product of the last cdx_pj_solve or cdx_batch_pj_solve (the batched solve
+ * leaves the same converged state; it records nothing, so it supports CDX_ADJ_IMPLICIT only --
+ * scenes are independent, so one implicit iteration over the whole batch is every scene's
+ * adjoint at once).  Inputs: g_v = dL/dv_{n+1} (nv, required),
  * g_p = dL/dp (3*nc canonical impulses, or NULL).  Outputs (each may be NULL):
  * gv_out = dL/dv_n (nv), gminv_out = dL/dM^-1 (nv, f_ext held fixed), gfext_out = dL/df_ext (nv),
  * gmu_out = dL/dmu per contact (nc). */

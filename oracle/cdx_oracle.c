@@ -189,6 +189,48 @@ static void spmv_t_acc(const orc_W* W, const double* p, double* y) {
     }
 }
 
+/* Threaded variants for the fully threaded CPU baseline (OpenMP over rows of W and of W^T);
+ * the reference itself runs its SpMV on one thread (examples/example_main.hpp:33). */
+static void spmv_mt(const orc_W* W, const double* x, double* y, int nthreads) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int r = 0; r < W->nrows; ++r) {
+        double s = 0;
+        for (int k = W->rowptr[r]; k < W->rowptr[r + 1]; ++k) s += W->val[k] * x[W->col[k]];
+        y[r] = s;
+    }
+}
+
+/* W^T as CSR over the nv columns of W: Wt_ptr[j]..Wt_ptr[j+1] lists (row, entry index). */
+typedef struct { int* ptr; int* row; int* ent; } orc_Wt;
+
+static void build_Wt(const orc_W* W, orc_Wt* T) {
+    const int nv = W->nv, nnz = W->rowptr[W->nrows];
+    T->ptr = (int*)calloc((size_t)nv + 1, sizeof(int));
+    T->row = (int*)malloc(sizeof(int) * (size_t)(nnz + 1));
+    T->ent = (int*)malloc(sizeof(int) * (size_t)(nnz + 1));
+    for (int k = 0; k < nnz; ++k) T->ptr[W->col[k] + 1]++;
+    for (int j = 0; j < nv; ++j) T->ptr[j + 1] += T->ptr[j];
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(nv + 1));
+    memcpy(fill, T->ptr, sizeof(int) * (size_t)nv);
+    for (int r = 0; r < W->nrows; ++r)
+        for (int k = W->rowptr[r]; k < W->rowptr[r + 1]; ++k) {
+            const int at = fill[W->col[k]]++;
+            T->row[at] = r;
+            T->ent[at] = k;
+        }
+    free(fill);
+}
+
+/* y = W^T p, one column of W per iteration. */
+static void spmv_t_mt(const orc_W* W, const orc_Wt* T, const double* p, double* y, int nthreads) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int j = 0; j < W->nv; ++j) {
+        double s = 0;
+        for (int k = T->ptr[j]; k < T->ptr[j + 1]; ++k) s += W->val[T->ent[k]] * p[T->row[k]];
+        y[j] = s;
+    }
+}
+
 /* D_i = sum_j W_ij^2 Minv_j (projected_jacobi.cpp:57-69; pgs_assembler.cpp:226-238). */
 static void delassus_diag(const orc_W* W, const double* minv, double* D) {
     for (int r = 0; r < W->nrows; ++r) {
@@ -242,6 +284,11 @@ static void fill_mdiag(int nv, const double* minv, double* M) {
 int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double theta, const orc_params* prm,
                  int nthreads, double* v_out, double* imp_out, orc_result* res) {
     (void)theta; /* theta only enters spring/damper rows and S for bilateral scenes */
+    /* nthreads < 0: fully threaded sweep (both products and the vector updates on |nthreads|
+     * threads) -- a timing variant only; the sums of W^T p then run column-wise. */
+    const int full = nthreads < 0;
+    if (full) nthreads = -nthreads;
+    if (nthreads < 1) nthreads = 1;
     orc_W W;
     int st = orc_build_W(B, C, &W);
     if (st) { orc_free_W(&W); return st; }
@@ -300,9 +347,28 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
 
     orc_W RW = W;
     RW.val = RWv;
+    orc_Wt WT = {0, 0, 0};
+    if (full) build_Wt(&W, &WT);
 
 #define PJ_SWEEP()                                                          \
     do {                                                                    \
+        if (full) {                                                         \
+            _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
+            for (int j = 0; j < nv; ++j) xp[j] = x[j];                      \
+            spmv_mt(&RW, x, dl, nthreads);                                  \
+            _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
+            for (int r = 0; r < nr; ++r) {                                  \
+                pp[r] = p[r];                                               \
+                p[r] -= dl[r] + bias[r];                                    \
+            }                                                               \
+            pj_project(&W, p, nthreads);                                    \
+            _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
+            for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];              \
+            spmv_t_mt(&W, &WT, dl, rhs, nthreads);                          \
+            _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
+            for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];             \
+            break;                                                          \
+        }                                                                   \
         memcpy(xp, x, sizeof(double) * (size_t)nv);                         \
         memcpy(pp, p, sizeof(double) * (size_t)nr);                         \
         spmv(&RW, x, dl);                                                   \
@@ -395,6 +461,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     }
     memcpy(v_out, x, sizeof(double) * (size_t)nv);
     free(R); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(M);
+    free(WT.ptr); free(WT.row); free(WT.ent);
     orc_free_W(&W);
     return ORC_OK;
 }

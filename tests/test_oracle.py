@@ -221,6 +221,21 @@ def test_invariants_and_momentum(seed):
         np.testing.assert_allclose(mom(v), mom(prob.v), atol=1e-12 * np.abs(prob.v).max() * 10)
 
 
+@pytest.mark.parametrize("nesterov", [0, 1])
+def test_threaded_sweep_variants_agree(nesterov):
+    """The CPU-baseline timing variants (OpenMP projection; fully threaded products over W and a
+    transposed W) visit every sum in the same order as the serial sweep, so all three agree
+    bit for bit, also through the Nesterov loop and its restarts."""
+    prob = scenes.random_problem(nb=400, nc=1200, seed=7, warm=True)
+    prm = oracle.params(max_iter=120, tol_abs=1e-9, tol_rel=1e-9, nesterov=nesterov)
+    ref = oracle.pj_solve(prob, 1e-3, prm, nthreads=1)
+    for nt in (4, -1, -4):
+        out = oracle.pj_solve(prob, 1e-3, prm, nthreads=nt)
+        assert out[0] == ref[0] == 0 and out[3].iters == ref[3].iters
+        np.testing.assert_array_equal(out[1], ref[1])
+        np.testing.assert_array_equal(out[2], ref[2])
+
+
 def test_residual_cadence_and_tolerance():
     """The PJ loop checks the residual only when iter % 10 == 0, so a converged solve ends at
     1, 11, 21, ... sweeps; with tol 0 it runs to max_iter.  PGS checks every sweep."""
