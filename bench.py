@@ -292,11 +292,12 @@ def batched_leg(args, native, scenes):
     log(f"batched: {ns} scenes, {nb} bodies, generated in {time.time() - t0:.1f}s")
     dt = 1e-3
     p = native.default_params()
-    p.tol_abs = p.tol_rel = args.tol
+    p.tol_abs = p.tol_rel = args.batch_tol
     p.max_iter = args.batch_max_iter
     p.warmstart = 1
     ctx = native.Context(0)
     ctx.set_bodies(bd["dof"], bd["minv"], bd["v"], bd["fext"], None, bd["q"])
+    scene_of = np.repeat(np.arange(b.nscenes), np.diff(b.body_off))
     one = native.Context(0)
     rng = np.random.default_rng(1)
     steps = args.batch_steps
@@ -313,6 +314,7 @@ def batched_leg(args, native, scenes):
     obo_solves = 0
     worst_dv = 0.0
     per_scene_max = 0
+    blown = 0  # scene-steps whose velocities or positions went non-finite (reset to rest)
     for s in range(steps):
         ta = time.perf_counter()
         con = b.contacts(q=q.reshape(-1, 7))
@@ -358,9 +360,25 @@ def batched_leg(args, native, scenes):
             batch_time_sampled += t_step_solve
             obo_solves += len(pick)
         ctx.moreau_update(dt, 1.0)
+        q_prev = q
         q = ctx.positions()
         v = ctx.velocities()
         imp = ctx.impulses()
+        bad_body = ~(np.isfinite(v.reshape(-1, 6)).all(axis=1) & np.isfinite(q.reshape(-1, 7)).all(axis=1))
+        if bad_body.any():
+            # a diverged scene (projected Jacobi is not guaranteed to converge): put it back at
+            # rest where it was and keep the other scenes going
+            bad = np.unique(scene_of[bad_body])
+            blown += len(bad)
+            mask = np.isin(scene_of, bad)
+            q = q.reshape(-1, 7).copy()
+            v = v.reshape(-1, 6).copy()
+            q[mask] = q_prev.reshape(-1, 7)[mask]
+            v[mask] = 0.0
+            q, v = q.reshape(-1), v.reshape(-1)
+            imp = np.where(np.isfinite(imp), imp, 0.0)
+            ctx.set_bodies(bd["dof"], bd["minv"], v, bd["fext"], None, q)
+            log(f"batched step {s}: scenes {bad.tolist()[:8]} went non-finite; reset to rest")
         tc = time.perf_counter()
         prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         t_cpu += tb - ta
@@ -371,7 +389,8 @@ def batched_leg(args, native, scenes):
     out = {
         "metric": "batched env-steps/s (scenes x Moreau steps per second)",
         "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box, "
-                    f"h=1e-3 s, {steps} steps, warm-started PJ to tol {args.tol:g} per scene, one batched solve launch per step",
+                    f"h=1e-3 s, {steps} steps, warm-started PJ to tol {args.batch_tol:g} per scene (reference default), "
+                    f"one batched solve launch per step",
         "steps": steps, "scenes": ns, "bodies": nb,
         "value": ns * steps / t_dev, "unit": "env-steps/s",
         "definition": "device path per step: contact upload, batched solve, position update, state download",
@@ -380,6 +399,7 @@ def batched_leg(args, native, scenes):
         "cpu_contact_generation_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
         "contact_updates_per_s": cu / t_solve, "mean_contacts_per_step": contacts_total / steps,
         "max_contacts_per_scene": per_scene_max, "scene_solves_at_iteration_cap": capped,
+        "scene_steps_reset_after_divergence": blown,
         "launch": {"block": tune[0], "groups": tune[1]} if tune else None,
         "one_by_one": None,
     }
@@ -468,6 +488,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-max-iter", type=int, default=20_000)
+    ap.add_argument("--batch-tol", type=float, default=1e-5, help="batched PJ tolerance (the reference default)")
     ap.add_argument("--tune-step", type=int, default=100)
     ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
     ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")

@@ -116,4 +116,41 @@ int run_sweeps(cdx_ctx* ctx, int max_iter, int chunk, int window, SweepFn&& swee
 }
 
 
+// Like run_sweeps, with every full chunk of `chunk` sweeps captured once as a HIP graph and
+// replayed: the host submits one graph per chunk instead of several kernels per sweep.
+// launch(it, pos) enqueues sweep `it` at position `pos` of its chunk; inside the capture it is
+// called with it = -1, so kernels that need the sweep index read the device sweep counter
+// (k_residual_finalize).  The tail shorter than a chunk is launched directly.  Without graphs
+// (profiling events, recording, or a single chunk) every sweep is launched directly.
+template <typename LaunchFn>
+int run_sweeps_graphed(cdx_ctx* ctx, int max_iter, int chunk, bool allow_graph, LaunchFn&& launch, int* iters_out) {
+    const int nchunks = max_iter / chunk;
+    if (!allow_graph || !ctx->graphs || ctx->profiling || nchunks < 2)
+        return run_sweeps(ctx, max_iter, chunk, 4, [&](int it) { return launch(it, it % chunk); }, iters_out);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int cst = CDX_OK;
+    for (int i = 0; i < chunk && cst == CDX_OK; ++i) cst = launch(-1, i);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (cst) {
+        if (graph) hipGraphDestroy(graph);
+        return cst;
+    }
+    CDX_HIP(ctx, ce);
+    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    CDX_HIP(ctx, ce);
+    auto step = [&](int it) -> int {
+        if (it / chunk < nchunks) {
+            if (it % chunk != 0) return CDX_OK;  // the whole chunk goes with its first sweep
+            return hip_check(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+        }
+        return launch(it, it % chunk);
+    };
+    const int st = run_sweeps(ctx, max_iter, chunk, 4, step, iters_out);
+    hipGraphExecDestroy(exec);
+    return st;
+}
+
 }  // namespace cdx

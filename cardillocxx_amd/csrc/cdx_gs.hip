@@ -324,6 +324,8 @@ __global__ void k_gs_finish_contacts(int nc, double* __restrict__ lam) {
 
 using namespace cdx;
 
+constexpr int kGsChunk = 8;
+
 extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm,
                                     cdx_solve_info* info) {
     (void)theta;
@@ -365,7 +367,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
                ctx->d_imp, ctx->d_x, ctx->d_uprev);
     CDX_HIP(ctx, hipGetLastError());
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb, &ctx->partial_cap, "residual partials"));
-    auto sweep = [&](int it) -> int {
+    auto sweep = [&](int it, int) -> int {
         for (int c = 0; c < ctx->ncolors; ++c) {
             const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
             if (n == 0) continue;
@@ -379,7 +381,8 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
         return hip_check(ctx, hipGetLastError(), "gs sweep launch");
     };
     int iters = 0;
-    const int st = run_sweeps(ctx, prm->max_iter, 4, 4, sweep, &iters);
+    // residual every sweep (projected_gauss_seidel.cpp:130-140); chunks of 8 sweeps as graphs
+    const int st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_bodies, gb, kBlock, nb, ctx->d_ufree, ctx->d_x);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_contacts, grid_for(nc), kBlock, nc, ctx->d_p);
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -606,33 +606,8 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
         }
         return launch_sweep(it, (it % check_every) == 0);
     };
-    const int nchunks = prm->max_iter / check_every;
-    if (!ctx->graphs || ctx->profiling || ctx->record || nchunks < 2)
-        return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);
-    // Graph path: one chunk of `check_every` sweeps captured once and replayed, so the host
-    // submits one graph per chunk instead of two kernels per sweep.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int cst = CDX_OK;
-    for (int i = 0; i < check_every && cst == CDX_OK; ++i) cst = launch_sweep(-1, i == 0);
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-    if (cst) {
-        if (graph) hipGraphDestroy(graph);
-        return cst;
-    }
-    CDX_HIP(ctx, ce);
-    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    CDX_HIP(ctx, ce);
-    auto chunk = [&](int it) -> int {
-        if (it / check_every < nchunks) {
-            if (it % check_every != 0) return CDX_OK;  // the whole chunk goes with its first sweep
-            return hip_check(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
-        }
-        return launch_sweep(it, (it % check_every) == 0);  // tail shorter than a chunk
-    };
-    const int st = run_sweeps(ctx, prm->max_iter, check_every, 4, chunk, iters_out);
-    hipGraphExecDestroy(exec);
-    return st;
+    if (ctx->record) return run_sweeps(ctx, prm->max_iter, check_every, 4, sweep, iters_out);
+    // one chunk = the sweeps between two residual checks
+    return run_sweeps_graphed(ctx, prm->max_iter, check_every, true,
+                              [&](int it, int pos) { return launch_sweep(it, pos == 0); }, iters_out);
 }

@@ -395,11 +395,55 @@ void box_box(int ia, int ib, const Body& A, const Body& B, double margin, std::v
         if (poly.empty()) return;
     }
     const double plane = dot(nr, Rb.c) + hcomp(Rb.h, rf);
+    std::vector<V3> pts;
+    std::vector<double> seps;
     for (const V3& P : poly) {
         const double sep = dot(nr, P) - plane;
         if (sep > margin) continue;
-        out.push_back(RawContact{ia, ib, -1, n, P - (0.5 * sep) * nr, std::max(0.0, -sep)});
+        pts.push_back(P);
+        seps.push_back(sep);
     }
+    // At most four points per face contact: the deepest, the point farthest from it, the one
+    // spanning the largest triangle with those two, and the one adding the largest triangle on
+    // an edge of that triangle.  A clipped polygon can carry up to eight vertices; more rows on
+    // one body pair tighten the coupling the Jacobi relaxation alpha = 0.3 has to absorb.
+    std::vector<int> keep;
+    if (pts.size() <= 4) {
+        for (size_t k = 0; k < pts.size(); ++k) keep.push_back((int)k);
+    } else {
+        auto tri = [&](int i, int j, int k) {
+            const V3 c = cross(pts[j] - pts[i], pts[k] - pts[i]);
+            return dot(c, c);
+        };
+        auto chosen = [&](int k) { return std::find(keep.begin(), keep.end(), k) != keep.end(); };
+        int s0 = 0;
+        for (int k = 1; k < (int)pts.size(); ++k)
+            if (seps[k] < seps[s0]) s0 = k;
+        keep.push_back(s0);
+        int s1 = -1;
+        double best1 = -1;
+        for (int k = 0; k < (int)pts.size(); ++k) {
+            const V3 e = pts[k] - pts[s0];
+            if (!chosen(k) && dot(e, e) > best1) best1 = dot(e, e), s1 = k;
+        }
+        keep.push_back(s1);
+        int s2 = -1;
+        double best2 = -1;
+        for (int k = 0; k < (int)pts.size(); ++k)
+            if (!chosen(k) && tri(s0, s1, k) > best2) best2 = tri(s0, s1, k), s2 = k;
+        keep.push_back(s2);
+        int s3 = -1;
+        double best3 = -1;
+        for (int k = 0; k < (int)pts.size(); ++k) {
+            if (chosen(k)) continue;
+            const double a = std::max(tri(s0, s1, k), std::max(tri(s1, s2, k), tri(s2, s0, k)));
+            if (a > best3) best3 = a, s3 = k;
+        }
+        keep.push_back(s3);
+        std::sort(keep.begin(), keep.end());  // polygon order
+    }
+    for (int k : keep)
+        out.push_back(RawContact{ia, ib, -1, n, pts[k] - (0.5 * seps[k]) * nr, std::max(0.0, -seps[k])});
 }
 
 void pair_contacts(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {

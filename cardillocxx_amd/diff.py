@@ -5,8 +5,9 @@ Each step is the reference Moreau step (moreau.cpp:6-13, theta = 1): CPU contact
 projected-Jacobi solve v_{n+1} = S(v_n; M^-1(m), f_ext(m), mu), q_{n+1} = q_n + h B(q_n) v_{n+1}.
 
 Backward: the adjoint of each solve comes from the device (``cdx_pj_adjoint``: implicit
-differentiation of the converged fixed point, or reverse mode through the recorded sweeps), with
-the contact geometry of every step held fixed (W_n, c_n from the forward trajectory).  Positions
+differentiation of the converged fixed point, or reverse mode through the recorded sweeps of a
+cold-start re-solve to the same tolerance, the FP64 reference gradient), with the contact geometry
+of every step held fixed (W_n, c_n from the forward trajectory).  Positions
 obey x_{n+1} = x_n + h v_{n+1}, so dL/dx_N reaches every step's v_{n+1} as h dL/dx_N.
 
 Memory: only (q, v) every ``checkpoint_every`` steps are kept from the forward pass.  The
@@ -46,6 +47,7 @@ class Rollout:
         self.prm.warmstart = 1
         self.adjoint_mode = adjoint_mode
         self.adjoint_tol = adjoint_tol
+        self.record_bytes = 4 << 30  # device memory for the sweeps recorded by the unrolled backward
         # batched: every forward solve is one cdx_batch_pj_solve over the batch's scenes, each
         # scene converging on its own residual exactly as a standalone solve of that scene would
         # (implicit adjoints only: the batched solve records no sweeps)
@@ -149,14 +151,24 @@ class Rollout:
                 q, v = q1, v1
             for n in range(e - 1, s - 1, -1):
                 rec = seg[n - s]
-                # restore the device state of step n: one sweep from its converged impulses
-                prm = native.default_params()
-                prm.max_iter, prm.tol_abs, prm.tol_rel, prm.warmstart = 1, 0.0, 0.0, 1
-                if self.adjoint_mode != native.CDX_ADJ_IMPLICIT:
-                    prm = self.prm
-                    self.ctx.set_recording(True, min(prm.max_iter, 20000))
-                self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"],
-                            rec["imp"] if self.adjoint_mode == native.CDX_ADJ_IMPLICIT else rec["warm"], prm)
+                if self.adjoint_mode == native.CDX_ADJ_IMPLICIT:
+                    # restore the device state of step n: one sweep from its converged impulses
+                    prm = native.default_params()
+                    prm.max_iter, prm.tol_abs, prm.tol_rel, prm.warmstart = 1, 0.0, 0.0, 1
+                    warm = rec["imp"]
+                else:
+                    # reverse mode needs every sweep: re-solve from a cold start to the same
+                    # tolerance.  Reverse mode holds the starting impulses constant, so a
+                    # warm-started solve that converges within a few sweeps would drop how its
+                    # warm start depends on the parameters; from p_0 = 0 the derivative of the
+                    # sweep map converges to that of the fixed point.
+                    prm = native.default_params()
+                    prm.tol_abs, prm.tol_rel, prm.max_iter, prm.warmstart = self.prm.tol_abs, self.prm.tol_rel, \
+                        self.prm.max_iter, 0
+                    warm = np.zeros_like(rec["warm"])
+                    per_sweep = 8 * (8 * len(self.mass) + 3 * len(rec["con"]["body_a"]))
+                    self.ctx.set_recording(True, max(1, min(prm.max_iter, self.record_bytes // per_sweep)))
+                self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"], warm, prm)
                 lv = a + self.dt * gx
                 g = self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=self.adjoint_mode, tol=self.adjoint_tol)
                 if self.adjoint_mode != native.CDX_ADJ_IMPLICIT:

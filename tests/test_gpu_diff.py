@@ -23,8 +23,15 @@ def loss_of(r, batch, target, v0, mu, mass):
     return diff.com_loss(pos, mass, batch.body_off, target)
 
 
-def test_rollout_gradients_vs_finite_differences():
-    batch, r, target = setup()
+@pytest.mark.parametrize("scene,mode", [("boxes", native.CDX_ADJ_IMPLICIT), ("spheres", native.CDX_ADJ_IMPLICIT),
+                                        ("spheres", native.CDX_ADJ_UNROLLED)])
+def test_rollout_gradients_vs_finite_differences(scene, mode):
+    if scene == "boxes":
+        batch, r, target = setup(mode=mode)
+    else:
+        batch = spheres_on_plane()
+        target = np.random.default_rng(0).normal(scale=0.1, size=(batch.nscenes, 3))
+        r = diff.Rollout(batch, steps=12, dt=1e-3, checkpoint_every=5, tol=1e-12, adjoint_mode=mode, adjoint_tol=1e-13)
     v0, mu, mass = batch.v.reshape(-1).copy(), batch.mu_body.copy(), batch.mass.copy()
     L, gpos, gm_direct = loss_of(r, batch, target, v0, mu, mass)
     g = r.backward(gpos)
@@ -34,7 +41,9 @@ def test_rollout_gradients_vs_finite_differences():
         for _ in range(2):
             d = rng.normal(size=base.shape)
             if name == "v0":
-                d.reshape(-1, 6)[:, 2:] = 0.0  # planar perturbation keeps the boxes on the plane
+                d.reshape(-1, 6)[:, 2] = 0.0  # no vertical perturbation keeps the bodies on the plane
+                if scene == "boxes":
+                    d.reshape(-1, 6)[:, 3:] = 0.0
             eps = 1e-6
 
             def f(x):
@@ -44,7 +53,7 @@ def test_rollout_gradients_vs_finite_differences():
 
             fd = (f(base + eps * d) - f(base - eps * d)) / (2 * eps)
             an = float(grad @ d)
-            assert an == pytest.approx(fd, rel=1e-4, abs=1e-9), (name, an, fd)
+            assert an == pytest.approx(fd, rel=1e-4, abs=1e-9), (scene, name, an, fd)
 
 
 def spheres_on_plane(nscenes=2, per_scene=4, seed=3, radius=0.05):
