@@ -145,6 +145,13 @@ def pile_leg(args, native, scenes, metrics):
     ctx = native.Context(0)
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
+    ctx.synchronize()
+    # host buffers in and out across the C ABI (PCIe), timed once; not part of `value`
+    tu = time.perf_counter()
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.synchronize()
+    t_upload = time.perf_counter() - tu
     p = native.default_params()
     p.tol_abs = p.tol_rel = args.tol
     p.max_iter = args.max_iter
@@ -166,8 +173,10 @@ def pile_leg(args, native, scenes, metrics):
     elapsed = time.perf_counter() - t1
     cups = prob.nc * sweeps / elapsed
     log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s, {cups:.3e} contact-updates/s")
+    td = time.perf_counter()
     v = ctx.velocities()
     imp = ctx.impulses()
+    t_download = time.perf_counter() - td
     cone_ok = bool((imp[:, 0] >= 0).all() and (np.linalg.norm(imp[:, 1:], axis=1) <= prob.mu * imp[:, 0] * (1 + 1e-12) + 1e-300).all())
     comp_pj = metrics.complementarity(prob, v, imp)
 
@@ -251,7 +260,11 @@ def pile_leg(args, native, scenes, metrics):
                                "mu~U[0.3,0.8], h=5e-4 s, cold-start PJ to tol 1e-8",
                    "spheres": prob.nb, "contacts": prob.nc, "rows": int(info.nrows), "sweeps_per_step": sweeps / args.steps,
                    "residual": float(np.max(residuals)), "alpha": 0.3, "cone_invariant": cone_ok,
-                   "complementarity": comp_pj},
+                   "complementarity": comp_pj,
+                   "pcie_inclusive": {"upload_ms": t_upload * 1e3, "download_ms": t_download * 1e3,
+                                      "value": prob.nc * sweeps / (elapsed + args.steps * (t_upload + t_download)),
+                                      "note": "each step with its host bodies/contacts uploaded and velocities/impulses "
+                                              "read back through the C ABI"}},
         "roofline": {"bound": "hbm", "kernel": "pj_contacts", "achieved": ach_c, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_c / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": bc, "avg_launch_us": avg_c * 1e6,
@@ -427,7 +440,7 @@ def diff_leg(args, native, scenes):
     rng = np.random.default_rng(0)
     target = rng.normal(scale=0.1, size=(batch.nscenes, 3)) + np.array([0.875, 0.875, 0.05])
     kw = dict(steps=args.diff_steps, dt=1e-3, checkpoint_every=10, tol=args.tol, adjoint_tol=args.adjoint_tol,
-              max_iter=args.batch_max_iter)
+              max_iter=args.batch_max_iter, adjoint_mode=native.CDX_ADJ_AUTO, adjoint_max_iter=args.adjoint_max_iter)
 
     def run(b, tgt, batched):
         r = diff.Rollout(b, batched=batched, **kw)
@@ -443,7 +456,8 @@ def diff_leg(args, native, scenes):
     r, loss, g, gm, tf, tb = run(batch, target, True)
     ns, steps = batch.nscenes, args.diff_steps
     log(f"differentiable: {ns} scenes, forward {tf:.2f}s, backward {tb:.2f}s, "
-        f"{r.stats['forward_solves']} solves, {r.stats['adjoint_iters']} adjoint iterations")
+        f"{r.stats['forward_solves']} solves, {r.stats['adjoint_iters']} adjoint iterations, "
+        f"{r.stats['unrolled_fallbacks']} unrolled fallbacks")
     pick = rng.choice(ns, size=min(args.diff_obo, ns), replace=False)
     t_one = 0.0
     worst = 0.0
@@ -459,11 +473,13 @@ def diff_leg(args, native, scenes):
     return {
         "metric": "fwd+bwd env-steps/s (scenes x steps through forward and checkpointed backward per second)",
         "workload": f"{ns} scenes x 64 boxes (edge 0.1 m, 1 kg) on a plane, v0~N(0,0.5^2), {steps} steps at h=1e-3 s, "
-                    f"loss sum_s ||COM_s(final) - target_s||^2, grads wrt v0, mu, mass; implicit FP64 adjoint, "
+                    f"loss sum_s ||COM_s(final) - target_s||^2, grads wrt v0, mu, mass; implicit FP64 adjoint "
+                    f"(unrolled fallback where it does not converge in {args.adjoint_max_iter} iterations), "
                     f"checkpoint every 10 steps, PJ tol {args.tol:g} per scene, adjoint tol {args.adjoint_tol:g}",
         "value": ns * steps / (tf + tb), "unit": "env-steps/s",
         "forward_s": tf, "backward_s": tb, "loss": loss,
         "forward_solves": r.stats["forward_solves"], "adjoint_iterations": r.stats["adjoint_iters"],
+        "unrolled_fallbacks": r.stats["unrolled_fallbacks"],
         "one_by_one": {"sampled_scenes": [int(k) for k in pick], "estimated_s": t_obo, "speedup": t_obo / (tf + tb),
                        "max_rel_gradient_difference": worst,
                        "method": "sampled scenes rolled out alone (forward + backward through the grid-wide solve and "
@@ -496,6 +512,7 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=100)
     ap.add_argument("--diff-obo", type=int, default=4, help="scenes rolled out one by one")
     ap.add_argument("--adjoint-tol", type=float, default=1e-10)
+    ap.add_argument("--adjoint-max-iter", type=int, default=20_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "

@@ -348,7 +348,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     // both ids become -1 and the kernels skip them.
     std::vector<double> J((size_t)kJ * nc), m(nc), r(3 * (size_t)nc), vb(3 * (size_t)nc), w(3 * (size_t)nc);
     std::vector<int> ia(body_a, body_a + nc), ib(body_b, body_b + nc);
-    bool any_rest = false;
+    bool any_rest = false, any_vbias = false;
     for (int c = 0; c < nc; ++c) {
         for (int k = 0; k < 9; ++k) {
             J[(size_t)k * nc + c] = frame[9 * (size_t)c + k];
@@ -368,6 +368,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
             r[(size_t)k * nc + c] = row ? rest[3 * (size_t)c + k] : 0.0;
             if (row && rest[3 * (size_t)c + k] != 0.0) any_rest = true;
             vb[(size_t)k * nc + c] = row ? vbias[3 * (size_t)c + k] : 0.0;
+            if (row && vbias[3 * (size_t)c + k] != 0.0) any_vbias = true;
             w[(size_t)k * nc + c] = row ? warm[3 * (size_t)c + k] : 0.0;
         }
     }
@@ -399,6 +400,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     ctx->nc = nc;
     ctx->nrows = nrows;
     ctx->any_rest = any_rest;
+    ctx->any_bias = any_rest || any_vbias;
     ctx->h_a = ia;
     ctx->h_b = ib;
     ctx->h_mu.assign(m.begin(), m.end());

@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     double* __restrict__ p, double* __restrict__ imp, const int* __restrict__ dof, const double* __restrict__ minv,
     const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
     const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
-    int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
+    int use_bias, int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
     double* __restrict__ res_out) {
     __shared__ double wsum[kMaxBatchBlock / 64];
     __shared__ int gdone[kMaxBatchBlock / 64];
@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     const int waves_per_group = gsize >> 6;
     for (int it = 0; it < max_iter; ++it) {
         if (!done)
-            for (int c = c0 + lt; c < c1; c += gsize) pj_contact_sweep(c, nc, ida, idb, J, R, bias, mu, p, x, imp);
+            for (int c = c0 + lt; c < c1; c += gsize)
+                pj_contact_sweep(c, nc, ida, idb, J, minv, alpha, bias, use_bias, mu, p, x, imp);
         __syncthreads();
         const int check = (it % check_every) == 0;
         double e2 = 0;
@@ -117,7 +118,7 @@ static int launch_batch(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, i
     CDX_LAUNCH(ctx, "batch_pj", k_batch_pj, grid, block, ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2,
                d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R,
                ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,
-               ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha, prm->warmstart, prm->max_iter, prm->check_every,
+               ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every,
                prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
     return hip_check(ctx, hipGetLastError(), "batch launch");
 }

@@ -68,6 +68,7 @@ struct cdx_ctx {
     std::vector<double> h_mu;
     int nrows = 0;
     bool any_rest = false;  // any non-zero restitution row (PgsAssembler::rhs, pgs_assembler.cpp:64-67)
+    bool any_bias = false;  // any row with restitution or a velocity bias: b = R (c + e (W v + c)) != 0
     int* d_ida = nullptr;
     int* d_idb = nullptr;
     double* d_J = nullptr;     // [27*nc]

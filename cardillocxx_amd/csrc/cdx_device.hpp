@@ -81,6 +81,80 @@ __device__ inline void write_side_impulses(double* __restrict__ imp, int c, int 
     }
 }
 
+// Compact side impulses of the projected-Jacobi sweep: one 9-double record per contact,
+// [lin | angA | angB] at imp9 + 9c with lin = sum_k d_k dp_k and angX = sum_k aX_k dp_k.  Side A
+// receives -[lin ; angA] and side B +[lin ; angB] (W row signs A -1, B +1), so the linear part is
+// stored once; a static side's angular part is not written.
+__device__ inline void write_side_impulses9(double* __restrict__ imp9, int c, int a, int b, int nr, const ContactJ& j,
+                                            const double dp[3]) {
+    double lin[3] = {0, 0, 0}, angA[3] = {0, 0, 0}, angB[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            lin[m] += j.d[k][m] * dp[k];
+            angA[m] += j.aA[k][m] * dp[k];
+            angB[m] += j.aB[k][m] * dp[k];
+        }
+    }
+    double* r = imp9 + 9 * (size_t)c;
+    if (a < 0 && b < 0) return;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) r[m] = lin[m];
+    if (a >= 0)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) r[3 + m] = angA[m];
+    if (b >= 0)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) r[6 + m] = angB[m];
+}
+
+// acc += J_side^T dp of incidence e = 2c + side from the compact records.  Adding -v and
+// subtracting v round identically, so the sums equal those over explicitly negated records.
+__device__ inline void gather_side9(const double* __restrict__ imp9, int e, double acc[6]) {
+    const double* r = imp9 + 9 * (size_t)(e >> 1);
+    if (e & 1) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            acc[m] += r[m];
+            acc[3 + m] += r[6 + m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            acc[m] -= r[m];
+            acc[3 + m] -= r[3 + m];
+        }
+    }
+}
+
+// Inverse mass of a body as [m^-1, Ixx^-1, Iyy^-1, Izz^-1]; zeros for a static side.
+__device__ inline double4 load_minv4(const double* __restrict__ minv, int b) {
+    if (b < 0) return make_double4(0.0, 0.0, 0.0, 0.0);
+    return *reinterpret_cast<const double4*>(minv + (size_t)b * kMinvStride);
+}
+
+// D = sum_j W_kj^2 M^-1_j of one row (rdiag_sparse, projected_jacobi.cpp:57-69): summed side by
+// side in column order (lower body index first), as the row-major compressed W stores the row.
+__device__ inline double row_D(const double d[3], const double aA[3], const double aB[3], double4 mA, double4 mB, int a,
+                               int b) {
+    double DA = 0, DB = 0;
+    DA += d[0] * d[0] * mA.x;
+    DA += d[1] * d[1] * mA.x;
+    DA += d[2] * d[2] * mA.x;
+    DA += aA[0] * aA[0] * mA.y;
+    DA += aA[1] * aA[1] * mA.z;
+    DA += aA[2] * aA[2] * mA.w;
+    DB += d[0] * d[0] * mB.x;
+    DB += d[1] * d[1] * mB.x;
+    DB += d[2] * d[2] * mB.x;
+    DB += aB[0] * aB[0] * mB.y;
+    DB += aB[1] * aB[1] * mB.z;
+    DB += aB[2] * aB[2] * mB.w;
+    return (a >= 0 && b >= 0 && b < a) ? DB + DA : DA + DB;
+}
+
 // Runs the sweep loop of a solve: enqueues sweeps in chunks (aligned to the residual cadence) and
 // keeps at most `window` chunks in flight, polling the pinned done flag.  Enqueued sweeps after
 // convergence return on their first instruction.

@@ -55,47 +55,22 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
 // Sweep
 // ---------------------------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock) k_pj_contacts(int nc, int nblocks, const int* __restrict__ ida,
-                                                        const int* __restrict__ idb, const double* __restrict__ J,
-                                                        const double* __restrict__ R, const double* __restrict__ bias,
-                                                        const double* __restrict__ mu, double* __restrict__ p,
-                                                        const double* __restrict__ x, double* __restrict__ imp,
-                                                        const int* __restrict__ ctrl) {
-    if (ctrl[0]) return;  // converged: the rest of the enqueued sweeps are no-ops
-    const int c = xcd_tile(nblocks) * kBlock + threadIdx.x;
-    if (c >= nc) return;
-    pj_contact_sweep(c, nc, ida, idb, J, R, bias, mu, p, x, imp);
-}
-
-__global__ void __launch_bounds__(kBlock) k_pj_bodies(int nb, const int* __restrict__ dof,
-                                                      const double* __restrict__ minv, const int* __restrict__ incptr,
-                                                      const int* __restrict__ incidx, const double* __restrict__ imp,
-                                                      double* __restrict__ x, int check, double tol_rel, double tol_abs,
-                                                      double* __restrict__ partial, const int* __restrict__ ctrl) {
-    if (ctrl[0]) return;
-    __shared__ double red[kBlock / 64];
-    const int bi = blockIdx.x * kBlock + threadIdx.x;
-    double e2 = 0;
-    if (bi < nb) e2 = pj_body_sweep(bi, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs);
-    if (check) {
-        const double s = block_sum(e2, red);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
-    }
-}
-
 // Contact pass with 4 lanes per contact (lane k = row k of n, t1, t2; lane 3 idle): each lane
-// loads only its row's 9 Jacobian entries, so the kernel needs about half the registers of the
-// one-lane-per-contact form and runs at twice the occupancy.  The normal and tangential values
-// meet through shuffles inside the group of 4 for the Coulomb projection (every lane evaluates the
-// same scale factor), and the side impulses J^T dp are summed over the rows by an xor tree; lane
-// j of the group then writes 3 of the contact's 12 side-impulse doubles (96 contiguous bytes).
+// loads only its row's 9 Jacobian entries, so the kernel needs about half the registers of a
+// one-lane-per-contact form and runs at twice the occupancy.  Each row lane recomputes its
+// R = alpha / D from J and the two bodies' inverse masses (cached: 32 bytes per body) rather than
+// streaming R, and reads the bias only when the problem has one.  The normal and tangential
+// values meet through shuffles inside the group of 4 for the Coulomb projection (every lane
+// evaluates the same scale factor), the side impulses J^T dp are summed over the rows by an xor
+// tree, and lanes 0-2 write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
 constexpr int kLanesPerContact = 4;
 
 __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, const int* __restrict__ ida,
                                                          const int* __restrict__ idb, const double* __restrict__ J,
-                                                         const double* __restrict__ R, const double* __restrict__ bias,
+                                                         const double* __restrict__ minv, double alpha,
+                                                         const double* __restrict__ bias, int use_bias,
                                                          const double* __restrict__ mu, double* __restrict__ p,
-                                                         const double* __restrict__ x, double* __restrict__ imp,
+                                                         const double* __restrict__ x, double* __restrict__ imp9,
                                                          const int* __restrict__ ctrl) {
     if (ctrl[0]) return;
     const int tid = xcd_tile(nblocks) * kBlock + threadIdx.x;
@@ -122,9 +97,11 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
         double ua[6], ub[6];
         load_body(x, a, ua);
         load_body(x, b, ub);
+        const double D = row_D(d, aA, aB, load_minv4(minv, a), load_minv4(minv, b), a, b);
+        const double Rk = D > 0 ? alpha / D : 0.0;
         const double g = row_dot(d, aA, aB, ua, ub);
         pold = p[(size_t)k * nc + c];
-        pt = pold - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+        pt = pold - (Rk * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
     // project() (projected_jacobi.cpp:100-119) on the group's three values
     const int base = threadIdx.x & ~(kLanesPerContact - 1);
@@ -158,21 +135,15 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
     for (int off = 1; off < kLanesPerContact; off <<= 1)
 #pragma unroll
         for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
-    if (live && (a >= 0 || b >= 0)) {
-        // lane 0: -lin (side A), 1: -angA, 2: +lin (side B), 3: +angB
-        const bool sideB = k >= 2;
-        const int side_body = sideB ? b : a;
-        if (side_body >= 0) {
-            double o0, o1, o2;
-            if (k == 0) o0 = -v9[0], o1 = -v9[1], o2 = -v9[2];
-            else if (k == 1) o0 = -v9[3], o1 = -v9[4], o2 = -v9[5];
-            else if (k == 2) o0 = v9[0], o1 = v9[1], o2 = v9[2];
-            else o0 = v9[6], o1 = v9[7], o2 = v9[8];
-            double* dst = imp + (size_t)(2 * c + (sideB ? 1 : 0)) * 6 + (k & 1) * 3;
-            dst[0] = o0;
-            dst[1] = o1;
-            dst[2] = o2;
-        }
+    // lane 0: lin, 1: angA (dynamic A only), 2: angB (dynamic B only)
+    if (live && (a >= 0 || b >= 0) && k < 3 && (k == 0 || (k == 1 ? a >= 0 : b >= 0))) {
+        const double o0 = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
+        const double o1 = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
+        const double o2 = k == 0 ? v9[2] : (k == 1 ? v9[5] : v9[8]);
+        double* dst = imp9 + 9 * (size_t)c + 3 * k;
+        dst[0] = o0;
+        dst[1] = o1;
+        dst[2] = o2;
     }
 }
 
@@ -184,7 +155,7 @@ constexpr int kLanesPerBody = 8;
 
 __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
                                                        const int* __restrict__ incptr, const int* __restrict__ incidx,
-                                                       const double* __restrict__ imp, double* __restrict__ x, int check,
+                                                       const double* __restrict__ imp9, double* __restrict__ x, int check,
                                                        double tol_rel, double tol_abs, double* __restrict__ partial,
                                                        const int* __restrict__ ctrl) {
     if (ctrl[0]) return;
@@ -197,16 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
     if (live) {
         i0 = incptr[bi];
         i1 = incptr[bi + 1];
-        for (int i = i0 + l; i < i1; i += kLanesPerBody) {
-            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
-            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
-            acc[0] += q0.x;
-            acc[1] += q0.y;
-            acc[2] += q1.x;
-            acc[3] += q1.y;
-            acc[4] += q2.x;
-            acc[5] += q2.y;
-        }
+        for (int i = i0 + l; i < i1; i += kLanesPerBody) gather_side9(imp9, incidx[i], acc);
     }
 #pragma unroll
     for (int off = kLanesPerBody / 2; off > 0; off >>= 1)
@@ -590,8 +552,8 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     }
     // one sweep; it < 0 inside a captured chunk (the residual kernel then reads the chunk counter)
     auto launch_sweep = [&](int it, int check) -> int {
-        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                   ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_minv,
+                   prm->alpha, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
         if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));

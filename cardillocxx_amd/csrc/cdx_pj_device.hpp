@@ -29,11 +29,7 @@ __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ i
     double ua[6], ub[6];
     load_body(v, a, ua);
     load_body(v, b, ub);
-    double mA[4] = {0, 0, 0, 0}, mB[4] = {0, 0, 0, 0};
-    if (a >= 0)
-        for (int m = 0; m < 4; ++m) mA[m] = minv[(size_t)a * kMinvStride + m];
-    if (b >= 0)
-        for (int m = 0; m < 4; ++m) mB[m] = minv[(size_t)b * kMinvStride + m];
+    const double4 mA = load_minv4(minv, a), mB = load_minv4(minv, b);
     double pw[3] = {0, 0, 0};
     for (int k = 0; k < 3; ++k) {
         if (k >= nr) {
@@ -42,14 +38,7 @@ __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ i
             p[(size_t)k * nc + c] = 0;
             continue;
         }
-        // D summed side by side in column order (lower body index first), as the row-major
-        // compressed W stores the row (rdiag_sparse iterates the row's entries in column order).
-        double DA = 0, DB = 0;
-        for (int m = 0; m < 3; ++m) DA += j.d[k][m] * j.d[k][m] * mA[0];
-        for (int m = 0; m < 3; ++m) DA += j.aA[k][m] * j.aA[k][m] * mA[1 + m];
-        for (int m = 0; m < 3; ++m) DB += j.d[k][m] * j.d[k][m] * mB[0];
-        for (int m = 0; m < 3; ++m) DB += j.aB[k][m] * j.aB[k][m] * mB[1 + m];
-        const double D = (a >= 0 && b >= 0 && b < a) ? DB + DA : DA + DB;
+        const double D = row_D(j.d[k], j.aA[k], j.aB[k], mA, mB, a, b);
         const double Rk = D > 0 ? alpha / D : 0.0;
         const double ck = vb[(size_t)k * nc + c];
         const double vprev = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub) + ck;
@@ -58,7 +47,7 @@ __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ i
         pw[k] = warmstart ? p[(size_t)k * nc + c] : 0.0;
         p[(size_t)k * nc + c] = pw[k];
     }
-    write_side_impulses(imp, c, a, b, nr, j, pw);
+    write_side_impulses9(imp, c, a, b, nr, j, pw);
 }
 
 // Per-body x0 (see k_pj_setup_bodies).
@@ -69,18 +58,8 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
                                      int have_contacts, double* __restrict__ x) {
     const int nd = dof[bi];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (have_contacts) {
-        for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) {
-            const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
-            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
-            acc[0] += q0.x;
-            acc[1] += q0.y;
-            acc[2] += q1.x;
-            acc[3] += q1.y;
-            acc[4] += q2.x;
-            acc[5] += q2.y;
-        }
-    }
+    if (have_contacts)
+        for (int i = incptr[bi]; i < incptr[bi + 1]; ++i) gather_side9(imp, incidx[i], acc);
     const double* vv = v + (size_t)bi * kBodyStride;
     const double* fe = fext + (size_t)bi * kBodyStride;
     const double* fg = fgyro + (size_t)bi * kBodyStride;
@@ -98,9 +77,12 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
 }
 
 // One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).
+// R = alpha / D is recomputed from J and the bodies' inverse masses (the setup's expression)
+// instead of being streamed, and the bias is only read when the problem has one (`use_bias`:
+// some restitution or velocity bias), so a sweep reads J, mu, p and the twists.
 __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
-                                        const double* __restrict__ J, const double* __restrict__ R,
-                                        const double* __restrict__ bias, const double* __restrict__ mu,
+                                        const double* __restrict__ J, const double* __restrict__ minv, double alpha,
+                                        const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
                                         double* __restrict__ p, const double* __restrict__ x,
                                         double* __restrict__ imp) {
     const int a = ida[c], b = idb[c];
@@ -112,13 +94,16 @@ __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ i
     double ua[6], ub[6];
     load_body(x, a, ua);
     load_body(x, b, ub);
+    const double4 mA = load_minv4(minv, a), mB = load_minv4(minv, b);
     double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (k >= nr) break;
+        const double D = row_D(j.d[k], j.aA[k], j.aB[k], mA, mB, a, b);
+        const double Rk = D > 0 ? alpha / D : 0.0;
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
         pold[k] = p[(size_t)k * nc + c];
-        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+        pn[k] = pold[k] - (Rk * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
     // project() (projected_jacobi.cpp:100-119)
     pn[0] = fmax(pn[0], 0.0);
@@ -138,7 +123,7 @@ __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ i
         dp[k] = pn[k] - pold[k];
         p[(size_t)k * nc + c] = pn[k];
     }
-    write_side_impulses(imp, c, a, b, nr, j, dp);
+    write_side_impulses9(imp, c, a, b, nr, j, dp);
 }
 
 // One body of a PJ sweep: x += (W^T dp) / M over its incidences in a fixed order; returns the
@@ -150,16 +135,7 @@ __device__ inline double pj_body_sweep(int bi, const int* __restrict__ dof, cons
     double e2 = 0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const int i0 = incptr[bi], i1 = incptr[bi + 1];
-    for (int i = i0; i < i1; ++i) {
-        const double2* q = reinterpret_cast<const double2*>(imp + (size_t)incidx[i] * 6);
-        const double2 q0 = q[0], q1 = q[1], q2 = q[2];
-        acc[0] += q0.x;
-        acc[1] += q0.y;
-        acc[2] += q1.x;
-        acc[3] += q1.y;
-        acc[4] += q2.x;
-        acc[5] += q2.y;
-    }
+    for (int i = i0; i < i1; ++i) gather_side9(imp, incidx[i], acc);
     double u[6];
     load_body(x, bi, u);
     const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);

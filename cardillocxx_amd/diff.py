@@ -35,7 +35,7 @@ class Grads:
 class Rollout:
     def __init__(self, batch: ShapeBatch, steps: int, dt: float = 1e-3, checkpoint_every: int = 10, tol: float = 1e-10,
                  adjoint_mode: int = native.CDX_ADJ_IMPLICIT, adjoint_tol: float = 1e-11, device: int = 0, ctx=None,
-                 max_iter: int = 200000, batched: bool = False):
+                 max_iter: int = 200000, batched: bool = False, adjoint_max_iter: int = 100000):
         self.batch = batch
         self.steps = steps
         self.dt = dt
@@ -47,15 +47,14 @@ class Rollout:
         self.prm.warmstart = 1
         self.adjoint_mode = adjoint_mode
         self.adjoint_tol = adjoint_tol
-        self.record_bytes = 4 << 30  # device memory for the sweeps recorded by the unrolled backward
+        self.record_bytes = 32 << 30  # device memory for the sweeps recorded by the unrolled backward
+        self.adjoint_max_iter = adjoint_max_iter
         # batched: every forward solve is one cdx_batch_pj_solve over the batch's scenes, each
         # scene converging on its own residual exactly as a standalone solve of that scene would
-        # (implicit adjoints only: the batched solve records no sweeps)
+        # (the adjoints run on the restored state of each step, so every mode applies)
         self.batched = batched and batch.nscenes > 1
-        if self.batched and adjoint_mode != native.CDX_ADJ_IMPLICIT:
-            raise ValueError("Rollout: batched solves support the implicit adjoint only")
         self.frozen = None  # optional per-step contact dicts to reuse (geometry held fixed)
-        self.stats = dict(forward_solves=0, sweeps=0, adjoint_iters=0)
+        self.stats = dict(forward_solves=0, sweeps=0, adjoint_iters=0, unrolled_fallbacks=0)
 
     # ---- model parameters -> solver inputs
     def _bodies(self, v, mass):
@@ -130,6 +129,39 @@ class Rollout:
         self.qN = q
         return q.reshape(-1, 7)[:, :3].copy()
 
+    def _step_adjoint(self, rec, lv):
+        """Adjoint of one step's solve given dL/dv_{n+1} = lv.  Implicit: restore the converged
+        state with one sweep from its impulses and iterate the adjoint.  AUTO: implicit with at
+        most `adjoint_max_iter` iterations, and where that does not converge (a contact set whose
+        impulses are not unique, e.g. a box face on four coplanar points) the unrolled fallback.
+        Unrolled: reverse mode needs every sweep, so the step is re-solved from a cold start to the
+        same tolerance with recording on.  Reverse mode holds the starting impulses constant, so a
+        warm-started solve that converges within a few sweeps would drop how its warm start
+        depends on the parameters; from p_0 = 0 the derivative of the sweep map converges to that
+        of the fixed point."""
+        mode = self.adjoint_mode
+        if mode in (native.CDX_ADJ_IMPLICIT, native.CDX_ADJ_AUTO):
+            prm = native.default_params()
+            prm.max_iter, prm.tol_abs, prm.tol_rel, prm.warmstart = 1, 0.0, 0.0, 1
+            self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"], rec["imp"], prm)
+            try:
+                return self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=native.CDX_ADJ_IMPLICIT,
+                                           max_iter=self.adjoint_max_iter, tol=self.adjoint_tol)
+            except native.CdxError as e:
+                if mode == native.CDX_ADJ_IMPLICIT or e.status != native.CDX_DIVERGED:
+                    raise
+                self.stats["unrolled_fallbacks"] += 1
+        prm = native.default_params()
+        prm.tol_abs, prm.tol_rel, prm.max_iter, prm.warmstart = self.prm.tol_abs, self.prm.tol_rel, self.prm.max_iter, 0
+        per_sweep = 8 * (8 * len(self.mass) + 3 * len(rec["con"]["body_a"]))
+        self.ctx.set_recording(True, max(1, min(prm.max_iter, self.record_bytes // per_sweep)))
+        try:
+            self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"],
+                        np.zeros_like(rec["warm"]), prm)
+            return self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=native.CDX_ADJ_UNROLLED)
+        finally:
+            self.ctx.set_recording(False)
+
     def backward(self, g_pos):
         """dL/d(v0, mu_body, mass) given dL/dx_N (positions, [nb, 3])."""
         nb = len(self.mass)
@@ -151,28 +183,7 @@ class Rollout:
                 q, v = q1, v1
             for n in range(e - 1, s - 1, -1):
                 rec = seg[n - s]
-                if self.adjoint_mode == native.CDX_ADJ_IMPLICIT:
-                    # restore the device state of step n: one sweep from its converged impulses
-                    prm = native.default_params()
-                    prm.max_iter, prm.tol_abs, prm.tol_rel, prm.warmstart = 1, 0.0, 0.0, 1
-                    warm = rec["imp"]
-                else:
-                    # reverse mode needs every sweep: re-solve from a cold start to the same
-                    # tolerance.  Reverse mode holds the starting impulses constant, so a
-                    # warm-started solve that converges within a few sweeps would drop how its
-                    # warm start depends on the parameters; from p_0 = 0 the derivative of the
-                    # sweep map converges to that of the fixed point.
-                    prm = native.default_params()
-                    prm.tol_abs, prm.tol_rel, prm.max_iter, prm.warmstart = self.prm.tol_abs, self.prm.tol_rel, \
-                        self.prm.max_iter, 0
-                    warm = np.zeros_like(rec["warm"])
-                    per_sweep = 8 * (8 * len(self.mass) + 3 * len(rec["con"]["body_a"]))
-                    self.ctx.set_recording(True, max(1, min(prm.max_iter, self.record_bytes // per_sweep)))
-                self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"], warm, prm)
-                lv = a + self.dt * gx
-                g = self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=self.adjoint_mode, tol=self.adjoint_tol)
-                if self.adjoint_mode != native.CDX_ADJ_IMPLICIT:
-                    self.ctx.set_recording(False)
+                g = self._step_adjoint(rec, a + self.dt * gx)
                 self.stats["adjoint_iters"] += g["info"].iters
                 a = g["v"]
                 g_minv += g["minv"]

@@ -124,3 +124,27 @@ def test_batched_rollout_matches_scene_by_scene():
         np.testing.assert_allclose(g1.v0, g.v0[b0:b1], rtol=1e-8, atol=1e-12)
         np.testing.assert_allclose(g1.mu, g.mu[b0:b1], rtol=1e-8, atol=1e-12)
         np.testing.assert_allclose(g1.mass, g.mass[b0:b1], rtol=1e-8, atol=1e-12)
+
+
+def test_auto_mode_falls_back_to_unrolled():
+    """AUTO takes the implicit adjoint where it converges and otherwise the unrolled one; with an
+    iteration cap the implicit iteration cannot meet, every step falls back and the gradients
+    equal those of the unrolled mode."""
+    batch = spheres_on_plane()
+    target = np.random.default_rng(0).normal(scale=0.1, size=(batch.nscenes, 3))
+    kw = dict(steps=4, dt=1e-3, checkpoint_every=2, tol=1e-12, adjoint_tol=1e-13)
+    ra = diff.Rollout(batch, adjoint_mode=native.CDX_ADJ_AUTO, adjoint_max_iter=3, **kw)
+    ru = diff.Rollout(batch, adjoint_mode=native.CDX_ADJ_UNROLLED, **kw)
+    v0, mu, mass = batch.v.reshape(-1).copy(), batch.mu_body.copy(), batch.mass.copy()
+    _, gpos, _ = loss_of(ra, batch, target, v0, mu, mass)
+    ga = ra.backward(gpos)
+    loss_of(ru, batch, target, v0, mu, mass)
+    gu = ru.backward(gpos)
+    assert ra.stats["unrolled_fallbacks"] == 4
+    for a, b in ((ga.v0, gu.v0), (ga.mu, gu.mu), (ga.mass, gu.mass)):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+    # with room to converge, AUTO is the implicit adjoint
+    rb = diff.Rollout(batch, adjoint_mode=native.CDX_ADJ_AUTO, **kw)
+    loss_of(rb, batch, target, v0, mu, mass)
+    rb.backward(gpos)
+    assert rb.stats["unrolled_fallbacks"] == 0
