@@ -512,7 +512,7 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=100)
     ap.add_argument("--diff-obo", type=int, default=4, help="scenes rolled out one by one")
     ap.add_argument("--adjoint-tol", type=float, default=1e-10)
-    ap.add_argument("--adjoint-max-iter", type=int, default=20_000)
+    ap.add_argument("--adjoint-max-iter", type=int, default=2_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "

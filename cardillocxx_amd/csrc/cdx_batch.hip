@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     for (int it = 0; it < max_iter; ++it) {
         if (!done)
             for (int c = c0 + lt; c < c1; c += gsize)
-                pj_contact_sweep(c, nc, ida, idb, J, minv, alpha, bias, use_bias, mu, p, x, imp);
+                pj_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, x, imp);
         __syncthreads();
         const int check = (it % check_every) == 0;
         double e2 = 0;

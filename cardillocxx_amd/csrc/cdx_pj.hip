@@ -57,9 +57,9 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
 
 // Contact pass with 4 lanes per contact (lane k = row k of n, t1, t2; lane 3 idle): each lane
 // loads only its row's 9 Jacobian entries, so the kernel needs about half the registers of a
-// one-lane-per-contact form and runs at twice the occupancy.  Each row lane recomputes its
-// R = alpha / D from J and the two bodies' inverse masses (cached: 32 bytes per body) rather than
-// streaming R, and reads the bias only when the problem has one.  The normal and tangential
+// one-lane-per-contact form and runs at twice the occupancy (64 VGPRs: recomputing R = alpha / D
+// in the sweep instead of streaming it costs 16 more and two waves per SIMD, a net loss).  The
+// bias is read only when the problem has one.  The normal and tangential
 // values meet through shuffles inside the group of 4 for the Coulomb projection (every lane
 // evaluates the same scale factor), the side impulses J^T dp are summed over the rows by an xor
 // tree, and lanes 0-2 write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
@@ -67,7 +67,7 @@ constexpr int kLanesPerContact = 4;
 
 __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, const int* __restrict__ ida,
                                                          const int* __restrict__ idb, const double* __restrict__ J,
-                                                         const double* __restrict__ minv, double alpha,
+                                                         const double* __restrict__ R,
                                                          const double* __restrict__ bias, int use_bias,
                                                          const double* __restrict__ mu, double* __restrict__ p,
                                                          const double* __restrict__ x, double* __restrict__ imp9,
@@ -97,11 +97,9 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
         double ua[6], ub[6];
         load_body(x, a, ua);
         load_body(x, b, ub);
-        const double D = row_D(d, aA, aB, load_minv4(minv, a), load_minv4(minv, b), a, b);
-        const double Rk = D > 0 ? alpha / D : 0.0;
         const double g = row_dot(d, aA, aB, ua, ub);
         pold = p[(size_t)k * nc + c];
-        pt = pold - (Rk * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+        pt = pold - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
     // project() (projected_jacobi.cpp:100-119) on the group's three values
     const int base = threadIdx.x & ~(kLanesPerContact - 1);
@@ -552,8 +550,8 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     }
     // one sweep; it < 0 inside a captured chunk (the residual kernel then reads the chunk counter)
     auto launch_sweep = [&](int it, int check) -> int {
-        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_minv,
-                   prm->alpha, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                   ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
         if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));

@@ -77,11 +77,9 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
 }
 
 // One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).
-// R = alpha / D is recomputed from J and the bodies' inverse masses (the setup's expression)
-// instead of being streamed, and the bias is only read when the problem has one (`use_bias`:
-// some restitution or velocity bias), so a sweep reads J, mu, p and the twists.
+// The bias is only read when the problem has one (`use_bias`: some restitution or velocity bias).
 __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
-                                        const double* __restrict__ J, const double* __restrict__ minv, double alpha,
+                                        const double* __restrict__ J, const double* __restrict__ R,
                                         const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
                                         double* __restrict__ p, const double* __restrict__ x,
                                         double* __restrict__ imp) {
@@ -94,16 +92,13 @@ __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ i
     double ua[6], ub[6];
     load_body(x, a, ua);
     load_body(x, b, ub);
-    const double4 mA = load_minv4(minv, a), mB = load_minv4(minv, b);
     double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (k >= nr) break;
-        const double D = row_D(j.d[k], j.aA[k], j.aB[k], mA, mB, a, b);
-        const double Rk = D > 0 ? alpha / D : 0.0;
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
         pold[k] = p[(size_t)k * nc + c];
-        pn[k] = pold[k] - (Rk * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
     // project() (projected_jacobi.cpp:100-119)
     pn[0] = fmax(pn[0], 0.0);
