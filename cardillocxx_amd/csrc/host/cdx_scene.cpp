@@ -216,10 +216,10 @@ int cdx_sphere_contacts(int nb, const double* q, const double* radius, const dou
 namespace {
 
 struct Body {
-    int type;  // 0 sphere, 1 box
+    int type;  // 0 sphere, 1 box, 2 capsule
     V3 c;
     double R[3][3];  // columns are the body axes in world coordinates
-    V3 h;            // sphere: h.x = radius; box: half extents
+    V3 h;            // sphere: h.x = radius; box: half extents; capsule: h.x = radius, h.y = half length (body z)
     double bound;    // bounding-sphere radius
 };
 
@@ -240,6 +240,9 @@ Body make_body(const double* q, int type, const double* dims) {
     if (type == 0) {
         b.h = {dims[0], 0, 0};
         b.bound = dims[0];
+    } else if (type == 2) {
+        b.h = {dims[0], dims[1], 0};
+        b.bound = dims[0] + dims[1];
     } else {
         b.h = {dims[0], dims[1], dims[2]};
         b.bound = std::sqrt(dims[0] * dims[0] + dims[1] * dims[1] + dims[2] * dims[2]);
@@ -249,8 +252,27 @@ Body make_body(const double* q, int type, const double* dims) {
 
 double hcomp(const V3& h, int k) { return k == 0 ? h.x : (k == 1 ? h.y : h.z); }
 
+// Capsule segment end points (the axis is the body z axis).
+inline void capsule_ends(const Body& C, V3& p0, V3& p1) {
+    const V3 ax = col(C.R, 2);
+    p0 = C.c - C.h.y * ax;
+    p1 = C.c + C.h.y * ax;
+}
+
+
 // Body (dynamic, index `bi`) against static half-space n.x >= d; the wall is side A.
 void wall_contacts(int bi, int w, const Body& B, V3 n, double d, double margin, std::vector<RawContact>& out) {
+    if (B.type == 2) {  // capsule: its two segment ends
+        V3 ends[2];
+        capsule_ends(B, ends[0], ends[1]);
+        for (const V3& P : ends) {
+            const double hgt = dot(n, P) - d;
+            const double gap = hgt - B.h.x;
+            if (gap > margin) continue;
+            out.push_back(RawContact{-1, bi, w, n, 0.5 * ((P - hgt * n) + (P - B.h.x * n)), std::max(0.0, -gap)});
+        }
+        return;
+    }
     if (B.type == 0) {
         const double hgt = dot(n, B.c) - d;
         const double gap = hgt - B.h.x;
@@ -446,10 +468,216 @@ void box_box(int ia, int ib, const Body& A, const Body& B, double margin, std::v
         out.push_back(RawContact{ia, ib, -1, n, pts[k] - (0.5 * seps[k]) * nr, std::max(0.0, -seps[k])});
 }
 
+// Closest point on segment [p0, p1] to x.
+inline V3 closest_on_segment(V3 p0, V3 p1, V3 x) {
+    const V3 e = p1 - p0;
+    const double ee = dot(e, e);
+    const double t = ee > 0 ? std::max(0.0, std::min(1.0, dot(x - p0, e) / ee)) : 0.0;
+    return p0 + t * e;
+}
+
+// A sphere of radius ra at pa against one of radius rb at pb: normal from a to b.
+inline void ball_ball(int ia, int ib, V3 pa, double ra, V3 pb, double rb, double margin, std::vector<RawContact>& out) {
+    const V3 d = pb - pa;
+    const double dist = std::sqrt(dot(d, d));
+    const double gap = dist - ra - rb;
+    if (gap > margin) return;
+    const V3 n = dist > 0 ? (1.0 / dist) * d : V3{0, 0, 1};
+    out.push_back(RawContact{ia, ib, -1, n, 0.5 * ((pa + ra * n) + (pb - rb * n)), std::max(0.0, -gap)});
+}
+
+// Capsule-capsule: closest points of the two segments (Ericson, Real-Time Collision Detection,
+// 5.1.9); for nearly parallel overlapping segments the two ends of the overlap, so a capsule
+// lying on another gets a two-point support.
+void capsule_capsule(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {
+    V3 a0, a1, b0, b1;
+    capsule_ends(A, a0, a1);
+    capsule_ends(B, b0, b1);
+    const V3 d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+    const double aa = dot(d1, d1), ee = dot(d2, d2), ff = dot(d2, r);
+    const double cc = dot(d1, r), bb = dot(d1, d2);
+    const double den = aa * ee - bb * bb;
+    if (aa > 0 && ee > 0 && den <= 1e-6 * aa * ee) {
+        // parallel: project B's ends onto A's axis and keep the overlap
+        const double t0 = std::max(0.0, std::min(1.0, dot(b0 - a0, d1) / aa));
+        const double t1 = std::max(0.0, std::min(1.0, dot(b1 - a0, d1) / aa));
+        const double lo = std::min(t0, t1), hi = std::max(t0, t1);
+        const V3 pa_lo = a0 + lo * d1, pa_hi = a0 + hi * d1;
+        ball_ball(ia, ib, pa_lo, A.h.x, closest_on_segment(b0, b1, pa_lo), B.h.x, margin, out);
+        if (hi - lo > 1e-9) ball_ball(ia, ib, pa_hi, A.h.x, closest_on_segment(b0, b1, pa_hi), B.h.x, margin, out);
+        return;
+    }
+    double sa = 0, tb = 0;
+    if (aa <= 0 && ee <= 0) {
+        sa = tb = 0;
+    } else if (aa <= 0) {
+        tb = std::max(0.0, std::min(1.0, ff / ee));
+    } else if (ee <= 0) {
+        sa = std::max(0.0, std::min(1.0, -cc / aa));
+    } else {
+        sa = std::max(0.0, std::min(1.0, (bb * ff - cc * ee) / den));
+        tb = (bb * sa + ff) / ee;
+        if (tb < 0) {
+            tb = 0;
+            sa = std::max(0.0, std::min(1.0, -cc / aa));
+        } else if (tb > 1) {
+            tb = 1;
+            sa = std::max(0.0, std::min(1.0, (bb - cc) / aa));
+        }
+    }
+    ball_ball(ia, ib, a0 + sa * d1, A.h.x, b0 + tb * d2, B.h.x, margin, out);
+}
+
+// Capsule-box by sphere probes: the two segment ends and the point of the segment nearest the
+// box centre, each against the box (closest point); probes closer than a radius to an earlier
+// one are skipped.
+void capsule_box(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {
+    const bool a_caps = A.type == 2;
+    const Body& C = a_caps ? A : B;
+    const Body& X = a_caps ? B : A;
+    V3 p0, p1;
+    capsule_ends(C, p0, p1);
+    const V3 probes[3] = {p0, p1, closest_on_segment(p0, p1, X.c)};
+    for (int k = 0; k < 3; ++k) {
+        bool dup = false;
+        for (int m = 0; m < k; ++m) {
+            const V3 d = probes[k] - probes[m];
+            if (dot(d, d) < C.h.x * C.h.x) dup = true;
+        }
+        if (dup) continue;
+        Body S;
+        S.type = 0;
+        S.c = probes[k];
+        S.h = {C.h.x, 0, 0};
+        S.bound = C.h.x;
+        V3 n, pbox, psph;
+        double gap;
+        if (!sphere_box(S, X, margin, n, pbox, psph, gap)) continue;
+        const V3 nab = a_caps ? -1.0 * n : n;  // n points from the box to the probe
+        out.push_back(RawContact{ia, ib, -1, nab, 0.5 * (pbox + psph), std::max(0.0, -gap)});
+    }
+}
+
+// ---- static heightfield: samples h[i + nx j] at (x0 + i dx, y0 + j dy); cell (i, j) is split
+// along the diagonal (i+1, j)-(i, j+1) into two triangles.
+struct HField {
+    int nx = 0, ny = 0;
+    double x0 = 0, y0 = 0, dx = 1, dy = 1;
+    const double* h = nullptr;
+    int wall = -1;  // friction slot
+    V3 vert(int i, int j) const { return {x0 + i * dx, y0 + j * dy, h[i + (size_t)nx * j]}; }
+};
+
+// Closest point of triangle (a, b, c) to p (Ericson, Real-Time Collision Detection, 5.1.5).
+V3 closest_on_triangle(V3 p, V3 a, V3 b, V3 c) {
+    const V3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return a;
+    const V3 bp = p - b;
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return a + (d1 / (d1 - d3)) * ab;
+    const V3 cp = p - c;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return a + (d2 / (d2 - d6)) * ac;
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) return b + ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b);
+    const double den = 1.0 / (va + vb + vc);
+    return a + (vb * den) * ab + (vc * den) * ac;
+}
+
+// One probe (a sphere of radius r at P; r = 0 for a point) against the heightfield.  Above the
+// surface: the closest point over the triangles under the probe's footprint; below it: the
+// triangle under P, its upward normal and the signed distance to its plane.  Normal from the
+// terrain (side A, static) to the body.  Outside the sampled rectangle there is no terrain.
+bool hf_probe(const HField& hf, V3 P, double r, double margin, V3& n, V3& q, double& gap) {
+    const double fx = (P.x - hf.x0) / hf.dx, fy = (P.y - hf.y0) / hf.dy;
+    if (!(fx >= 0 && fy >= 0 && fx <= hf.nx - 1 && fy <= hf.ny - 1)) return false;
+    const int ci = std::min(hf.nx - 2, (int)fx), cj = std::min(hf.ny - 2, (int)fy);
+    V3 t[3];
+    const bool upper = (fx - ci) + (fy - cj) > 1.0;
+    if (!upper) t[0] = hf.vert(ci, cj), t[1] = hf.vert(ci + 1, cj), t[2] = hf.vert(ci, cj + 1);
+    else t[0] = hf.vert(ci + 1, cj + 1), t[1] = hf.vert(ci, cj + 1), t[2] = hf.vert(ci + 1, cj);
+    V3 tn = cross(t[1] - t[0], t[2] - t[0]);
+    if (tn.z < 0) tn = -1.0 * tn;
+    tn = (1.0 / std::sqrt(dot(tn, tn))) * tn;
+    const double plane = dot(P - t[0], tn);
+    if (plane < 0) {  // below the surface
+        gap = plane - r;
+        if (gap > margin) return false;
+        n = tn;
+        q = P - plane * tn;
+        return true;
+    }
+    const double reach = r + margin;
+    const int i0 = std::max(0, (int)std::floor((P.x - reach - hf.x0) / hf.dx));
+    const int i1 = std::min(hf.nx - 2, (int)std::floor((P.x + reach - hf.x0) / hf.dx));
+    const int j0 = std::max(0, (int)std::floor((P.y - reach - hf.y0) / hf.dy));
+    const int j1 = std::min(hf.ny - 2, (int)std::floor((P.y + reach - hf.y0) / hf.dy));
+    double best = 1e300;
+    V3 bq{0, 0, 0};
+    for (int j = j0; j <= j1; ++j)
+        for (int i = i0; i <= i1; ++i) {
+            const V3 v00 = hf.vert(i, j), v10 = hf.vert(i + 1, j), v01 = hf.vert(i, j + 1), v11 = hf.vert(i + 1, j + 1);
+            const V3 c0 = closest_on_triangle(P, v00, v10, v01), c1 = closest_on_triangle(P, v11, v01, v10);
+            const V3 e0 = P - c0, e1 = P - c1;
+            const double q0 = dot(e0, e0), q1 = dot(e1, e1);
+            if (q0 < best) best = q0, bq = c0;
+            if (q1 < best) best = q1, bq = c1;
+        }
+    const double dist = std::sqrt(best);
+    gap = dist - r;
+    if (gap > margin) return false;
+    n = dist > 1e-12 ? (1.0 / dist) * (P - bq) : tn;
+    q = bq;
+    return true;
+}
+
+// Probes of a body against the heightfield: sphere centre; capsule ends and midpoint; box corners.
+void hf_contacts(int bi, const Body& B, const HField& hf, double margin, std::vector<RawContact>& out) {
+    V3 probes[8];
+    int np = 0;
+    double r = 0;
+    if (B.type == 0) {
+        probes[np++] = B.c;
+        r = B.h.x;
+    } else if (B.type == 2) {
+        capsule_ends(B, probes[0], probes[1]);
+        probes[2] = B.c;
+        np = 3;
+        r = B.h.x;
+    } else {
+        for (int k = 0; k < 8; ++k)
+            probes[np++] = B.c + (((k & 1) ? 1.0 : -1.0) * B.h.x) * col(B.R, 0) + (((k & 2) ? 1.0 : -1.0) * B.h.y) * col(B.R, 1) +
+                           (((k & 4) ? 1.0 : -1.0) * B.h.z) * col(B.R, 2);
+    }
+    for (int k = 0; k < np; ++k) {
+        V3 n, q;
+        double gap;
+        if (!hf_probe(hf, probes[k], r, margin, n, q, gap)) continue;
+        out.push_back(RawContact{-1, bi, hf.wall, n, 0.5 * (q + (probes[k] - r * n)), std::max(0.0, -gap)});
+    }
+}
+
 void pair_contacts(int ia, int ib, const Body& A, const Body& B, double margin, std::vector<RawContact>& out) {
     const V3 d = B.c - A.c;
     const double reach = A.bound + B.bound + margin;
     if (dot(d, d) > reach * reach) return;
+    if (A.type == 2 || B.type == 2) {
+        if (A.type == 2 && B.type == 2) return capsule_capsule(ia, ib, A, B, margin, out);
+        if (A.type == 1 || B.type == 1) return capsule_box(ia, ib, A, B, margin, out);
+        // capsule-sphere: the sphere against the nearest point of the segment
+        V3 p0, p1;
+        const Body& C = A.type == 2 ? A : B;
+        const Body& S = A.type == 2 ? B : A;
+        capsule_ends(C, p0, p1);
+        const V3 pc = closest_on_segment(p0, p1, S.c);
+        if (A.type == 2) return ball_ball(ia, ib, pc, C.h.x, S.c, S.h.x, margin, out);
+        return ball_ball(ia, ib, S.c, S.h.x, pc, C.h.x, margin, out);
+    }
     if (A.type == 0 && B.type == 0) {
         const double dist = std::sqrt(dot(d, d));
         const double gap = dist - A.h.x - B.h.x;
@@ -477,10 +705,39 @@ int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const 
                        const double* mu_body, int nw, const double* wall_n, const double* wall_d, const double* mu_wall,
                        double margin, int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA,
                        double* angB, double* point, double* depth, int* contact_off) {
-    if (nscenes < 1 || !body_off || nw < 0) return -1;
+    return cdx_shape_contacts_hf(nscenes, body_off, q, shape, dims, mu_body, nw, wall_n, wall_d, mu_wall, nullptr, 0, nullptr,
+                                 nullptr, margin, cap, body_a, body_b, mu, frame, angA, angB, point, depth, contact_off);
+}
+
+int cdx_shape_contacts_hf(int nscenes, const int* body_off, const double* q, const int* shape, const double* dims,
+                          const double* mu_body, int nw, const double* wall_n, const double* wall_d, const double* mu_wall,
+                          const cdx_heightfield* terrain, int ndisabled, const int* dis_a, const int* dis_b, double margin,
+                          int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA, double* angB,
+                          double* point, double* depth, int* contact_off) {
+    if (nscenes < 1 || !body_off || nw < 0 || ndisabled < 0 || (ndisabled > 0 && (!dis_a || !dis_b))) return -1;
+    // pairs that never collide (CollisionCoal::disablePair / isPairDisabled, collision_coal.cpp:558-570)
+    std::vector<uint64_t> disabled(ndisabled);
+    for (int k = 0; k < ndisabled; ++k) {
+        const uint64_t lo = (uint32_t)std::min(dis_a[k], dis_b[k]), hi = (uint32_t)std::max(dis_a[k], dis_b[k]);
+        disabled[k] = (lo << 32) | hi;
+    }
+    std::sort(disabled.begin(), disabled.end());
+    auto is_disabled = [&](int a, int b) {
+        return !disabled.empty() &&
+               std::binary_search(disabled.begin(), disabled.end(), ((uint64_t)(uint32_t)a << 32) | (uint32_t)b);
+    };
     const int nb = body_off[nscenes];
     for (int i = 0; i < nb; ++i)
-        if (shape[i] != 0 && shape[i] != 1) return -1;
+        if (shape[i] != 0 && shape[i] != 1 && shape[i] != 2) return -1;
+    HField hf;
+    std::vector<double> mu_static(mu_wall, mu_wall + nw);
+    if (terrain) {
+        if (terrain->nx < 2 || terrain->ny < 2 || !(terrain->dx > 0) || !(terrain->dy > 0) || !terrain->h) return -1;
+        hf.nx = terrain->nx, hf.ny = terrain->ny, hf.x0 = terrain->x0, hf.y0 = terrain->y0;
+        hf.dx = terrain->dx, hf.dy = terrain->dy, hf.h = terrain->h;
+        hf.wall = nw;
+        mu_static.push_back(terrain->mu);
+    }
     std::vector<Body> bodies(nb);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nb; ++i) bodies[i] = make_body(q + 7 * (size_t)i, shape[i], dims + 3 * (size_t)i);
@@ -507,12 +764,14 @@ int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const 
                 for (int w = 0; w < nw; ++w)
                     wall_contacts(a, w, bodies[a], V3{wall_n[3 * w], wall_n[3 * w + 1], wall_n[3 * w + 2]}, wall_d[w], margin,
                                   out);
+                if (hf.h) hf_contacts(a, bodies[a], hf, margin, out);
                 nbrs.clear();
                 g.around(&cen[3 * (a - b0)], [&](int j) {
                     if (j + b0 > a) nbrs.push_back(j + b0);
                 });
                 std::sort(nbrs.begin(), nbrs.end());
-                for (int b : nbrs) pair_contacts(a, b, bodies[a], bodies[b], margin, out);
+                for (int b : nbrs)
+                    if (!is_disabled(a, b)) pair_contacts(a, b, bodies[a], bodies[b], margin, out);
             }
         }
     }
@@ -525,7 +784,8 @@ int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const 
 #pragma omp parallel for schedule(dynamic, 16)
     for (int s = 0; s < nscenes; ++s)
         for (size_t k = 0; k < per[s].size(); ++k)
-            fill_contact(per[s][k], q, mu_body, mu_wall, off[s] + (int)k, body_a, body_b, mu, frame, angA, angB, point, depth);
+            fill_contact(per[s][k], q, mu_body, mu_static.data(), off[s] + (int)k, body_a, body_b, mu, frame, angA, angB, point,
+                         depth);
     return total;
 }
 

@@ -54,6 +54,31 @@ int cdx_shape_contacts(int nscenes, const int* body_off, const double* q, const 
                        double margin, int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA,
                        double* angB, double* point, double* depth, int* contact_off);
 
+/* A static heightfield shared by all scenes: samples h[i + nx*j] at (x0 + i*dx, y0 + j*dy),
+ * nx, ny >= 2; each cell is split along its (i+1, j)-(i, j+1) diagonal into two triangles;
+ * no terrain outside the sampled rectangle. */
+typedef struct {
+    int nx, ny;
+    double x0, y0, dx, dy;
+    const double* h;
+    double mu; /* friction of the terrain (combined with the body's by "min") */
+} cdx_heightfield;
+
+/* cdx_shape_contacts with capsules (shape 2: dims[3b] = radius, dims[3b+1] = half length of the
+ * segment along the body z axis) and an optional heightfield (NULL: none).  Capsule pairs:
+ * wall -- the two segment ends; sphere -- nearest segment point; capsule -- closest points of
+ * the two segments (the two ends of the overlap when parallel); box -- sphere probes at the
+ * ends and at the segment point nearest the box centre.  Heightfield probes (one contact each,
+ * terrain = side A): sphere centre, capsule ends and midpoint, box corners.  Pairs listed in
+ * (dis_a[k], dis_b[k]), k < ndisabled, never collide (CollisionCoal::disablePair,
+ * collision_coal.cpp:558-570; e.g. bodies sharing a joint).  Per body a the order is walls,
+ * heightfield, then pairs (a, b > a). */
+int cdx_shape_contacts_hf(int nscenes, const int* body_off, const double* q, const int* shape, const double* dims,
+                          const double* mu_body, int nw, const double* wall_n, const double* wall_d, const double* mu_wall,
+                          const cdx_heightfield* terrain, int ndisabled, const int* dis_a, const int* dis_b, double margin,
+                          int cap, int* body_a, int* body_b, double* mu, double* frame, double* angA, double* angB,
+                          double* point, double* depth, int* contact_off);
+
 /* Random sequential placement of bounding spheres (radius bound[b]) without overlap inside the
  * box [lo, hi] of each scene, largest first; a scene's column grows upward when the box is too
  * full.  pos[3*nb]. */

@@ -141,6 +141,12 @@ def lib():
     return _lib
 
 
+class Heightfield(C.Structure):
+    """cdx_heightfield (cdx_scene.h): samples h[i + nx*j] at (x0 + i*dx, y0 + j*dy)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("x0", C.c_double), ("y0", C.c_double), ("dx", C.c_double),
+                ("dy", C.c_double), ("h", _dp), ("mu", C.c_double)]
+
+
 def scene_lib():
     global _scene
     if _scene is None:
@@ -155,6 +161,10 @@ def scene_lib():
         _scene.cdx_shape_contacts.restype = C.c_int
         _scene.cdx_shape_contacts.argtypes = [C.c_int, _ip, _dp, _ip, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
                                               _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        _scene.cdx_shape_contacts_hf.restype = C.c_int
+        _scene.cdx_shape_contacts_hf.argtypes = [C.c_int, _ip, _dp, _ip, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                                 C.POINTER(Heightfield), C.c_int, _ip, _ip, C.c_double, C.c_int, _ip, _ip,
+                                                 _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         _scene.cdx_scene_rsa.restype = C.c_int
         _scene.cdx_scene_rsa.argtypes = [C.c_int, _ip, _dp, _dp, _dp, C.c_uint64, _dp]
         _scene.cdx_match_warmstart.restype = C.c_int

@@ -13,6 +13,7 @@ Scenes (seeded, deterministic):
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 import math
 
@@ -186,6 +187,8 @@ class SphereScene:
     mu_body: np.ndarray
     walls: tuple
     margin: float = 0.0
+    terrain: dict | None = None  # shared static heightfield (fractal_heightfield)
+    disabled: tuple | None = None  # (a, b) body pairs that never collide (e.g. jointed links)
 
     def bodies(self, v=None):
         return sphere_bodies(self.radius, self.mass, self.q, v)
@@ -329,23 +332,46 @@ def split_velocities(problems, v):
 # spheres and boxes in many independent scenes
 # --------------------------------------------------------------------------------------------
 
-SPHERE, BOX = 0, 1
+SPHERE, BOX, CAPSULE = 0, 1, 2
+
+
+def capsule_unit_inertia(radius, half_length):
+    """Inertia per unit mass of a capsule about its centre (axis = body z): the reference divides
+    Coal's Capsule::computeMomentofInertia() (unit density) by the volume
+    (rigid_body_factory.cpp:76-79; Coal v3.0.4, the cylinder plus two hemispheres by the
+    parallel-axis theorem)."""
+    r = np.asarray(radius, dtype=np.float64)
+    h = np.asarray(half_length, dtype=np.float64)
+    v_cyl = math.pi * r * r * (2.0 * h)
+    v_sph = 4.0 / 3.0 * math.pi * r ** 3
+    ix = v_cyl * (h * h / 3.0 + r * r / 4.0) + v_sph * (0.4 * r * r + h * h + 0.75 * r * h)
+    iz = (0.5 * v_cyl + 0.4 * v_sph) * r * r
+    vol = v_cyl + v_sph
+    return np.stack([ix / vol, ix / vol, iz / vol], axis=-1)
+
+
+def capsule_volume(radius, half_length):
+    return math.pi * radius * radius * (2.0 * half_length) + 4.0 / 3.0 * math.pi * radius ** 3
 
 
 def shape_bodies(shape, dims, mass, q, v=None):
-    """Rigid spheres/boxes: sphere I = 2/5 m r^2, box I = m (h_j^2 + h_k^2) / 3
-    (computeUnitInertia, rigid_body_factory.cpp:59-91)."""
+    """Rigid spheres/boxes/capsules: sphere I = 2/5 m r^2, box I = m (h_j^2 + h_k^2) / 3, capsule
+    from capsule_unit_inertia (computeUnitInertia, rigid_body_factory.cpp:59-91)."""
     shape = np.asarray(shape)
     dims = np.asarray(dims, dtype=np.float64).reshape(-1, 3)
     mass = np.asarray(mass, dtype=np.float64)
     nb = len(shape)
     I = np.empty((nb, 3))
     sph = shape == SPHERE
+    box = shape == BOX
+    cap = shape == CAPSULE
     I[sph] = (0.4 * mass[sph] * dims[sph, 0] ** 2)[:, None]
     h2 = dims ** 2
-    I[~sph, 0] = mass[~sph] * (h2[~sph, 1] + h2[~sph, 2]) / 3.0
-    I[~sph, 1] = mass[~sph] * (h2[~sph, 0] + h2[~sph, 2]) / 3.0
-    I[~sph, 2] = mass[~sph] * (h2[~sph, 0] + h2[~sph, 1]) / 3.0
+    I[box, 0] = mass[box] * (h2[box, 1] + h2[box, 2]) / 3.0
+    I[box, 1] = mass[box] * (h2[box, 0] + h2[box, 2]) / 3.0
+    I[box, 2] = mass[box] * (h2[box, 0] + h2[box, 1]) / 3.0
+    if cap.any():
+        I[cap] = mass[cap, None] * capsule_unit_inertia(dims[cap, 0], dims[cap, 1])
     minv = np.concatenate([np.repeat((1.0 / mass)[:, None], 3, axis=1), 1.0 / I], axis=1)
     fext = np.zeros((nb, 6))
     fext[:, :3] = mass[:, None] * GRAVITY[None, :]
@@ -356,9 +382,23 @@ def shape_bodies(shape, dims, mass, q, v=None):
                 q=np.ascontiguousarray(q, dtype=np.float64).reshape(-1))
 
 
-def shape_contacts(body_off, q, shape, dims, mu_body, walls, margin=0.0, friction=True):
-    """CPU contacts for spheres/boxes in independent scenes (libcdx_scene cdx_shape_contacts)."""
+def _terrain_struct(terrain):
+    if terrain is None:
+        return None, None
+    h = np.ascontiguousarray(terrain["h"], dtype=np.float64).reshape(-1)
+    t = native.Heightfield(int(terrain["nx"]), int(terrain["ny"]), float(terrain["x0"]), float(terrain["y0"]),
+                           float(terrain["dx"]), float(terrain["dy"]), native.dptr(h), float(terrain["mu"]))
+    return t, h
+
+
+def shape_contacts(body_off, q, shape, dims, mu_body, walls, margin=0.0, friction=True, terrain=None, disabled=None):
+    """CPU contacts for spheres/boxes/capsules in independent scenes, optionally on a shared
+    heightfield, skipping the body pairs in `disabled` ((a, b) arrays) (libcdx_scene
+    cdx_shape_contacts_hf)."""
     L = native.scene_lib()
+    hf, hf_keep = _terrain_struct(terrain)
+    da, db = (np.zeros(0, np.int32), np.zeros(0, np.int32)) if disabled is None else \
+        (np.ascontiguousarray(disabled[0], np.int32), np.ascontiguousarray(disabled[1], np.int32))
     body_off = np.ascontiguousarray(body_off, np.int32)
     q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
     shape = np.ascontiguousarray(shape, np.int32)
@@ -372,13 +412,14 @@ def shape_contacts(body_off, q, shape, dims, mu_body, walls, margin=0.0, frictio
         ia = np.empty(cap, np.int32); ib = np.empty(cap, np.int32)
         mu = np.empty(cap); frame = np.empty((cap, 9)); aA = np.empty((cap, 9)); aB = np.empty((cap, 9))
         pt = np.empty((cap, 3)); dep = np.empty(cap)
-        n = L.cdx_shape_contacts(ns, native.iptr(body_off), native.dptr(q), native.iptr(shape), native.dptr(dims),
-                                 native.dptr(mu_body), len(wd), native.dptr(wn.reshape(-1)), native.dptr(wd),
-                                 native.dptr(wmu), margin, cap, native.iptr(ia), native.iptr(ib), native.dptr(mu),
-                                 native.dptr(frame), native.dptr(aA), native.dptr(aB), native.dptr(pt), native.dptr(dep),
-                                 native.iptr(coff))
+        n = L.cdx_shape_contacts_hf(ns, native.iptr(body_off), native.dptr(q), native.iptr(shape), native.dptr(dims),
+                                    native.dptr(mu_body), len(wd), native.dptr(wn.reshape(-1)), native.dptr(wd),
+                                    native.dptr(wmu), None if hf is None else C.byref(hf), len(da), native.iptr(da),
+                                    native.iptr(db), margin, cap, native.iptr(ia),
+                                    native.iptr(ib), native.dptr(mu), native.dptr(frame), native.dptr(aA), native.dptr(aB),
+                                    native.dptr(pt), native.dptr(dep), native.iptr(coff))
         if n < 0:
-            raise RuntimeError("cdx_shape_contacts failed")
+            raise RuntimeError("cdx_shape_contacts_hf failed")
         if n <= cap:
             break
         cap = n
@@ -401,6 +442,8 @@ class ShapeBatch:
     v: np.ndarray  # [nb, 6]
     walls: tuple
     margin: float = 0.0
+    terrain: dict | None = None  # shared static heightfield (fractal_heightfield)
+    disabled: tuple | None = None  # (a, b) body pairs that never collide (e.g. jointed links)
 
     @property
     def nscenes(self):
@@ -414,15 +457,22 @@ class ShapeBatch:
         b0, b1 = int(self.body_off[s]), int(self.body_off[s + 1])
         return ShapeBatch(body_off=np.array([0, b1 - b0], np.int32), shape=self.shape[b0:b1].copy(),
                           dims=self.dims[b0:b1].copy(), mass=self.mass[b0:b1].copy(), mu_body=self.mu_body[b0:b1].copy(),
-                          q=self.q[b0:b1].copy(), v=self.v[b0:b1].copy(), walls=self.walls, margin=self.margin)
+                          q=self.q[b0:b1].copy(), v=self.v[b0:b1].copy(), walls=self.walls, margin=self.margin,
+                          terrain=self.terrain, disabled=None if self.disabled is None else _scene_pairs(self.disabled, b0, b1))
 
     def contacts(self, q=None, friction=True):
         return shape_contacts(self.body_off, self.q if q is None else q, self.shape, self.dims, self.mu_body, self.walls,
-                              self.margin, friction)
+                              self.margin, friction, self.terrain, self.disabled)
 
     def problem(self, friction=True):
         c = self.contacts(friction=friction)
         return make_problem(self.bodies(), c), c["contact_off"]
+
+
+def _scene_pairs(pairs, b0, b1):
+    a, b = np.asarray(pairs[0]), np.asarray(pairs[1])
+    keep = (a >= b0) & (a < b1) & (b >= b0) & (b < b1)
+    return (a[keep] - b0).astype(np.int32), (b[keep] - b0).astype(np.int32)
 
 
 def rsa(body_off, bound, lo, hi, seed):
@@ -480,3 +530,83 @@ def boxes_on_plane(nscenes=256, per_scene=64, seed=0, edge=0.1, mass=1.0, vstd=0
     return ShapeBatch(body_off=(np.arange(nscenes + 1) * per_scene).astype(np.int32), shape=np.full(nb, BOX, np.int32),
                       dims=np.full((nb, 3), 0.5 * edge), mass=np.full(nb, float(mass)), mu_body=rng.uniform(0.2, 0.9, nb),
                       q=q, v=v, walls=walls, margin=1e-4)
+
+
+def fractal_heightfield(n=256, extent=6.4, amp=0.1, octaves=6, seed=0, mu=0.5):
+    """Seeded fractal value noise on an n x n grid over [0, extent]^2: octave k is a random
+    lattice of 2^(k+2) + 1 values bilinearly interpolated, weighted 0.5^k; scaled so that
+    max |h| = amp."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, n)
+    h = np.zeros((n, n))
+    for k in range(octaves):
+        m = 2 ** (k + 2) + 1
+        lat = rng.uniform(-1.0, 1.0, (m, m))
+        x = u * (m - 1)
+        i = np.minimum(np.floor(x).astype(int), m - 2)
+        f = x - i
+        rows = lat[i] * (1 - f)[:, None] + lat[i + 1] * f[:, None]            # interpolate along the first axis
+        h += 0.5 ** k * (rows[:, i] * (1 - f)[None, :] + rows[:, i + 1] * f[None, :])
+    h *= amp / max(np.abs(h).max(), 1e-300)
+    dx = extent / (n - 1)
+    # h[i + n j] at (x0 + i dx, y0 + j dy): row index j (y), column index i (x)
+    return dict(nx=n, ny=n, x0=0.0, y0=0.0, dx=dx, dy=dx, h=np.ascontiguousarray(h.T.reshape(-1)), mu=mu)
+
+
+def terrain_height(terrain, x, y):
+    """Height of the triangulated heightfield at (x, y) (same split as the contact generator)."""
+    fx = (np.asarray(x) - terrain["x0"]) / terrain["dx"]
+    fy = (np.asarray(y) - terrain["y0"]) / terrain["dy"]
+    nx, ny = terrain["nx"], terrain["ny"]
+    i = np.clip(np.floor(fx).astype(int), 0, nx - 2)
+    j = np.clip(np.floor(fy).astype(int), 0, ny - 2)
+    a, b = fx - i, fy - j
+    H = terrain["h"]
+    h00, h10, h01, h11 = H[i + nx * j], H[i + 1 + nx * j], H[i + nx * (j + 1)], H[i + 1 + nx * (j + 1)]
+    lower = a + b <= 1.0
+    return np.where(lower, h00 + a * (h10 - h00) + b * (h01 - h00),
+                    h11 + (1 - a) * (h01 - h11) + (1 - b) * (h10 - h11))
+
+
+def capsule_chains(nchains=2048, links=32, radius=0.02, length=0.1, density=1000.0, terrain=None, drop=0.15, seed=0,
+                   mu=0.5):
+    """Chains of `links` capsules (total length `length` including the caps, radius `radius`),
+    one chain per scene, laid out tip to tip along a random horizontal direction `drop` above the
+    highest terrain point, over a fractal heightfield shared by all scenes.  Returns the batch and
+    the spherical joints between consecutive links: (body_a, body_b, anchor_a, anchor_b) with
+    the anchors at the touching tips in body coordinates."""
+    terrain = terrain if terrain is not None else fractal_heightfield(seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    hl = 0.5 * length - radius
+    nb = nchains * links
+    extent = terrain["dx"] * (terrain["nx"] - 1)
+    span = links * length
+    q = np.zeros((nb, 7))
+    top = float(np.max(terrain["h"])) + radius + drop
+    for s in range(nchains):
+        ang = rng.uniform(0, 2 * math.pi)
+        d = np.array([math.cos(ang), math.sin(ang), 0.0])
+        c = rng.uniform(0.5 * span + 0.05, extent - 0.5 * span - 0.05, 2) if extent > span + 0.1 else \
+            np.full(2, 0.5 * extent)
+        start = np.array([c[0], c[1], top]) - 0.5 * span * d
+        # body z axis along d: rotate e_z by 90 degrees about e_z x d
+        axis = np.cross([0.0, 0.0, 1.0], d)
+        axis /= np.linalg.norm(axis)
+        quat = np.concatenate([axis * math.sin(math.pi / 4), [math.cos(math.pi / 4)]])
+        for k in range(links):
+            b = s * links + k
+            q[b, :3] = start + (k + 0.5) * length * d
+            q[b, 3:] = quat
+    dims = np.zeros((nb, 3))
+    dims[:, 0] = radius
+    dims[:, 1] = hl
+    mass = np.full(nb, density * capsule_volume(radius, hl))
+    batch = ShapeBatch(body_off=(np.arange(nchains + 1) * links).astype(np.int32), shape=np.full(nb, CAPSULE, np.int32),
+                       dims=dims, mass=mass, mu_body=np.full(nb, mu), q=q, v=np.zeros((nb, 6)),
+                       walls=(np.zeros((0, 3)), np.zeros(0), np.zeros(0)), margin=1e-3, terrain=terrain)
+    ja = np.concatenate([np.arange(s * links, (s + 1) * links - 1) for s in range(nchains)]).astype(np.int32)
+    jb = ja + 1
+    batch.disabled = (ja, jb)  # jointed links do not collide (beam_factory.cpp:235)
+    anchor_a = np.tile([0.0, 0.0, hl + radius], (len(ja), 1))
+    anchor_b = np.tile([0.0, 0.0, -(hl + radius)], (len(ja), 1))
+    return batch, dict(body_a=ja, body_b=jb, anchor_a=anchor_a, anchor_b=anchor_b)

@@ -98,3 +98,149 @@ def test_batched_drop_manifolds_bounded():
     box = lambda i: i >= 0 and b.shape[i] == scenes.BOX
     assert max(v for (a, bb), v in pairs.items() if box(a) and box(bb)) <= 4
     assert np.isfinite(c["frame"]).all() and np.isfinite(c["angA"]).all() and np.isfinite(c["angB"]).all()
+
+
+# ---- capsules and heightfields ------------------------------------------------------------
+
+def quat_y(angle):
+    return [0.0, math.sin(angle / 2), 0.0, math.cos(angle / 2)]
+
+
+def test_capsule_inertia_limits_and_integration():
+    r, h = 0.02, 0.03
+    # a capsule of zero length is a sphere; a vanishing radius leaves a rod of length 2h
+    np.testing.assert_allclose(scenes.capsule_unit_inertia(r, 0.0), [0.4 * r * r] * 3, rtol=1e-14)
+    np.testing.assert_allclose(scenes.capsule_unit_inertia(1e-9, h)[:2], [(2 * h) ** 2 / 12] * 2, rtol=1e-6)
+    # midpoint-rule volume integral on a fine grid
+    n = 160
+    x = (np.arange(n) + 0.5) / n * 2 * r - r
+    z = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * (h + r) - (h + r)
+    X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+    dz = np.clip(np.abs(Z) - h, 0.0, None)
+    inside = X ** 2 + Y ** 2 + dz ** 2 <= r * r
+    m = inside.sum()
+    Ix = ((Y ** 2 + Z ** 2) * inside).sum() / m
+    Iz = ((X ** 2 + Y ** 2) * inside).sum() / m
+    np.testing.assert_allclose(scenes.capsule_unit_inertia(r, h), [Ix, Ix, Iz], rtol=2e-3)
+    assert m * (2 * r / n) ** 2 * (2 * (h + r) / (2 * n)) == pytest.approx(scenes.capsule_volume(r, h), rel=2e-3)
+
+
+def capsule_batch(qs, r=0.02, hl=0.03, walls=None, terrain=None, shapes=None, dims=None):
+    nb = len(qs)
+    shape = np.array(shapes if shapes is not None else [scenes.CAPSULE] * nb, np.int32)
+    d = np.array(dims if dims is not None else [[r, hl, 0.0]] * nb, float)
+    w = walls if walls is not None else (np.zeros((0, 3)), np.zeros(0), np.zeros(0))
+    return scenes.ShapeBatch(body_off=np.array([0, nb], np.int32), shape=shape, dims=d, mass=np.ones(nb),
+                             mu_body=np.full(nb, 0.5), q=np.array(qs, float), v=np.zeros((nb, 6)), walls=w,
+                             margin=1e-4, terrain=terrain)
+
+
+def test_capsule_lying_on_plane_two_end_contacts():
+    r, hl, sink = 0.02, 0.03, 1e-4
+    q = [0, 0, r - sink] + quat_y(math.pi / 2)  # axis along world x
+    b = capsule_batch([q], walls=(np.array([[0, 0, 1.0]]), np.zeros(1), np.full(1, 0.4)))
+    c = b.contacts()
+    assert len(c["body_a"]) == 2 and (c["body_a"] == -1).all()
+    check_conventions(c, b.q)
+    np.testing.assert_allclose(np.sort(c["point"][:, 0]), [-hl, hl], atol=1e-12)
+    np.testing.assert_allclose(c["frame"][:, :3], [[0, 0, 1]] * 2, atol=1e-15)
+    np.testing.assert_allclose(c["depth"], sink, rtol=1e-9)
+    assert (c["mu"] == 0.4).all()
+
+
+def test_capsule_capsule_crossing_and_parallel():
+    r, hl = 0.02, 0.03
+    # crossing at right angles, one on top of the other, 1e-4 overlap
+    qa = [0, 0, 0] + quat_y(math.pi / 2)  # along x
+    qb = [0.01, 0.005, 2 * r - 1e-4] + [math.sin(math.pi / 4), 0, 0, math.cos(math.pi / 4)]  # along y
+    c = capsule_batch([qa, qb]).contacts()
+    assert len(c["body_a"]) == 1
+    check_conventions(c, np.array([qa, qb], float))
+    np.testing.assert_allclose(c["frame"][0, :3], [0, 0, 1], atol=1e-12)
+    # closest points of the two axes: (0.01, 0, 0) on A, (0.01, 0, 2r - 1e-4) on B
+    np.testing.assert_allclose(c["point"][0], [0.01, 0.0, r - 5e-5], atol=1e-12)
+    assert c["depth"][0] == pytest.approx(1e-4, rel=1e-9)
+    # parallel, half overlapping along the axis: two contacts at the ends of the overlap
+    qb = [hl, 0, 2 * r - 1e-4] + quat_y(math.pi / 2)
+    c = capsule_batch([qa, qb]).contacts()
+    assert len(c["body_a"]) == 2
+    np.testing.assert_allclose(np.sort(c["point"][:, 0]), [0.0, hl], atol=1e-12)
+    np.testing.assert_allclose(c["depth"], 1e-4, rtol=1e-9)
+
+
+def test_capsule_sphere_and_capsule_box():
+    r, hl = 0.02, 0.03
+    qa = [0, 0, 0] + quat_y(math.pi / 2)
+    qs = [0.01, 0.0, r + 0.05 - 2e-4, 0, 0, 0, 1]
+    c = capsule_batch([qa, qs], shapes=[scenes.CAPSULE, scenes.SPHERE], dims=[[r, hl, 0], [0.05, 0, 0]]).contacts()
+    assert len(c["body_a"]) == 1 and c["body_a"][0] == 0 and c["body_b"][0] == 1
+    np.testing.assert_allclose(c["frame"][0, :3], [0, 0, 1], atol=1e-12)
+    assert c["depth"][0] == pytest.approx(2e-4, rel=1e-9)
+    # a capsule resting across the top face of a box: both end probes and the middle probe touch
+    qbox = [0, 0, -0.1, 0, 0, 0, 1]
+    qc = [0, 0, r - 1e-4] + quat_y(math.pi / 2)
+    c = capsule_batch([qbox, qc], shapes=[scenes.BOX, scenes.CAPSULE], dims=[[0.1, 0.1, 0.1], [r, hl, 0]]).contacts()
+    assert len(c["body_a"]) == 3 and (c["body_a"] == 0).all() and (c["body_b"] == 1).all()
+    check_conventions(c, np.array([qbox, qc], float))
+    np.testing.assert_allclose(np.sort(c["point"][:, 0]), [-hl, 0.0, hl], atol=1e-12)
+    np.testing.assert_allclose(c["frame"][:, :3], [[0, 0, 1]] * 3, atol=1e-12)
+    np.testing.assert_allclose(c["depth"], 1e-4, rtol=1e-6)
+    # standing on its end, only the lower end probe touches
+    qc = [0, 0, hl + r - 1e-4, 0, 0, 0, 1]
+    c = capsule_batch([qbox, qc], shapes=[scenes.BOX, scenes.CAPSULE], dims=[[0.1, 0.1, 0.1], [r, hl, 0]]).contacts()
+    assert len(c["body_a"]) == 1 and c["point"][0, 2] == pytest.approx(-5e-5, abs=1e-12)
+
+
+def plane_terrain(a, b, c0, n=33, extent=1.0, mu=0.3):
+    x = np.linspace(0, extent, n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    h = a * X + b * Y + c0
+    return dict(nx=n, ny=n, x0=0.0, y0=0.0, dx=extent / (n - 1), dy=extent / (n - 1),
+                h=np.ascontiguousarray(h.T.reshape(-1)), mu=mu)
+
+
+def test_heightfield_plane_closed_form():
+    """On a planar heightfield h = a x + b y + c the contact normal is the plane normal and the
+    depth the sphere's signed distance to the plane, above and below the surface."""
+    a, bb, c0, R = 0.2, -0.1, 0.05, 0.03
+    t = plane_terrain(a, bb, c0)
+    nrm = np.array([-a, -bb, 1.0]) / math.sqrt(a * a + bb * bb + 1)
+    for sink in (1e-4, -5e-5, 0.02):  # penetrating, within the margin, centre below the surface
+        x, y = 0.41, 0.37
+        ctr = np.array([x, y, a * x + bb * y + c0]) + (R - sink) * nrm
+        b = capsule_batch([list(ctr) + [0, 0, 0, 1]], shapes=[scenes.SPHERE], dims=[[R, 0, 0]], terrain=t)
+        c = b.contacts()
+        assert len(c["body_a"]) == 1 and c["body_a"][0] == -1 and c["mu"][0] == pytest.approx(0.3)
+        check_conventions(c, b.q)
+        np.testing.assert_allclose(c["frame"][0, :3], nrm, atol=1e-12)
+        assert c["depth"][0] == pytest.approx(max(sink, 0.0), abs=1e-12)
+    np.testing.assert_allclose(scenes.terrain_height(t, [0.41, 0.9], [0.37, 0.1]),
+                               [a * 0.41 + bb * 0.37 + c0, a * 0.9 + bb * 0.1 + c0], atol=1e-14)
+    # outside the sampled rectangle there is no terrain
+    b = capsule_batch([[1.5, 0.5, 0.0, 0, 0, 0, 1]], shapes=[scenes.SPHERE], dims=[[R, 0, 0]], terrain=t)
+    assert len(b.contacts()["body_a"]) == 0
+
+
+def test_capsule_chain_scene():
+    t = scenes.fractal_heightfield(n=64, extent=6.4, amp=0.1, seed=2)
+    assert np.abs(t["h"]).max() == pytest.approx(0.1)
+    batch, joints = scenes.capsule_chains(nchains=6, links=8, terrain=t, seed=1)
+    assert batch.nscenes == 6 and len(joints["body_a"]) == 6 * 7
+    q = batch.q
+
+    def world(b, anchor):
+        x, y, z, w = q[b, 3:]
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        return q[b, :3] + R @ anchor
+
+    for ja, jb, aa, ab in zip(joints["body_a"], joints["body_b"], joints["anchor_a"], joints["anchor_b"]):
+        np.testing.assert_allclose(world(ja, aa), world(jb, ab), atol=1e-12)
+    # released above the terrain: nothing touches yet; lowered onto it, every capsule does
+    assert len(batch.contacts()["body_a"]) == 0
+    low = q.copy()
+    low[:, 2] = scenes.terrain_height(t, low[:, 0], low[:, 1]) + 0.01
+    c = batch.contacts(q=low)
+    assert set(np.unique(c["body_b"])) == set(range(len(q)))
+    assert np.isfinite(c["frame"]).all()
