@@ -207,7 +207,7 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void* bufs[] = {ctx->d_x, ctx->d_v, ctx->d_minv, ctx->d_fext, ctx->d_fgyro, ctx->d_q, ctx->d_uprev, ctx->d_ufree, ctx->d_dof,
                     ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_p, ctx->d_R, ctx->d_bias,
-                    ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_color_ptr, ctx->d_ctrl, ctx->d_res, ctx->d_partial,
+                    ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial,
                     ctx->d_partial2, ctx->d_pk, ctx->d_nest, ctx->d_scene_i, ctx->d_scene_res,
                     ctx->d_rec_x, ctx->d_rec_p, ctx->d_adj, ctx->d_adj_imp, ctx->d_adj_body, ctx->d_adj_partial};
     for (void* p : bufs)

@@ -89,10 +89,6 @@ struct cdx_ctx {
     int ncolors = 0;
     std::vector<int> color_ptr;  // [ncolors+1] into the colour-sorted block list
     int* d_color_blocks = nullptr;  // [nblocks] contact index per colour-sorted block
-    int* d_color_ptr = nullptr;     // device copy of color_ptr
-    size_t color_ptr_cap = 0;
-    int gs_coop_blocks = 0;         // co-resident workgroups of the persistent Gauss-Seidel kernel
-    int max_color = 0;              // largest colour class
     size_t color_cap = 0;
 
     // ---- batched scenes (cdx_set_scenes); cleared whenever bodies or contacts are set

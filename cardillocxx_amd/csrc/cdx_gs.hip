@@ -10,8 +10,6 @@
 // colour is one kernel launch with one lane per block and no write conflicts.  Within a colour
 // the visiting order does not change the result.  The residual is evaluated after every sweep
 // (:124-141); NaN/Inf ends the solve with CDX_DIVERGED.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -90,10 +88,6 @@ static int build_coloring(cdx_ctx* ctx) {
     for (int k = 0; k < n; ++k) sorted[f[color_of[k]]++] = cidx[k];
     CDX_TRY(ensure_alloc(ctx, &ctx->d_color_blocks, (size_t)n, &ctx->color_cap, "colour classes"));
     if (n) CDX_HIP(ctx, hipMemcpy(ctx->d_color_blocks, sorted.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_color_ptr, (size_t)ncol + 1, &ctx->color_ptr_cap, "colour offsets"));
-    CDX_HIP(ctx, hipMemcpy(ctx->d_color_ptr, ptr.data(), sizeof(int) * (ncol + 1), hipMemcpyHostToDevice));
-    ctx->max_color = 0;
-    for (int c = 0; c < ncol; ++c) ctx->max_color = std::max(ctx->max_color, ptr[c + 1] - ptr[c]);
     ctx->color_ptr = ptr;
     ctx->ncolors = ncol;
     ctx->coloring_valid = true;
@@ -322,66 +316,6 @@ __global__ void __launch_bounds__(kBlock) k_gs_residual(int nb, const int* __res
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// The whole coloured sweep loop in one cooperative launch: the grid strides over each colour class
-// and meets at a grid barrier before the next colour (a colour's updates must land before the next
-// colour reads them), then computes the residual partials; after one more barrier every workgroup
-// sums the partials in the same order, so all of them take the same stop decision and leave the
-// loop together (converged, diverged, or max_iter).  One launch per solve replaces
-// (colours + 2) launches per sweep, whose small grids left the device idle between them.
-__global__ void __launch_bounds__(kBlock) k_gs_persistent(
-    int ncolors, const int* __restrict__ cptr, const int* __restrict__ blocks, const int* __restrict__ ida,
-    const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ Dinv,
-    const double* __restrict__ rhs, const double* __restrict__ mu, const double* __restrict__ minv, int nc, double relax,
-    double alpha, double* __restrict__ lam, double* __restrict__ uc, int nb, const int* __restrict__ dof,
-    const double* __restrict__ uf, double* __restrict__ up, double tol_rel, double tol_abs, int nv, int max_iter,
-    double* __restrict__ partial, int* __restrict__ ctrl, double* __restrict__ res, int* __restrict__ hflag) {
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    __shared__ double red[kBlock / 64];
-    __shared__ int stop_s;
-    const int gthreads = gridDim.x * kBlock;
-    const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    for (int it = 0; it < max_iter; ++it) {
-        for (int col = 0; col < ncolors; ++col) {
-            const int c1 = cptr[col + 1];
-            for (int i = cptr[col] + gtid; i < c1; i += gthreads)
-                gs_block_update(blocks[i], ida, idb, J, Dinv, rhs, mu, minv, nc, relax, alpha, lam, uc);
-            grid.sync();
-        }
-        double e2 = 0;
-        for (int b = gtid; b < nb; b += gthreads) e2 += gs_body_residual(b, dof, uf, uc, up, tol_rel, tol_abs);
-        const double s = block_sum(e2, red);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
-        grid.sync();
-        // partial[] is written again only after the next sweep's colour barriers, which every
-        // workgroup reaches after this read
-        double t = 0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) t += partial[i];
-        t = block_sum(t, red);
-        if (threadIdx.x == 0) {
-            const double r = 1.0 / sqrt((double)nv) * sqrt(t);
-            int stop = 0, status = 0;
-            if (isnan(r) || isinf(r)) {
-                stop = 1;
-                status = CDX_DIVERGED;
-            } else if (r < 1.0) {
-                stop = 1;
-            }
-            if (blockIdx.x == 0) {
-                res[0] = r;
-                if (stop) {
-                    ctrl[1] = it + 1;
-                    ctrl[2] = status;
-                    ctrl[0] = 1;
-                    hflag[0] = 1;
-                }
-            }
-            stop_s = stop;
-        }
-        __syncthreads();
-        if (stop_s) break;
-    }
-}
-
 // v = u_free - u_corr (:267) and canonical impulses (normal sign flipped back, :265).
 __global__ void k_gs_finish_bodies(int nb, const double* __restrict__ uf, double* __restrict__ x) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,36 +338,6 @@ __global__ void k_gs_finish_contacts(int nc, double* __restrict__ lam) {
 using namespace cdx;
 
 constexpr int kGsChunk = 8;
-
-// Launches k_gs_persistent with as many workgroups as the largest colour class needs, capped at
-// the number that can be co-resident (a cooperative launch fails rather than oversubscribe).
-static int gs_persistent(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters, int* status) {
-    if (ctx->gs_coop_blocks == 0) {
-        int coop = 0, per_cu = 0, ncu = 0;
-        CDX_HIP(ctx, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
-        if (!coop) return set_err(ctx, CDX_HIP_ERROR, "cooperative launch not supported on this device");
-        CDX_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_persistent, kBlock, 0));
-        CDX_HIP(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        ctx->gs_coop_blocks = std::max(1, per_cu * ncu);
-    }
-    const int grid = std::max(1, std::min(ctx->gs_coop_blocks, (ctx->max_color + kBlock - 1) / kBlock));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)grid, &ctx->partial_cap, "residual partials"));
-    int ncolors = ctx->ncolors, nc = ctx->nc, nb = ctx->nb, nv = ctx->nv, max_iter = prm->max_iter;
-    double relax = prm->relaxation, alpha = prm->alpha, tol_rel = prm->tol_rel, tol_abs = prm->tol_abs;
-    void* args[] = {&ncolors,      &ctx->d_color_ptr, &ctx->d_color_blocks, &ctx->d_ida,   &ctx->d_idb,     &ctx->d_J,
-                    &ctx->d_R,     &ctx->d_bias,      &ctx->d_mu,           &ctx->d_minv,  &nc,             &relax,
-                    &alpha,        &ctx->d_p,         &ctx->d_x,            &nb,           &ctx->d_dof,     &ctx->d_ufree,
-                    &ctx->d_uprev, &tol_rel,          &tol_abs,             &nv,           &max_iter,       &ctx->d_partial,
-                    &ctx->d_ctrl,  &ctx->d_res,       &ctx->h_flag_dev};
-    CDX_HIP(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_gs_persistent), dim3(grid), dim3(kBlock), args, 0,
-                                            ctx->stream));
-    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int ctrl[3] = {0, 0, 0};
-    CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
-    *iters = ctrl[0] ? ctrl[1] : max_iter;
-    *status = ctrl[0] ? ctrl[2] : CDX_OK;
-    return CDX_OK;
-}
 
 extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm,
                                     cdx_solve_info* info) {
@@ -490,15 +394,11 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
         return hip_check(ctx, hipGetLastError(), "gs sweep launch");
     };
     int iters = 0;
-    int st = CDX_OK;
-    if (!ctx->profiling) {
-        // production: the whole sweep loop in one cooperative launch (residual every sweep,
-        // projected_gauss_seidel.cpp:130-140)
-        CDX_TRY(gs_persistent(ctx, prm, &iters, &st));
-    } else {
-        // per-kernel timing: one launch per colour, chunks of sweeps replayed as graphs
-        st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
-    }
+    // residual every sweep (projected_gauss_seidel.cpp:130-140); one launch per colour, chunks of
+    // 8 sweeps replayed as graphs.  (A single cooperative launch with grid barriers between the
+    // colours measured twice as slow per sweep: each device-scope barrier writes back and
+    // invalidates the L2 of every XCD, so every colour re-fetched its data.)
+    const int st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_bodies, gb, kBlock, nb, ctx->d_ufree, ctx->d_x);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_contacts, grid_for(nc), kBlock, nc, ctx->d_p);
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));

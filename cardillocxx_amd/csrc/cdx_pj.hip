@@ -158,7 +158,9 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
                                                        const int* __restrict__ ctrl) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    // XCD-aware tiles, as the contact pass: a contiguous range of bodies per XCD gathers the
+    // records its own XCD's contact tiles wrote (contacts are ordered by body), from its L2
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
     const bool live = bi < nb;
     double acc[6] = {0, 0, 0, 0, 0, 0};
