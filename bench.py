@@ -134,6 +134,28 @@ class Hip:
 # large single scene (headline)
 # --------------------------------------------------------------------------------------------
 
+def colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed):
+    """Coloured Gauss-Seidel on the pile already on the device, same tolerance (residual every sweep)."""
+    pg = native.default_params()
+    pg.tol_abs = pg.tol_rel = args.tol
+    pg.max_iter = args.gs_max_iter
+    pg.warmstart = 0
+    ctx.synchronize()
+    tg0 = time.perf_counter()
+    ginfo = ctx.gs_colored_solve(dt, 1.0, pg, raise_on_divergence=False)
+    ctx.synchronize()
+    tg = time.perf_counter() - tg0
+    vg = ctx.velocities()
+    ig = ctx.impulses()
+    ig[:, 1:] *= -1.0  # the Gauss-Seidel solver stores tangents with the opposite sign
+    log(f"colored GS: {ginfo.iters} sweeps, {ginfo.ncolors} colours, {tg:.3f}s")
+    return {"value": prob.nc * ginfo.iters / tg, "unit": "contact-updates/s", "sweeps": ginfo.iters,
+            "colors": ginfo.ncolors, "residual": ginfo.residual, "seconds_to_solution": tg,
+            "pj_seconds_to_solution": elapsed / args.steps, "complementarity": metrics.complementarity(prob, vg, ig),
+            "pj_complementarity": comp_pj, "kinetic_energy": float(0.5 * (vg ** 2 / prob.minv).sum()),
+            "pj_kinetic_energy": float(0.5 * (v ** 2 / prob.minv).sum())}
+
+
 def pile_leg(args, native, scenes, metrics):
     t0 = time.time()
     pile = scenes.sphere_pile(n=args.spheres, settle_steps=args.settle, seed=0)
@@ -201,23 +223,10 @@ def pile_leg(args, native, scenes, metrics):
     traffic, traffic_src = committed_traffic("k_pj_contacts4", prob.nc)
 
     # coloured Gauss-Seidel on the same problem and tolerance (residual every sweep)
-    pg = native.default_params()
-    pg.tol_abs = pg.tol_rel = args.tol
-    pg.max_iter = args.gs_max_iter
-    pg.warmstart = 0
-    ctx.synchronize()
-    tg0 = time.perf_counter()
-    ginfo = ctx.gs_colored_solve(dt, 1.0, pg, raise_on_divergence=False)
-    ctx.synchronize()
-    tg = time.perf_counter() - tg0
-    vg = ctx.velocities()
-    ig = ctx.impulses()
-    ig[:, 1:] *= -1.0  # the Gauss-Seidel solver stores tangents with the opposite sign
-    gs = {"value": prob.nc * ginfo.iters / tg, "unit": "contact-updates/s", "sweeps": ginfo.iters, "colors": ginfo.ncolors,
-          "residual": ginfo.residual, "seconds_to_solution": tg, "pj_seconds_to_solution": elapsed / args.steps,
-          "complementarity": metrics.complementarity(prob, vg, ig), "pj_complementarity": comp_pj,
-          "kinetic_energy": float(0.5 * (vg ** 2 / prob.minv).sum()), "pj_kinetic_energy": float(0.5 * (v ** 2 / prob.minv).sum())}
-    log(f"colored GS: {ginfo.iters} sweeps, {ginfo.ncolors} colours, {tg:.3f}s")
+    if args.no_gs:
+        gs = None
+    else:
+        gs = colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed)
 
     cpu = None
     if not args.no_cpu:
@@ -497,6 +506,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=200_000)
     ap.add_argument("--gs-max-iter", type=int, default=50_000)
+    ap.add_argument("--no-gs", action="store_true", help="skip the coloured Gauss-Seidel solve of the pile")
     ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
