@@ -466,6 +466,201 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     return ORC_OK;
 }
 
+/* ---- extended system with bilateral rows ---------------------------------------------- */
+
+typedef struct {
+    int n;
+    double* a;  /* n x n row-major, LU in place */
+    int* piv;
+} orc_lu;
+
+/* Doolittle LU with partial pivoting; returns 0 on success, -1 on a zero pivot. */
+static int lu_factor(orc_lu* f) {
+    const int n = f->n;
+    double* a = f->a;
+    for (int k = 0; k < n; ++k) {
+        int pr = k;
+        double best = fabs(a[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(a[(size_t)i * n + k]) > best) best = fabs(a[(size_t)i * n + k]), pr = i;
+        if (best == 0.0) return -1;
+        f->piv[k] = pr;
+        if (pr != k)
+            for (int j = 0; j < n; ++j) {
+                const double t = a[(size_t)k * n + j];
+                a[(size_t)k * n + j] = a[(size_t)pr * n + j];
+                a[(size_t)pr * n + j] = t;
+            }
+        const double d = a[(size_t)k * n + k];
+        for (int i = k + 1; i < n; ++i) {
+            const double l = a[(size_t)i * n + k] / d;
+            a[(size_t)i * n + k] = l;
+            if (l != 0.0)
+                for (int j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= l * a[(size_t)k * n + j];
+        }
+    }
+    return 0;
+}
+
+static void lu_solve(const orc_lu* f, const double* b, double* x) {
+    const int n = f->n;
+    const double* a = f->a;
+    memcpy(x, b, sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const int pr = f->piv[k];
+        if (pr != k) { const double t = x[k]; x[k] = x[pr]; x[pr] = t; }
+    }
+    for (int i = 1; i < n; ++i) {
+        double s = x[i];
+        for (int j = 0; j < i; ++j) s -= a[(size_t)i * n + j] * x[j];
+        x[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = x[i];
+        for (int j = i + 1; j < n; ++j) s -= a[(size_t)i * n + j] * x[j];
+        x[i] = s / a[(size_t)i * n + i];
+    }
+}
+
+/* y[r] = sum_j G_rj v_j over the body dofs of row r */
+static double joint_row_dot(const orc_joints* jt, const int* voff, const int* dof, int r, const double* v) {
+    double s = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int b = side ? jt->body_b[r] : jt->body_a[r];
+        if (b < 0) continue;
+        const double* w = (side ? jt->wb : jt->wa) + 6 * (size_t)r;
+        for (int j = 0; j < dof[b]; ++j) s += w[j] * v[voff[b] + j];
+    }
+    return s;
+}
+
+int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
+                        const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov) return ORC_BAD_ARG;
+    orc_W W;
+    int st = orc_build_W(B, C, &W);
+    if (st) { orc_free_W(&W); return st; }
+    const int nv = W.nv, nr = W.nrows, ns = jt->ns, nd = jt->nd, nj = ns + nd, N = nv + nj;
+    res->iters = 0;
+    res->residual = NAN;
+    res->nrows = nr;
+    res->ncolors = 0;
+    for (int r = 0; r < nj; ++r) {
+        const int a = jt->body_a[r], b = jt->body_b[r];
+        if (a < -1 || a >= B->nb || b < -1 || b >= B->nb || (a >= 0 && a == b)) { orc_free_W(&W); return ORC_BAD_ARG; }
+    }
+    double* M = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
+    fill_mdiag(nv, B->minv, M);
+    /* S = [M, Wg^T, Wgamma^T; Wg, -C/(theta dt^2), 0; Wgamma, 0, -A/(theta dt)] (:66-73) */
+    orc_lu lu = {N, (double*)calloc((size_t)N * N + 1, sizeof(double)), (int*)malloc(sizeof(int) * (size_t)(N + 1))};
+    for (int i = 0; i < nv; ++i) lu.a[(size_t)i * N + i] = M[i];
+    const double cScale = -1.0 / (theta * dt * dt), aScale = -1.0 / (theta * dt);
+    for (int r = 0; r < nj; ++r) {
+        const int R = nv + r;
+        for (int side = 0; side < 2; ++side) {
+            const int b = side ? jt->body_b[r] : jt->body_a[r];
+            if (b < 0) continue;
+            const double* w = (side ? jt->wb : jt->wa) + 6 * (size_t)r;
+            for (int j = 0; j < B->dof[b]; ++j) {
+                if (w[j] == 0.0) continue; /* zero entries dropped (emitColRef) */
+                lu.a[(size_t)R * N + W.voff[b] + j] += w[j];
+                lu.a[(size_t)(W.voff[b] + j) * N + R] += w[j];
+            }
+        }
+        lu.a[(size_t)R * N + R] = jt->comp[r] * (r < ns ? cScale : aScale);
+    }
+    if (lu_factor(&lu)) { free(lu.a); free(lu.piv); free(M); orc_free_W(&W); return ORC_NOT_FACTORIZED; }
+
+    /* rhs (PjAssembler::rhs, :127-170) */
+    double* rhs = (double*)calloc((size_t)N + 1, sizeof(double));
+    for (int j = 0; j < nv; ++j) rhs[j] = M[j] * B->v[j] + dt * B->fext[j];
+    for (int j = 0; j < nv; ++j) rhs[j] += dt * B->fgyro[j];
+    for (int r = 0; r < nj; ++r) {
+        const double wv = joint_row_dot(jt, W.voff, B->dof, r, B->v);
+        if (r < ns)
+            rhs[nv + r] = -(1.0 / (theta * dt * dt)) * (jt->comp[r] * jt->lam_prev[r]) - ((1.0 - theta) / theta) * wv -
+                          (1.0 / theta) * jt->vsrc[r];
+        else
+            rhs[nv + r] = -((1.0 - theta) / theta) * wv - (1.0 / theta) * jt->vsrc[r];
+    }
+    if (jt->bias_factor > 0)
+        for (int r = 0; r < ns; ++r)
+            rhs[nv + r] -= (-(jt->comp[r] * jt->lam_prev[r]) / dt + jt->gerr[r]) * (jt->bias_factor / (dt * theta));
+
+    double* x = (double*)malloc(sizeof(double) * ((size_t)N + 1));
+    if (nr == 0) { /* no contact rows: free solution (:460-467) */
+        lu_solve(&lu, rhs, x);
+        memcpy(v_out, x, sizeof(double) * (size_t)nv);
+        memcpy(lam_out, x + nv, sizeof(double) * (size_t)nj);
+        for (int c = 0; c < C->nc; ++c) { imp_out[3 * c] = imp_out[3 * c + 1] = imp_out[3 * c + 2] = 0; }
+        free(x); free(rhs); free(lu.a); free(lu.piv); free(M); orc_free_W(&W);
+        return ORC_OK;
+    }
+    double* Rr = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* bias = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* p = (double*)calloc((size_t)nr, sizeof(double));
+    double* pp = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* dl = (double*)malloc(sizeof(double) * (size_t)nr);
+    double* xp = (double*)malloc(sizeof(double) * (size_t)nv);
+    double* d = (double*)calloc((size_t)N + 1, sizeof(double));
+    double* dx = (double*)malloc(sizeof(double) * ((size_t)N + 1));
+    double* RWv = (double*)malloc(sizeof(double) * (W.rowptr[nr] + 1));
+    spmv(&W, B->v, bias);
+    for (int r = 0; r < nr; ++r) bias[r] += W.cvec[r];
+    for (int r = 0; r < nr; ++r) bias[r] = W.cvec[r] + W.rest[r] * bias[r];
+    delassus_diag(&W, B->minv, Rr);
+    for (int r = 0; r < nr; ++r) Rr[r] = (Rr[r] > 0) ? (prm->alpha / Rr[r]) : 0.0;
+    for (int r = 0; r < nr; ++r) bias[r] = Rr[r] * bias[r];
+    for (int r = 0; r < nr; ++r)
+        for (int k = W.rowptr[r]; k < W.rowptr[r + 1]; ++k) RWv[k] = Rr[r] * W.val[k];
+    if (prm->warmstart)
+        for (int ci = 0; ci < C->nc; ++ci) {
+            const int base = W.row_base[ci];
+            if (base < 0) continue;
+            p[base] = C->warm[3 * ci];
+            if (W.row_size[ci] > 1) { p[base + 1] = C->warm[3 * ci + 1]; p[base + 2] = C->warm[3 * ci + 2]; }
+        }
+    /* x0 = S^-1 (rhs_free + [W^T p0; 0]) (:358-363) */
+    spmv_t_acc(&W, p, rhs);
+    lu_solve(&lu, rhs, x);
+    orc_W RW = W;
+    RW.val = RWv;
+    /* standard_loop with pj_sweep (:121-138): x += S^-1 [W^T dp; 0] */
+    for (int iter = 0; iter < prm->max_iter; ++iter) {
+        memcpy(xp, x, sizeof(double) * (size_t)nv);
+        memcpy(pp, p, sizeof(double) * (size_t)nr);
+        spmv(&RW, x, dl);
+        for (int r = 0; r < nr; ++r) dl[r] += bias[r];
+        for (int r = 0; r < nr; ++r) p[r] -= dl[r];
+        pj_project(&W, p, 1);
+        for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];
+        memset(d, 0, sizeof(double) * (size_t)N);
+        spmv_t_acc(&W, dl, d);
+        lu_solve(&lu, d, dx);
+        for (int k = 0; k < N; ++k) x[k] += dx[k];
+        res->iters = iter + 1;
+        if (iter % 10 == 0) {
+            const double e = pj_residual(nv, x, xp, prm->tol_rel, prm->tol_abs);
+            res->residual = e;
+            if (e <= 1) break;
+        }
+    }
+    for (int ci = 0; ci < C->nc; ++ci) {
+        double* o = imp_out + 3 * (size_t)ci;
+        o[0] = o[1] = o[2] = 0;
+        const int base = W.row_base[ci];
+        if (base < 0) continue;
+        o[0] = p[base];
+        if (W.row_size[ci] > 1) { o[1] = p[base + 1]; o[2] = p[base + 2]; }
+    }
+    memcpy(v_out, x, sizeof(double) * (size_t)nv);
+    memcpy(lam_out, x + nv, sizeof(double) * (size_t)nj);
+    free(Rr); free(bias); free(p); free(pp); free(dl); free(xp); free(d); free(dx); free(RWv);
+    free(x); free(rhs); free(lu.a); free(lu.piv); free(M);
+    orc_free_W(&W);
+    return ORC_OK;
+}
+
 /* u_free = v + Minv .* (dt f_ext) (+ Minv .* (dt f_gyro)) (pgs_assembler.cpp:75-82) */
 static void u_free_of(int nv, const orc_bodies* B, double dt, double* uf) {
     for (int j = 0; j < nv; ++j) uf[j] = B->v[j] + B->minv[j] * (dt * B->fext[j]);

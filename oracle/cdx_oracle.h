@@ -97,6 +97,32 @@ void orc_cross(const double a[3], const double b[3], double out[3]);
 int orc_pj_solve(const orc_bodies* b, const orc_contacts* c, double dt, double theta, const orc_params* prm,
                  int nthreads, double* v_out, double* imp_out, orc_result* res);
 
+/* Bilateral rows of the extended system, as DynamicsAssembler::rebuildInteractionW_ emits them
+ * (dynamics_assembler.cpp:341-439): ns spring rows (Wg) then nd damper rows (Wgamma).  Row r
+ * couples body_a[r] and body_b[r] (-1: static, its velocity already folded into vsrc) with the
+ * coefficients wa[6r..6r+5] / wb[6r..6r+5] on the body's velocity dofs (first dof entries used);
+ * comp[r] is C (spring) or A (damper), vsrc[r] C_v / A_v, gerr[r] the position error of a spring
+ * row, lam_prev[r] Lambda_g / Lambda_gamma of the previous step. */
+typedef struct {
+    int ns, nd;
+    const int* body_a;
+    const int* body_b;
+    const double* wa;
+    const double* wb;
+    const double* comp;
+    const double* vsrc;
+    const double* gerr;
+    const double* lam_prev;
+    double bias_factor; /* constraint_bias_factor (Baumgarte, pj_assembler.cpp:163-164) */
+} orc_joints;
+
+/* Projected Jacobi on the extended system S [v; Lg; Lgamma] (PjAssembler::buildAndFactorS / rhs /
+ * solveS, pj_assembler.cpp:9-182, with implicit gyroscopy and lambda-theta off), standard loop.
+ * S is factorised once by dense LU with partial pivoting (the reference uses SparseLU).
+ * lam_out[ns+nd] receives the new Lambda_g / Lambda_gamma. */
+int orc_pj_solve_joints(const orc_bodies* b, const orc_contacts* c, const orc_joints* jt, double dt, double theta,
+                        const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res);
+
 /* Projected Gauss-Seidel (sequential, lambda <= 0 internally). */
 int orc_pgs_solve(const orc_bodies* b, const orc_contacts* c, double dt, double theta, const orc_params* prm,
                   double* v_out, double* imp_out, orc_result* res);

@@ -35,6 +35,11 @@ class Params(C.Structure):
                 ("nesterov_beta_threshold", C.c_double), ("nesterov_restart_limit", C.c_int)]
 
 
+class Joints(C.Structure):
+    _fields_ = [("ns", C.c_int), ("nd", C.c_int), ("body_a", _ip), ("body_b", _ip), ("wa", _dp), ("wb", _dp),
+                ("comp", _dp), ("vsrc", _dp), ("gerr", _dp), ("lam_prev", _dp), ("bias_factor", C.c_double)]
+
+
 class Result(C.Structure):
     _fields_ = [("iters", C.c_int), ("residual", C.c_double), ("nrows", C.c_int), ("ncolors", C.c_int)]
 
@@ -56,6 +61,9 @@ def lib():
             getattr(_lib, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
                                             C.POINTER(Params), _dp, _dp, C.POINTER(Result)]
             getattr(_lib, name).restype = C.c_int
+        _lib.orc_pj_solve_joints.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.POINTER(Joints), C.c_double,
+                                             C.c_double, C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Result)]
+        _lib.orc_pj_solve_joints.restype = C.c_int
         _lib.orc_color_blocks.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
         _lib.orc_color_blocks.restype = C.c_int
         _lib.orc_tangent_frame.argtypes = [_dp, _dp, _dp]
@@ -111,6 +119,26 @@ def _solve(name, prob, dt, prm, nthreads=None):
     else:
         st = getattr(L, name)(C.byref(pk.B), C.byref(pk.C), dt, 1.0, C.byref(prm), _d(v), _d(imp), C.byref(res))
     return st, v, imp, res
+
+
+def pj_solve_joints(prob, rows, dt, prm, theta=1.0, lam_prev=None, bias_factor=0.001):
+    """Extended-system PJ (orc_pj_solve_joints).  `rows`: the dict of joints.spherical_rows (ns,
+    nd, body_a, body_b, wa, wb, comp, vsrc, gerr).  Returns (status, v, impulses, Lambda, result)."""
+    L = lib()
+    pk = _Pack(prob)
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    g = lambda x: np.ascontiguousarray(x, dtype=np.int32).reshape(-1)
+    n = int(rows["ns"]) + int(rows["nd"])
+    keep = [g(rows["body_a"]), g(rows["body_b"]), f(rows["wa"]), f(rows["wb"]), f(rows["comp"]), f(rows["vsrc"]),
+            f(rows["gerr"]), f(np.zeros(n) if lam_prev is None else lam_prev)]
+    J = Joints(int(rows["ns"]), int(rows["nd"]), _i(keep[0]), _i(keep[1]), *[_d(x) for x in keep[2:]], bias_factor)
+    v = np.zeros(int(pk.dof.sum()))
+    imp = np.zeros((len(pk.a), 3))
+    lam = np.zeros(max(n, 1))
+    res = Result()
+    st = L.orc_pj_solve_joints(C.byref(pk.B), C.byref(pk.C), C.byref(J), dt, theta, C.byref(prm), _d(v), _d(imp),
+                               _d(lam), C.byref(res))
+    return st, v, imp, lam[:n], res
 
 
 def pj_solve(prob, dt, prm, nthreads=1):
