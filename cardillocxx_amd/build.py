@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 
-HIP_SOURCES = ["cdx_api.hip", "cdx_pj.hip", "cdx_gs.hip", "cdx_batch.hip", "cdx_adjoint.hip"]
+HIP_SOURCES = ["cdx_api.hip", "cdx_pj.hip", "cdx_gs.hip", "cdx_batch.hip", "cdx_adjoint.hip", "cdx_joints.hip"]
 HIP_FLAGS = [
     "-O3",
     "--offload-arch=gfx950",

@@ -209,7 +209,10 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_p, ctx->d_R, ctx->d_bias,
                     ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial,
                     ctx->d_partial2, ctx->d_pk, ctx->d_nest, ctx->d_scene_i, ctx->d_scene_res,
-                    ctx->d_rec_x, ctx->d_rec_p, ctx->d_adj, ctx->d_adj_imp, ctx->d_adj_body, ctx->d_adj_partial};
+                    ctx->d_rec_x, ctx->d_rec_p, ctx->d_adj, ctx->d_adj_imp, ctx->d_adj_body, ctx->d_adj_partial,
+                    ctx->d_jc_off, ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jbody,
+                    ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jr, ctx->d_jrlam, ctx->d_jz, ctx->d_jlam, ctx->d_jL,
+                    ctx->d_jDinv, ctx->d_jtr};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);
@@ -320,6 +323,8 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
         ctx->coloring_valid = false;
     }
     ctx->nscenes = 0;
+    ctx->njrows = ctx->nchains = ctx->njbodies = 0;  // joint rows depend on the body state: set them again
+    ctx->joints_solved = false;
     ctx->solved = false;
     return CDX_OK;
 }

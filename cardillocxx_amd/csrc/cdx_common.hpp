@@ -67,6 +67,18 @@ struct cdx_ctx {
     std::vector<int> h_a, h_b;
     std::vector<double> h_mu;
     int nrows = 0;
+    // bilateral joint rows (cdx_set_joints): blocks of rows per body pair, ordered along chains
+    int njrows = 0, nchains = 0, njbodies = 0;
+    double joint_bias = 0.0;
+    bool joints_solved = false;
+    std::vector<int> h_jorig;  // block-ordered row -> caller's row index
+    int *d_jc_off = nullptr, *d_jcb_off = nullptr, *d_jb_off = nullptr, *d_jb_body = nullptr, *d_jr_kind = nullptr;
+    int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr;
+    double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
+    double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
+    size_t jc_off_cap = 0, jcb_off_cap = 0, jb_off_cap = 0, jb_body_cap = 0, jr_kind_cap = 0, jbody_cap = 0,
+           jbody_blk_cap = 0, jslot_cap = 0, jr_cap = 0, jrlam_cap = 0, jz_cap = 0, jlam_cap = 0, jL_cap = 0, jDinv_cap = 0,
+           jtr_cap = 0;
     bool any_rest = false;  // any non-zero restitution row (PgsAssembler::rhs, pgs_assembler.cpp:64-67)
     bool any_bias = false;  // any row with restitution or a velocity bias: b = R (c + e (W v + c)) != 0
     int* d_ida = nullptr;
@@ -157,6 +169,8 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
 // Reduces ctx->d_partial[0:nparts) into the residual; mode 0 = PJ test (res <= 1),
 // mode 1 = Gauss-Seidel test (res < 1, NaN/Inf -> CDX_DIVERGED).
 int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence = 1);
+int joints_factor(cdx_ctx* ctx, double dt, double theta);
+int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial);
 
 // Launch helper: brackets the launch with profiling events when enabled.
 // With profiling on, the start/stop events travel in the kernel's own dispatch

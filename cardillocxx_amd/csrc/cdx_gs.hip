@@ -345,6 +345,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
+    if (ctx->njrows > 0) return set_err(ctx, CDX_BAD_ARG, "cdx_gs_colored_solve: joint rows are solved by cdx_pj_solve only");
     if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_gs_colored_solve: acceleration mode not supported");
     if (info) {
         info->iters = 0;

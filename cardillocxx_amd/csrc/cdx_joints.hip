@@ -1,0 +1,445 @@
+// cdx_joints.hip -- bilateral joint rows in the projected-Jacobi solve (the extended system S).
+//
+// The reference puts joint rows into the system matrix and factorises it with SparseLU
+// (PjAssembler::buildAndFactorS / rhs / solveS, pj_assembler.cpp:9-182; rows from
+// DynamicsAssembler::rebuildInteractionW_, dynamics_assembler.cpp:341-439):
+//
+//     S = [ M   G^T ]      G = [Wg; Wgamma],  E = diag(C / (theta h^2), A / (theta h))
+//         [ G   -E  ]
+//
+// and every PJ sweep adds x += S^-1 [W^T dp; 0] (projected_jacobi.cpp:121-138).  Here S^-1 is
+// applied through the Schur complement on the joint rows,
+//     K = G M^-1 G^T + E,   Lambda = K^-1 (G M^-1 r_v - r_lambda),   v = M^-1 (r_v - G^T Lambda),
+// which for joints forming chains (each body in at most two joint blocks, no loops: capsule chains,
+// beams) is block tridiagonal.  It is factorised once per solve as K = L D L^T (block rows of
+// up to kMaxJRows) by one thread per chain, and each application is a forward and a backward
+// substitution along the chain -- the same arithmetic as an LU of S up to rounding.
+//
+// A joint block is the set of rows that couple the same pair of bodies (a body and the world
+// when one side is static).  Bodies in no block keep the diagonal solve of the plain path.
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "cdx_device.hpp"
+
+namespace cdx {
+
+constexpr int kMaxJRows = 12;  // rows per joint block (a full 6-dof joint with dampers)
+constexpr int kJRowStride = 16;  // doubles per row: wa[6], wb[6], comp, vsrc, gerr, lam_prev
+
+// inverse mass of dof k of a rigid body
+__device__ inline double minv_k(const double* __restrict__ minv, int b, int k) {
+    return minv[(size_t)b * kMinvStride + (k < 3 ? 0 : k - 2)];
+}
+
+// coefficient row i of block j on body `side` (0: a, 1: b)
+__device__ inline const double* jrow(const double* __restrict__ jr, int row, int side) {
+    return jr + (size_t)row * kJRowStride + 6 * side;
+}
+
+// Once per solve, one thread per chain: the joint-row right-hand side of PjAssembler::rhs
+// (:158-164; implicit gyroscopy and lambda-theta off) and the block LDL^T factors of K.
+__global__ void k_joint_factor(int nchains, const int* __restrict__ jc_off, const int* __restrict__ jb_off,
+                               const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
+                               const double* __restrict__ jr, const double* __restrict__ minv,
+                               const double* __restrict__ v, double dt, double theta, double beta,
+                               double* __restrict__ rlam, double* __restrict__ Lm, double* __restrict__ Dinv,
+                               int* __restrict__ ctrl) {
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= nchains) return;
+    double K[kMaxJRows * kMaxJRows], Dp[kMaxJRows * kMaxJRows], A[kMaxJRows * 2 * kMaxJRows];
+    int mp = 0;
+    for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j) {
+        const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
+        const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
+        // right-hand side of the rows
+        for (int i = 0; i < m; ++i) {
+            const int r = r0 + i;
+            double gv = 0;
+            for (int side = 0; side < 2; ++side) {
+                const int b = side ? bb : ba;
+                if (b < 0) continue;
+                const double* w = jrow(jr, r, side);
+                for (int k = 0; k < 6; ++k) gv += w[k] * v[(size_t)b * kBodyStride + k];
+            }
+            const double* rr = jr + (size_t)r * kJRowStride;
+            const double comp = rr[12], vsrc = rr[13], gerr = rr[14], lprev = rr[15];
+            double o;
+            if (jr_kind[r] == 0) {
+                o = -(1.0 / (theta * dt * dt)) * (comp * lprev) - ((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
+                if (beta > 0) o -= (-(comp * lprev) / dt + gerr) * (beta / (dt * theta));
+            } else {
+                o = -((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
+            }
+            rlam[r] = o;
+        }
+        // K_jj = sum over the block's bodies of G M^-1 G^T, plus E
+        for (int i = 0; i < m; ++i)
+            for (int l = 0; l < m; ++l) {
+                double s = 0;
+                for (int side = 0; side < 2; ++side) {
+                    const int b = side ? bb : ba;
+                    if (b < 0) continue;
+                    const double* wi = jrow(jr, r0 + i, side);
+                    const double* wl = jrow(jr, r0 + l, side);
+                    for (int k = 0; k < 6; ++k) s += wi[k] * minv_k(minv, b, k) * wl[k];
+                }
+                K[i * m + l] = s;
+            }
+        for (int i = 0; i < m; ++i) {
+            const double* rr = jr + (size_t)(r0 + i) * kJRowStride;
+            K[i * m + i] += rr[12] * (jr_kind[r0 + i] == 0 ? 1.0 / (theta * dt * dt) : 1.0 / (theta * dt));
+        }
+        if (j > jc_off[ch]) {
+            // K_{j,j-1} through the body shared with the previous block, L_j = K_{j,j-1} D_{j-1}^-1,
+            // D_j = K_jj - L_j K_{j,j-1}^T
+            const int p0 = jb_off[j - 1];
+            const int pa = jb_body[2 * (j - 1)], pb = jb_body[2 * (j - 1) + 1];
+            const int s = (ba >= 0 && (ba == pa || ba == pb)) ? ba : bb;
+            const int side_j = s == ba ? 0 : 1, side_p = s == pa ? 0 : 1;
+            double Kjp[kMaxJRows * kMaxJRows];
+            for (int i = 0; i < m; ++i)
+                for (int l = 0; l < mp; ++l) {
+                    const double* wi = jrow(jr, r0 + i, side_j);
+                    const double* wl = jrow(jr, p0 + l, side_p);
+                    double t = 0;
+                    for (int k = 0; k < 6; ++k) t += wi[k] * minv_k(minv, s, k) * wl[k];
+                    Kjp[i * mp + l] = t;
+                }
+            double* L = Lm + (size_t)j * kMaxJRows * kMaxJRows;
+            for (int i = 0; i < m; ++i)
+                for (int l = 0; l < mp; ++l) {
+                    double t = 0;
+                    for (int q = 0; q < mp; ++q) t += Kjp[i * mp + q] * Dp[q * mp + l];
+                    L[i * mp + l] = t;
+                }
+            for (int i = 0; i < m; ++i)
+                for (int l = 0; l < m; ++l) {
+                    double t = 0;
+                    for (int q = 0; q < mp; ++q) t += L[i * mp + q] * Kjp[l * mp + q];
+                    K[i * m + l] -= t;
+                }
+        }
+        // D_j^-1 by Gauss-Jordan with partial pivoting on [D_j | I]
+        const int w2 = 2 * m;
+        for (int i = 0; i < m; ++i)
+            for (int l = 0; l < w2; ++l) A[i * w2 + l] = l < m ? K[i * m + l] : (l - m == i ? 1.0 : 0.0);
+        for (int c = 0; c < m; ++c) {
+            int pr = c;
+            for (int i = c + 1; i < m; ++i)
+                if (fabs(A[i * w2 + c]) > fabs(A[pr * w2 + c])) pr = i;
+            if (pr != c)
+                for (int l = 0; l < w2; ++l) {
+                    const double t = A[c * w2 + l];
+                    A[c * w2 + l] = A[pr * w2 + l];
+                    A[pr * w2 + l] = t;
+                }
+            const double d = A[c * w2 + c];
+            if (!(fabs(d) > 0.0)) {  // singular joint block: S cannot be factorised (pj_assembler.cpp:175)
+                ctrl[2] = CDX_NOT_FACTORIZED;
+                ctrl[0] = 1;
+                return;
+            }
+            for (int l = 0; l < w2; ++l) A[c * w2 + l] /= d;
+            for (int i = 0; i < m; ++i) {
+                if (i == c) continue;
+                const double f = A[i * w2 + c];
+                if (f != 0.0)
+                    for (int l = 0; l < w2; ++l) A[i * w2 + l] -= f * A[c * w2 + l];
+            }
+        }
+        double* Di = Dinv + (size_t)j * kMaxJRows * kMaxJRows;
+        for (int i = 0; i < m; ++i)
+            for (int l = 0; l < m; ++l) {
+                Di[i * m + l] = A[i * w2 + m + l];
+                Dp[i * m + l] = A[i * w2 + m + l];
+            }
+        mp = m;
+    }
+}
+
+// x += S^-1 [r; -r_lambda] restricted to the jointed bodies, one thread per chain.
+//   setup = 1: x0 and Lambda_0 are set (absolute; tr holds the full body right-hand side);
+//   setup = 0: a sweep increment (tr holds W^T dp of each jointed body); x and Lambda accumulate,
+//              and with `check` the residual partials of the chain's bodies are written.
+__global__ void __launch_bounds__(kBlock) k_joint_solve(
+    int nchains, int setup, int check, const int* __restrict__ jc_off, const int* __restrict__ jcb_off,
+    const int* __restrict__ jb_off, const int* __restrict__ jb_body, const double* __restrict__ jr,
+    const double* __restrict__ rlam, const double* __restrict__ Lm, const double* __restrict__ Dinv,
+    const int* __restrict__ jbody, const int* __restrict__ jbody_blk, const double* __restrict__ minv,
+    const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ lam, double* __restrict__ x,
+    double tol_rel, double tol_abs, double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int ch = blockIdx.x * kBlock + threadIdx.x;
+    double e2 = 0;
+    if (ch < nchains) {
+        // forward: z_j = s_j - L_j z_{j-1}, then y_j = D_j^-1 z_j (stored in z)
+        double zp[kMaxJRows], zc[kMaxJRows];
+        int mp = 0;
+        for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j) {
+            const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
+            const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
+            for (int i = 0; i < m; ++i) {
+                double s = 0;
+                for (int side = 0; side < 2; ++side) {
+                    const int b = side ? bb : ba;
+                    if (b < 0) continue;
+                    const double* w = jrow(jr, r0 + i, side);
+                    const double* t = tr + (size_t)b * 6;
+                    for (int k = 0; k < 6; ++k) s += w[k] * (minv_k(minv, b, k) * t[k]);
+                }
+                zc[i] = setup ? s - rlam[r0 + i] : s;
+            }
+            if (j > jc_off[ch]) {
+                const double* L = Lm + (size_t)j * kMaxJRows * kMaxJRows;
+                for (int i = 0; i < m; ++i) {
+                    double t = 0;
+                    for (int l = 0; l < mp; ++l) t += L[i * mp + l] * zp[l];
+                    zc[i] -= t;
+                }
+            }
+            for (int i = 0; i < m; ++i) zp[i] = zc[i];
+            const double* Di = Dinv + (size_t)j * kMaxJRows * kMaxJRows;
+            for (int i = 0; i < m; ++i) {
+                double t = 0;
+                for (int l = 0; l < m; ++l) t += Di[i * m + l] * zc[l];
+                z[r0 + i] = t;
+            }
+            mp = m;
+        }
+        // backward: Lambda_j = y_j - L_{j+1}^T Lambda_{j+1} (kept in z)
+        for (int j = jc_off[ch + 1] - 2; j >= jc_off[ch]; --j) {
+            const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
+            const int n0 = jb_off[j + 1], mn = jb_off[j + 2] - n0;
+            const double* L = Lm + (size_t)(j + 1) * kMaxJRows * kMaxJRows;  // mn x m
+            for (int i = 0; i < m; ++i) {
+                double t = 0;
+                for (int l = 0; l < mn; ++l) t += L[l * m + i] * z[n0 + l];
+                z[r0 + i] -= t;
+            }
+        }
+        for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j)
+            for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) lam[r] = setup ? z[r] : lam[r] + z[r];
+        // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda)
+        for (int q = jcb_off[ch]; q < jcb_off[ch + 1]; ++q) {
+            const int b = jbody[q];
+            double acc[6];
+            for (int k = 0; k < 6; ++k) acc[k] = tr[(size_t)b * 6 + k];
+            for (int e = 0; e < 2; ++e) {
+                const int j = jbody_blk[4 * (size_t)q + 2 * e], side = jbody_blk[4 * (size_t)q + 2 * e + 1];
+                if (j < 0) continue;
+                for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) {
+                    const double* w = jrow(jr, r, side);
+                    for (int k = 0; k < 6; ++k) acc[k] -= w[k] * z[r];
+                }
+            }
+            double* xb = x + (size_t)b * kBodyStride;
+            for (int k = 0; k < 6; ++k) {
+                const double dv = minv_k(minv, b, k) * acc[k];
+                const double u = xb[k];
+                const double un = setup ? dv : u + dv;
+                xb[k] = un;
+                if (check) {
+                    const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                    const double ee = (un - u) / qd;
+                    e2 += ee * ee;
+                }
+            }
+        }
+    }
+    if (check) {
+        const double s = block_sum(e2, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+int joints_factor(cdx_ctx* ctx, double dt, double theta) {
+    if (ctx->nchains == 0) return CDX_OK;
+    CDX_LAUNCH(ctx, "joint_factor", k_joint_factor, grid_for(ctx->nchains), kBlock, ctx->nchains, ctx->d_jc_off, ctx->d_jb_off,
+               ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jr, ctx->d_minv, ctx->d_v, dt, theta, ctx->joint_bias, ctx->d_jrlam,
+               ctx->d_jL, ctx->d_jDinv, ctx->d_ctrl);
+    return hip_check(ctx, hipGetLastError(), "joint factor launch");
+}
+
+int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial) {
+    if (ctx->nchains == 0) return CDX_OK;
+    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, grid_for(ctx->nchains), kBlock, ctx->nchains, setup, check, ctx->d_jc_off,
+               ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
+               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_x, tol_rel, tol_abs, partial,
+               setup ? nullptr : ctx->d_ctrl);
+    return hip_check(ctx, hipGetLastError(), "joint solve launch");
+}
+
+}  // namespace cdx
+
+using namespace cdx;
+
+extern "C" {
+
+int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* body_b, const double* wa, const double* wb,
+                   const double* comp, const double* vsrc, const double* gerr, const double* lam_prev, double bias_factor) {
+    if (!ctx) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    ctx->solved = false;
+    ctx->nchains = 0;
+    ctx->njrows = 0;
+    if (ns < 0 || nd < 0) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: negative row count");
+    const int n = ns + nd;
+    if (n == 0) return CDX_OK;
+    if (!body_a || !body_b || !wa || !wb || !comp || !vsrc || !gerr || !lam_prev)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: null array");
+    if (!std::isfinite(bias_factor) || bias_factor < 0) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: bad bias factor");
+    const int nb = ctx->nb;
+    for (int r = 0; r < n; ++r) {
+        const int a = body_a[r], b = body_b[r];
+        if (a < -1 || a >= nb || b < -1 || b >= nb || (a < 0 && b < 0) || a == b)
+            return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: bad body index");
+        for (int side = 0; side < 2; ++side) {
+            const int body = side ? b : a;
+            if (body >= 0 && ctx->dof[body] != 6)
+                return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: jointed bodies must be 6-dof rigid bodies");
+        }
+        for (int k = 0; k < 6; ++k)
+            if (!std::isfinite(wa[6 * (size_t)r + k]) || !std::isfinite(wb[6 * (size_t)r + k]))
+                return set_err(ctx, CDX_NONFINITE, "cdx_set_joints: non-finite row");
+        if (!std::isfinite(comp[r]) || comp[r] < 0 || !std::isfinite(vsrc[r]) || !std::isfinite(gerr[r]) ||
+            !std::isfinite(lam_prev[r]))
+            return set_err(ctx, CDX_NONFINITE, "cdx_set_joints: bad compliance or source term");
+    }
+    // blocks: rows grouped by body pair, in order of first appearance
+    std::map<std::pair<int, int>, int> key;
+    std::vector<std::vector<int>> blk_rows;
+    std::vector<std::pair<int, int>> blk_body;
+    for (int r = 0; r < n; ++r) {
+        const auto k = std::make_pair(body_a[r], body_b[r]);
+        auto it = key.find(k);
+        if (it == key.end()) {
+            it = key.emplace(k, (int)blk_rows.size()).first;
+            blk_rows.emplace_back();
+            blk_body.push_back(k);
+        }
+        blk_rows[it->second].push_back(r);
+    }
+    const int nblk = (int)blk_rows.size();
+    for (const auto& rows : blk_rows)
+        if ((int)rows.size() > kMaxJRows) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: more than 12 rows on one body pair");
+    // chains: every dynamic body in at most two blocks, no loops
+    std::vector<std::vector<int>> body_blk(nb);
+    for (int j = 0; j < nblk; ++j)
+        for (int b : {blk_body[j].first, blk_body[j].second})
+            if (b >= 0) body_blk[b].push_back(j);
+    for (int b = 0; b < nb; ++b)
+        if (body_blk[b].size() > 2)
+            return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: a body in more than two joint blocks (joints must form chains)");
+    auto neighbours = [&](int j, int other[2]) {
+        int c = 0;
+        for (int b : {blk_body[j].first, blk_body[j].second}) {
+            if (b < 0) continue;
+            for (int k : body_blk[b])
+                if (k != j) other[c++] = k;
+        }
+        return c;
+    };
+    std::vector<int> order, chain_off{0}, visited(nblk, 0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int j0 = 0; j0 < nblk; ++j0) {
+            if (visited[j0]) continue;
+            int nbrs[4];
+            if (pass == 0 && neighbours(j0, nbrs) > 1) continue;  // start chains at their ends
+            if (pass == 1) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: joint blocks form a loop");
+            int prev = -1, cur = j0;
+            while (cur >= 0) {
+                visited[cur] = 1;
+                order.push_back(cur);
+                const int cnt = neighbours(cur, nbrs);
+                int next = -1;
+                for (int k = 0; k < cnt; ++k)
+                    if (nbrs[k] != prev && !visited[nbrs[k]]) next = nbrs[k];
+                prev = cur;
+                cur = next;
+            }
+            chain_off.push_back((int)order.size());
+        }
+    const int nchains = (int)chain_off.size() - 1;
+    // block-ordered rows and per-block data
+    std::vector<int> jb_off{0}, jb_body, jr_kind, jorig;
+    std::vector<double> jr;
+    for (int j : order) {
+        for (int r : blk_rows[j]) {
+            jorig.push_back(r);
+            jr_kind.push_back(r < ns ? 0 : 1);
+            for (int k = 0; k < 6; ++k) jr.push_back(wa[6 * (size_t)r + k]);
+            for (int k = 0; k < 6; ++k) jr.push_back(wb[6 * (size_t)r + k]);
+            jr.push_back(comp[r]);
+            jr.push_back(vsrc[r]);
+            jr.push_back(gerr[r]);
+            jr.push_back(lam_prev[r]);
+        }
+        jb_off.push_back((int)jorig.size());
+        jb_body.push_back(blk_body[j].first);
+        jb_body.push_back(blk_body[j].second);
+    }
+    // jointed bodies in chain order, each with its (block, side) pairs
+    std::vector<int> pos_of(nblk);
+    for (int i = 0; i < nblk; ++i) pos_of[order[i]] = i;
+    std::vector<int> jbody, jbody_blk, jcb_off{0}, jslot(nb, -1);
+    for (int c = 0; c < nchains; ++c) {
+        for (int i = chain_off[c]; i < chain_off[c + 1]; ++i)
+            for (int b : {jb_body[2 * i], jb_body[2 * i + 1]}) {
+                if (b < 0 || jslot[b] >= 0) continue;
+                jslot[b] = (int)jbody.size();
+                jbody.push_back(b);
+                int e = 0, ent[4] = {-1, -1, -1, -1};
+                for (int k : body_blk[b]) {
+                    const int pk = pos_of[k];
+                    ent[2 * e] = pk;
+                    ent[2 * e + 1] = jb_body[2 * pk] == b ? 0 : 1;
+                    ++e;
+                }
+                jbody_blk.insert(jbody_blk.end(), ent, ent + 4);
+            }
+        jcb_off.push_back((int)jbody.size());
+    }
+    const size_t R = jorig.size(), NJB = jbody.size();
+    auto upi = [&](int** d, const std::vector<int>& h, size_t* cap, const char* what) -> int {
+        CDX_TRY(ensure_alloc(ctx, d, h.size(), cap, what));
+        return hip_check(ctx, hipMemcpy(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice), what);
+    };
+    CDX_TRY(upi(&ctx->d_jc_off, chain_off, &ctx->jc_off_cap, "joint chains"));
+    CDX_TRY(upi(&ctx->d_jcb_off, jcb_off, &ctx->jcb_off_cap, "joint chain bodies"));
+    CDX_TRY(upi(&ctx->d_jb_off, jb_off, &ctx->jb_off_cap, "joint blocks"));
+    CDX_TRY(upi(&ctx->d_jb_body, jb_body, &ctx->jb_body_cap, "joint block bodies"));
+    CDX_TRY(upi(&ctx->d_jr_kind, jr_kind, &ctx->jr_kind_cap, "joint row kinds"));
+    CDX_TRY(upi(&ctx->d_jbody, jbody, &ctx->jbody_cap, "jointed bodies"));
+    CDX_TRY(upi(&ctx->d_jbody_blk, jbody_blk, &ctx->jbody_blk_cap, "jointed body blocks"));
+    CDX_TRY(upi(&ctx->d_jslot, jslot, &ctx->jslot_cap, "joint slots"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
+    CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jrlam, R, &ctx->jrlam_cap, "joint rhs"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jz, R, &ctx->jz_cap, "joint scratch"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jlam, R, &ctx->jlam_cap, "joint impulses"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jL, (size_t)nblk * kMaxJRows * kMaxJRows, &ctx->jL_cap, "joint factors"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jDinv, (size_t)nblk * kMaxJRows * kMaxJRows, &ctx->jDinv_cap, "joint factors"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jtr, (size_t)nb * 6, &ctx->jtr_cap, "joint body rhs"));
+    ctx->h_jorig = jorig;
+    ctx->njrows = (int)R;
+    ctx->nchains = nchains;
+    ctx->njbodies = (int)NJB;
+    ctx->joint_bias = bias_factor;
+    return CDX_OK;
+}
+
+int cdx_get_joint_impulses(cdx_ctx* ctx, double* lam_out) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (ctx->njrows == 0) return CDX_OK;
+    if (!lam_out) return CDX_BAD_ARG;
+    if (!ctx->solved || !ctx->joints_solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_joint_impulses: no joint solve yet");
+    std::vector<double> h(ctx->njrows);
+    CDX_HIP(ctx, hipMemcpy(h.data(), ctx->d_jlam, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < ctx->njrows; ++i) lam_out[ctx->h_jorig[i]] = h[i];
+    return CDX_OK;
+}
+
+}  // extern "C"

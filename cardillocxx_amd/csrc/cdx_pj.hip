@@ -45,10 +45,11 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
                                   const double* __restrict__ v, const double* __restrict__ fext,
                                   const double* __restrict__ fgyro, const int* __restrict__ incptr,
                                   const int* __restrict__ incidx, const double* __restrict__ imp, double dt,
-                                  int have_contacts, double* __restrict__ x) {
+                                  int have_contacts, double* __restrict__ x, const int* __restrict__ jslot,
+                                  double* __restrict__ tr) {
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= nb) return;
-    pj_setup_body(bi, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, have_contacts, x);
+    pj_setup_body(bi, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, have_contacts, x, jslot, tr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -155,7 +156,8 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
                                                        const int* __restrict__ incptr, const int* __restrict__ incidx,
                                                        const double* __restrict__ imp9, double* __restrict__ x, int check,
                                                        double tol_rel, double tol_abs, double* __restrict__ partial,
-                                                       const int* __restrict__ ctrl) {
+                                                       const int* __restrict__ ctrl, const int* __restrict__ jslot,
+                                                       double* __restrict__ tr) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     // XCD-aware tiles, as the contact pass: a contiguous range of bodies per XCD gathers the
@@ -181,7 +183,9 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
 #pragma unroll
         for (int k = 1; k < 6; ++k)
             if (l == k) a = acc[k];
-        if (l < nd) {
+        if (jslot && jslot[bi] >= 0) {
+            if (l < nd) tr[(size_t)bi * 6 + l] = a;  // W^T dp of a jointed body: the chain solve applies it
+        } else if (l < nd) {
             const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
             const double M = mk > 0 ? 1.0 / mk : 0.0;
             double* xp = x + (size_t)bi * kBodyStride + l;
@@ -456,12 +460,17 @@ using namespace cdx;
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
 
 extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info) {
-    (void)theta;  // theta only enters bilateral rows and S of spring/damper scenes
+    // theta only enters the joint rows of S (PjAssembler::buildAndFactorS / rhs)
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
     if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: unknown acceleration mode");
+    const bool joints = ctx->njrows > 0;
+    if (joints && prm->accel != CDX_ACCEL_NONE)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the accelerated loop does not take joint rows");
+    if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: theta must be positive with joint rows");
+    ctx->joints_solved = false;
     if (info) {
         info->iters = 0;
         info->residual = std::nan("");
@@ -485,13 +494,24 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
                    ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, prm->alpha, prm->warmstart, ctx->d_R,
                    ctx->d_bias, ctx->d_p, ctx->d_imp);
     CDX_LAUNCH(ctx, "pj_setup_bodies", k_pj_setup_bodies, grid_for(nb), kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v,
-               ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_imp, dt, have_contacts ? 1 : 0, ctx->d_x);
+               ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_imp, dt, have_contacts ? 1 : 0, ctx->d_x,
+               joints ? ctx->d_jslot : nullptr, joints ? ctx->d_jtr : nullptr);
     CDX_HIP(ctx, hipGetLastError());
+    if (joints) {  // x0 = S^-1 (rhs + [W^T p0; 0]) for the jointed bodies, and Lambda_0
+        CDX_TRY(joints_factor(ctx, dt, theta));
+        CDX_TRY(joints_solve(ctx, 1, 0, 0.0, 0.0, nullptr));
+    }
     if (!have_contacts) {  // no contacts: free solution only (projected_jacobi.cpp:460-467)
         CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (joints) {
+            int ctrl[3];
+            CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+            if (ctrl[0] && ctrl[2]) return set_err(ctx, ctrl[2], "cdx_pj_solve: S is not factorized (singular joint block)");
+        }
         ctx->solved = true;
-        ctx->pj_last = true;
-        ctx->record_valid = ctx->record;
+        ctx->pj_last = !joints;
+        ctx->joints_solved = joints;
+        ctx->record_valid = ctx->record && !joints;
         return CDX_OK;
     }
     const int gb = grid_for(nb), gc = grid_for(nc);
@@ -524,7 +544,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     } else {
         st = run_plain_sweeps(ctx, prm, gb, gc, &iters);
     }
-    if (st) return set_err(ctx, st, "cdx_pj_solve failed");
+    if (st) return set_err(ctx, st, st == CDX_NOT_FACTORIZED ? "cdx_pj_solve: S is not factorized (singular joint block)"
+                                                             : "cdx_pj_solve failed");
     double r = 0;
     CDX_TRY(read_residual(ctx, &r));
     if (info) {
@@ -532,7 +553,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         info->residual = r;
     }
     ctx->solved = true;
-    ctx->pj_last = true;
+    ctx->joints_solved = joints;
+    ctx->pj_last = !joints;  // the adjoint differentiates the diagonal-S solve only
     ctx->recorded = iters;
     ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && iters <= ctx->record_max;
     return CDX_OK;
@@ -544,7 +566,8 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     const int gb8 = grid_for(nb * kLanesPerBody);
     const int gc4 = grid_for(nc * kLanesPerContact);
     (void)gc;
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
+    const int gj = ctx->njrows > 0 ? grid_for(ctx->nchains) : 0;  // partials of the chain solve follow the bodies'
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)(gb8 + gj), &ctx->partial_cap, "residual partials"));
     (void)gb;
     if (ctx->record) {
         CDX_TRY(ensure_alloc(ctx, &ctx->d_rec_x, (size_t)ctx->record_max * kBodyStride * nb, &ctx->rec_x_cap, "recorded x"));
@@ -555,8 +578,10 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
         CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
                    ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                   ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
-        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));
+                   ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl,
+                   gj ? ctx->d_jslot : nullptr, gj ? ctx->d_jtr : nullptr);
+        if (gj) CDX_TRY(joints_solve(ctx, 0, check, prm->tol_rel, prm->tol_abs, ctx->d_partial + gb8));
+        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8 + gj, it, 0, check_every));
         return hip_check(ctx, hipGetLastError(), "pj sweep launch");
     };
     auto sweep = [&](int it) -> int {

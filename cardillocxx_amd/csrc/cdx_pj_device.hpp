@@ -55,7 +55,8 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
                                      const double* __restrict__ v, const double* __restrict__ fext,
                                      const double* __restrict__ fgyro, const int* __restrict__ incptr,
                                      const int* __restrict__ incidx, const double* __restrict__ imp, double dt,
-                                     int have_contacts, double* __restrict__ x) {
+                                     int have_contacts, double* __restrict__ x, const int* __restrict__ jslot = nullptr,
+                                     double* __restrict__ tr = nullptr) {
     const int nd = dof[bi];
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (have_contacts)
@@ -65,15 +66,18 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
     const double* fg = fgyro + (size_t)bi * kBodyStride;
     const double* mi = minv + (size_t)bi * kMinvStride;
     double out[6] = {0, 0, 0, 0, 0, 0};
+    const bool jointed = jslot && jslot[bi] >= 0;
     for (int k = 0; k < nd; ++k) {
         const double mk = k < 3 ? mi[0] : mi[k - 2];
         const double M = mk > 0 ? 1.0 / mk : 0.0;
         double r = M * vv[k] + dt * fe[k];
         r += dt * fg[k];
         r += acc[k];
-        out[k] = r / M;
+        out[k] = jointed ? r : r / M;
     }
-    store6(x + (size_t)bi * kBodyStride, out);
+    // a jointed body's right-hand side goes to the chain solve, which writes its x0
+    if (jointed) store6(tr + (size_t)bi * 6, out);
+    else store6(x + (size_t)bi * kBodyStride, out);
 }
 
 // One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).

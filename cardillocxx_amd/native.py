@@ -43,6 +43,8 @@ EXPORTED = [
     "cdx_set_bodies",
     "cdx_set_contacts",
     "cdx_pj_solve",
+    "cdx_set_joints",
+    "cdx_get_joint_impulses",
     "cdx_gs_colored_solve",
     "cdx_get_velocities",
     "cdx_get_impulses",
@@ -108,6 +110,8 @@ def _declare(lib):
         "cdx_set_bodies": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
         "cdx_set_contacts": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "cdx_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), C.POINTER(SolveInfo)]),
+        "cdx_set_joints": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]),
+        "cdx_get_joint_impulses": (C.c_int, [_vp, _dp]),
         "cdx_gs_colored_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), C.POINTER(SolveInfo)]),
         "cdx_get_velocities": (C.c_int, [_vp, _dp]),
         "cdx_get_impulses": (C.c_int, [_vp, _dp]),
@@ -252,6 +256,25 @@ class Context:
         self._check(self._l.cdx_set_contacts(self.h, len(ia), iptr(ia), iptr(ib), *[dptr(x) for x in d]),
                     "cdx_set_contacts")
         self.nc = len(ia)
+
+    def set_joints(self, rows, lam_prev=None, bias_factor=0.001):
+        """Joint rows in the layout of joints.spherical_rows (ns, nd, body_a, body_b, wa, wb, comp,
+        vsrc, gerr); lam_prev: Lambda of the previous step (zeros if None)."""
+        ns, nd = int(rows["ns"]), int(rows["nd"])
+        n = ns + nd
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        a = np.ascontiguousarray(rows["body_a"], dtype=np.int32)
+        b = np.ascontiguousarray(rows["body_b"], dtype=np.int32)
+        arr = [f(rows["wa"]), f(rows["wb"]), f(rows["comp"]), f(rows["vsrc"]), f(rows["gerr"]),
+               f(np.zeros(n) if lam_prev is None else lam_prev)]
+        self._check(self._l.cdx_set_joints(self.h, ns, nd, iptr(a), iptr(b), *[dptr(x) for x in arr], bias_factor),
+                    "cdx_set_joints")
+        self.njrows = n
+
+    def joint_impulses(self):
+        out = np.zeros(max(getattr(self, "njrows", 0), 1))
+        self._check(self._l.cdx_get_joint_impulses(self.h, dptr(out)), "cdx_get_joint_impulses")
+        return out[:getattr(self, "njrows", 0)]
 
     def _solve(self, fn, dt, theta, params):
         info = SolveInfo()

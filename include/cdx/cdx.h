@@ -104,9 +104,28 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
                      const double* angA, const double* angB, const double* rest, const double* vbias,
                      const double* warm);
 
+/* Bilateral rows of the extended system: the output of DynamicsAssembler::rebuildInteractionW_
+ * (dynamics_assembler.cpp:341-439) that PjAssembler puts into S = [M, G^T; G, -E]
+ * (pj_assembler.cpp:9-182).  ns spring rows (Wg, compliance C) then nd damper rows (Wgamma,
+ * compliance A).  Row r couples body_a[r] and body_b[r] (-1: static; its velocity is folded into
+ * vsrc) with wa[6r..6r+5] / wb[6r..6r+5] on the body's [v; omega]; comp[r] = C or A (finite,
+ * >= 0); vsrc[r] = C_v / A_v; gerr[r] = position error of a spring row (Baumgarte bias, with
+ * bias_factor = constraint_bias_factor, pj_assembler.cpp:163-164); lam_prev[r] = Lambda_g /
+ * Lambda_gamma of the previous step.  Rows of one body pair form a joint block (<= 12 rows);
+ * blocks must form chains (each dynamic body in at most two blocks, no loops) -- capsule chains,
+ * beams -- and jointed bodies must be 6-dof.  Taken by cdx_pj_solve (standard loop); cleared by
+ * cdx_set_bodies. */
+int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* body_b, const double* wa, const double* wb,
+                   const double* comp, const double* vsrc, const double* gerr, const double* lam_prev, double bias_factor);
+
+/* Lambda_g / Lambda_gamma of the last cdx_pj_solve with joint rows, in the caller's row order
+ * (setLambda_g / setLambda_gamma, projected_jacobi.cpp:495-497). */
+int cdx_get_joint_impulses(cdx_ctx* ctx, double* lam_out);
+
 /* Projected Jacobi: replaces ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500).
  * Results stay on the device: fetch with cdx_get_velocities / cdx_get_impulses.
- * The new impulses also become the next warm start (storeImpulse, warmstart.hpp:41-56). */
+ * The new impulses also become the next warm start (storeImpulse, warmstart.hpp:41-56).
+ * With joint rows (cdx_set_joints) x = [v; Lambda] solves the extended system. */
 int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info);
 
 /* Graph-coloured projected Gauss-Seidel: the PGS block update
