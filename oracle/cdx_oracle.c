@@ -277,6 +277,35 @@ static void fill_mdiag(int nv, const double* minv, double* M) {
     for (int j = 0; j < nv; ++j) M[j] = minv[j] > 0 ? 1.0 / minv[j] : 0.0;
 }
 
+/* Spectral radius of the linear sweep map v -> v - S^-1 W^T R W v (estimateSpectralRadius,
+ * projected_jacobi.cpp:155-178): 8 power iterations from the normalised ones vector, clamped to
+ * 0.995; here S = diag(M). */
+static double estimate_rho(const orc_W* W, const orc_W* RW, const double* M, int nv) {
+    if (nv <= 0) return 0.0;
+    double* v = (double*)malloc(sizeof(double) * (size_t)nv);
+    double* r = (double*)malloc(sizeof(double) * (size_t)nv);
+    double* d = (double*)malloc(sizeof(double) * (size_t)(W->nrows + 1));
+    const double inv = 1.0 / sqrt((double)nv);
+    for (int j = 0; j < nv; ++j) v[j] = inv;
+    double lambda = 0.0;
+    for (int it = 0; it < 8; ++it) {
+        spmv(RW, v, d);
+        memset(r, 0, sizeof(double) * (size_t)nv);
+        spmv_t_acc(W, d, r);
+        double n2 = 0.0;
+        for (int j = 0; j < nv; ++j) {
+            v[j] = v[j] - r[j] / M[j];
+            n2 += v[j] * v[j];
+        }
+        const double n = sqrt(n2);
+        if (!(n > 1e-300)) { lambda = 0.0; break; }
+        lambda = n;
+        for (int j = 0; j < nv; ++j) v[j] = v[j] / n;
+    }
+    free(v); free(r); free(d);
+    return dmin(lambda, 0.995);
+}
+
 /* PJ solve: ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500) with build_workspace
  * (:316-369), standard_loop (:371-377) / nesterov_loop (:379-449), pj_sweep (:121-138),
  * rhs (pj_assembler.cpp:127-170, contact-only), solveS on S = diag(M) (pj_assembler.cpp:26-30,173-182),
@@ -296,6 +325,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     res->iters = 0;
     res->residual = NAN;
     res->nrows = nr;
+    res->rho = NAN;
     res->ncolors = 0;
     double* M = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
     fill_mdiag(nv, B->minv, M);
@@ -381,7 +411,44 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];                 \
     } while (0)
 
-    if (!prm->nesterov) {
+    if (!prm->nesterov && prm->chebyshev) {
+        /* chebyshev_loop (:180-228): residual every sweep against the sweep's input; on
+         * convergence the loop returns that input (x = xk1), not the swept iterate */
+        const double rho = estimate_rho(&W, &RW, M, nv);
+        const double rho2 = rho * rho;
+        res->rho = rho;
+        double* xk = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* pk = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xk1 = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* pk1 = (double*)malloc(sizeof(double) * (size_t)nr);
+        memcpy(xk, x, sizeof(double) * nv); memcpy(xk1, x, sizeof(double) * nv);
+        memcpy(pk, p, sizeof(double) * nr); memcpy(pk1, p, sizeof(double) * nr);
+        double omega = 1.0;
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(x, xk1, sizeof(double) * nv);
+            memcpy(p, pk1, sizeof(double) * nr);
+            PJ_SWEEP();
+            res->iters = iter + 1;
+            const double err = pj_residual(nv, x, xk1, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (err <= 1) break;
+            double omega_next = (iter == 0) ? (1.0 / (1.0 - rho2 * 0.5)) : (1.0 / (1.0 - (rho2 * 0.25) * omega));
+            int finite = 1;
+            for (int j = 0; j < nv && finite; ++j) finite = isfinite(x[j]);
+            if (!isfinite(omega_next) || omega_next <= 0.0 || !finite) {
+                memcpy(xk, xk1, sizeof(double) * nv); memcpy(pk, pk1, sizeof(double) * nr);
+                memcpy(xk1, x, sizeof(double) * nv); memcpy(pk1, p, sizeof(double) * nr);
+                omega = 1.0;
+            } else {
+                for (int j = 0; j < nv; ++j) { const double t = omega_next * (x[j] - xk[j]) + xk[j]; xk[j] = xk1[j]; xk1[j] = t; }
+                for (int r = 0; r < nr; ++r) { const double t = omega_next * (p[r] - pk[r]) + pk[r]; pk[r] = pk1[r]; pk1[r] = t; }
+                omega = omega_next;
+            }
+        }
+        memcpy(x, xk1, sizeof(double) * nv);
+        memcpy(p, pk1, sizeof(double) * nr);
+        free(xk); free(pk); free(xk1); free(pk1);
+    } else if (!prm->nesterov) {
         /* standard_loop (:371-377): residual only when iter % 10 == 0 */
         for (int iter = 0; iter < prm->max_iter; ++iter) {
             PJ_SWEEP();
@@ -536,7 +603,7 @@ static double joint_row_dot(const orc_joints* jt, const int* voff, const int* do
 
 int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
                         const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
-    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov) return ORC_BAD_ARG;
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev) return ORC_BAD_ARG;
     orc_W W;
     int st = orc_build_W(B, C, &W);
     if (st) { orc_free_W(&W); return st; }
@@ -544,6 +611,7 @@ int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_jo
     res->iters = 0;
     res->residual = NAN;
     res->nrows = nr;
+    res->rho = NAN;
     res->ncolors = 0;
     for (int r = 0; r < nj; ++r) {
         const int a = jt->body_a[r], b = jt->body_b[r];
@@ -708,7 +776,7 @@ static int pgs_core(const orc_bodies* B, const orc_contacts* C, double dt, const
     int st = orc_build_W(B, C, &W);
     if (st) { orc_free_W(&W); return st; }
     const int nv = W.nv, nr = W.nrows;
-    res->iters = 0; res->residual = NAN; res->nrows = nr; res->ncolors = 0;
+    res->iters = 0; res->residual = NAN; res->nrows = nr; res->ncolors = 0; res->rho = NAN;
     double* uf = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
     u_free_of(nv, B, dt, uf);
     if (nr == 0) {

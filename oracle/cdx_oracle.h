@@ -64,6 +64,7 @@ typedef struct {
     int nesterov;      /* pj.nesterov */
     double nesterov_beta_threshold;
     int nesterov_restart_limit;
+    int chebyshev;     /* pj.chebyshev (ignored when nesterov is set, projected_jacobi.cpp:469) */
 } orc_params;
 
 typedef struct {
@@ -71,6 +72,7 @@ typedef struct {
     double residual; /* last residual evaluated (NaN if none) */
     int nrows;
     int ncolors; /* colored GS only */
+    double rho;  /* Chebyshev loop: the spectral-radius estimate it used (NaN otherwise) */
 } orc_result;
 
 /* Sparse contact Jacobian W in CSR, rows ordered frictionless-then-frictional. */

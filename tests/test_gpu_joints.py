@@ -92,6 +92,11 @@ def test_chain_rollout_matches_oracle(ctx):
     starts regenerated every step, Lambda carried over, each step against the oracle."""
     t = scenes.fractal_heightfield(n=33, extent=1.6, amp=0.03, seed=4)
     batch, jt = scenes.capsule_chains(nchains=2, links=5, terrain=t, seed=4, drop=0.01)
+    # lower each straight chain until its lowest capsule touches the terrain
+    for s in range(batch.nscenes):
+        b0, b1 = batch.body_off[s], batch.body_off[s + 1]
+        gap = batch.q[b0:b1, 2] - scenes.terrain_height(t, batch.q[b0:b1, 0], batch.q[b0:b1, 1]) - batch.dims[b0:b1, 0]
+        batch.q[b0:b1, 2] -= gap.min() + 1e-4
     frames = joints.joint_frames(batch.q, jt)
     dt, q, v = 1e-3, batch.q.copy().reshape(-1), batch.v.copy().reshape(-1)
     lam_prev, prev = None, None

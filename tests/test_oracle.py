@@ -374,3 +374,89 @@ def test_gyroscopic_force():
     # isotropic inertia: omega x (I omega) = 0
     f = oracle.gyro_force(dof, np.array([1, 1, 1, 0.5, 0.5, 0.5]), v)
     np.testing.assert_allclose(f, 0, atol=1e-15)
+
+
+def _jacobi_map(prob, alpha=0.3):
+    """Dense linear sweep map G = I - M^-1 W^T R W with R_i = alpha / sum_j W_ij^2 M^-1_j."""
+    W, _, _ = delassus(prob)
+    den = (W ** 2) @ prob.minv
+    R = np.where(den > 0, alpha / np.where(den > 0, den, 1.0), 0.0)
+    return np.eye(prob.nv) - np.diag(prob.minv) @ W.T @ np.diag(R) @ W
+
+
+@pytest.mark.parametrize("seed", [0, 2, 6, 7])
+def test_chebyshev_rho_is_eight_power_iterations(seed):
+    """estimate_rho runs 8 normalised power iterations from ones/sqrt(nv), so its estimate is
+    ||G^8 v0|| / ||G^7 v0|| (dense matrix powers here); with 9 point-mass dof under >= 18 rows
+    W has full column rank and the estimate stays below the 0.995 clamp."""
+    prob = scenes.random_problem(nb=3, nc=8, seed=seed, p_static=0.3, p_point=1.0, warm=False)
+    G = _jacobi_map(prob)
+    v0 = np.ones(prob.nv) / np.sqrt(prob.nv)
+    g7 = np.linalg.matrix_power(G, 7) @ v0
+    lam = np.linalg.norm(G @ g7) / np.linalg.norm(g7)
+    assert 0.5 < lam < 0.99
+    assert lam <= np.abs(np.linalg.eigvals(G)).max() + 1e-12
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, max_iter=2, tol_abs=0, tol_rel=0))
+    assert st == 0 and res.rho == pytest.approx(lam, rel=1e-13)
+    # the standard loop reports no estimate
+    assert np.isnan(oracle.pj_solve(prob, 1e-3, oracle.params(max_iter=2))[3].rho)
+
+
+@pytest.mark.parametrize("alpha", [0.3, 0.7])
+@pytest.mark.parametrize("k", [1, 2, 5, 17])
+def test_chebyshev_single_contact_scalar_recurrence(alpha, k):
+    """One sphere on a plane moving along the normal: W has a null space (eigenvalue 1 of the
+    sweep map), so rho is clamped to 0.995, and the sweep scales the normal velocity error by
+    g = 1 - alpha while the impulse stays positive.  The loop is then the scalar two-step
+    recurrence e2 = w (g e1 - e0) + e0 with w_1 = 1/(1 - rho^2/2), w_{j+1} = 1/(1 - rho^2 w_j / 4),
+    and p = m (v - v_free)."""
+    prob = plane_scene().problem()
+    prob.v[2] = -1.0
+    dt = 1e-3
+    st, v, imp, res = oracle.pj_solve(prob, dt, oracle.params(alpha=alpha, max_iter=k, tol_abs=0, tol_rel=0,
+                                                              warmstart=0, chebyshev=1))
+    assert st == 0 and res.iters == k and res.rho == 0.995
+    g, rho2 = 1.0 - alpha, 0.995 ** 2
+    vfree = -1.0 - G * dt
+    e_prev = e = vfree
+    w = 1.0
+    for j in range(k):
+        swept = g * e
+        w = 1.0 / (1.0 - rho2 * 0.5) if j == 0 else 1.0 / (1.0 - rho2 * 0.25 * w)
+        e_prev, e = e, w * (swept - e_prev) + e_prev
+        assert e - vfree >= 0  # impulse non-negative: the projection never acts
+    assert v[2] == pytest.approx(e, rel=1e-12, abs=1e-15)
+    assert imp[0, 0] == pytest.approx(e - vfree, rel=1e-12)
+    np.testing.assert_array_equal(imp[0, 1:], 0.0)
+
+
+def test_chebyshev_returns_the_sweep_input():
+    """The residual is taken every sweep against the sweep's input, and on convergence the loop
+    returns that input: a solve that stops after n sweeps returns exactly what a fixed run of
+    n - 1 sweeps returns."""
+    prob = scenes.random_problem(nb=12, nc=40, seed=3, warm=True)
+    for tol in (1e-4, 1e-8):
+        st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, tol_abs=tol, tol_rel=tol))
+        assert st == 0 and res.iters > 3 and res.residual <= 1
+        st2, v2, imp2, res2 = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, max_iter=res.iters - 1,
+                                                                        tol_abs=0, tol_rel=0))
+        assert res2.iters == res.iters - 1
+        np.testing.assert_array_equal(v, v2)
+        np.testing.assert_array_equal(imp, imp2)
+
+
+def test_chebyshev_same_fixed_point_and_nesterov_precedence():
+    """Extrapolation leaves fixed points alone: run to tight tolerance, Chebyshev and the
+    standard loop agree on a frictionless (unique-solution) problem.  With nesterov set, the
+    chebyshev flag is ignored (projected_jacobi.cpp:469)."""
+    prob = scenes.random_problem(nb=8, nc=12, seed=11, p_frictionless=1.0, p_point=0.0)
+    tight = dict(tol_abs=1e-14, tol_rel=1e-14, max_iter=5_000_000)
+    ref = oracle.pj_solve(prob, 1e-3, oracle.params(**tight))
+    che = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, **tight))
+    assert ref[0] == che[0] == 0 and che[3].iters < ref[3].iters
+    np.testing.assert_allclose(che[1], ref[1], atol=1e-9)
+    prm = dict(max_iter=300, tol_abs=1e-9, tol_rel=1e-9)
+    nes = oracle.pj_solve(prob, 1e-3, oracle.params(nesterov=1, **prm))
+    both = oracle.pj_solve(prob, 1e-3, oracle.params(nesterov=1, chebyshev=1, **prm))
+    assert nes[3].iters == both[3].iters
+    np.testing.assert_array_equal(nes[1], both[1])
