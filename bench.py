@@ -222,6 +222,9 @@ def pile_leg(args, native, scenes, metrics):
     ach_sweep = (bc + bb) * sweeps / elapsed / 1e9
     traffic, traffic_src = committed_traffic("k_pj_contacts4", prob.nc)
 
+    # accelerated loops on the same problem and tolerance: time to solution
+    accel = None if args.no_accel else accel_leg(native, ctx, prob, dt, args, metrics, elapsed / args.steps, sweeps / args.steps)
+
     # coloured Gauss-Seidel on the same problem and tolerance (residual every sweep)
     if args.no_gs:
         gs = None
@@ -283,9 +286,33 @@ def pile_leg(args, native, scenes, metrics):
                                "solve stream, one profiled step of the same workload run right after the timed steps",
                      "profiled_step_ms": tp * 1e3, "profiled_sweeps": pinfo.iters},
         "cpu_baseline": cpu,
+        "accelerated": accel,
         "colored_gs": gs,
     }
     del ctx
+    return out
+
+
+def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std):
+    """Chebyshev (pj.chebyshev) and Nesterov (pj.nesterov) loops on the pile, cold start, to the
+    same tolerance as the standard loop: sweeps and wall time of one solve each."""
+    out = {"standard": {"sweeps": sweeps_std, "s": t_std}}
+    for name, mode in (("chebyshev", native.CDX_ACCEL_CHEBYSHEV), ("nesterov", native.CDX_ACCEL_NESTEROV)):
+        p = native.default_params()
+        p.tol_abs = p.tol_rel = args.tol
+        p.max_iter = args.max_iter
+        p.warmstart = 0
+        p.accel = mode
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        info = ctx.pj_solve(dt, 1.0, p)
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        v, imp = ctx.velocities(), ctx.impulses()
+        out[name] = {"sweeps": int(info.iters), "s": t, "ms_per_sweep": t / max(info.iters, 1) * 1e3,
+                     "converged": bool(info.residual <= 1.0), "residual": float(info.residual),
+                     "speedup_vs_standard": t_std / t, "complementarity": metrics.complementarity(prob, v, imp)}
+        log(f"{name}: {info.iters} sweeps, {t:.3f}s ({t_std / t:.2f}x the standard loop), residual {info.residual:.3e}")
     return out
 
 
@@ -507,6 +534,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200_000)
     ap.add_argument("--gs-max-iter", type=int, default=50_000)
     ap.add_argument("--no-gs", action="store_true", help="skip the coloured Gauss-Seidel solve of the pile")
+    ap.add_argument("--no-accel", action="store_true", help="skip the Chebyshev / Nesterov solves of the pile")
     ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")

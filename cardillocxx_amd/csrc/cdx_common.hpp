@@ -138,6 +138,11 @@ struct cdx_ctx {
     double* d_pk = nullptr;        // Nesterov: previous impulse iterate [3*nc]
     size_t pk_cap = 0;
     double* d_nest = nullptr;      // Nesterov scalars: beta, thk, err_prev, restarts, disabled
+    double* d_xk1 = nullptr;       // Chebyshev: sweep input [nb*8] (power-iteration vector first)
+    size_t xk1_cap = 0;
+    double* d_pk1 = nullptr;       // Chebyshev: impulse sweep input [3*nc]
+    size_t pk1_cap = 0;
+    double* d_cheb = nullptr;      // Chebyshev scalars: w, take-swept, rho^2, rho, stopped, lambda, norm
     int* h_flag = nullptr;         // pinned mirror of the done flag
     int* h_flag_dev = nullptr;
     bool solved = false;

@@ -66,6 +66,54 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
 // tree, and lanes 0-2 write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
 constexpr int kLanesPerContact = 4;
 
+// project() (projected_jacobi.cpp:100-119) of one contact held by a group of 4 lanes (lane k has
+// the trial value of row k): every lane evaluates the same scale factor from the group's values.
+__device__ __forceinline__ double project4(double pt, int k, double muc) {
+    const int base = threadIdx.x & ~(kLanesPerContact - 1);
+    const double zn = __shfl(pt, base, 64);
+    const double z1 = __shfl(pt, base + 1, 64);
+    const double z2 = __shfl(pt, base + 2, 64);
+    const double pn = fmax(zn, 0.0);
+    double pnew = pn;
+    if (k > 0) {
+        pnew = k == 1 ? z1 : z2;
+        const double tn2 = z1 * z1 + z2 * z2;
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (muc * pn) / sqrt(tn2));
+            pnew = sc * pnew;
+        }
+    }
+    return pnew;
+}
+
+// The contact's compact record [lin | angA | angB] = sum over rows of (d_k, aA_k, aB_k) dp_k,
+// summed over the group of 4 by an xor tree; lane 0 writes lin, lane 1 angA (dynamic A only),
+// lane 2 angB (dynamic B only).
+__device__ __forceinline__ void write_record4(bool contact, int k, int c, int a, int b, const double d[3],
+                                              const double aA[3], const double aB[3], double dp,
+                                              double* __restrict__ imp9) {
+    double v9[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        v9[m] = d[m] * dp;
+        v9[3 + m] = aA[m] * dp;
+        v9[6 + m] = aB[m] * dp;
+    }
+#pragma unroll
+    for (int off = 1; off < kLanesPerContact; off <<= 1)
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
+    if (contact && k < 3 && (k == 0 || (k == 1 ? a >= 0 : b >= 0))) {
+        const double o0 = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
+        const double o1 = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
+        const double o2 = k == 0 ? v9[2] : (k == 1 ? v9[5] : v9[8]);
+        double* dst = imp9 + 9 * (size_t)c + 3 * k;
+        dst[0] = o0;
+        dst[1] = o1;
+        dst[2] = o2;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, const int* __restrict__ ida,
                                                          const int* __restrict__ idb, const double* __restrict__ J,
                                                          const double* __restrict__ R,
@@ -102,48 +150,13 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
         pold = p[(size_t)k * nc + c];
         pt = pold - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
-    // project() (projected_jacobi.cpp:100-119) on the group's three values
-    const int base = threadIdx.x & ~(kLanesPerContact - 1);
-    const double zn = __shfl(pt, base, 64);
-    const double z1 = __shfl(pt, base + 1, 64);
-    const double z2 = __shfl(pt, base + 2, 64);
-    const double pn = fmax(zn, 0.0);
-    double pnew = pn;
-    if (k > 0) {
-        pnew = k == 1 ? z1 : z2;
-        const double tn2 = z1 * z1 + z2 * z2;
-        if (tn2 > 0) {
-            const double sc = fmin(1.0, (muc * pn) / sqrt(tn2));
-            pnew = sc * pnew;
-        }
-    }
+    const double pnew = project4(pt, k, muc);
     double dp = 0;
     if (row) {
         dp = pnew - pold;
         p[(size_t)k * nc + c] = pnew;
     }
-    // side impulses: sum over rows of d_k dp_k, aA_k dp_k, aB_k dp_k
-    double v9[9];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        v9[m] = d[m] * dp;
-        v9[3 + m] = aA[m] * dp;
-        v9[6 + m] = aB[m] * dp;
-    }
-#pragma unroll
-    for (int off = 1; off < kLanesPerContact; off <<= 1)
-#pragma unroll
-        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
-    // lane 0: lin, 1: angA (dynamic A only), 2: angB (dynamic B only)
-    if (live && (a >= 0 || b >= 0) && k < 3 && (k == 0 || (k == 1 ? a >= 0 : b >= 0))) {
-        const double o0 = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
-        const double o1 = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
-        const double o2 = k == 0 ? v9[2] : (k == 1 ? v9[5] : v9[8]);
-        double* dst = imp9 + 9 * (size_t)c + 3 * k;
-        dst[0] = o0;
-        dst[1] = o1;
-        dst[2] = o2;
-    }
+    write_record4(live && (a >= 0 || b >= 0), k, c, a, b, d, aA, aB, dp, imp9);
 }
 
 // Body pass with 8 lanes per body: lane l sums incidences l, l+8, ... of its body, the 8 partial
@@ -453,18 +466,348 @@ __global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const doubl
     nest[4] = disabled ? 1.0 : 0.0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Chebyshev-accelerated loop (chebyshev_loop, projected_jacobi.cpp:180-228) and its spectral
+// radius estimate (estimateSpectralRadius, :155-178).
+//
+// Spectral radius: 8 normalised power iterations of the linear sweep map v -> v - M^-1 W^T R W v
+// from ones/sqrt(nv), each one linear contact pass (dp = -R W v, no projection, no bias), one body
+// pass (v' = v + W^T dp / M and per-block sums of v'^2), a one-workgroup norm and a scaling pass.
+// A norm <= 1e-300 (or NaN) ends the estimate at rho = 0; otherwise rho = min(lambda_8, 0.995).
+//
+// Loop: the reference keeps xk, xk1 (the sweep input) and x = sweep(xk1), and after a sweep that
+// did not converge sets xk2 = w (x - xk) + xk (or xk2 = x on a fallback), xk <- xk1, xk1 <- xk2.
+// Here that update is fused into the next sweep: with X = the swept iterate, XK = xk and XK1 =
+// the previous input, the contact and body passes form the new input u = w (X - XK) + XK (u = X
+// on a fallback, and on the first sweep) on the fly, the body pass rotates XK <- XK1, XK1 <- u
+// and stores X <- sweep(u), and the contact pass does the same for P / PK / PK1.  The residual
+// (every sweep) compares X with u.  The finalize kernel ends the loop (res <= 1) or computes the
+// next w; the finish kernels return XK1 / PK1 after convergence (the reference returns the sweep
+// input) and apply the pending update after max_iter sweeps.
+//
+// cheb[]: 0 w, 1 take the swept iterate as the next input (1 on the first sweep and after a
+// fallback), 2 rho^2, 3 rho, 4 power iteration stopped, 5 lambda, 6 current norm.
+// ---------------------------------------------------------------------------------------------
+
+// w (x - xk) + xk with separate roundings, as the reference's vector expression
+__device__ inline double cheb_extrap(double w, double x, double xk) {
+#pragma clang fp contract(off)
+    return w * (x - xk) + xk;
+}
+
+__device__ inline void cheb_input6(const double* __restrict__ X, const double* __restrict__ XK, int b, double w, bool take,
+                                   double u[6]) {
+    load_body(X, b, u);
+    if (take || b < 0) return;
+    double k6[6];
+    load_body(XK, b, k6);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = cheb_extrap(w, u[m], k6[m]);
+}
+
+__global__ void k_cheb_fill(int nb, const int* __restrict__ dof, double inv, double* __restrict__ v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * 6) return;
+    const int bi = t / 6, l = t % 6;
+    v[(size_t)bi * kBodyStride + l] = l < dof[bi] ? inv : 0.0;
+}
+
+// linear contact pass of the power iteration: dp_k = -R_k (W_k v)
+__global__ void __launch_bounds__(kBlock) k_cheb_lin_contacts4(int nc, int nblocks, const int* __restrict__ ida,
+                                                               const int* __restrict__ idb, const double* __restrict__ J,
+                                                               const double* __restrict__ R, const double* __restrict__ mu,
+                                                               const double* __restrict__ v, double* __restrict__ imp9,
+                                                               const double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    const int tid = xcd_tile(nblocks) * kBlock + threadIdx.x;
+    const int c = tid / kLanesPerContact, k = tid % kLanesPerContact;
+    const bool live = c < nc;
+    int a = -1, b = -1;
+    double muc = 0;
+    if (live) {
+        a = ida[c];
+        b = idb[c];
+        muc = mu[c];
+    }
+    const int nr = muc > 0 ? 3 : 1;
+    const bool row = live && (a >= 0 || b >= 0) && k < nr;
+    double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
+    double dp = 0;
+    if (row) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            d[m] = J[(size_t)(3 * k + m) * nc + c];
+            aA[m] = J[(size_t)(9 + 3 * k + m) * nc + c];
+            aB[m] = J[(size_t)(18 + 3 * k + m) * nc + c];
+        }
+        double ua[6], ub[6];
+        load_body(v, a, ua);
+        load_body(v, b, ub);
+        dp = -(R[(size_t)k * nc + c] * row_dot(d, aA, aB, ua, ub));
+    }
+    write_record4(live && (a >= 0 || b >= 0), k, c, a, b, d, aA, aB, dp, imp9);
+}
+
+// Gathered W^T dp of body bi into acc (lanes combined), as in the sweep's body pass.
+__device__ __forceinline__ void gather_body8(int bi, int l, bool live, const int* __restrict__ incptr,
+                                             const int* __restrict__ incidx, const double* __restrict__ imp9,
+                                             double acc[6], bool* touched) {
+    int i0 = 0, i1 = 0;
+    if (live) {
+        i0 = incptr[bi];
+        i1 = incptr[bi + 1];
+        for (int i = i0 + l; i < i1; i += kLanesPerBody) gather_side9(imp9, incidx[i], acc);
+    }
+#pragma unroll
+    for (int off = kLanesPerBody / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += __shfl_xor(acc[k], off, kLanesPerBody);
+    *touched = i1 > i0;
+}
+
+__device__ __forceinline__ double lane_value(const double acc[6], int l) {
+    double a = acc[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+        if (l == k) a = acc[k];
+    return a;
+}
+
+__device__ __forceinline__ double lane_mass(const double* __restrict__ minv, int bi, int l) {
+    const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+    return mk > 0 ? 1.0 / mk : 0.0;
+}
+
+// linear body pass: v <- v + W^T dp / M, per-block sums of the new v^2
+__global__ void __launch_bounds__(kBlock) k_cheb_lin_bodies8(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                                             const int* __restrict__ incptr, const int* __restrict__ incidx,
+                                                             const double* __restrict__ imp9, double* __restrict__ v,
+                                                             double* __restrict__ partial, const double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    __shared__ double red[kBlock / 64];
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    const bool live = bi < nb;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    bool touched = false;
+    gather_body8(bi, l, live, incptr, incidx, imp9, acc, &touched);
+    double s2 = 0;
+    if (live && l < dof[bi]) {
+        double* vp = v + (size_t)bi * kBodyStride + l;
+        const double un = *vp + lane_value(acc, l) / lane_mass(minv, bi, l);
+        *vp = un;
+        s2 = un * un;
+    }
+    const double s = block_sum(s2, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cheb_norm(int nparts, const double* __restrict__ partial, double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    __shared__ double red[kBlock / 64];
+    double s = 0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) s += partial[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        const double n = sqrt(s);
+        if (!(n > 1e-300)) {
+            cheb[4] = 1.0;  // nothing to accelerate: rho = 0
+            cheb[5] = 0.0;
+        } else {
+            cheb[5] = n;
+            cheb[6] = n;
+        }
+    }
+}
+
+__global__ void k_cheb_scale(int nb, const int* __restrict__ dof, double* __restrict__ v, const double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * 6) return;
+    const int bi = t / 6, l = t % 6;
+    if (l < dof[bi]) v[(size_t)bi * kBodyStride + l] = v[(size_t)bi * kBodyStride + l] / cheb[6];
+}
+
+__global__ void k_cheb_start(double* __restrict__ cheb) {
+    const double rho = cheb[4] != 0.0 ? 0.0 : fmin(cheb[5], 0.995);
+    cheb[0] = 1.0;
+    cheb[1] = 1.0;
+    cheb[2] = rho * rho;
+    cheb[3] = rho;
+}
+
+// Contact pass of a Chebyshev sweep: input p = w (P - PK) + PK (or P), rotation PK <- PK1,
+// PK1 <- input, then the projected update of the plain sweep from the input velocities.
+__global__ void __launch_bounds__(kBlock) k_pjc_contacts4(int nc, int nblocks, const int* __restrict__ ida,
+                                                          const int* __restrict__ idb, const double* __restrict__ J,
+                                                          const double* __restrict__ R, const double* __restrict__ bias,
+                                                          int use_bias, const double* __restrict__ mu, double* __restrict__ P,
+                                                          double* __restrict__ PK, double* __restrict__ PK1,
+                                                          const double* __restrict__ X, const double* __restrict__ XK,
+                                                          double* __restrict__ imp9, const int* __restrict__ ctrl,
+                                                          const double* __restrict__ cheb) {
+    if (ctrl[0]) return;
+    const double w = cheb[0];
+    const bool take = cheb[1] != 0.0;
+    const int tid = xcd_tile(nblocks) * kBlock + threadIdx.x;
+    const int c = tid / kLanesPerContact, k = tid % kLanesPerContact;
+    const bool live = c < nc;
+    int a = -1, b = -1;
+    double muc = 0;
+    if (live) {
+        a = ida[c];
+        b = idb[c];
+        muc = mu[c];
+    }
+    const int nr = muc > 0 ? 3 : 1;
+    const bool row = live && (a >= 0 || b >= 0) && k < nr;
+    double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
+    double pin = 0, pt = 0;
+    if (row) {
+        const size_t r = (size_t)k * nc + c;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            d[m] = J[(size_t)(3 * k + m) * nc + c];
+            aA[m] = J[(size_t)(9 + 3 * k + m) * nc + c];
+            aB[m] = J[(size_t)(18 + 3 * k + m) * nc + c];
+        }
+        const double ps = P[r];
+        pin = take ? ps : cheb_extrap(w, ps, PK[r]);
+        PK[r] = PK1[r];
+        PK1[r] = pin;
+        double ua[6], ub[6];
+        cheb_input6(X, XK, a, w, take, ua);
+        cheb_input6(X, XK, b, w, take, ub);
+        const double g = row_dot(d, aA, aB, ua, ub);
+        pt = pin - (R[r] * g + (use_bias ? bias[r] : 0.0));
+    }
+    const double pnew = project4(pt, k, muc);
+    double dp = 0;
+    if (row) {
+        dp = pnew - pin;
+        P[(size_t)k * nc + c] = pnew;
+    }
+    write_record4(live && (a >= 0 || b >= 0), k, c, a, b, d, aA, aB, dp, imp9);
+}
+
+// Body pass of a Chebyshev sweep: input u = w (X - XK) + XK (or X), XK <- XK1, XK1 <- u,
+// X <- u + W^T dp / M; per-block sums of the residual terms and of non-finite entries of X.
+__global__ void __launch_bounds__(kBlock) k_pjc_bodies8(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                                        const int* __restrict__ incptr, const int* __restrict__ incidx,
+                                                        const double* __restrict__ imp9, double* __restrict__ X,
+                                                        double* __restrict__ XK, double* __restrict__ XK1, double tol_rel,
+                                                        double tol_abs, double* __restrict__ partial,
+                                                        const int* __restrict__ ctrl, const double* __restrict__ cheb) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const double w = cheb[0];
+    const bool take = cheb[1] != 0.0;
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    const bool live = bi < nb;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    bool touched = false;
+    gather_body8(bi, l, live, incptr, incidx, imp9, acc, &touched);
+    double e2 = 0, bad = 0;
+    if (live && l < dof[bi]) {
+        const size_t o = (size_t)bi * kBodyStride + l;
+        const double xs = X[o];
+        const double u = take ? xs : cheb_extrap(w, xs, XK[o]);
+        XK[o] = XK1[o];
+        XK1[o] = u;
+        const double un = touched ? u + lane_value(acc, l) / lane_mass(minv, bi, l) : u;
+        X[o] = un;
+        const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+        const double e = (un - u) / q;
+        e2 = e * e;
+        bad = isfinite(un) ? 0.0 : 1.0;
+    }
+    const double s = block_sum(e2, red);
+    const double nf = block_sum(bad, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = nf;
+    }
+}
+
+// res <= 1 ends the loop; otherwise the next w (or the fallback to the swept iterate when w is
+// not a positive finite number or the swept iterate holds a non-finite entry).  it < 0: the sweep
+// index is the device sweep counter ctrl[3] (graph-replayed chunks).
+__global__ void __launch_bounds__(kBlock) k_pjc_finalize(int nparts, const double* __restrict__ partial, int nv, int it,
+                                                         int* __restrict__ ctrl, double* __restrict__ res,
+                                                         int* __restrict__ hflag, double* __restrict__ cheb) {
+#pragma clang fp contract(off)
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    double s = 0, nf = 0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        s += partial[i];
+        nf += partial[nparts + i];
+    }
+    s = block_sum(s, red);
+    nf = block_sum(nf, red);
+    if (threadIdx.x != 0) return;
+    if (it < 0) it = ctrl[3];
+    ctrl[3] = ctrl[3] + 1;
+    const double err = 1.0 / sqrt((double)nv) * sqrt(s);
+    res[0] = err;
+    if (err <= 1.0) {
+        ctrl[1] = it + 1;
+        ctrl[2] = 0;
+        ctrl[0] = 1;
+        hflag[0] = 1;
+        return;
+    }
+    const double rho2 = cheb[2];
+    const double wn = it == 0 ? 1.0 / (1.0 - rho2 * 0.5) : 1.0 / (1.0 - (rho2 * 0.25) * cheb[0]);
+    if (!isfinite(wn) || wn <= 0.0 || nf != 0.0) {
+        cheb[0] = 1.0;
+        cheb[1] = 1.0;
+    } else {
+        cheb[0] = wn;
+        cheb[1] = 0.0;
+    }
+}
+
+// After the loop: the sweep input of the converged sweep (XK1, PK1), or the pending update of
+// the last sweep after max_iter; written to X / P (the solve's velocity and impulse buffers).
+__global__ void k_pjc_finish_bodies(int nb, const int* __restrict__ dof, double* __restrict__ X,
+                                    const double* __restrict__ XK, const double* __restrict__ XK1,
+                                    const int* __restrict__ ctrl, const double* __restrict__ cheb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * 6) return;
+    const int bi = t / 6, l = t % 6;
+    if (l >= dof[bi]) return;
+    const size_t o = (size_t)bi * kBodyStride + l;
+    if (ctrl[0]) X[o] = XK1[o];
+    else if (cheb[1] == 0.0) X[o] = cheb_extrap(cheb[0], X[o], XK[o]);
+}
+
+__global__ void k_pjc_finish_contacts(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                      const double* __restrict__ mu, double* __restrict__ P, const double* __restrict__ PK,
+                                      const double* __restrict__ PK1, const int* __restrict__ ctrl,
+                                      const double* __restrict__ cheb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nc) return;
+    const int k = t / nc, c = t % nc;
+    if ((ida[c] < 0 && idb[c] < 0) || k >= (mu[c] > 0 ? 3 : 1)) return;
+    if (ctrl[0]) P[t] = PK1[t];
+    else if (cheb[1] == 0.0) P[t] = cheb_extrap(cheb[0], P[t], PK[t]);
+}
+
 }  // namespace cdx
 
 using namespace cdx;
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
+static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
 
 extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info) {
     // theta only enters the joint rows of S (PjAssembler::buildAndFactorS / rhs)
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
-    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV)
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV && prm->accel != CDX_ACCEL_CHEBYSHEV)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: unknown acceleration mode");
     const bool joints = ctx->njrows > 0;
     if (joints && prm->accel != CDX_ACCEL_NONE)
@@ -541,6 +884,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
             return hip_check(ctx, hipGetLastError(), "pj nesterov sweep launch");
         };
         st = run_sweeps(ctx, prm->max_iter, 10, 4, nsweep, &iters);
+    } else if (prm->accel == CDX_ACCEL_CHEBYSHEV) {
+        st = run_chebyshev(ctx, prm, &iters);
     } else {
         st = run_plain_sweeps(ctx, prm, gb, gc, &iters);
     }
@@ -597,4 +942,59 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     // one chunk = the sweeps between two residual checks
     return run_sweeps_graphed(ctx, prm->max_iter, check_every, true,
                               [&](int it, int pos) { return launch_sweep(it, pos == 0); }, iters_out);
+}
+
+// chebyshev_loop with estimateSpectralRadius (projected_jacobi.cpp:155-228, 470-476): buffers
+// X = d_x, XK = d_uprev, XK1 = d_xk1, P = d_p, PK = d_pk, PK1 = d_pk1.
+static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out) {
+    const int nb = ctx->nb, nc = ctx->nc;
+    const int gb8 = grid_for(nb * kLanesPerBody);
+    const int gc4 = grid_for(nc * kLanesPerContact);
+    const int gb6 = grid_for(nb * 6);
+    const size_t n8 = (size_t)nb * kBodyStride;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_xk1, n8, &ctx->xk1_cap, "chebyshev xk1"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)nc, &ctx->pk_cap, "chebyshev pk"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_pk1, 3 * (size_t)nc, &ctx->pk1_cap, "chebyshev pk1"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_cheb, 8, nullptr, "chebyshev scalars"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)gb8, &ctx->partial2_cap, "chebyshev partials"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "power-iteration partials"));
+    // spectral radius (the vector lives in d_xk1 until the loop starts)
+    const double c0[8] = {1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_cheb, c0, sizeof(c0), hipMemcpyHostToDevice, ctx->stream));
+    CDX_LAUNCH(ctx, "cheb_power", k_cheb_fill, gb6, kBlock, nb, ctx->d_dof, 1.0 / std::sqrt((double)ctx->nv), ctx->d_xk1);
+    for (int it = 0; it < 8; ++it) {
+        CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                   ctx->d_mu, ctx->d_xk1, ctx->d_imp, ctx->d_cheb);
+        CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
+                   ctx->d_incidx, ctx->d_imp, ctx->d_xk1, ctx->d_partial, ctx->d_cheb);
+        CDX_LAUNCH(ctx, "cheb_power", k_cheb_norm, 1, kBlock, gb8, ctx->d_partial, ctx->d_cheb);
+        CDX_LAUNCH(ctx, "cheb_power", k_cheb_scale, gb6, kBlock, nb, ctx->d_dof, ctx->d_xk1, ctx->d_cheb);
+    }
+    CDX_LAUNCH(ctx, "cheb_power", k_cheb_start, 1, 1, ctx->d_cheb);
+    CDX_HIP(ctx, hipGetLastError());
+    // xk = xk1 = x0, pk = pk1 = p0
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_uprev, ctx->d_x, sizeof(double) * n8, hipMemcpyDeviceToDevice, ctx->stream));
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_xk1, ctx->d_x, sizeof(double) * n8, hipMemcpyDeviceToDevice, ctx->stream));
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk1, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
+    auto launch_sweep = [&](int it, int) -> int {
+        CDX_LAUNCH(ctx, "pjc_contacts", k_pjc_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                   ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_x, ctx->d_uprev,
+                   ctx->d_imp, ctx->d_ctrl, ctx->d_cheb);
+        CDX_LAUNCH(ctx, "pjc_bodies", k_pjc_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   ctx->d_imp, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
+                   ctx->d_cheb);
+        CDX_LAUNCH(ctx, "pjc_residual", k_pjc_finalize, 1, kBlock, gb8, ctx->d_partial2, ctx->nv, it, ctx->d_ctrl, ctx->d_res,
+                   ctx->h_flag_dev, ctx->d_cheb);
+        return hip_check(ctx, hipGetLastError(), "pj chebyshev sweep launch");
+    };
+    const int st = run_sweeps_graphed(ctx, prm->max_iter, 8, true, launch_sweep, iters_out);
+    if (st) return st;
+    CDX_LAUNCH(ctx, "pjc_finish", k_pjc_finish_bodies, gb6, kBlock, nb, ctx->d_dof, ctx->d_x, ctx->d_uprev, ctx->d_xk1,
+               ctx->d_ctrl, ctx->d_cheb);
+    CDX_LAUNCH(ctx, "pjc_finish", k_pjc_finish_contacts, grid_for(3 * nc), kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_mu,
+               ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_ctrl, ctx->d_cheb);
+    CDX_HIP(ctx, hipGetLastError());
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CDX_OK;
 }

@@ -32,6 +32,7 @@ STATUS_NAMES = {
 }
 CDX_ACCEL_NONE = 0
 CDX_ACCEL_NESTEROV = 1
+CDX_ACCEL_CHEBYSHEV = 2
 
 # Every symbol include/cdx/cdx.h declares.
 EXPORTED = [

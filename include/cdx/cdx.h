@@ -57,8 +57,9 @@ enum cdx_status {
 };
 
 enum cdx_accel {
-    CDX_ACCEL_NONE = 0,    /* standard_loop (projected_jacobi.cpp:371-377) */
-    CDX_ACCEL_NESTEROV = 1 /* nesterov_loop (projected_jacobi.cpp:379-449) */
+    CDX_ACCEL_NONE = 0,     /* standard_loop (projected_jacobi.cpp:371-377) */
+    CDX_ACCEL_NESTEROV = 1, /* nesterov_loop (projected_jacobi.cpp:379-449) */
+    CDX_ACCEL_CHEBYSHEV = 2 /* chebyshev_loop (projected_jacobi.cpp:180-228), pj.chebyshev without pj.nesterov */
 };
 
 /* Solver parameters; the Config keys they stand for (src/config/config.hpp:61-113). */
@@ -70,7 +71,7 @@ typedef struct {
     double tol_rel;    /* pj.tol_rel (default 1e-5) */
     int check_every;   /* PJ residual cadence (reference: every 10 sweeps, projected_jacobi.cpp:375) */
     int warmstart;     /* pj.warmstart (default 1) */
-    int accel;         /* enum cdx_accel (pj.nesterov) */
+    int accel;         /* enum cdx_accel (pj.nesterov, pj.chebyshev) */
     double nesterov_beta_threshold; /* pj.nesterov_beta_threshold (default 1.0) */
     int nesterov_restart_limit;     /* pj.nesterov_restart_limit (default 10000) */
 } cdx_solver_params;

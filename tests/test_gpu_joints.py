@@ -145,8 +145,9 @@ def test_joint_errors(ctx):
     ctx.set_joints(rows)
     with pytest.raises(native.CdxError):
         ctx.gs_colored_solve(1e-3, 1.0, params())
-    with pytest.raises(native.CdxError):
-        ctx.pj_solve(1e-3, 1.0, params(accel=native.CDX_ACCEL_NESTEROV))
+    for accel in (native.CDX_ACCEL_NESTEROV, native.CDX_ACCEL_CHEBYSHEV):
+        with pytest.raises(native.CdxError):
+            ctx.pj_solve(1e-3, 1.0, params(accel=accel))
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
     info = ctx.pj_solve(1e-3, 1.0, params())

@@ -460,3 +460,27 @@ def test_chebyshev_same_fixed_point_and_nesterov_precedence():
     both = oracle.pj_solve(prob, 1e-3, oracle.params(nesterov=1, chebyshev=1, **prm))
     assert nes[3].iters == both[3].iters
     np.testing.assert_array_equal(nes[1], both[1])
+
+
+def orthogonal_point_mass(m=2.0):
+    """One point mass against three static frictionless planes with normals e_x, e_y, e_z:
+    W = I, so the linear sweep map is G = (1 - alpha) I exactly."""
+    fr = np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1.0], [0, 1, 0, 0, 0, 1, 1, 0, 0.0], [0, 0, 1, 1, 0, 0, 0, 1, 0.0]])
+    return scenes.ContactProblem(
+        dof=np.array([3], np.int32), minv=np.full(3, 1 / m), v=np.array([0.3, -0.2, -1.0]),
+        fext=np.array([0, 0, -m * G]), fgyro=np.zeros(3), q=np.zeros(3), body_a=np.full(3, -1, np.int32),
+        body_b=np.zeros(3, np.int32), mu=np.zeros(3), frame=fr, angA=np.zeros((3, 9)), angB=np.zeros((3, 9)),
+        rest=np.zeros((3, 3)), vbias=np.zeros((3, 3)), warm=np.zeros((3, 3)))
+
+
+@pytest.mark.parametrize("alpha", [0.5, 0.8, 1.0])
+def test_chebyshev_rho_closed_form(alpha):
+    """G = (1 - alpha) I: the power iteration returns 1 - alpha, and at alpha = 1 the mapped
+    vector vanishes, which ends the estimate at rho = 0 (no acceleration: w = 1 throughout).
+    The solution is the same in every case: the +x plane separates, the -y and -z ones hold."""
+    prob = orthogonal_point_mass()
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(alpha=alpha, chebyshev=1, tol_abs=1e-12,
+                                                                tol_rel=1e-12))
+    assert st == 0 and res.rho == pytest.approx(1.0 - alpha, abs=1e-15)
+    np.testing.assert_allclose(v, [0.3, 0.0, 0.0], atol=1e-9)
+    np.testing.assert_allclose(imp[:, 0], [0.0, 0.4, 2.0 * (1.0 + G * 1e-3)], rtol=1e-9)
