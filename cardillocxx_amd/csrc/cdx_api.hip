@@ -177,6 +177,7 @@ void cdx_default_params(cdx_solver_params* p) {
     p->accel = CDX_ACCEL_NONE;
     p->nesterov_beta_threshold = 1.0;
     p->nesterov_restart_limit = 10000;
+    p->block_precond = 0;
 }
 
 int cdx_ctx_create(int device, cdx_ctx** out) {
@@ -208,7 +209,7 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
     void* bufs[] = {ctx->d_x, ctx->d_v, ctx->d_minv, ctx->d_fext, ctx->d_fgyro, ctx->d_q, ctx->d_uprev, ctx->d_ufree, ctx->d_dof,
                     ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_p, ctx->d_R, ctx->d_bias,
                     ctx->d_imp, ctx->d_incptr, ctx->d_incidx, ctx->d_color_blocks, ctx->d_ctrl, ctx->d_res, ctx->d_partial,
-                    ctx->d_partial2, ctx->d_pk, ctx->d_nest, ctx->d_xk1, ctx->d_pk1, ctx->d_cheb, ctx->d_scene_i, ctx->d_scene_res,
+                    ctx->d_partial2, ctx->d_pk, ctx->d_nest, ctx->d_xk1, ctx->d_pk1, ctx->d_cheb, ctx->d_Rb, ctx->d_scene_i, ctx->d_scene_res,
                     ctx->d_rec_x, ctx->d_rec_p, ctx->d_adj, ctx->d_adj_imp, ctx->d_adj_body, ctx->d_adj_partial,
                     ctx->d_jc_off, ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jbody,
                     ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jr, ctx->d_jrlam, ctx->d_jz, ctx->d_jlam, ctx->d_jL,

@@ -178,6 +178,8 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     CDX_TRY(check_solve_args(ctx, prm));
     if (ctx->njrows > 0) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: joint rows are solved by cdx_pj_solve only");
     if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: only the standard loop is batched");
+    if (prm->block_precond)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve uses the row scaling only");
     if (ctx->nscenes < 1 || ctx->scene_boff.back() != ctx->nb || ctx->scene_coff.back() != ctx->nc)
         return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
     ctx->solved = false;
@@ -204,6 +206,7 @@ int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
+    if (prm->block_precond) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve uses the row scaling only");
     if (ctx->nscenes < 1) return set_err(ctx, CDX_NO_STATE, "cdx_batch_autotune: scenes not set");
     CDX_TRY(build_incidence(ctx));
     const size_t np = 3 * (size_t)ctx->nc;

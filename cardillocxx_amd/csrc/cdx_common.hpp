@@ -90,6 +90,8 @@ struct cdx_ctx {
     double* d_p = nullptr;     // [3*nc] impulses (PJ: normal >= 0, GS: lambda <= 0 during the solve)
     double* d_R = nullptr;     // [3*nc]
     double* d_bias = nullptr;  // [3*nc]
+    double* d_Rb = nullptr;    // [9*nc] inverse Delassus blocks, entry (k, j) at (3k + j) * nc + c (block mode)
+    size_t Rb_cap = 0;
     double* d_imp = nullptr;   // [12*nc]
     int* d_incptr = nullptr;   // [nb+1]
     int* d_incidx = nullptr;   // [2*nc]

@@ -26,17 +26,18 @@ namespace cdx {
 // Setup (build_workspace, projected_jacobi.cpp:316-369)
 // ---------------------------------------------------------------------------------------------
 
-// R = alpha / D (rdiag_sparse, :57-69); b = R .* (c + e .* (W v + c)) (:337-347);
+// R = alpha / D (rdiag_sparse, :57-69); b = R .* (c + e .* (W v + c)) (:337-347), or with Rb the
+// inverse Delassus blocks and b unscaled (block mode, :341-343);
 // p0 = warm start (:358-359); side impulses of W^T p0 for x0 (:361-363).
 __global__ void k_pj_setup_contacts(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
                                     const double* __restrict__ J, const double* __restrict__ mu,
                                     const double* __restrict__ rest, const double* __restrict__ vb,
                                     const double* __restrict__ minv, const double* __restrict__ v, double alpha,
                                     int warmstart, double* __restrict__ R, double* __restrict__ bias,
-                                    double* __restrict__ p, double* __restrict__ imp) {
+                                    double* __restrict__ p, double* __restrict__ imp, double* __restrict__ Rb) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+    pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp, Rb);
 }
 
 // x0 = S^-1 (M v + dt f_ext + dt f_gyro + W^T p0) (pj_assembler.cpp:150-153,173-182).
@@ -86,6 +87,21 @@ __device__ __forceinline__ double project4(double pt, int k, double muc) {
     return pnew;
 }
 
+// Row k of R_block t for the contact held by a group of 4 lanes (lane k has t_k; block mode):
+// sum_l Rb(k, l) t_l over the contact's rows, in row order (BlockDiagonal::applyTo,
+// block_diagonal.hpp:103-124).  Every lane takes part in the shuffles; rows only read Rb.
+__device__ __forceinline__ double block_apply4(double t, int k, int c, int nc, int nr, bool row,
+                                               const double* __restrict__ Rb) {
+    const int base = threadIdx.x & ~(kLanesPerContact - 1);
+    const double t0 = __shfl(t, base, 64);
+    const double t1 = __shfl(t, base + 1, 64);
+    const double t2 = __shfl(t, base + 2, 64);
+    if (!row) return 0.0;
+    const double* r = Rb + (size_t)(3 * k) * nc + c;
+    if (nr == 1) return r[0] * t0;
+    return r[0] * t0 + r[nc] * t1 + r[2 * (size_t)nc] * t2;
+}
+
 // The contact's compact record [lin | angA | angB] = sum over rows of (d_k, aA_k, aB_k) dp_k,
 // summed over the group of 4 by an xor tree; lane 0 writes lin, lane 1 angA (dynamic A only),
 // lane 2 angB (dynamic B only).
@@ -114,6 +130,9 @@ __device__ __forceinline__ void write_record4(bool contact, int k, int c, int a,
     }
 }
 
+// Block = true: R points to the inverse Delassus blocks and the bias is unscaled
+// (pj.rdiag_true_delassus): dp = Pi(p - Rb (W x + b)) - p.
+template <bool Block>
 __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, const int* __restrict__ ida,
                                                          const int* __restrict__ idb, const double* __restrict__ J,
                                                          const double* __restrict__ R,
@@ -148,8 +167,12 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
         load_body(x, b, ub);
         const double g = row_dot(d, aA, aB, ua, ub);
         pold = p[(size_t)k * nc + c];
-        pt = pold - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+        if constexpr (Block)
+            pt = g + (use_bias ? bias[(size_t)k * nc + c] : 0.0);
+        else
+            pt = pold - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
+    if constexpr (Block) pt = pold - block_apply4(pt, k, c, nc, nr, row, R);
     const double pnew = project4(pt, k, muc);
     double dp = 0;
     if (row) {
@@ -310,7 +333,8 @@ __global__ void __launch_bounds__(kBlock) k_pjn_contacts(int nc, int nblocks, co
                                                          const double* __restrict__ mu, double* __restrict__ p,
                                                          double* __restrict__ pk, const double* __restrict__ x,
                                                          const double* __restrict__ xk, double* __restrict__ imp,
-                                                         const int* __restrict__ ctrl, const double* __restrict__ nest) {
+                                                         const int* __restrict__ ctrl, const double* __restrict__ nest,
+                                                         const double* __restrict__ Rb) {
     if (ctrl[0]) return;
     const double beta = nest[0];
     const int c = xcd_tile(nblocks) * kBlock + threadIdx.x;
@@ -336,7 +360,16 @@ __global__ void __launch_bounds__(kBlock) k_pjn_contacts(int nc, int nblocks, co
         py[k] = beta == 0.0 ? pc : pc + beta * (pc - pp);
         pk[(size_t)k * nc + c] = pc;
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
-        pn[k] = py[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+        pn[k] = Rb ? g + bias[(size_t)k * nc + c] : py[k] - (R[(size_t)k * nc + c] * g + bias[(size_t)k * nc + c]);
+    }
+    if (Rb) {  // p_y - R_block (W y + b)
+        double t[3] = {pn[0], pn[1], pn[2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nr) break;
+            const double* r = Rb + (size_t)(3 * k) * nc + c;
+            pn[k] = py[k] - (nr == 1 ? r[0] * t[0] : r[0] * t[0] + r[nc] * t[1] + r[2 * (size_t)nc] * t[2]);
+        }
     }
     pn[0] = fmax(pn[0], 0.0);
     if (nr == 3) {
@@ -517,7 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_lin_contacts4(int nc, int nbloc
                                                                const int* __restrict__ idb, const double* __restrict__ J,
                                                                const double* __restrict__ R, const double* __restrict__ mu,
                                                                const double* __restrict__ v, double* __restrict__ imp9,
-                                                               const double* __restrict__ cheb) {
+                                                               const double* __restrict__ cheb, const double* __restrict__ Rb) {
     if (cheb[4] != 0.0) return;
     const int tid = xcd_tile(nblocks) * kBlock + threadIdx.x;
     const int c = tid / kLanesPerContact, k = tid % kLanesPerContact;
@@ -543,8 +576,10 @@ __global__ void __launch_bounds__(kBlock) k_cheb_lin_contacts4(int nc, int nbloc
         double ua[6], ub[6];
         load_body(v, a, ua);
         load_body(v, b, ub);
-        dp = -(R[(size_t)k * nc + c] * row_dot(d, aA, aB, ua, ub));
+        dp = row_dot(d, aA, aB, ua, ub);
+        if (!Rb) dp = -(R[(size_t)k * nc + c] * dp);
     }
+    if (Rb) dp = -block_apply4(dp, k, c, nc, nr, row, Rb);
     write_record4(live && (a >= 0 || b >= 0), k, c, a, b, d, aA, aB, dp, imp9);
 }
 
@@ -645,7 +680,7 @@ __global__ void __launch_bounds__(kBlock) k_pjc_contacts4(int nc, int nblocks, c
                                                           double* __restrict__ PK, double* __restrict__ PK1,
                                                           const double* __restrict__ X, const double* __restrict__ XK,
                                                           double* __restrict__ imp9, const int* __restrict__ ctrl,
-                                                          const double* __restrict__ cheb) {
+                                                          const double* __restrict__ cheb, const double* __restrict__ Rb) {
     if (ctrl[0]) return;
     const double w = cheb[0];
     const bool take = cheb[1] != 0.0;
@@ -679,8 +714,9 @@ __global__ void __launch_bounds__(kBlock) k_pjc_contacts4(int nc, int nblocks, c
         cheb_input6(X, XK, a, w, take, ua);
         cheb_input6(X, XK, b, w, take, ub);
         const double g = row_dot(d, aA, aB, ua, ub);
-        pt = pin - (R[r] * g + (use_bias ? bias[r] : 0.0));
+        pt = Rb ? g + (use_bias ? bias[r] : 0.0) : pin - (R[r] * g + (use_bias ? bias[r] : 0.0));
     }
+    if (Rb) pt = pin - block_apply4(pt, k, c, nc, nr, row, Rb);
     const double pnew = project4(pt, k, muc);
     double dp = 0;
     if (row) {
@@ -810,8 +846,11 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV && prm->accel != CDX_ACCEL_CHEBYSHEV)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: unknown acceleration mode");
     const bool joints = ctx->njrows > 0;
+    const bool block = prm->block_precond != 0;
     if (joints && prm->accel != CDX_ACCEL_NONE)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the accelerated loop does not take joint rows");
+    if (joints && block)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the block preconditioner does not take joint rows");
     if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: theta must be positive with joint rows");
     ctx->joints_solved = false;
     if (info) {
@@ -832,10 +871,11 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     CDX_TRY(build_incidence(ctx));
     CDX_TRY(reset_control(ctx));
     const bool have_contacts = ctx->nrows > 0;
+    if (have_contacts && block) CDX_TRY(ensure_alloc(ctx, &ctx->d_Rb, 9 * (size_t)nc, &ctx->Rb_cap, "inverse Delassus blocks"));
     if (have_contacts)
         CDX_LAUNCH(ctx, "pj_setup_contacts", k_pj_setup_contacts, grid_for(nc), kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
                    ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, prm->alpha, prm->warmstart, ctx->d_R,
-                   ctx->d_bias, ctx->d_p, ctx->d_imp);
+                   ctx->d_bias, ctx->d_p, ctx->d_imp, block ? ctx->d_Rb : nullptr);
     CDX_LAUNCH(ctx, "pj_setup_bodies", k_pj_setup_bodies, grid_for(nb), kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v,
                ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_imp, dt, have_contacts ? 1 : 0, ctx->d_x,
                joints ? ctx->d_jslot : nullptr, joints ? ctx->d_jtr : nullptr);
@@ -852,9 +892,9 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
             if (ctrl[0] && ctrl[2]) return set_err(ctx, ctrl[2], "cdx_pj_solve: S is not factorized (singular joint block)");
         }
         ctx->solved = true;
-        ctx->pj_last = !joints;
+        ctx->pj_last = !joints && !block;
         ctx->joints_solved = joints;
-        ctx->record_valid = ctx->record && !joints;
+        ctx->record_valid = ctx->record && !joints && !block;
         return CDX_OK;
     }
     const int gb = grid_for(nb), gc = grid_for(nc);
@@ -874,7 +914,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         auto nsweep = [&](int it) -> int {
             CDX_LAUNCH(ctx, "pj_contacts", k_pjn_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
                        ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_x, ctx->d_uprev, ctx->d_imp, ctx->d_ctrl,
-                       ctx->d_nest);
+                       ctx->d_nest, block ? ctx->d_Rb : nullptr);
             CDX_LAUNCH(ctx, "pj_bodies", k_pjn_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                        ctx->d_imp, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
                        ctx->d_nest);
@@ -899,15 +939,16 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     }
     ctx->solved = true;
     ctx->joints_solved = joints;
-    ctx->pj_last = !joints;  // the adjoint differentiates the diagonal-S solve only
+    ctx->pj_last = !joints && !block;  // the adjoint differentiates the row-scaled, diagonal-S solve only
     ctx->recorded = iters;
-    ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && iters <= ctx->record_max;
+    ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && !block && iters <= ctx->record_max;
     return CDX_OK;
 }
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out) {
     const int nb = ctx->nb, nc = ctx->nc;
     const int check_every = prm->check_every;
+    const bool block = prm->block_precond != 0;
     const int gb8 = grid_for(nb * kLanesPerBody);
     const int gc4 = grid_for(nc * kLanesPerContact);
     (void)gc;
@@ -920,8 +961,12 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     }
     // one sweep; it < 0 inside a captured chunk (the residual kernel then reads the chunk counter)
     auto launch_sweep = [&](int it, int check) -> int {
-        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                   ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        if (block)
+            CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<true>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_Rb, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        else
+            CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<false>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl,
                    gj ? ctx->d_jslot : nullptr, gj ? ctx->d_jtr : nullptr);
@@ -952,6 +997,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     const int gc4 = grid_for(nc * kLanesPerContact);
     const int gb6 = grid_for(nb * 6);
     const size_t n8 = (size_t)nb * kBodyStride;
+    const double* Rb = prm->block_precond ? ctx->d_Rb : nullptr;
     CDX_TRY(ensure_alloc(ctx, &ctx->d_xk1, n8, &ctx->xk1_cap, "chebyshev xk1"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)nc, &ctx->pk_cap, "chebyshev pk"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_pk1, 3 * (size_t)nc, &ctx->pk1_cap, "chebyshev pk1"));
@@ -964,7 +1010,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     CDX_LAUNCH(ctx, "cheb_power", k_cheb_fill, gb6, kBlock, nb, ctx->d_dof, 1.0 / std::sqrt((double)ctx->nv), ctx->d_xk1);
     for (int it = 0; it < 8; ++it) {
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                   ctx->d_mu, ctx->d_xk1, ctx->d_imp, ctx->d_cheb);
+                   ctx->d_mu, ctx->d_xk1, ctx->d_imp, ctx->d_cheb, Rb);
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
                    ctx->d_incidx, ctx->d_imp, ctx->d_xk1, ctx->d_partial, ctx->d_cheb);
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_norm, 1, kBlock, gb8, ctx->d_partial, ctx->d_cheb);
@@ -980,7 +1026,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     auto launch_sweep = [&](int it, int) -> int {
         CDX_LAUNCH(ctx, "pjc_contacts", k_pjc_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
                    ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_x, ctx->d_uprev,
-                   ctx->d_imp, ctx->d_ctrl, ctx->d_cheb);
+                   ctx->d_imp, ctx->d_ctrl, ctx->d_cheb, Rb);
         CDX_LAUNCH(ctx, "pjc_bodies", k_pjc_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
                    ctx->d_cheb);

@@ -7,14 +7,81 @@
 
 namespace cdx {
 
-// Per-contact setup of one PJ solve (see k_pj_setup_contacts).
+// Entry (k, l) of the contact's Delassus block W_c M^-1 W_c^T: sums over each side's columns in
+// column order, the side with the lower body index first (as row_D).
+__device__ inline double block_entry(const ContactJ& j, int k, int l, double4 mA, double4 mB, int a, int b) {
+    double DA = 0, DB = 0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) DA += j.d[k][m] * j.d[l][m] * mA.x;
+    DA += j.aA[k][0] * j.aA[l][0] * mA.y;
+    DA += j.aA[k][1] * j.aA[l][1] * mA.z;
+    DA += j.aA[k][2] * j.aA[l][2] * mA.w;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) DB += j.d[k][m] * j.d[l][m] * mB.x;
+    DB += j.aB[k][0] * j.aB[l][0] * mB.y;
+    DB += j.aB[k][1] * j.aB[l][1] * mB.z;
+    DB += j.aB[k][2] * j.aB[l][2] * mB.w;
+    return (a >= 0 && b >= 0 && b < a) ? DB + DA : DA + DB;
+}
+
+// Inverse of an SPD 3x3 block by Cholesky and two triangular solves per column (invertSmallSpd,
+// block_diagonal.hpp:15-27); the inverse of the diagonal when a pivot is not positive.
+__device__ inline void spd_inverse3(const double A[3][3], double Ai[3][3]) {
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double d = A[k][k];
+        for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
+        ok = ok && d > 0;
+        L[k][k] = sqrt(fmax(d, 0.0));
+        for (int i = k + 1; i < 3; ++i) {
+            double t = A[i][k];
+            for (int j = 0; j < k; ++j) t -= L[i][j] * L[k][j];
+            L[i][k] = t / L[k][k];
+        }
+    }
+    if (!ok) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Ai[i][j] = i == j ? 1.0 / A[i][i] : 0.0;
+        return;
+    }
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        double y[3], x[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double t = i == col ? 1.0 : 0.0;
+            for (int j = 0; j < i; ++j) t -= L[i][j] * y[j];
+            y[i] = t / L[i][i];
+        }
+#pragma unroll
+        for (int i = 2; i >= 0; --i) {
+            double t = y[i];
+            for (int j = i + 1; j < 3; ++j) t -= L[j][i] * x[j];
+            x[i] = t / L[i][i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Ai[i][col] = x[i];
+    }
+}
+
+// Per-contact setup of one PJ solve (see k_pj_setup_contacts).  With Rb (block mode,
+// pj.rdiag_true_delassus, projected_jacobi.cpp:330-348) the bias stays unscaled and Rb receives
+// alpha (W_c M^-1 W_c^T)^-1 (buildBlockPreconditioner, :81-98): the 1x1 block of a frictionless
+// row or the 3x3 block of a frictional contact, entry (k, l) at (3k + l) nc + c.
 __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
                                         const double* __restrict__ J, const double* __restrict__ mu,
                                         const double* __restrict__ rest, const double* __restrict__ vb,
                                         const double* __restrict__ minv, const double* __restrict__ v, double alpha,
                                         int warmstart, double* __restrict__ R, double* __restrict__ bias,
-                                        double* __restrict__ p, double* __restrict__ imp) {
+                                        double* __restrict__ p, double* __restrict__ imp,
+                                        double* __restrict__ Rb = nullptr) {
     const int a = ida[c], b = idb[c];
+    if (Rb)
+        for (int k = 0; k < 9; ++k) Rb[(size_t)k * nc + c] = 0;
     if (a < 0 && b < 0) {
         for (int k = 0; k < 3; ++k) {
             R[(size_t)k * nc + c] = 0;
@@ -42,10 +109,28 @@ __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ i
         const double Rk = D > 0 ? alpha / D : 0.0;
         const double ck = vb[(size_t)k * nc + c];
         const double vprev = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub) + ck;
+        const double bk = ck + rest[(size_t)k * nc + c] * vprev;
         R[(size_t)k * nc + c] = Rk;
-        bias[(size_t)k * nc + c] = Rk * (ck + rest[(size_t)k * nc + c] * vprev);
+        bias[(size_t)k * nc + c] = Rb ? bk : Rk * bk;
         pw[k] = warmstart ? p[(size_t)k * nc + c] : 0.0;
         p[(size_t)k * nc + c] = pw[k];
+    }
+    if (Rb) {
+        const double inv_alpha = alpha > 0 ? 1.0 / alpha : 0.0;
+        if (nr == 1) {
+            Rb[c] = 1.0 / (block_entry(j, 0, 0, mA, mB, a, b) * inv_alpha);
+        } else {
+            double A[3][3], Ai[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) A[k][l] = block_entry(j, k, l, mA, mB, a, b) * inv_alpha;
+            spd_inverse3(A, Ai);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) Rb[(size_t)(3 * k + l) * nc + c] = Ai[k][l];
+        }
     }
     write_side_impulses9(imp, c, a, b, nr, j, pw);
 }

@@ -78,6 +78,7 @@ class SolverParams(C.Structure):
         ("accel", C.c_int),
         ("nesterov_beta_threshold", C.c_double),
         ("nesterov_restart_limit", C.c_int),
+        ("block_precond", C.c_int),
     ]
 
 

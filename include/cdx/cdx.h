@@ -74,6 +74,9 @@ typedef struct {
     int accel;         /* enum cdx_accel (pj.nesterov, pj.chebyshev) */
     double nesterov_beta_threshold; /* pj.nesterov_beta_threshold (default 1.0) */
     int nesterov_restart_limit;     /* pj.nesterov_restart_limit (default 10000) */
+    int block_precond; /* pj.rdiag_true_delassus (default 0): per-contact inverse Delassus blocks alpha (W_c M^-1 W_c^T)^-1
+                          instead of the row scaling alpha / D_ii (projected_jacobi.cpp:71-98, 330-348); projected
+                          Jacobi only (ignored by Gauss-Seidel, refused by the batched solve and by cdx_pj_adjoint) */
 } cdx_solver_params;
 
 typedef struct {

@@ -243,6 +243,106 @@ static void delassus_diag(const orc_W* W, const double* minv, double* D) {
     }
 }
 
+/* sum_j W_rj W_sj M^-1_j over the common columns of rows r and s, in column order */
+static double row_pair_dot(const orc_W* W, int r, int s, const double* minv) {
+    double acc = 0;
+    int a = W->rowptr[r], b = W->rowptr[s];
+    const int ae = W->rowptr[r + 1], be = W->rowptr[s + 1];
+    while (a < ae && b < be) {
+        if (W->col[a] < W->col[b]) ++a;
+        else if (W->col[b] < W->col[a]) ++b;
+        else { acc += W->val[a] * W->val[b] * minv[W->col[a]]; ++a; ++b; }
+    }
+    return acc;
+}
+
+/* Inverse of a small SPD block by Cholesky (Eigen::LLT::solve of the identity, invertSmallSpd,
+ * block_diagonal.hpp:15-27): A = L L^T, then L y = e_i and L^T x = y per column.  Returns 0, or 1
+ * when a pivot is not positive (the LLT failure the reference then falls back from). */
+static int llt_inverse(int n, const double* A, double* Ainv) {
+    double L[9] = {0};
+    for (int k = 0; k < n; ++k) {
+        double d = A[k * n + k];
+        for (int j = 0; j < k; ++j) d -= L[k * 3 + j] * L[k * 3 + j];
+        if (!(d > 0)) return 1;
+        L[k * 3 + k] = sqrt(d);
+        for (int i = k + 1; i < n; ++i) {
+            double t = A[i * n + k];
+            for (int j = 0; j < k; ++j) t -= L[i * 3 + j] * L[k * 3 + j];
+            L[i * 3 + k] = t / L[k * 3 + k];
+        }
+    }
+    for (int col = 0; col < n; ++col) {
+        double y[3], x[3];
+        for (int i = 0; i < n; ++i) {
+            double t = (i == col) ? 1.0 : 0.0;
+            for (int j = 0; j < i; ++j) t -= L[i * 3 + j] * y[j];
+            y[i] = t / L[i * 3 + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double t = y[i];
+            for (int j = i + 1; j < n; ++j) t -= L[j * 3 + i] * x[j];
+            x[i] = t / L[i * 3 + i];
+        }
+        for (int i = 0; i < n; ++i) Ainv[i * n + col] = x[i];
+    }
+    return 0;
+}
+
+/* Exact per-contact Delassus block preconditioner (pj.rdiag_true_delassus:
+ * buildBlockPreconditioner, projected_jacobi.cpp:71-98, with BlockDiagonal::calculateInverse,
+ * block_diagonal.hpp:81-93): the block (W_c M^-1 W_c^T) / alpha of each contact (1x1 for a
+ * frictionless row, 3x3 for a frictional contact's rows) is inverted -- elementwise when every
+ * block is diagonal, otherwise each by Cholesky, falling back to the inverse of its diagonal when
+ * the factorisation fails.  (The reference tries LDLT in between; a block with a dynamic side is
+ * >= m^-1 I on its orthonormal frame rows, so the Cholesky step does not fail for contacts.)
+ * Rb holds, per row, that row of its block inverse: 3 entries (unused ones 0). */
+static void build_rblock(const orc_W* W, const double* minv, double alpha, double* Rb) {
+    const int nr = W->nrows, nnfc = W->nnfc;
+    const double inv_alpha = alpha > 0 ? 1.0 / alpha : 0.0;
+    double* blk = (double*)malloc(sizeof(double) * 3 * (size_t)(nr + 1));
+    int is_diag = 1;
+    for (int r = 0; r < nnfc; ++r) blk[3 * r] = row_pair_dot(W, r, r, minv) * inv_alpha;
+    for (int r = nnfc; r < nr; r += 3)
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) {
+                const double v = row_pair_dot(W, r + k, r + j, minv) * inv_alpha;
+                blk[3 * (r + k) + j] = v;
+                if (k != j && v != 0.0) is_diag = 0;
+            }
+    memset(Rb, 0, sizeof(double) * 3 * (size_t)nr);
+    for (int r = 0; r < nnfc; ++r) {
+        double inv = 1.0 / blk[3 * r];
+        if (!is_diag && llt_inverse(1, &blk[3 * r], &inv)) inv = 1.0 / blk[3 * r];
+        Rb[3 * r] = inv;
+    }
+    for (int r = nnfc; r < nr; r += 3) {
+        double A[9], Ai[9];
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) A[k * 3 + j] = blk[3 * (r + k) + j];
+        if (is_diag || llt_inverse(3, A, Ai)) {
+            memset(Ai, 0, sizeof(Ai));
+            for (int k = 0; k < 3; ++k) Ai[k * 3 + k] = 1.0 / A[k * 3 + k];
+        }
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) Rb[3 * (r + k) + j] = Ai[k * 3 + j];
+    }
+    free(blk);
+}
+
+/* out = R_block in (BlockDiagonal::applyTo, block_diagonal.hpp:103-124); out may alias in */
+static void apply_rblock(const orc_W* W, const double* Rb, double* v) {
+    for (int r = 0; r < W->nnfc; ++r) v[r] = Rb[3 * r] * v[r];
+    for (int r = W->nnfc; r < W->nrows; r += 3) {
+        double t[3];
+        for (int k = 0; k < 3; ++k) {
+            const double* b = Rb + 3 * (r + k);
+            t[k] = b[0] * v[r] + b[1] * v[r + 1] + b[2] * v[r + 2];
+        }
+        v[r] = t[0]; v[r + 1] = t[1]; v[r + 2] = t[2];
+    }
+}
+
 /* Residual (projected_jacobi.cpp:140-144). */
 static double pj_residual(int nv, const double* v, const double* vp, double tol_rel, double tol_abs) {
     double s = 0;
@@ -277,10 +377,10 @@ static void fill_mdiag(int nv, const double* minv, double* M) {
     for (int j = 0; j < nv; ++j) M[j] = minv[j] > 0 ? 1.0 / minv[j] : 0.0;
 }
 
-/* Spectral radius of the linear sweep map v -> v - S^-1 W^T R W v (estimateSpectralRadius,
- * projected_jacobi.cpp:155-178): 8 power iterations from the normalised ones vector, clamped to
+/* Spectral radius of the linear sweep map v -> v - S^-1 W^T R W v, R the row scaling or R_block
+ * (estimateSpectralRadius, projected_jacobi.cpp:155-178): 8 power iterations from the normalised ones vector, clamped to
  * 0.995; here S = diag(M). */
-static double estimate_rho(const orc_W* W, const orc_W* RW, const double* M, int nv) {
+static double estimate_rho(const orc_W* W, const orc_W* RW, const double* Rb, const double* M, int nv) {
     if (nv <= 0) return 0.0;
     double* v = (double*)malloc(sizeof(double) * (size_t)nv);
     double* r = (double*)malloc(sizeof(double) * (size_t)nv);
@@ -290,6 +390,7 @@ static double estimate_rho(const orc_W* W, const orc_W* RW, const double* M, int
     double lambda = 0.0;
     for (int it = 0; it < 8; ++it) {
         spmv(RW, v, d);
+        if (Rb) apply_rblock(W, Rb, d);
         memset(r, 0, sizeof(double) * (size_t)nv);
         spmv_t_acc(W, d, r);
         double n2 = 0.0;
@@ -357,11 +458,18 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     spmv(&W, B->v, bias);
     for (int r = 0; r < nr; ++r) bias[r] += W.cvec[r];
     for (int r = 0; r < nr; ++r) bias[r] = W.cvec[r] + W.rest[r] * bias[r];
-    delassus_diag(&W, B->minv, R);
-    for (int r = 0; r < nr; ++r) R[r] = (R[r] > 0) ? (prm->alpha / R[r]) : 0.0;
-    for (int r = 0; r < nr; ++r) bias[r] = R[r] * bias[r];
-    for (int r = 0; r < nr; ++r)
-        for (int k = W.rowptr[r]; k < W.rowptr[r + 1]; ++k) RWv[k] = R[r] * W.val[k];
+    double* Rb = NULL;
+    if (prm->block_precond) { /* RW = W, bias unscaled, R_block applied in the sweep (:341-343) */
+        Rb = (double*)malloc(sizeof(double) * 3 * (size_t)nr);
+        build_rblock(&W, B->minv, prm->alpha, Rb);
+        for (int k = 0; k < W.rowptr[nr]; ++k) RWv[k] = W.val[k];
+    } else {
+        delassus_diag(&W, B->minv, R);
+        for (int r = 0; r < nr; ++r) R[r] = (R[r] > 0) ? (prm->alpha / R[r]) : 0.0;
+        for (int r = 0; r < nr; ++r) bias[r] = R[r] * bias[r];
+        for (int r = 0; r < nr; ++r)
+            for (int k = W.rowptr[r]; k < W.rowptr[r + 1]; ++k) RWv[k] = R[r] * W.val[k];
+    }
 
     /* warm start (:358-359, warmstart.hpp:26-37) */
     if (prm->warmstart)
@@ -387,9 +495,12 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
             for (int j = 0; j < nv; ++j) xp[j] = x[j];                      \
             spmv_mt(&RW, x, dl, nthreads);                                  \
             _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
+            for (int r = 0; r < nr; ++r) dl[r] += bias[r];                  \
+            if (Rb) apply_rblock(&W, Rb, dl);                               \
+            _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
             for (int r = 0; r < nr; ++r) {                                  \
                 pp[r] = p[r];                                               \
-                p[r] -= dl[r] + bias[r];                                    \
+                p[r] -= dl[r];                                              \
             }                                                               \
             pj_project(&W, p, nthreads);                                    \
             _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
@@ -403,6 +514,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         memcpy(pp, p, sizeof(double) * (size_t)nr);                         \
         spmv(&RW, x, dl);                                                   \
         for (int r = 0; r < nr; ++r) dl[r] += bias[r];                      \
+        if (Rb) apply_rblock(&W, Rb, dl);                                   \
         for (int r = 0; r < nr; ++r) p[r] -= dl[r];                         \
         pj_project(&W, p, nthreads);                                        \
         for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];                  \
@@ -414,7 +526,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     if (!prm->nesterov && prm->chebyshev) {
         /* chebyshev_loop (:180-228): residual every sweep against the sweep's input; on
          * convergence the loop returns that input (x = xk1), not the swept iterate */
-        const double rho = estimate_rho(&W, &RW, M, nv);
+        const double rho = estimate_rho(&W, &RW, Rb, M, nv);
         const double rho2 = rho * rho;
         res->rho = rho;
         double* xk = (double*)malloc(sizeof(double) * (size_t)nv);
@@ -527,7 +639,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         if (W.row_size[ci] > 1) { o[1] = p[base + 1]; o[2] = p[base + 2]; }
     }
     memcpy(v_out, x, sizeof(double) * (size_t)nv);
-    free(R); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(M);
+    free(R); free(Rb); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(M);
     free(WT.ptr); free(WT.row); free(WT.ent);
     orc_free_W(&W);
     return ORC_OK;
@@ -603,7 +715,7 @@ static double joint_row_dot(const orc_joints* jt, const int* voff, const int* do
 
 int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
                         const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
-    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev) return ORC_BAD_ARG;
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev || prm->block_precond) return ORC_BAD_ARG;
     orc_W W;
     int st = orc_build_W(B, C, &W);
     if (st) { orc_free_W(&W); return st; }

@@ -65,6 +65,7 @@ typedef struct {
     double nesterov_beta_threshold;
     int nesterov_restart_limit;
     int chebyshev;     /* pj.chebyshev (ignored when nesterov is set, projected_jacobi.cpp:469) */
+    int block_precond; /* pj.rdiag_true_delassus: exact per-contact Delassus blocks instead of the row scaling */
 } orc_params;
 
 typedef struct {

@@ -32,7 +32,8 @@ class Contacts(C.Structure):
 class Params(C.Structure):
     _fields_ = [("alpha", C.c_double), ("relaxation", C.c_double), ("max_iter", C.c_int), ("tol_abs", C.c_double),
                 ("tol_rel", C.c_double), ("warmstart", C.c_int), ("nesterov", C.c_int),
-                ("nesterov_beta_threshold", C.c_double), ("nesterov_restart_limit", C.c_int), ("chebyshev", C.c_int)]
+                ("nesterov_beta_threshold", C.c_double), ("nesterov_restart_limit", C.c_int), ("chebyshev", C.c_int),
+                ("block_precond", C.c_int)]
 
 
 class Joints(C.Structure):
@@ -103,9 +104,9 @@ class _Pack:
 
 
 def params(alpha=0.3, relaxation=1.0, max_iter=200000, tol_abs=1e-5, tol_rel=1e-5, warmstart=1, nesterov=0,
-           nesterov_beta_threshold=1.0, nesterov_restart_limit=10000, chebyshev=0):
+           nesterov_beta_threshold=1.0, nesterov_restart_limit=10000, chebyshev=0, block_precond=0):
     return Params(alpha, relaxation, max_iter, tol_abs, tol_rel, warmstart, nesterov, nesterov_beta_threshold,
-                  nesterov_restart_limit, chebyshev)
+                  nesterov_restart_limit, chebyshev, block_precond)
 
 
 def _solve(name, prob, dt, prm, nthreads=None):

@@ -1,0 +1,123 @@
+"""GPU parity of the exact per-contact Delassus block preconditioner (pj.rdiag_true_delassus,
+projected_jacobi.cpp:71-98, 330-348) against the oracle: velocities and impulses to <= 1e-10
+relative after the same sweeps, in the standard, Nesterov and Chebyshev loops, and the error
+paths of the solvers that do not take it."""
+import numpy as np
+import pytest
+
+import oracle
+from cardillocxx_amd import native, scenes
+from test_oracle import sticking_contact
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel(a, b):
+    a = np.asarray(a, float).ravel()
+    b = np.asarray(b, float).ravel()
+    return np.abs(a - b).max(initial=0.0) / max(np.abs(b).max(initial=0.0), 1e-300)
+
+
+def params(**kw):
+    p = native.default_params()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def both(ctx, prob, dt=1e-3, accel=native.CDX_ACCEL_NONE, **kw):
+    okw = dict(kw)
+    if accel == native.CDX_ACCEL_NESTEROV:
+        okw["nesterov"] = 1
+    if accel == native.CDX_ACCEL_CHEBYSHEV:
+        okw["chebyshev"] = 1
+    st, v_ref, imp_ref, res = oracle.pj_solve(prob, dt, oracle.params(block_precond=1, **okw))
+    assert st == 0
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    info = ctx.pj_solve(dt, 1.0, params(block_precond=1, accel=accel, **kw))
+    return (ctx.velocities(), ctx.impulses(), info), (v_ref, imp_ref, res)
+
+
+CASES = [
+    dict(nb=2, nc=1, seed=0, p_static=1.0),
+    dict(nb=20, nc=60, seed=2),
+    dict(nb=50, nc=200, seed=3, p_point=0.3),
+    dict(nb=200, nc=300, seed=4, rest=0.4),
+    dict(nb=300, nc=450, seed=5, p_frictionless=0.0, p_static=0.3),
+    dict(nb=1000, nc=1500, seed=6, warm=False),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"nb{c['nb']}_nc{c['nc']}_s{c['seed']}")
+@pytest.mark.parametrize("sweeps", [1, 10, 137])
+def test_block_fixed_sweeps(ctx, case, sweeps):
+    prob = scenes.random_problem(**case)
+    (v, imp, info), (v_ref, imp_ref, res) = both(ctx, prob, max_iter=sweeps, tol_abs=0.0, tol_rel=0.0)
+    assert info.iters == res.iters == sweeps
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+@pytest.mark.parametrize("case", CASES[1:4], ids=lambda c: f"nb{c['nb']}_nc{c['nc']}_s{c['seed']}")
+@pytest.mark.parametrize("tol", [1e-5, 1e-9])
+def test_block_to_tolerance_same_iterations(ctx, case, tol):
+    prob = scenes.random_problem(**case)
+    (v, imp, info), (v_ref, imp_ref, res) = both(ctx, prob, tol_abs=tol, tol_rel=tol)
+    assert info.iters == res.iters and res.residual <= 1
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+@pytest.mark.parametrize("alpha", [0.3, 1.0])
+def test_block_sticking_contact_closed_form(ctx, alpha):
+    """A coupled 3x3 block: the contact velocity shrinks by exactly (1 - alpha) per sweep."""
+    prob, W = sticking_contact(1)
+    g0 = W @ prob.v
+    for k in (1, 3):
+        (v, imp, info), (v_ref, imp_ref, res) = both(ctx, prob, alpha=alpha, max_iter=k, tol_abs=0.0, tol_rel=0.0,
+                                                     warmstart=0)
+        np.testing.assert_allclose(W @ v, (1 - alpha) ** k * g0, rtol=0, atol=1e-13 * np.abs(g0).max())
+        # at alpha = 1 the solved velocity is zero up to rounding: compare on the scale of v_0
+        assert np.abs(v - v_ref).max() <= TOL * np.abs(prob.v).max() and rel(imp, imp_ref) <= TOL
+
+
+@pytest.mark.parametrize("accel", [native.CDX_ACCEL_NESTEROV, native.CDX_ACCEL_CHEBYSHEV])
+def test_block_accelerated_loops(ctx, accel):
+    for case in CASES[1:4]:
+        prob = scenes.random_problem(**case)
+        (v, imp, info), (v_ref, imp_ref, res) = both(ctx, prob, accel=accel, max_iter=40, tol_abs=0.0, tol_rel=0.0)
+        assert info.iters == res.iters == 40
+        assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+def test_block_pile_parity(ctx):
+    """A settled pile (sphere contacts: coupled tangential blocks through the lever arms)."""
+    pile = scenes.sphere_pile(n=5000, settle_steps=150, box=(1.5, 1.5, 3.0), seed=5)
+    prob = pile.problem()
+    (v, imp, info), (v_ref, imp_ref, res) = both(ctx, prob, dt=5e-4, max_iter=200, tol_abs=0.0, tol_rel=0.0)
+    assert info.iters == 200
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+def test_block_refused_where_not_supported(ctx):
+    prob = scenes.random_problem(nb=20, nc=40, seed=3)
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.pj_solve(1e-3, 1.0, params(block_precond=1, max_iter=20))
+    with pytest.raises(native.CdxError) as e:  # the adjoint differentiates the row-scaled solve only
+        ctx.pj_adjoint(1e-3, params(block_precond=1), np.ones(prob.nv))
+    assert e.value.status == native.CDX_NO_STATE
+    ctx.set_scenes([0, prob.nb], [0, prob.nc])
+    with pytest.raises(native.CdxError) as e:
+        ctx.batch_pj_solve(1e-3, 1.0, params(block_precond=1))
+    assert e.value.status == native.CDX_BAD_ARG
+    # Gauss-Seidel has its own local solves: the flag does not change it (each solve warm-starts
+    # from the contacts' impulses, so both start from the uploaded ones)
+    out = []
+    for blk in (0, 1):
+        ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+        ctx.set_contacts(*prob.contact_arrays())
+        info = ctx.gs_colored_solve(1e-3, 1.0, params(block_precond=blk, max_iter=30, tol_abs=1e-14, tol_rel=1e-14))
+        out.append((info.iters, ctx.velocities()))
+    assert out[0][0] == out[1][0] == 30 and np.array_equal(out[0][1], out[1][1])

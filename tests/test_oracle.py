@@ -484,3 +484,106 @@ def test_chebyshev_rho_closed_form(alpha):
     assert st == 0 and res.rho == pytest.approx(1.0 - alpha, abs=1e-15)
     np.testing.assert_allclose(v, [0.3, 0.0, 0.0], atol=1e-9)
     np.testing.assert_allclose(imp[:, 0], [0.0, 0.4, 2.0 * (1.0 + G * 1e-3)], rtol=1e-9)
+
+
+def _block_R(prob, alpha=0.3):
+    """Dense R_block = alpha * blockdiag(W_c M^-1 W_c^T)^-1 in the row order of delassus()."""
+    W, _, order = delassus(prob)
+    R = np.zeros((len(W), len(W)))
+    r = 0
+    for c in order:
+        n = 3 if prob.mu[c] > 0 else 1
+        blk = W[r:r + n] @ np.diag(prob.minv) @ W[r:r + n].T
+        R[r:r + n, r:r + n] = alpha * np.linalg.inv(blk)
+        r += n
+    return W, R
+
+
+def sticking_contact(seed=0, p_inf=(1.0, 0.1, -0.2), mu=0.9):
+    """One 6-dof body against a static side (and a free one) with random lever arms (a coupled 3x3 Delassus
+    block), no forces, no bias, velocity v = -M^-1 W^T p_inf: the unconstrained contact solution
+    is p_inf (inside the cone), after which the contact velocity is zero."""
+    prob = scenes.random_problem(nb=2, nc=1, seed=seed, p_static=1.0, p_frictionless=0.0, p_point=0.0, warm=False)
+    prob.mu[:] = mu
+    prob.fext[:] = 0
+    prob.fgyro[:] = 0
+    prob.vbias[:] = 0
+    prob.rest[:] = 0
+    W, _, _ = delassus(prob)
+    prob.v[:] = -prob.minv * (W.T @ np.asarray(p_inf))
+    return prob, W
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("alpha", [0.3, 1.0])
+def test_block_preconditioner_single_contact_closed_form(seed, alpha):
+    """With R_block = alpha D^-1 (D the contact's own coupled 3x3 block) and the projection
+    inactive, a sweep maps the contact velocity gamma to (1 - alpha) gamma in all three
+    components: after k sweeps W x_k = (1 - alpha)^k gamma_0 and p_k = (1 - (1 - alpha)^k) p_inf
+    (alpha = 1 solves the contact in one sweep).  The row scaling cannot do this for a coupled
+    block, which the last assertion checks."""
+    p_inf = np.array([1.0, 0.1, -0.2])
+    prob, W = sticking_contact(seed, p_inf)
+    blk = W @ np.diag(prob.minv) @ W.T
+    assert np.abs(blk - np.diag(np.diag(blk))).max() > 1e-3 * np.abs(blk).max()  # coupled
+    g0 = W @ prob.v
+    for k in (1, 2, 5):
+        prm = dict(alpha=alpha, max_iter=k, tol_abs=0, tol_rel=0, warmstart=0)
+        st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, **prm))
+        assert st == 0 and res.iters == k
+        np.testing.assert_allclose(W @ v, (1 - alpha) ** k * g0, rtol=0, atol=1e-13 * np.abs(g0).max())
+        np.testing.assert_allclose(imp[0] * [1, 1, 1], (1 - (1 - alpha) ** k) * p_inf, rtol=0, atol=1e-13)
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(max_iter=1, tol_abs=0, tol_rel=0, warmstart=0,
+                                                              alpha=alpha))
+    assert np.abs(W @ v - (1 - alpha) * g0).max() > 1e-6 * np.abs(g0).max()
+
+
+def test_block_preconditioner_diagonal_blocks_equal_row_scaling():
+    """Frictionless rows (1x1 blocks) and point masses on orthonormal frames (diagonal 3x3
+    blocks): R_block is the row scaling alpha / D_ii, so both modes give the same iterates."""
+    for kw in (dict(nb=30, nc=60, seed=1, p_frictionless=1.0), dict(nb=30, nc=60, seed=2, p_point=1.0)):
+        prob = scenes.random_problem(**kw)
+        prm = dict(max_iter=40, tol_abs=0, tol_rel=0)
+        a = oracle.pj_solve(prob, 1e-3, oracle.params(**prm))
+        b = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, **prm))
+        np.testing.assert_allclose(b[1], a[1], rtol=0, atol=1e-13 * np.abs(a[1]).max())
+        np.testing.assert_allclose(b[2], a[2], rtol=0, atol=1e-13 * np.abs(a[2]).max())
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_block_preconditioner_lcp_and_rho(seed):
+    """Frictionless LCP (brute force over active sets) solved in block mode, and the Chebyshev
+    estimate against dense powers of G = I - M^-1 W^T R_block W on a coupled problem."""
+    rng = np.random.default_rng(seed)
+    prob = scenes.random_problem(nb=3, nc=2 + seed % 3, seed=100 + seed, p_static=0.3, p_frictionless=1.0,
+                                 p_point=0.0, warm=False)
+    prob.v[:] = rng.normal(size=prob.nv)
+    W, cvec, order = delassus(prob)
+    uf = prob.v + prob.minv * (1e-3 * prob.fext) + prob.minv * (1e-3 * prob.fgyro)
+    A = W @ np.diag(prob.minv) @ W.T
+    b = W @ uf + cvec
+    best = None
+    for act in itertools.product([0, 1], repeat=len(b)):
+        S = [i for i in range(len(b)) if act[i]]
+        p = np.zeros(len(b))
+        if S:
+            p[S] = np.linalg.solve(A[np.ix_(S, S)], -b[S])
+        w = A @ p + b
+        if (p >= -1e-12).all() and (w >= -1e-10).all() and abs(p @ w) < 1e-10:
+            best = p
+            break
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, tol_abs=1e-15, tol_rel=1e-15,
+                                                                max_iter=2_000_000))
+    assert st == 0
+    np.testing.assert_allclose(imp[order, 0], best, atol=1e-8 * max(1, np.abs(best).max()))
+    # rho on a coupled frictional problem with W of full column rank
+    prob = scenes.random_problem(nb=2, nc=6, seed=200 + seed, p_static=0.3, p_frictionless=0.0, p_point=0.0,
+                                 warm=False)
+    W, R = _block_R(prob)
+    G = np.eye(prob.nv) - np.diag(prob.minv) @ W.T @ R @ W
+    v0 = np.ones(prob.nv) / np.sqrt(prob.nv)
+    g7 = np.linalg.matrix_power(G, 7) @ v0
+    lam = np.linalg.norm(G @ g7) / np.linalg.norm(g7)
+    st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, chebyshev=1, max_iter=2, tol_abs=0,
+                                                                tol_rel=0))
+    assert res.rho == pytest.approx(min(lam, 0.995), rel=1e-12)
