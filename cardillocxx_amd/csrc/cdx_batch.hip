@@ -15,12 +15,40 @@
 #include <cmath>
 #include <numeric>
 
+#include "cdx_joints_device.hpp"
 #include "cdx_pj_device.hpp"
 
 namespace cdx {
 
 constexpr int kMaxBatchBlock = 512;
 
+// Joint rows of the batch (null sc_off: none).  The chains of scene s are sc_list[sc_off[s] ..
+// sc_off[s+1]); each is factorised and solved by one lane of the scene's group, as the
+// grid-wide kernels do with one thread per chain.
+struct BatchJoints {
+    const int* sc_off;
+    const int* sc_list;
+    const int* jc_off;
+    const int* jcb_off;
+    const int* jb_off;
+    const int* jb_body;
+    const int* jr_kind;
+    const double* jr;
+    double* rlam;
+    double* Lm;
+    double* Dinv;
+    const int* jbody;
+    const int* jbody_blk;
+    const int* jslot;
+    double* tr;
+    double* z;
+    double* lam;
+    double theta;
+    double beta;
+};
+
+// Joints = false compiles none of the chain code (no scratch for its per-lane block matrices).
+template <bool Joints>
 __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     int ns, int groups, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
     const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
@@ -30,9 +58,10 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
     const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
     int use_bias, int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
-    double* __restrict__ res_out) {
+    double* __restrict__ res_out, BatchJoints ja) {
     __shared__ double wsum[kMaxBatchBlock / 64];
     __shared__ int gdone[kMaxBatchBlock / 64];
+    __shared__ int gfail[kMaxBatchBlock / 64];
     __shared__ int all_done;
     const int gsize = blockDim.x / groups;
     const int g = threadIdx.x / gsize, lt = threadIdx.x - g * gsize;
@@ -47,9 +76,26 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
         for (int c = c0 + lt; c < c1; c += gsize)
             pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
     __syncthreads();
-    for (int b = b0 + lt; b < b1; b += gsize) pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
-    if (lt == 0) gdone[g] = rows ? 0 : 1;
-    bool done = !rows;
+    for (int b = b0 + lt; b < b1; b += gsize)
+        pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x, ja.jslot, ja.tr);
+    if (lt == 0) gfail[g] = 0;
+    if constexpr (Joints) {  // factor the scene's chains, then x0 and Lambda_0 of its jointed bodies
+        __syncthreads();
+        if (s >= 0)
+            for (int q = ja.sc_off[s] + lt; q < ja.sc_off[s + 1]; q += gsize) {
+                const int ch = ja.sc_list[q];
+                if (joint_factor_chain(ch, ja.jc_off, ja.jb_off, ja.jb_body, ja.jr_kind, ja.jr, minv, v, dt, ja.theta,
+                                       ja.beta, ja.rlam, ja.Lm, ja.Dinv))
+                    gfail[g] = 1;
+                else
+                    joint_solve_chain(ch, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr, ja.rlam, ja.Lm,
+                                      ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.lam, x, tol_rel, tol_abs);
+            }
+    }
+    __syncthreads();
+    const bool failed = gfail[g] != 0;
+    if (lt == 0) gdone[g] = (rows && !failed) ? 0 : 1;
+    bool done = !rows || failed;
     int iters = 0;
     double res = rows ? NAN : NAN;
     __syncthreads();
@@ -63,7 +109,15 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
         double e2 = 0;
         if (!done)
             for (int b = b0 + lt; b < b1; b += gsize)
-                e2 += pj_body_sweep(b, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs);
+                e2 += pj_body_sweep(b, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs, ja.jslot, ja.tr);
+        if constexpr (Joints) {  // the chains apply the jointed bodies' W^T dp through S^-1
+            __syncthreads();
+            if (!done)
+                for (int q = ja.sc_off[s] + lt; q < ja.sc_off[s + 1]; q += gsize)
+                    e2 += joint_solve_chain(ja.sc_list[q], 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr,
+                                            ja.rlam, ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.lam, x,
+                                            tol_rel, tol_abs);
+        }
         if (check) {
             for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
             if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
@@ -91,7 +145,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
         if (all_done) break;
     }
     if (s >= 0 && lt == 0) {
-        iters_out[s] = rows ? iters : 0;
+        iters_out[s] = failed ? -1 : (rows ? iters : 0);  // -1: a singular joint block
         res_out[s] = res;
     }
 }
@@ -111,15 +165,58 @@ int upload_scenes(cdx_ctx* ctx) {
     return CDX_OK;
 }
 
-static int launch_batch(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, int block, int groups) {
+// Chains of each scene for the batched kernel: a chain belongs to the scene of its bodies (all of
+// its bodies must lie in one scene).
+static int upload_scene_chains(cdx_ctx* ctx) {
+    const int ns = ctx->nscenes, nch = ctx->nchains;
+    std::vector<int> scene_of(nch, -1), sc_off(ns + 1, 0), sc_list(nch);
+    for (int c = 0; c < nch; ++c) {
+        for (int j = ctx->h_jc_off[c]; j < ctx->h_jc_off[c + 1]; ++j)
+            for (int side = 0; side < 2; ++side) {
+                const int b = ctx->h_jb_body[2 * j + side];
+                if (b < 0) continue;
+                const int sc = int(std::upper_bound(ctx->scene_boff.begin(), ctx->scene_boff.end(), b) - ctx->scene_boff.begin()) - 1;
+                if (scene_of[c] >= 0 && scene_of[c] != sc)
+                    return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: a joint chain spans two scenes");
+                scene_of[c] = sc;
+            }
+        ++sc_off[scene_of[c] + 1];
+    }
+    for (int s = 0; s < ns; ++s) sc_off[s + 1] += sc_off[s];
+    std::vector<int> fill(sc_off.begin(), sc_off.end() - 1);
+    for (int c = 0; c < nch; ++c) sc_list[fill[scene_of[c]]++] = c;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_sc_off, (size_t)ns + 1, &ctx->sc_off_cap, "scene chains"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_sc_list, (size_t)std::max(nch, 1), &ctx->sc_list_cap, "scene chains"));
+    CDX_HIP(ctx, hipMemcpy(ctx->d_sc_off, sc_off.data(), sizeof(int) * sc_off.size(), hipMemcpyHostToDevice));
+    CDX_HIP(ctx, hipMemcpy(ctx->d_sc_list, sc_list.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+    return CDX_OK;
+}
+
+static BatchJoints batch_joints(cdx_ctx* ctx, double theta) {
+    BatchJoints ja{};
+    if (ctx->njrows == 0) return ja;
+    ja = BatchJoints{ctx->d_sc_off,  ctx->d_sc_list, ctx->d_jc_off, ctx->d_jcb_off,    ctx->d_jb_off, ctx->d_jb_body,
+                     ctx->d_jr_kind, ctx->d_jr,      ctx->d_jrlam,  ctx->d_jL,         ctx->d_jDinv,  ctx->d_jbody,
+                     ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jtr,    ctx->d_jz,         ctx->d_jlam,   theta,
+                     ctx->joint_bias};
+    return ja;
+}
+
+static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int block, int groups) {
     const int ns = ctx->nscenes;
     const int* d = ctx->d_scene_i;
     const int grid = (ns + groups - 1) / groups;
-    CDX_LAUNCH(ctx, "batch_pj", k_batch_pj, grid, block, ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2,
-               d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R,
-               ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,
-               ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every,
-               prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+#define CDX_BATCH_ARGS                                                                                               \
+    ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,     \
+        ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv,         \
+        ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha,                     \
+        ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs,              \
+        ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta)
+    if (ctx->njrows > 0)
+        CDX_LAUNCH(ctx, "batch_pj", k_batch_pj<true>, grid, block, CDX_BATCH_ARGS);
+    else
+        CDX_LAUNCH(ctx, "batch_pj", k_batch_pj<false>, grid, block, CDX_BATCH_ARGS);
+#undef CDX_BATCH_ARGS
     return hip_check(ctx, hipGetLastError(), "batch launch");
 }
 
@@ -172,11 +269,11 @@ int cdx_batch_set_launch(cdx_ctx* ctx, int block, int groups) {
 }
 
 int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* iters, double* residuals) {
-    (void)theta;
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
-    if (ctx->njrows > 0) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: joint rows are solved by cdx_pj_solve only");
+    const bool joints = ctx->njrows > 0;
+    if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: theta must be positive with joint rows");
     if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: only the standard loop is batched");
     if (prm->block_precond)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve uses the row scaling only");
@@ -184,15 +281,22 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
     ctx->solved = false;
     ctx->pj_last = false;
+    ctx->joints_solved = false;
     ctx->record_valid = false;
     CDX_TRY(build_incidence(ctx));
-    CDX_TRY(launch_batch(ctx, dt, prm, ctx->batch_block, ctx->batch_groups));
+    if (joints) CDX_TRY(upload_scene_chains(ctx));
+    CDX_TRY(launch_batch(ctx, dt, theta, prm, ctx->batch_block, ctx->batch_groups));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int ns = ctx->nscenes;
-    if (iters) CDX_HIP(ctx, hipMemcpy(iters, ctx->d_scene_i + 5 * ns + 2, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    std::vector<int> it(ns);
+    CDX_HIP(ctx, hipMemcpy(it.data(), ctx->d_scene_i + 5 * ns + 2, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    if (iters) std::copy(it.begin(), it.end(), iters);
     if (residuals) CDX_HIP(ctx, hipMemcpy(residuals, ctx->d_scene_res, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    for (int s = 0; s < ns; ++s)
+        if (it[s] < 0) return set_err(ctx, CDX_NOT_FACTORIZED, "cdx_batch_pj_solve: S is not factorized (singular joint block)");
     ctx->solved = true;
-    ctx->pj_last = true;  // same converged state as cdx_pj_solve: the implicit adjoint applies
+    ctx->joints_solved = joints;
+    ctx->pj_last = !joints;  // same converged state as cdx_pj_solve: the implicit adjoint applies (no joint rows)
     ctx->record_valid = false;
     ctx->recorded = 0;
     return CDX_OK;
@@ -202,13 +306,14 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
 // impulses are restored before each trial so every candidate solves the same problem.
 int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* block_out, int* groups_out,
                        double* best_ms) {
-    (void)theta;
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
     if (prm->block_precond) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve uses the row scaling only");
     if (ctx->nscenes < 1) return set_err(ctx, CDX_NO_STATE, "cdx_batch_autotune: scenes not set");
+    if (ctx->njrows > 0 && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: theta must be positive with joint rows");
     CDX_TRY(build_incidence(ctx));
+    if (ctx->njrows > 0) CDX_TRY(upload_scene_chains(ctx));
     const size_t np = 3 * (size_t)ctx->nc;
     double* saved = nullptr;
     CDX_HIP(ctx, hipMalloc((void**)&saved, sizeof(double) * (np ? np : 1)));
@@ -225,7 +330,7 @@ int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         for (int rep = 0; rep < 2 && st == CDX_OK; ++rep) {  // first run warms caches and code
             if (np) hipMemcpyAsync(ctx->d_p, saved, sizeof(double) * np, hipMemcpyDeviceToDevice, ctx->stream);
             hipEventRecord(e0, ctx->stream);
-            st = launch_batch(ctx, dt, prm, cd[0], cd[1]);
+            st = launch_batch(ctx, dt, theta, prm, cd[0], cd[1]);
             hipEventRecord(e1, ctx->stream);
             hipEventSynchronize(e1);
             hipEventElapsedTime(&ms, e0, e1);

@@ -72,6 +72,9 @@ struct cdx_ctx {
     double joint_bias = 0.0;
     bool joints_solved = false;
     std::vector<int> h_jorig;  // block-ordered row -> caller's row index
+    std::vector<int> h_jc_off, h_jb_body;  // host copies of the chain and block tables
+    int *d_sc_off = nullptr, *d_sc_list = nullptr;  // batched solve: chains of each scene
+    size_t sc_off_cap = 0, sc_list_cap = 0;
     int *d_jc_off = nullptr, *d_jcb_off = nullptr, *d_jb_off = nullptr, *d_jb_body = nullptr, *d_jr_kind = nullptr;
     int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;

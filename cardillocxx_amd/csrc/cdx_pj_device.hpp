@@ -211,15 +211,21 @@ __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ i
 }
 
 // One body of a PJ sweep: x += (W^T dp) / M over its incidences in a fixed order; returns the
-// body's part of the residual sum (projected_jacobi.cpp:140-144) when `check`.
+// body's part of the residual sum (projected_jacobi.cpp:140-144) when `check`.  A jointed body
+// (jslot >= 0) instead hands W^T dp to the chain solve through tr, which updates its x.
 __device__ inline double pj_body_sweep(int bi, const int* __restrict__ dof, const double* __restrict__ minv,
                                        const int* __restrict__ incptr, const int* __restrict__ incidx,
                                        const double* __restrict__ imp, double* __restrict__ x, int check,
-                                       double tol_rel, double tol_abs) {
+                                       double tol_rel, double tol_abs, const int* __restrict__ jslot = nullptr,
+                                       double* __restrict__ tr = nullptr) {
     double e2 = 0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const int i0 = incptr[bi], i1 = incptr[bi + 1];
     for (int i = i0; i < i1; ++i) gather_side9(imp, incidx[i], acc);
+    if (jslot && jslot[bi] >= 0) {
+        store6(tr + (size_t)bi * 6, acc);
+        return 0.0;
+    }
     double u[6];
     load_body(x, bi, u);
     const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);

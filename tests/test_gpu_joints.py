@@ -152,3 +152,98 @@ def test_joint_errors(ctx):
     ctx.set_contacts(*prob.contact_arrays())
     info = ctx.pj_solve(1e-3, 1.0, params())
     assert info.iters == 0 and np.allclose(ctx.velocities(), prob.v + 1e-3 * prob.minv * (prob.fext + prob.fgyro))
+
+
+def chains_batch(nchains=6, links=6, seed=3, free=(), damping=0.05):
+    """Capsule chains on a shared terrain, one chain per scene; the chains listed in `free` keep
+    no joint rows (plain contact scenes inside the same batch)."""
+    t = scenes.fractal_heightfield(n=33, extent=1.6, amp=0.03, seed=seed)
+    batch, jt = scenes.capsule_chains(nchains=nchains, links=links, terrain=t, seed=seed)
+    q = batch.q.copy()
+    rng = np.random.default_rng(seed + 11)
+    qq = q[:, 3:] + 0.15 * rng.normal(size=(len(q), 4))
+    q[:, 3:] = qq / np.linalg.norm(qq, axis=1, keepdims=True)
+    q[:, 2] = scenes.terrain_height(t, q[:, 0], q[:, 1]) + 0.02 - 2e-4
+    v = 0.3 * rng.normal(size=(len(q), 6))
+    bd = scenes.shape_bodies(batch.shape, batch.dims, batch.mass, q, v.reshape(-1))
+    con = batch.contacts(q=q)
+    prob = scenes.make_problem(bd, con)
+    keep = ~np.isin(jt["body_a"] // links, list(free))
+    jt = {k: np.asarray(x)[keep] for k, x in jt.items()}
+    rows = joints.spherical_rows(q, jt, joints.joint_frames(batch.q, jt), damping)
+    return prob, rows, np.asarray(batch.body_off), np.asarray(con["contact_off"])
+
+
+def scene_slice(prob, rows, body_off, contact_off, s, lam_prev=None):
+    """Scene s of a batch of 6-dof bodies as a standalone problem with its own joint rows."""
+    b0, b1, c0, c1 = int(body_off[s]), int(body_off[s + 1]), int(contact_off[s]), int(contact_off[s + 1])
+
+    def rebase(ids):
+        ids = np.asarray(ids[c0:c1]).copy()
+        ids[ids >= 0] -= b0
+        return ids.astype(np.int32)
+
+    sub = scenes.ContactProblem(dof=prob.dof[b0:b1], minv=prob.minv[6 * b0:6 * b1], v=prob.v[6 * b0:6 * b1],
+                                fext=prob.fext[6 * b0:6 * b1], fgyro=prob.fgyro[6 * b0:6 * b1], q=prob.q[7 * b0:7 * b1],
+                                body_a=rebase(prob.body_a), body_b=rebase(prob.body_b), mu=prob.mu[c0:c1],
+                                frame=prob.frame[c0:c1], angA=prob.angA[c0:c1], angB=prob.angB[c0:c1],
+                                rest=prob.rest[c0:c1], vbias=prob.vbias[c0:c1], warm=prob.warm[c0:c1])
+    inside = lambda ids: (ids >= b0) & (ids < b1)
+    sel = inside(rows["body_a"]) | inside(rows["body_b"])
+    n = rows["ns"] + rows["nd"]
+    spring = np.arange(n) < rows["ns"]
+    sub_rows = {k: np.asarray(rows[k])[sel] for k in ("body_a", "body_b", "wa", "wb", "comp", "vsrc", "gerr", "joint")}
+    for k in ("body_a", "body_b"):
+        sub_rows[k] = np.where(sub_rows[k] >= 0, sub_rows[k] - b0, -1).astype(np.int32)
+    sub_rows["ns"], sub_rows["nd"] = int((sel & spring).sum()), int((sel & ~spring).sum())
+    return sub, sub_rows, (None if lam_prev is None else np.asarray(lam_prev)[sel]), sel
+
+
+@pytest.mark.parametrize("launch", [(256, 1), (256, 4), (64, 1)])
+@pytest.mark.parametrize("kw", [dict(max_iter=23, tol_abs=0.0, tol_rel=0.0), dict(tol_abs=1e-9, tol_rel=1e-9)],
+                         ids=["fixed", "tol"])
+def test_batched_chains_match_oracle(ctx, launch, kw):
+    """Joint rows in the batched kernel: every scene (a chain on the terrain, or a chain without
+    joints) equals the oracle's solve of that scene alone, with its own iteration count."""
+    prob, rows, boff, coff = chains_batch(free=(2,))
+    lam_prev = np.random.default_rng(5).normal(size=rows["ns"] + rows["nd"]) * 1e-3
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.set_joints(rows, lam_prev=lam_prev)
+    ctx.set_scenes(boff, coff)
+    ctx.batch_set_launch(*launch)
+    it, res = ctx.batch_pj_solve(1e-3, 1.0, params(**kw))
+    v, imp, lam = ctx.velocities(), ctx.impulses(), ctx.joint_impulses()
+    for s in range(len(boff) - 1):
+        sub, sub_rows, lp, sel = scene_slice(prob, rows, boff, coff, s, lam_prev)
+        b0, b1, c0, c1 = boff[s], boff[s + 1], coff[s], coff[s + 1]
+        if sub_rows["ns"] + sub_rows["nd"]:
+            st, v_ref, imp_ref, lam_ref, r = oracle.pj_solve_joints(sub, sub_rows, 1e-3, oracle.params(**kw), lam_prev=lp)
+            assert rel(lam[sel], lam_ref) <= TOL
+        else:
+            st, v_ref, imp_ref, r = oracle.pj_solve(sub, 1e-3, oracle.params(**kw))
+        assert st == 0 and it[s] == r.iters
+        assert rel(v[6 * b0:6 * b1], v_ref) <= TOL and rel(imp[c0:c1], imp_ref) <= TOL
+
+
+def test_batched_chains_free_and_errors(ctx):
+    """Free chains (no contact rows): each scene is its free solution S^-1 rhs, as the oracle's;
+    a chain split between two scenes and theta = 0 are refused."""
+    prob, rows = chain_case(nchains=2, links=4, lower=False)
+    assert prob.nc == 0
+    lam_prev = np.random.default_rng(3).normal(size=rows["ns"] + rows["nd"])
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.set_joints(rows, lam_prev=lam_prev)
+    ctx.set_scenes([0, 2, 8], [0, 0, 0])  # bodies 0-3 form the first chain
+    with pytest.raises(native.CdxError) as e:
+        ctx.batch_pj_solve(1e-3, 1.0, params())
+    assert e.value.status == native.CDX_BAD_ARG
+    ctx.set_scenes([0, 4, 8], [0, 0, 0])
+    with pytest.raises(native.CdxError) as e:
+        ctx.batch_pj_solve(1e-3, 0.0, params())
+    assert e.value.status == native.CDX_BAD_ARG
+    it, res = ctx.batch_pj_solve(1e-3, 0.7, params())
+    assert (it == 0).all()
+    st, v_ref, _, lam_ref, r = oracle.pj_solve_joints(prob, rows, 1e-3, oracle.params(), theta=0.7, lam_prev=lam_prev)
+    assert st == 0 and rel(ctx.velocities(), v_ref) <= TOL and rel(ctx.joint_impulses(), lam_ref) <= TOL
