@@ -466,6 +466,139 @@ def batched_leg(args, native, scenes):
 
 
 # --------------------------------------------------------------------------------------------
+# jointed multibody: capsule chains with spherical joints on a heightfield, batched
+# --------------------------------------------------------------------------------------------
+
+def _scene_rows(rows, lam, b0, b1):
+    """Joint rows of the bodies [b0, b1) (a whole chain), rebased, with their Lambda_prev."""
+    n = rows["ns"] + rows["nd"]
+    sel = (rows["body_a"] >= b0) & (rows["body_a"] < b1)
+    spring = np.arange(n) < rows["ns"]
+    sub = {k: np.asarray(rows[k])[sel] for k in ("body_a", "body_b", "wa", "wb", "comp", "vsrc", "gerr")}
+    for k in ("body_a", "body_b"):
+        sub[k] = np.where(sub[k] >= 0, sub[k] - b0, -1).astype(np.int32)
+    sub["ns"], sub["nd"] = int((sel & spring).sum()), int((sel & ~spring).sum())
+    return sub, (None if lam is None else lam[sel])
+
+
+def chains_leg(args, native, scenes):
+    from cardillocxx_amd import joints
+
+    t0 = time.time()
+    terrain = scenes.fractal_heightfield(n=256, amp=0.1, seed=0)
+    b, jt = scenes.capsule_chains(nchains=args.chain_scenes, links=32, terrain=terrain, seed=0, drop=args.chain_drop,
+                                  local=True)
+    frames = joints.joint_frames(b.q, jt)
+    bd = scenes.shape_bodies(b.shape, b.dims, b.mass, b.q, b.v.reshape(-1))
+    ns, nb = b.nscenes, len(b.mass)
+    log(f"chains: {ns} chains x 32 capsules, {len(jt['body_a'])} joints, generated in {time.time() - t0:.1f}s")
+    dt, damping = 1e-3, args.chain_damping
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = args.batch_tol
+    p.max_iter = args.batch_max_iter
+    p.warmstart = 1
+    ctx = native.Context(0)
+    ctx.set_bodies(bd["dof"], bd["minv"], bd["v"], bd["fext"], None, bd["q"])
+    one = native.Context(0)
+    rng = np.random.default_rng(2)
+    steps = args.chain_steps
+    sample_steps = set(int(s) for s in np.linspace(steps // 2, steps - 1, args.obo_steps).round()) if args.obo_steps else set()
+    q, v = bd["q"].copy(), bd["v"].copy()
+    prev, lam = None, None
+    t_cpu = t_dev = t_solve = 0.0
+    cu = contacts_total = capped = 0
+    obo_time = batch_time_sampled = 0.0
+    obo_solves, worst_dv, tune = 0, 0.0, None
+    for s in range(steps):
+        ta = time.perf_counter()
+        con = b.contacts(q=q.reshape(-1, 7))
+        nc = len(con["body_a"])
+        warm = scenes.match_warmstart(prev, con) if prev is not None and nc else np.zeros((nc, 3))
+        rows = joints.spherical_rows(q, jt, frames, damping)
+        tb = time.perf_counter()
+        ctx.set_contacts(con["body_a"], con["body_b"], con["mu"], con["frame"], con["angA"], con["angB"], con["rest"],
+                         con["vbias"], warm)
+        ctx.set_joints(rows, lam_prev=lam)
+        ctx.set_scenes(b.body_off, con["contact_off"])
+        if s == min(args.tune_step, steps - 1):
+            tune = ctx.batch_autotune(dt, 1.0, p)
+            log(f"chains: autotuned (block, groups) = ({tune[0]}, {tune[1]}), {tune[2]:.3f} ms at step {s}")
+        ctx.synchronize()
+        ts = time.perf_counter()
+        iters, res = ctx.batch_pj_solve(dt, 1.0, p)
+        t_step = time.perf_counter() - ts
+        t_solve += t_step
+        coff = con["contact_off"]
+        per = np.diff(coff)
+        cu += int((per.astype(np.int64) * iters).sum())
+        contacts_total += nc
+        capped += int((iters >= p.max_iter).sum())
+        lam_new = ctx.joint_impulses()
+        if s in sample_steps:  # the same chains one by one through cdx_pj_solve
+            vb = ctx.velocities()
+            pick = rng.choice(ns, size=min(args.obo_scenes, ns), replace=False)
+            t_one = 0.0
+            for k in pick:
+                b0, b1 = int(b.body_off[k]), int(b.body_off[k + 1])
+                c0, c1 = int(coff[k]), int(coff[k + 1])
+                bodies, contacts = _scene_problem(bd, con, warm, v, q, b0, b1, c0, c1)
+                sub, sl = _scene_rows(rows, lam, b0, b1)
+                one.set_bodies(*bodies)
+                one.set_contacts(*contacts)
+                one.set_joints(sub, lam_prev=sl)
+                one.synchronize()
+                t1 = time.perf_counter()
+                one.pj_solve(dt, 1.0, p)
+                t_one += time.perf_counter() - t1
+                v1 = one.velocities()
+                scale = max(np.abs(vb[6 * b0:6 * b1]).max(), 1e-30)
+                worst_dv = max(worst_dv, float(np.abs(v1 - vb[6 * b0:6 * b1]).max() / scale))
+            obo_time += t_one * ns / len(pick)
+            batch_time_sampled += t_step
+            obo_solves += len(pick)
+        ctx.moreau_update(dt, 1.0)
+        q, v, imp = ctx.positions(), ctx.velocities(), ctx.impulses()
+        lam = lam_new
+        tc = time.perf_counter()
+        if not (np.isfinite(q).all() and np.isfinite(v).all()):
+            raise RuntimeError(f"chains: non-finite state at step {s}")
+        prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
+        t_cpu += tb - ta
+        t_dev += tc - tb
+        if s % 50 == 0 or s == steps - 1:
+            log(f"chains step {s}: {nc} contacts, max sweeps {int(iters.max())}, solve {t_step * 1e3:.2f} ms")
+    out = {
+        "metric": "batched jointed env-steps/s (chains x Moreau steps per second)",
+        "workload": f"{ns} chains of 32 capsules (r=0.02 m, length 0.1 m) joined by damped spherical joints "
+                    f"(D_rot={damping:g}), dropped {args.chain_drop:g} m onto a 256x256 fractal heightfield (amp 0.1 m), "
+                    f"h=1e-3 s, {steps} steps, joint rows and contacts in one warm-started PJ per chain to tol "
+                    f"{args.batch_tol:g}, one batched solve launch per step",
+        "steps": steps, "scenes": ns, "bodies": nb, "joints": int(len(jt["body_a"])),
+        "joint_rows": int(rows["ns"] + rows["nd"]),
+        "value": ns * steps / t_dev, "unit": "env-steps/s",
+        "definition": "device path per step: contact and joint-row upload, batched solve, position update, state download",
+        "solve_only_env_steps_per_s": ns * steps / t_solve,
+        "end_to_end_env_steps_per_s": ns * steps / (t_cpu + t_dev),
+        "cpu_contacts_and_joint_rows_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
+        "contact_updates_per_s": cu / t_solve, "mean_contacts_per_step": contacts_total / steps,
+        "scene_solves_at_iteration_cap": capped,
+        "launch": {"block": tune[0], "groups": tune[1]} if tune else None,
+        "one_by_one": None,
+    }
+    if obo_solves:
+        out["one_by_one"] = {
+            "sampled_steps": sorted(sample_steps), "sampled_scene_solves": obo_solves,
+            "estimated_solve_s_per_step": obo_time / len(sample_steps),
+            "batched_solve_s_per_step": batch_time_sampled / len(sample_steps),
+            "speedup": obo_time / max(batch_time_sampled, 1e-12),
+            "max_rel_velocity_difference": worst_dv,
+            "method": "at each sampled step a random sample of chains is solved alone with cdx_pj_solve and its joint "
+                      "rows (same inputs, same per-chain tolerance); time scaled to all chains, compared with the "
+                      "batched solve of that step"}
+    return out
+
+
+# --------------------------------------------------------------------------------------------
 # differentiable: forward + checkpointed backward of batched rollouts
 # --------------------------------------------------------------------------------------------
 
@@ -546,6 +679,11 @@ def main():
     ap.add_argument("--tune-step", type=int, default=100)
     ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
     ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")
+    ap.add_argument("--no-chains", action="store_true", help="skip the jointed capsule-chain leg")
+    ap.add_argument("--chain-scenes", type=int, default=2048)
+    ap.add_argument("--chain-steps", type=int, default=150)
+    ap.add_argument("--chain-drop", type=float, default=0.03, help="drop height above the terrain under each chain")
+    ap.add_argument("--chain-damping", type=float, default=0.05, help="rotational joint damping D_rot")
     ap.add_argument("--diff-scenes", type=int, default=256)
     ap.add_argument("--diff-steps", type=int, default=100)
     ap.add_argument("--diff-obo", type=int, default=4, help="scenes rolled out one by one")
@@ -553,8 +691,8 @@ def main():
     ap.add_argument("--adjoint-max-iter", type=int, default=2_000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); a single scene does not "
-              "shard across GPUs here -- see DESIGN.md for how it would.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); scenes and the pile are "
+              "not sharded across GPUs here.", file=sys.stderr)
         sys.exit(2)
 
     from cardillocxx_amd import metrics, native, scenes
@@ -572,6 +710,8 @@ def main():
     if not args.pile_only:
         out["batched"] = batched_leg(args, native, scenes)
         out["differentiable"] = diff_leg(args, native, scenes)
+        if not args.no_chains:
+            out["jointed_chains"] = chains_leg(args, native, scenes)
     print(json.dumps(out), flush=True)
 
 

@@ -569,12 +569,13 @@ def terrain_height(terrain, x, y):
 
 
 def capsule_chains(nchains=2048, links=32, radius=0.02, length=0.1, density=1000.0, terrain=None, drop=0.15, seed=0,
-                   mu=0.5):
+                   mu=0.5, local=False):
     """Chains of `links` capsules (total length `length` including the caps, radius `radius`),
     one chain per scene, laid out tip to tip along a random horizontal direction `drop` above the
-    highest terrain point, over a fractal heightfield shared by all scenes.  Returns the batch and
-    the spherical joints between consecutive links: (body_a, body_b, anchor_a, anchor_b) with
-    the anchors at the touching tips in body coordinates."""
+    highest terrain point (with `local`, above the highest point under that chain), over a
+    fractal heightfield shared by all scenes.  Returns the batch and the spherical joints between
+    consecutive links: (body_a, body_b, anchor_a, anchor_b) with the anchors at the touching tips
+    in body coordinates."""
     terrain = terrain if terrain is not None else fractal_heightfield(seed=seed)
     rng = np.random.default_rng(seed + 1)
     hl = 0.5 * length - radius
@@ -588,7 +589,12 @@ def capsule_chains(nchains=2048, links=32, radius=0.02, length=0.1, density=1000
         d = np.array([math.cos(ang), math.sin(ang), 0.0])
         c = rng.uniform(0.5 * span + 0.05, extent - 0.5 * span - 0.05, 2) if extent > span + 0.1 else \
             np.full(2, 0.5 * extent)
-        start = np.array([c[0], c[1], top]) - 0.5 * span * d
+        z = top
+        if local:
+            sx = c[0] + np.linspace(-0.5 * span, 0.5 * span, 4 * links + 1) * d[0]
+            sy = c[1] + np.linspace(-0.5 * span, 0.5 * span, 4 * links + 1) * d[1]
+            z = float(np.max(terrain_height(terrain, sx, sy))) + radius + drop
+        start = np.array([c[0], c[1], z]) - 0.5 * span * d
         # body z axis along d: rotate e_z by 90 degrees about e_z x d
         axis = np.cross([0.0, 0.0, 1.0], d)
         axis /= np.linalg.norm(axis)
