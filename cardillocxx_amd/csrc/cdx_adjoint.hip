@@ -96,13 +96,14 @@ __device__ inline double cbias(const ContactJ& j, int k, int nc, int c, const do
 }
 
 // Implicit setup: P, y, dmu at the fixed point, g_p, lambda_0 = g_p, xi_0, and the side impulses
-// of W^T (R xi_0).
+// of W^T (R xi_0).  g_p = W S^-1_vv L_x (+ L_p): S^-1_vv = M^-1 applied here, or (lx_applied, with
+// joint rows) already applied by the caller.
 __global__ void k_adj_setup(int nc, const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
                             const double* __restrict__ R, const double* __restrict__ bias, const double* __restrict__ mu,
                             const double* __restrict__ rest, const double* __restrict__ vb, const double* __restrict__ minv,
                             const double* __restrict__ v, const double* __restrict__ x, const double* __restrict__ p,
                             const double* __restrict__ lx, const double* __restrict__ lp, double* __restrict__ S,
-                            double* __restrict__ imp) {
+                            double* __restrict__ imp, int lx_applied) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int a = ida[c], b = idb[c];
@@ -123,10 +124,11 @@ __global__ void k_adj_setup(int nc, const int* __restrict__ ida, const int* __re
     load_body(lx, b, lb);
     minv_body(minv, a, ma);
     minv_body(minv, b, mb);
-    for (int k = 0; k < 6; ++k) {
-        la[k] *= ma[k];
-        lb[k] *= mb[k];
-    }
+    if (!lx_applied)  // lx holds L_x; with joint rows it already holds S^-1_vv L_x
+        for (int k = 0; k < 6; ++k) {
+            la[k] *= ma[k];
+            lb[k] *= mb[k];
+        }
     double z[3] = {0, 0, 0}, y[3] = {0, 0, 0}, gp[3] = {0, 0, 0};
     for (int k = 0; k < nr; ++k) {
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], xa, xb);
@@ -154,10 +156,12 @@ __global__ void k_adj_setup(int nc, const int* __restrict__ ida, const int* __re
     write_side_impulses(imp, c, a, b, nr, j, s);
 }
 
-// t = M^-1 (sum of side impulses)
+// t = M^-1 (sum of side impulses); a jointed body (jslot >= 0) gets t = 0 and hands the sum to
+// the chain solve through tr, which adds its share of S^-1.
 __global__ void k_adj_body_t(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
                              const int* __restrict__ incptr, const int* __restrict__ incidx, const double* __restrict__ imp,
-                             double* __restrict__ t, const int* __restrict__ ctrl) {
+                             double* __restrict__ t, const int* __restrict__ ctrl, const int* __restrict__ jslot,
+                             double* __restrict__ tr) {
     if (ctrl && ctrl[0]) return;
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= nb) return;
@@ -175,8 +179,32 @@ __global__ void k_adj_body_t(int nb, const int* __restrict__ dof, const double* 
     double m[6];
     minv_body(minv, bi, m);
     double o[6];
-    for (int k = 0; k < 6; ++k) o[k] = k < dof[bi] ? m[k] * acc[k] : 0.0;
+    if (jslot && jslot[bi] >= 0) {
+        store6(tr + (size_t)bi * 6, acc);
+        for (int k = 0; k < 6; ++k) o[k] = 0.0;
+    } else {
+        for (int k = 0; k < 6; ++k) o[k] = k < dof[bi] ? m[k] * acc[k] : 0.0;
+    }
     store6(t + (size_t)bi * kBodyStride, o);
+}
+
+// out = M^-1 in for the bodies outside joint chains; a jointed body gets out = 0 and tr = in
+// (the chain solve then adds S^-1 [tr; 0]).
+__global__ void k_adj_minv_apply(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                 const double* __restrict__ in, double* __restrict__ out, const int* __restrict__ jslot,
+                                 double* __restrict__ tr) {
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= nb) return;
+    double u[6], m[6], o[6];
+    load_body(in, bi, u);
+    minv_body(minv, bi, m);
+    if (jslot[bi] >= 0) {
+        store6(tr + (size_t)bi * 6, u);
+        for (int k = 0; k < 6; ++k) o[k] = 0.0;
+    } else {
+        for (int k = 0; k < 6; ++k) o[k] = k < dof[bi] ? m[k] * u[k] : 0.0;
+    }
+    store6(out + (size_t)bi * kBodyStride, o);
 }
 
 // lambda <- g_p + xi - W t; xi <- P^T lambda; side impulses of W^T (R xi); convergence partials.
@@ -399,7 +427,8 @@ __global__ void k_adj_final_bodies(int nb, const int* __restrict__ dof, const do
                                    const double* __restrict__ ib, const double* __restrict__ ic,
                                    const double* __restrict__ id, const double* __restrict__ base,
                                    const double* __restrict__ gacc, double h, double* __restrict__ gv,
-                                   double* __restrict__ gm, double* __restrict__ gf) {
+                                   double* __restrict__ gm, double* __restrict__ gf, const int* __restrict__ jslot,
+                                   double* __restrict__ tr) {
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= nb) return;
     double sa[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0}, sc[6] = {0, 0, 0, 0, 0, 0}, sd[6] = {0, 0, 0, 0, 0, 0};
@@ -421,6 +450,20 @@ __global__ void k_adj_final_bodies(int nb, const int* __restrict__ dof, const do
     load_body(fgyro, bi, fg);
     minv_body(minv, bi, m);
     double xb[6], gmo[6] = {0, 0, 0, 0, 0, 0}, gvo[6] = {0, 0, 0, 0, 0, 0}, gfo[6] = {0, 0, 0, 0, 0, 0};
+    if (jslot && jslot[bi] >= 0) {
+        // jointed body: r_bar = S^-1 [x_bar; 0] comes from the chain solve; here x_bar goes to tr,
+        // dL/dv_n starts from the restitution term, and dL/dM^-1 is not offered (NaN)
+        for (int k = 0; k < 6; ++k) {
+            xb[k] = g[k] - sa[k];
+            gvo[k] = sd[k];
+            gmo[k] = NAN;
+        }
+        store6(tr + (size_t)bi * 6, xb);
+        store6(gv + (size_t)bi * kBodyStride, gvo);
+        store6(gm + (size_t)bi * kBodyStride, gmo);
+        store6(gf + (size_t)bi * kBodyStride, gfo);
+        return;
+    }
     for (int k = 0; k < nd; ++k) {
         xb[k] = g[k] - sa[k];  // u_free_bar
         gmo[k] = acc[k] + xb[k] * sb[k] + sc[k] + h * xb[k] * (fe[k] + fg[k]);
@@ -450,6 +493,70 @@ __global__ void k_adj_final_bodies(int nb, const int* __restrict__ dof, const do
     store6(gf + (size_t)bi * kBodyStride, gfo);
 }
 
+// Jointed bodies after the chain solve r = S^-1 [x_bar; 0] (velocity part rv): dL/dv_n += M r_v
+// (r_v = M v + h f_ext + h f_gyro(v) + ...), the gyroscopic Jacobian, and the (1 - theta) G v term
+// of the joint rows' right-hand side; dL/df_ext = h r_v.
+__global__ void k_adj_final_jointed(int njb, const int* __restrict__ jbody, const int* __restrict__ jbody_blk,
+                                    const int* __restrict__ jb_off, const double* __restrict__ jr,
+                                    const double* __restrict__ rlb, const double* __restrict__ minv,
+                                    const double* __restrict__ v, const double* __restrict__ rv, double h, double theta,
+                                    double* __restrict__ gv, double* __restrict__ gf) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= njb) return;
+    const int bi = jbody[q];
+    double r[6], m[6], vv[6], gvo[6], gfo[6];
+    load_body(rv, bi, r);
+    minv_body(minv, bi, m);
+    load_body(v, bi, vv);
+    load_body(gv, bi, gvo);
+    for (int k = 0; k < 6; ++k) {
+        gvo[k] += (m[k] > 0 ? 1.0 / m[k] : 0.0) * r[k];
+        gfo[k] = h * r[k];
+    }
+    const double w[3] = {vv[3], vv[4], vv[5]};
+    double I[3], Iw[3], gp[3];
+    for (int k = 0; k < 3; ++k) {
+        I[k] = m[3 + k] > 0 ? 1.0 / m[3 + k] : 0.0;
+        Iw[k] = I[k] * w[k];
+        gp[k] = r[3 + k];
+    }
+    const double wxg[3] = {w[1] * gp[2] - w[2] * gp[1], w[2] * gp[0] - w[0] * gp[2], w[0] * gp[1] - w[1] * gp[0]};
+    const double Iwxg[3] = {Iw[1] * gp[2] - Iw[2] * gp[1], Iw[2] * gp[0] - Iw[0] * gp[2], Iw[0] * gp[1] - Iw[1] * gp[0]};
+    for (int k = 0; k < 3; ++k) gvo[3 + k] += -h * (-I[k] * wxg[k] + Iwxg[k]);
+    if (theta != 1.0) {
+        const double f = -(1.0 - theta) / theta;
+        for (int e = 0; e < 2; ++e) {
+            const int j = jbody_blk[4 * (size_t)q + 2 * e], side = jbody_blk[4 * (size_t)q + 2 * e + 1];
+            if (j < 0) continue;
+            for (int row = jb_off[j]; row < jb_off[j + 1]; ++row) {
+                const double* wr = jr + (size_t)row * 16 + 6 * side;
+                for (int k = 0; k < 6; ++k) gvo[k] += f * wr[k] * rlb[row];
+            }
+        }
+    }
+    store6(gv + (size_t)bi * kBodyStride, gvo);
+    store6(gf + (size_t)bi * kBodyStride, gfo);
+}
+
+// dL/dcomp per joint row (block order): E = C / (theta h^2) for a spring row, A / (theta h) for a
+// damper row enters S as -E, so dL/dE = r_bar_Lambda Lambda; a spring row's right-hand side
+// -C Lambda_prev / (theta h^2) - beta (-C Lambda_prev / h + g) / (h theta) adds r_bar dr/dC.
+__global__ void k_adj_comp(int n, const int* __restrict__ kind, const double* __restrict__ jr,
+                           const double* __restrict__ lam, const double* __restrict__ rlb, double dt, double theta,
+                           double beta, double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double lprev = jr[(size_t)r * 16 + 15];
+    double g;
+    if (kind[r] == 0) {
+        g = rlb[r] * lam[r] / (theta * dt * dt);
+        g += rlb[r] * (-lprev / (theta * dt * dt) + (beta > 0 ? lprev * beta / (dt * dt * theta) : 0.0));
+    } else {
+        g = rlb[r] * lam[r] / (theta * dt);
+    }
+    out[r] = g;
+}
+
 }  // namespace cdx
 
 using namespace cdx;
@@ -458,7 +565,7 @@ static int adj_alloc(cdx_ctx* ctx) {
     const size_t nc = (size_t)ctx->nc, nb = (size_t)ctx->nb;
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj, (size_t)A_COUNT * nc, &ctx->adj_cap, "adjoint contact scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_imp, 48 * nc, &ctx->adj_imp_cap, "adjoint side vectors"));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_body, 6 * kBodyStride * nb, &ctx->adj_body_cap, "adjoint body scratch"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_body, 8 * kBodyStride * nb, &ctx->adj_body_cap, "adjoint body scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_partial, 2 * (size_t)grid_for((int)nc) + 2, &ctx->adj_partial_cap, "adjoint partials"));
     return CDX_OK;
 }
@@ -501,15 +608,15 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
     if (rows && mode == 0) {
         CDX_TRY(reset_control(ctx));
         CDX_LAUNCH(ctx, "adj_setup", k_adj_setup, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias,
-                   ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, ctx->d_x, ctx->d_p, lx, dlp, S, I0);
+                   ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, ctx->d_x, ctx->d_p, lx, dlp, S, I0, 0);
         CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx, I0, t,
-                   nullptr);
+                   nullptr, nullptr, nullptr);
         auto iter = [&](int it) -> int {
             const int check = (it % 10) == 9;
             CDX_LAUNCH(ctx, "adj_iter", k_adj_iter, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, t, S,
                        I0, check, ctx->d_adj_partial, ctx->d_ctrl);
             CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                       I0, t, ctx->d_ctrl);
+                       I0, t, ctx->d_ctrl, nullptr, nullptr);
             if (check)
                 CDX_LAUNCH(ctx, "adj_residual", k_adj_finalize, 1, kBlock, gc, ctx->d_adj_partial, it, aprm->tol, ctx->d_ctrl,
                            ctx->d_res, ctx->h_flag_dev);
@@ -566,7 +673,7 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
     }
     if (nb)
         CDX_LAUNCH(ctx, "adj_final_bodies", k_adj_final_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext,
-                   ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, I0, I1, I2, I3, base, gacc, dt, gvd, gmd, gfd);
+                   ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, I0, I1, I2, I3, base, gacc, dt, gvd, gmd, gfd, nullptr, nullptr);
     CDX_HIP(ctx, hipGetLastError());
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<double> hv((size_t)kBodyStride * nb), hm((size_t)kBodyStride * nb), hf((size_t)kBodyStride * nb);
@@ -582,6 +689,109 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
             if (gminv_out) gminv_out[o] = hm[(size_t)b * kBodyStride + k];
             if (gfext_out) gfext_out[o] = hf[(size_t)b * kBodyStride + k];
         }
+    if (gmu_out && nc) CDX_HIP(ctx, hipMemcpy(gmu_out, dgmu, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost));
+    return CDX_OK;
+}
+
+// Implicit adjoint of a solve with joint rows (see cdx_pj_adjoint_joints): the iteration and the
+// final stage of adjoint_run with S^-1_vv in place of M^-1 (the chain solve applies the joint
+// rows' share), then r_bar = S^-1 [x_bar; 0] gives dL/dv_n, dL/df_ext and dL/dcomp.
+static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
+                              const double* g_v, const double* g_p, double* gv_out, double* gfext_out, double* gmu_out,
+                              double* gcomp_out, double** dlp_out, double** dgmu_out, int* iters_out, double* res_out) {
+    const int nb = ctx->nb, nc = ctx->nc, nj = ctx->njrows;
+    const double theta = ctx->last_theta;
+    const size_t n8 = (size_t)kBodyStride * nb;
+    double* B = ctx->d_adj_body;  // [lx | t | tlx | gacc | gv | gm | gf | rv]
+    double *lx = B, *t = B + n8, *tlx = B + 2 * n8, *gacc = B + 3 * n8, *gvd = B + 4 * n8, *gmd = B + 5 * n8,
+           *gfd = B + 6 * n8, *rv = B + 7 * n8;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_jl, 3 * (size_t)nj, &ctx->adj_jl_cap, "joint adjoint rows"));
+    double *jl = ctx->d_adj_jl, *rlb = jl + nj, *gcomp = jl + 2 * (size_t)nj;
+    {
+        std::vector<double> h(n8, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < ctx->dof[b]; ++k) h[(size_t)b * kBodyStride + k] = g_v[ctx->voff[b] + k];
+        CDX_HIP(ctx, hipMemcpyAsync(lx, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemsetAsync(gacc, 0, sizeof(double) * n8, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (g_p && nc) {
+        CDX_HIP(ctx, hipMalloc((void**)dlp_out, sizeof(double) * 3 * (size_t)nc));
+        CDX_HIP(ctx, hipMemcpy(*dlp_out, g_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyHostToDevice));
+    }
+    double* dlp = *dlp_out;
+    CDX_HIP(ctx, hipMalloc((void**)dgmu_out, sizeof(double) * (size_t)(nc ? nc : 1)));
+    double* dgmu = *dgmu_out;
+    double* S = ctx->d_adj;
+    double* I0 = ctx->d_adj_imp;
+    double* I1 = I0 + 12 * (size_t)nc;
+    double* I2 = I0 + 24 * (size_t)nc;
+    double* I3 = I0 + 36 * (size_t)nc;
+    const int gc = grid_for(nc), gb = grid_for(nb);
+    const bool rows = nc > 0 && ctx->nrows > 0;
+    *iters_out = 0;
+    *res_out = 0.0;
+    if (rows) {
+        // tlx = S^-1_vv L_x
+        CDX_LAUNCH(ctx, "adj_body", k_adj_minv_apply, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, lx, tlx, ctx->d_jslot, ctx->d_jtr);
+        CDX_HIP(ctx, hipMemsetAsync(jl, 0, sizeof(double) * nj, ctx->stream));
+        CDX_TRY(joints_apply(ctx, tlx, jl, nullptr));
+        CDX_TRY(reset_control(ctx));
+        CDX_LAUNCH(ctx, "adj_setup", k_adj_setup, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias,
+                   ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, ctx->d_x, ctx->d_p, tlx, dlp, S, I0, 1);
+        CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx, I0, t,
+                   nullptr, ctx->d_jslot, ctx->d_jtr);
+        CDX_TRY(joints_apply(ctx, t, jl, nullptr));
+        auto iter = [&](int it) -> int {
+            const int check = (it % 10) == 9;
+            CDX_LAUNCH(ctx, "adj_iter", k_adj_iter, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, t, S,
+                       I0, check, ctx->d_adj_partial, ctx->d_ctrl);
+            CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                       I0, t, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
+            CDX_TRY(joints_apply(ctx, t, jl, ctx->d_ctrl));
+            if (check)
+                CDX_LAUNCH(ctx, "adj_residual", k_adj_finalize, 1, kBlock, gc, ctx->d_adj_partial, it, aprm->tol, ctx->d_ctrl,
+                           ctx->d_res, ctx->h_flag_dev);
+            return hip_check(ctx, hipGetLastError(), "adjoint iteration launch");
+        };
+        int it_done = 0;
+        CDX_TRY(run_sweeps(ctx, aprm->max_iter, 10, 4, iter, &it_done));
+        int ctrl[3];
+        CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+        CDX_TRY(read_residual(ctx, res_out));
+        *iters_out = it_done;
+        if (!ctrl[0]) return set_err(ctx, CDX_DIVERGED, "cdx_pj_adjoint_joints: implicit adjoint iteration did not converge");
+        CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, 1, prm->alpha, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                   ctx->d_R, ctx->d_mu, ctx->d_rest, ctx->d_p, S, I0, I1, I2, I3, dgmu);
+    } else {
+        CDX_HIP(ctx, hipMemsetAsync(I0, 0, sizeof(double) * 48 * (size_t)(nc ? nc : 1), ctx->stream));
+        CDX_HIP(ctx, hipMemsetAsync(dgmu, 0, sizeof(double) * (size_t)(nc ? nc : 1), ctx->stream));
+    }
+    // x_bar = L_x - W^T s; non-jointed bodies finish here, jointed ones hand x_bar to the chains
+    CDX_LAUNCH(ctx, "adj_final_bodies", k_adj_final_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext,
+               ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, I0, I1, I2, I3, lx, gacc, dt, gvd, gmd, gfd, ctx->d_jslot,
+               ctx->d_jtr);
+    CDX_HIP(ctx, hipMemsetAsync(rv, 0, sizeof(double) * n8, ctx->stream));
+    CDX_HIP(ctx, hipMemsetAsync(rlb, 0, sizeof(double) * nj, ctx->stream));
+    CDX_TRY(joints_apply(ctx, rv, rlb, nullptr));
+    CDX_LAUNCH(ctx, "adj_final_bodies", k_adj_final_jointed, grid_for(ctx->njbodies), kBlock, ctx->njbodies, ctx->d_jbody,
+               ctx->d_jbody_blk, ctx->d_jb_off, ctx->d_jr, rlb, ctx->d_minv, ctx->d_v, rv, dt, theta, gvd, gfd);
+    CDX_LAUNCH(ctx, "adj_final_bodies", k_adj_comp, grid_for(nj), kBlock, nj, ctx->d_jr_kind, ctx->d_jr, ctx->d_jlam, rlb,
+               dt, theta, ctx->joint_bias, gcomp);
+    CDX_HIP(ctx, hipGetLastError());
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> hv(n8), hf(n8), hc(nj);
+    CDX_HIP(ctx, hipMemcpy(hv.data(), gvd, n8 * sizeof(double), hipMemcpyDeviceToHost));
+    CDX_HIP(ctx, hipMemcpy(hf.data(), gfd, n8 * sizeof(double), hipMemcpyDeviceToHost));
+    CDX_HIP(ctx, hipMemcpy(hc.data(), gcomp, (size_t)nj * sizeof(double), hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < ctx->dof[b]; ++k) {
+            const int o = ctx->voff[b] + k;
+            if (gv_out) gv_out[o] = hv[(size_t)b * kBodyStride + k];
+            if (gfext_out) gfext_out[o] = hf[(size_t)b * kBodyStride + k];
+        }
+    if (gcomp_out)
+        for (int i = 0; i < nj; ++i) gcomp_out[ctx->h_jorig[i]] = hc[i];
     if (gmu_out && nc) CDX_HIP(ctx, hipMemcpy(gmu_out, dgmu, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost));
     return CDX_OK;
 }
@@ -623,6 +833,36 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
         info->iters = iters;
         info->residual = res;
         info->mode_used = mode == 0 ? CDX_ADJ_IMPLICIT : CDX_ADJ_UNROLLED;
+    }
+    return st;
+}
+
+int cdx_pj_adjoint_joints(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
+                          const double* g_v, const double* g_p, double* gv_out, double* gfext_out, double* gmu_out,
+                          double* gcomp_out, cdx_adjoint_info* info) {
+    if (!ctx || !prm || !aprm || !g_v) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    if (!ctx->solved || !ctx->pjj_last || ctx->njrows == 0)
+        return set_err(ctx, CDX_NO_STATE, "cdx_pj_adjoint_joints: needs a preceding joint solve (cdx_pj_solve or cdx_batch_pj_solve)");
+    if (aprm->mode == CDX_ADJ_UNROLLED || aprm->mode < 0 || aprm->mode > 2 || aprm->max_iter < 1)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_adjoint_joints: implicit mode only");
+    if (info) {
+        info->iters = 0;
+        info->residual = std::nan("");
+        info->mode_used = CDX_ADJ_IMPLICIT;
+    }
+    CDX_TRY(adj_alloc(ctx));
+    double* dlp = nullptr;
+    double* dgmu = nullptr;
+    int iters = 0;
+    double res = std::nan("");
+    const int st = adjoint_joints_run(ctx, dt, prm, aprm, g_v, g_p, gv_out, gfext_out, gmu_out, gcomp_out, &dlp, &dgmu,
+                                      &iters, &res);
+    if (dlp) hipFree(dlp);
+    if (dgmu) hipFree(dgmu);
+    if (info) {
+        info->iters = iters;
+        info->residual = res;
     }
     return st;
 }

@@ -281,6 +281,7 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
     ctx->solved = false;
     ctx->pj_last = false;
+    ctx->pjj_last = false;
     ctx->joints_solved = false;
     ctx->record_valid = false;
     CDX_TRY(build_incidence(ctx));
@@ -296,7 +297,9 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         if (it[s] < 0) return set_err(ctx, CDX_NOT_FACTORIZED, "cdx_batch_pj_solve: S is not factorized (singular joint block)");
     ctx->solved = true;
     ctx->joints_solved = joints;
-    ctx->pj_last = !joints;  // same converged state as cdx_pj_solve: the implicit adjoint applies (no joint rows)
+    ctx->pj_last = !joints;  // same converged state as cdx_pj_solve: the implicit adjoint applies
+    ctx->pjj_last = joints;
+    ctx->last_theta = theta;
     ctx->record_valid = false;
     ctx->recorded = 0;
     return CDX_OK;

@@ -119,6 +119,8 @@ struct cdx_ctx {
 
     // ---- differentiation
     bool pj_last = false;  // the last solve was cdx_pj_solve (adjoint available)
+    bool pjj_last = false;  // the last solve was a standard-loop joint solve (cdx_pj_adjoint_joints available)
+    double last_theta = 1.0;
     bool record = false, record_valid = false;
     int record_max = 0, recorded = 0;
     double* d_rec_x = nullptr;  // [record_max][nb*8]
@@ -130,6 +132,8 @@ struct cdx_ctx {
     size_t adj_imp_cap = 0;
     double* d_adj_body = nullptr;
     size_t adj_body_cap = 0;
+    double* d_adj_jl = nullptr;  // joint adjoint: [lambda scratch | r_bar_Lambda | dL/dcomp] per joint row
+    size_t adj_jl_cap = 0;
     double* d_adj_partial = nullptr;
     size_t adj_partial_cap = 0;
 
@@ -181,6 +185,7 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
 int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence = 1);
 int joints_factor(cdx_ctx* ctx, double dt, double theta);
 int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial);
+int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl);
 
 // Launch helper: brackets the launch with profiling events when enabled.
 // With profiling on, the start/stop events travel in the kernel's own dispatch

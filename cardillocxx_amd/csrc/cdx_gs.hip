@@ -358,6 +358,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     if (nb == 0) {
         ctx->solved = true;
     ctx->pj_last = false;
+    ctx->pjj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_incidence(ctx));
@@ -371,6 +372,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
         CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->solved = true;
     ctx->pj_last = false;
+    ctx->pjj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_coloring(ctx));
@@ -412,6 +414,7 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     }
     ctx->solved = true;
     ctx->pj_last = false;
+    ctx->pjj_last = false;
     if (st) return set_err(ctx, st, "Projected Gauss-Seidel diverged");
     return CDX_OK;
 }

@@ -78,6 +78,16 @@ int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_
     return hip_check(ctx, hipGetLastError(), "joint solve launch");
 }
 
+// out += S^-1 [tr; 0] for the jointed bodies (velocity part into x_out, joint part into lam_out;
+// both accumulate): the joint rows' share of S^-1 applied to a body vector, no right-hand side.
+int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) {
+    if (ctx->nchains == 0) return CDX_OK;
+    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, grid_for(ctx->nchains), kBlock, ctx->nchains, 0, 0, ctx->d_jc_off,
+               ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
+               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, lam_out, x_out, 0.0, 0.0, nullptr, ctrl);
+    return hip_check(ctx, hipGetLastError(), "joint apply launch");
+}
+
 }  // namespace cdx
 
 using namespace cdx;

@@ -861,6 +861,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     }
     ctx->solved = false;
     ctx->pj_last = false;
+    ctx->pjj_last = false;
+    ctx->last_theta = theta;
     ctx->record_valid = false;
     ctx->recorded = 0;
     const int nb = ctx->nb, nc = ctx->nc;
@@ -893,6 +895,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         }
         ctx->solved = true;
         ctx->pj_last = !joints && !block;
+        ctx->pjj_last = joints;
         ctx->joints_solved = joints;
         ctx->record_valid = ctx->record && !joints && !block;
         return CDX_OK;
@@ -940,6 +943,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     ctx->solved = true;
     ctx->joints_solved = joints;
     ctx->pj_last = !joints && !block;  // the adjoint differentiates the row-scaled, diagonal-S solve only
+    ctx->pjj_last = joints && prm->accel == CDX_ACCEL_NONE;  // joint rows refuse block mode and acceleration
     ctx->recorded = iters;
     ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && !block && iters <= ctx->record_max;
     return CDX_OK;

@@ -60,6 +60,7 @@ EXPORTED = [
     "cdx_batch_autotune",
     "cdx_set_recording",
     "cdx_pj_adjoint",
+    "cdx_pj_adjoint_joints",
 ]
 CDX_ADJ_IMPLICIT = 0
 CDX_ADJ_UNROLLED = 1
@@ -126,6 +127,8 @@ def _declare(lib):
         "cdx_set_recording": (C.c_int, [_vp, C.c_int, C.c_int]),
         "cdx_pj_adjoint": (C.c_int, [_vp, C.c_double, C.POINTER(SolverParams), C.POINTER(AdjointParams), _dp, _dp, _dp, _dp,
                                      _dp, _dp, C.POINTER(AdjointInfo)]),
+        "cdx_pj_adjoint_joints": (C.c_int, [_vp, C.c_double, C.POINTER(SolverParams), C.POINTER(AdjointParams), _dp, _dp,
+                                            _dp, _dp, _dp, _dp, C.POINTER(AdjointInfo)]),
         "cdx_batch_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _dp]),
         "cdx_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
         "cdx_batch_autotune": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _ip, _dp]),
@@ -330,6 +333,20 @@ class Context:
         self._check(self._l.cdx_pj_adjoint(self.h, dt, C.byref(params), C.byref(a), dptr(gv), dptr(gp), dptr(ov), dptr(om),
                                            dptr(of), dptr(omu), C.byref(info)), "cdx_pj_adjoint")
         return dict(v=ov, minv=om, fext=of, mu=omu[:self.nc], info=info)
+
+    def pj_adjoint_joints(self, dt, params, g_v, g_p=None, max_iter=100000, tol=1e-12):
+        """dL/dv_n, dL/df_ext (stacked, nv), dL/dmu per contact and dL/dcomp per joint row (caller's
+        row order) of the last joint solve, by the implicit adjoint."""
+        a = AdjointParams(CDX_ADJ_IMPLICIT, max_iter, tol)
+        info = AdjointInfo()
+        gv = np.ascontiguousarray(g_v, dtype=np.float64)
+        gp = None if g_p is None else np.ascontiguousarray(g_p, dtype=np.float64).reshape(-1)
+        ov, of = np.zeros(self.nv), np.zeros(self.nv)
+        omu = np.zeros(max(self.nc, 1))
+        oc = np.zeros(max(self.njrows, 1))
+        self._check(self._l.cdx_pj_adjoint_joints(self.h, dt, C.byref(params), C.byref(a), dptr(gv), dptr(gp), dptr(ov),
+                                                  dptr(of), dptr(omu), dptr(oc), C.byref(info)), "cdx_pj_adjoint_joints")
+        return dict(v=ov, fext=of, mu=omu[:self.nc], comp=oc[:self.njrows], info=info)
 
     def velocities(self):
         out = np.empty(self.nv, dtype=np.float64)

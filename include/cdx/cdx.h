@@ -219,6 +219,20 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
                    const double* g_v, const double* g_p, double* gv_out, double* gminv_out, double* gfext_out,
                    double* gmu_out, cdx_adjoint_info* info);
 
+/* Vector-Jacobian product of the last cdx_pj_solve or cdx_batch_pj_solve with joint rows
+ * (implicit mode; CDX_ADJ_AUTO has no unrolled fallback here).  The same adjoint iteration as
+ * cdx_pj_adjoint with S^-1 (the extended system of PjAssembler, pj_assembler.cpp:9-124) in place
+ * of M^-1, contact geometry and joint Jacobians held fixed; dt must be the solve's step (theta is
+ * taken from the solve).  Inputs: g_v = dL/dv_{n+1} (nv, required), g_p = dL/dp (3*nc or NULL).
+ * Outputs (each may be NULL): gv_out = dL/dv_n (nv), gfext_out = dL/df_ext (nv), gmu_out = dL/dmu
+ * per contact (nc), gcomp_out = dL/dcomp per joint row in the order given to cdx_set_joints (the
+ * compliance C of a spring row, A of a damper row; a damper of damping D has A = 1/D, so
+ * dL/dD = -dL/dA / D^2).  Mass gradients are not offered with joint rows.  No counterpart in the
+ * reference, which has no differentiable solve. */
+int cdx_pj_adjoint_joints(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
+                          const double* g_v, const double* g_p, double* gv_out, double* gfext_out, double* gmu_out,
+                          double* gcomp_out, cdx_adjoint_info* info);
+
 /* Kernel timing with HIP events on the context's stream (the analogue of the reference's
  * TimingManager scopes, src/misc/timings/TimingManager.hpp:16-58).  When enabled every
  * launch is bracketed by events; cdx_kernel_time reads back the call count and total ms for

@@ -1,0 +1,140 @@
+"""Gradients through joint rows: the implicit adjoint of the extended-system solve
+(cdx_pj_adjoint_joints) against central finite differences of the oracle's forward solve run to
+tight tolerance, for dL/d(joint damping), dL/dv_n, dL/df_ext and dL/dmu, after the grid-wide and
+the batched solve."""
+import numpy as np
+import pytest
+
+import oracle
+from cardillocxx_amd import joints, native, scenes
+from test_gpu_joints import chain_case, chains_batch, params
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-3
+TIGHT = dict(tol_abs=1e-14, tol_rel=1e-14, max_iter=2_000_000)
+
+
+def forward(prob, rows, theta=1.0, lam_prev=None):
+    st, v, imp, lam, res = oracle.pj_solve_joints(prob, rows, DT, oracle.params(**TIGHT), theta=theta, lam_prev=lam_prev)
+    assert st == 0
+    return v
+
+
+def device_grad(ctx, prob, rows, gv, theta=1.0, lam_prev=None, batched=None):
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.set_joints(rows, lam_prev=lam_prev)
+    p = params(tol_abs=1e-13, tol_rel=1e-13)
+    if batched is None:
+        ctx.pj_solve(DT, theta, p)
+    else:
+        ctx.set_scenes(*batched)
+        ctx.batch_pj_solve(DT, theta, p)
+    return ctx.pj_adjoint_joints(DT, p, gv, tol=1e-13)
+
+
+def with_damping(rows, joint, D):
+    """Rows with the damper rows of `joint` at damping D (A = 1/D)."""
+    r = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in rows.items()}
+    sel = (r["joint"] == joint) & (np.arange(r["ns"] + r["nd"]) >= r["ns"])
+    r["comp"][sel] = 1.0 / D
+    return r
+
+
+def fd(f, x0, h):
+    return (f(x0 + h) - f(x0 - h)) / (2 * h)
+
+
+def check(a, b, rtol, atol):
+    assert abs(a - b) <= rtol * max(abs(a), abs(b)) + atol, (a, b)
+
+
+@pytest.mark.parametrize("lower", [True, False], ids=["terrain", "free"])
+def test_damping_gradient_matches_finite_differences(ctx, lower):
+    prob, rows = chain_case(nchains=2, links=5, seed=1, lower=lower, damping=0.05)
+    gv = np.random.default_rng(0).normal(size=prob.nv)
+    out = device_grad(ctx, prob, rows, gv)
+    assert out["info"].iters >= 0
+    nd = rows["ns"]
+    for j in (0, 3, 6):
+        sel = (rows["joint"] == j) & (np.arange(rows["ns"] + rows["nd"]) >= nd)
+        dA = out["comp"][sel].sum()
+        D = 0.05
+        g_dev = -dA / D ** 2
+        g_fd = fd(lambda d: gv @ forward(prob, with_damping(rows, j, d)), D, 1e-4 * D)
+        check(g_dev, g_fd, 1e-5, 1e-9 * np.abs(gv).max())
+
+
+def test_velocity_force_and_friction_gradients(ctx):
+    prob, rows = chain_case(nchains=2, links=5, seed=1, lower=True)
+    assert prob.nc > 0
+    rng = np.random.default_rng(1)
+    gv = rng.normal(size=prob.nv)
+    out = device_grad(ctx, prob, rows, gv)
+    for i in rng.choice(prob.nv, size=6, replace=False):
+        def L(x, field):
+            p = prob.copy()
+            getattr(p, field)[i] = x
+            p.fgyro = scenes.gyro_force(p.dof, p.minv, p.v)  # f_gyro = -w x (I w) follows v
+            return gv @ forward(p, rows)
+        check(out["v"][i], fd(lambda x: L(x, "v"), prob.v[i], 1e-6), 1e-5, 1e-8)
+        check(out["fext"][i], fd(lambda x: L(x, "fext"), prob.fext[i], 1e-4), 1e-5, 1e-10)
+    for c in rng.choice(prob.nc, size=4, replace=False):
+        def Lmu(x):
+            p = prob.copy()
+            p.mu[c] = x
+            return gv @ forward(p, rows)
+        check(out["mu"][c], fd(Lmu, prob.mu[c], 1e-6), 1e-4, 1e-8)
+
+
+def test_theta_and_spring_compliance(ctx):
+    """theta < 1 adds the (1 - theta) G v term to the joint rows' right-hand side; compliant
+    spring rows (C > 0) with Lambda_prev give the dL/dC of E and of the Baumgarte term."""
+    prob, rows = chain_case(nchains=2, links=4, seed=2, lower=True)
+    rows = dict(rows)
+    rows["comp"] = rows["comp"].copy()
+    rows["comp"][: rows["ns"]] = 1e-7
+    lam_prev = np.random.default_rng(4).normal(size=rows["ns"] + rows["nd"]) * 1e-2
+    gv = np.random.default_rng(2).normal(size=prob.nv)
+    out = device_grad(ctx, prob, rows, gv, theta=0.7, lam_prev=lam_prev)
+    rng = np.random.default_rng(3)
+    for i in rng.choice(prob.nv, size=4, replace=False):
+        def L(x):
+            p = prob.copy()
+            p.v[i] = x
+            p.fgyro = scenes.gyro_force(p.dof, p.minv, p.v)
+            return gv @ forward(p, rows, theta=0.7, lam_prev=lam_prev)
+        check(out["v"][i], fd(L, prob.v[i], 1e-6), 1e-5, 1e-8)
+    for r in (0, 4, rows["ns"] + 1):
+        def Lc(x):
+            rr = dict(rows)
+            rr["comp"] = rows["comp"].copy()
+            rr["comp"][r] = x
+            return gv @ forward(prob, rr, theta=0.7, lam_prev=lam_prev)
+        c0 = rows["comp"][r]
+        check(out["comp"][r], fd(Lc, c0, 1e-3 * c0), 1e-4, 1e-9 * abs(out["comp"]).max())
+
+
+def test_after_batched_solve_equals_grid_solve(ctx):
+    prob, rows, boff, coff = chains_batch(nchains=4, links=5, seed=6)
+    gv = np.random.default_rng(7).normal(size=prob.nv)
+    a = device_grad(ctx, prob, rows, gv)
+    b = device_grad(ctx, prob, rows, gv, batched=(boff, coff))
+    for k in ("v", "fext", "mu", "comp"):
+        np.testing.assert_allclose(b[k], a[k], rtol=1e-8, atol=1e-10 * max(np.abs(a[k]).max(), 1e-300))
+
+
+def test_refusals(ctx):
+    prob, rows = chain_case(nchains=1, links=3, lower=False)
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    with pytest.raises(native.CdxError):  # no joint solve yet
+        ctx.pj_adjoint_joints(DT, params(), np.ones(prob.nv))
+    ctx.set_joints(rows)
+    ctx.pj_solve(DT, 1.0, params())
+    with pytest.raises(native.CdxError) as e:  # the row-scaled contact adjoint does not take joint rows
+        ctx.pj_adjoint(DT, params(), np.ones(prob.nv))
+    assert e.value.status == native.CDX_NO_STATE
+    out = ctx.pj_adjoint_joints(DT, params(), np.ones(prob.nv))
+    assert np.isfinite(out["v"]).all() and np.isfinite(out["comp"]).all()
