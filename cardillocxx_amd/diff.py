@@ -198,6 +198,125 @@ class Rollout:
         return Grads(v0=a.reshape(nb, 6), mu=g_mu, mass=g_mass)
 
 
+@dataclasses.dataclass
+class ChainGrads:
+    v0: np.ndarray  # [nb, 6]
+    damping: np.ndarray  # [nj] dL/dD_rot per joint
+    mu: np.ndarray  # [nc_total] dL/dmu summed per contact of every step (diagnostic)
+
+
+class ChainRollout(Rollout):
+    """Rollout of bodies joined by damped spherical joints (joints.spherical_rows): joint rows and
+    contacts in one projected Jacobi per step (cdx_pj_solve, or cdx_batch_pj_solve per scene),
+    with Lambda carried from step to step.  Parameters: initial velocities v0 and the rotational
+    damping D of every joint; masses and friction are fixed.  The backward uses the implicit
+    adjoint through the joint rows (cdx_pj_adjoint_joints) with contact geometry and joint
+    Jacobians of every step held fixed; a damper row has A = 1/D, so dL/dD = -dL/dA / D^2."""
+
+    def __init__(self, batch: ShapeBatch, joints_spec: dict, steps: int, **kw):
+        super().__init__(batch, steps, **kw)
+        from . import joints as jmod
+        self.jmod = jmod
+        self.spec = joints_spec
+        self.frames = jmod.joint_frames(batch.q, joints_spec)
+        self.nj = len(joints_spec["body_a"])
+        self.frozen_rows = None  # optional per-step joint rows to reuse (geometry held fixed)
+
+    def _rows(self, step, q):
+        if self.frozen_rows is None:
+            return self.jmod.spherical_rows(q, self.spec, self.frames, self.damping)
+        r = dict(self.frozen_rows[step])  # frozen Jacobians; the damper rows follow the damping
+        damper = np.arange(r["ns"] + r["nd"]) >= r["ns"]
+        r["comp"] = r["comp"].copy()
+        r["comp"][damper] = 1.0 / self.damping[r["joint"][damper]]
+        return r
+
+    def _solve_joints(self, q, v, fgyro, con, warm, rows, lam_prev, prm=None):
+        ctx = self.ctx
+        ctx.set_bodies(np.full(len(q) // 7, 6, np.int32), self.minv, v, self.fext, fgyro, q)
+        ctx.set_contacts(con["body_a"], con["body_b"], con["mu"], con["frame"], con["angA"], con["angB"], con["rest"],
+                         con["vbias"], warm)
+        ctx.set_joints(rows, lam_prev=lam_prev)
+        self.stats["forward_solves"] += 1
+        if self.batched and prm is None:
+            ctx.set_scenes(self.batch.body_off, con["contact_off"])
+            iters, _ = ctx.batch_pj_solve(self.dt, 1.0, self.prm)
+            self.stats["sweeps"] += int(iters.max()) if len(iters) else 0
+            return
+        info = ctx.pj_solve(self.dt, 1.0, prm or self.prm)
+        self.stats["sweeps"] += info.iters
+
+    def _chain_step(self, step, q, v, prev, lam_prev):
+        from .scenes import match_warmstart
+
+        con = self._contacts(step, q)
+        warm = match_warmstart(prev, con) if prev is not None and len(con["body_a"]) else np.zeros((len(con["body_a"]), 3))
+        rows = self._rows(step, q)
+        fgyro = _gyro(self.minv, v)
+        self._solve_joints(q, v, fgyro, con, warm, rows, lam_prev)
+        v1, imp, lam = self.ctx.velocities(), self.ctx.impulses(), self.ctx.joint_impulses()
+        self.ctx.moreau_update(self.dt, 1.0)
+        q1 = self.ctx.positions()
+        rec = dict(q=q, v=v, con=con, warm=warm, imp=imp, fgyro=fgyro, rows=rows, lam_prev=lam_prev)
+        nxt = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
+        return q1, v1, rec, nxt, lam
+
+    def forward(self, v0, damping):
+        """Runs the rollout; returns final positions [nb, 3]."""
+        b = self.batch
+        self.v0 = np.ascontiguousarray(v0, np.float64).reshape(-1)
+        self.damping = np.broadcast_to(np.asarray(damping, np.float64), (self.nj,)).copy()
+        self.mass = np.asarray(b.mass, np.float64)
+        bd = self._bodies(self.v0, self.mass)
+        self.minv, self.fext = bd["minv"], bd["fext"]
+        q, v = bd["q"].copy(), self.v0.copy()
+        self.checkpoints = {}
+        self.contacts_log, self.rows_log = [], []
+        prev, lam = None, None
+        for n in range(self.steps):
+            if n % self.k == 0:
+                self.checkpoints[n] = (q.copy(), v.copy(), prev, lam)
+            q, v, rec, prev, lam = self._chain_step(n, q, v, prev, lam)
+            self.contacts_log.append(rec["con"])
+            self.rows_log.append(rec["rows"])
+        self.qN = q
+        return q.reshape(-1, 7)[:, :3].copy()
+
+    def backward(self, g_pos):
+        """dL/d(v0, damping) given dL/dx_N (positions, [nb, 3])."""
+        nb = len(self.mass)
+        gx = np.zeros((nb, 6))
+        gx[:, :3] = np.asarray(g_pos, np.float64).reshape(nb, 3)
+        gx = gx.reshape(-1)
+        a = np.zeros(nb * 6)
+        g_damp = np.zeros(self.nj)
+        g_mu = []
+        prm1 = native.default_params()
+        prm1.max_iter, prm1.tol_abs, prm1.tol_rel, prm1.warmstart = 1, 0.0, 0.0, 1
+        for s in reversed(sorted(self.checkpoints)):
+            e = min(s + self.k, self.steps)
+            q, v, prev, lam = self.checkpoints[s]
+            seg = []
+            for n in range(s, e):
+                q1, v1, rec, prev, lam = self._chain_step(n, q, v, prev, lam)
+                seg.append(rec)
+                q, v = q1, v1
+            for n in range(e - 1, s - 1, -1):
+                rec = seg[n - s]
+                # restore the converged state: one sweep warm-started from the converged impulses
+                self._solve_joints(rec["q"], rec["v"], rec["fgyro"], rec["con"], rec["imp"], rec["rows"], rec["lam_prev"],
+                                   prm1)
+                g = self.ctx.pj_adjoint_joints(self.dt, self.prm, a + self.dt * gx, max_iter=self.adjoint_max_iter,
+                                               tol=self.adjoint_tol)
+                self.stats["adjoint_iters"] += g["info"].iters
+                a = g["v"]
+                rows = rec["rows"]
+                damper = np.arange(rows["ns"] + rows["nd"]) >= rows["ns"]
+                np.add.at(g_damp, rows["joint"][damper], -g["comp"][damper] / self.damping[rows["joint"][damper]] ** 2)
+                g_mu.append(g["mu"])
+        return ChainGrads(v0=a.reshape(nb, 6), damping=g_damp, mu=np.concatenate(g_mu[::-1]) if g_mu else np.zeros(0))
+
+
 def _gyro(minv, v):
     nb = len(v) // 6
     w = v.reshape(nb, 6)[:, 3:]

@@ -138,3 +138,36 @@ def test_refusals(ctx):
     assert e.value.status == native.CDX_NO_STATE
     out = ctx.pj_adjoint_joints(DT, params(), np.ones(prob.nv))
     assert np.isfinite(out["v"]).all() and np.isfinite(out["comp"]).all()
+
+
+def test_chain_rollout_damping_gradient_vs_finite_differences():
+    """Checkpointed rollout of chains dropping onto the terrain: dL/dD per joint and dL/dv0 of
+    the COM loss against central differences of the same rollout with the geometry of every step
+    (contacts and joint Jacobians) held at the nominal trajectory's, as the adjoint holds it."""
+    from cardillocxx_amd import diff
+
+    t = scenes.fractal_heightfield(n=33, extent=1.6, amp=0.03, seed=4)
+    batch, jt = scenes.capsule_chains(nchains=2, links=5, terrain=t, seed=4, drop=0.0005, local=True)
+    batch.v[:] = np.random.default_rng(9).normal(scale=0.2, size=batch.v.shape)
+    r = diff.ChainRollout(batch, jt, steps=14, dt=1e-3, checkpoint_every=5, tol=1e-12, adjoint_tol=1e-13)
+    target = np.random.default_rng(0).normal(scale=0.1, size=(batch.nscenes, 3))
+    D0 = np.full(len(jt["body_a"]), 0.05)
+    v0 = batch.v.reshape(-1).copy()
+    pos = r.forward(v0, D0)
+    L, gpos, _ = diff.com_loss(pos, batch.mass, batch.body_off, target)
+    g = r.backward(gpos)
+    assert sum(len(c["body_a"]) for c in r.contacts_log) > 0
+    r.frozen, r.frozen_rows = r.contacts_log, r.rows_log
+
+    def loss(v, D):
+        return diff.com_loss(r.forward(v, D), batch.mass, batch.body_off, target)[0]
+
+    rng = np.random.default_rng(1)
+    for _ in range(2):
+        d = rng.normal(size=D0.shape)
+        eps = 1e-5
+        fdv = (loss(v0, D0 + eps * d * D0) - loss(v0, D0 - eps * d * D0)) / (2 * eps)
+        assert float(g.damping @ (d * D0)) == pytest.approx(fdv, rel=1e-4, abs=1e-12)
+        dv = rng.normal(size=v0.shape)
+        fdv = (loss(v0 + 1e-6 * dv, D0) - loss(v0 - 1e-6 * dv, D0)) / 2e-6
+        assert float(g.v0.reshape(-1) @ dv) == pytest.approx(fdv, rel=1e-4, abs=1e-12)
