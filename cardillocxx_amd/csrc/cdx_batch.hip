@@ -23,8 +23,8 @@ namespace cdx {
 constexpr int kMaxBatchBlock = 512;
 
 // Joint rows of the batch (null sc_off: none).  The chains of scene s are sc_list[sc_off[s] ..
-// sc_off[s+1]); each is factorised and solved by one lane of the scene's group, as the
-// grid-wide kernels do with one thread per chain.
+// sc_off[s+1]); each is factorised and solved by one wave of the scene's group, as the grid-wide
+// kernels do with one workgroup per chain.
 struct BatchJoints {
     const int* sc_off;
     const int* sc_list;
@@ -33,6 +33,7 @@ struct BatchJoints {
     const int* jb_off;
     const int* jb_body;
     const int* jr_kind;
+    const int* jr_blk;
     const double* jr;
     double* rlam;
     double* Lm;
@@ -42,6 +43,7 @@ struct BatchJoints {
     const int* jslot;
     double* tr;
     double* z;
+    double* y;
     double* lam;
     double theta;
     double beta;
@@ -63,6 +65,8 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     __shared__ int gdone[kMaxBatchBlock / 64];
     __shared__ int gfail[kMaxBatchBlock / 64];
     __shared__ int all_done;
+    // per wave: the chain factorisation, then the staging window of the chain solve
+    __shared__ double jlds[Joints ? kMaxBatchBlock / 64 : 1][Joints ? kFactorLds : 1];
     const int gsize = blockDim.x / groups;
     const int g = threadIdx.x / gsize, lt = threadIdx.x - g * gsize;
     const int slot = blockIdx.x * groups + g;
@@ -79,17 +83,19 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     for (int b = b0 + lt; b < b1; b += gsize)
         pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x, ja.jslot, ja.tr);
     if (lt == 0) gfail[g] = 0;
+    const int wv = lt >> 6, nwv = gsize >> 6, lane = lt & 63;  // the group's waves take its chains
     if constexpr (Joints) {  // factor the scene's chains, then x0 and Lambda_0 of its jointed bodies
         __syncthreads();
         if (s >= 0)
-            for (int q = ja.sc_off[s] + lt; q < ja.sc_off[s + 1]; q += gsize) {
+            for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv) {
                 const int ch = ja.sc_list[q];
-                if (joint_factor_chain(ch, ja.jc_off, ja.jb_off, ja.jb_body, ja.jr_kind, ja.jr, minv, v, dt, ja.theta,
-                                       ja.beta, ja.rlam, ja.Lm, ja.Dinv))
+                if (joint_factor_chain(ch, lane, ja.jc_off, ja.jb_off, ja.jb_body, ja.jr_kind, ja.jr_blk, ja.jr, minv, v, dt,
+                                       ja.theta, ja.beta, ja.rlam, ja.Lm, ja.Dinv, jlds[threadIdx.x >> 6]))
                     gfail[g] = 1;
                 else
-                    joint_solve_chain(ch, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr, ja.rlam, ja.Lm,
-                                      ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.lam, x, tol_rel, tol_abs);
+                    joint_solve_chain(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr_blk, ja.jr, ja.rlam,
+                                      ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel,
+                                      tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
     }
     __syncthreads();
@@ -112,11 +118,12 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
                 e2 += pj_body_sweep(b, dof, minv, incptr, incidx, imp, x, check, tol_rel, tol_abs, ja.jslot, ja.tr);
         if constexpr (Joints) {  // the chains apply the jointed bodies' W^T dp through S^-1
             __syncthreads();
-            if (!done)
-                for (int q = ja.sc_off[s] + lt; q < ja.sc_off[s + 1]; q += gsize)
-                    e2 += joint_solve_chain(ja.sc_list[q], 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr,
-                                            ja.rlam, ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.lam, x,
-                                            tol_rel, tol_abs);
+            if (!done) {
+                for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv)
+                    e2 += joint_solve_chain(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body,
+                                            ja.jr_blk, ja.jr, ja.rlam, ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z,
+                                            ja.y, ja.lam, x, tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
+            }
         }
         if (check) {
             for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
@@ -195,10 +202,11 @@ static int upload_scene_chains(cdx_ctx* ctx) {
 static BatchJoints batch_joints(cdx_ctx* ctx, double theta) {
     BatchJoints ja{};
     if (ctx->njrows == 0) return ja;
-    ja = BatchJoints{ctx->d_sc_off,  ctx->d_sc_list, ctx->d_jc_off, ctx->d_jcb_off,    ctx->d_jb_off, ctx->d_jb_body,
-                     ctx->d_jr_kind, ctx->d_jr,      ctx->d_jrlam,  ctx->d_jL,         ctx->d_jDinv,  ctx->d_jbody,
-                     ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jtr,    ctx->d_jz,         ctx->d_jlam,   theta,
-                     ctx->joint_bias};
+    ja = BatchJoints{ctx->d_sc_off, ctx->d_sc_list, ctx->d_jc_off,    ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body,
+                     ctx->d_jr_kind, ctx->d_jr_blk, ctx->d_jr,       ctx->d_jrlam,   ctx->d_jL,     ctx->d_jDinv,
+                     ctx->d_jbody,   ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jtr,     ctx->d_jz,     ctx->d_jy,
+                     ctx->d_jlam,
+                     theta,          ctx->joint_bias};
     return ja;
 }
 

@@ -76,12 +76,14 @@ struct cdx_ctx {
     int *d_sc_off = nullptr, *d_sc_list = nullptr;  // batched solve: chains of each scene
     size_t sc_off_cap = 0, sc_list_cap = 0;
     int *d_jc_off = nullptr, *d_jcb_off = nullptr, *d_jb_off = nullptr, *d_jb_body = nullptr, *d_jr_kind = nullptr;
-    int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr;
+    int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr, *d_jr_blk = nullptr;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
+    double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)
+    size_t jy_cap = 0;
     size_t jc_off_cap = 0, jcb_off_cap = 0, jb_off_cap = 0, jb_body_cap = 0, jr_kind_cap = 0, jbody_cap = 0,
            jbody_blk_cap = 0, jslot_cap = 0, jr_cap = 0, jrlam_cap = 0, jz_cap = 0, jlam_cap = 0, jL_cap = 0, jDinv_cap = 0,
-           jtr_cap = 0;
+           jtr_cap = 0, jr_blk_cap = 0;
     bool any_rest = false;  // any non-zero restitution row (PgsAssembler::rhs, pgs_assembler.cpp:64-67)
     bool any_bias = false;  // any row with restitution or a velocity bias: b = R (c + e (W v + c)) != 0
     int* d_ida = nullptr;

@@ -25,56 +25,60 @@
 
 namespace cdx {
 
-// One thread per chain (see joint_factor_chain); a singular block stops the solve.
-__global__ void k_joint_factor(int nchains, const int* __restrict__ jc_off, const int* __restrict__ jb_off,
-                               const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
-                               const double* __restrict__ jr, const double* __restrict__ minv,
-                               const double* __restrict__ v, double dt, double theta, double beta,
-                               double* __restrict__ rlam, double* __restrict__ Lm, double* __restrict__ Dinv,
-                               int* __restrict__ ctrl) {
-    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+// One 64-lane workgroup per chain (see joint_factor_chain); a singular block stops the solve.
+__global__ void __launch_bounds__(64) k_joint_factor(int nchains, const int* __restrict__ jc_off, const int* __restrict__ jb_off,
+                                                     const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
+                                                     const int* __restrict__ jr_blk, const double* __restrict__ jr,
+                                                     const double* __restrict__ minv, const double* __restrict__ v, double dt,
+                                                     double theta, double beta, double* __restrict__ rlam,
+                                                     double* __restrict__ Lm, double* __restrict__ Dinv,
+                                                     int* __restrict__ ctrl) {
+    __shared__ double lds[kFactorLds];
+    const int ch = blockIdx.x;
     if (ch >= nchains) return;
-    if (joint_factor_chain(ch, jc_off, jb_off, jb_body, jr_kind, jr, minv, v, dt, theta, beta, rlam, Lm, Dinv)) {
+    if (joint_factor_chain(ch, threadIdx.x, jc_off, jb_off, jb_body, jr_kind, jr_blk, jr, minv, v, dt, theta, beta, rlam, Lm,
+                           Dinv, lds) &&
+        threadIdx.x == 0) {
         ctrl[2] = CDX_NOT_FACTORIZED;
         ctrl[0] = 1;
     }
 }
 
-// One thread per chain (see joint_solve_chain); with `check` the per-workgroup residual partials.
-__global__ void __launch_bounds__(kBlock) k_joint_solve(
+// One 64-lane workgroup per chain (see joint_solve_chain); with `check` one residual partial per chain.
+__global__ void __launch_bounds__(64) k_joint_solve(
     int nchains, int setup, int check, const int* __restrict__ jc_off, const int* __restrict__ jcb_off,
-    const int* __restrict__ jb_off, const int* __restrict__ jb_body, const double* __restrict__ jr,
-    const double* __restrict__ rlam, const double* __restrict__ Lm, const double* __restrict__ Dinv,
-    const int* __restrict__ jbody, const int* __restrict__ jbody_blk, const double* __restrict__ minv,
-    const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ lam, double* __restrict__ x,
-    double tol_rel, double tol_abs, double* __restrict__ partial, const int* __restrict__ ctrl) {
+    const int* __restrict__ jb_off, const int* __restrict__ jb_body, const int* __restrict__ jr_blk,
+    const double* __restrict__ jr, const double* __restrict__ rlam, const double* __restrict__ Lm,
+    const double* __restrict__ Dinv, const int* __restrict__ jbody, const int* __restrict__ jbody_blk,
+    const double* __restrict__ minv, const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ lam,
+    double* __restrict__ yb, double* __restrict__ x, double tol_rel, double tol_abs, double* __restrict__ partial,
+    const int* __restrict__ ctrl) {
+    __shared__ double lds[kSolveLdsL + kSolveLdsR];
     if (ctrl && ctrl[0]) return;
-    __shared__ double red[kBlock / 64];
-    const int ch = blockIdx.x * kBlock + threadIdx.x;
-    double e2 = 0;
-    if (ch < nchains)
-        e2 = joint_solve_chain(ch, setup, check, jc_off, jcb_off, jb_off, jb_body, jr, rlam, Lm, Dinv, jbody, jbody_blk,
-                               minv, tr, z, lam, x, tol_rel, tol_abs);
+    const int ch = blockIdx.x;
+    if (ch >= nchains) return;
+    double e2 = joint_solve_chain(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jb_body, jr_blk, jr, rlam, Lm, Dinv,
+                                  jbody, jbody_blk, minv, tr, z, yb, lam, x, tol_rel, tol_abs, lds, kSolveLdsL, kSolveLdsR);
     if (check) {
-        const double s = block_sum(e2, red);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+        for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+        if (threadIdx.x == 0) partial[ch] = e2;
     }
 }
 
 int joints_factor(cdx_ctx* ctx, double dt, double theta) {
     if (ctx->nchains == 0) return CDX_OK;
-    CDX_LAUNCH(ctx, "joint_factor", k_joint_factor, grid_for(ctx->nchains), kBlock, ctx->nchains, ctx->d_jc_off, ctx->d_jb_off,
-               ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jr, ctx->d_minv, ctx->d_v, dt, theta, ctx->joint_bias, ctx->d_jrlam,
-               ctx->d_jL, ctx->d_jDinv, ctx->d_ctrl);
+    CDX_LAUNCH(ctx, "joint_factor", k_joint_factor, ctx->nchains, 64, ctx->nchains, ctx->d_jc_off, ctx->d_jb_off,
+               ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jr_blk, ctx->d_jr, ctx->d_minv, ctx->d_v, dt, theta, ctx->joint_bias,
+               ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_ctrl);
     return hip_check(ctx, hipGetLastError(), "joint factor launch");
 }
 
 int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial) {
     if (ctx->nchains == 0) return CDX_OK;
-    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, grid_for(ctx->nchains), kBlock, ctx->nchains, setup, check, ctx->d_jc_off,
-               ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
-               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_x, tol_rel, tol_abs, partial,
-               setup ? nullptr : ctx->d_ctrl);
+    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, setup, check, ctx->d_jc_off, ctx->d_jcb_off,
+               ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_blk, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
+               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_jy, ctx->d_x, tol_rel, tol_abs,
+               partial, setup ? nullptr : ctx->d_ctrl);
     return hip_check(ctx, hipGetLastError(), "joint solve launch");
 }
 
@@ -82,9 +86,9 @@ int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_
 // both accumulate): the joint rows' share of S^-1 applied to a body vector, no right-hand side.
 int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) {
     if (ctx->nchains == 0) return CDX_OK;
-    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, grid_for(ctx->nchains), kBlock, ctx->nchains, 0, 0, ctx->d_jc_off,
-               ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
-               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, lam_out, x_out, 0.0, 0.0, nullptr, ctrl);
+    CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, 0, 0, ctx->d_jc_off, ctx->d_jcb_off,
+               ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_blk, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
+               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, lam_out, ctx->d_jy, x_out, 0.0, 0.0, nullptr, ctrl);
     return hip_check(ctx, hipGetLastError(), "joint apply launch");
 }
 
@@ -180,10 +184,11 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
         }
     const int nchains = (int)chain_off.size() - 1;
     // block-ordered rows and per-block data
-    std::vector<int> jb_off{0}, jb_body, jr_kind, jorig;
+    std::vector<int> jb_off{0}, jb_body, jr_kind, jorig, jr_blk;
     std::vector<double> jr;
     for (int j : order) {
         for (int r : blk_rows[j]) {
+            jr_blk.push_back((int)jb_off.size() - 1);
             jorig.push_back(r);
             jr_kind.push_back(r < ns ? 0 : 1);
             for (int k = 0; k < 6; ++k) jr.push_back(wa[6 * (size_t)r + k]);
@@ -228,6 +233,7 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     CDX_TRY(upi(&ctx->d_jb_off, jb_off, &ctx->jb_off_cap, "joint blocks"));
     CDX_TRY(upi(&ctx->d_jb_body, jb_body, &ctx->jb_body_cap, "joint block bodies"));
     CDX_TRY(upi(&ctx->d_jr_kind, jr_kind, &ctx->jr_kind_cap, "joint row kinds"));
+    CDX_TRY(upi(&ctx->d_jr_blk, jr_blk, &ctx->jr_blk_cap, "joint row blocks"));
     CDX_TRY(upi(&ctx->d_jbody, jbody, &ctx->jbody_cap, "jointed bodies"));
     CDX_TRY(upi(&ctx->d_jbody_blk, jbody_blk, &ctx->jbody_blk_cap, "jointed body blocks"));
     CDX_TRY(upi(&ctx->d_jslot, jslot, &ctx->jslot_cap, "joint slots"));
@@ -235,6 +241,7 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jrlam, R, &ctx->jrlam_cap, "joint rhs"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jz, R, &ctx->jz_cap, "joint scratch"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jy, R, &ctx->jy_cap, "joint scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jlam, R, &ctx->jlam_cap, "joint impulses"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jL, (size_t)nblk * kMaxJRows * kMaxJRows, &ctx->jL_cap, "joint factors"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jDinv, (size_t)nblk * kMaxJRows * kMaxJRows, &ctx->jDinv_cap, "joint factors"));

@@ -1,14 +1,30 @@
 // cdx_joints_device.hpp -- per-chain work of the joint rows (block LDL^T of the Schur complement
 // K = G M^-1 G^T + E and its substitutions), shared by the grid-wide kernels (cdx_joints.hip) and
 // the batched kernel (cdx_batch.hip).
+//
+// One wave (64 lanes) works on one chain.  The parts that are independent per row or per body
+// (right-hand sides, G M^-1 r, the body updates) use every lane; the block recurrences along the
+// chain (factorisation, forward and backward substitution) put block component i on lane i and
+// pass the previous block's components between lanes with readlane, so a step along the chain
+// costs a few register moves and FMAs instead of a chain of dependent memory loads.  Every dot
+// product is summed in the same order as a one-thread-per-chain loop would sum it.
 #pragma once
 
 #include "cdx_device.hpp"
 
 namespace cdx {
 
-constexpr int kMaxJRows = 12;  // rows per joint block (a full 6-dof joint with dampers)
+constexpr int kMaxJRows = 12;    // rows per joint block (a full 6-dof joint with dampers)
 constexpr int kJRowStride = 16;  // doubles per row: wa[6], wb[6], comp, vsrc, gerr, lam_prev
+constexpr int kJBlk = kMaxJRows * kMaxJRows;
+// LDS of one wave's factorisation: K_jj, K_{j,j-1}, the previous D^-1, [D | I] and the
+// elimination factors of one Gauss-Jordan column
+constexpr int kFactorLds = 3 * kJBlk + 2 * kJBlk + kMaxJRows;
+// staging window of the grid-wide chain solve (L entries, rows): a 32-link chain of spherical
+// joints with dampers (31 blocks of 6 rows) fits in one window
+constexpr int kSolveLdsL = 1536, kSolveLdsR = 256;
+// the batched kernel stages through each wave's factorisation LDS
+constexpr int kBatchLdsL = 600, kBatchLdsR = kFactorLds - kBatchLdsL;
 
 // inverse mass of dof k of a rigid body
 __device__ inline double minv_k(const double* __restrict__ minv, int b, int k) {
@@ -20,209 +36,335 @@ __device__ inline const double* jrow(const double* __restrict__ jr, int row, int
     return jr + (size_t)row * kJRowStride + 6 * side;
 }
 
-// Once per solve, for chain ch: the joint-row right-hand side of PjAssembler::rhs (:158-164;
-// implicit gyroscopy and lambda-theta off) and the block LDL^T factors of K.  Returns
-// CDX_NOT_FACTORIZED when a diagonal block is singular (S cannot be factorised, pj_assembler.cpp:175).
-__device__ inline int joint_factor_chain(int ch, const int* __restrict__ jc_off, const int* __restrict__ jb_off,
+// value of v held by lane l (l uniform across the wave)
+__device__ inline double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// the wave's earlier global and LDS writes are visible to all of its lanes
+__device__ inline void wave_sync() {
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Right-hand side of joint row r in block j (PjAssembler::rhs, pj_assembler.cpp:158-164;
+// implicit gyroscopy and lambda-theta off).
+__device__ inline void joint_rhs_row(int r, int j, const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
+                                     const double* __restrict__ jr, const double* __restrict__ v, double dt, double theta,
+                                     double beta, double* __restrict__ rlam) {
+    const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
+    double gv = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int b = side ? bb : ba;
+        if (b < 0) continue;
+        const double* w = jrow(jr, r, side);
+        for (int k = 0; k < 6; ++k) gv += w[k] * v[(size_t)b * kBodyStride + k];
+    }
+    const double* rr = jr + (size_t)r * kJRowStride;
+    const double comp = rr[12], vsrc = rr[13], gerr = rr[14], lprev = rr[15];
+    double o;
+    if (jr_kind[r] == 0) {
+        o = -(1.0 / (theta * dt * dt)) * (comp * lprev) - ((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
+        if (beta > 0) o -= (-(comp * lprev) / dt + gerr) * (beta / (dt * theta));
+    } else {
+        o = -((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
+    }
+    rlam[r] = o;
+}
+
+// Once per solve, chain ch on one wave: the joint-row right-hand sides and the block LDL^T
+// factors of K (L_j = K_{j,j-1} D_{j-1}^-1, D_j = K_jj - L_j K_{j,j-1}^T, D_j^-1 by Gauss-Jordan
+// with partial pivoting).  `lds` holds kFactorLds doubles private to the wave.  Returns
+// CDX_NOT_FACTORIZED (on every lane) when a diagonal block is singular, i.e. S cannot be
+// factorised (pj_assembler.cpp:175).
+__device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict__ jc_off, const int* __restrict__ jb_off,
                                          const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
-                                         const double* __restrict__ jr, const double* __restrict__ minv,
-                                         const double* __restrict__ v, double dt, double theta, double beta,
-                                         double* __restrict__ rlam, double* __restrict__ Lm, double* __restrict__ Dinv) {
-    double K[kMaxJRows * kMaxJRows], Dp[kMaxJRows * kMaxJRows], A[kMaxJRows * 2 * kMaxJRows];
+                                         const int* __restrict__ jr_blk, const double* __restrict__ jr,
+                                         const double* __restrict__ minv, const double* __restrict__ v, double dt,
+                                         double theta, double beta, double* __restrict__ rlam, double* __restrict__ Lm,
+                                         double* __restrict__ Dinv, double* __restrict__ lds) {
+    const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
+    for (int r = jb_off[j0] + lane; r < jb_off[j1]; r += 64)
+        joint_rhs_row(r, jr_blk[r], jb_body, jr_kind, jr, v, dt, theta, beta, rlam);
+    double* K = lds;
+    double* Kjp = lds + kJBlk;
+    double* Dp = lds + 2 * kJBlk;
+    double* A = lds + 3 * kJBlk;  // 2 * kJBlk
+    double* F = lds + 5 * kJBlk;  // kMaxJRows
     int mp = 0;
-    for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j) {
+    for (int j = j0; j < j1; ++j) {
         const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
         const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
-        // right-hand side of the rows
-        for (int i = 0; i < m; ++i) {
-            const int r = r0 + i;
-            double gv = 0;
+        // K_jj = sum over the block's bodies of G M^-1 G^T, plus E
+        for (int e = lane; e < m * m; e += 64) {
+            const int i = e / m, l = e - (e / m) * m;
+            double s = 0;
             for (int side = 0; side < 2; ++side) {
                 const int b = side ? bb : ba;
                 if (b < 0) continue;
-                const double* w = jrow(jr, r, side);
-                for (int k = 0; k < 6; ++k) gv += w[k] * v[(size_t)b * kBodyStride + k];
+                const double* wi = jrow(jr, r0 + i, side);
+                const double* wl = jrow(jr, r0 + l, side);
+                for (int k = 0; k < 6; ++k) s += wi[k] * minv_k(minv, b, k) * wl[k];
             }
-            const double* rr = jr + (size_t)r * kJRowStride;
-            const double comp = rr[12], vsrc = rr[13], gerr = rr[14], lprev = rr[15];
-            double o;
-            if (jr_kind[r] == 0) {
-                o = -(1.0 / (theta * dt * dt)) * (comp * lprev) - ((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
-                if (beta > 0) o -= (-(comp * lprev) / dt + gerr) * (beta / (dt * theta));
-            } else {
-                o = -((1.0 - theta) / theta) * gv - (1.0 / theta) * vsrc;
+            if (i == l) {
+                const double* rr = jr + (size_t)(r0 + i) * kJRowStride;
+                s += rr[12] * (jr_kind[r0 + i] == 0 ? 1.0 / (theta * dt * dt) : 1.0 / (theta * dt));
             }
-            rlam[r] = o;
+            K[e] = s;
         }
-        // K_jj = sum over the block's bodies of G M^-1 G^T, plus E
-        for (int i = 0; i < m; ++i)
-            for (int l = 0; l < m; ++l) {
-                double s = 0;
-                for (int side = 0; side < 2; ++side) {
-                    const int b = side ? bb : ba;
-                    if (b < 0) continue;
-                    const double* wi = jrow(jr, r0 + i, side);
-                    const double* wl = jrow(jr, r0 + l, side);
-                    for (int k = 0; k < 6; ++k) s += wi[k] * minv_k(minv, b, k) * wl[k];
-                }
-                K[i * m + l] = s;
-            }
-        for (int i = 0; i < m; ++i) {
-            const double* rr = jr + (size_t)(r0 + i) * kJRowStride;
-            K[i * m + i] += rr[12] * (jr_kind[r0 + i] == 0 ? 1.0 / (theta * dt * dt) : 1.0 / (theta * dt));
-        }
-        if (j > jc_off[ch]) {
-            // K_{j,j-1} through the body shared with the previous block, L_j = K_{j,j-1} D_{j-1}^-1,
-            // D_j = K_jj - L_j K_{j,j-1}^T
+        if (j > j0) {
+            // K_{j,j-1} through the body shared with the previous block
             const int p0 = jb_off[j - 1];
             const int pa = jb_body[2 * (j - 1)], pb = jb_body[2 * (j - 1) + 1];
-            const int s = (ba >= 0 && (ba == pa || ba == pb)) ? ba : bb;
-            const int side_j = s == ba ? 0 : 1, side_p = s == pa ? 0 : 1;
-            double Kjp[kMaxJRows * kMaxJRows];
-            for (int i = 0; i < m; ++i)
-                for (int l = 0; l < mp; ++l) {
-                    const double* wi = jrow(jr, r0 + i, side_j);
-                    const double* wl = jrow(jr, p0 + l, side_p);
-                    double t = 0;
-                    for (int k = 0; k < 6; ++k) t += wi[k] * minv_k(minv, s, k) * wl[k];
-                    Kjp[i * mp + l] = t;
-                }
-            double* L = Lm + (size_t)j * kMaxJRows * kMaxJRows;
-            for (int i = 0; i < m; ++i)
-                for (int l = 0; l < mp; ++l) {
-                    double t = 0;
-                    for (int q = 0; q < mp; ++q) t += Kjp[i * mp + q] * Dp[q * mp + l];
-                    L[i * mp + l] = t;
-                }
-            for (int i = 0; i < m; ++i)
-                for (int l = 0; l < m; ++l) {
-                    double t = 0;
-                    for (int q = 0; q < mp; ++q) t += L[i * mp + q] * Kjp[l * mp + q];
-                    K[i * m + l] -= t;
-                }
+            const int sb = (ba >= 0 && (ba == pa || ba == pb)) ? ba : bb;
+            const int side_j = sb == ba ? 0 : 1, side_p = sb == pa ? 0 : 1;
+            for (int e = lane; e < m * mp; e += 64) {
+                const int i = e / mp, l = e - (e / mp) * mp;
+                const double* wi = jrow(jr, r0 + i, side_j);
+                const double* wl = jrow(jr, p0 + l, side_p);
+                double t = 0;
+                for (int k = 0; k < 6; ++k) t += wi[k] * minv_k(minv, sb, k) * wl[k];
+                Kjp[e] = t;
+            }
+            wave_sync();
+            double* L = Lm + (size_t)j * kJBlk;
+            for (int e = lane; e < m * mp; e += 64) {
+                const int i = e / mp, l = e - (e / mp) * mp;
+                double t = 0;
+                for (int q = 0; q < mp; ++q) t += Kjp[i * mp + q] * Dp[q * mp + l];
+                L[e] = t;
+                A[e] = t;
+            }
+            wave_sync();
+            for (int e = lane; e < m * m; e += 64) {
+                const int i = e / m, l = e - (e / m) * m;
+                double t = 0;
+                for (int q = 0; q < mp; ++q) t += A[i * mp + q] * Kjp[l * mp + q];
+                K[e] -= t;
+            }
         }
+        wave_sync();
         // D_j^-1 by Gauss-Jordan with partial pivoting on [D_j | I]
         const int w2 = 2 * m;
-        for (int i = 0; i < m; ++i)
-            for (int l = 0; l < w2; ++l) A[i * w2 + l] = l < m ? K[i * m + l] : (l - m == i ? 1.0 : 0.0);
+        for (int e = lane; e < m * w2; e += 64) {
+            const int i = e / w2, l = e - (e / w2) * w2;
+            A[e] = l < m ? K[i * m + l] : (l - m == i ? 1.0 : 0.0);
+        }
+        wave_sync();
         for (int c = 0; c < m; ++c) {
             int pr = c;
             for (int i = c + 1; i < m; ++i)
                 if (fabs(A[i * w2 + c]) > fabs(A[pr * w2 + c])) pr = i;
             if (pr != c)
-                for (int l = 0; l < w2; ++l) {
+                for (int l = lane; l < w2; l += 64) {
                     const double t = A[c * w2 + l];
                     A[c * w2 + l] = A[pr * w2 + l];
                     A[pr * w2 + l] = t;
                 }
+            wave_sync();
             const double d = A[c * w2 + c];
-            if (!(fabs(d) > 0.0)) return CDX_NOT_FACTORIZED;  // singular joint block
-            for (int l = 0; l < w2; ++l) A[c * w2 + l] /= d;
-            for (int i = 0; i < m; ++i) {
-                if (i == c) continue;
-                const double f = A[i * w2 + c];
-                if (f != 0.0)
-                    for (int l = 0; l < w2; ++l) A[i * w2 + l] -= f * A[c * w2 + l];
+            if (!(fabs(d) > 0.0)) return CDX_NOT_FACTORIZED;
+            wave_sync();
+            for (int l = lane; l < w2; l += 64) A[c * w2 + l] /= d;
+            for (int i = lane; i < m; i += 64) F[i] = A[i * w2 + c];
+            wave_sync();
+            for (int e = lane; e < m * w2; e += 64) {
+                const int i = e / w2, l = e - (e / w2) * w2;
+                const double f = F[i];
+                if (i != c && f != 0.0) A[e] -= f * A[c * w2 + l];
             }
+            wave_sync();
         }
-        double* Di = Dinv + (size_t)j * kMaxJRows * kMaxJRows;
-        for (int i = 0; i < m; ++i)
-            for (int l = 0; l < m; ++l) {
-                Di[i * m + l] = A[i * w2 + m + l];
-                Dp[i * m + l] = A[i * w2 + m + l];
-            }
+        double* Di = Dinv + (size_t)j * kJBlk;
+        for (int e = lane; e < m * m; e += 64) {
+            const int i = e / m, l = e - (e / m) * m;
+            Di[e] = A[i * w2 + m + l];
+            Dp[e] = A[i * w2 + m + l];
+        }
+        wave_sync();
         mp = m;
     }
     return CDX_OK;
 }
 
-// x += S^-1 [r; -r_lambda] restricted to the jointed bodies of chain ch.
+// Staging windows along a chain (uniform across the wave; a window always holds at least one
+// block, whose L has at most kJBlk entries and whose rows at most kMaxJRows).
+// Forward: blocks [j, je) with their L_jj (m_jj x m_jj-1; none for the chain's first block j0).
+__device__ inline int stage_fwd_end(int j, int j1, int j0, const int* __restrict__ jb_off, int capL, int capR) {
+    int nl = 0, nr = 0, je = j;
+    while (je < j1) {
+        const int m = jb_off[je + 1] - jb_off[je];
+        const int ml = je > j0 ? m * (jb_off[je] - jb_off[je - 1]) : 0;
+        if (je != j && (nl + ml > capL || nr + m > capR)) break;
+        nl += ml;
+        nr += m;
+        ++je;
+    }
+    return je;
+}
+
+// Backward: blocks jj = j-1 down to je (>= j0) with L_{jj+1} (m_{jj+1} x m_jj); returns je.
+__device__ inline int stage_bwd_end(int j, int j0, const int* __restrict__ jb_off, int capL, int capR) {
+    int nl = 0, nr = 0, je = j;
+    while (je > j0) {
+        const int jj = je - 1;
+        const int m = jb_off[jj + 1] - jb_off[jj];
+        const int ml = (jb_off[jj + 2] - jb_off[jj + 1]) * m;
+        if (je != j && (nl + ml > capL || nr + m > capR)) break;
+        nl += ml;
+        nr += m;
+        --je;
+    }
+    return je;
+}
+
+// x += S^-1 [r; -r_lambda] restricted to the jointed bodies of chain ch, on one wave.
 //   setup = 1: x0 and Lambda_0 are set (absolute; tr holds the full body right-hand side);
 //   setup = 0: a sweep increment (tr holds W^T dp of each jointed body); x and Lambda accumulate.
-// Returns the chain bodies' part of the residual sum (projected_jacobi.cpp:140-144) when `check`.
-__device__ inline double joint_solve_chain(int ch, int setup, int check, const int* __restrict__ jc_off,
+// Returns this lane's share of the chain bodies' residual sum (projected_jacobi.cpp:140-144)
+// when `check`.  z and y are scratch of one double per joint row; `lds` holds capL + capR
+// doubles private to the wave, the staging window of the two passes along the chain.
+//
+//   s = G M^-1 t (rows in parallel);  z_j = s_j - L_j z_{j-1} (along the chain);
+//   y_j = D_j^-1 z_j (rows in parallel);  Lambda_j = y_j - L_{j+1}^T Lambda_{j+1} (back along it);
+//   Lambda, then the bodies' dv = M^-1 (t - G^T Lambda) (in parallel).
+__device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
-                                           const int* __restrict__ jb_body, const double* __restrict__ jr,
-                                           const double* __restrict__ rlam, const double* __restrict__ Lm,
-                                           const double* __restrict__ Dinv, const int* __restrict__ jbody,
-                                           const int* __restrict__ jbody_blk, const double* __restrict__ minv,
-                                           const double* __restrict__ tr, double* __restrict__ z,
-                                           double* __restrict__ lam, double* __restrict__ x, double tol_rel,
-                                           double tol_abs) {
-    double e2 = 0;
-    {
-        // forward: z_j = s_j - L_j z_{j-1}, then y_j = D_j^-1 z_j (stored in z)
-        double zp[kMaxJRows], zc[kMaxJRows];
-        int mp = 0;
-        for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j) {
-            const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
-            const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
-            for (int i = 0; i < m; ++i) {
-                double s = 0;
-                for (int side = 0; side < 2; ++side) {
-                    const int b = side ? bb : ba;
-                    if (b < 0) continue;
-                    const double* w = jrow(jr, r0 + i, side);
-                    const double* t = tr + (size_t)b * 6;
-                    for (int k = 0; k < 6; ++k) s += w[k] * (minv_k(minv, b, k) * t[k]);
-                }
-                zc[i] = setup ? s - rlam[r0 + i] : s;
+                                           const int* __restrict__ jb_body, const int* __restrict__ jr_blk,
+                                           const double* __restrict__ jr, const double* __restrict__ rlam,
+                                           const double* __restrict__ Lm, const double* __restrict__ Dinv,
+                                           const int* __restrict__ jbody, const int* __restrict__ jbody_blk,
+                                           const double* __restrict__ minv, const double* __restrict__ tr,
+                                           double* __restrict__ z, double* __restrict__ yb, double* __restrict__ lam,
+                                           double* __restrict__ x, double tol_rel, double tol_abs, double* __restrict__ lds,
+                                           int capL, int capR) {
+    const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
+    const int rb = jb_off[j0], re = jb_off[j1];
+    double* sL = lds;         // staged L entries, packed block by block (m_j x m_{j-1})
+    double* sR = lds + capL;  // staged row values
+    for (int r = rb + lane; r < re; r += 64) {
+        const int j = jr_blk[r];
+        const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
+        double s = 0;
+        for (int side = 0; side < 2; ++side) {
+            const int b = side ? bb : ba;
+            if (b < 0) continue;
+            const double* w = jrow(jr, r, side);
+            const double* t = tr + (size_t)b * 6;
+            for (int k = 0; k < 6; ++k) s += w[k] * (minv_k(minv, b, k) * t[k]);
+        }
+        z[r] = setup ? s - rlam[r] : s;
+    }
+    wave_sync();
+    // forward along the chain, a staged window at a time
+    double zp = 0.0;
+    int mp = 0;
+    for (int j = j0; j < j1;) {
+        const int je = stage_fwd_end(j, j1, j0, jb_off, capL, capR);
+        const int rs = jb_off[j];
+        {
+            int off = 0;
+            for (int jj = j; jj < je; ++jj) {
+                const int ml = jj > j0 ? (jb_off[jj + 1] - jb_off[jj]) * (jb_off[jj] - jb_off[jj - 1]) : 0;
+                for (int e = lane; e < ml; e += 64) sL[off + e] = Lm[(size_t)jj * kJBlk + e];
+                off += ml;
             }
-            if (j > jc_off[ch]) {
-                const double* L = Lm + (size_t)j * kMaxJRows * kMaxJRows;
-                for (int i = 0; i < m; ++i) {
-                    double t = 0;
-                    for (int l = 0; l < mp; ++l) t += L[i * mp + l] * zp[l];
-                    zc[i] -= t;
-                }
+            for (int r = rs + lane; r < jb_off[je]; r += 64) sR[r - rs] = z[r];
+        }
+        wave_sync();
+        int off = 0;
+        for (int jj = j; jj < je; ++jj) {
+            const int r0 = jb_off[jj], m = jb_off[jj + 1] - r0;
+            const bool act = lane < m;
+            double t = 0;
+            if (jj > j0) {
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l)
+                    if (l < mp) t += (act ? sL[off + lane * mp + l] : 0.0) * readlane_d(zp, l);
+                off += m * mp;
             }
-            for (int i = 0; i < m; ++i) zp[i] = zc[i];
-            const double* Di = Dinv + (size_t)j * kMaxJRows * kMaxJRows;
-            for (int i = 0; i < m; ++i) {
-                double t = 0;
-                for (int l = 0; l < m; ++l) t += Di[i * m + l] * zc[l];
-                z[r0 + i] = t;
-            }
+            const double zc = (act ? sR[r0 - rs + lane] : 0.0) - t;
+            if (act) z[r0 + lane] = zc;
+            zp = zc;
             mp = m;
         }
-        // backward: Lambda_j = y_j - L_{j+1}^T Lambda_{j+1} (kept in z)
-        for (int j = jc_off[ch + 1] - 2; j >= jc_off[ch]; --j) {
-            const int r0 = jb_off[j], m = jb_off[j + 1] - r0;
-            const int n0 = jb_off[j + 1], mn = jb_off[j + 2] - n0;
-            const double* L = Lm + (size_t)(j + 1) * kMaxJRows * kMaxJRows;  // mn x m
-            for (int i = 0; i < m; ++i) {
+        wave_sync();
+        j = je;
+    }
+    // y_j = D_j^-1 z_j, every row in parallel
+    for (int r = rb + lane; r < re; r += 64) {
+        const int j = jr_blk[r];
+        const int r0 = jb_off[j], m = jb_off[j + 1] - r0, i = r - r0;
+        const double* Di = Dinv + (size_t)j * kJBlk + i * m;
+        double t = 0;
+        for (int l = 0; l < m; ++l) t += Di[l] * z[r0 + l];
+        yb[r] = t;
+    }
+    wave_sync();
+    // backward along the chain: the last block keeps y; windows of blocks [je, j) going down
+    {
+        const int rl = jb_off[j1 - 1], ml = re - rl;
+        double lnext = lane < ml ? yb[rl + lane] : 0.0;
+        if (lane < ml) z[rl + lane] = lnext;
+        int mnx = ml;
+        for (int j = j1 - 1; j > j0;) {
+            // blocks jj = j-1 down to je need L_{jj+1} and y_jj
+            const int je = stage_bwd_end(j, j0, jb_off, capL, capR);
+            const int rs = jb_off[je];
+            {
+                int off = 0;
+                for (int jj = j - 1; jj >= je; --jj) {
+                    const int n = jj + 1;
+                    const int mlj = (jb_off[n + 1] - jb_off[n]) * (jb_off[n] - jb_off[n - 1]);
+                    for (int e = lane; e < mlj; e += 64) sL[off + e] = Lm[(size_t)n * kJBlk + e];
+                    off += mlj;
+                }
+                for (int r = rs + lane; r < jb_off[j]; r += 64) sR[r - rs] = yb[r];
+            }
+            wave_sync();
+            int off = 0;
+            for (int jj = j - 1; jj >= je; --jj) {
+                const int r0 = jb_off[jj], m = jb_off[jj + 1] - r0;
+                const bool act = lane < m;
                 double t = 0;
-                for (int l = 0; l < mn; ++l) t += L[l * m + i] * z[n0 + l];
-                z[r0 + i] -= t;
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l)
+                    if (l < mnx) t += (act ? sL[off + l * m + lane] : 0.0) * readlane_d(lnext, l);
+                off += mnx * m;
+                const double lj = (act ? sR[r0 - rs + lane] : 0.0) - t;
+                if (act) z[r0 + lane] = lj;
+                lnext = lj;
+                mnx = m;
             }
+            wave_sync();
+            j = je;
         }
-        for (int j = jc_off[ch]; j < jc_off[ch + 1]; ++j)
-            for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) lam[r] = setup ? z[r] : lam[r] + z[r];
-        // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda)
-        for (int q = jcb_off[ch]; q < jcb_off[ch + 1]; ++q) {
-            const int b = jbody[q];
-            double acc[6];
-            for (int k = 0; k < 6; ++k) acc[k] = tr[(size_t)b * 6 + k];
-            for (int e = 0; e < 2; ++e) {
-                const int j = jbody_blk[4 * (size_t)q + 2 * e], side = jbody_blk[4 * (size_t)q + 2 * e + 1];
-                if (j < 0) continue;
-                for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) {
-                    const double* w = jrow(jr, r, side);
-                    for (int k = 0; k < 6; ++k) acc[k] -= w[k] * z[r];
-                }
-            }
-            double* xb = x + (size_t)b * kBodyStride;
-            for (int k = 0; k < 6; ++k) {
-                const double dv = minv_k(minv, b, k) * acc[k];
-                const double u = xb[k];
-                const double un = setup ? dv : u + dv;
-                xb[k] = un;
-                if (check) {
-                    const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
-                    const double ee = (un - u) / qd;
-                    e2 += ee * ee;
-                }
-            }
+    }
+    for (int r = rb + lane; r < re; r += 64) lam[r] = setup ? z[r] : lam[r] + z[r];
+    // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda), one (body, dof) per lane
+    double e2 = 0;
+    const int qb = jcb_off[ch], nq = jcb_off[ch + 1] - qb;
+    for (int e = lane; e < 6 * nq; e += 64) {
+        const int q = qb + e / 6, k = e - (e / 6) * 6;
+        const int b = jbody[q];
+        double acc = tr[(size_t)b * 6 + k];
+        for (int ee = 0; ee < 2; ++ee) {
+            const int j = jbody_blk[4 * (size_t)q + 2 * ee], side = jbody_blk[4 * (size_t)q + 2 * ee + 1];
+            if (j < 0) continue;
+            for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) acc -= jrow(jr, r, side)[k] * z[r];
+        }
+        double* xb = x + (size_t)b * kBodyStride;
+        const double dv = minv_k(minv, b, k) * acc;
+        const double u = xb[k];
+        const double un = setup ? dv : u + dv;
+        xb[k] = un;
+        if (check) {
+            const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+            const double ee2 = (un - u) / qd;
+            e2 += ee2 * ee2;
         }
     }
     return e2;

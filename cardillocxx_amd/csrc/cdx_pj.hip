@@ -956,7 +956,7 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     const int gb8 = grid_for(nb * kLanesPerBody);
     const int gc4 = grid_for(nc * kLanesPerContact);
     (void)gc;
-    const int gj = ctx->njrows > 0 ? grid_for(ctx->nchains) : 0;  // partials of the chain solve follow the bodies'
+    const int gj = ctx->njrows > 0 ? ctx->nchains : 0;  // one partial per chain follows the bodies'
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)(gb8 + gj), &ctx->partial_cap, "residual partials"));
     (void)gb;
     if (ctx->record) {

@@ -247,3 +247,34 @@ def test_batched_chains_free_and_errors(ctx):
     assert (it == 0).all()
     st, v_ref, _, lam_ref, r = oracle.pj_solve_joints(prob, rows, 1e-3, oracle.params(), theta=0.7, lam_prev=lam_prev)
     assert st == 0 and rel(ctx.velocities(), v_ref) <= TOL and rel(ctx.joint_impulses(), lam_ref) <= TOL
+
+
+def test_long_chains_staged_in_windows(ctx):
+    """60-link chains (59 joint blocks): the chain solve stages L and the row values through
+    several LDS windows (grid-wide solve and batched kernel); both match the oracle."""
+    t = scenes.fractal_heightfield(n=65, extent=8.0, amp=0.05, seed=8)
+    batch, jt = scenes.capsule_chains(nchains=2, links=60, terrain=t, seed=8, drop=0.0005, local=True)
+    rng = np.random.default_rng(8)
+    v = 0.3 * rng.normal(size=(len(batch.q), 6))
+    bd = scenes.shape_bodies(batch.shape, batch.dims, batch.mass, batch.q, v.reshape(-1))
+    con = batch.contacts()
+    prob = scenes.make_problem(bd, con)
+    rows = joints.spherical_rows(batch.q, jt, joints.joint_frames(batch.q, jt), 0.05)
+    assert prob.nc > 0
+    kw = dict(max_iter=25, tol_abs=0.0, tol_rel=0.0)
+    (vg, imp, lam, info), (v_ref, imp_ref, lam_ref, res) = run_both(ctx, prob, rows, 1e-3, **kw)
+    assert info.iters == res.iters == 25
+    assert rel(vg, v_ref) <= TOL and rel(imp, imp_ref) <= TOL and rel(lam, lam_ref) <= TOL
+    # the batched solve starts again from the uploaded warm start (a solve leaves its impulses)
+    ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+    ctx.set_contacts(*prob.contact_arrays())
+    ctx.set_joints(rows)
+    ctx.set_scenes(batch.body_off, con["contact_off"])
+    ctx.batch_set_launch(256, 2)
+    it, _ = ctx.batch_pj_solve(1e-3, 1.0, params(**kw))
+    for s in range(batch.nscenes):
+        sub, sr, lp, sel = scene_slice(prob, rows, batch.body_off, con["contact_off"], s)
+        st, v1, i1, l1, r1 = oracle.pj_solve_joints(sub, sr, 1e-3, oracle.params(**kw))
+        b0, b1 = batch.body_off[s], batch.body_off[s + 1]
+        assert it[s] == r1.iters
+        assert rel(ctx.velocities()[6 * b0:6 * b1], v1) <= TOL and rel(ctx.joint_impulses()[sel], l1) <= TOL
