@@ -656,6 +656,70 @@ def diff_leg(args, native, scenes):
     }
 
 
+def chains_diff_leg(args, native, scenes):
+    """Forward + checkpointed backward of the jointed capsule chains: gradient of the COM loss
+    w.r.t. every joint's rotational damping (and v0) through the implicit joint adjoint."""
+    from cardillocxx_amd import diff
+
+    terrain = scenes.fractal_heightfield(n=256, amp=0.1, seed=0)
+    batch, jt = scenes.capsule_chains(nchains=args.chain_diff_scenes, links=32, terrain=terrain, seed=1, drop=0.0005,
+                                      local=True)
+    rng = np.random.default_rng(3)
+    batch.v[:] = rng.normal(scale=0.2, size=batch.v.shape)
+    ns, steps = batch.nscenes, args.chain_diff_steps
+    com0 = np.stack([batch.q[batch.body_off[s]:batch.body_off[s + 1], :3].mean(0) for s in range(ns)])
+    target = com0 + rng.normal(scale=0.05, size=(ns, 3))
+    D0 = np.full(len(jt["body_a"]), args.chain_damping)
+    kw = dict(steps=steps, dt=1e-3, checkpoint_every=10, tol=args.tol, adjoint_tol=args.adjoint_tol,
+              max_iter=args.max_iter, adjoint_max_iter=args.chain_adjoint_max_iter)
+
+    def run(b, spec, D, tgt, batched):
+        r = diff.ChainRollout(b, spec, batched=batched, **kw)
+        t0 = time.perf_counter()
+        pos = r.forward(b.v.reshape(-1).copy(), D)
+        t1 = time.perf_counter()
+        loss, gpos, _ = diff.com_loss(pos, b.mass, b.body_off, tgt)
+        g = r.backward(gpos)
+        t2 = time.perf_counter()
+        return r, loss, g, t1 - t0, t2 - t1
+
+    r, loss, g, tf, tb = run(batch, jt, D0, target, True)
+    log(f"chains differentiable: {ns} chains, forward {tf:.2f}s, backward {tb:.2f}s, {r.stats['forward_solves']} solves, "
+        f"{r.stats['adjoint_iters']} adjoint iterations")
+    pick = rng.choice(ns, size=min(args.chain_diff_obo, ns), replace=False)
+    t_one, worst = 0.0, 0.0
+    nj_per = 31
+    for k in pick:
+        k = int(k)
+        b0 = int(batch.body_off[k])
+        js = slice(k * nj_per, (k + 1) * nj_per)
+        spec = {key: (np.asarray(x)[js] - (b0 if key in ("body_a", "body_b") else 0)) for key, x in jt.items()}
+        _, _, g1, tf1, tb1 = run(batch.scene(k), spec, D0[js], target[k][None, :], False)
+        t_one += tf1 + tb1
+        for a, c in ((g1.damping, g.damping[js]), (g1.v0, g.v0[b0:int(batch.body_off[k + 1])])):
+            worst = max(worst, float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-30)))
+    t_obo = t_one * ns / len(pick)
+    log(f"chains differentiable one by one: {len(pick)} chains in {t_one:.2f}s")
+    return {
+        "metric": "fwd+bwd jointed env-steps/s (chains x steps through forward and checkpointed backward per second)",
+        "workload": f"{ns} chains of 32 capsules (r=0.02 m, length 0.1 m), damped spherical joints (D_rot="
+                    f"{args.chain_damping:g}), dropped onto a 256x256 fractal heightfield (amp 0.1 m), v0~N(0,0.2^2), "
+                    f"{steps} steps at h=1e-3 s, loss sum_s ||COM_s(final) - target_s||^2, grads wrt every joint's damping "
+                    f"and v0; implicit FP64 adjoint through joints and contacts, checkpoint every 10 steps, PJ tol "
+                    f"{args.tol:g} per chain, adjoint tol {args.adjoint_tol:g}",
+        "value": ns * steps / (tf + tb), "unit": "env-steps/s",
+        "forward_s": tf, "backward_s": tb, "loss": loss,
+        "forward_solves": r.stats["forward_solves"], "adjoint_applications": r.stats["adjoint_iters"],
+        "least_squares_adjoint_steps": r.stats.get("inexact_adjoints", 0),
+        "worst_adjoint_relative_residual": r.stats.get("worst_adjoint_residual", 0.0),
+        "grad_damping_norm": float(np.linalg.norm(g.damping)),
+        "one_by_one": {"sampled_chains": [int(k) for k in pick], "estimated_s": t_obo, "speedup": t_obo / (tf + tb),
+                       "max_rel_gradient_difference": worst,
+                       "method": "sampled chains rolled out alone (forward + backward through the grid-wide joint solve "
+                                 "and adjoint), time scaled to all chains"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -684,6 +748,10 @@ def main():
     ap.add_argument("--chain-steps", type=int, default=150)
     ap.add_argument("--chain-drop", type=float, default=0.03, help="drop height above the terrain under each chain")
     ap.add_argument("--chain-damping", type=float, default=0.05, help="rotational joint damping D_rot")
+    ap.add_argument("--chain-diff-scenes", type=int, default=2048)
+    ap.add_argument("--chain-diff-steps", type=int, default=40)
+    ap.add_argument("--chain-diff-obo", type=int, default=4, help="chains rolled out one by one")
+    ap.add_argument("--chain-adjoint-max-iter", type=int, default=20_000)
     ap.add_argument("--diff-scenes", type=int, default=256)
     ap.add_argument("--diff-steps", type=int, default=100)
     ap.add_argument("--diff-obo", type=int, default=4, help="scenes rolled out one by one")
@@ -712,6 +780,7 @@ def main():
         out["differentiable"] = diff_leg(args, native, scenes)
         if not args.no_chains:
             out["jointed_chains"] = chains_leg(args, native, scenes)
+            out["jointed_chains_differentiable"] = chains_diff_leg(args, native, scenes)
     print(json.dumps(out), flush=True)
 
 

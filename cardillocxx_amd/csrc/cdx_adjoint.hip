@@ -493,6 +493,135 @@ __global__ void k_adj_final_bodies(int nb, const int* __restrict__ dof, const do
     store6(gf + (size_t)bi * kBodyStride, gfo);
 }
 
+// Convergence of the body-space adjoint t = S^-1_vv W^T R xi between two iterations: per-block sums
+// of |t - t_prev|^2 and |t|^2.  Where contacts are statically indeterminate (A = W S^-1 W^T
+// singular on the stick rows) lambda may drift along directions with W^T R v = 0, which leave t,
+// and with it every velocity-space gradient, unchanged.
+__global__ void __launch_bounds__(kBlock) k_adj_tcheck(int nb, const double* __restrict__ t, const double* __restrict__ tp,
+                                                       double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int bi = blockIdx.x * kBlock + threadIdx.x;
+    double d2 = 0, n2 = 0;
+    if (bi < nb) {
+        double a[6], b[6];
+        load_body(t, bi, a);
+        load_body(tp, bi, b);
+        for (int k = 0; k < 6; ++k) {
+            d2 += (a[k] - b[k]) * (a[k] - b[k]);
+            n2 += a[k] * a[k];
+        }
+    }
+    const double sd = block_sum(d2, red);
+    const double sn = block_sum(n2, red);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sd;
+        partial[2 * blockIdx.x + 1] = sn;
+    }
+}
+
+// ---- GMRES on the implicit adjoint system B lambda = g_p, B = I - (I - A R) P^T, A = W S^-1_vv W^T
+// (used with joint rows, where the fixed-point iteration of the contact adjoint can be too slow).
+// Vectors live on the contact rows, component-major [k * nc + c] (rows a contact does not have
+// stay zero).  One application of B: xi = P^T v and the side impulses of W^T (R xi) (k_gm_pt),
+// t = S^-1_vv W^T R xi (body pass + chain solve), B v = v - xi + W t (k_gm_out).
+constexpr int kGmRestart = 30;
+
+__global__ void k_gm_pt(int nc, const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
+                        const double* __restrict__ R, const double* __restrict__ mu, const double* __restrict__ S,
+                        const double* __restrict__ v, double* __restrict__ xi_out, double* __restrict__ imp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) {
+        for (int k = 0; k < 3; ++k) xi_out[(size_t)k * nc + c] = 0.0;
+        return;
+    }
+    const int nr = mu[c] > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double P[9], vv[3] = {0, 0, 0}, xi[3], s[3] = {0, 0, 0};
+    for (int k = 0; k < 9; ++k) P[k] = S[(size_t)(A_P + k) * nc + c];
+    for (int k = 0; k < nr; ++k) vv[k] = v[(size_t)k * nc + c];
+    mat_t_vec(P, vv, xi);
+    for (int k = 0; k < 3; ++k) {
+        xi_out[(size_t)k * nc + c] = k < nr ? xi[k] : 0.0;
+        s[k] = k < nr ? R[(size_t)k * nc + c] * xi[k] : 0.0;
+    }
+    write_side_impulses(imp, c, a, b, nr, j, s);
+}
+
+__global__ void k_gm_out(int nc, const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
+                         const double* __restrict__ mu, const double* __restrict__ t, const double* __restrict__ v,
+                         const double* __restrict__ xi, double* __restrict__ w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    const int nr = (a < 0 && b < 0) ? 0 : (mu[c] > 0 ? 3 : 1);
+    ContactJ j;
+    if (nr) load_J(J, nc, c, nr, j);
+    double ta[6], tb[6];
+    load_body(t, a, ta);
+    load_body(t, b, tb);
+    for (int k = 0; k < 3; ++k) {
+        const size_t r = (size_t)k * nc + c;
+        w[r] = k < nr ? v[r] - xi[r] + row_dot(j.d[k], j.aA[k], j.aB[k], ta, tb) : 0.0;
+    }
+}
+
+// partial[i * gridDim.x + block] = this block's part of <w, V_i>, i < nvec
+__global__ void __launch_bounds__(kBlock) k_gm_dots(int n, int nvec, const double* __restrict__ V, size_t ld,
+                                                    const double* __restrict__ w, double* __restrict__ partial) {
+    __shared__ double red[kBlock / 64];
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    const double wr = r < n ? w[r] : 0.0;
+    for (int i = 0; i < nvec; ++i) {
+        const double d = block_sum(r < n ? wr * V[i * ld + r] : 0.0, red);
+        if (threadIdx.x == 0) partial[(size_t)i * gridDim.x + blockIdx.x] = d;
+    }
+}
+
+// h[i] = (h[i] +) sum of the partials of dot i; one workgroup
+__global__ void __launch_bounds__(kBlock) k_gm_reduce(int nvec, int nblk, const double* __restrict__ partial,
+                                                      double* __restrict__ h, int accumulate) {
+    __shared__ double red[kBlock / 64];
+    for (int i = 0; i < nvec; ++i) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[(size_t)i * nblk + b];
+        s = block_sum(s, red);
+        if (threadIdx.x == 0) h[i] = accumulate ? h[i] + s : s;
+    }
+}
+
+// w += sign * sum_i h_i V_i
+__global__ void k_gm_axpy(int n, int nvec, const double* __restrict__ V, size_t ld, const double* __restrict__ h,
+                          double sign, double* __restrict__ w) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double acc = 0;
+    for (int i = 0; i < nvec; ++i) acc += h[i] * V[i * ld + r];
+    w[r] += sign * acc;
+}
+
+// out = w / sqrt(*nrm2) (zero after a breakdown); *beta = sqrt(*nrm2)
+__global__ void k_gm_normalize(int n, const double* __restrict__ w, const double* __restrict__ nrm2, double* __restrict__ beta,
+                               double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const double b = sqrt(fmax(*nrm2, 0.0));
+    if (r == 0) *beta = b;
+    if (r < n) out[r] = b > 1e-300 ? w[r] / b : 0.0;
+}
+
+// out[i] += a[i], i < n (one workgroup)
+__global__ void k_gm_add(int n, const double* __restrict__ a, double* __restrict__ out) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] += a[i];
+}
+
+__global__ void k_gm_sub(int n, const double* __restrict__ g, const double* __restrict__ w, double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = g[r] - w[r];
+}
+
 // Jointed bodies after the chain solve r = S^-1 [x_bar; 0] (velocity part rv): dL/dv_n += M r_v
 // (r_v = M v + h f_ext + h f_gyro(v) + ...), the gyroscopic Jacobian, and the (1 - theta) G v term
 // of the joint rows' right-hand side; dL/df_ext = h r_v.
@@ -562,11 +691,12 @@ __global__ void k_adj_comp(int n, const int* __restrict__ kind, const double* __
 using namespace cdx;
 
 static int adj_alloc(cdx_ctx* ctx) {
-    const size_t nc = (size_t)ctx->nc, nb = (size_t)ctx->nb;
+    const size_t nc = (size_t)std::max(ctx->nc, 1), nb = (size_t)ctx->nb;  // contact scratch is cleared even without contacts
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj, (size_t)A_COUNT * nc, &ctx->adj_cap, "adjoint contact scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_imp, 48 * nc, &ctx->adj_imp_cap, "adjoint side vectors"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_body, 8 * kBodyStride * nb, &ctx->adj_body_cap, "adjoint body scratch"));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_partial, 2 * (size_t)grid_for((int)nc) + 2, &ctx->adj_partial_cap, "adjoint partials"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_adj_partial, 2 * (size_t)std::max(grid_for((int)nc), grid_for((int)nb)) + 2,
+                         &ctx->adj_partial_cap, "adjoint partials"));
     return CDX_OK;
 }
 
@@ -742,25 +872,125 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
         CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx, I0, t,
                    nullptr, ctx->d_jslot, ctx->d_jtr);
         CDX_TRY(joints_apply(ctx, t, jl, nullptr));
-        auto iter = [&](int it) -> int {
-            const int check = (it % 10) == 9;
-            CDX_LAUNCH(ctx, "adj_iter", k_adj_iter, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, t, S,
-                       I0, check, ctx->d_adj_partial, ctx->d_ctrl);
+        // restarted GMRES on B lambda = g_p (lambda_0 = g_p from the setup), classical Gram-Schmidt
+        // with one reorthogonalisation; the small least-squares problem is solved on the host
+        const int n = 3 * nc, m = kGmRestart;
+        const int gn = grid_for(n);
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_gm, (size_t)(m + 3) * n + (size_t)(m + 1) * (m + 1) + 8, &ctx->gm_cap,
+                             "adjoint GMRES vectors"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_gm_part, (size_t)(m + 1) * gn, &ctx->gm_part_cap, "adjoint GMRES partials"));
+        double* V = ctx->d_gm;                        // m + 1 basis vectors
+        double* Wk = V + (size_t)(m + 1) * n;         // work vector
+        double* Xi = Wk + n;                          // P^T v of the last application
+        double* H = Xi + n;                           // (m + 1) x m, column-major, then scalars
+        double* sc = H + (size_t)(m + 1) * m;         // [0] |.|^2, [1] beta, [2..] second-pass dots
+        double* lamv = S + (size_t)A_LAM * nc;
+        const double* gvec = S + (size_t)A_GP * nc;
+        double* part = ctx->d_gm_part;
+        auto apply = [&](const double* v, double* w) -> int {
+            CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, v, Xi,
+                       I0);
             CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                       I0, t, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
-            CDX_TRY(joints_apply(ctx, t, jl, ctx->d_ctrl));
-            if (check)
-                CDX_LAUNCH(ctx, "adj_residual", k_adj_finalize, 1, kBlock, gc, ctx->d_adj_partial, it, aprm->tol, ctx->d_ctrl,
-                           ctx->d_res, ctx->h_flag_dev);
-            return hip_check(ctx, hipGetLastError(), "adjoint iteration launch");
+                       I0, t, nullptr, ctx->d_jslot, ctx->d_jtr);
+            CDX_TRY(joints_apply(ctx, t, jl, nullptr));
+            CDX_LAUNCH(ctx, "adj_iter", k_gm_out, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, t, v, Xi, w);
+            return hip_check(ctx, hipGetLastError(), "adjoint GMRES apply");
         };
-        int it_done = 0;
-        CDX_TRY(run_sweeps(ctx, aprm->max_iter, 10, 4, iter, &it_done));
-        int ctrl[3];
-        CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
-        CDX_TRY(read_residual(ctx, res_out));
-        *iters_out = it_done;
-        if (!ctrl[0]) return set_err(ctx, CDX_DIVERGED, "cdx_pj_adjoint_joints: implicit adjoint iteration did not converge");
+        auto dots = [&](int nvec, const double* Vb, const double* w, double* out, int accumulate) -> int {
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_dots, gn, kBlock, n, nvec, Vb, (size_t)n, w, part);
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_reduce, 1, kBlock, nvec, gn, part, out, accumulate);
+            return hip_check(ctx, hipGetLastError(), "adjoint GMRES dots");
+        };
+        auto host_scalar = [&](const double* d, double* h) -> int {
+            CDX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            return CDX_OK;
+        };
+        double gnorm2 = 0;
+        CDX_TRY(dots(1, gvec, gvec, sc, 0));
+        CDX_TRY(host_scalar(sc, &gnorm2));
+        const double gnorm = std::sqrt(gnorm2);
+        int applies = 0, stalls = 0;
+        bool converged = gnorm == 0.0, stalled = false;
+        double rel = 0.0, rel_prev = INFINITY;
+        std::vector<double> Hh((size_t)(m + 1) * m), y(m);
+        while (!converged && !stalled && applies < aprm->max_iter) {
+            CDX_TRY(apply(lamv, Wk));
+            ++applies;
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_sub, gn, kBlock, n, gvec, Wk, Wk);
+            CDX_TRY(dots(1, Wk, Wk, sc, 0));
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, sc + 1, V);
+            double beta = 0;
+            CDX_TRY(host_scalar(sc + 1, &beta));
+            rel = beta / gnorm;
+            if (rel <= aprm->tol) {
+                converged = true;
+                break;
+            }
+            // three restarts in a row without a 1% gain: the system is (nearly) inconsistent -- contacts
+            // on a stick/slip or contact/separation boundary, where the projection is not differentiable
+            stalls = rel > 0.99 * rel_prev ? stalls + 1 : 0;
+            rel_prev = rel;
+            if (stalls >= 3) {
+                stalled = true;
+                break;
+            }
+            CDX_HIP(ctx, hipMemsetAsync(H, 0, sizeof(double) * (size_t)(m + 1) * m, ctx->stream));
+            int k = 0;
+            for (; k < m && applies < aprm->max_iter; ++k) {
+                double* vk = V + (size_t)k * n;
+                CDX_TRY(apply(vk, Wk));
+                ++applies;
+                double* hk = H + (size_t)k * (m + 1);
+                CDX_TRY(dots(k + 1, V, Wk, hk, 0));
+                CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, hk, -1.0, Wk);
+                CDX_TRY(dots(k + 1, V, Wk, sc + 2, 0));
+                CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, sc + 2, -1.0, Wk);
+                CDX_LAUNCH(ctx, "adj_residual", k_gm_add, 1, kBlock, k + 1, sc + 2, hk);
+                CDX_TRY(dots(1, Wk, Wk, sc, 0));
+                CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, hk + k + 1, V + (size_t)(k + 1) * n);
+            }
+            // least squares min |beta e1 - H y| by Givens rotations
+            CDX_HIP(ctx, hipMemcpyAsync(Hh.data(), H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            std::vector<double> gq(k + 1, 0.0), cs(k), sn(k);
+            gq[0] = beta;
+            for (int c = 0; c < k; ++c) {
+                double* col = &Hh[(size_t)c * (m + 1)];
+                for (int i = 0; i < c; ++i) {
+                    const double a0 = col[i], a1 = col[i + 1];
+                    col[i] = cs[i] * a0 + sn[i] * a1;
+                    col[i + 1] = -sn[i] * a0 + cs[i] * a1;
+                }
+                const double r = std::hypot(col[c], col[c + 1]);
+                cs[c] = r > 0 ? col[c] / r : 1.0;
+                sn[c] = r > 0 ? col[c + 1] / r : 0.0;
+                col[c] = r;
+                col[c + 1] = 0.0;
+                const double g0 = gq[c];
+                gq[c] = cs[c] * g0;
+                gq[c + 1] = -sn[c] * g0;
+            }
+            for (int i = k - 1; i >= 0; --i) {
+                double s = gq[i];
+                for (int l = i + 1; l < k; ++l) s -= Hh[(size_t)l * (m + 1) + i] * y[l];
+                const double d = Hh[(size_t)i * (m + 1) + i];
+                y[i] = d != 0.0 ? s / d : 0.0;
+            }
+            CDX_HIP(ctx, hipMemcpyAsync(sc + 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k, V, (size_t)n, sc + 2, 1.0, lamv);
+            rel = std::fabs(gq[k]) / gnorm;
+            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        *iters_out = applies;
+        *res_out = rel;
+        if (!converged && aprm->mode != CDX_ADJ_AUTO)
+            return set_err(ctx, CDX_DIVERGED,
+                           "cdx_pj_adjoint_joints: adjoint GMRES did not converge (relative residual " + std::to_string(rel) +
+                               " after " + std::to_string(applies) + " applications)");
+        // xi = P^T lambda for the final stage
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, lamv,
+                   S + (size_t)A_XI * nc, I0);
         CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, 1, prm->alpha, ctx->d_ida, ctx->d_idb, ctx->d_J,
                    ctx->d_R, ctx->d_mu, ctx->d_rest, ctx->d_p, S, I0, I1, I2, I3, dgmu);
     } else {

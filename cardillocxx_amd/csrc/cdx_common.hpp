@@ -135,6 +135,10 @@ struct cdx_ctx {
     double* d_adj_body = nullptr;
     size_t adj_body_cap = 0;
     double* d_adj_jl = nullptr;  // joint adjoint: [lambda scratch | r_bar_Lambda | dL/dcomp] per joint row
+    double* d_gm = nullptr;       // joint adjoint GMRES: basis, work vectors, Hessenberg, scalars
+    size_t gm_cap = 0;
+    double* d_gm_part = nullptr;  // GMRES dot-product partials
+    size_t gm_part_cap = 0;
     size_t adj_jl_cap = 0;
     double* d_adj_partial = nullptr;
     size_t adj_partial_cap = 0;

@@ -307,8 +307,12 @@ class ChainRollout(Rollout):
                 self._solve_joints(rec["q"], rec["v"], rec["fgyro"], rec["con"], rec["imp"], rec["rows"], rec["lam_prev"],
                                    prm1)
                 g = self.ctx.pj_adjoint_joints(self.dt, self.prm, a + self.dt * gx, max_iter=self.adjoint_max_iter,
-                                               tol=self.adjoint_tol)
+                                               tol=self.adjoint_tol, mode=native.CDX_ADJ_AUTO)
                 self.stats["adjoint_iters"] += g["info"].iters
+                if g["info"].residual > self.adjoint_tol:  # least-squares adjoint of an inconsistent system
+                    self.stats["inexact_adjoints"] = self.stats.get("inexact_adjoints", 0) + 1
+                    self.stats["worst_adjoint_residual"] = max(self.stats.get("worst_adjoint_residual", 0.0),
+                                                               g["info"].residual)
                 a = g["v"]
                 rows = rec["rows"]
                 damper = np.arange(rows["ns"] + rows["nd"]) >= rows["ns"]

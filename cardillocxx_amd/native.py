@@ -334,10 +334,11 @@ class Context:
                                            dptr(of), dptr(omu), C.byref(info)), "cdx_pj_adjoint")
         return dict(v=ov, minv=om, fext=of, mu=omu[:self.nc], info=info)
 
-    def pj_adjoint_joints(self, dt, params, g_v, g_p=None, max_iter=100000, tol=1e-12):
+    def pj_adjoint_joints(self, dt, params, g_v, g_p=None, max_iter=100000, tol=1e-12, mode=CDX_ADJ_IMPLICIT):
         """dL/dv_n, dL/df_ext (stacked, nv), dL/dmu per contact and dL/dcomp per joint row (caller's
-        row order) of the last joint solve, by the implicit adjoint."""
-        a = AdjointParams(CDX_ADJ_IMPLICIT, max_iter, tol)
+        row order) of the last joint solve, by the implicit adjoint (mode CDX_ADJ_AUTO accepts the
+        least-squares adjoint where the system is inconsistent; info.residual tells)."""
+        a = AdjointParams(mode, max_iter, tol)
         info = AdjointInfo()
         gv = np.ascontiguousarray(g_v, dtype=np.float64)
         gp = None if g_p is None else np.ascontiguousarray(g_p, dtype=np.float64).reshape(-1)

@@ -227,8 +227,13 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
  * Outputs (each may be NULL): gv_out = dL/dv_n (nv), gfext_out = dL/df_ext (nv), gmu_out = dL/dmu
  * per contact (nc), gcomp_out = dL/dcomp per joint row in the order given to cdx_set_joints (the
  * compliance C of a spring row, A of a damper row; a damper of damping D has A = 1/D, so
- * dL/dD = -dL/dA / D^2).  Mass gradients are not offered with joint rows.  No counterpart in the
- * reference, which has no differentiable solve. */
+ * dL/dD = -dL/dA / D^2).  Mass gradients are not offered with joint rows.  The adjoint system
+ * (I - (I - A R) P^T) lambda = g_p is solved by restarted GMRES to a relative residual of
+ * aprm->tol within aprm->max_iter applications (each one transposed sweep with the chain solve).
+ * Where contacts sit on a stick/slip or contact/separation boundary the system can be
+ * inconsistent and GMRES stalls: CDX_ADJ_IMPLICIT then fails with CDX_DIVERGED, CDX_ADJ_AUTO
+ * returns the least-squares adjoint (info->residual gives its relative residual).  No
+ * counterpart in the reference, which has no differentiable solve. */
 int cdx_pj_adjoint_joints(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
                           const double* g_v, const double* g_p, double* gv_out, double* gfext_out, double* gmu_out,
                           double* gcomp_out, cdx_adjoint_info* info);
