@@ -40,6 +40,8 @@ struct BatchJoints {
     double* Dinv;
     const int* jbody;
     const int* jbody_blk;
+    const int* jr_info;
+    const int* jbody_info;
     const int* jslot;
     double* tr;
     double* z;
@@ -93,9 +95,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
                                        ja.theta, ja.beta, ja.rlam, ja.Lm, ja.Dinv, jlds[threadIdx.x >> 6]))
                     gfail[g] = 1;
                 else
-                    joint_solve_chain(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body, ja.jr_blk, ja.jr, ja.rlam,
-                                      ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel,
-                                      tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
+                    joint_solve_chain(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jr_info, ja.jr, ja.rlam, ja.Lm,
+                                      ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel, tol_abs,
+                                      jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
     }
     __syncthreads();
@@ -120,9 +122,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             __syncthreads();
             if (!done) {
                 for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv)
-                    e2 += joint_solve_chain(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jb_body,
-                                            ja.jr_blk, ja.jr, ja.rlam, ja.Lm, ja.Dinv, ja.jbody, ja.jbody_blk, minv, ja.tr, ja.z,
-                                            ja.y, ja.lam, x, tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
+                    e2 += joint_solve_chain(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jr_info,
+                                            ja.jr, ja.rlam, ja.Lm, ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x,
+                                            tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
         }
         if (check) {
@@ -202,11 +204,10 @@ static int upload_scene_chains(cdx_ctx* ctx) {
 static BatchJoints batch_joints(cdx_ctx* ctx, double theta) {
     BatchJoints ja{};
     if (ctx->njrows == 0) return ja;
-    ja = BatchJoints{ctx->d_sc_off, ctx->d_sc_list, ctx->d_jc_off,    ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body,
-                     ctx->d_jr_kind, ctx->d_jr_blk, ctx->d_jr,       ctx->d_jrlam,   ctx->d_jL,     ctx->d_jDinv,
-                     ctx->d_jbody,   ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jtr,     ctx->d_jz,     ctx->d_jy,
-                     ctx->d_jlam,
-                     theta,          ctx->joint_bias};
+    ja = BatchJoints{ctx->d_sc_off,  ctx->d_sc_list,   ctx->d_jc_off,  ctx->d_jcb_off,     ctx->d_jb_off, ctx->d_jb_body,
+                     ctx->d_jr_kind, ctx->d_jr_blk,    ctx->d_jr,      ctx->d_jrlam,       ctx->d_jL,     ctx->d_jDinv,
+                     ctx->d_jbody,   ctx->d_jbody_blk, ctx->d_jr_info, ctx->d_jbody_info,  ctx->d_jslot,  ctx->d_jtr,
+                     ctx->d_jz,      ctx->d_jy,        ctx->d_jlam,    theta,              ctx->joint_bias};
     return ja;
 }
 

@@ -77,6 +77,8 @@ struct cdx_ctx {
     size_t sc_off_cap = 0, sc_list_cap = 0;
     int *d_jc_off = nullptr, *d_jcb_off = nullptr, *d_jb_off = nullptr, *d_jb_body = nullptr, *d_jr_kind = nullptr;
     int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr, *d_jr_blk = nullptr;
+    int *d_jr_info = nullptr, *d_jbody_info = nullptr;  // chain-solve records per row / per jointed body
+    size_t jr_info_cap = 0, jbody_info_cap = 0;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
     double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)

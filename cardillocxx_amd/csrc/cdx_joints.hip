@@ -47,18 +47,18 @@ __global__ void __launch_bounds__(64) k_joint_factor(int nchains, const int* __r
 // One 64-lane workgroup per chain (see joint_solve_chain); with `check` one residual partial per chain.
 __global__ void __launch_bounds__(64) k_joint_solve(
     int nchains, int setup, int check, const int* __restrict__ jc_off, const int* __restrict__ jcb_off,
-    const int* __restrict__ jb_off, const int* __restrict__ jb_body, const int* __restrict__ jr_blk,
-    const double* __restrict__ jr, const double* __restrict__ rlam, const double* __restrict__ Lm,
-    const double* __restrict__ Dinv, const int* __restrict__ jbody, const int* __restrict__ jbody_blk,
-    const double* __restrict__ minv, const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ lam,
+    const int* __restrict__ jb_off, const int* __restrict__ jr_info, const double* __restrict__ jr,
+    const double* __restrict__ rlam, const double* __restrict__ Lm, const double* __restrict__ Dinv,
+    const int* __restrict__ jbody_info, const double* __restrict__ minv, const double* __restrict__ tr,
+    double* __restrict__ z, double* __restrict__ lam,
     double* __restrict__ yb, double* __restrict__ x, double tol_rel, double tol_abs, double* __restrict__ partial,
     const int* __restrict__ ctrl) {
     __shared__ double lds[kSolveLdsL + kSolveLdsR];
     if (ctrl && ctrl[0]) return;
     const int ch = blockIdx.x;
     if (ch >= nchains) return;
-    double e2 = joint_solve_chain(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jb_body, jr_blk, jr, rlam, Lm, Dinv,
-                                  jbody, jbody_blk, minv, tr, z, yb, lam, x, tol_rel, tol_abs, lds, kSolveLdsL, kSolveLdsR);
+    double e2 = joint_solve_chain(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jr_info, jr, rlam, Lm, Dinv,
+                                  jbody_info, minv, tr, z, yb, lam, x, tol_rel, tol_abs, lds, kSolveLdsL, kSolveLdsR);
     if (check) {
         for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
         if (threadIdx.x == 0) partial[ch] = e2;
@@ -76,9 +76,9 @@ int joints_factor(cdx_ctx* ctx, double dt, double theta) {
 int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, setup, check, ctx->d_jc_off, ctx->d_jcb_off,
-               ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_blk, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
-               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_jy, ctx->d_x, tol_rel, tol_abs,
-               partial, setup ? nullptr : ctx->d_ctrl);
+               ctx->d_jb_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
+               ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_jy, ctx->d_x, tol_rel, tol_abs, partial,
+               setup ? nullptr : ctx->d_ctrl);
     return hip_check(ctx, hipGetLastError(), "joint solve launch");
 }
 
@@ -87,8 +87,8 @@ int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_
 int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, 0, 0, ctx->d_jc_off, ctx->d_jcb_off,
-               ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_blk, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody,
-               ctx->d_jbody_blk, ctx->d_minv, ctx->d_jtr, ctx->d_jz, lam_out, ctx->d_jy, x_out, 0.0, 0.0, nullptr, ctrl);
+               ctx->d_jb_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
+               ctx->d_jtr, ctx->d_jz, lam_out, ctx->d_jy, x_out, 0.0, 0.0, nullptr, ctrl);
     return hip_check(ctx, hipGetLastError(), "joint apply launch");
 }
 
@@ -236,6 +236,28 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     CDX_TRY(upi(&ctx->d_jr_blk, jr_blk, &ctx->jr_blk_cap, "joint row blocks"));
     CDX_TRY(upi(&ctx->d_jbody, jbody, &ctx->jbody_cap, "jointed bodies"));
     CDX_TRY(upi(&ctx->d_jbody_blk, jbody_blk, &ctx->jbody_blk_cap, "jointed body blocks"));
+    // per-row and per-jointed-body records of the chain solve (one load instead of an index chain):
+    // row r: {body a, body b, block, r0 << 4 | m}; jointed body q: {body, rows of its first block,
+    // rows of its second block (-1: none), 0} with rows encoded r0 << 5 | m << 1 | side
+    std::vector<int> jr_info(4 * jorig.size()), jbody_info(4 * jbody.size());
+    for (size_t j = 0; j + 1 < jb_off.size(); ++j)
+        for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) {
+            jr_info[4 * (size_t)r] = jb_body[2 * j];
+            jr_info[4 * (size_t)r + 1] = jb_body[2 * j + 1];
+            jr_info[4 * (size_t)r + 2] = (int)j;
+            jr_info[4 * (size_t)r + 3] = (jb_off[j] << 4) | (jb_off[j + 1] - jb_off[j]);
+        }
+    for (size_t q = 0; q < jbody.size(); ++q) {
+        jbody_info[4 * q] = jbody[q];
+        for (int e = 0; e < 2; ++e) {
+            const int j = jbody_blk[4 * q + 2 * e], side = jbody_blk[4 * q + 2 * e + 1];
+            jbody_info[4 * q + 1 + e] = j < 0 ? -1 : (jb_off[j] << 5) | ((jb_off[j + 1] - jb_off[j]) << 1) | side;
+        }
+        jbody_info[4 * q + 3] = 0;
+    }
+    if (jorig.size() >= (1u << 26)) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: too many joint rows");
+    CDX_TRY(upi(&ctx->d_jr_info, jr_info, &ctx->jr_info_cap, "joint row records"));
+    CDX_TRY(upi(&ctx->d_jbody_info, jbody_info, &ctx->jbody_info_cap, "jointed body records"));
     CDX_TRY(upi(&ctx->d_jslot, jslot, &ctx->jslot_cap, "joint slots"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
     CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));

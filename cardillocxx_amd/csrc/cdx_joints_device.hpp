@@ -191,13 +191,15 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
 }
 
 // Staging windows along a chain (uniform across the wave; a window always holds at least one
-// block, whose L has at most kJBlk entries and whose rows at most kMaxJRows).
+// block, whose L has at most kJBlk entries and whose rows at most kMaxJRows).  off(j) is the
+// first row of block j.
 // Forward: blocks [j, je) with their L_jj (m_jj x m_jj-1; none for the chain's first block j0).
-__device__ inline int stage_fwd_end(int j, int j1, int j0, const int* __restrict__ jb_off, int capL, int capR) {
+template <typename Off>
+__device__ inline int stage_fwd_end(int j, int j1, int j0, Off off, int capL, int capR) {
     int nl = 0, nr = 0, je = j;
     while (je < j1) {
-        const int m = jb_off[je + 1] - jb_off[je];
-        const int ml = je > j0 ? m * (jb_off[je] - jb_off[je - 1]) : 0;
+        const int m = off(je + 1) - off(je);
+        const int ml = je > j0 ? m * (off(je) - off(je - 1)) : 0;
         if (je != j && (nl + ml > capL || nr + m > capR)) break;
         nl += ml;
         nr += m;
@@ -207,12 +209,13 @@ __device__ inline int stage_fwd_end(int j, int j1, int j0, const int* __restrict
 }
 
 // Backward: blocks jj = j-1 down to je (>= j0) with L_{jj+1} (m_{jj+1} x m_jj); returns je.
-__device__ inline int stage_bwd_end(int j, int j0, const int* __restrict__ jb_off, int capL, int capR) {
+template <typename Off>
+__device__ inline int stage_bwd_end(int j, int j0, Off off, int capL, int capR) {
     int nl = 0, nr = 0, je = j;
     while (je > j0) {
         const int jj = je - 1;
-        const int m = jb_off[jj + 1] - jb_off[jj];
-        const int ml = (jb_off[jj + 2] - jb_off[jj + 1]) * m;
+        const int m = off(jj + 1) - off(jj);
+        const int ml = (off(jj + 2) - off(jj + 1)) * m;
         if (je != j && (nl + ml > capL || nr + m > capR)) break;
         nl += ml;
         nr += m;
@@ -231,26 +234,33 @@ __device__ inline int stage_bwd_end(int j, int j0, const int* __restrict__ jb_of
 //   s = G M^-1 t (rows in parallel);  z_j = s_j - L_j z_{j-1} (along the chain);
 //   y_j = D_j^-1 z_j (rows in parallel);  Lambda_j = y_j - L_{j+1}^T Lambda_{j+1} (back along it);
 //   Lambda, then the bodies' dv = M^-1 (t - G^T Lambda) (in parallel).
+// jr_info per row: {body a, body b, block, r0 << 4 | m}; jbody_info per jointed body: {body,
+// rows of its first block, rows of its second block (-1: none), 0}, rows as r0 << 5 | m << 1 | side.
+// The first 64 block offsets of the chain sit in a register, one per lane, read by readlane.
 __device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
-                                           const int* __restrict__ jb_body, const int* __restrict__ jr_blk,
-                                           const double* __restrict__ jr, const double* __restrict__ rlam,
-                                           const double* __restrict__ Lm, const double* __restrict__ Dinv,
-                                           const int* __restrict__ jbody, const int* __restrict__ jbody_blk,
+                                           const int* __restrict__ jr_info, const double* __restrict__ jr,
+                                           const double* __restrict__ rlam, const double* __restrict__ Lm,
+                                           const double* __restrict__ Dinv, const int* __restrict__ jbody_info,
                                            const double* __restrict__ minv, const double* __restrict__ tr,
                                            double* __restrict__ z, double* __restrict__ yb, double* __restrict__ lam,
                                            double* __restrict__ x, double tol_rel, double tol_abs, double* __restrict__ lds,
                                            int capL, int capR) {
     const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
-    const int rb = jb_off[j0], re = jb_off[j1];
+    const int boff = lane <= j1 - j0 ? jb_off[j0 + lane] : 0;
+    auto off = [&](int jj) -> int {
+        const int i = jj - j0;
+        return i < 64 ? __builtin_amdgcn_readlane(boff, i) : jb_off[jj];
+    };
+    const int rb = off(j0), re = off(j1);
+    const int4* ri4 = reinterpret_cast<const int4*>(jr_info);
     double* sL = lds;         // staged L entries, packed block by block (m_j x m_{j-1})
     double* sR = lds + capL;  // staged row values
     for (int r = rb + lane; r < re; r += 64) {
-        const int j = jr_blk[r];
-        const int ba = jb_body[2 * j], bb = jb_body[2 * j + 1];
+        const int4 ri = ri4[r];
         double s = 0;
         for (int side = 0; side < 2; ++side) {
-            const int b = side ? bb : ba;
+            const int b = side ? ri.y : ri.x;
             if (b < 0) continue;
             const double* w = jrow(jr, r, side);
             const double* t = tr + (size_t)b * 6;
@@ -263,28 +273,28 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     double zp = 0.0;
     int mp = 0;
     for (int j = j0; j < j1;) {
-        const int je = stage_fwd_end(j, j1, j0, jb_off, capL, capR);
-        const int rs = jb_off[j];
+        const int je = stage_fwd_end(j, j1, j0, off, capL, capR);
+        const int rs = off(j);
         {
-            int off = 0;
+            int o = 0;
             for (int jj = j; jj < je; ++jj) {
-                const int ml = jj > j0 ? (jb_off[jj + 1] - jb_off[jj]) * (jb_off[jj] - jb_off[jj - 1]) : 0;
-                for (int e = lane; e < ml; e += 64) sL[off + e] = Lm[(size_t)jj * kJBlk + e];
-                off += ml;
+                const int ml = jj > j0 ? (off(jj + 1) - off(jj)) * (off(jj) - off(jj - 1)) : 0;
+                for (int e = lane; e < ml; e += 64) sL[o + e] = Lm[(size_t)jj * kJBlk + e];
+                o += ml;
             }
-            for (int r = rs + lane; r < jb_off[je]; r += 64) sR[r - rs] = z[r];
+            for (int r = rs + lane; r < off(je); r += 64) sR[r - rs] = z[r];
         }
         wave_sync();
-        int off = 0;
+        int o = 0;
         for (int jj = j; jj < je; ++jj) {
-            const int r0 = jb_off[jj], m = jb_off[jj + 1] - r0;
+            const int r0 = off(jj), m = off(jj + 1) - r0;
             const bool act = lane < m;
             double t = 0;
             if (jj > j0) {
 #pragma unroll
                 for (int l = 0; l < kMaxJRows; ++l)
-                    if (l < mp) t += (act ? sL[off + lane * mp + l] : 0.0) * readlane_d(zp, l);
-                off += m * mp;
+                    if (l < mp) t += (act ? sL[o + lane * mp + l] : 0.0) * readlane_d(zp, l);
+                o += m * mp;
             }
             const double zc = (act ? sR[r0 - rs + lane] : 0.0) - t;
             if (act) z[r0 + lane] = zc;
@@ -296,9 +306,9 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     }
     // y_j = D_j^-1 z_j, every row in parallel
     for (int r = rb + lane; r < re; r += 64) {
-        const int j = jr_blk[r];
-        const int r0 = jb_off[j], m = jb_off[j + 1] - r0, i = r - r0;
-        const double* Di = Dinv + (size_t)j * kJBlk + i * m;
+        const int4 ri = ri4[r];
+        const int r0 = ri.w >> 4, m = ri.w & 15, i = r - r0;
+        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * m;
         double t = 0;
         for (int l = 0; l < m; ++l) t += Di[l] * z[r0 + l];
         yb[r] = t;
@@ -306,34 +316,34 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     wave_sync();
     // backward along the chain: the last block keeps y; windows of blocks [je, j) going down
     {
-        const int rl = jb_off[j1 - 1], ml = re - rl;
+        const int rl = off(j1 - 1), ml = re - rl;
         double lnext = lane < ml ? yb[rl + lane] : 0.0;
         if (lane < ml) z[rl + lane] = lnext;
         int mnx = ml;
         for (int j = j1 - 1; j > j0;) {
             // blocks jj = j-1 down to je need L_{jj+1} and y_jj
-            const int je = stage_bwd_end(j, j0, jb_off, capL, capR);
-            const int rs = jb_off[je];
+            const int je = stage_bwd_end(j, j0, off, capL, capR);
+            const int rs = off(je);
             {
-                int off = 0;
+                int o = 0;
                 for (int jj = j - 1; jj >= je; --jj) {
                     const int n = jj + 1;
-                    const int mlj = (jb_off[n + 1] - jb_off[n]) * (jb_off[n] - jb_off[n - 1]);
-                    for (int e = lane; e < mlj; e += 64) sL[off + e] = Lm[(size_t)n * kJBlk + e];
-                    off += mlj;
+                    const int mlj = (off(n + 1) - off(n)) * (off(n) - off(n - 1));
+                    for (int e = lane; e < mlj; e += 64) sL[o + e] = Lm[(size_t)n * kJBlk + e];
+                    o += mlj;
                 }
-                for (int r = rs + lane; r < jb_off[j]; r += 64) sR[r - rs] = yb[r];
+                for (int r = rs + lane; r < off(j); r += 64) sR[r - rs] = yb[r];
             }
             wave_sync();
-            int off = 0;
+            int o = 0;
             for (int jj = j - 1; jj >= je; --jj) {
-                const int r0 = jb_off[jj], m = jb_off[jj + 1] - r0;
+                const int r0 = off(jj), m = off(jj + 1) - r0;
                 const bool act = lane < m;
                 double t = 0;
 #pragma unroll
                 for (int l = 0; l < kMaxJRows; ++l)
-                    if (l < mnx) t += (act ? sL[off + l * m + lane] : 0.0) * readlane_d(lnext, l);
-                off += mnx * m;
+                    if (l < mnx) t += (act ? sL[o + l * m + lane] : 0.0) * readlane_d(lnext, l);
+                o += mnx * m;
                 const double lj = (act ? sR[r0 - rs + lane] : 0.0) - t;
                 if (act) z[r0 + lane] = lj;
                 lnext = lj;
@@ -346,15 +356,18 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     for (int r = rb + lane; r < re; r += 64) lam[r] = setup ? z[r] : lam[r] + z[r];
     // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda), one (body, dof) per lane
     double e2 = 0;
+    const int4* bi4 = reinterpret_cast<const int4*>(jbody_info);
     const int qb = jcb_off[ch], nq = jcb_off[ch + 1] - qb;
     for (int e = lane; e < 6 * nq; e += 64) {
         const int q = qb + e / 6, k = e - (e / 6) * 6;
-        const int b = jbody[q];
+        const int4 bi = bi4[q];
+        const int b = bi.x;
         double acc = tr[(size_t)b * 6 + k];
         for (int ee = 0; ee < 2; ++ee) {
-            const int j = jbody_blk[4 * (size_t)q + 2 * ee], side = jbody_blk[4 * (size_t)q + 2 * ee + 1];
-            if (j < 0) continue;
-            for (int r = jb_off[j]; r < jb_off[j + 1]; ++r) acc -= jrow(jr, r, side)[k] * z[r];
+            const int enc = ee ? bi.z : bi.y;
+            if (enc < 0) continue;
+            const int r0 = enc >> 5, m = (enc >> 1) & 15, side = enc & 1;
+            for (int r = r0; r < r0 + m; ++r) acc -= jrow(jr, r, side)[k] * z[r];
         }
         double* xb = x + (size_t)b * kBodyStride;
         const double dv = minv_k(minv, b, k) * acc;
