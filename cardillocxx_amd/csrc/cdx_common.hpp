@@ -79,6 +79,8 @@ struct cdx_ctx {
     int *d_jbody = nullptr, *d_jbody_blk = nullptr, *d_jslot = nullptr, *d_jr_blk = nullptr;
     int *d_jr_info = nullptr, *d_jbody_info = nullptr;  // chain-solve records per row / per jointed body
     size_t jr_info_cap = 0, jbody_info_cap = 0;
+    int* d_jl_off = nullptr;  // [nblocks+1] packed offset of each block's L factor in d_jL
+    size_t jl_off_cap = 0;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
     double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)

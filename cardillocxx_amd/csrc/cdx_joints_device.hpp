@@ -84,7 +84,8 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
                                          const int* __restrict__ jb_body, const int* __restrict__ jr_kind,
                                          const int* __restrict__ jr_blk, const double* __restrict__ jr,
                                          const double* __restrict__ minv, const double* __restrict__ v, double dt,
-                                         double theta, double beta, double* __restrict__ rlam, double* __restrict__ Lm,
+                                         double theta, double beta, double* __restrict__ rlam,
+                                         const int* __restrict__ jl_off, double* __restrict__ Lm,
                                          double* __restrict__ Dinv, double* __restrict__ lds) {
     const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
     for (int r = jb_off[j0] + lane; r < jb_off[j1]; r += 64)
@@ -130,7 +131,7 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
                 Kjp[e] = t;
             }
             wave_sync();
-            double* L = Lm + (size_t)j * kJBlk;
+            double* L = Lm + jl_off[j];  // packed: m x mp entries
             for (int e = lane; e < m * mp; e += 64) {
                 const int i = e / mp, l = e - (e / mp) * mp;
                 double t = 0;
@@ -224,163 +225,246 @@ __device__ inline int stage_bwd_end(int j, int j0, Off off, int capL, int capR) 
     return je;
 }
 
+// dst[i] = src[i] for i < n by one wave; every load of a lane is issued before its first store,
+// so a copy costs about one memory latency per 512 elements instead of one per element.
+__device__ inline void wave_copy(double* __restrict__ dst, const double* __restrict__ src, int n, int lane) {
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < n; i0 += 64 * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u + lane;
+            v[u] = src[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u + lane;
+            if (i < n) dst[i] = v[u];
+        }
+    }
+}
+
+// rows of one lane in the single-window solve (capR <= 64 * kWinRowsPerLane)
+constexpr int kWinRowsPerLane = kSolveLdsR / 64;
+static_assert(kBatchLdsR <= kSolveLdsR, "row window beyond the single-window row count");
+
 // x += S^-1 [r; -r_lambda] restricted to the jointed bodies of chain ch, on one wave.
 //   setup = 1: x0 and Lambda_0 are set (absolute; tr holds the full body right-hand side);
 //   setup = 0: a sweep increment (tr holds W^T dp of each jointed body); x and Lambda accumulate.
 // Returns this lane's share of the chain bodies' residual sum (projected_jacobi.cpp:140-144)
 // when `check`.  z and y are scratch of one double per joint row; `lds` holds capL + capR
-// doubles private to the wave, the staging window of the two passes along the chain.
+// doubles private to the wave (capR <= kSolveLdsR).
 //
 //   s = G M^-1 t (rows in parallel);  z_j = s_j - L_j z_{j-1} (along the chain);
 //   y_j = D_j^-1 z_j (rows in parallel);  Lambda_j = y_j - L_{j+1}^T Lambda_{j+1} (back along it);
 //   Lambda, then the bodies' dv = M^-1 (t - G^T Lambda) (in parallel).
+// A chain whose rows fit in capR and whose L fits in capL keeps s, z, y and Lambda in LDS for
+// the whole solve; a longer one goes through z and y in memory, a staged window at a time.
+// L_j (m_j x m_{j-1}, row-major) of the chain's blocks are packed back to back from jl_off[j].
 // jr_info per row: {body a, body b, block, r0 << 4 | m}; jbody_info per jointed body: {body,
 // rows of its first block, rows of its second block (-1: none), 0}, rows as r0 << 5 | m << 1 | side.
-// The first 64 block offsets of the chain sit in a register, one per lane, read by readlane.
+// The first 64 block and L offsets of the chain sit in registers, one per lane, read by readlane.
 __device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
-                                           const int* __restrict__ jr_info, const double* __restrict__ jr,
-                                           const double* __restrict__ rlam, const double* __restrict__ Lm,
-                                           const double* __restrict__ Dinv, const int* __restrict__ jbody_info,
-                                           const double* __restrict__ minv, const double* __restrict__ tr,
-                                           double* __restrict__ z, double* __restrict__ yb, double* __restrict__ lam,
-                                           double* __restrict__ x, double tol_rel, double tol_abs, double* __restrict__ lds,
-                                           int capL, int capR) {
+                                           const int* __restrict__ jl_off, const int* __restrict__ jr_info,
+                                           const double* __restrict__ jr, const double* __restrict__ rlam,
+                                           const double* __restrict__ Lm, const double* __restrict__ Dinv,
+                                           const int* __restrict__ jbody_info, const double* __restrict__ minv,
+                                           const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ yb,
+                                           double* __restrict__ lam, double* __restrict__ x, double tol_rel, double tol_abs,
+                                           double* __restrict__ lds, int capL, int capR) {
     const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
     const int boff = lane <= j1 - j0 ? jb_off[j0 + lane] : 0;
+    const int loffr = lane <= j1 - j0 ? jl_off[j0 + lane] : 0;
     auto off = [&](int jj) -> int {
         const int i = jj - j0;
         return i < 64 ? __builtin_amdgcn_readlane(boff, i) : jb_off[jj];
     };
+    auto loff = [&](int jj) -> int {
+        const int i = jj - j0;
+        return i < 64 ? __builtin_amdgcn_readlane(loffr, i) : jl_off[jj];
+    };
     const int rb = off(j0), re = off(j1);
+    const int lb = loff(j0), le = loff(j1);
+    const bool one = re - rb <= capR && le - lb <= capL;  // uniform
     const int4* ri4 = reinterpret_cast<const int4*>(jr_info);
-    double* sL = lds;         // staged L entries, packed block by block (m_j x m_{j-1})
+    double* sL = lds;         // staged L entries (packed as in Lm)
     double* sR = lds + capL;  // staged row values
+    // s = G M^-1 t (minus r_lambda at setup), into LDS or z
     for (int r = rb + lane; r < re; r += 64) {
         const int4 ri = ri4[r];
         double s = 0;
+#pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int b = side ? ri.y : ri.x;
-            if (b < 0) continue;
+            const int bc = b < 0 ? 0 : b;
             const double* w = jrow(jr, r, side);
-            const double* t = tr + (size_t)b * 6;
-            for (int k = 0; k < 6; ++k) s += w[k] * (minv_k(minv, b, k) * t[k]);
+            const double* t = tr + (size_t)bc * 6;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double term = w[k] * (minv_k(minv, bc, k) * t[k]);
+                if (b >= 0) s += term;
+            }
         }
-        z[r] = setup ? s - rlam[r] : s;
+        const double val = setup ? s - rlam[r] : s;
+        if (one)
+            sR[r - rb] = val;
+        else
+            z[r] = val;
     }
+    if (one) wave_copy(sL, Lm + lb, le - lb, lane);
     wave_sync();
-    // forward along the chain, a staged window at a time
+    // z_j = s_j - L_j z_{j-1} over blocks [ja, jb) held in LDS from L offset lbase and row rs
     double zp = 0.0;
     int mp = 0;
-    for (int j = j0; j < j1;) {
-        const int je = stage_fwd_end(j, j1, j0, off, capL, capR);
-        const int rs = off(j);
-        {
-            int o = 0;
-            for (int jj = j; jj < je; ++jj) {
-                const int ml = jj > j0 ? (off(jj + 1) - off(jj)) * (off(jj) - off(jj - 1)) : 0;
-                for (int e = lane; e < ml; e += 64) sL[o + e] = Lm[(size_t)jj * kJBlk + e];
-                o += ml;
-            }
-            for (int r = rs + lane; r < off(je); r += 64) sR[r - rs] = z[r];
-        }
-        wave_sync();
-        int o = 0;
-        for (int jj = j; jj < je; ++jj) {
+    auto fwd_blocks = [&](int ja, int jb, int lbase, int rs) {
+        for (int jj = ja; jj < jb; ++jj) {
             const int r0 = off(jj), m = off(jj + 1) - r0;
             const bool act = lane < m;
             double t = 0;
             if (jj > j0) {
+                const int o = loff(jj) - lbase;
 #pragma unroll
                 for (int l = 0; l < kMaxJRows; ++l)
                     if (l < mp) t += (act ? sL[o + lane * mp + l] : 0.0) * readlane_d(zp, l);
-                o += m * mp;
             }
             const double zc = (act ? sR[r0 - rs + lane] : 0.0) - t;
-            if (act) z[r0 + lane] = zc;
+            if (act) sR[r0 - rs + lane] = zc;
             zp = zc;
             mp = m;
         }
-        wave_sync();
-        j = je;
-    }
-    // y_j = D_j^-1 z_j, every row in parallel
-    for (int r = rb + lane; r < re; r += 64) {
+    };
+    // Lambda_jj = y_jj - L_{jj+1}^T Lambda_{jj+1} for jj = jhi-1 down to jlo
+    double lnext = 0.0;
+    int mnx = 0;
+    auto bwd_blocks = [&](int jhi, int jlo, int lbase, int rs) {
+        for (int jj = jhi - 1; jj >= jlo; --jj) {
+            const int r0 = off(jj), m = off(jj + 1) - r0;
+            const bool act = lane < m;
+            const int o = loff(jj + 1) - lbase;
+            double t = 0;
+#pragma unroll
+            for (int l = 0; l < kMaxJRows; ++l)
+                if (l < mnx) t += (act ? sL[o + l * m + lane] : 0.0) * readlane_d(lnext, l);
+            const double lj = (act ? sR[r0 - rs + lane] : 0.0) - t;
+            if (act) sR[r0 - rs + lane] = lj;
+            lnext = lj;
+            mnx = m;
+        }
+    };
+    // y_r = (D_j^-1 z_j)_i for row r = r0 + i of block j, z read through zat
+    auto y_row = [&](int r, auto zat) -> double {
         const int4 ri = ri4[r];
         const int r0 = ri.w >> 4, m = ri.w & 15, i = r - r0;
-        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * m;
+        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * m;  // i m + l < kJBlk for every l below
         double t = 0;
-        for (int l = 0; l < m; ++l) t += Di[l] * z[r0 + l];
-        yb[r] = t;
-    }
-    wave_sync();
-    // backward along the chain: the last block keeps y; windows of blocks [je, j) going down
-    {
-        const int rl = off(j1 - 1), ml = re - rl;
-        double lnext = lane < ml ? yb[rl + lane] : 0.0;
-        if (lane < ml) z[rl + lane] = lnext;
-        int mnx = ml;
-        for (int j = j1 - 1; j > j0;) {
-            // blocks jj = j-1 down to je need L_{jj+1} and y_jj
-            const int je = stage_bwd_end(j, j0, off, capL, capR);
-            const int rs = off(je);
-            {
-                int o = 0;
-                for (int jj = j - 1; jj >= je; --jj) {
-                    const int n = jj + 1;
-                    const int mlj = (off(n + 1) - off(n)) * (off(n) - off(n - 1));
-                    for (int e = lane; e < mlj; e += 64) sL[o + e] = Lm[(size_t)n * kJBlk + e];
-                    o += mlj;
-                }
-                for (int r = rs + lane; r < off(j); r += 64) sR[r - rs] = yb[r];
-            }
-            wave_sync();
-            int o = 0;
-            for (int jj = j - 1; jj >= je; --jj) {
-                const int r0 = off(jj), m = off(jj + 1) - r0;
-                const bool act = lane < m;
-                double t = 0;
 #pragma unroll
-                for (int l = 0; l < kMaxJRows; ++l)
-                    if (l < mnx) t += (act ? sL[o + l * m + lane] : 0.0) * readlane_d(lnext, l);
-                o += mnx * m;
-                const double lj = (act ? sR[r0 - rs + lane] : 0.0) - t;
-                if (act) z[r0 + lane] = lj;
-                lnext = lj;
-                mnx = m;
-            }
+        for (int l = 0; l < kMaxJRows; ++l) {
+            const double d = Di[l];
+            const double zz = zat(r0 + (l < m ? l : m - 1));
+            if (l < m) t += d * zz;
+        }
+        return t;
+    };
+    if (one) {
+        fwd_blocks(j0, j1, lb, rb);
+        wave_sync();
+        double yv[kWinRowsPerLane];
+#pragma unroll
+        for (int u = 0; u < kWinRowsPerLane; ++u) {
+            const int r = rb + lane + 64 * u;
+            yv[u] = y_row(r < re ? r : re - 1, [&](int rr) { return sR[rr - rb]; });
+        }
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < kWinRowsPerLane; ++u) {
+            const int r = rb + lane + 64 * u;
+            if (r < re) sR[r - rb] = yv[u];
+        }
+        wave_sync();
+        const int rl = off(j1 - 1);
+        lnext = lane < re - rl ? sR[rl - rb + lane] : 0.0;
+        mnx = re - rl;
+        bwd_blocks(j1 - 1, j0, lb, rb);
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < kWinRowsPerLane; ++u) {
+            const int r = rb + lane + 64 * u;
+            if (r < re) lam[r] = setup ? sR[r - rb] : lam[r] + sR[r - rb];
+        }
+    } else {
+        // forward along the chain, a staged window at a time
+        for (int j = j0; j < j1;) {
+            const int je = stage_fwd_end(j, j1, j0, off, capL, capR);
+            const int rs = off(j), nr = off(je) - rs;
+            wave_copy(sL, Lm + loff(j), loff(je) - loff(j), lane);
+            wave_copy(sR, z + rs, nr, lane);
+            wave_sync();
+            fwd_blocks(j, je, loff(j), rs);
+            wave_sync();
+            wave_copy(z + rs, sR, nr, lane);
             wave_sync();
             j = je;
         }
+        for (int r = rb + lane; r < re; r += 64) yb[r] = y_row(r, [&](int rr) { return z[rr]; });
+        wave_sync();
+        // backward: the last block keeps y; windows of blocks [je, j) going down
+        const int rl = off(j1 - 1), ml = re - rl;
+        lnext = lane < ml ? yb[rl + lane] : 0.0;
+        if (lane < ml) z[rl + lane] = lnext;
+        mnx = ml;
+        for (int j = j1 - 1; j > j0;) {
+            const int je = stage_bwd_end(j, j0, off, capL, capR);
+            const int rs = off(je), nr = off(j) - rs;
+            wave_copy(sL, Lm + loff(je + 1), loff(j + 1) - loff(je + 1), lane);
+            wave_copy(sR, yb + rs, nr, lane);
+            wave_sync();
+            bwd_blocks(j, je, loff(je + 1), rs);
+            wave_sync();
+            wave_copy(z + rs, sR, nr, lane);
+            wave_sync();
+            j = je;
+        }
+        for (int r = rb + lane; r < re; r += 64) lam[r] = setup ? z[r] : lam[r] + z[r];
     }
-    for (int r = rb + lane; r < re; r += 64) lam[r] = setup ? z[r] : lam[r] + z[r];
     // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda), one (body, dof) per lane
-    double e2 = 0;
     const int4* bi4 = reinterpret_cast<const int4*>(jbody_info);
     const int qb = jcb_off[ch], nq = jcb_off[ch + 1] - qb;
-    for (int e = lane; e < 6 * nq; e += 64) {
-        const int q = qb + e / 6, k = e - (e / 6) * 6;
-        const int4 bi = bi4[q];
-        const int b = bi.x;
-        double acc = tr[(size_t)b * 6 + k];
-        for (int ee = 0; ee < 2; ++ee) {
-            const int enc = ee ? bi.z : bi.y;
-            if (enc < 0) continue;
-            const int r0 = enc >> 5, m = (enc >> 1) & 15, side = enc & 1;
-            for (int r = r0; r < r0 + m; ++r) acc -= jrow(jr, r, side)[k] * z[r];
+    auto bodies = [&](auto lamat) -> double {
+        double e2 = 0;
+        for (int e = lane; e < 6 * nq; e += 64) {
+            const int q = qb + e / 6, k = e - (e / 6) * 6;
+            const int4 bi = bi4[q];
+            const int b = bi.x;
+            double acc = tr[(size_t)b * 6 + k];
+#pragma unroll
+            for (int ee = 0; ee < 2; ++ee) {
+                const int enc = ee ? bi.z : bi.y;
+                const bool has = enc >= 0;
+                const int r0 = has ? enc >> 5 : rb, m = has ? (enc >> 1) & 15 : 1, side = enc & 1;
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l) {
+                    const int r = r0 + (l < m ? l : m - 1);
+                    const double w = jrow(jr, r, side)[k];
+                    const double lv = lamat(r);
+                    if (has && l < m) acc -= w * lv;
+                }
+            }
+            double* xb = x + (size_t)b * kBodyStride;
+            const double dv = minv_k(minv, b, k) * acc;
+            const double u = xb[k];
+            const double un = setup ? dv : u + dv;
+            xb[k] = un;
+            if (check) {
+                const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                const double ee2 = (un - u) / qd;
+                e2 += ee2 * ee2;
+            }
         }
-        double* xb = x + (size_t)b * kBodyStride;
-        const double dv = minv_k(minv, b, k) * acc;
-        const double u = xb[k];
-        const double un = setup ? dv : u + dv;
-        xb[k] = un;
-        if (check) {
-            const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
-            const double ee2 = (un - u) / qd;
-            e2 += ee2 * ee2;
-        }
-    }
-    return e2;
+        return e2;
+    };
+    if (one) return bodies([&](int r) { return sR[r - rb]; });
+    return bodies([&](int r) { return z[r]; });
 }
 
 }  // namespace cdx
