@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
                                        ja.theta, ja.beta, ja.rlam, ja.jl_off, ja.Lm, ja.Dinv, jlds[threadIdx.x >> 6]))
                     gfail[g] = 1;
                 else
-                    joint_solve_chain(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.rlam, ja.Lm,
+                    joint_solve_chain<false>(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.rlam, ja.Lm,
                                       ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel, tol_abs,
                                       jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             __syncthreads();
             if (!done) {
                 for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv)
-                    e2 += joint_solve_chain(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info,
+                    e2 += joint_solve_chain<false>(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info,
                                             ja.jr, ja.rlam, ja.Lm, ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x,
                                             tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }

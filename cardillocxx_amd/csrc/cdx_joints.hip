@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(64) k_joint_solve(
     if (ctrl && ctrl[0]) return;
     const int ch = blockIdx.x;
     if (ch >= nchains) return;
-    double e2 = joint_solve_chain(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jl_off, jr_info, jr, rlam, Lm, Dinv,
+    double e2 = joint_solve_chain<true>(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jl_off, jr_info, jr, rlam, Lm, Dinv,
                                   jbody_info, minv, tr, z, yb, lam, x, tol_rel, tol_abs, lds, kSolveLdsL, kSolveLdsR);
     if (check) {
         for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);

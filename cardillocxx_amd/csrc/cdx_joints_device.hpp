@@ -10,6 +10,8 @@
 // product is summed in the same order as a one-thread-per-chain loop would sum it.
 #pragma once
 
+#include <type_traits>
+
 #include "cdx_device.hpp"
 
 namespace cdx {
@@ -22,7 +24,7 @@ constexpr int kJBlk = kMaxJRows * kMaxJRows;
 constexpr int kFactorLds = 3 * kJBlk + 2 * kJBlk + kMaxJRows;
 // staging window of the grid-wide chain solve (L entries, rows): a 32-link chain of spherical
 // joints with dampers (31 blocks of 6 rows) fits in one window
-constexpr int kSolveLdsL = 1536, kSolveLdsR = 256;
+constexpr int kSolveLdsL = 1536, kSolveLdsR = 192;
 // the batched kernel stages through each wave's factorisation LDS
 constexpr int kBatchLdsL = 600, kBatchLdsR = kFactorLds - kBatchLdsL;
 
@@ -179,10 +181,15 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
             }
             wave_sync();
         }
+        // D_j^-1 row i at Dinv[j kJBlk + i kMaxJRows], zero beyond column m: a substitution may sum
+        // all kMaxJRows columns of a row
         double* Di = Dinv + (size_t)j * kJBlk;
+        for (int e = lane; e < m * kMaxJRows; e += 64) {
+            const int i = e / kMaxJRows, l = e - (e / kMaxJRows) * kMaxJRows;
+            Di[e] = l < m ? A[i * w2 + m + l] : 0.0;
+        }
         for (int e = lane; e < m * m; e += 64) {
             const int i = e / m, l = e - (e / m) * m;
-            Di[e] = A[i * w2 + m + l];
             Dp[e] = A[i * w2 + m + l];
         }
         wave_sync();
@@ -225,6 +232,43 @@ __device__ inline int stage_bwd_end(int j, int j0, Off off, int capL, int capR) 
     return je;
 }
 
+// sum over l < N of a[l * stride] * (v of lane l), in order of l; all N loads are issued first
+template <int N>
+__device__ inline double lane_dot(const double* __restrict__ a, int stride, double v) {
+    double ar[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) ar[l] = a[l * stride];
+    double t = 0;
+#pragma unroll
+    for (int l = 0; l < N; ++l) t = fma(ar[l], readlane_d(v, l), t);
+    return t;
+}
+
+// f(std::integral_constant<int, n>) for a wave-uniform n in [1, kMaxJRows]: one straight-line
+// body per block width instead of a loop of runtime length
+template <typename F>
+__device__ inline auto with_width(int n, F&& f) {
+    using std::integral_constant;
+    switch (n) {
+        case 1: return f(integral_constant<int, 1>{});
+        case 2: return f(integral_constant<int, 2>{});
+        case 3: return f(integral_constant<int, 3>{});
+        case 4: return f(integral_constant<int, 4>{});
+        case 5: return f(integral_constant<int, 5>{});
+        case 6: return f(integral_constant<int, 6>{});
+        case 7: return f(integral_constant<int, 7>{});
+        case 8: return f(integral_constant<int, 8>{});
+        case 9: return f(integral_constant<int, 9>{});
+        case 10: return f(integral_constant<int, 10>{});
+        case 11: return f(integral_constant<int, 11>{});
+        default: return f(integral_constant<int, kMaxJRows>{});
+    }
+}
+
+__device__ inline double lane_dot_n(int n, const double* __restrict__ a, int stride, double v) {
+    return with_width(n, [&](auto N) { return lane_dot<decltype(N)::value>(a, stride, v); });
+}
+
 // dst[i] = src[i] for i < n by one wave; every load of a lane is issued before its first store,
 // so a copy costs about one memory latency per 512 elements instead of one per element.
 __device__ inline void wave_copy(double* __restrict__ dst, const double* __restrict__ src, int n, int lane) {
@@ -244,8 +288,9 @@ __device__ inline void wave_copy(double* __restrict__ dst, const double* __restr
     }
 }
 
-// rows of one lane in the single-window solve (capR <= 64 * kWinRowsPerLane)
+// rows and (body, dof) items of one lane in the single-window solve
 constexpr int kWinRowsPerLane = kSolveLdsR / 64;
+constexpr int kWinItemsPerLane = 6;
 static_assert(kBatchLdsR <= kSolveLdsR, "row window beyond the single-window row count");
 
 // x += S^-1 [r; -r_lambda] restricted to the jointed bodies of chain ch, on one wave.
@@ -264,6 +309,10 @@ static_assert(kBatchLdsR <= kSolveLdsR, "row window beyond the single-window row
 // jr_info per row: {body a, body b, block, r0 << 4 | m}; jbody_info per jointed body: {body,
 // rows of its first block, rows of its second block (-1: none), 0}, rows as r0 << 5 | m << 1 | side.
 // The first 64 block and L offsets of the chain sit in registers, one per lane, read by readlane.
+// Widths = true specialises the block steps and row sums on the (wave-uniform) block width: for
+// kernels where the chain's offsets are scalar values, i.e. where this function is inlined with
+// one chain per workgroup (k_joint_solve).  The batched kernel uses the fixed-width form.
+template <bool Widths>
 __device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
                                            const int* __restrict__ jl_off, const int* __restrict__ jr_info,
@@ -286,69 +335,71 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     };
     const int rb = off(j0), re = off(j1);
     const int lb = loff(j0), le = loff(j1);
-    const bool one = re - rb <= capR && le - lb <= capL;  // uniform
     const int4* ri4 = reinterpret_cast<const int4*>(jr_info);
+    const int4* bi4 = reinterpret_cast<const int4*>(jbody_info);
+    const int qb = jcb_off[ch], nq = jcb_off[ch + 1] - qb;
+    // uniform: the chain's L, its rows (also 6 products per row) and its jointed bodies fit, and
+    // all of its block offsets sit in the registers
+    const bool one = j1 - j0 < 64 && re - rb <= capR && le - lb <= capL && 6 * (re - rb) <= capL &&
+                     6 * nq <= 64 * kWinItemsPerLane;
+    auto off_r = [&](int jj) -> int { return __builtin_amdgcn_readlane(boff, jj - j0); };
+    auto loff_r = [&](int jj) -> int { return __builtin_amdgcn_readlane(loffr, jj - j0); };
     double* sL = lds;         // staged L entries (packed as in Lm)
     double* sR = lds + capL;  // staged row values
-    // s = G M^-1 t (minus r_lambda at setup), into LDS or z
-    for (int r = rb + lane; r < re; r += 64) {
-        const int4 ri = ri4[r];
-        double s = 0;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int b = side ? ri.y : ri.x;
-            const int bc = b < 0 ? 0 : b;
-            const double* w = jrow(jr, r, side);
-            const double* t = tr + (size_t)bc * 6;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const double term = w[k] * (minv_k(minv, bc, k) * t[k]);
-                if (b >= 0) s += term;
-            }
-        }
-        const double val = setup ? s - rlam[r] : s;
-        if (one)
-            sR[r - rb] = val;
-        else
-            z[r] = val;
-    }
-    if (one) wave_copy(sL, Lm + lb, le - lb, lane);
-    wave_sync();
     // z_j = s_j - L_j z_{j-1} over blocks [ja, jb) held in LDS from L offset lbase and row rs
     double zp = 0.0;
     int mp = 0;
-    auto fwd_blocks = [&](int ja, int jb, int lbase, int rs) {
+    // (block offsets through offf / lofff)
+    auto fwd_blocks = [&](int ja, int jb, int lbase, int rs, auto offf, auto lofff) {
         for (int jj = ja; jj < jb; ++jj) {
-            const int r0 = off(jj), m = off(jj + 1) - r0;
+            const int r0 = offf(jj), m = offf(jj + 1) - r0;
             const bool act = lane < m;
+            const int li = act ? lane : 0;
             double t = 0;
-            if (jj > j0) {
-                const int o = loff(jj) - lbase;
+            if constexpr (Widths) {
+                if (jj > j0) t = lane_dot_n(mp, sL + lofff(jj) - lbase + li * mp, 1, zp);
+            } else if (jj > j0) {
+                const int o = lofff(jj) - lbase + li * mp;
+                double lr[kMaxJRows];
 #pragma unroll
-                for (int l = 0; l < kMaxJRows; ++l)
-                    if (l < mp) t += (act ? sL[o + lane * mp + l] : 0.0) * readlane_d(zp, l);
+                for (int l = 0; l < kMaxJRows; ++l) lr[l] = sL[o + (l < mp ? l : mp - 1)];
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l) {
+                    const double zl = readlane_d(zp, l);
+                    t = l < mp ? fma(lr[l], zl, t) : t;
+                }
             }
-            const double zc = (act ? sR[r0 - rs + lane] : 0.0) - t;
+            const double zc = sR[r0 - rs + li] - t;
             if (act) sR[r0 - rs + lane] = zc;
-            zp = zc;
+            zp = act ? zc : 0.0;
             mp = m;
         }
     };
     // Lambda_jj = y_jj - L_{jj+1}^T Lambda_{jj+1} for jj = jhi-1 down to jlo
     double lnext = 0.0;
     int mnx = 0;
-    auto bwd_blocks = [&](int jhi, int jlo, int lbase, int rs) {
+    auto bwd_blocks = [&](int jhi, int jlo, int lbase, int rs, auto offf, auto lofff) {
         for (int jj = jhi - 1; jj >= jlo; --jj) {
-            const int r0 = off(jj), m = off(jj + 1) - r0;
+            const int r0 = offf(jj), m = offf(jj + 1) - r0;
             const bool act = lane < m;
-            const int o = loff(jj + 1) - lbase;
+            const int li = act ? lane : 0;
             double t = 0;
+            if constexpr (Widths) {
+                t = lane_dot_n(mnx, sL + lofff(jj + 1) - lbase + li, m, lnext);
+            } else {
+                const int o = lofff(jj + 1) - lbase + li;
+                double lr[kMaxJRows];
 #pragma unroll
-            for (int l = 0; l < kMaxJRows; ++l)
-                if (l < mnx) t += (act ? sL[o + l * m + lane] : 0.0) * readlane_d(lnext, l);
-            const double lj = (act ? sR[r0 - rs + lane] : 0.0) - t;
+                for (int l = 0; l < kMaxJRows; ++l) lr[l] = sL[o + (l < mnx ? l : mnx - 1) * m];
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l) {
+                    const double ll = readlane_d(lnext, l);
+                    t = l < mnx ? fma(lr[l], ll, t) : t;
+                }
+            }
+            const double lj = sR[r0 - rs + li] - t;
             if (act) sR[r0 - rs + lane] = lj;
-            lnext = lj;
+            lnext = act ? lj : 0.0;
             mnx = m;
         }
     };
@@ -356,24 +407,77 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     auto y_row = [&](int r, auto zat) -> double {
         const int4 ri = ri4[r];
         const int r0 = ri.w >> 4, m = ri.w & 15, i = r - r0;
-        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * m;  // i m + l < kJBlk for every l below
+        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * kMaxJRows;  // zero beyond column m
         double t = 0;
 #pragma unroll
-        for (int l = 0; l < kMaxJRows; ++l) {
-            const double d = Di[l];
-            const double zz = zat(r0 + (l < m ? l : m - 1));
-            if (l < m) t += d * zz;
-        }
+        for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[l], zat(r0 + (l < m ? l : m - 1)), t);
         return t;
     };
     if (one) {
-        fwd_blocks(j0, j1, lb, rb);
+        // widest block of the chain (wave-uniform)
+        int kw = kMaxJRows;
+        if constexpr (Widths) {
+            const int bnext = __shfl_down(boff, 1, 64);  // every lane takes part in the shuffle
+            int mw = lane < j1 - j0 ? bnext - boff : 0;
+            for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o, 64));
+            kw = __builtin_amdgcn_readfirstlane(mw);
+        }
+        // this lane's rows rb + lane + 64 u: records and coefficients stay in registers
+        int4 ri[kWinRowsPerLane];
+        double w[kWinRowsPerLane][12];
+#pragma unroll
+        for (int u = 0; u < kWinRowsPerLane; ++u) {
+            const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
+            ri[u] = ri4[rc];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) w[u][k] = jr[(size_t)rc * kJRowStride + k];
+        }
+        // s = G M^-1 t (minus r_lambda at setup)
+#pragma unroll
+        for (int u = 0; u < kWinRowsPerLane; ++u) {
+            const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
+            double s = 0;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int b = side ? ri[u].y : ri[u].x;
+                const int bc = b < 0 ? 0 : b;
+                const double* t = tr + (size_t)bc * 6;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {  // a static side adds exact zeros
+                    const double wk = b >= 0 ? w[u][6 * side + k] : 0.0;
+                    s = fma(wk, minv_k(minv, bc, k) * t[k], s);
+                }
+            }
+            const double val = setup ? s - rlam[rc] : s;
+            if (r < re) sR[r - rb] = val;
+        }
+        wave_copy(sL, Lm + lb, le - lb, lane);
+        wave_sync();
+        fwd_blocks(j0, j1, lb, rb, off_r, loff_r);
         wave_sync();
         double yv[kWinRowsPerLane];
 #pragma unroll
         for (int u = 0; u < kWinRowsPerLane; ++u) {
-            const int r = rb + lane + 64 * u;
-            yv[u] = y_row(r < re ? r : re - 1, [&](int rr) { return sR[rr - rb]; });
+            const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
+            const int r0 = ri[u].w >> 4, m = ri[u].w & 15, i = rc - r0;
+            const double* Di = Dinv + (size_t)ri[u].z * kJBlk + i * kMaxJRows;  // zero beyond column m
+            auto row = [&](auto N) {
+                constexpr int n = decltype(N)::value;
+                double d[n], t = 0;
+#pragma unroll
+                for (int l = 0; l < n; ++l) d[l] = Di[l];
+#pragma unroll
+                for (int l = 0; l < n; ++l) t = fma(d[l], sR[r0 - rb + (l < m ? l : m - 1)], t);
+                return t;
+            };
+            if constexpr (Widths) {
+                yv[u] = with_width(kw, row);
+            } else {
+                double t = 0;
+#pragma unroll
+                for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[l], sR[r0 - rb + (l < m ? l : m - 1)], t);
+                yv[u] = t;
+            }
         }
         wave_sync();
 #pragma unroll
@@ -382,17 +486,107 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             if (r < re) sR[r - rb] = yv[u];
         }
         wave_sync();
-        const int rl = off(j1 - 1);
+        const int rl = off_r(j1 - 1);
         lnext = lane < re - rl ? sR[rl - rb + lane] : 0.0;
         mnx = re - rl;
-        bwd_blocks(j1 - 1, j0, lb, rb);
+        bwd_blocks(j1 - 1, j0, lb, rb, off_r, loff_r);
         wave_sync();
+        double lv[kWinRowsPerLane];
 #pragma unroll
         for (int u = 0; u < kWinRowsPerLane; ++u) {
             const int r = rb + lane + 64 * u;
-            if (r < re) lam[r] = setup ? sR[r - rb] : lam[r] + sR[r - rb];
+            lv[u] = r < re ? sR[r - rb] : 0.0;
+            if (r < re) lam[r] = setup ? lv[u] : lam[r] + lv[u];
         }
+        // bodies: t - G_b^T Lambda from the row products w Lambda, staged in the (now free) L
+        // window one side at a time; one (body, dof) item per lane and slot
+        const int ni = 6 * nq;
+        int4 bi[kWinItemsPerLane];
+        double acc[kWinItemsPerLane];
+#pragma unroll
+        for (int i = 0; i < kWinItemsPerLane; ++i) {
+            const int e = lane + 64 * i, ec = e < ni ? e : ni - 1;
+            const int q = qb + ec / 6, k = ec - (ec / 6) * 6;
+            bi[i] = bi4[q];
+            acc[i] = tr[(size_t)bi[i].x * 6 + k];
+        }
+        double* P = sL;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+#pragma unroll
+            for (int u = 0; u < kWinRowsPerLane; ++u) {
+                const int r = rb + lane + 64 * u;
+                if (r < re)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) P[(r - rb) * 6 + k] = w[u][6 * side + k] * lv[u];
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < kWinItemsPerLane; ++i) {
+                const int e = lane + 64 * i, k = e - (e / 6) * 6;
+                if (e >= ni) continue;
+#pragma unroll
+                for (int ee = 0; ee < 2; ++ee) {
+                    const int enc = ee ? bi[i].z : bi[i].y;
+                    const bool use = enc >= 0 && (enc & 1) == side;
+                    const int r0 = use ? enc >> 5 : rb, m = use ? (enc >> 1) & 15 : 0;
+                    auto sum = [&](auto N) {
+                        constexpr int n = decltype(N)::value;
+                        double pv[n], a = acc[i];
+#pragma unroll
+                        for (int l = 0; l < n; ++l) pv[l] = P[(r0 - rb + (l < m ? l : 0)) * 6 + k];
+#pragma unroll
+                        for (int l = 0; l < n; ++l) a = l < m ? a - pv[l] : a;
+                        return a;
+                    };
+                    if constexpr (Widths) {
+                        acc[i] = with_width(kw, sum);
+                    } else {
+                        double pv[kMaxJRows];
+#pragma unroll
+                        for (int l = 0; l < kMaxJRows; ++l) pv[l] = P[(r0 - rb + (l < m ? l : 0)) * 6 + k];
+#pragma unroll
+                        for (int l = 0; l < kMaxJRows; ++l) acc[i] = l < m ? acc[i] - pv[l] : acc[i];
+                    }
+                }
+            }
+            wave_sync();
+        }
+        double e2 = 0;
+#pragma unroll
+        for (int i = 0; i < kWinItemsPerLane; ++i) {
+            const int e = lane + 64 * i, k = e - (e / 6) * 6;
+            if (e >= ni) continue;
+            const int b = bi[i].x;
+            double* xb = x + (size_t)b * kBodyStride;
+            const double dv = minv_k(minv, b, k) * acc[i];
+            const double u = xb[k];
+            const double un = setup ? dv : u + dv;
+            xb[k] = un;
+            if (check) {
+                const double qd = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                const double ee2 = (un - u) / qd;
+                e2 += ee2 * ee2;
+            }
+        }
+        return e2;
     } else {
+        // s = G M^-1 t (minus r_lambda at setup) into z
+        for (int r = rb + lane; r < re; r += 64) {
+            const int4 ri = ri4[r];
+            double s = 0;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int b = side ? ri.y : ri.x;
+                const int bc = b < 0 ? 0 : b;
+                const double* w = jrow(jr, r, side);
+                const double* t = tr + (size_t)bc * 6;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s = fma(b >= 0 ? w[k] : 0.0, minv_k(minv, bc, k) * t[k], s);
+            }
+            z[r] = setup ? s - rlam[r] : s;
+        }
+        wave_sync();
         // forward along the chain, a staged window at a time
         for (int j = j0; j < j1;) {
             const int je = stage_fwd_end(j, j1, j0, off, capL, capR);
@@ -400,7 +594,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             wave_copy(sL, Lm + loff(j), loff(je) - loff(j), lane);
             wave_copy(sR, z + rs, nr, lane);
             wave_sync();
-            fwd_blocks(j, je, loff(j), rs);
+            fwd_blocks(j, je, loff(j), rs, off, loff);
             wave_sync();
             wave_copy(z + rs, sR, nr, lane);
             wave_sync();
@@ -419,7 +613,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             wave_copy(sL, Lm + loff(je + 1), loff(j + 1) - loff(je + 1), lane);
             wave_copy(sR, yb + rs, nr, lane);
             wave_sync();
-            bwd_blocks(j, je, loff(je + 1), rs);
+            bwd_blocks(j, je, loff(je + 1), rs, off, loff);
             wave_sync();
             wave_copy(z + rs, sR, nr, lane);
             wave_sync();
@@ -428,8 +622,6 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
         for (int r = rb + lane; r < re; r += 64) lam[r] = setup ? z[r] : lam[r] + z[r];
     }
     // bodies: dv = M^-1 (r - sum over its blocks of G_b^T Lambda), one (body, dof) per lane
-    const int4* bi4 = reinterpret_cast<const int4*>(jbody_info);
-    const int qb = jcb_off[ch], nq = jcb_off[ch + 1] - qb;
     auto bodies = [&](auto lamat) -> double {
         double e2 = 0;
         for (int e = lane; e < 6 * nq; e += 64) {
@@ -447,7 +639,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
                     const int r = r0 + (l < m ? l : m - 1);
                     const double w = jrow(jr, r, side)[k];
                     const double lv = lamat(r);
-                    if (has && l < m) acc -= w * lv;
+                    acc = has && l < m ? fma(-w, lv, acc) : acc;
                 }
             }
             double* xb = x + (size_t)b * kBodyStride;
@@ -463,7 +655,6 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
         }
         return e2;
     };
-    if (one) return bodies([&](int r) { return sR[r - rb]; });
     return bodies([&](int r) { return z[r]; });
 }
 
