@@ -36,6 +36,8 @@ struct BatchJoints {
     const int* jr_kind;
     const int* jr_blk;
     const double* jr;
+    const double* jw;
+    int nrow;
     double* rlam;
     double* Lm;
     double* Dinv;
@@ -93,10 +95,10 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv) {
                 const int ch = ja.sc_list[q];
                 if (joint_factor_chain(ch, lane, ja.jc_off, ja.jb_off, ja.jb_body, ja.jr_kind, ja.jr_blk, ja.jr, minv, v, dt,
-                                       ja.theta, ja.beta, ja.rlam, ja.jl_off, ja.Lm, ja.Dinv, jlds[threadIdx.x >> 6]))
+                                       ja.theta, ja.beta, ja.rlam, ja.jl_off, ja.Lm, ja.Dinv, ja.nrow, jlds[threadIdx.x >> 6]))
                     gfail[g] = 1;
                 else
-                    joint_solve_chain<false>(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.rlam, ja.Lm,
+                    joint_solve_chain<false>(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.jw, ja.nrow, ja.rlam, ja.Lm,
                                       ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel, tol_abs,
                                       jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
@@ -124,7 +126,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             if (!done) {
                 for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv)
                     e2 += joint_solve_chain<false>(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info,
-                                            ja.jr, ja.rlam, ja.Lm, ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x,
+                                            ja.jr, ja.jw, ja.nrow, ja.rlam, ja.Lm, ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x,
                                             tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
             }
         }
@@ -206,7 +208,7 @@ static BatchJoints batch_joints(cdx_ctx* ctx, double theta) {
     BatchJoints ja{};
     if (ctx->njrows == 0) return ja;
     ja = BatchJoints{ctx->d_sc_off,  ctx->d_sc_list,   ctx->d_jc_off,  ctx->d_jcb_off,     ctx->d_jb_off, ctx->d_jl_off, ctx->d_jb_body,
-                     ctx->d_jr_kind, ctx->d_jr_blk,    ctx->d_jr,      ctx->d_jrlam,       ctx->d_jL,     ctx->d_jDinv,
+                     ctx->d_jr_kind, ctx->d_jr_blk,    ctx->d_jr,      ctx->d_jw,          ctx->njrows,   ctx->d_jrlam,       ctx->d_jL,     ctx->d_jDinv,
                      ctx->d_jbody,   ctx->d_jbody_blk, ctx->d_jr_info, ctx->d_jbody_info,  ctx->d_jslot,  ctx->d_jtr,
                      ctx->d_jz,      ctx->d_jy,        ctx->d_jlam,    theta,              ctx->joint_bias};
     return ja;

@@ -81,6 +81,8 @@ struct cdx_ctx {
     size_t jr_info_cap = 0, jbody_info_cap = 0;
     int* d_jl_off = nullptr;  // [nblocks+1] packed offset of each block's L factor in d_jL
     size_t jl_off_cap = 0;
+    double* d_jw = nullptr;  // [12*njrows] row coefficients (wa, wb) by coefficient: d_jw[k * njrows + r]
+    size_t jw_cap = 0;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
     double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)

@@ -32,13 +32,13 @@ __global__ void __launch_bounds__(64) k_joint_factor(int nchains, const int* __r
                                                      const double* __restrict__ minv, const double* __restrict__ v, double dt,
                                                      double theta, double beta, double* __restrict__ rlam,
                                                      const int* __restrict__ jl_off, double* __restrict__ Lm,
-                                                     double* __restrict__ Dinv,
+                                                     double* __restrict__ Dinv, int nrow,
                                                      int* __restrict__ ctrl) {
     __shared__ double lds[kFactorLds];
     const int ch = blockIdx.x;
     if (ch >= nchains) return;
     if (joint_factor_chain(ch, threadIdx.x, jc_off, jb_off, jb_body, jr_kind, jr_blk, jr, minv, v, dt, theta, beta, rlam, jl_off,
-                           Lm, Dinv, lds) &&
+                           Lm, Dinv, nrow, lds) &&
         threadIdx.x == 0) {
         ctrl[2] = CDX_NOT_FACTORIZED;
         ctrl[0] = 1;
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) k_joint_factor(int nchains, const int* __r
 __global__ void __launch_bounds__(64) k_joint_solve(
     int nchains, int setup, int check, const int* __restrict__ jc_off, const int* __restrict__ jcb_off,
     const int* __restrict__ jb_off, const int* __restrict__ jl_off, const int* __restrict__ jr_info,
-    const double* __restrict__ jr,
+    const double* __restrict__ jr, const double* __restrict__ jw, int nrow,
     const double* __restrict__ rlam, const double* __restrict__ Lm, const double* __restrict__ Dinv,
     const int* __restrict__ jbody_info, const double* __restrict__ minv, const double* __restrict__ tr,
     double* __restrict__ z, double* __restrict__ lam,
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(64) k_joint_solve(
     if (ctrl && ctrl[0]) return;
     const int ch = blockIdx.x;
     if (ch >= nchains) return;
-    double e2 = joint_solve_chain<true>(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jl_off, jr_info, jr, rlam, Lm, Dinv,
+    double e2 = joint_solve_chain<true>(ch, threadIdx.x, setup, check, jc_off, jcb_off, jb_off, jl_off, jr_info, jr, jw, nrow, rlam, Lm, Dinv,
                                   jbody_info, minv, tr, z, yb, lam, x, tol_rel, tol_abs, lds, kSolveLdsL, kSolveLdsR);
     if (check) {
         for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
@@ -71,14 +71,14 @@ int joints_factor(cdx_ctx* ctx, double dt, double theta) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_factor", k_joint_factor, ctx->nchains, 64, ctx->nchains, ctx->d_jc_off, ctx->d_jb_off,
                ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jr_blk, ctx->d_jr, ctx->d_minv, ctx->d_v, dt, theta, ctx->joint_bias,
-               ctx->d_jrlam, ctx->d_jl_off, ctx->d_jL, ctx->d_jDinv, ctx->d_ctrl);
+               ctx->d_jrlam, ctx->d_jl_off, ctx->d_jL, ctx->d_jDinv, ctx->njrows, ctx->d_ctrl);
     return hip_check(ctx, hipGetLastError(), "joint factor launch");
 }
 
 int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, setup, check, ctx->d_jc_off, ctx->d_jcb_off,
-               ctx->d_jb_off, ctx->d_jl_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
+               ctx->d_jb_off, ctx->d_jl_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jw, ctx->njrows, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
                ctx->d_jtr, ctx->d_jz, ctx->d_jlam, ctx->d_jy, ctx->d_x, tol_rel, tol_abs, partial,
                setup ? nullptr : ctx->d_ctrl);
     return hip_check(ctx, hipGetLastError(), "joint solve launch");
@@ -89,7 +89,7 @@ int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_
 int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_solve", k_joint_solve, ctx->nchains, 64, ctx->nchains, 0, 0, ctx->d_jc_off, ctx->d_jcb_off,
-               ctx->d_jb_off, ctx->d_jl_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
+               ctx->d_jb_off, ctx->d_jl_off, ctx->d_jr_info, ctx->d_jr, ctx->d_jw, ctx->njrows, ctx->d_jrlam, ctx->d_jL, ctx->d_jDinv, ctx->d_jbody_info, ctx->d_minv,
                ctx->d_jtr, ctx->d_jz, lam_out, ctx->d_jy, x_out, 0.0, 0.0, nullptr, ctrl);
     return hip_check(ctx, hipGetLastError(), "joint apply launch");
 }
@@ -272,12 +272,21 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     CDX_TRY(upi(&ctx->d_jslot, jslot, &ctx->jslot_cap, "joint slots"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
     CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
+    // the same coefficients by coefficient, so that a wave reading one coefficient of 64 rows
+    // touches 4 cache lines instead of 64
+    {
+        std::vector<double> jw(12 * R);
+        for (size_t r = 0; r < R; ++r)
+            for (int k = 0; k < 12; ++k) jw[k * R + r] = jr[r * kJRowStride + k];
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, jw.size(), &ctx->jw_cap, "joint rows"));
+        CDX_HIP(ctx, hipMemcpy(ctx->d_jw, jw.data(), sizeof(double) * jw.size(), hipMemcpyHostToDevice));
+    }
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jrlam, R, &ctx->jrlam_cap, "joint rhs"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jz, R, &ctx->jz_cap, "joint scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jy, R, &ctx->jy_cap, "joint scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jlam, R, &ctx->jlam_cap, "joint impulses"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jL, (size_t)std::max(jl_off.back(), 1), &ctx->jL_cap, "joint factors"));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_jDinv, (size_t)nblk * kMaxJRows * kMaxJRows, &ctx->jDinv_cap, "joint factors"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jDinv, (size_t)kMaxJRows * R, &ctx->jDinv_cap, "joint factors"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jtr, (size_t)nb * 6, &ctx->jtr_cap, "joint body rhs"));
     ctx->h_jorig = jorig;
     ctx->h_jc_off = chain_off;

@@ -88,7 +88,7 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
                                          const double* __restrict__ minv, const double* __restrict__ v, double dt,
                                          double theta, double beta, double* __restrict__ rlam,
                                          const int* __restrict__ jl_off, double* __restrict__ Lm,
-                                         double* __restrict__ Dinv, double* __restrict__ lds) {
+                                         double* __restrict__ Dinv, int nrow, double* __restrict__ lds) {
     const int j0 = jc_off[ch], j1 = jc_off[ch + 1];
     for (int r = jb_off[j0] + lane; r < jb_off[j1]; r += 64)
         joint_rhs_row(r, jr_blk[r], jb_body, jr_kind, jr, v, dt, theta, beta, rlam);
@@ -181,12 +181,12 @@ __device__ inline int joint_factor_chain(int ch, int lane, const int* __restrict
             }
             wave_sync();
         }
-        // D_j^-1 row i at Dinv[j kJBlk + i kMaxJRows], zero beyond column m: a substitution may sum
+        // entry (i, l) of D_j^-1 at Dinv[l nrow + r0 + i] (by column, so that the rows of a wave
+        // read each column in one coalesced load), zero beyond column m: a substitution may sum
         // all kMaxJRows columns of a row
-        double* Di = Dinv + (size_t)j * kJBlk;
         for (int e = lane; e < m * kMaxJRows; e += 64) {
             const int i = e / kMaxJRows, l = e - (e / kMaxJRows) * kMaxJRows;
-            Di[e] = l < m ? A[i * w2 + m + l] : 0.0;
+            Dinv[(size_t)l * nrow + r0 + i] = l < m ? A[i * w2 + m + l] : 0.0;
         }
         for (int e = lane; e < m * m; e += 64) {
             const int i = e / m, l = e - (e / m) * m;
@@ -269,6 +269,72 @@ __device__ inline double lane_dot_n(int n, const double* __restrict__ a, int str
     return with_width(n, [&](auto N) { return lane_dot<decltype(N)::value>(a, stride, v); });
 }
 
+// The two recurrences along a chain of nblk blocks that all have W rows, held in LDS: block b's
+// rows at sR[b W, b W + W), L_b (b >= 1, W x W row-major) at sL[(b - 1) W W].  The next block's L
+// row (column, going back) and row value are read before the current step's FMAs, so LDS
+// latency stays off the chain of dependent steps.  Same arithmetic, in the same order, as the
+// general steps.
+//   forward:  z_b = s_b - L_b z_{b-1}          (z_0 = s_0)
+template <int W>
+__device__ inline void fwd_uniform(int nblk, int lane, const double* __restrict__ sL, double* __restrict__ sR) {
+    const bool act = lane < W;
+    const int li = act ? lane : 0;
+    double zp = sR[li];
+    double lr[W], sn = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) lr[l] = nblk > 1 ? sL[li * W + l] : 0.0;
+    if (nblk > 1) sn = sR[W + li];
+    for (int b = 1; b < nblk; ++b) {
+        double cl[W];
+#pragma unroll
+        for (int l = 0; l < W; ++l) cl[l] = lr[l];
+        const double cs = sn;
+        if (b + 1 < nblk) {
+            const double* ln = sL + b * W * W + li * W;
+#pragma unroll
+            for (int l = 0; l < W; ++l) lr[l] = ln[l];
+            sn = sR[(b + 1) * W + li];
+        }
+        double t = 0;
+#pragma unroll
+        for (int l = 0; l < W; ++l) t = fma(cl[l], readlane_d(zp, l), t);
+        const double zc = cs - t;
+        if (act) sR[b * W + lane] = zc;
+        zp = act ? zc : 0.0;
+    }
+}
+
+//   backward: Lambda_b = y_b - L_{b+1}^T Lambda_{b+1}   (Lambda_{nblk-1} = y_{nblk-1})
+template <int W>
+__device__ inline void bwd_uniform(int nblk, int lane, const double* __restrict__ sL, double* __restrict__ sR) {
+    const bool act = lane < W;
+    const int li = act ? lane : 0;
+    double ln = sR[(nblk - 1) * W + li];
+    double lc[W], yn = 0;
+    const int b0 = nblk - 2;
+#pragma unroll
+    for (int i = 0; i < W; ++i) lc[i] = b0 >= 0 ? sL[b0 * W * W + i * W + li] : 0.0;
+    if (b0 >= 0) yn = sR[b0 * W + li];
+    for (int b = b0; b >= 0; --b) {
+        double cl[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) cl[i] = lc[i];
+        const double cy = yn;
+        if (b >= 1) {
+            const double* lcol = sL + (b - 1) * W * W + li;
+#pragma unroll
+            for (int i = 0; i < W; ++i) lc[i] = lcol[i * W];
+            yn = sR[(b - 1) * W + li];
+        }
+        double t = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) t = fma(cl[i], readlane_d(ln, i), t);
+        const double lj = cy - t;
+        if (act) sR[b * W + lane] = lj;
+        ln = act ? lj : 0.0;
+    }
+}
+
 // dst[i] = src[i] for i < n by one wave; every load of a lane is issued before its first store,
 // so a copy costs about one memory latency per 512 elements instead of one per element.
 __device__ inline void wave_copy(double* __restrict__ dst, const double* __restrict__ src, int n, int lane) {
@@ -316,7 +382,8 @@ template <bool Widths>
 __device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
                                            const int* __restrict__ jl_off, const int* __restrict__ jr_info,
-                                           const double* __restrict__ jr, const double* __restrict__ rlam,
+                                           const double* __restrict__ jr, const double* __restrict__ jw, int nrow,
+                                           const double* __restrict__ rlam,
                                            const double* __restrict__ Lm, const double* __restrict__ Dinv,
                                            const int* __restrict__ jbody_info, const double* __restrict__ minv,
                                            const double* __restrict__ tr, double* __restrict__ z, double* __restrict__ yb,
@@ -406,21 +473,26 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     // y_r = (D_j^-1 z_j)_i for row r = r0 + i of block j, z read through zat
     auto y_row = [&](int r, auto zat) -> double {
         const int4 ri = ri4[r];
-        const int r0 = ri.w >> 4, m = ri.w & 15, i = r - r0;
-        const double* Di = Dinv + (size_t)ri.z * kJBlk + i * kMaxJRows;  // zero beyond column m
-        double t = 0;
+        const int r0 = ri.w >> 4, m = ri.w & 15;
+        double t = 0;  // D^-1 is zero beyond column m
 #pragma unroll
-        for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[l], zat(r0 + (l < m ? l : m - 1)), t);
+        for (int l = 0; l < kMaxJRows; ++l) t = fma(Dinv[(size_t)l * nrow + r], zat(r0 + (l < m ? l : m - 1)), t);
         return t;
     };
     if (one) {
         // widest block of the chain (wave-uniform)
         int kw = kMaxJRows;
+        bool same = false;  // every block of the chain has kw rows
         if constexpr (Widths) {
-            const int bnext = __shfl_down(boff, 1, 64);  // every lane takes part in the shuffle
+            const int bnext = __shfl_down(boff, 1, 64);  // every lane takes part in the shuffles
             int mw = lane < j1 - j0 ? bnext - boff : 0;
-            for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o, 64));
+            int mn = lane < j1 - j0 ? bnext - boff : kMaxJRows + 1;
+            for (int o = 32; o > 0; o >>= 1) {
+                mw = max(mw, __shfl_xor(mw, o, 64));
+                mn = min(mn, __shfl_xor(mn, o, 64));
+            }
             kw = __builtin_amdgcn_readfirstlane(mw);
+            same = __builtin_amdgcn_readfirstlane(mn) == kw;
         }
         // this lane's rows rb + lane + 64 u: records and coefficients stay in registers
         int4 ri[kWinRowsPerLane];
@@ -430,7 +502,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
             ri[u] = ri4[rc];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) w[u][k] = jr[(size_t)rc * kJRowStride + k];
+            for (int k = 0; k < 12; ++k) w[u][k] = jw[(size_t)k * nrow + rc];
         }
         // s = G M^-1 t (minus r_lambda at setup)
 #pragma unroll
@@ -453,19 +525,25 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
         }
         wave_copy(sL, Lm + lb, le - lb, lane);
         wave_sync();
-        fwd_blocks(j0, j1, lb, rb, off_r, loff_r);
+        if (same)
+            with_width(kw, [&](auto N) {
+                fwd_uniform<decltype(N)::value>(j1 - j0, lane, sL, sR);
+                return 0;
+            });
+        else
+            fwd_blocks(j0, j1, lb, rb, off_r, loff_r);
         wave_sync();
         double yv[kWinRowsPerLane];
 #pragma unroll
         for (int u = 0; u < kWinRowsPerLane; ++u) {
             const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
-            const int r0 = ri[u].w >> 4, m = ri[u].w & 15, i = rc - r0;
-            const double* Di = Dinv + (size_t)ri[u].z * kJBlk + i * kMaxJRows;  // zero beyond column m
+            const int r0 = ri[u].w >> 4, m = ri[u].w & 15;
+            const double* Di = Dinv + rc;  // column l at Di[l nrow]; zero beyond column m
             auto row = [&](auto N) {
                 constexpr int n = decltype(N)::value;
                 double d[n], t = 0;
 #pragma unroll
-                for (int l = 0; l < n; ++l) d[l] = Di[l];
+                for (int l = 0; l < n; ++l) d[l] = Di[(size_t)l * nrow];
 #pragma unroll
                 for (int l = 0; l < n; ++l) t = fma(d[l], sR[r0 - rb + (l < m ? l : m - 1)], t);
                 return t;
@@ -475,7 +553,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             } else {
                 double t = 0;
 #pragma unroll
-                for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[l], sR[r0 - rb + (l < m ? l : m - 1)], t);
+                for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[(size_t)l * nrow], sR[r0 - rb + (l < m ? l : m - 1)], t);
                 yv[u] = t;
             }
         }
@@ -486,10 +564,17 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             if (r < re) sR[r - rb] = yv[u];
         }
         wave_sync();
-        const int rl = off_r(j1 - 1);
-        lnext = lane < re - rl ? sR[rl - rb + lane] : 0.0;
-        mnx = re - rl;
-        bwd_blocks(j1 - 1, j0, lb, rb, off_r, loff_r);
+        if (same) {
+            with_width(kw, [&](auto N) {
+                bwd_uniform<decltype(N)::value>(j1 - j0, lane, sL, sR);
+                return 0;
+            });
+        } else {
+            const int rl = off_r(j1 - 1);
+            lnext = lane < re - rl ? sR[rl - rb + lane] : 0.0;
+            mnx = re - rl;
+            bwd_blocks(j1 - 1, j0, lb, rb, off_r, loff_r);
+        }
         wave_sync();
         double lv[kWinRowsPerLane];
 #pragma unroll
