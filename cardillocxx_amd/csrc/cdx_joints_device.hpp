@@ -244,29 +244,36 @@ __device__ inline double lane_dot(const double* __restrict__ a, int stride, doub
     return t;
 }
 
-// f(std::integral_constant<int, n>) for a wave-uniform n in [1, kMaxJRows]: one straight-line
-// body per block width instead of a loop of runtime length
+// Block widths with a straight-line body of their own (a spherical joint: 3 rows, with its
+// dampers: 6); any other width takes the kMaxJRows-wide form, masked to its width.
+__device__ inline bool special_width(int n) { return n == 3 || n == 6; }
+
+// f(std::integral_constant<int, N>) for a wave-uniform n in [1, kMaxJRows], with N = n for the
+// special widths and N = kMaxJRows >= n otherwise: f must mask columns beyond n itself.
 template <typename F>
 __device__ inline auto with_width(int n, F&& f) {
     using std::integral_constant;
     switch (n) {
-        case 1: return f(integral_constant<int, 1>{});
-        case 2: return f(integral_constant<int, 2>{});
         case 3: return f(integral_constant<int, 3>{});
-        case 4: return f(integral_constant<int, 4>{});
-        case 5: return f(integral_constant<int, 5>{});
         case 6: return f(integral_constant<int, 6>{});
-        case 7: return f(integral_constant<int, 7>{});
-        case 8: return f(integral_constant<int, 8>{});
-        case 9: return f(integral_constant<int, 9>{});
-        case 10: return f(integral_constant<int, 10>{});
-        case 11: return f(integral_constant<int, 11>{});
         default: return f(integral_constant<int, kMaxJRows>{});
     }
 }
 
+// lane_dot for a wave-uniform n in [1, kMaxJRows]; reads a[l * stride] for l < n only
 __device__ inline double lane_dot_n(int n, const double* __restrict__ a, int stride, double v) {
-    return with_width(n, [&](auto N) { return lane_dot<decltype(N)::value>(a, stride, v); });
+    if (n == 3) return lane_dot<3>(a, stride, v);
+    if (n == 6) return lane_dot<6>(a, stride, v);
+    double ar[kMaxJRows];
+#pragma unroll
+    for (int l = 0; l < kMaxJRows; ++l) ar[l] = a[(l < n ? l : n - 1) * stride];
+    double t = 0;
+#pragma unroll
+    for (int l = 0; l < kMaxJRows; ++l) {
+        const double vl = readlane_d(v, l);
+        t = l < n ? fma(ar[l], vl, t) : t;
+    }
+    return t;
 }
 
 // The two recurrences along a chain of nblk blocks that all have W rows, held in LDS: block b's
@@ -482,7 +489,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     if (one) {
         // widest block of the chain (wave-uniform)
         int kw = kMaxJRows;
-        bool same = false;  // every block of the chain has kw rows
+        bool same = false;  // every block of the chain has kw rows, a special width
         if constexpr (Widths) {
             const int bnext = __shfl_down(boff, 1, 64);  // every lane takes part in the shuffles
             int mw = lane < j1 - j0 ? bnext - boff : 0;
@@ -492,7 +499,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
                 mn = min(mn, __shfl_xor(mn, o, 64));
             }
             kw = __builtin_amdgcn_readfirstlane(mw);
-            same = __builtin_amdgcn_readfirstlane(mn) == kw;
+            same = __builtin_amdgcn_readfirstlane(mn) == kw && special_width(kw);
         }
         // this lane's rows rb + lane + 64 u: records and coefficients stay in registers
         int4 ri[kWinRowsPerLane];

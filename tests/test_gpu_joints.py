@@ -79,6 +79,30 @@ def test_chains_on_terrain_fixed_sweeps(ctx, sweeps):
     assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL and rel(lam, lam_ref) <= TOL
 
 
+def drop_rows(rows, keep_prob, seed):
+    """A random subset of each joint's spring and damper rows: blocks of 0..6 rows, mixed along
+    the chains (widths other than a full joint's 3 or 6)."""
+    rng = np.random.default_rng(seed)
+    ns, nd = int(rows["ns"]), int(rows["nd"])
+    ks, kd = rng.random(ns) < keep_prob, rng.random(nd) < keep_prob
+    sel = np.concatenate([np.flatnonzero(ks), ns + np.flatnonzero(kd)])
+    out = {k: np.asarray(rows[k])[sel] for k in ("body_a", "body_b", "wa", "wb", "comp", "vsrc", "gerr")}
+    out.update(ns=int(ks.sum()), nd=int(kd.sum()))
+    return out
+
+
+@pytest.mark.parametrize("sweeps", [1, 23])
+def test_mixed_block_widths_match_oracle(ctx, sweeps):
+    prob, rows = chain_case(nchains=4, links=9, seed=5)
+    rows = drop_rows(rows, 0.6, seed=11)
+    sizes = np.unique(np.stack([rows["body_a"], rows["body_b"]], 1), axis=0, return_counts=True)[1]
+    assert len(set(sizes.tolist())) >= 4  # several block widths, not only 3 and 6
+    (v, imp, lam, info), (v_ref, imp_ref, lam_ref, res) = run_both(ctx, prob, rows, 1e-3, max_iter=sweeps, tol_abs=0.0,
+                                                                  tol_rel=0.0)
+    assert info.iters == res.iters == sweeps
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL and rel(lam, lam_ref) <= TOL
+
+
 @pytest.mark.parametrize("tol", [1e-5, 1e-9])
 def test_chains_on_terrain_to_tolerance(ctx, tol):
     prob, rows = chain_case(seed=2, drop_chain=1)
