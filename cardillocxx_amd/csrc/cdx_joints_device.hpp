@@ -540,30 +540,28 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
         else
             fwd_blocks(j0, j1, lb, rb, off_r, loff_r);
         wave_sync();
+        // y = D^-1 z of this lane's rows, n >= the chain's widest block columns (D^-1 is zero
+        // beyond a block's width); one straight-line copy per width
         double yv[kWinRowsPerLane];
+        auto ycalc = [&](auto N) {
+            constexpr int n = decltype(N)::value;
 #pragma unroll
-        for (int u = 0; u < kWinRowsPerLane; ++u) {
-            const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
-            const int r0 = ri[u].w >> 4, m = ri[u].w & 15;
-            const double* Di = Dinv + rc;  // column l at Di[l nrow]; zero beyond column m
-            auto row = [&](auto N) {
-                constexpr int n = decltype(N)::value;
+            for (int u = 0; u < kWinRowsPerLane; ++u) {
+                const int r = rb + lane + 64 * u, rc = r < re ? r : re - 1;
+                const int r0 = ri[u].w >> 4, m = ri[u].w & 15;
                 double d[n], t = 0;
 #pragma unroll
-                for (int l = 0; l < n; ++l) d[l] = Di[(size_t)l * nrow];
+                for (int l = 0; l < n; ++l) d[l] = Dinv[(size_t)l * nrow + rc];
 #pragma unroll
                 for (int l = 0; l < n; ++l) t = fma(d[l], sR[r0 - rb + (l < m ? l : m - 1)], t);
-                return t;
-            };
-            if constexpr (Widths) {
-                yv[u] = with_width(kw, row);
-            } else {
-                double t = 0;
-#pragma unroll
-                for (int l = 0; l < kMaxJRows; ++l) t = fma(Di[(size_t)l * nrow], sR[r0 - rb + (l < m ? l : m - 1)], t);
                 yv[u] = t;
             }
-        }
+            return 0;
+        };
+        if constexpr (Widths)
+            with_width(kw, ycalc);
+        else
+            ycalc(std::integral_constant<int, kMaxJRows>{});
         wave_sync();
 #pragma unroll
         for (int u = 0; u < kWinRowsPerLane; ++u) {
@@ -613,35 +611,31 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
                     for (int k = 0; k < 6; ++k) P[(r - rb) * 6 + k] = w[u][6 * side + k] * lv[u];
             }
             wave_sync();
+            // each item subtracts the products of its block rows on this side (n >= the widest block)
+            auto isum = [&](auto N) {
+                constexpr int n = decltype(N)::value;
 #pragma unroll
-            for (int i = 0; i < kWinItemsPerLane; ++i) {
-                const int e = lane + 64 * i, k = e - (e / 6) * 6;
-                if (e >= ni) continue;
+                for (int i = 0; i < kWinItemsPerLane; ++i) {
+                    const int e = lane + 64 * i, k = e - (e / 6) * 6;
+                    if (e >= ni) continue;
 #pragma unroll
-                for (int ee = 0; ee < 2; ++ee) {
-                    const int enc = ee ? bi[i].z : bi[i].y;
-                    const bool use = enc >= 0 && (enc & 1) == side;
-                    const int r0 = use ? enc >> 5 : rb, m = use ? (enc >> 1) & 15 : 0;
-                    auto sum = [&](auto N) {
-                        constexpr int n = decltype(N)::value;
-                        double pv[n], a = acc[i];
+                    for (int ee = 0; ee < 2; ++ee) {
+                        const int enc = ee ? bi[i].z : bi[i].y;
+                        const bool use = enc >= 0 && (enc & 1) == side;
+                        const int r0 = use ? enc >> 5 : rb, m = use ? (enc >> 1) & 15 : 0;
+                        double pv[n];
 #pragma unroll
                         for (int l = 0; l < n; ++l) pv[l] = P[(r0 - rb + (l < m ? l : 0)) * 6 + k];
 #pragma unroll
-                        for (int l = 0; l < n; ++l) a = l < m ? a - pv[l] : a;
-                        return a;
-                    };
-                    if constexpr (Widths) {
-                        acc[i] = with_width(kw, sum);
-                    } else {
-                        double pv[kMaxJRows];
-#pragma unroll
-                        for (int l = 0; l < kMaxJRows; ++l) pv[l] = P[(r0 - rb + (l < m ? l : 0)) * 6 + k];
-#pragma unroll
-                        for (int l = 0; l < kMaxJRows; ++l) acc[i] = l < m ? acc[i] - pv[l] : acc[i];
+                        for (int l = 0; l < n; ++l) acc[i] = l < m ? acc[i] - pv[l] : acc[i];
                     }
                 }
-            }
+                return 0;
+            };
+            if constexpr (Widths)
+                with_width(kw, isum);
+            else
+                isum(std::integral_constant<int, kMaxJRows>{});
             wave_sync();
         }
         double e2 = 0;
