@@ -52,6 +52,7 @@ struct BatchJoints {
     double* lam;
     double theta;
     double beta;
+    int wave_lds;  // doubles of dynamic LDS per wave (>= kMinWaveLds): factorisation, then the chain window
 };
 
 // Joints = false compiles none of the chain code (no scratch for its per-lane block matrices).
@@ -70,8 +71,11 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     __shared__ int gdone[kMaxBatchBlock / 64];
     __shared__ int gfail[kMaxBatchBlock / 64];
     __shared__ int all_done;
-    // per wave: the chain factorisation, then the staging window of the chain solve
-    __shared__ double jlds[Joints ? kMaxBatchBlock / 64 : 1][Joints ? kFactorLds : 1];
+    // per wave (dynamic, ja.wave_lds doubles each): the chain factorisation, then the staging
+    // window of the chain solve
+    extern __shared__ double jdyn[];
+    double* jlds = jdyn + (size_t)(threadIdx.x >> 6) * (Joints ? ja.wave_lds : 0);
+    const int capR = kSolveLdsR, capL = Joints ? ja.wave_lds - kSolveLdsR : 0;
     const int gsize = blockDim.x / groups;
     const int g = threadIdx.x / gsize, lt = threadIdx.x - g * gsize;
     const int slot = blockIdx.x * groups + g;
@@ -95,12 +99,12 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv) {
                 const int ch = ja.sc_list[q];
                 if (joint_factor_chain(ch, lane, ja.jc_off, ja.jb_off, ja.jb_body, ja.jr_kind, ja.jr_blk, ja.jr, minv, v, dt,
-                                       ja.theta, ja.beta, ja.rlam, ja.jl_off, ja.Lm, ja.Dinv, ja.nrow, jlds[threadIdx.x >> 6]))
+                                       ja.theta, ja.beta, ja.rlam, ja.jl_off, ja.Lm, ja.Dinv, ja.nrow, jlds))
                     gfail[g] = 1;
                 else
-                    joint_solve_chain<false>(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.jw, ja.nrow, ja.rlam, ja.Lm,
+                    joint_solve_chain<true>(ch, lane, 1, 0, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info, ja.jr, ja.jw, ja.nrow, ja.rlam, ja.Lm,
                                       ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x, tol_rel, tol_abs,
-                                      jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
+                                      jlds, capL, capR);
             }
     }
     __syncthreads();
@@ -125,9 +129,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
             __syncthreads();
             if (!done) {
                 for (int q = ja.sc_off[s] + wv; q < ja.sc_off[s + 1]; q += nwv)
-                    e2 += joint_solve_chain<false>(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info,
+                    e2 += joint_solve_chain<true>(ja.sc_list[q], lane, 0, check, ja.jc_off, ja.jcb_off, ja.jb_off, ja.jl_off, ja.jr_info,
                                             ja.jr, ja.jw, ja.nrow, ja.rlam, ja.Lm, ja.Dinv, ja.jbody_info, minv, ja.tr, ja.z, ja.y, ja.lam, x,
-                                            tol_rel, tol_abs, jlds[threadIdx.x >> 6], kBatchLdsL, kBatchLdsR);
+                                            tol_rel, tol_abs, jlds, capL, capR);
             }
         }
         if (check) {
@@ -204,28 +208,54 @@ static int upload_scene_chains(cdx_ctx* ctx) {
     return CDX_OK;
 }
 
-static BatchJoints batch_joints(cdx_ctx* ctx, double theta) {
+static BatchJoints batch_joints(cdx_ctx* ctx, double theta, int wave_lds) {
     BatchJoints ja{};
     if (ctx->njrows == 0) return ja;
     ja = BatchJoints{ctx->d_sc_off,  ctx->d_sc_list,   ctx->d_jc_off,  ctx->d_jcb_off,     ctx->d_jb_off, ctx->d_jl_off, ctx->d_jb_body,
                      ctx->d_jr_kind, ctx->d_jr_blk,    ctx->d_jr,      ctx->d_jw,          ctx->njrows,   ctx->d_jrlam,       ctx->d_jL,     ctx->d_jDinv,
                      ctx->d_jbody,   ctx->d_jbody_blk, ctx->d_jr_info, ctx->d_jbody_info,  ctx->d_jslot,  ctx->d_jtr,
-                     ctx->d_jz,      ctx->d_jy,        ctx->d_jlam,    theta,              ctx->joint_bias};
+                     ctx->d_jz,      ctx->d_jy,        ctx->d_jlam,    theta,              ctx->joint_bias,
+                     wave_lds};
     return ja;
+}
+
+// Doubles of dynamic LDS per wave for a batched launch of `block` threads with joint rows: a
+// single-window chain solve's worth where the device's per-workgroup LDS allows it (32-link chains
+// of damped spherical joints then stay in LDS for the whole solve), less for large blocks.
+static int batch_wave_lds(cdx_ctx* ctx, int block, size_t* shm) {
+    if (ctx->max_lds_block == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        ctx->max_lds_block = v;
+    }
+    const int nwv = block / 64;
+    const int avail = ctx->max_lds_block - 512;  // the kernel's own sums and flags
+    const int per = std::min(kSolveLdsL + kSolveLdsR, avail / (int)(sizeof(double) * nwv));
+    *shm = (size_t)per * sizeof(double) * nwv;
+    return per;
 }
 
 static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int block, int groups) {
     const int ns = ctx->nscenes;
     const int* d = ctx->d_scene_i;
     const int grid = (ns + groups - 1) / groups;
+    size_t shm = 0;
+    const int wave_lds = ctx->njrows > 0 ? batch_wave_lds(ctx, block, &shm) : 0;
+    if (ctx->njrows > 0 && wave_lds < kMinWaveLds)
+        return set_err(ctx, CDX_BAD_ARG, "batched solve: too little LDS per wave for the joint chains at this block size");
 #define CDX_BATCH_ARGS                                                                                               \
     ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,     \
         ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv,         \
         ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha,                     \
         ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs,              \
-        ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta)
-    if (ctx->njrows > 0)
-        CDX_LAUNCH(ctx, "batch_pj", k_batch_pj<true>, grid, block, CDX_BATCH_ARGS);
+        ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta, wave_lds)
+    if (ctx->njrows > 0) {
+        if (shm > 64 * 1024)
+            CDX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_pj<true>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj<true>, grid, block, shm, CDX_BATCH_ARGS);
+    }
     else
         CDX_LAUNCH(ctx, "batch_pj", k_batch_pj<false>, grid, block, CDX_BATCH_ARGS);
 #undef CDX_BATCH_ARGS

@@ -83,6 +83,7 @@ struct cdx_ctx {
     size_t jl_off_cap = 0;
     double* d_jw = nullptr;  // [12*njrows] row coefficients (wa, wb) by coefficient: d_jw[k * njrows + r]
     size_t jw_cap = 0;
+    int max_lds_block = 0;  // LDS bytes a workgroup may use (queried at the first batched joint solve)
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
     double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)
@@ -203,18 +204,19 @@ int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl);
 // With profiling on, the start/stop events travel in the kernel's own dispatch
 // (hipExtLaunchKernelGGL), so the measured interval is the kernel's execution, not the gap
 // between separate marker packets.
-#define CDX_LAUNCH(ctx, name, kernel, grid, block, ...)                                                  \
+#define CDX_LAUNCH_SHM(ctx, name, kernel, grid, block, shm, ...)                                         \
     do {                                                                                                  \
         if ((ctx)->profiling) {                                                                           \
             hipEvent_t cdx_ev_start__, cdx_ev_stop__;                                                     \
             ::cdx::prof_events((ctx), &cdx_ev_start__, &cdx_ev_stop__);                                   \
-            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, cdx_ev_start__,     \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shm), (ctx)->stream, cdx_ev_start__, \
                                   cdx_ev_stop__, 0, __VA_ARGS__);                                         \
             ::cdx::prof_push((ctx), (name), cdx_ev_start__, cdx_ev_stop__);                               \
         } else {                                                                                          \
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__);           \
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shm), (ctx)->stream, __VA_ARGS__);       \
         }                                                                                                 \
     } while (0)
+#define CDX_LAUNCH(ctx, name, kernel, grid, block, ...) CDX_LAUNCH_SHM(ctx, name, kernel, grid, block, 0, __VA_ARGS__)
 
 #define CDX_TRY(expr)               \
     do {                            \

@@ -25,8 +25,9 @@ constexpr int kFactorLds = 3 * kJBlk + 2 * kJBlk + kMaxJRows;
 // staging window of the grid-wide chain solve (L entries, rows): a 32-link chain of spherical
 // joints with dampers (31 blocks of 6 rows) fits in one window
 constexpr int kSolveLdsL = 1536, kSolveLdsR = 192;
-// the batched kernel stages through each wave's factorisation LDS
-constexpr int kBatchLdsL = 600, kBatchLdsR = kFactorLds - kBatchLdsL;
+// the batched kernel gives each wave at least this many doubles of LDS (the factorisation's
+// and a staging window of at least one block)
+constexpr int kMinWaveLds = kFactorLds > kSolveLdsR + kJBlk ? kFactorLds : kSolveLdsR + kJBlk;
 
 // inverse mass of dof k of a rigid body
 __device__ inline double minv_k(const double* __restrict__ minv, int b, int k) {
@@ -364,7 +365,6 @@ __device__ inline void wave_copy(double* __restrict__ dst, const double* __restr
 // rows and (body, dof) items of one lane in the single-window solve
 constexpr int kWinRowsPerLane = kSolveLdsR / 64;
 constexpr int kWinItemsPerLane = 6;
-static_assert(kBatchLdsR <= kSolveLdsR, "row window beyond the single-window row count");
 
 // x += S^-1 [r; -r_lambda] restricted to the jointed bodies of chain ch, on one wave.
 //   setup = 1: x0 and Lambda_0 are set (absolute; tr holds the full body right-hand side);
@@ -382,9 +382,9 @@ static_assert(kBatchLdsR <= kSolveLdsR, "row window beyond the single-window row
 // jr_info per row: {body a, body b, block, r0 << 4 | m}; jbody_info per jointed body: {body,
 // rows of its first block, rows of its second block (-1: none), 0}, rows as r0 << 5 | m << 1 | side.
 // The first 64 block and L offsets of the chain sit in registers, one per lane, read by readlane.
-// Widths = true specialises the block steps and row sums on the (wave-uniform) block width: for
-// kernels where the chain's offsets are scalar values, i.e. where this function is inlined with
-// one chain per workgroup (k_joint_solve).  The batched kernel uses the fixed-width form.
+// Widths = true specialises the single-window path's block steps and row sums on the block
+// width, a scalar value there (register reads of the chain's offsets); the windowed path always
+// uses the fixed kMaxJRows-wide form.
 template <bool Widths>
 __device__ inline double joint_solve_chain(int ch, int lane, int setup, int check, const int* __restrict__ jc_off,
                                            const int* __restrict__ jcb_off, const int* __restrict__ jb_off,
@@ -423,14 +423,15 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     // z_j = s_j - L_j z_{j-1} over blocks [ja, jb) held in LDS from L offset lbase and row rs
     double zp = 0.0;
     int mp = 0;
-    // (block offsets through offf / lofff)
-    auto fwd_blocks = [&](int ja, int jb, int lbase, int rs, auto offf, auto lofff) {
+    // (block offsets through offf / lofff; Spec: width-specialised steps, only where offf and
+    // lofff are register reads, i.e. the widths are scalar values)
+    auto fwd_blocks = [&](int ja, int jb, int lbase, int rs, auto offf, auto lofff, auto Spec) {
         for (int jj = ja; jj < jb; ++jj) {
             const int r0 = offf(jj), m = offf(jj + 1) - r0;
             const bool act = lane < m;
             const int li = act ? lane : 0;
             double t = 0;
-            if constexpr (Widths) {
+            if constexpr (decltype(Spec)::value) {
                 if (jj > j0) t = lane_dot_n(mp, sL + lofff(jj) - lbase + li * mp, 1, zp);
             } else if (jj > j0) {
                 const int o = lofff(jj) - lbase + li * mp;
@@ -452,13 +453,13 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
     // Lambda_jj = y_jj - L_{jj+1}^T Lambda_{jj+1} for jj = jhi-1 down to jlo
     double lnext = 0.0;
     int mnx = 0;
-    auto bwd_blocks = [&](int jhi, int jlo, int lbase, int rs, auto offf, auto lofff) {
+    auto bwd_blocks = [&](int jhi, int jlo, int lbase, int rs, auto offf, auto lofff, auto Spec) {
         for (int jj = jhi - 1; jj >= jlo; --jj) {
             const int r0 = offf(jj), m = offf(jj + 1) - r0;
             const bool act = lane < m;
             const int li = act ? lane : 0;
             double t = 0;
-            if constexpr (Widths) {
+            if constexpr (decltype(Spec)::value) {
                 t = lane_dot_n(mnx, sL + lofff(jj + 1) - lbase + li, m, lnext);
             } else {
                 const int o = lofff(jj + 1) - lbase + li;
@@ -538,7 +539,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
                 return 0;
             });
         else
-            fwd_blocks(j0, j1, lb, rb, off_r, loff_r);
+            fwd_blocks(j0, j1, lb, rb, off_r, loff_r, std::integral_constant<bool, Widths>{});
         wave_sync();
         // y = D^-1 z of this lane's rows, n >= the chain's widest block columns (D^-1 is zero
         // beyond a block's width); one straight-line copy per width
@@ -578,7 +579,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             const int rl = off_r(j1 - 1);
             lnext = lane < re - rl ? sR[rl - rb + lane] : 0.0;
             mnx = re - rl;
-            bwd_blocks(j1 - 1, j0, lb, rb, off_r, loff_r);
+            bwd_blocks(j1 - 1, j0, lb, rb, off_r, loff_r, std::integral_constant<bool, Widths>{});
         }
         wave_sync();
         double lv[kWinRowsPerLane];
@@ -680,7 +681,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             wave_copy(sL, Lm + loff(j), loff(je) - loff(j), lane);
             wave_copy(sR, z + rs, nr, lane);
             wave_sync();
-            fwd_blocks(j, je, loff(j), rs, off, loff);
+            fwd_blocks(j, je, loff(j), rs, off, loff, std::false_type{});
             wave_sync();
             wave_copy(z + rs, sR, nr, lane);
             wave_sync();
@@ -699,7 +700,7 @@ __device__ inline double joint_solve_chain(int ch, int lane, int setup, int chec
             wave_copy(sL, Lm + loff(je + 1), loff(j + 1) - loff(je + 1), lane);
             wave_copy(sR, yb + rs, nr, lane);
             wave_sync();
-            bwd_blocks(j, je, loff(je + 1), rs, off, loff);
+            bwd_blocks(j, je, loff(je + 1), rs, off, loff, std::false_type{});
             wave_sync();
             wave_copy(z + rs, sR, nr, lane);
             wave_sync();
