@@ -744,6 +744,8 @@ def main():
     ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
     ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")
     ap.add_argument("--no-chains", action="store_true", help="skip the jointed capsule-chain leg")
+    ap.add_argument("--chains-only", action="store_true",
+                    help="run only the two jointed-chain legs and print them (for profiling; not the bench line)")
     ap.add_argument("--chain-scenes", type=int, default=2048)
     ap.add_argument("--chain-steps", type=int, default=150)
     ap.add_argument("--chain-drop", type=float, default=0.03, help="drop height above the terrain under each chain")
@@ -772,6 +774,10 @@ def main():
     copy_gbs = hip.copy_bandwidth()
     log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
 
+    if args.chains_only:
+        print(json.dumps({"jointed_chains": chains_leg(args, native, scenes),
+                          "jointed_chains_differentiable": chains_diff_leg(args, native, scenes)}), flush=True)
+        return
     out = pile_leg(args, native, scenes, metrics)
     out["roofline"]["copy_GBs"] = copy_gbs
     out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
