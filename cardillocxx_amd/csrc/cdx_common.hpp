@@ -84,6 +84,11 @@ struct cdx_ctx {
     double* d_jw = nullptr;  // [12*njrows] row coefficients (wa, wb) by coefficient: d_jw[k * njrows + r]
     size_t jw_cap = 0;
     int max_lds_block = 0;  // LDS bytes a workgroup may use (queried at the first batched joint solve)
+    // Anderson loop: history rings Z and F ((m+1) slots each) and the candidate, over the mixed
+    // vector [x; p]; least-squares coefficients; dot-product partials; ring state
+    double *d_and = nullptr, *d_and_gam = nullptr, *d_and_part = nullptr;
+    int* d_and_st = nullptr;
+    size_t and_cap = 0;
     double *d_jr = nullptr, *d_jrlam = nullptr, *d_jz = nullptr, *d_jlam = nullptr, *d_jL = nullptr, *d_jDinv = nullptr;
     double* d_jtr = nullptr;  // [nb*6] right-hand side of each jointed body for the chain solve
     double* d_jy = nullptr;   // [njrows] chain-solve scratch (D^-1 z)

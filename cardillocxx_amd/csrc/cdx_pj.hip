@@ -831,20 +831,289 @@ __global__ void k_pjc_finish_contacts(int nc, const int* __restrict__ ida, const
     else if (cheb[1] == 0.0) P[t] = cheb_extrap(cheb[0], P[t], PK[t]);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Anderson mixing (anderson_loop, projected_jacobi.cpp:224-310).
+//
+// The mixed vector z = [x; p] over the entries the reference has: x (b, k) for k < dof[b] (entry
+// i = 6 b + k), then the contact rows (entry 6 nb + k nc + c, row k < 1 or 3 of a contact with a
+// dynamic side).  History rings Z and F of S = m + 1 slots each hold (z_i, f_i = g(z_i) - z_i); the
+// state st = {cnt, head, mk, gamma ok, non-finite candidate} lives on the device, so a chunk of
+// iterations replays as one graph.  Per iteration, after the sweep w = g(z) (contact and body
+// passes, residual every iteration):
+//   push:      f = w - z into F[zs] (zs = the slot after head, which holds z), the dot products of the
+//              difference columns dF_j = F[s_{j+1}] - F[s_j] with each other and with f;
+//   solve:     gamma = argmin ||f - dF gamma|| by a pivoted Cholesky of the Gram matrix with the
+//              pivot rule of Eigen's ColPivHouseholderQR; advance the ring;
+//   candidate: w - (dZ + dF) gamma, flagging a non-finite entry;
+//   commit:    z_next = the candidate, or w (and the history dropped) when gamma or the candidate is
+//              not finite; z_next also goes into the ring slot after the new head.
+// The dot products and the least squares run in a different order from Eigen's Householder QR,
+// so the device matches the oracle to rounding, not bitwise.
+// ---------------------------------------------------------------------------------------------
+constexpr int kAndMaxM = 8;
+constexpr int kAndDots = kAndMaxM * (kAndMaxM + 1) / 2 + kAndMaxM;
+constexpr int kAndBlocks = 1024;  // grid of the dot-product pass (fixed: deterministic partials)
+
+// address of mixed entry i, or null where the reference has no such row
+__device__ inline double* and_entry(int i, int nb, int nc, const int* __restrict__ dof, const int* __restrict__ ida,
+                                    const int* __restrict__ idb, const double* __restrict__ mu, double* x, double* p) {
+    if (i < 6 * nb) {
+        const int b = i / 6, k = i - 6 * (i / 6);
+        return k < dof[b] ? x + (size_t)b * kBodyStride + k : nullptr;
+    }
+    const int t = i - 6 * nb, k = t / nc, c = t - (t / nc) * nc;
+    if ((ida[c] < 0 && idb[c] < 0) || k >= (mu[c] > 0 ? 3 : 1)) return nullptr;
+    return p + t;
+}
+
+// oldest-first slot j (0 .. cnt-1) of a ring whose newest entry is `head`
+__device__ inline int and_slot(int j, int cnt, int head, int S) { return (head - (cnt - 1) + j + 2 * S) % S; }
+
+__global__ void __launch_bounds__(kBlock) k_and_init(int n, int nb, int nc, const int* __restrict__ dof,
+                                                     const int* __restrict__ ida, const int* __restrict__ idb,
+                                                     const double* __restrict__ mu, double* x, double* p,
+                                                     double* __restrict__ Z0) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double* e = and_entry(i, nb, nc, dof, ida, idb, mu, x, p);
+        Z0[i] = e ? *e : 0.0;
+    }
+}
+
+template <int MK>
+__device__ inline void and_push_body(int n, int nb, int nc, const int* __restrict__ dof, const int* __restrict__ ida,
+                                     const int* __restrict__ idb, const double* __restrict__ mu, double* x, double* p,
+                                     double* __restrict__ Z, double* __restrict__ F, int S, int zs,
+                                     double* __restrict__ part) {
+    constexpr int ND = MK * (MK + 1) / 2 + MK;
+    double acc[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double* e = and_entry(i, nb, nc, dof, ida, idb, mu, x, p);
+        if (!e) continue;
+        const double f = *e - Z[(size_t)zs * n + i];
+        F[(size_t)zs * n + i] = f;
+        if constexpr (MK > 0) {
+            double Fo[MK + 1];  // F of the history, oldest first; the newest is f
+#pragma unroll
+            for (int j = 0; j < MK; ++j) Fo[j] = F[(size_t)((zs - MK + j + S) % S) * n + i];
+            Fo[MK] = f;
+            double dF[MK];
+#pragma unroll
+            for (int j = 0; j < MK; ++j) dF[j] = Fo[j + 1] - Fo[j];
+            int d = 0;
+#pragma unroll
+            for (int j = 0; j < MK; ++j) {
+#pragma unroll
+                for (int l = j; l < MK; ++l) acc[d++] += dF[j] * dF[l];
+            }
+#pragma unroll
+            for (int j = 0; j < MK; ++j) acc[d++] += dF[j] * f;
+        }
+    }
+    if constexpr (ND > 0) {
+        __shared__ double red[kBlock / 64][ND];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double v = acc[d];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[wv][d] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < ND) {
+            double v = 0;
+            for (int w = 0; w < kBlock / 64; ++w) v += red[w][threadIdx.x];
+            part[(size_t)blockIdx.x * kAndDots + threadIdx.x] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_and_push(int n, int nb, int nc, const int* __restrict__ dof,
+                                                     const int* __restrict__ ida, const int* __restrict__ idb,
+                                                     const double* __restrict__ mu, double* x, double* p,
+                                                     double* __restrict__ ring, int m, const int* __restrict__ st,
+                                                     double* __restrict__ part, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const int S = m + 1;
+    const int zs = (st[1] + 1) % S;
+    const int mk = min(st[0] + 1, S) - 1;
+    double* Z = ring;
+    double* F = ring + (size_t)S * n;
+    switch (mk) {
+        case 0: and_push_body<0>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 1: and_push_body<1>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 2: and_push_body<2>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 3: and_push_body<3>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 4: and_push_body<4>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 5: and_push_body<5>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 6: and_push_body<6>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        case 7: and_push_body<7>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+        default: and_push_body<8>(n, nb, nc, dof, ida, idb, mu, x, p, Z, F, S, zs, part); break;
+    }
+}
+
+// One workgroup: the Gram matrix G_jl = dF_j . dF_l and r_j = dF_j . f from the partials, then
+// gamma.  Pivot k takes the remaining column of largest norm (Schur complement diagonal of G);
+// as in ColPivHouseholderQR the solve uses the pivots before the first whose remaining norm^2 is
+// below (max column norm * eps)^2 (rows - k) / rows, the other coefficients are 0.  A pivot that
+// passes that test with a non-positive remaining norm^2 (possible only through the rounding of
+// the Gram matrix; the Householder solve would divide by a zero or rounding-level diagonal)
+// makes gamma non-finite, i.e. the plain step.  Advances the ring: head = zs, cnt += 1 (<= S).
+__global__ void __launch_bounds__(kBlock) k_and_solve(int nparts, const double* __restrict__ part, int m, double rows,
+                                                      int* __restrict__ st, double* __restrict__ gam,
+                                                      const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    __shared__ double dots[kAndDots];
+    const int S = m + 1;
+    const int cnt = min(st[0] + 1, S), mk = cnt - 1, zs = (st[1] + 1) % S;
+    const int nd = mk * (mk + 1) / 2 + mk;
+    for (int d = 0; d < nd; ++d) {
+        double v = 0;
+        for (int b = threadIdx.x; b < nparts; b += kBlock) v += part[(size_t)b * kAndDots + d];
+        v = block_sum(v, red);
+        if (threadIdx.x == 0) dots[d] = v;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double A[kAndMaxM][kAndMaxM], c[kAndMaxM], g[kAndMaxM];
+    int perm[kAndMaxM];
+    int d = 0;
+    for (int j = 0; j < mk; ++j)
+        for (int l = j; l < mk; ++l) A[j][l] = A[l][j] = dots[d++];
+    for (int j = 0; j < mk; ++j) c[j] = dots[d++];
+    for (int j = 0; j < mk; ++j) perm[j] = j;
+    double maxn2 = 0;
+    for (int j = 0; j < mk; ++j) maxn2 = fmax(maxn2, A[j][j]);
+    const double eps = 2.220446049250313e-16;
+    const double thr = (sqrt(maxn2) * eps) * (sqrt(maxn2) * eps) / rows;
+    int nz = mk, ok = 1;
+    for (int k = 0; k < mk; ++k) {
+        int big = k;
+        for (int j = k + 1; j < mk; ++j)
+            if (A[j][j] > A[big][big]) big = j;
+        if (A[big][big] < thr * (rows - k)) {
+            nz = k;
+            break;
+        }
+        if (big != k) {
+            for (int j = 0; j < mk; ++j) {
+                const double t = A[k][j]; A[k][j] = A[big][j]; A[big][j] = t;
+            }
+            for (int j = 0; j < mk; ++j) {
+                const double t = A[j][k]; A[j][k] = A[j][big]; A[j][big] = t;
+            }
+            const double t = c[k]; c[k] = c[big]; c[big] = t;
+            const int ti = perm[k]; perm[k] = perm[big]; perm[big] = ti;
+        }
+        if (!(A[k][k] > 0.0)) {
+            ok = 0;
+            break;
+        }
+        const double rkk = sqrt(A[k][k]);
+        A[k][k] = rkk;
+        for (int j = k + 1; j < mk; ++j) A[k][j] /= rkk;  // row k of R
+        for (int j = k + 1; j < mk; ++j)
+            for (int l = k + 1; l < mk; ++l) A[j][l] -= A[k][j] * A[k][l];
+    }
+    for (int j = 0; j < mk; ++j) g[j] = 0.0;
+    if (ok && nz > 0) {
+        double y[kAndMaxM];
+        for (int k = 0; k < nz; ++k) {  // R^T y = c
+            double t = c[k];
+            for (int i = 0; i < k; ++i) t -= A[i][k] * y[i];
+            y[k] = t / A[k][k];
+        }
+        for (int k = nz - 1; k >= 0; --k) {  // R gamma' = y
+            double t = y[k];
+            for (int j = k + 1; j < nz; ++j) t -= A[k][j] * g[j];
+            g[k] = t / A[k][k];
+        }
+        for (int k = 0; k < nz; ++k) ok = ok && isfinite(g[k]);
+    }
+    double out[kAndMaxM];
+    for (int j = 0; j < mk; ++j) out[j] = 0.0;
+    for (int k = 0; k < nz; ++k) out[perm[k]] = g[k];
+    for (int j = 0; j < mk; ++j) gam[j] = out[j];
+    st[0] = cnt;
+    st[1] = zs;
+    st[2] = mk;
+    st[3] = ok;
+    st[4] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_and_candidate(int n, int nb, int nc, const int* __restrict__ dof,
+                                                          const int* __restrict__ ida, const int* __restrict__ idb,
+                                                          const double* __restrict__ mu, double* x, double* p,
+                                                          const double* __restrict__ ring, int m, int* __restrict__ st,
+                                                          const double* __restrict__ gam, double* __restrict__ C,
+                                                          const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const int mk = st[2];
+    if (mk < 1 || !st[3]) return;
+    const int S = m + 1, cnt = st[0], head = st[1];
+    const double* Z = ring;
+    const double* F = ring + (size_t)S * n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double* e = and_entry(i, nb, nc, dof, ida, idb, mu, x, p);
+        if (!e) continue;
+        double t = 0.0;
+        for (int j = 0; j < mk; ++j) {
+            const size_t s0 = (size_t)and_slot(j, cnt, head, S) * n + i, s1 = (size_t)and_slot(j + 1, cnt, head, S) * n + i;
+            const double a = (Z[s1] - Z[s0]) + (F[s1] - F[s0]);
+            t += a * gam[j];
+        }
+        const double v = *e - t;
+        C[i] = v;
+        if (!isfinite(v)) atomicOr(st + 4, 1);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_and_commit(int n, int nb, int nc, const int* __restrict__ dof,
+                                                       const int* __restrict__ ida, const int* __restrict__ idb,
+                                                       const double* __restrict__ mu, double* x, double* p,
+                                                       double* __restrict__ ring, int m, int* __restrict__ st,
+                                                       const double* __restrict__ C, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const int S = m + 1, mk = st[2];
+    const bool ok = mk >= 1 && st[3] && !st[4];
+    const int zn = (st[1] + 1) % S;
+    double* Z = ring;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double* e = and_entry(i, nb, nc, dof, ida, idb, mu, x, p);
+        if (!e) {
+            Z[(size_t)zn * n + i] = 0.0;
+            continue;
+        }
+        double v = *e;
+        if (ok) {
+            v = C[i];
+            *e = v;
+        }
+        Z[(size_t)zn * n + i] = v;
+    }
+    if (mk >= 1 && !ok && blockIdx.x == 0 && threadIdx.x == 0) st[0] = 0;  // drop the history
+}
+
 }  // namespace cdx
 
 using namespace cdx;
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
 static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
+static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
 
 extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info) {
     // theta only enters the joint rows of S (PjAssembler::buildAndFactorS / rhs)
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
-    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV && prm->accel != CDX_ACCEL_CHEBYSHEV)
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV && prm->accel != CDX_ACCEL_CHEBYSHEV &&
+        prm->accel != CDX_ACCEL_ANDERSON)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: unknown acceleration mode");
+    if (prm->accel == CDX_ACCEL_ANDERSON && (prm->anderson_m < 1 || prm->anderson_m > kAndMaxM))
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: anderson_m must be in 1..8");
     const bool joints = ctx->njrows > 0;
     const bool block = prm->block_precond != 0;
     if (joints && prm->accel != CDX_ACCEL_NONE)
@@ -929,6 +1198,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         st = run_sweeps(ctx, prm->max_iter, 10, 4, nsweep, &iters);
     } else if (prm->accel == CDX_ACCEL_CHEBYSHEV) {
         st = run_chebyshev(ctx, prm, &iters);
+    } else if (prm->accel == CDX_ACCEL_ANDERSON) {
+        st = run_anderson(ctx, prm, &iters);
     } else {
         st = run_plain_sweeps(ctx, prm, gb, gc, &iters);
     }
@@ -1045,6 +1316,54 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     CDX_LAUNCH(ctx, "pjc_finish", k_pjc_finish_contacts, grid_for(3 * nc), kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_mu,
                ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_ctrl, ctx->d_cheb);
     CDX_HIP(ctx, hipGetLastError());
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CDX_OK;
+}
+
+// anderson_loop (projected_jacobi.cpp:224-310): plain sweeps in place on (x, p) with the residual
+// every iteration, then push / solve / candidate / commit; 8 iterations per graph chunk.
+static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out) {
+    const int nb = ctx->nb, nc = ctx->nc, m = prm->anderson_m, S = m + 1;
+    const int n = 6 * nb + 3 * nc;
+    const bool block = prm->block_precond != 0;
+    const int gb8 = grid_for(nb * kLanesPerBody);
+    const int gc4 = grid_for(nc * kLanesPerContact);
+    const int gn = std::min(grid_for(n), kAndBlocks);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_and, (size_t)(2 * S + 1) * n, &ctx->and_cap, "anderson history"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_and_gam, kAndMaxM, nullptr, "anderson coefficients"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_and_part, (size_t)kAndBlocks * kAndDots, nullptr, "anderson partials"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_and_st, 8, nullptr, "anderson state"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
+    double* ring = ctx->d_and;  // Z: S slots, F: S slots, then the candidate
+    double* cand = ctx->d_and + (size_t)2 * S * n;
+    const int st0[8] = {0, m, 0, 0, 0, 0, 0, 0};  // empty history, head = m so that z_0 goes to slot 0
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_and_st, st0, sizeof(st0), hipMemcpyHostToDevice, ctx->stream));
+    CDX_LAUNCH(ctx, "and_init", k_and_init, gn, kBlock, n, nb, nc, ctx->d_dof, ctx->d_ida, ctx->d_idb, ctx->d_mu, ctx->d_x,
+               ctx->d_p, ring);
+    CDX_HIP(ctx, hipGetLastError());
+    const double rows = (double)ctx->nv + (double)ctx->nrows;  // the rows of the reference's dF
+    auto launch_iter = [&](int it, int) -> int {
+        if (block)
+            CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<true>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_Rb, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        else
+            CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<false>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   ctx->d_imp, ctx->d_x, 1, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, nullptr, nullptr);
+        CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, 1));
+        CDX_LAUNCH(ctx, "and_push", k_and_push, gn, kBlock, n, nb, nc, ctx->d_dof, ctx->d_ida, ctx->d_idb, ctx->d_mu,
+                   ctx->d_x, ctx->d_p, ring, m, ctx->d_and_st, ctx->d_and_part, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "and_solve", k_and_solve, 1, kBlock, gn, ctx->d_and_part, m, rows, ctx->d_and_st, ctx->d_and_gam,
+                   ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "and_candidate", k_and_candidate, gn, kBlock, n, nb, nc, ctx->d_dof, ctx->d_ida, ctx->d_idb,
+                   ctx->d_mu, ctx->d_x, ctx->d_p, ring, m, ctx->d_and_st, ctx->d_and_gam, cand, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "and_commit", k_and_commit, gn, kBlock, n, nb, nc, ctx->d_dof, ctx->d_ida, ctx->d_idb, ctx->d_mu,
+                   ctx->d_x, ctx->d_p, ring, m, ctx->d_and_st, cand, ctx->d_ctrl);
+        return hip_check(ctx, hipGetLastError(), "pj anderson iteration launch");
+    };
+    const int st = run_sweeps_graphed(ctx, prm->max_iter, 8, true, launch_iter, iters_out);
+    if (st) return st;
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CDX_OK;
 }

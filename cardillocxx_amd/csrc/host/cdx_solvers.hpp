@@ -30,6 +30,10 @@ struct Config {  // hot-path keys of cardillo::config::Config (config.hpp:46-113
     bool pj_nesterov = false;
     double pj_nesterov_beta_threshold = 1.0;
     int pj_nesterov_restart_limit = 10000;
+    bool pj_chebyshev = false;
+    bool pj_anderson = false;
+    int pj_anderson_m = 2;
+    bool pj_rdiag_true_delassus = false;
     bool pj_warmstart = true;
     double moreau_theta = 1.0;
 
@@ -42,7 +46,11 @@ struct Config {  // hot-path keys of cardillo::config::Config (config.hpp:46-113
         p.tol_abs = pj_tol_abs;
         p.tol_rel = pj_tol_rel;
         p.warmstart = pj_warmstart ? 1 : 0;
-        p.accel = pj_nesterov ? CDX_ACCEL_NESTEROV : CDX_ACCEL_NONE;
+        // precedence pj_nesterov > pj_chebyshev > pj_anderson (projected_jacobi.cpp:469-472)
+        p.accel = pj_nesterov ? CDX_ACCEL_NESTEROV
+                              : (pj_chebyshev ? CDX_ACCEL_CHEBYSHEV : (pj_anderson ? CDX_ACCEL_ANDERSON : CDX_ACCEL_NONE));
+        p.anderson_m = pj_anderson_m < 1 ? 1 : pj_anderson_m;
+        p.block_precond = pj_rdiag_true_delassus ? 1 : 0;
         p.nesterov_beta_threshold = pj_nesterov_beta_threshold;
         p.nesterov_restart_limit = pj_nesterov_restart_limit;
         return p;

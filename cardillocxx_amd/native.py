@@ -33,6 +33,7 @@ STATUS_NAMES = {
 CDX_ACCEL_NONE = 0
 CDX_ACCEL_NESTEROV = 1
 CDX_ACCEL_CHEBYSHEV = 2
+CDX_ACCEL_ANDERSON = 3
 
 # Every symbol include/cdx/cdx.h declares.
 EXPORTED = [
@@ -80,6 +81,7 @@ class SolverParams(C.Structure):
         ("nesterov_beta_threshold", C.c_double),
         ("nesterov_restart_limit", C.c_int),
         ("block_precond", C.c_int),
+        ("anderson_m", C.c_int),
     ]
 
 

@@ -30,6 +30,10 @@ class Config:
     pj_nesterov: bool = False
     pj_nesterov_beta_threshold: float = 1.0
     pj_nesterov_restart_limit: int = 10000
+    pj_chebyshev: bool = False  # pj.chebyshev
+    pj_anderson: bool = False  # pj.anderson
+    pj_anderson_m: int = 2  # pj.anderson_m
+    pj_rdiag_true_delassus: bool = False  # pj.rdiag_true_delassus
     pj_warmstart: bool = True
     moreau_theta: float = 1.0
     collision_match_max_dist: float = 0.02
@@ -44,7 +48,11 @@ class Config:
         p.tol_rel = self.pj_tol_rel
         p.check_every = self.pj_check_every
         p.warmstart = 1 if self.pj_warmstart else 0
-        p.accel = native.CDX_ACCEL_NESTEROV if self.pj_nesterov else native.CDX_ACCEL_NONE
+        # precedence pj_nesterov > pj_chebyshev > pj_anderson (projected_jacobi.cpp:469-472)
+        p.accel = (native.CDX_ACCEL_NESTEROV if self.pj_nesterov else native.CDX_ACCEL_CHEBYSHEV if self.pj_chebyshev
+                   else native.CDX_ACCEL_ANDERSON if self.pj_anderson else native.CDX_ACCEL_NONE)
+        p.anderson_m = max(1, self.pj_anderson_m)
+        p.block_precond = 1 if self.pj_rdiag_true_delassus else 0
         p.nesterov_beta_threshold = self.pj_nesterov_beta_threshold
         p.nesterov_restart_limit = self.pj_nesterov_restart_limit
         return p

@@ -59,7 +59,8 @@ enum cdx_status {
 enum cdx_accel {
     CDX_ACCEL_NONE = 0,     /* standard_loop (projected_jacobi.cpp:371-377) */
     CDX_ACCEL_NESTEROV = 1, /* nesterov_loop (projected_jacobi.cpp:379-449) */
-    CDX_ACCEL_CHEBYSHEV = 2 /* chebyshev_loop (projected_jacobi.cpp:180-228), pj.chebyshev without pj.nesterov */
+    CDX_ACCEL_CHEBYSHEV = 2, /* chebyshev_loop (projected_jacobi.cpp:180-228), pj.chebyshev without pj.nesterov */
+    CDX_ACCEL_ANDERSON = 3   /* anderson_loop (projected_jacobi.cpp:224-310), pj.anderson without the two above */
 };
 
 /* Solver parameters; the Config keys they stand for (src/config/config.hpp:61-113). */
@@ -77,6 +78,7 @@ typedef struct {
     int block_precond; /* pj.rdiag_true_delassus (default 0): per-contact inverse Delassus blocks alpha (W_c M^-1 W_c^T)^-1
                           instead of the row scaling alpha / D_ii (projected_jacobi.cpp:71-98, 330-348); projected
                           Jacobi only (ignored by Gauss-Seidel, refused by the batched solve and by cdx_pj_adjoint) */
+    int anderson_m;    /* pj.anderson_m (default 2): Anderson history window, 1..8 */
 } cdx_solver_params;
 
 typedef struct {

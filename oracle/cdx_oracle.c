@@ -15,6 +15,7 @@
  */
 #include "cdx_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -407,6 +408,105 @@ static double estimate_rho(const orc_W* W, const orc_W* RW, const double* Rb, co
     return dmin(lambda, 0.995);
 }
 
+/* Householder vector of v = [c0; tail] (length n) in place (Eigen's makeHouseholderInPlace): v becomes
+ * [beta; essential], returns tau; tau = 0 when the tail is zero. */
+static double householder_in_place(int n, double* v) {
+    double tail2 = 0;
+    for (int i = 1; i < n; ++i) tail2 += v[i] * v[i];
+    const double c0 = v[0];
+    if (tail2 <= DBL_MIN) {
+        for (int i = 1; i < n; ++i) v[i] = 0;
+        return 0.0;
+    }
+    double beta = sqrt(c0 * c0 + tail2);
+    if (c0 >= 0) beta = -beta;
+    for (int i = 1; i < n; ++i) v[i] = v[i] / (c0 - beta);
+    v[0] = beta;
+    return (beta - c0) / beta;
+}
+
+/* y <- (I - tau [1; e][1; e]^T) y for y of length n (Eigen's applyHouseholderOnTheLeft) */
+static void apply_householder(int n, const double* e, double tau, double* y) {
+    if (n == 1) { y[0] *= 1.0 - tau; return; }
+    if (tau == 0.0) return;
+    double t = 0;
+    for (int i = 1; i < n; ++i) t += e[i] * y[i];
+    t += y[0];
+    y[0] -= tau * t;
+    for (int i = 1; i < n; ++i) y[i] -= (tau * e[i]) * t;
+}
+
+/* ColPivHouseholderQR::compute and solve (Eigen, the reference's pinned 5.0.1; Businger-Golub column
+ * pivoting on the downdated column norms, a pivot counted while the largest remaining column norm^2
+ * is at least (maxColNorm eps)^2 (rows - k) / rows) for the least-squares step of anderson_loop. */
+int orc_lstsq_colpiv(int rows, int cols, double* A, const double* b, double* g) {
+    const int size = rows < cols ? rows : cols;
+    double* nd = (double*)malloc(sizeof(double) * (size_t)(cols + 1)); /* direct norms */
+    double* nu = (double*)malloc(sizeof(double) * (size_t)(cols + 1)); /* updated norms */
+    double* tau = (double*)malloc(sizeof(double) * (size_t)(size + 1));
+    int* perm = (int*)malloc(sizeof(int) * (size_t)(cols + 1));
+    double* c = (double*)malloc(sizeof(double) * (size_t)(rows + 1));
+    double maxnorm = 0;
+    for (int j = 0; j < cols; ++j) {
+        double s = 0;
+        for (int i = 0; i < rows; ++i) s += A[(size_t)j * rows + i] * A[(size_t)j * rows + i];
+        nd[j] = nu[j] = sqrt(s);
+        perm[j] = j;
+        if (nd[j] > maxnorm) maxnorm = nd[j];
+    }
+    const double eps = DBL_EPSILON;
+    const double thr = (maxnorm * eps) * (maxnorm * eps) / (double)rows;
+    const double downdate_thr = sqrt(eps);
+    int nz = size;
+    for (int k = 0; k < size; ++k) {
+        int big = k;
+        for (int j = k + 1; j < cols; ++j)
+            if (nu[j] > nu[big]) big = j;
+        const double big2 = nu[big] * nu[big];
+        if (nz == size && big2 < thr * (double)(rows - k)) nz = k;
+        if (big != k) {
+            for (int i = 0; i < rows; ++i) {
+                const double t = A[(size_t)k * rows + i];
+                A[(size_t)k * rows + i] = A[(size_t)big * rows + i];
+                A[(size_t)big * rows + i] = t;
+            }
+            double t = nu[k]; nu[k] = nu[big]; nu[big] = t;
+            t = nd[k]; nd[k] = nd[big]; nd[big] = t;
+            const int ti = perm[k]; perm[k] = perm[big]; perm[big] = ti;
+        }
+        double* ck = A + (size_t)k * rows + k;
+        tau[k] = householder_in_place(rows - k, ck);
+        for (int j = k + 1; j < cols; ++j) apply_householder(rows - k, ck, tau[k], A + (size_t)j * rows + k);
+        for (int j = k + 1; j < cols; ++j) {
+            if (nu[j] == 0.0) continue;
+            double t = fabs(A[(size_t)j * rows + k]) / nu[j];
+            t = (1.0 + t) * (1.0 - t);
+            if (t < 0) t = 0;
+            const double t2 = t * (nu[j] / nd[j]) * (nu[j] / nd[j]);
+            if (t2 <= downdate_thr) {
+                double s = 0;
+                for (int i = k + 1; i < rows; ++i) s += A[(size_t)j * rows + i] * A[(size_t)j * rows + i];
+                nd[j] = nu[j] = sqrt(s);
+            } else {
+                nu[j] *= sqrt(t);
+            }
+        }
+    }
+    for (int j = 0; j < cols; ++j) g[j] = 0;
+    if (nz > 0) {
+        memcpy(c, b, sizeof(double) * (size_t)rows);
+        for (int k = 0; k < nz; ++k) apply_householder(rows - k, A + (size_t)k * rows + k, tau[k], c + k);
+        for (int i = nz - 1; i >= 0; --i) { /* R(0:nz, 0:nz) upper triangular */
+            double s = c[i];
+            for (int j = i + 1; j < nz; ++j) s -= A[(size_t)j * rows + i] * c[j];
+            c[i] = s / A[(size_t)i * rows + i];
+        }
+        for (int i = 0; i < nz; ++i) g[perm[i]] = c[i];
+    }
+    free(nd); free(nu); free(tau); free(perm); free(c);
+    return nz;
+}
+
 /* PJ solve: ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500) with build_workspace
  * (:316-369), standard_loop (:371-377) / nesterov_loop (:379-449), pj_sweep (:121-138),
  * rhs (pj_assembler.cpp:127-170, contact-only), solveS on S = diag(M) (pj_assembler.cpp:26-30,173-182),
@@ -523,7 +623,81 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];                 \
     } while (0)
 
-    if (!prm->nesterov && prm->chebyshev) {
+    if (!prm->nesterov && !prm->chebyshev && prm->anderson) {
+        /* anderson_loop (:242-310): z = [x; p]; one plain sweep w = g(z) per iteration, residual every
+         * iteration on the velocity part; the history keeps the last m + 1 pairs (z_i, f_i = g(z_i) - z_i)
+         * and z_{k+1} = w - (dZ + dF) gamma with gamma = argmin ||f_k - dF gamma|| (column-pivoted
+         * Householder QR); a non-finite gamma or candidate takes the plain step and drops the history. */
+        const int m = prm->anderson_m < 1 ? 1 : prm->anderson_m;
+        const int N = nv + nr;
+        double* Zh = (double*)malloc(sizeof(double) * (size_t)(m + 1) * N);
+        double* Fh = (double*)malloc(sizeof(double) * (size_t)(m + 1) * N);
+        double* z = (double*)malloc(sizeof(double) * (size_t)N);
+        double* w = (double*)malloc(sizeof(double) * (size_t)N);
+        double* dF = (double*)malloc(sizeof(double) * (size_t)m * N);
+        double* dZ = (double*)malloc(sizeof(double) * (size_t)m * N);
+        double* gam = (double*)malloc(sizeof(double) * (size_t)m);
+        double* cand = (double*)malloc(sizeof(double) * (size_t)N);
+        double* tmp = (double*)malloc(sizeof(double) * (size_t)N);
+        memcpy(z, x, sizeof(double) * nv);
+        memcpy(z + nv, p, sizeof(double) * nr);
+        int cnt = 0; /* pairs in the history, oldest at index 0 */
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(x, z, sizeof(double) * nv);
+            memcpy(p, z + nv, sizeof(double) * nr);
+            PJ_SWEEP();
+            memcpy(w, x, sizeof(double) * nv);
+            memcpy(w + nv, p, sizeof(double) * nr);
+            res->iters = iter + 1;
+            const double err = pj_residual(nv, x, xp, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (err <= 1) {
+                memcpy(z, w, sizeof(double) * N);
+                break;
+            }
+            if (cnt == m + 1) { /* pop_front after the push below */
+                memmove(Zh, Zh + N, sizeof(double) * (size_t)m * N);
+                memmove(Fh, Fh + N, sizeof(double) * (size_t)m * N);
+                --cnt;
+            }
+            memcpy(Zh + (size_t)cnt * N, z, sizeof(double) * N);
+            for (int i = 0; i < N; ++i) Fh[(size_t)cnt * N + i] = w[i] - z[i];
+            ++cnt;
+            const int mk = cnt - 1;
+            if (mk < 1) {
+                memcpy(z, w, sizeof(double) * N);
+                continue;
+            }
+            for (int j = 0; j < mk; ++j)
+                for (int i = 0; i < N; ++i) {
+                    dF[(size_t)j * N + i] = Fh[(size_t)(j + 1) * N + i] - Fh[(size_t)j * N + i];
+                    dZ[(size_t)j * N + i] = Zh[(size_t)(j + 1) * N + i] - Zh[(size_t)j * N + i];
+                }
+            const double* f = Fh + (size_t)mk * N;
+            /* (dZ + dF) before dF is overwritten by its factorisation */
+            for (int j = 0; j < mk; ++j)
+                for (int i = 0; i < N; ++i) dZ[(size_t)j * N + i] = dZ[(size_t)j * N + i] + dF[(size_t)j * N + i];
+            orc_lstsq_colpiv(N, mk, dF, f, gam);
+            int finite = 1;
+            for (int j = 0; j < mk; ++j) finite = finite && isfinite(gam[j]);
+            for (int i = 0; i < N; ++i) tmp[i] = 0;
+            for (int j = 0; j < mk; ++j)
+                for (int i = 0; i < N; ++i) tmp[i] += dZ[(size_t)j * N + i] * gam[j];
+            for (int i = 0; i < N; ++i) {
+                cand[i] = w[i] - tmp[i];
+                finite = finite && isfinite(cand[i]);
+            }
+            if (finite) {
+                memcpy(z, cand, sizeof(double) * N);
+            } else {
+                memcpy(z, w, sizeof(double) * N);
+                cnt = 0;
+            }
+        }
+        memcpy(x, z, sizeof(double) * nv);
+        memcpy(p, z + nv, sizeof(double) * nr);
+        free(Zh); free(Fh); free(z); free(w); free(dF); free(dZ); free(gam); free(cand); free(tmp);
+    } else if (!prm->nesterov && prm->chebyshev) {
         /* chebyshev_loop (:180-228): residual every sweep against the sweep's input; on
          * convergence the loop returns that input (x = xk1), not the swept iterate */
         const double rho = estimate_rho(&W, &RW, Rb, M, nv);
@@ -715,7 +889,8 @@ static double joint_row_dot(const orc_joints* jt, const int* voff, const int* do
 
 int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
                         const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
-    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev || prm->block_precond) return ORC_BAD_ARG;
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev || prm->anderson || prm->block_precond)
+        return ORC_BAD_ARG;
     orc_W W;
     int st = orc_build_W(B, C, &W);
     if (st) { orc_free_W(&W); return st; }

@@ -66,6 +66,8 @@ typedef struct {
     int nesterov_restart_limit;
     int chebyshev;     /* pj.chebyshev (ignored when nesterov is set, projected_jacobi.cpp:469) */
     int block_precond; /* pj.rdiag_true_delassus: exact per-contact Delassus blocks instead of the row scaling */
+    int anderson;      /* pj.anderson (ignored when nesterov or chebyshev is set, projected_jacobi.cpp:470-472) */
+    int anderson_m;    /* pj.anderson_m: history window (>= 1) */
 } orc_params;
 
 typedef struct {
@@ -97,6 +99,11 @@ void orc_tangent_frame(const double n_in[3], double t1[3], double t2[3]);
 void orc_cross(const double a[3], const double b[3], double out[3]);
 
 /* Projected Jacobi: v_out[nv], imp_out[3*nc] (canonical normal >= 0). */
+/* min ||A g - b||_2 for A (rows x cols, column-major, overwritten by its factorisation): Householder QR
+ * with column pivoting and Eigen's pivot count (ColPivHouseholderQR::compute / solve); returns the
+ * number of pivots used (columns beyond it get g = 0). */
+int orc_lstsq_colpiv(int rows, int cols, double* A, const double* b, double* g);
+
 int orc_pj_solve(const orc_bodies* b, const orc_contacts* c, double dt, double theta, const orc_params* prm,
                  int nthreads, double* v_out, double* imp_out, orc_result* res);
 

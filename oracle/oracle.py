@@ -33,7 +33,7 @@ class Params(C.Structure):
     _fields_ = [("alpha", C.c_double), ("relaxation", C.c_double), ("max_iter", C.c_int), ("tol_abs", C.c_double),
                 ("tol_rel", C.c_double), ("warmstart", C.c_int), ("nesterov", C.c_int),
                 ("nesterov_beta_threshold", C.c_double), ("nesterov_restart_limit", C.c_int), ("chebyshev", C.c_int),
-                ("block_precond", C.c_int)]
+                ("block_precond", C.c_int), ("anderson", C.c_int), ("anderson_m", C.c_int)]
 
 
 class Joints(C.Structure):
@@ -63,6 +63,9 @@ def lib():
             getattr(_lib, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
                                             C.POINTER(Params), _dp, _dp, C.POINTER(Result)]
             getattr(_lib, name).restype = C.c_int
+        _lib.orc_lstsq_colpiv.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+        _lib.orc_lstsq_colpiv.restype = C.c_int
         _lib.orc_pj_solve_joints.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.POINTER(Joints), C.c_double,
                                              C.c_double, C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Result)]
         _lib.orc_pj_solve_joints.restype = C.c_int
@@ -103,10 +106,21 @@ class _Pack:
                           _d(self.angB), _d(self.rest), _d(self.vbias), _d(self.warm))
 
 
+def lstsq_colpiv(A, b):
+    """min ||A g - b|| by the column-pivoted Householder QR of the Anderson step; returns (g, pivots)."""
+    A = np.asfortranarray(np.array(A, dtype=np.float64, copy=True))
+    rows, cols = A.shape
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    g = np.zeros(cols)
+    nz = lib().orc_lstsq_colpiv(rows, cols, _d(A.reshape(-1, order="F")), _d(b), _d(g))
+    return g, nz
+
+
 def params(alpha=0.3, relaxation=1.0, max_iter=200000, tol_abs=1e-5, tol_rel=1e-5, warmstart=1, nesterov=0,
-           nesterov_beta_threshold=1.0, nesterov_restart_limit=10000, chebyshev=0, block_precond=0):
+           nesterov_beta_threshold=1.0, nesterov_restart_limit=10000, chebyshev=0, block_precond=0, anderson=0,
+           anderson_m=2):
     return Params(alpha, relaxation, max_iter, tol_abs, tol_rel, warmstart, nesterov, nesterov_beta_threshold,
-                  nesterov_restart_limit, chebyshev, block_precond)
+                  nesterov_restart_limit, chebyshev, block_precond, anderson, anderson_m)
 
 
 def _solve(name, prob, dt, prm, nthreads=None):

@@ -587,3 +587,99 @@ def test_block_preconditioner_lcp_and_rho(seed):
     st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, chebyshev=1, max_iter=2, tol_abs=0,
                                                                 tol_rel=0))
     assert res.rho == pytest.approx(min(lam, 0.995), rel=1e-12)
+
+
+# ---- Anderson mixing (anderson_loop, projected_jacobi.cpp:224-310) ----------------------------
+
+@pytest.mark.parametrize("shape", [(3, 1), (7, 2), (40, 3), (200, 5)])
+def test_lstsq_colpiv_matches_numpy(shape):
+    """The column-pivoted Householder least squares of the Anderson step against LAPACK's."""
+    rng = np.random.default_rng(shape[0])
+    A = rng.normal(size=shape) * np.logspace(0, 3, shape[1])
+    b = rng.normal(size=shape[0])
+    g, nz = oracle.lstsq_colpiv(A, b)
+    ref = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert nz == shape[1]
+    np.testing.assert_allclose(g, ref, rtol=1e-11, atol=1e-14)
+
+
+def test_lstsq_colpiv_zero_column():
+    """A zero column is not a pivot: its coefficient is exactly 0 and the others solve the rest."""
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(30, 3))
+    b = rng.normal(size=30)
+    A[:, 1] = 0
+    g, nz = oracle.lstsq_colpiv(A, b)
+    assert nz == 2 and g[1] == 0.0
+    ref = np.linalg.lstsq(A[:, [0, 2]], b, rcond=None)[0]
+    np.testing.assert_allclose(g[[0, 2]], ref, rtol=1e-12)
+
+
+def two_planes(m=2.0, angle=60.0):
+    """A point mass pressed against two static frictionless planes whose normals meet at `angle`:
+    while both impulses stay positive the sweep is affine with a coupled 2x2 Delassus matrix, so the
+    iteration error lives in a 2-dimensional invariant subspace with two distinct contraction factors
+    (1 - 1.5 alpha, 1 - 0.5 alpha at 60 degrees).  Returns the problem and the exact impulses."""
+    a = np.radians(angle)
+    normals = [np.array([0, 0, 1.0]), np.array([np.sin(a), 0, np.cos(a)])]
+
+    def frame(n):
+        t1 = np.cross(n, [0, 1.0, 0])
+        t1 /= np.linalg.norm(t1)
+        return np.r_[n, t1, np.cross(n, t1)]
+
+    prob = scenes.ContactProblem(
+        dof=np.array([3], np.int32), minv=np.full(3, 1 / m), v=np.array([-0.4, 0.1, -1.0]),
+        fext=np.array([0, 0, -m * G]), fgyro=np.zeros(3), q=np.zeros(3), body_a=np.array([-1, -1], np.int32),
+        body_b=np.array([0, 0], np.int32), mu=np.zeros(2), frame=np.array([frame(n) for n in normals]),
+        angA=np.zeros((2, 9)), angB=np.zeros((2, 9)), rest=np.zeros((2, 3)), vbias=np.zeros((2, 3)),
+        warm=np.zeros((2, 3)))
+    W = np.array(normals)
+    vfree = prob.v + 1e-3 * prob.minv * prob.fext
+    return prob, W, -np.linalg.solve(W @ np.diag(prob.minv) @ W.T, W @ vfree)
+
+
+def test_anderson_exact_on_affine_sweeps():
+    """Type-II Anderson on an affine map is a Krylov method: with the error in a 2-dimensional
+    subspace, two difference columns (m >= 2, available from the third iteration) give the exact
+    fixed point after 3 sweeps, whereas one column (m = 1) and the plain loop do not.  The first
+    iteration is a plain sweep; a single contact (1-dimensional error) is exact after 2 sweeps."""
+    prob, W, pstar = two_planes()
+    run = lambda k, **kw: oracle.pj_solve(prob, 1e-3, oracle.params(max_iter=k, tol_abs=0, tol_rel=0, warmstart=0, **kw))
+    plain1 = run(1)
+    for m in (1, 2, 3):
+        a1 = run(1, anderson=1, anderson_m=m)
+        np.testing.assert_array_equal(a1[1], plain1[1])
+        np.testing.assert_array_equal(a1[2], plain1[2])
+    for m in (2, 3):
+        st, v, imp, res = run(3, anderson=1, anderson_m=m)
+        assert st == 0 and res.iters == 3
+        np.testing.assert_allclose(imp[:, 0], pstar, rtol=0, atol=1e-13)
+        assert np.abs(W @ v).max() < 1e-13
+    assert np.abs(run(3, anderson=1, anderson_m=1)[2][:, 0] - pstar).max() > 1e-3
+    assert np.abs(run(3)[2][:, 0] - pstar).max() > 1e-2
+    one = plane_scene().problem()
+    one.v[2] = -1.0
+    st, v, imp, res = oracle.pj_solve(one, 1e-3, oracle.params(anderson=1, anderson_m=1, max_iter=2, tol_abs=0,
+                                                               tol_rel=0, warmstart=0))
+    assert imp[0, 0] == pytest.approx(1.0 + G * 1e-3, rel=1e-13)  # p* = -m g0, g0 = -1 - g dt
+
+
+def test_anderson_converges_there_and_to_the_same_fixed_point():
+    """Run to tolerance the Anderson loop stops with a residual <= 1 checked every iteration, on the
+    same solution as the standard loop (frictionless: unique impulses), in fewer sweeps; nesterov and
+    chebyshev take precedence over it (projected_jacobi.cpp:470-472)."""
+    prob = scenes.random_problem(nb=8, nc=12, seed=11, p_frictionless=1.0, p_point=0.0)
+    tight = dict(tol_abs=1e-13, tol_rel=1e-13, max_iter=1_000_000)
+    ref = oracle.pj_solve(prob, 1e-3, oracle.params(**tight))
+    for m in (1, 2, 5):
+        st, v, imp, res = oracle.pj_solve(prob, 1e-3, oracle.params(anderson=1, anderson_m=m, **tight))
+        assert st == 0 and res.residual <= 1 and res.iters < ref[3].iters
+        np.testing.assert_allclose(v, ref[1], atol=1e-9)
+        np.testing.assert_allclose(imp, ref[2], atol=1e-8)
+    prm = dict(max_iter=300, tol_abs=1e-9, tol_rel=1e-9)
+    for flag in ("nesterov", "chebyshev"):
+        a = oracle.pj_solve(prob, 1e-3, oracle.params(**{flag: 1}, **prm))
+        b = oracle.pj_solve(prob, 1e-3, oracle.params(**{flag: 1}, anderson=1, **prm))
+        assert a[3].iters == b[3].iters
+        np.testing.assert_array_equal(a[1], b[1])
