@@ -663,7 +663,8 @@ def diff_leg(args, native, scenes):
 
 
 def chains_diff_leg(args, native, scenes):
-    """Forward + checkpointed backward of the jointed capsule chains: gradient of the COM loss
+    """Forward + backward of the jointed capsule chains (the forward keeps every step's state, so
+    the backward recomputes nothing): gradient of the COM loss
     w.r.t. every joint's rotational damping (and v0) through the implicit joint adjoint."""
     from cardillocxx_amd import diff
 
@@ -707,11 +708,11 @@ def chains_diff_leg(args, native, scenes):
     t_obo = t_one * ns / len(pick)
     log(f"chains differentiable one by one: {len(pick)} chains in {t_one:.2f}s")
     return {
-        "metric": "fwd+bwd jointed env-steps/s (chains x steps through forward and checkpointed backward per second)",
+        "metric": "fwd+bwd jointed env-steps/s (chains x steps through forward and backward per second)",
         "workload": f"{ns} chains of 32 capsules (r=0.02 m, length 0.1 m), damped spherical joints (D_rot="
                     f"{args.chain_damping:g}), dropped onto a 256x256 fractal heightfield (amp 0.1 m), v0~N(0,0.2^2), "
                     f"{steps} steps at h=1e-3 s, loss sum_s ||COM_s(final) - target_s||^2, grads wrt every joint's damping "
-                    f"and v0; implicit FP64 adjoint through joints and contacts, checkpoint every 10 steps, PJ tol "
+                    f"and v0; implicit FP64 adjoint through joints and contacts, step states kept by the forward, PJ tol "
                     f"{args.tol:g} per chain, adjoint tol {args.adjoint_tol:g}",
         "value": ns * steps / (tf + tb), "unit": "env-steps/s",
         "forward_s": tf, "backward_s": tb, "loss": loss,

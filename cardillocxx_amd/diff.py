@@ -211,10 +211,15 @@ class ChainRollout(Rollout):
     with Lambda carried from step to step.  Parameters: initial velocities v0 and the rotational
     damping D of every joint; masses and friction are fixed.  The backward uses the implicit
     adjoint through the joint rows (cdx_pj_adjoint_joints) with contact geometry and joint
-    Jacobians of every step held fixed; a damper row has A = 1/D, so dL/dD = -dL/dA / D^2."""
+    Jacobians of every step held fixed; a damper row has A = 1/D, so dL/dD = -dL/dA / D^2.
+    store_steps (default): the forward keeps every step's state (the contacts and joint rows it
+    keeps anyway, plus q, v and the impulses), so the backward needs no recomputation; otherwise
+    it recomputes each segment from its checkpoint (every `checkpoint_every` steps)."""
 
-    def __init__(self, batch: ShapeBatch, joints_spec: dict, steps: int, **kw):
+    def __init__(self, batch: ShapeBatch, joints_spec: dict, steps: int, store_steps: bool = True, **kw):
         super().__init__(batch, steps, **kw)
+        self.store_steps = store_steps
+        self.records = None
         from . import joints as jmod
         self.jmod = jmod
         self.spec = joints_spec
@@ -272,6 +277,7 @@ class ChainRollout(Rollout):
         q, v = bd["q"].copy(), self.v0.copy()
         self.checkpoints = {}
         self.contacts_log, self.rows_log = [], []
+        self.records = [] if self.store_steps else None
         prev, lam = None, None
         for n in range(self.steps):
             if n % self.k == 0:
@@ -279,6 +285,8 @@ class ChainRollout(Rollout):
             q, v, rec, prev, lam = self._chain_step(n, q, v, prev, lam)
             self.contacts_log.append(rec["con"])
             self.rows_log.append(rec["rows"])
+            if self.records is not None:
+                self.records.append(rec)
         self.qN = q
         return q.reshape(-1, 7)[:, :3].copy()
 
@@ -293,15 +301,23 @@ class ChainRollout(Rollout):
         g_mu = []
         prm1 = native.default_params()
         prm1.max_iter, prm1.tol_abs, prm1.tol_rel, prm1.warmstart = 1, 0.0, 0.0, 1
-        for s in reversed(sorted(self.checkpoints)):
-            e = min(s + self.k, self.steps)
-            q, v, prev, lam = self.checkpoints[s]
-            seg = []
-            for n in range(s, e):
-                q1, v1, rec, prev, lam = self._chain_step(n, q, v, prev, lam)
-                seg.append(rec)
-                q, v = q1, v1
-            for n in range(e - 1, s - 1, -1):
+
+        def segments():  # (first step, records of the segment's steps), last segment first
+            if self.records is not None:
+                yield 0, self.records
+                return
+            for s in reversed(sorted(self.checkpoints)):
+                e = min(s + self.k, self.steps)
+                q, v, prev, lam = self.checkpoints[s]
+                seg = []
+                for n in range(s, e):
+                    q1, v1, rec, prev, lam = self._chain_step(n, q, v, prev, lam)
+                    seg.append(rec)
+                    q, v = q1, v1
+                yield s, seg
+
+        for s, seg in segments():
+            for n in range(s + len(seg) - 1, s - 1, -1):
                 rec = seg[n - s]
                 # restore the converged state: one sweep warm-started from the converged impulses
                 self._solve_joints(rec["q"], rec["v"], rec["fgyro"], rec["con"], rec["imp"], rec["rows"], rec["lam_prev"],

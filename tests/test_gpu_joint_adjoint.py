@@ -171,3 +171,25 @@ def test_chain_rollout_damping_gradient_vs_finite_differences():
         dv = rng.normal(size=v0.shape)
         fdv = (loss(v0 + 1e-6 * dv, D0) - loss(v0 - 1e-6 * dv, D0)) / 2e-6
         assert float(g.v0.reshape(-1) @ dv) == pytest.approx(fdv, rel=1e-4, abs=1e-12)
+
+
+def test_chain_rollout_stored_steps_equal_checkpoint_recompute():
+    """The backward from the forward's stored step states gives the gradients of the checkpointed
+    backward that recomputes each segment, bitwise (the recomputed steps are the same solves)."""
+    from cardillocxx_amd import diff
+
+    t = scenes.fractal_heightfield(n=33, extent=1.6, amp=0.03, seed=4)
+    batch, jt = scenes.capsule_chains(nchains=2, links=5, terrain=t, seed=4, drop=0.0005, local=True)
+    batch.v[:] = np.random.default_rng(9).normal(scale=0.2, size=batch.v.shape)
+    target = np.random.default_rng(0).normal(scale=0.1, size=(batch.nscenes, 3))
+    D0 = np.full(len(jt["body_a"]), 0.05)
+    grads = []
+    for store in (True, False):
+        r = diff.ChainRollout(batch, jt, steps=12, dt=1e-3, checkpoint_every=5, tol=1e-12, adjoint_tol=1e-13,
+                              store_steps=store)
+        pos = r.forward(batch.v.reshape(-1).copy(), D0)
+        grads.append(r.backward(diff.com_loss(pos, batch.mass, batch.body_off, target)[1]))
+        assert (r.records is not None) == store
+    np.testing.assert_array_equal(grads[0].damping, grads[1].damping)
+    np.testing.assert_array_equal(grads[0].v0, grads[1].v0)
+    np.testing.assert_array_equal(grads[0].mu, grads[1].mu)
