@@ -572,24 +572,33 @@ __global__ void k_gm_out(int nc, const int* __restrict__ ida, const int* __restr
 // partial[i * gridDim.x + block] = this block's part of <w, V_i>, i < nvec
 __global__ void __launch_bounds__(kBlock) k_gm_dots(int n, int nvec, const double* __restrict__ V, size_t ld,
                                                     const double* __restrict__ w, double* __restrict__ partial) {
-    __shared__ double red[kBlock / 64];
+    // all nvec products reduced within each wave by shuffles, then one barrier for the block's waves
+    __shared__ double red[kBlock / 64][kGmRestart + 1];
     const int r = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double wr = r < n ? w[r] : 0.0;
     for (int i = 0; i < nvec; ++i) {
-        const double d = block_sum(r < n ? wr * V[i * ld + r] : 0.0, red);
-        if (threadIdx.x == 0) partial[(size_t)i * gridDim.x + blockIdx.x] = d;
+        double d = r < n ? wr * V[i * ld + r] : 0.0;
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+        if (lane == 0) red[wv][i] = d;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nvec; i += kBlock) {
+        double s = 0;
+        for (int q = 0; q < kBlock / 64; ++q) s += red[q][i];
+        partial[(size_t)i * gridDim.x + blockIdx.x] = s;
     }
 }
 
-// h[i] = (h[i] +) sum of the partials of dot i; one workgroup
+// h[i] = (h[i] +) sum of the partials of dot i; one workgroup, a wave per dot (no barriers)
 __global__ void __launch_bounds__(kBlock) k_gm_reduce(int nvec, int nblk, const double* __restrict__ partial,
                                                       double* __restrict__ h, int accumulate) {
-    __shared__ double red[kBlock / 64];
-    for (int i = 0; i < nvec; ++i) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i < nvec; i += kBlock / 64) {
         double s = 0;
-        for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[(size_t)i * nblk + b];
-        s = block_sum(s, red);
-        if (threadIdx.x == 0) h[i] = accumulate ? h[i] + s : s;
+        for (int b = lane; b < nblk; b += 64) s += partial[(size_t)i * nblk + b];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) h[i] = accumulate ? h[i] + s : s;
     }
 }
 
@@ -906,6 +915,15 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
             CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
             return CDX_OK;
         };
+        // the restart cycle's graph (one per call: it holds this call's pointers)
+        const bool use_graph = ctx->graphs && !ctx->profiling;
+        hipGraphExec_t cycle = nullptr;
+        struct CycleGuard {
+            hipGraphExec_t* e;
+            ~CycleGuard() {
+                if (*e) hipGraphExecDestroy(*e);
+            }
+        } cycle_guard{&cycle};
         double gnorm2 = 0;
         CDX_TRY(dots(1, gvec, gvec, sc, 0));
         CDX_TRY(host_scalar(sc, &gnorm2));
@@ -936,11 +954,10 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
                 break;
             }
             CDX_HIP(ctx, hipMemsetAsync(H, 0, sizeof(double) * (size_t)(m + 1) * m, ctx->stream));
-            int k = 0;
-            for (; k < m && applies < aprm->max_iter; ++k) {
+            // Arnoldi step k: w = B v_k, two Gram-Schmidt passes against v_0..v_k, v_{k+1} = w / |w|
+            auto arnoldi = [&](int k) -> int {
                 double* vk = V + (size_t)k * n;
                 CDX_TRY(apply(vk, Wk));
-                ++applies;
                 double* hk = H + (size_t)k * (m + 1);
                 CDX_TRY(dots(k + 1, V, Wk, hk, 0));
                 CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, hk, -1.0, Wk);
@@ -949,6 +966,34 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
                 CDX_LAUNCH(ctx, "adj_residual", k_gm_add, 1, kBlock, k + 1, sc + 2, hk);
                 CDX_TRY(dots(1, Wk, Wk, sc, 0));
                 CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, hk + k + 1, V + (size_t)(k + 1) * n);
+                return hip_check(ctx, hipGetLastError(), "adjoint GMRES Arnoldi step");
+            };
+            int k = 0;
+            if (use_graph && aprm->max_iter - applies >= m) {
+                // a full cycle is the same launches every restart: captured once, replayed
+                if (!cycle) {
+                    CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    int cst = CDX_OK;
+                    for (int kk = 0; kk < m && cst == CDX_OK; ++kk) cst = arnoldi(kk);
+                    hipGraph_t g = nullptr;
+                    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+                    if (cst) {
+                        if (g) hipGraphDestroy(g);
+                        return cst;
+                    }
+                    CDX_HIP(ctx, ce);
+                    const hipError_t ie = hipGraphInstantiate(&cycle, g, nullptr, nullptr, 0);
+                    hipGraphDestroy(g);
+                    CDX_HIP(ctx, ie);
+                }
+                CDX_HIP(ctx, hipGraphLaunch(cycle, ctx->stream));
+                applies += m;
+                k = m;
+            } else {
+                for (; k < m && applies < aprm->max_iter; ++k) {
+                    CDX_TRY(arnoldi(k));
+                    ++applies;
+                }
             }
             // least squares min |beta e1 - H y| by Givens rotations
             CDX_HIP(ctx, hipMemcpyAsync(Hh.data(), H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, ctx->stream));

@@ -193,3 +193,21 @@ def test_chain_rollout_stored_steps_equal_checkpoint_recompute():
     np.testing.assert_array_equal(grads[0].damping, grads[1].damping)
     np.testing.assert_array_equal(grads[0].v0, grads[1].v0)
     np.testing.assert_array_equal(grads[0].mu, grads[1].mu)
+
+
+def test_adjoint_graph_cycles_equal_direct_launches(ctx):
+    """The GMRES restart cycle replayed as a graph (several restarts, with the host's least-squares
+    work in between) gives the directly launched adjoint bitwise; profiling launches every kernel
+    directly."""
+    prob, rows = chain_case(nchains=12, links=14, seed=2, lower=True, damping=0.05)
+    gv = np.random.default_rng(4).normal(size=prob.nv)
+    outs = []
+    for prof in (False, True, False):
+        ctx.set_profiling(prof)
+        outs.append(device_grad(ctx, prob, rows, gv))
+    ctx.set_profiling(False)
+    assert outs[0]["info"].iters > 62  # the cycle graph replayed at least once
+    for o in outs[1:]:
+        assert o["info"].iters == outs[0]["info"].iters
+        for key in ("v", "fext", "mu", "comp"):
+            np.testing.assert_array_equal(o[key], outs[0][key])
