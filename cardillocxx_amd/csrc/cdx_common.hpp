@@ -73,6 +73,10 @@ struct cdx_ctx {
     bool joints_solved = false;
     std::vector<int> h_jorig;  // block-ordered row -> caller's row index
     std::vector<int> h_jc_off, h_jb_body;  // host copies of the chain and block tables
+    // the row bodies of the last cdx_set_joints call whose tables are on the device: a call with the
+    // same bodies (the same joints at a new step) only refreshes the row values
+    std::vector<int> h_jsig_a, h_jsig_b;
+    int jsig_ns = -1, jsig_nd = -1, jsig_nb = -1, jsig_nchains = 0, jsig_njb = 0;
     int *d_sc_off = nullptr, *d_sc_list = nullptr;  // batched solve: chains of each scene
     size_t sc_off_cap = 0, sc_list_cap = 0;
     int *d_jc_off = nullptr, *d_jcb_off = nullptr, *d_jb_off = nullptr, *d_jb_body = nullptr, *d_jr_kind = nullptr;

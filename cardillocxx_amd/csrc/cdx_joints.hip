@@ -98,6 +98,31 @@ int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) 
 
 using namespace cdx;
 
+// Row values in block order (jorig: block-ordered row -> caller's row): jr, 16 doubles per row,
+// and the same coefficients by coefficient (jw[k R + r]), so that a wave reading one coefficient
+// of 64 rows touches 4 cache lines instead of 64; both uploaded.
+static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, const double* wa, const double* wb,
+                               const double* comp, const double* vsrc, const double* gerr, const double* lam_prev) {
+    const size_t R = jorig.size();
+    std::vector<double> jr(R * kJRowStride), jw(12 * R);
+    for (size_t i = 0; i < R; ++i) {
+        const size_t r = (size_t)jorig[i];
+        double* o = jr.data() + i * kJRowStride;
+        for (int k = 0; k < 6; ++k) o[k] = wa[6 * r + k];
+        for (int k = 0; k < 6; ++k) o[6 + k] = wb[6 * r + k];
+        o[12] = comp[r];
+        o[13] = vsrc[r];
+        o[14] = gerr[r];
+        o[15] = lam_prev[r];
+        for (int k = 0; k < 12; ++k) jw[k * R + i] = o[k];
+    }
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
+    CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, jw.size(), &ctx->jw_cap, "joint rows"));
+    CDX_HIP(ctx, hipMemcpy(ctx->d_jw, jw.data(), sizeof(double) * jw.size(), hipMemcpyHostToDevice));
+    return CDX_OK;
+}
+
 extern "C" {
 
 int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* body_b, const double* wa, const double* wb,
@@ -107,6 +132,9 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     ctx->solved = false;
     ctx->nchains = 0;
     ctx->njrows = 0;
+    const bool had_tables = ctx->jsig_ns >= 0;
+    const int sig_ns = ctx->jsig_ns, sig_nd = ctx->jsig_nd, sig_nb = ctx->jsig_nb;
+    ctx->jsig_ns = -1;  // the device tables count as stale until this call completes
     if (ns < 0 || nd < 0) return set_err(ctx, CDX_BAD_ARG, "cdx_set_joints: negative row count");
     const int n = ns + nd;
     if (n == 0) return CDX_OK;
@@ -129,6 +157,17 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
         if (!std::isfinite(comp[r]) || comp[r] < 0 || !std::isfinite(vsrc[r]) || !std::isfinite(gerr[r]) ||
             !std::isfinite(lam_prev[r]))
             return set_err(ctx, CDX_NONFINITE, "cdx_set_joints: bad compliance or source term");
+    }
+    // the same row bodies as the tables on the device: only the row values change
+    if (had_tables && ns == sig_ns && nd == sig_nd && nb == sig_nb && std::equal(body_a, body_a + n, ctx->h_jsig_a.begin()) &&
+        std::equal(body_b, body_b + n, ctx->h_jsig_b.begin())) {
+        CDX_TRY(upload_joint_values(ctx, ctx->h_jorig, wa, wb, comp, vsrc, gerr, lam_prev));
+        ctx->jsig_ns = ns;
+        ctx->njrows = n;
+        ctx->nchains = ctx->jsig_nchains;
+        ctx->njbodies = ctx->jsig_njb;
+        ctx->joint_bias = bias_factor;
+        return CDX_OK;
     }
     // blocks: rows grouped by body pair, in order of first appearance
     std::map<std::pair<int, int>, int> key;
@@ -187,18 +226,11 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     const int nchains = (int)chain_off.size() - 1;
     // block-ordered rows and per-block data
     std::vector<int> jb_off{0}, jb_body, jr_kind, jorig, jr_blk;
-    std::vector<double> jr;
     for (int j : order) {
         for (int r : blk_rows[j]) {
             jr_blk.push_back((int)jb_off.size() - 1);
             jorig.push_back(r);
             jr_kind.push_back(r < ns ? 0 : 1);
-            for (int k = 0; k < 6; ++k) jr.push_back(wa[6 * (size_t)r + k]);
-            for (int k = 0; k < 6; ++k) jr.push_back(wb[6 * (size_t)r + k]);
-            jr.push_back(comp[r]);
-            jr.push_back(vsrc[r]);
-            jr.push_back(gerr[r]);
-            jr.push_back(lam_prev[r]);
         }
         jb_off.push_back((int)jorig.size());
         jb_body.push_back(blk_body[j].first);
@@ -270,17 +302,7 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     CDX_TRY(upi(&ctx->d_jr_info, jr_info, &ctx->jr_info_cap, "joint row records"));
     CDX_TRY(upi(&ctx->d_jbody_info, jbody_info, &ctx->jbody_info_cap, "jointed body records"));
     CDX_TRY(upi(&ctx->d_jslot, jslot, &ctx->jslot_cap, "joint slots"));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
-    CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
-    // the same coefficients by coefficient, so that a wave reading one coefficient of 64 rows
-    // touches 4 cache lines instead of 64
-    {
-        std::vector<double> jw(12 * R);
-        for (size_t r = 0; r < R; ++r)
-            for (int k = 0; k < 12; ++k) jw[k * R + r] = jr[r * kJRowStride + k];
-        CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, jw.size(), &ctx->jw_cap, "joint rows"));
-        CDX_HIP(ctx, hipMemcpy(ctx->d_jw, jw.data(), sizeof(double) * jw.size(), hipMemcpyHostToDevice));
-    }
+    CDX_TRY(upload_joint_values(ctx, jorig, wa, wb, comp, vsrc, gerr, lam_prev));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jrlam, R, &ctx->jrlam_cap, "joint rhs"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jz, R, &ctx->jz_cap, "joint scratch"));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jy, R, &ctx->jy_cap, "joint scratch"));
@@ -295,6 +317,13 @@ int cdx_set_joints(cdx_ctx* ctx, int ns, int nd, const int* body_a, const int* b
     ctx->nchains = nchains;
     ctx->njbodies = (int)NJB;
     ctx->joint_bias = bias_factor;
+    ctx->h_jsig_a.assign(body_a, body_a + n);
+    ctx->h_jsig_b.assign(body_b, body_b + n);
+    ctx->jsig_ns = ns;
+    ctx->jsig_nd = nd;
+    ctx->jsig_nb = nb;
+    ctx->jsig_nchains = nchains;
+    ctx->jsig_njb = (int)NJB;
     return CDX_OK;
 }
 

@@ -302,3 +302,21 @@ def test_long_chains_staged_in_windows(ctx):
         b0, b1 = batch.body_off[s], batch.body_off[s + 1]
         assert it[s] == r1.iters
         assert rel(ctx.velocities()[6 * b0:6 * b1], v1) <= TOL and rel(ctx.joint_impulses()[sel], l1) <= TOL
+
+
+def test_set_joints_reuses_tables_only_for_the_same_bodies(ctx):
+    """cdx_set_joints keeps the chain tables of the last call and only refreshes the row values
+    when the row bodies are the same: new values on the same joints, then the same rows listed in
+    another order (a new block order), then new values again -- each solve matches the oracle."""
+    prob, rows = chain_case(nchains=3, links=6, seed=1)
+    rng = np.random.default_rng(6)
+    perm = rng.permutation(rows["ns"])  # reorder the spring rows, dampers unchanged
+    n = rows["ns"] + rows["nd"]
+    order = np.concatenate([perm, np.arange(rows["ns"], n)])
+    permuted = {k: (np.asarray(v)[order] if isinstance(v, np.ndarray) and len(v) == n else v) for k, v in rows.items()}
+    scaled = dict(rows, gerr=np.asarray(rows["gerr"]) * 1.5, comp=np.asarray(rows["comp"]) * 0.7)
+    for r in (rows, scaled, permuted, dict(permuted, gerr=np.asarray(permuted["gerr"]) * 0.5)):
+        (v, imp, lam, info), (v_ref, imp_ref, lam_ref, res) = run_both(ctx, prob, r, 1e-3, max_iter=20, tol_abs=0.0,
+                                                                      tol_rel=0.0)
+        assert info.iters == res.iters == 20
+        assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL and rel(lam, lam_ref) <= TOL
