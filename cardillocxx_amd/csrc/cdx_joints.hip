@@ -67,6 +67,14 @@ __global__ void __launch_bounds__(64) k_joint_solve(
     }
 }
 
+// jw[k R + r] = jr[r kJRowStride + k], k < 12: the coefficients by coefficient
+__global__ void k_joint_by_coef(int R, const double* __restrict__ jr, double* __restrict__ jw) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 12 * R) return;
+    const int k = t / R, r = t - (t / R) * R;
+    jw[t] = jr[(size_t)r * kJRowStride + k];
+}
+
 int joints_factor(cdx_ctx* ctx, double dt, double theta) {
     if (ctx->nchains == 0) return CDX_OK;
     CDX_LAUNCH(ctx, "joint_factor", k_joint_factor, ctx->nchains, 64, ctx->nchains, ctx->d_jc_off, ctx->d_jb_off,
@@ -99,12 +107,12 @@ int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl) 
 using namespace cdx;
 
 // Row values in block order (jorig: block-ordered row -> caller's row): jr, 16 doubles per row,
-// and the same coefficients by coefficient (jw[k R + r]), so that a wave reading one coefficient
-// of 64 rows touches 4 cache lines instead of 64; both uploaded.
+// uploaded, and on the device the same coefficients by coefficient (jw[k R + r]), so that a wave
+// reading one coefficient of 64 rows touches 4 cache lines instead of 64.
 static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, const double* wa, const double* wb,
                                const double* comp, const double* vsrc, const double* gerr, const double* lam_prev) {
     const size_t R = jorig.size();
-    std::vector<double> jr(R * kJRowStride), jw(12 * R);
+    std::vector<double> jr(R * kJRowStride);
     for (size_t i = 0; i < R; ++i) {
         const size_t r = (size_t)jorig[i];
         double* o = jr.data() + i * kJRowStride;
@@ -114,13 +122,12 @@ static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, cons
         o[13] = vsrc[r];
         o[14] = gerr[r];
         o[15] = lam_prev[r];
-        for (int k = 0; k < 12; ++k) jw[k * R + i] = o[k];
     }
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
     CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, jw.size(), &ctx->jw_cap, "joint rows"));
-    CDX_HIP(ctx, hipMemcpy(ctx->d_jw, jw.data(), sizeof(double) * jw.size(), hipMemcpyHostToDevice));
-    return CDX_OK;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, 12 * R, &ctx->jw_cap, "joint rows"));
+    CDX_LAUNCH(ctx, "joint_by_coef", k_joint_by_coef, grid_for((int)(12 * R)), kBlock, (int)R, ctx->d_jr, ctx->d_jw);
+    return hip_check(ctx, hipGetLastError(), "joint coefficient launch");
 }
 
 extern "C" {
