@@ -304,7 +304,7 @@ def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std):
                        ("anderson", native.CDX_ACCEL_ANDERSON)):
         p = native.default_params()
         p.tol_abs = p.tol_rel = args.tol
-        p.max_iter = args.max_iter if mode != native.CDX_ACCEL_ANDERSON else min(args.max_iter, 3 * max(sweeps_std, 1))
+        p.max_iter = args.max_iter if mode != native.CDX_ACCEL_ANDERSON else min(args.max_iter, int(3 * max(sweeps_std, 1)))
         p.warmstart = 0
         p.accel = mode
         p.anderson_m = args.anderson_m
