@@ -23,7 +23,7 @@ workload, and a fully threaded variant as an upper bound.
 every sweep).  `batched`: 4096 independent sphere/box scenes dropped into a unit box, one batched
 solve (one launch) per Moreau step, against the same scenes solved one by one through the plain
 solver on sampled steps.  `differentiable`: 256 scenes of 64 boxes on a plane, forward and
-checkpointed backward of a 100-step rollout, against the same scenes rolled out one by one.
+backward of a 100-step rollout (step states kept by the forward), against the same scenes rolled out one by one.
 
 --copy-mb N also runs one device-to-device copy of N MiB, the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters collected under rocprofv3 --pmc.
@@ -605,7 +605,7 @@ def chains_leg(args, native, scenes):
 
 
 # --------------------------------------------------------------------------------------------
-# differentiable: forward + checkpointed backward of batched rollouts
+# differentiable: forward + backward of batched rollouts
 # --------------------------------------------------------------------------------------------
 
 def diff_leg(args, native, scenes):
@@ -646,11 +646,11 @@ def diff_leg(args, native, scenes):
     t_obo = t_one * ns / len(pick)
     log(f"differentiable one by one: {len(pick)} scenes in {t_one:.2f}s")
     return {
-        "metric": "fwd+bwd env-steps/s (scenes x steps through forward and checkpointed backward per second)",
+        "metric": "fwd+bwd env-steps/s (scenes x steps through forward and backward per second)",
         "workload": f"{ns} scenes x 64 boxes (edge 0.1 m, 1 kg) on a plane, v0~N(0,0.5^2), {steps} steps at h=1e-3 s, "
                     f"loss sum_s ||COM_s(final) - target_s||^2, grads wrt v0, mu, mass; implicit FP64 adjoint "
                     f"(unrolled fallback where it does not converge in {args.adjoint_max_iter} iterations), "
-                    f"checkpoint every 10 steps, PJ tol {args.tol:g} per scene, adjoint tol {args.adjoint_tol:g}",
+                    f"step states kept by the forward, PJ tol {args.tol:g} per scene, adjoint tol {args.adjoint_tol:g}",
         "value": ns * steps / (tf + tb), "unit": "env-steps/s",
         "forward_s": tf, "backward_s": tb, "loss": loss,
         "forward_solves": r.stats["forward_solves"], "adjoint_iterations": r.stats["adjoint_iters"],

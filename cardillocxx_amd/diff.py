@@ -35,7 +35,8 @@ class Grads:
 class Rollout:
     def __init__(self, batch: ShapeBatch, steps: int, dt: float = 1e-3, checkpoint_every: int = 10, tol: float = 1e-10,
                  adjoint_mode: int = native.CDX_ADJ_IMPLICIT, adjoint_tol: float = 1e-11, device: int = 0, ctx=None,
-                 max_iter: int = 200000, batched: bool = False, adjoint_max_iter: int = 100000):
+                 max_iter: int = 200000, batched: bool = False, adjoint_max_iter: int = 100000,
+                 store_steps: bool = True):
         self.batch = batch
         self.steps = steps
         self.dt = dt
@@ -55,6 +56,10 @@ class Rollout:
         self.batched = batched and batch.nscenes > 1
         self.frozen = None  # optional per-step contact dicts to reuse (geometry held fixed)
         self.stats = dict(forward_solves=0, sweeps=0, adjoint_iters=0, unrolled_fallbacks=0)
+        # store_steps: the forward keeps every step's record (contacts, q, v, impulses), so the
+        # backward recomputes nothing; otherwise it recomputes each segment from its checkpoint
+        self.store_steps = store_steps
+        self.records = None
 
     # ---- model parameters -> solver inputs
     def _bodies(self, v, mass):
@@ -120,12 +125,15 @@ class Rollout:
         q, v = bd["q"].copy(), self.v0.copy()
         self.checkpoints = {}
         self.contacts_log = []
+        self.records = [] if self.store_steps else None
         prev = None
         for n in range(self.steps):
             if n % self.k == 0:
                 self.checkpoints[n] = (q.copy(), v.copy(), prev)
             q, v, rec, prev = self._step(n, q, v, self.minv, self.fext, self.mu_body, prev)
             self.contacts_log.append(rec["con"])
+            if self.records is not None:
+                self.records.append(rec)
         self.qN = q
         return q.reshape(-1, 7)[:, :3].copy()
 
@@ -172,16 +180,23 @@ class Rollout:
         g_minv = np.zeros(nb * 6)
         g_fext = np.zeros(nb * 6)
         g_mu = np.zeros(nb)
-        starts = sorted(self.checkpoints)
-        for s in reversed(starts):
-            e = min(s + self.k, self.steps)
-            q, v, prev = self.checkpoints[s]
-            seg = []
-            for n in range(s, e):  # recompute the segment forward from its checkpoint
-                q1, v1, rec, prev = self._step(n, q, v, self.minv, self.fext, self.mu_body, prev)
-                seg.append(rec)
-                q, v = q1, v1
-            for n in range(e - 1, s - 1, -1):
+
+        def segments():  # (first step, records of the segment's steps), last segment first
+            if self.records is not None:
+                yield 0, self.records
+                return
+            for s in reversed(sorted(self.checkpoints)):
+                e = min(s + self.k, self.steps)
+                q, v, prev = self.checkpoints[s]
+                seg = []
+                for n in range(s, e):  # recompute the segment forward from its checkpoint
+                    q1, v1, rec, prev = self._step(n, q, v, self.minv, self.fext, self.mu_body, prev)
+                    seg.append(rec)
+                    q, v = q1, v1
+                yield s, seg
+
+        for s, seg in segments():
+            for n in range(s + len(seg) - 1, s - 1, -1):
                 rec = seg[n - s]
                 g = self._step_adjoint(rec, a + self.dt * gx)
                 self.stats["adjoint_iters"] += g["info"].iters
@@ -212,14 +227,11 @@ class ChainRollout(Rollout):
     damping D of every joint; masses and friction are fixed.  The backward uses the implicit
     adjoint through the joint rows (cdx_pj_adjoint_joints) with contact geometry and joint
     Jacobians of every step held fixed; a damper row has A = 1/D, so dL/dD = -dL/dA / D^2.
-    store_steps (default): the forward keeps every step's state (the contacts and joint rows it
-    keeps anyway, plus q, v and the impulses), so the backward needs no recomputation; otherwise
-    it recomputes each segment from its checkpoint (every `checkpoint_every` steps)."""
+    As in Rollout, store_steps (default) keeps every step's state in the forward (the contacts and
+    joint rows it keeps anyway, plus q, v and the impulses)."""
 
-    def __init__(self, batch: ShapeBatch, joints_spec: dict, steps: int, store_steps: bool = True, **kw):
+    def __init__(self, batch: ShapeBatch, joints_spec: dict, steps: int, **kw):
         super().__init__(batch, steps, **kw)
-        self.store_steps = store_steps
-        self.records = None
         from . import joints as jmod
         self.jmod = jmod
         self.spec = joints_spec

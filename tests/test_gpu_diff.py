@@ -148,3 +148,20 @@ def test_auto_mode_falls_back_to_unrolled():
     loss_of(rb, batch, target, v0, mu, mass)
     rb.backward(gpos)
     assert rb.stats["unrolled_fallbacks"] == 0
+
+
+@pytest.mark.parametrize("mode", [native.CDX_ADJ_IMPLICIT, native.CDX_ADJ_AUTO])
+def test_stored_steps_equal_checkpoint_recompute(mode):
+    """Backward from the forward's stored step records = the checkpointed backward that recomputes
+    each segment (the recomputed steps are the same solves), bitwise."""
+    grads = []
+    for store in (True, False):
+        batch = scenes.boxes_on_plane(nscenes=2, per_scene=4, seed=3, vstd=0.5)
+        r = diff.Rollout(batch, steps=12, dt=1e-3, checkpoint_every=5, tol=1e-12, adjoint_mode=mode, adjoint_tol=1e-13,
+                         store_steps=store)
+        target = np.random.default_rng(0).normal(scale=0.1, size=(batch.nscenes, 3))
+        _, gpos, _ = loss_of(r, batch, target, batch.v.reshape(-1).copy(), batch.mu_body.copy(), batch.mass.copy())
+        grads.append(r.backward(gpos))
+        assert (r.records is not None) == store
+    for key in ("v0", "mu", "mass"):
+        np.testing.assert_array_equal(getattr(grads[0], key), getattr(grads[1], key))
