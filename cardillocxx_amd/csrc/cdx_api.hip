@@ -215,7 +215,8 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jc_off, ctx->d_jcb_off, ctx->d_jb_off, ctx->d_jb_body, ctx->d_jr_kind, ctx->d_jbody,
                     ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jr, ctx->d_jrlam, ctx->d_jz, ctx->d_jlam, ctx->d_jL,
                     ctx->d_jDinv, ctx->d_jtr, ctx->d_sc_off, ctx->d_sc_list, ctx->d_adj_jl, ctx->d_jr_blk, ctx->d_jy, ctx->d_gm, ctx->d_gm_part, ctx->d_jr_info, ctx->d_jbody_info,
-                    ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st};
+                    ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
+                    ctx->d_gs_pack, ctx->d_gs_ids};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);

@@ -125,6 +125,11 @@ struct cdx_ctx {
     std::vector<int> color_ptr;  // [ncolors+1] into the colour-sorted block list
     int* d_color_blocks = nullptr;  // [nblocks] contact index per colour-sorted block
     size_t color_cap = 0;
+    // the sweep's per-contact data copied into colour order, so that a colour class is one
+    // contiguous range: [27][n] J, [3][n] D^-1, [3][n] rhs, [3][n] lambda, [n] mu; ids [2][n]
+    double* d_gs_pack = nullptr;
+    int* d_gs_ids = nullptr;
+    size_t gs_pack_cap = 0, gs_ids_cap = 0;
 
     // ---- batched scenes (cdx_set_scenes); cleared whenever bodies or contacts are set
     int nscenes = 0;

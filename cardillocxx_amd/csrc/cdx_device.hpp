@@ -8,6 +8,8 @@
 
 namespace cdx {
 
+constexpr int kLanesPerContact = 4;  // lanes per contact in the contact passes: one per row, one idle
+
 __device__ inline void load_body(const double* __restrict__ x, int b, double u[6]) {
     if (b < 0) {
 #pragma unroll

@@ -200,88 +200,128 @@ __global__ void k_gs_init_ucorr(int nb, const int* __restrict__ dof, const doubl
     store6(up + (size_t)b * kBodyStride, o);
 }
 
-// PGS block update of contact c (projected_gauss_seidel.cpp:144-169 with project(), :11-26),
-// writing lambda and the u_corr of both bodies directly.
-__device__ inline void gs_block_update(int c, const int* __restrict__ ida, const int* __restrict__ idb,
-                                       const double* __restrict__ J, const double* __restrict__ Dinv,
-                                       const double* __restrict__ rhs, const double* __restrict__ mu,
-                                       const double* __restrict__ minv, int nc, double relax, double alpha,
-                                       double* __restrict__ lam, double* __restrict__ uc) {
-    const int a = ida[c], b = idb[c];
-    const double muc = mu[c];
+// One colour class of the PGS block update (projected_gauss_seidel.cpp:144-169 with project(),
+// :11-26), 4 lanes per contact as the projected-Jacobi contact pass: lane k < 3 loads row k's
+// 9 Jacobian entries and computes that row's correction; the group's values meet through
+// shuffles for the projection, and the rows' side impulses are summed by an xor tree, which adds
+// them in row order (lane 3 holds exact zeros), as a sequential sum over the rows does.  Lane 0
+// writes u_corr of body A, lane 1 that of body B.  Blocks of a colour share no body, so the
+// read-modify-write of u_corr is race-free.  The contact data is in colour order (k_gs_pack):
+// the class is the contiguous range [off, off + n) of arrays with row stride ns, so every load
+// is coalesced; four lanes per contact keep four times the waves in flight on the small classes,
+// whose time is load latency.
+__global__ void __launch_bounds__(kBlock) k_gs_color4(int n, int off, int ns, const int* __restrict__ ids,
+                                                      const double* __restrict__ pack, const double* __restrict__ minv,
+                                                      double relax, double alpha, double* __restrict__ lam,
+                                                      double* __restrict__ uc, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int i = t / kLanesPerContact, k = t % kLanesPerContact;
+    const bool live = i < n;
+    const int c = off + i;
+    const double* J = pack;
+    const double* Dinv = pack + 27 * (size_t)ns;
+    const double* rhs = pack + 30 * (size_t)ns;
+    const double* mu = pack + 36 * (size_t)ns;
+    int a = -1, b = -1;
+    double muc = 0;
+    if (live) {
+        a = ids[c];
+        b = ids[ns + c];
+        muc = mu[c];
+    }
     const int nr = muc > 0 ? 3 : 1;
-    ContactJ j;
-    load_J(J, nc, c, nr, j);
-    double ua[6], ub[6];
-    load_body(uc, a, ua);
-    load_body(uc, b, ub);
-    double l[3] = {0, 0, 0}, dl[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) break;
-        const double r = rhs[(size_t)k * nc + c] - row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
-        dl[k] = (Dinv[(size_t)k * nc + c] * r) * relax;
-        dl[k] *= alpha;
-        l[k] = lam[(size_t)k * nc + c] + dl[k];
+    const bool row = live && k < nr;
+    double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
+    double ua[6] = {0, 0, 0, 0, 0, 0}, ub[6] = {0, 0, 0, 0, 0, 0};
+    double l = 0, dl = 0;
+    if (live) {
+        load_body(uc, a, ua);
+        load_body(uc, b, ub);
     }
-    // project (projected_gauss_seidel.cpp:11-26)
-    const double ln = l[0];
-    l[0] = fmin(ln, 0.0);
-    dl[0] += l[0] - ln;
-    if (nr == 3) {
-        const double t1 = l[1], t2 = l[2];
-        const double tn = sqrt(t1 * t1 + t2 * t2);
-        if (tn > 0) {
-            const double s = fmin(1.0, -(muc * l[0]) / tn);
-            const double p1 = s * t1, p2 = s * t2;
-            dl[1] += p1 - t1;
-            dl[2] += p2 - t2;
-            l[1] = p1;
-            l[2] = p2;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) break;
-        lam[(size_t)k * nc + c] = l[k];
-    }
-    double lin[3] = {0, 0, 0}, angA[3] = {0, 0, 0}, angB[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (k >= nr) break;
+    if (row) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            lin[m] += j.d[k][m] * dl[k];
-            angA[m] += j.aA[k][m] * dl[k];
-            angB[m] += j.aB[k][m] * dl[k];
+            d[m] = J[(size_t)(3 * k + m) * ns + c];
+            aA[m] = J[(size_t)(9 + 3 * k + m) * ns + c];
+            aB[m] = J[(size_t)(18 + 3 * k + m) * ns + c];
+        }
+        const double r = rhs[(size_t)k * ns + c] - row_dot(d, aA, aB, ua, ub);
+        dl = (Dinv[(size_t)k * ns + c] * r) * relax;
+        dl *= alpha;
+        l = lam[(size_t)k * ns + c] + dl;
+    }
+    // project (projected_gauss_seidel.cpp:11-26): lambda_n <= 0, tangential part scaled into the cone
+    const int base = threadIdx.x & ~(kLanesPerContact - 1);
+    const double l0 = __shfl(l, base, 64);
+    const double l1 = __shfl(l, base + 1, 64);
+    const double l2 = __shfl(l, base + 2, 64);
+    const double ln = fmin(l0, 0.0);
+    if (k == 0) {
+        dl += ln - l;
+        l = ln;
+    } else if (nr == 3 && k < 3) {
+        const double tn = sqrt(l1 * l1 + l2 * l2);
+        if (tn > 0) {
+            const double sc = fmin(1.0, -(muc * ln) / tn);
+            const double pv = sc * l;
+            dl += pv - l;
+            l = pv;
         }
     }
-    if (a >= 0) {
+    if (row) lam[(size_t)k * ns + c] = l;
+    double v9[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        v9[m] = d[m] * dl;
+        v9[3 + m] = aA[m] * dl;
+        v9[6 + m] = aB[m] * dl;
+    }
+#pragma unroll
+    for (int off = 1; off < kLanesPerContact; off <<= 1)
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
+    if (live && k == 0 && a >= 0) {
         const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)a * kMinvStride);
-        const double o[6] = {ua[0] + m4.x * -lin[0], ua[1] + m4.x * -lin[1], ua[2] + m4.x * -lin[2],
-                             ua[3] + m4.y * -angA[0], ua[4] + m4.z * -angA[1], ua[5] + m4.w * -angA[2]};
+        const double o[6] = {ua[0] + m4.x * -v9[0], ua[1] + m4.x * -v9[1], ua[2] + m4.x * -v9[2],
+                             ua[3] + m4.y * -v9[3], ua[4] + m4.z * -v9[4], ua[5] + m4.w * -v9[5]};
         store6(uc + (size_t)a * kBodyStride, o);
     }
-    if (b >= 0) {
+    if (live && k == 1 && b >= 0) {
         const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)b * kMinvStride);
-        const double o[6] = {ub[0] + m4.x * lin[0], ub[1] + m4.x * lin[1], ub[2] + m4.x * lin[2],
-                             ub[3] + m4.y * angB[0], ub[4] + m4.z * angB[1], ub[5] + m4.w * angB[2]};
+        const double o[6] = {ub[0] + m4.x * v9[0], ub[1] + m4.x * v9[1], ub[2] + m4.x * v9[2],
+                             ub[3] + m4.y * v9[6], ub[4] + m4.z * v9[7], ub[5] + m4.w * v9[8]};
         store6(uc + (size_t)b * kBodyStride, o);
     }
 }
 
-// One colour class: each lane runs the PGS block update of one contact.  Blocks of a colour
-// share no body, so the read-modify-write of u_corr is race-free.
-__global__ void __launch_bounds__(kBlock) k_gs_color(int n, const int* __restrict__ blocks, const int* __restrict__ ida,
-                                                     const int* __restrict__ idb, const double* __restrict__ J,
-                                                     const double* __restrict__ Dinv, const double* __restrict__ rhs,
-                                                     const double* __restrict__ mu, const double* __restrict__ minv,
-                                                     int nc, double relax, double alpha, double* __restrict__ lam,
-                                                     double* __restrict__ uc, const int* __restrict__ ctrl) {
-    if (ctrl[0]) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    gs_block_update(blocks[i], ida, idb, J, Dinv, rhs, mu, minv, nc, relax, alpha, lam, uc);
+// Colour-order copy of the sweep's contact data: position i of the colour-sorted list holds
+// contact blocks[i] (layout of cdx_ctx::d_gs_pack).
+__global__ void k_gs_pack(int ns, int nc, const int* __restrict__ blocks, const int* __restrict__ ida,
+                          const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ Dinv,
+                          const double* __restrict__ rhs, const double* __restrict__ lam, const double* __restrict__ mu,
+                          double* __restrict__ pack, int* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int c = blocks[i];
+    for (int r = 0; r < 27; ++r) pack[(size_t)r * ns + i] = J[(size_t)r * nc + c];
+    for (int k = 0; k < 3; ++k) {
+        pack[(size_t)(27 + k) * ns + i] = Dinv[(size_t)k * nc + c];
+        pack[(size_t)(30 + k) * ns + i] = rhs[(size_t)k * nc + c];
+        pack[(size_t)(33 + k) * ns + i] = lam[(size_t)k * nc + c];
+    }
+    pack[(size_t)36 * ns + i] = mu[c];
+    ids[i] = ida[c];
+    ids[ns + i] = idb[c];
+}
+
+// lambda back from colour order to contact order.
+__global__ void k_gs_unpack(int ns, int nc, const int* __restrict__ blocks, const double* __restrict__ pack,
+                            double* __restrict__ lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int c = blocks[i];
+    for (int k = 0; k < 3; ++k) lam[(size_t)k * nc + c] = pack[(size_t)(33 + k) * ns + i];
 }
 
 // Per-body part of the residual on v = u_free - u_corr (projected_gauss_seidel.cpp:124-131), and
@@ -357,8 +397,8 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     const int nb = ctx->nb, nc = ctx->nc;
     if (nb == 0) {
         ctx->solved = true;
-    ctx->pj_last = false;
-    ctx->pjj_last = false;
+        ctx->pj_last = false;
+        ctx->pjj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_incidence(ctx));
@@ -371,8 +411,8 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
                                     ctx->stream));
         CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->solved = true;
-    ctx->pj_last = false;
-    ctx->pjj_last = false;
+        ctx->pj_last = false;
+        ctx->pjj_last = false;
         return CDX_OK;
     }
     CDX_TRY(build_coloring(ctx));
@@ -381,15 +421,23 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
                ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp);
     CDX_LAUNCH(ctx, "gs_init", k_gs_init_ucorr, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                ctx->d_imp, ctx->d_x, ctx->d_uprev);
+    const int ns = ctx->color_ptr.empty() ? 0 : ctx->color_ptr.back();
+    if (ns > 0) {
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_gs_pack, (size_t)37 * ns, &ctx->gs_pack_cap, "colour-ordered contacts"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_gs_ids, (size_t)2 * ns, &ctx->gs_ids_cap, "colour-ordered contact ids"));
+        CDX_LAUNCH(ctx, "gs_pack", k_gs_pack, grid_for(ns), kBlock, ns, nc, ctx->d_color_blocks, ctx->d_ida, ctx->d_idb,
+                   ctx->d_J, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_mu, ctx->d_gs_pack, ctx->d_gs_ids);
+    }
     CDX_HIP(ctx, hipGetLastError());
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb, &ctx->partial_cap, "residual partials"));
+    double* lam_ns = ctx->d_gs_pack + (size_t)33 * ns;
     auto sweep = [&](int it, int) -> int {
         for (int c = 0; c < ctx->ncolors; ++c) {
             const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
             if (n == 0) continue;
-            CDX_LAUNCH(ctx, "gs_color", k_gs_color, grid_for(n), kBlock, n, ctx->d_color_blocks + ctx->color_ptr[c], ctx->d_ida,
-                       ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias, ctx->d_mu, ctx->d_minv, nc, prm->relaxation, prm->alpha,
-                       ctx->d_p, ctx->d_x, ctx->d_ctrl);
+            CDX_LAUNCH(ctx, "gs_color", k_gs_color4, grid_for(n * kLanesPerContact), kBlock, n, ctx->color_ptr[c], ns,
+                       ctx->d_gs_ids, ctx->d_gs_pack, ctx->d_minv, prm->relaxation, prm->alpha, lam_ns, ctx->d_x,
+                       ctx->d_ctrl);
         }
         CDX_LAUNCH(ctx, "gs_body_residual", k_gs_residual, gb, kBlock, nb, ctx->d_dof, ctx->d_ufree, ctx->d_x, ctx->d_uprev,
                    prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl);
@@ -402,6 +450,9 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     // colours measured twice as slow per sweep: each device-scope barrier writes back and
     // invalidates the L2 of every XCD, so every colour re-fetched its data.)
     const int st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
+    if (ns > 0)
+        CDX_LAUNCH(ctx, "gs_unpack", k_gs_unpack, grid_for(ns), kBlock, ns, nc, ctx->d_color_blocks, ctx->d_gs_pack,
+                   ctx->d_p);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_bodies, gb, kBlock, nb, ctx->d_ufree, ctx->d_x);
     CDX_LAUNCH(ctx, "gs_finish", k_gs_finish_contacts, grid_for(nc), kBlock, nc, ctx->d_p);
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -65,7 +65,6 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
 // values meet through shuffles inside the group of 4 for the Coulomb projection (every lane
 // evaluates the same scale factor), the side impulses J^T dp are summed over the rows by an xor
 // tree, and lanes 0-2 write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
-constexpr int kLanesPerContact = 4;
 
 // project() (projected_jacobi.cpp:100-119) of one contact held by a group of 4 lanes (lane k has
 // the trial value of row k): every lane evaluates the same scale factor from the group's values.
@@ -201,6 +200,15 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
     const bool live = bi < nb;
+    // the body's own values are loaded before the gather chain, so their latency overlaps it
+    int nd = 0, js = -1;
+    double mk = 0, u = 0;
+    if (live && l < 6) {
+        nd = dof[bi];
+        if (jslot) js = jslot[bi];
+        mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+        u = x[(size_t)bi * kBodyStride + l];
+    }
     double acc[6] = {0, 0, 0, 0, 0, 0};
     int i0 = 0, i1 = 0;
     if (live) {
@@ -214,20 +222,16 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
         for (int k = 0; k < 6; ++k) acc[k] += __shfl_xor(acc[k], off, kLanesPerBody);
     double e2 = 0;
     if (live && l < 6) {
-        const int nd = dof[bi];
         double a = acc[0];
 #pragma unroll
         for (int k = 1; k < 6; ++k)
             if (l == k) a = acc[k];
-        if (jslot && jslot[bi] >= 0) {
+        if (js >= 0) {
             if (l < nd) tr[(size_t)bi * 6 + l] = a;  // W^T dp of a jointed body: the chain solve applies it
         } else if (l < nd) {
-            const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
             const double M = mk > 0 ? 1.0 / mk : 0.0;
-            double* xp = x + (size_t)bi * kBodyStride + l;
-            const double u = *xp;
             const double un = u + a / M;
-            if (i1 > i0) *xp = un;
+            if (i1 > i0) x[(size_t)bi * kBodyStride + l] = un;
             if (check) {
                 const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
                 const double e = (un - u) / q;
