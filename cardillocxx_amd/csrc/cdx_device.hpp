@@ -10,6 +10,37 @@ namespace cdx {
 
 constexpr int kLanesPerContact = 4;  // lanes per contact in the contact passes: one per row, one idle
 
+// Exchanges inside aligned groups of lanes through DPP (a VALU modifier; ds_bpermute would go
+// through the LDS crossbar).  quad_perm<CTRL>: lane j of each group of 4 reads lane
+// (CTRL >> 2j) & 3 of its group.  half_mirror: lane j of each group of 8 reads lane 7 - j.
+// Every lane of the group must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int I>
+__device__ __forceinline__ double quad_bcast(double v) {  // lane I of the group of 4
+    return dpp_double<I * 0x55>(v);
+}
+__device__ __forceinline__ double quad_xor1(double v) { return dpp_double<0xB1>(v); }  // quad_perm [1,0,3,2]
+__device__ __forceinline__ double quad_xor2(double v) { return dpp_double<0x4E>(v); }  // quad_perm [2,3,0,1]
+__device__ __forceinline__ double half_mirror(double v) { return dpp_double<0x141>(v); }
+
+// Sum over a group of 4 lanes, the same value in every lane: (v0 + v1) + (v2 + v3).
+__device__ __forceinline__ double quad_sum(double v) {
+    v += quad_xor1(v);
+    return v + quad_xor2(v);
+}
+
+// Sum over a group of 8 lanes, the same value in every lane: the two quad sums added.
+__device__ __forceinline__ double oct_sum(double v) {
+    v = quad_sum(v);
+    return v + half_mirror(v);
+}
+
 __device__ inline void load_body(const double* __restrict__ x, int b, double u[6]) {
     if (b < 0) {
 #pragma unroll

@@ -203,8 +203,8 @@ __global__ void k_gs_init_ucorr(int nb, const int* __restrict__ dof, const doubl
 // One colour class of the PGS block update (projected_gauss_seidel.cpp:144-169 with project(),
 // :11-26), 4 lanes per contact as the projected-Jacobi contact pass: lane k < 3 loads row k's
 // 9 Jacobian entries and computes that row's correction; the group's values meet through
-// shuffles for the projection, and the rows' side impulses are summed by an xor tree, which adds
-// them in row order (lane 3 holds exact zeros), as a sequential sum over the rows does.  Lane 0
+// DPP exchanges for the projection, and the rows' side impulses are summed by quad_sum,
+// (r0 + r1) + (r2 + 0), which is the sequential sum over the rows (lane 3 holds exact zeros).  Lane 0
 // writes u_corr of body A, lane 1 that of body B.  Blocks of a colour share no body, so the
 // read-modify-write of u_corr is race-free.  The contact data is in colour order (k_gs_pack):
 // the class is the contiguous range [off, off + n) of arrays with row stride ns, so every load
@@ -252,10 +252,9 @@ __global__ void __launch_bounds__(kBlock) k_gs_color4(int n, int off, int ns, co
         l = lam[(size_t)k * ns + c] + dl;
     }
     // project (projected_gauss_seidel.cpp:11-26): lambda_n <= 0, tangential part scaled into the cone
-    const int base = threadIdx.x & ~(kLanesPerContact - 1);
-    const double l0 = __shfl(l, base, 64);
-    const double l1 = __shfl(l, base + 1, 64);
-    const double l2 = __shfl(l, base + 2, 64);
+    const double l0 = quad_bcast<0>(l);
+    const double l1 = quad_bcast<1>(l);
+    const double l2 = quad_bcast<2>(l);
     const double ln = fmin(l0, 0.0);
     if (k == 0) {
         dl += ln - l;
@@ -278,9 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_gs_color4(int n, int off, int ns, co
         v9[6 + m] = aB[m] * dl;
     }
 #pragma unroll
-    for (int off = 1; off < kLanesPerContact; off <<= 1)
-#pragma unroll
-        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
+    for (int m = 0; m < 9; ++m) v9[m] = quad_sum(v9[m]);
     if (live && k == 0 && a >= 0) {
         const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)a * kMinvStride);
         const double o[6] = {ua[0] + m4.x * -v9[0], ua[1] + m4.x * -v9[1], ua[2] + m4.x * -v9[2],

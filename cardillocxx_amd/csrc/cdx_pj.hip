@@ -61,18 +61,17 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
 // loads only its row's 9 Jacobian entries, so the kernel needs about half the registers of a
 // one-lane-per-contact form and runs at twice the occupancy (64 VGPRs: recomputing R = alpha / D
 // in the sweep instead of streaming it costs 16 more and two waves per SIMD, a net loss).  The
-// bias is read only when the problem has one.  The normal and tangential
-// values meet through shuffles inside the group of 4 for the Coulomb projection (every lane
-// evaluates the same scale factor), the side impulses J^T dp are summed over the rows by an xor
-// tree, and lanes 0-2 write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
+// bias is read only when the problem has one.  The normal and tangential values meet through
+// DPP exchanges inside the group of 4 for the Coulomb projection (every lane evaluates the same
+// scale factor), the side impulses J^T dp are summed over the rows by quad_sum, and lanes 0-2
+// write the contact's compact record [lin | angA | angB] (72 contiguous bytes).
 
 // project() (projected_jacobi.cpp:100-119) of one contact held by a group of 4 lanes (lane k has
 // the trial value of row k): every lane evaluates the same scale factor from the group's values.
 __device__ __forceinline__ double project4(double pt, int k, double muc) {
-    const int base = threadIdx.x & ~(kLanesPerContact - 1);
-    const double zn = __shfl(pt, base, 64);
-    const double z1 = __shfl(pt, base + 1, 64);
-    const double z2 = __shfl(pt, base + 2, 64);
+    const double zn = quad_bcast<0>(pt);
+    const double z1 = quad_bcast<1>(pt);
+    const double z2 = quad_bcast<2>(pt);
     const double pn = fmax(zn, 0.0);
     double pnew = pn;
     if (k > 0) {
@@ -88,13 +87,12 @@ __device__ __forceinline__ double project4(double pt, int k, double muc) {
 
 // Row k of R_block t for the contact held by a group of 4 lanes (lane k has t_k; block mode):
 // sum_l Rb(k, l) t_l over the contact's rows, in row order (BlockDiagonal::applyTo,
-// block_diagonal.hpp:103-124).  Every lane takes part in the shuffles; rows only read Rb.
+// block_diagonal.hpp:103-124).  Every lane takes part in the exchanges; rows only read Rb.
 __device__ __forceinline__ double block_apply4(double t, int k, int c, int nc, int nr, bool row,
                                                const double* __restrict__ Rb) {
-    const int base = threadIdx.x & ~(kLanesPerContact - 1);
-    const double t0 = __shfl(t, base, 64);
-    const double t1 = __shfl(t, base + 1, 64);
-    const double t2 = __shfl(t, base + 2, 64);
+    const double t0 = quad_bcast<0>(t);
+    const double t1 = quad_bcast<1>(t);
+    const double t2 = quad_bcast<2>(t);
     if (!row) return 0.0;
     const double* r = Rb + (size_t)(3 * k) * nc + c;
     if (nr == 1) return r[0] * t0;
@@ -102,7 +100,7 @@ __device__ __forceinline__ double block_apply4(double t, int k, int c, int nc, i
 }
 
 // The contact's compact record [lin | angA | angB] = sum over rows of (d_k, aA_k, aB_k) dp_k,
-// summed over the group of 4 by an xor tree; lane 0 writes lin, lane 1 angA (dynamic A only),
+// summed over the group of 4 by quad_sum; lane 0 writes lin, lane 1 angA (dynamic A only),
 // lane 2 angB (dynamic B only).
 __device__ __forceinline__ void write_record4(bool contact, int k, int c, int a, int b, const double d[3],
                                               const double aA[3], const double aB[3], double dp,
@@ -115,9 +113,7 @@ __device__ __forceinline__ void write_record4(bool contact, int k, int c, int a,
         v9[6 + m] = aB[m] * dp;
     }
 #pragma unroll
-    for (int off = 1; off < kLanesPerContact; off <<= 1)
-#pragma unroll
-        for (int m = 0; m < 9; ++m) v9[m] += __shfl_xor(v9[m], off, kLanesPerContact);
+    for (int m = 0; m < 9; ++m) v9[m] = quad_sum(v9[m]);
     if (contact && k < 3 && (k == 0 || (k == 1 ? a >= 0 : b >= 0))) {
         const double o0 = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
         const double o1 = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
@@ -182,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_contacts4(int nc, int nblocks, co
 }
 
 // Body pass with 8 lanes per body: lane l sums incidences l, l+8, ... of its body, the 8 partial
-// twists are combined by a fixed xor-shuffle tree (deterministic), and lanes 0..5 each update one
+// twists are combined by a fixed DPP tree (oct_sum, deterministic), and lanes 0..5 each update one
 // velocity component.  Eight times the lanes of one-lane-per-body keeps enough waves in flight to
 // hide the latency of the index -> side-impulse gather chain.
 constexpr int kLanesPerBody = 8;
@@ -217,9 +213,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
         for (int i = i0 + l; i < i1; i += kLanesPerBody) gather_side9(imp9, incidx[i], acc);
     }
 #pragma unroll
-    for (int off = kLanesPerBody / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] += __shfl_xor(acc[k], off, kLanesPerBody);
+    for (int k = 0; k < 6; ++k) acc[k] = oct_sum(acc[k]);
     double e2 = 0;
     if (live && l < 6) {
         double a = acc[0];
@@ -598,9 +592,7 @@ __device__ __forceinline__ void gather_body8(int bi, int l, bool live, const int
         for (int i = i0 + l; i < i1; i += kLanesPerBody) gather_side9(imp9, incidx[i], acc);
     }
 #pragma unroll
-    for (int off = kLanesPerBody / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] += __shfl_xor(acc[k], off, kLanesPerBody);
+    for (int k = 0; k < 6; ++k) acc[k] = oct_sum(acc[k]);
     *touched = i1 > i0;
 }
 
