@@ -135,6 +135,32 @@ def test_colored_gs_matches_oracle(ctx, case):
         assert rel(ctx.impulses(), imp_ref) <= TOL
 
 
+def test_colored_gs_context_reuse_and_static_pairs(ctx):
+    """The colour-ordered contact copy is rebuilt for every solve: one context solving problems of
+    different sizes in turn (its buffers grow and are reused) gives, bit for bit, what a fresh
+    context gives; contacts with two static sides are left out of the colouring and keep their
+    impulses' slots untouched in contact order."""
+    dt = 1e-3
+    big = scenes.random_problem(nb=400, nc=900, seed=21)
+    small = scenes.random_problem(nb=30, nc=50, seed=22, p_frictionless=0.5)
+    both_static = scenes.random_problem(nb=60, nc=120, seed=23)
+    both_static.body_a[::7] = -1
+    both_static.body_b[::7] = -1
+    prm = dict(max_iter=40, tol_abs=1e-30, tol_rel=1e-30)
+    for prob in (big, small, both_static, big):
+        upload(ctx, prob)
+        info = ctx.gs_colored_solve(dt, 1.0, gpu_params(**prm))
+        v, imp = ctx.velocities(), ctx.impulses()
+        fresh = native.Context(0)
+        upload(fresh, prob)
+        finfo = fresh.gs_colored_solve(dt, 1.0, gpu_params(**prm))
+        assert info.iters == finfo.iters and info.ncolors == finfo.ncolors
+        assert np.array_equal(v, fresh.velocities()) and np.array_equal(imp, fresh.impulses())
+        st, v_ref, imp_ref, res = oracle.colored_gs_solve(prob, dt, oracle.params(**prm))
+        assert st == 0 and res.iters == info.iters
+        assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
 def kinetic_energy(prob, v):
     return 0.5 * np.sum(v * v / prob.minv)
 
