@@ -166,6 +166,174 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     }
 }
 
+// ---- one scene per workgroup with its velocities and side-impulse records in LDS
+//
+// When every scene's bodies (6 doubles each) and contact records (9 doubles each) fit in
+// kBatchLdsBytes, a workgroup hosting one scene keeps both in LDS through the sweeps: a sweep then
+// reads from global memory only the contact rows (ids, J, R, bias, mu, p) and the incidence
+// lists, and the body pass and the contact pass's body gathers run at LDS latency.  A stagnating
+// scene that runs to the iteration cap sets a batch step's time, and its sweep is a chain of
+// such latencies.  The arithmetic is that of pj_contact_sweep / pj_body_sweep, on local indices.
+constexpr int kBatchLdsBytes = 60 * 1024;  // dynamic LDS, under the 64 KiB a workgroup gets by default
+
+__device__ inline void lds_body6(const double* __restrict__ xs, int lb, double u[6]) {
+    if (lb < 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) u[k] = 0.0;
+        return;
+    }
+    const double2* q = reinterpret_cast<const double2*>(xs + 6 * (size_t)lb);
+    const double2 a0 = q[0], a1 = q[1], a2 = q[2];
+    u[0] = a0.x;
+    u[1] = a0.y;
+    u[2] = a1.x;
+    u[3] = a1.y;
+    u[4] = a2.x;
+    u[5] = a2.y;
+}
+
+// pj_contact_sweep with x read from the scene's LDS copy (body b at xs + 6 (b - b0)) and the
+// record written to the LDS records (contact c at rs + 9 (c - c0)).
+__device__ inline void contact_sweep_lds(int c, int c0, int b0, int nc, const int* __restrict__ ida,
+                                         const int* __restrict__ idb, const double* __restrict__ J,
+                                         const double* __restrict__ R, const double* __restrict__ bias, int use_bias,
+                                         const double* __restrict__ mu, double* __restrict__ p,
+                                         const double* __restrict__ xs, double* __restrict__ rs) {
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6];
+    lds_body6(xs, a < 0 ? -1 : a - b0, ua);
+    lds_body6(xs, b < 0 ? -1 : b - b0, ub);
+    double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        pold[k] = p[(size_t)k * nc + c];
+        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+    }
+    pn[0] = fmax(pn[0], 0.0);
+    if (nr == 3) {
+        const double t1 = pn[1], t2 = pn[2];
+        const double tn2 = t1 * t1 + t2 * t2;
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (muc * pn[0]) / sqrt(tn2));
+            pn[1] = sc * t1;
+            pn[2] = sc * t2;
+        }
+    }
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = pn[k] - pold[k];
+        p[(size_t)k * nc + c] = pn[k];
+    }
+    write_side_impulses9(rs, c - c0, a, b, nr, j, dp);
+}
+
+// pj_body_sweep (no joints) on the LDS copies: incidence e = 2c + side reads rs at 9 (c - c0).
+__device__ inline double body_sweep_lds(int bi, int b0, int c0, const int* __restrict__ dof,
+                                        const double* __restrict__ minv, const int* __restrict__ incptr,
+                                        const int* __restrict__ incidx, const double* __restrict__ rs,
+                                        double* __restrict__ xs, int check, double tol_rel, double tol_abs) {
+    double e2 = 0;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int i0 = incptr[bi], i1 = incptr[bi + 1];
+    for (int i = i0; i < i1; ++i) gather_side9(rs, incidx[i] - 2 * c0, acc);
+    double u[6];
+    lds_body6(xs, bi - b0, u);
+    const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
+    const int nd = dof[bi];
+    double un[6];
+    const double Ml = m4.x > 0 ? 1.0 / m4.x : 0.0;
+    const double Ma[3] = {m4.y > 0 ? 1.0 / m4.y : 0.0, m4.z > 0 ? 1.0 / m4.z : 0.0, m4.w > 0 ? 1.0 / m4.w : 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double M = k < 3 ? Ml : Ma[k - 3];
+        un[k] = (k < nd) ? u[k] + acc[k] / M : u[k];
+    }
+    if (i1 > i0) {
+        double2* q = reinterpret_cast<double2*>(xs + 6 * (size_t)(bi - b0));
+        q[0] = make_double2(un[0], un[1]);
+        q[1] = make_double2(un[2], un[3]);
+        q[2] = make_double2(un[4], un[5]);
+    }
+    if (check) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= nd) break;
+            const double q = fmax(fabs(un[k]), fabs(u[k])) * tol_rel + tol_abs;
+            const double e = (un[k] - u[k]) / q;
+            e2 += e * e;
+        }
+    }
+    return e2;
+}
+
+__global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
+    int ns, int lds_bodies, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
+    const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
+    const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ mu,
+    const double* __restrict__ rest, const double* __restrict__ vb, double* __restrict__ R, double* __restrict__ bias,
+    double* __restrict__ p, double* __restrict__ imp, const int* __restrict__ dof, const double* __restrict__ minv,
+    const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
+    const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
+    int use_bias, int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
+    double* __restrict__ res_out) {
+    extern __shared__ double sdyn[];
+    __shared__ double wsum[kMaxBatchBlock / 64];
+    if ((int)blockIdx.x >= ns) return;  // uniform over the workgroup
+    const int s = order[blockIdx.x];
+    const int b0 = boff[s], b1 = boff[s + 1], c0 = coff[s], c1 = coff[s + 1];
+    const bool rows = srows[s] > 0;
+    const int lt = threadIdx.x, gsize = blockDim.x;
+    double* xs = sdyn;                                // [lds_bodies][6]
+    double* rs = sdyn + 6 * (size_t)lds_bodies;      // [contacts of the scene][9]
+    // setup in global memory, as the grid-wide solve, then x0 into LDS
+    for (int c = c0 + lt; c < c1; c += gsize)
+        pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+    __syncthreads();
+    for (int b = b0 + lt; b < b1; b += gsize)
+        pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
+    __syncthreads();
+    for (int i = lt; i < 6 * (b1 - b0); i += gsize) xs[i] = x[(size_t)(b0 + i / 6) * kBodyStride + i % 6];
+    __syncthreads();
+    bool done = !rows;  // the same in every thread: one scene per workgroup
+    int iters = 0;
+    double res = NAN;
+    for (int it = 0; it < max_iter && !done; ++it) {
+        for (int c = c0 + lt; c < c1; c += gsize)
+            contact_sweep_lds(c, c0, b0, nc, ida, idb, J, R, bias, use_bias, mu, p, xs, rs);
+        __syncthreads();
+        const int check = (it % check_every) == 0;
+        double e2 = 0;
+        for (int b = b0 + lt; b < b1; b += gsize)
+            e2 += body_sweep_lds(b, b0, c0, dof, minv, incptr, incidx, rs, xs, check, tol_rel, tol_abs);
+        if (check) {
+            for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+            if ((lt & 63) == 0) wsum[lt >> 6] = e2;
+        }
+        __syncthreads();
+        iters = it + 1;
+        if (check) {
+            double t = 0;
+            for (int w = 0; w < (gsize >> 6); ++w) t += wsum[w];
+            res = 1.0 / sqrt((double)snv[s]) * sqrt(t);
+            if (res <= 1.0) done = true;
+        }
+    }
+    for (int i = lt; i < 6 * (b1 - b0); i += gsize) x[(size_t)(b0 + i / 6) * kBodyStride + i % 6] = xs[i];
+    if (lt == 0) {
+        iters_out[s] = rows ? iters : 0;
+        res_out[s] = res;
+    }
+}
+
 int upload_scenes(cdx_ctx* ctx) {
     const int ns = ctx->nscenes;
     CDX_TRY(ensure_alloc(ctx, &ctx->d_scene_i, 6 * (size_t)ns + 2, &ctx->scene_cap, "scene tables"));
@@ -250,6 +418,17 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
         ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha,                     \
         ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs,              \
         ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta, wave_lds)
+    const size_t lds_bytes =
+        sizeof(double) * (6 * (size_t)ctx->scene_max_bodies + 9 * (size_t)ctx->scene_max_contacts);
+    if (ctx->njrows == 0 && groups == 1 && lds_bytes <= (size_t)kBatchLdsBytes) {
+        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, block, lds_bytes, ns, ctx->scene_max_bodies, d, d + ns,
+                       d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof,
+                       ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt,
+                       prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every,
+                       prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+        return hip_check(ctx, hipGetLastError(), "batch launch");
+    }
     if (ctx->njrows > 0) {
         if (shm > 64 * 1024)
             CDX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_pj<true>),
@@ -296,6 +475,11 @@ int cdx_set_scenes(cdx_ctx* ctx, int nscenes, const int* body_off, const int* co
     ctx->scene_coff.assign(contact_off, contact_off + nscenes + 1);
     ctx->scene_nv = nv;
     ctx->scene_rows = rows;
+    ctx->scene_max_bodies = ctx->scene_max_contacts = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        ctx->scene_max_bodies = std::max(ctx->scene_max_bodies, body_off[s + 1] - body_off[s]);
+        ctx->scene_max_contacts = std::max(ctx->scene_max_contacts, contact_off[s + 1] - contact_off[s]);
+    }
     ctx->scene_order.resize(nscenes);
     std::iota(ctx->scene_order.begin(), ctx->scene_order.end(), 0);
     std::stable_sort(ctx->scene_order.begin(), ctx->scene_order.end(), [&](int x, int y) { return cost[x] > cost[y]; });

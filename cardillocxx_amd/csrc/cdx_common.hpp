@@ -134,6 +134,7 @@ struct cdx_ctx {
     // ---- batched scenes (cdx_set_scenes); cleared whenever bodies or contacts are set
     int nscenes = 0;
     std::vector<int> scene_boff, scene_coff, scene_nv, scene_rows, scene_order;
+    int scene_max_bodies = 0, scene_max_contacts = 0;
     int* d_scene_i = nullptr;  // [order | boff | coff | nv | rows | iters]
     size_t scene_cap = 0;
     double* d_scene_res = nullptr;

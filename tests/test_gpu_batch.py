@@ -52,6 +52,45 @@ def test_batch_matches_standalone_oracle(ctx, launch):
             assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL
 
 
+def test_batch_lds_scenes_equal_global_scenes(ctx):
+    """One scene per workgroup keeps the scene's velocities and side-impulse records in LDS when
+    every scene fits; two scenes per workgroup keep them in global memory.  The per-contact and
+    per-body arithmetic is the same, so are iterations, velocities and impulses, bit for bit; the
+    residual is summed over the group's waves and may differ in the last bits."""
+    probs = make_batch(24, seed=5)
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = 1e-7
+    out = []
+    try:
+        for launch in ((128, 1), (128, 2), (512, 1)):
+            upload_batch(ctx, probs)  # the same warm start for every launch
+            ctx.batch_set_launch(*launch)
+            iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+            out.append((iters, res, ctx.velocities(), ctx.impulses()))
+    finally:
+        ctx.batch_set_launch(256, 1)
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        assert np.allclose(out[0][1], o[1], rtol=1e-12, atol=0, equal_nan=True)
+        assert np.array_equal(out[0][2], o[2]) and np.array_equal(out[0][3], o[3])
+
+
+def test_batch_scene_too_large_for_lds(ctx):
+    """A scene whose velocities and records do not fit in the LDS budget sends the whole batch
+    through the global-memory kernel, also at one scene per workgroup."""
+    probs = [scenes.random_problem(nb=1400, nc=2000, seed=7), scenes.random_problem(nb=10, nc=12, seed=8)]
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.tol_abs = p.tol_rel = 1e-8
+    ctx.batch_set_launch(256, 1)
+    iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    v = ctx.velocities()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(tol_abs=1e-8, tol_rel=1e-8))
+        assert iters[s] == r.iters, s
+        assert rel(v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()], v_ref) <= TOL
+
+
 def test_batch_fixed_sweeps_equals_grid_solve(ctx):
     """Same per-contact arithmetic as the grid-wide solve (which sums the side impulses in a
     different order): a one-scene batch agrees with it to rounding."""
