@@ -168,10 +168,11 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
 
 // ---- one scene per workgroup with its velocities and side-impulse records in LDS
 //
-// When every scene's bodies (6 doubles each) and contact records (9 doubles each) fit in
-// kBatchLdsBytes, a workgroup hosting one scene keeps both in LDS through the sweeps: a sweep then
-// reads from global memory only the contact rows (ids, J, R, bias, mu, p) and the incidence
-// lists, and the body pass and the contact pass's body gathers run at LDS latency.  A stagnating
+// When every scene's bodies (6 doubles each), contact records (9 doubles each) and incidence
+// lists fit in kBatchLdsBytes, a workgroup hosting one scene keeps them in LDS through the sweeps:
+// a sweep then reads from global memory only the contact rows (ids, J, R, bias, mu, p) and the
+// bodies' inverse masses, and the body pass and the contact pass's body gathers run at LDS
+// latency.  A stagnating
 // scene that runs to the iteration cap sets a batch step's time, and its sweep is a chain of
 // such latencies.  The arithmetic is that of pj_contact_sweep / pj_body_sweep, on local indices.
 constexpr int kBatchLdsBytes = 60 * 1024;  // dynamic LDS, under the 64 KiB a workgroup gets by default
@@ -236,15 +237,16 @@ __device__ inline void contact_sweep_lds(int c, int c0, int b0, int nc, const in
     write_side_impulses9(rs, c - c0, a, b, nr, j, dp);
 }
 
-// pj_body_sweep (no joints) on the LDS copies: incidence e = 2c + side reads rs at 9 (c - c0).
-__device__ inline double body_sweep_lds(int bi, int b0, int c0, const int* __restrict__ dof,
-                                        const double* __restrict__ minv, const int* __restrict__ incptr,
-                                        const int* __restrict__ incidx, const double* __restrict__ rs,
+// pj_body_sweep (no joints) on the LDS copies: the scene's incidence lists with local offsets
+// (ip, body bi at ip[bi - b0]) and local entries e - 2 c0 (ix), which read rs at 9 (c - c0).
+__device__ inline double body_sweep_lds(int bi, int b0, const int* __restrict__ dof,
+                                        const double* __restrict__ minv, const int* __restrict__ ip,
+                                        const int* __restrict__ ix, const double* __restrict__ rs,
                                         double* __restrict__ xs, int check, double tol_rel, double tol_abs) {
     double e2 = 0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    const int i0 = incptr[bi], i1 = incptr[bi + 1];
-    for (int i = i0; i < i1; ++i) gather_side9(rs, incidx[i] - 2 * c0, acc);
+    const int i0 = ip[bi - b0], i1 = ip[bi - b0 + 1];
+    for (int i = i0; i < i1; ++i) gather_side9(rs, ix[i], acc);
     double u[6];
     lds_body6(xs, bi - b0, u);
     const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
@@ -276,7 +278,7 @@ __device__ inline double body_sweep_lds(int bi, int b0, int c0, const int* __res
 }
 
 __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
-    int ns, int lds_bodies, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
+    int ns, int lds_bodies, int lds_contacts, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
     const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
     const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ mu,
     const double* __restrict__ rest, const double* __restrict__ vb, double* __restrict__ R, double* __restrict__ bias,
@@ -293,7 +295,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     const bool rows = srows[s] > 0;
     const int lt = threadIdx.x, gsize = blockDim.x;
     double* xs = sdyn;                                // [lds_bodies][6]
-    double* rs = sdyn + 6 * (size_t)lds_bodies;      // [contacts of the scene][9]
+    double* rs = sdyn + 6 * (size_t)lds_bodies;      // [lds_contacts][9]
+    int* ip = reinterpret_cast<int*>(rs + 9 * (size_t)lds_contacts);  // [lds_bodies + 1]
+    int* ix = ip + lds_bodies + 1;                                     // [2 lds_contacts]
     // setup in global memory, as the grid-wide solve, then x0 into LDS
     for (int c = c0 + lt; c < c1; c += gsize)
         pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
@@ -302,6 +306,9 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
         pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
     __syncthreads();
     for (int i = lt; i < 6 * (b1 - b0); i += gsize) xs[i] = x[(size_t)(b0 + i / 6) * kBodyStride + i % 6];
+    const int e0 = incptr[b0], ne = incptr[b1] - e0;  // the scene's incidences: contiguous, at most 2 per contact
+    for (int i = lt; i <= b1 - b0; i += gsize) ip[i] = incptr[b0 + i] - e0;
+    for (int i = lt; i < ne; i += gsize) ix[i] = incidx[e0 + i] - 2 * c0;
     __syncthreads();
     bool done = !rows;  // the same in every thread: one scene per workgroup
     int iters = 0;
@@ -313,7 +320,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
         const int check = (it % check_every) == 0;
         double e2 = 0;
         for (int b = b0 + lt; b < b1; b += gsize)
-            e2 += body_sweep_lds(b, b0, c0, dof, minv, incptr, incidx, rs, xs, check, tol_rel, tol_abs);
+            e2 += body_sweep_lds(b, b0, dof, minv, ip, ix, rs, xs, check, tol_rel, tol_abs);
         if (check) {
             for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
             if ((lt & 63) == 0) wsum[lt >> 6] = e2;
@@ -419,9 +426,11 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
         ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs,              \
         ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta, wave_lds)
     const size_t lds_bytes =
-        sizeof(double) * (6 * (size_t)ctx->scene_max_bodies + 9 * (size_t)ctx->scene_max_contacts);
+        sizeof(double) * (6 * (size_t)ctx->scene_max_bodies + 9 * (size_t)ctx->scene_max_contacts) +
+        sizeof(int) * ((size_t)ctx->scene_max_bodies + 1 + 2 * (size_t)ctx->scene_max_contacts);
     if (ctx->njrows == 0 && groups == 1 && lds_bytes <= (size_t)kBatchLdsBytes) {
-        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, block, lds_bytes, ns, ctx->scene_max_bodies, d, d + ns,
+        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, block, lds_bytes, ns, ctx->scene_max_bodies,
+                       ctx->scene_max_contacts, d, d + ns,
                        d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
                        ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof,
                        ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt,
