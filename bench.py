@@ -393,8 +393,10 @@ def batched_leg(args, native, scenes):
         cu += int((per.astype(np.int64) * iters).sum())
         contacts_total += nc
         capped += int((iters >= p.max_iter).sum())
+        t_sample = 0.0  # the one-by-one comparison below is not part of the batched device path
         if s in sample_steps:
             # the same scenes one by one through the plain solver, on a sample of scenes
+            tsa = time.perf_counter()
             vb = ctx.velocities()
             pick = rng.choice(ns, size=min(args.obo_scenes, ns), replace=False)
             t_one = 0.0
@@ -414,6 +416,7 @@ def batched_leg(args, native, scenes):
             obo_time += t_one * ns / len(pick)
             batch_time_sampled += t_step_solve
             obo_solves += len(pick)
+            t_sample = time.perf_counter() - tsa
         ctx.moreau_update(dt, 1.0)
         q_prev = q
         q = ctx.positions()
@@ -437,7 +440,7 @@ def batched_leg(args, native, scenes):
         tc = time.perf_counter()
         prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         t_cpu += tb - ta
-        t_dev += tc - tb
+        t_dev += tc - tb - t_sample
         if s % 50 == 0 or s == steps - 1:
             log(f"batched step {s}: {nc} contacts (max {int(per.max()) if len(per) else 0}/scene), "
                 f"max sweeps {int(iters.max())}, solve {t_step_solve * 1e3:.2f} ms")
@@ -540,7 +543,9 @@ def chains_leg(args, native, scenes):
         contacts_total += nc
         capped += int((iters >= p.max_iter).sum())
         lam_new = ctx.joint_impulses()
+        t_sample = 0.0  # the one-by-one comparison below is not part of the batched device path
         if s in sample_steps:  # the same chains one by one through cdx_pj_solve
+            tsa = time.perf_counter()
             vb = ctx.velocities()
             pick = rng.choice(ns, size=min(args.obo_scenes, ns), replace=False)
             t_one = 0.0
@@ -562,6 +567,7 @@ def chains_leg(args, native, scenes):
             obo_time += t_one * ns / len(pick)
             batch_time_sampled += t_step
             obo_solves += len(pick)
+            t_sample = time.perf_counter() - tsa
         ctx.moreau_update(dt, 1.0)
         q, v, imp = ctx.positions(), ctx.velocities(), ctx.impulses()
         lam = lam_new
@@ -570,7 +576,7 @@ def chains_leg(args, native, scenes):
             raise RuntimeError(f"chains: non-finite state at step {s}")
         prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         t_cpu += tb - ta
-        t_dev += tc - tb
+        t_dev += tc - tb - t_sample
         if s % 50 == 0 or s == steps - 1:
             log(f"chains step {s}: {nc} contacts, max sweeps {int(iters.max())}, solve {t_step * 1e3:.2f} ms")
     out = {
