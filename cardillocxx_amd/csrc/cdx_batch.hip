@@ -429,7 +429,11 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
         sizeof(double) * (6 * (size_t)ctx->scene_max_bodies + 9 * (size_t)ctx->scene_max_contacts) +
         sizeof(int) * ((size_t)ctx->scene_max_bodies + 1 + 2 * (size_t)ctx->scene_max_contacts);
     if (ctx->njrows == 0 && groups == 1 && lds_bytes <= (size_t)kBatchLdsBytes) {
-        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, block, lds_bytes, ns, ctx->scene_max_bodies,
+        // at least one lane per contact and per body of the largest scene (up to 512), so that a
+        // stagnating scene's passes take one round each: its sweep latency sets the step time
+        const int need = (std::max(ctx->scene_max_bodies, ctx->scene_max_contacts) + 63) / 64 * 64;
+        const int lds_block = std::min(kMaxBatchBlock, std::max(block, need));
+        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, lds_block, lds_bytes, ns, ctx->scene_max_bodies,
                        ctx->scene_max_contacts, d, d + ns,
                        d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
                        ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof,

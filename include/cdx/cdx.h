@@ -170,7 +170,10 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
                        double* residuals);
 
 /* Launch shape of the batched solve: `block` threads per workgroup (64..512) hosting `groups`
- * scenes (block/groups must be a multiple of 64). */
+ * scenes (block/groups must be a multiple of 64).  With groups = 1, no joint rows and every
+ * scene's velocities, side-impulse records and incidence lists within 60 KiB, each scene is kept
+ * in LDS through its sweeps, and the workgroup gets at least one thread per contact and per body
+ * of the largest scene (up to 512). */
 int cdx_batch_set_launch(cdx_ctx* ctx, int block, int groups);
 
 /* Autotuner: times every launch shape on the current batch (warm start restored before each
