@@ -55,7 +55,7 @@ def log(msg):
 def committed_traffic(kernel, contacts):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if it was
     taken on a problem with the same contact count (FETCH_SIZE/WRITE_SIZE passes scaled by a
-    256 MiB copy calibration); None otherwise."""
+    device-copy calibration); None otherwise."""
     path = os.path.join(ROOT, "profiles", "pj_contacts4_pile100k_hbm_traffic.txt")
     try:
         vals = {}
