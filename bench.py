@@ -458,7 +458,10 @@ def batched_leg(args, native, scenes):
         "contact_updates_per_s": cu / t_solve, "mean_contacts_per_step": contacts_total / steps,
         "max_contacts_per_scene": per_scene_max, "scene_solves_at_iteration_cap": capped,
         "scene_steps_reset_after_divergence": blown,
-        "launch": {"block": tune[0], "groups": tune[1]} if tune else None,
+        "launch": {"block": tune[0], "groups": tune[1],
+                   "note": "the tuner's shape; with one scene per workgroup and every scene within 60 KiB of LDS the "
+                           "solve keeps each scene in LDS and raises the block to one thread per contact and per body "
+                           "of the largest scene (up to 512)"} if tune else None,
         "one_by_one": None,
     }
     if obo_solves:
