@@ -746,7 +746,9 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
             }
         }
     } else {
-        /* nesterov_loop (:379-449) */
+        /* nesterov_loop (:379-449).  The ORC_MUT_NES_* macros each plant one deliberate mistake
+         * (theta or beta recurrence, growth factor, direction test, restart limit, extrapolation
+         * anchor) for the mutation tests; the oracle itself is built with none of them. */
         double* xk = (double*)malloc(sizeof(double) * (size_t)nv);
         double* pk = (double*)malloc(sizeof(double) * (size_t)nr);
         double* xy = (double*)malloc(sizeof(double) * (size_t)nv);
@@ -767,29 +769,53 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
             const double err = pj_residual(nv, xk1, xy, prm->tol_rel, prm->tol_abs);
             res->residual = err;
             if (err <= 1) break;
+#if defined(ORC_MUT_NES_THETA)
+            const double thk1 = 0.5 * (1.0 + sqrt(thk * thk + 1.0));
+#else
             const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+#endif
+#if defined(ORC_MUT_NES_BETA)
+            const double betak1 = thk / thk1;
+#else
             const double betak1 = (thk - 1.0) / thk1;
+#endif
             int restart = 0;
             if (!isfinite(err)) restart = 1;
+#if defined(ORC_MUT_NES_GROWTH)
+            else if (err_prev < INFINITY && err > 1.5 * err_prev) restart = 1;
+#else
             else if (err_prev < INFINITY && err > 1.05 * err_prev) restart = 1;
+#endif
             else if (betak1 > prm->nesterov_beta_threshold) restart = 1;
             else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = 1;
             else {
                 double dot = 0;
                 for (int j = 0; j < nv; ++j) dot += (xy[j] - xk1[j]) * (xk1[j] - xk[j]);
+#if defined(ORC_MUT_NES_DIRECTION)
+                if (dot < 0.0) restart = 1;
+#else
                 if (dot > 0.0) restart = 1;
+#endif
             }
             if (restart) {
                 memcpy(xy, xk1, sizeof(double) * nv); memcpy(py, pk1, sizeof(double) * nr);
                 memcpy(xk, xk1, sizeof(double) * nv); memcpy(pk, pk1, sizeof(double) * nr);
                 thk = 1.0;
+#if defined(ORC_MUT_NES_LIMIT)
+                if (++restarts > prm->nesterov_restart_limit) momentum_disabled = 1;
+#else
                 if (++restarts >= prm->nesterov_restart_limit) momentum_disabled = 1;
+#endif
             } else {
                 if (momentum_disabled) {
                     memcpy(xy, xk1, sizeof(double) * nv); memcpy(py, pk1, sizeof(double) * nr);
                     thk = 1.0;
                 } else {
+#if defined(ORC_MUT_NES_ANCHOR)
+                    for (int j = 0; j < nv; ++j) xy[j] = xk1[j] + betak1 * (xk1[j] - xy[j]);
+#else
                     for (int j = 0; j < nv; ++j) xy[j] = xk1[j] + betak1 * (xk1[j] - xk[j]);
+#endif
                     for (int r = 0; r < nr; ++r) py[r] = pk1[r] + betak1 * (pk1[r] - pk[r]);
                     thk = thk1;
                 }

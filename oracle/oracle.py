@@ -47,6 +47,35 @@ class Result(C.Structure):
 
 
 _lib = None
+_variants = {}
+
+
+def _declare(L):
+    for name in ("orc_pj_solve",):
+        getattr(L, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
+                                     C.POINTER(Params), C.c_int, _dp, _dp, C.POINTER(Result)]
+        getattr(L, name).restype = C.c_int
+    for name in ("orc_pgs_solve", "orc_colored_gs_solve"):
+        getattr(L, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
+                                     C.POINTER(Params), _dp, _dp, C.POINTER(Result)]
+        getattr(L, name).restype = C.c_int
+    L.orc_lstsq_colpiv.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    L.orc_lstsq_colpiv.restype = C.c_int
+    L.orc_pj_solve_joints.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.POINTER(Joints), C.c_double,
+                                      C.c_double, C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Result)]
+    L.orc_pj_solve_joints.restype = C.c_int
+    L.orc_color_blocks.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
+    L.orc_color_blocks.restype = C.c_int
+    L.orc_tangent_frame.argtypes = [_dp, _dp, _dp]
+    L.orc_tangent_frame.restype = None
+    L.orc_position_update.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double]
+    L.orc_position_update.restype = None
+    L.orc_gyro_force.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
+    L.orc_gyro_force.restype = None
+    L.orc_complementarity.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), _dp, _dp]
+    L.orc_complementarity.restype = C.c_double
+    return L
 
 
 def lib():
@@ -54,32 +83,18 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             subprocess.run(["make", "-C", HERE, "-s"], check=True)
-        _lib = C.CDLL(LIB)
-        for name in ("orc_pj_solve",):
-            getattr(_lib, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
-                                            C.POINTER(Params), C.c_int, _dp, _dp, C.POINTER(Result)]
-            getattr(_lib, name).restype = C.c_int
-        for name in ("orc_pgs_solve", "orc_colored_gs_solve"):
-            getattr(_lib, name).argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.c_double, C.c_double,
-                                            C.POINTER(Params), _dp, _dp, C.POINTER(Result)]
-            getattr(_lib, name).restype = C.c_int
-        _lib.orc_lstsq_colpiv.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                          C.POINTER(C.c_double)]
-        _lib.orc_lstsq_colpiv.restype = C.c_int
-        _lib.orc_pj_solve_joints.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), C.POINTER(Joints), C.c_double,
-                                             C.c_double, C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Result)]
-        _lib.orc_pj_solve_joints.restype = C.c_int
-        _lib.orc_color_blocks.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip]
-        _lib.orc_color_blocks.restype = C.c_int
-        _lib.orc_tangent_frame.argtypes = [_dp, _dp, _dp]
-        _lib.orc_tangent_frame.restype = None
-        _lib.orc_position_update.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double]
-        _lib.orc_position_update.restype = None
-        _lib.orc_gyro_force.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
-        _lib.orc_gyro_force.restype = None
-        _lib.orc_complementarity.argtypes = [C.POINTER(Bodies), C.POINTER(Contacts), _dp, _dp]
-        _lib.orc_complementarity.restype = C.c_double
+        _lib = _declare(C.CDLL(LIB))
     return _lib
+
+
+def mutant(name):
+    """The oracle built with one deliberate mistake (macro ORC_MUT_<name>, `make mutants`): used
+    only to show that the pinning tests fail on it."""
+    if name not in _variants:
+        path = os.path.join(HERE, "_mut", f"libcdx_oracle_{name}.so")
+        subprocess.run(["make", "-C", HERE, "-s", os.path.join("_mut", f"libcdx_oracle_{name}.so")], check=True)
+        _variants[name] = _declare(C.CDLL(path))
+    return _variants[name]
 
 
 def _d(a):
@@ -123,8 +138,8 @@ def params(alpha=0.3, relaxation=1.0, max_iter=200000, tol_abs=1e-5, tol_rel=1e-
                   nesterov_restart_limit, chebyshev, block_precond, anderson, anderson_m)
 
 
-def _solve(name, prob, dt, prm, nthreads=None):
-    L = lib()
+def _solve(name, prob, dt, prm, nthreads=None, L=None):
+    L = L or lib()
     pk = _Pack(prob)
     v = np.zeros(int(pk.dof.sum()))
     imp = np.zeros((len(pk.a), 3))
@@ -157,8 +172,9 @@ def pj_solve_joints(prob, rows, dt, prm, theta=1.0, lam_prev=None, bias_factor=0
     return st, v, imp, lam[:n], res
 
 
-def pj_solve(prob, dt, prm, nthreads=1):
-    return _solve("orc_pj_solve", prob, dt, prm, nthreads)
+def pj_solve(prob, dt, prm, nthreads=1, L=None):
+    """L: a variant library (mutant()) in place of the oracle."""
+    return _solve("orc_pj_solve", prob, dt, prm, nthreads, L)
 
 
 def pgs_solve(prob, dt, prm):
