@@ -7,7 +7,7 @@
 #include <cstdio>
 #include <cstring>
 
-#include "cdx_common.hpp"
+#include "cdx_device.hpp"
 
 namespace cdx {
 
@@ -100,14 +100,7 @@ __global__ void k_gyro(int nb, const int* __restrict__ dof, const double* __rest
     double* o = fg + (size_t)b * kBodyStride;
     for (int k = 0; k < kBodyStride; ++k) o[k] = 0.0;
     if (dof[b] != 6) return;
-    const double* w = v + (size_t)b * kBodyStride + 3;
-    const double* mi = minv + (size_t)b * kMinvStride;
-    double I[3], Iw[3];
-    for (int k = 0; k < 3; ++k) I[k] = mi[1 + k] > 0 ? 1.0 / mi[1 + k] : 0.0;
-    for (int k = 0; k < 3; ++k) Iw[k] = I[k] * w[k];
-    o[3] = -(w[1] * Iw[2] - w[2] * Iw[1]);
-    o[4] = -(w[2] * Iw[0] - w[0] * Iw[2]);
-    o[5] = -(w[0] * Iw[1] - w[1] * Iw[0]);
+    gyro_force3(v + (size_t)b * kBodyStride + 3, minv + (size_t)b * kMinvStride + 1, o + 3);
 }
 
 // Explicit position update (integration_base.cpp:6-30) with alignQuaternionTo
@@ -229,7 +222,7 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
     delete ctx;
 }
 
-const char* cdx_last_error(const cdx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* cdx_last_error(const cdx_ctx* ctx) { return ctx ? ctx->err.c_str() : noalloc_last_error(); }
 
 int cdx_synchronize(cdx_ctx* ctx) {
     if (!ctx) return CDX_BAD_ARG;

@@ -211,6 +211,8 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm);
 // Reduces ctx->d_partial[0:nparts) into the residual; mode 0 = PJ test (res <= 1),
 // mode 1 = Gauss-Seidel test (res < 1, NaN/Inf -> CDX_DIVERGED).
 int launch_residual_finalize(cdx_ctx* ctx, int nparts, int it, int mode, int cadence = 1);
+// Message of the calling thread's last failed cdx_pj_solve_noalloc ("" after a success).
+const char* noalloc_last_error();
 int joints_factor(cdx_ctx* ctx, double dt, double theta);
 int joints_solve(cdx_ctx* ctx, int setup, int check, double tol_rel, double tol_abs, double* partial);
 int joints_apply(cdx_ctx* ctx, double* x_out, double* lam_out, const int* ctrl);

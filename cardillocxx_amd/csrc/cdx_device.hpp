@@ -162,6 +162,17 @@ __device__ inline void gather_side9(const double* __restrict__ imp9, int e, doub
     }
 }
 
+// f_gyro = -omega x (I omega) of a rigid body from its body-frame omega and inverse principal
+// inertia (world.cpp:152-173; I = 1 / I^-1 as rebuildMass_ inverts it, dynamics_assembler.cpp:56-60).
+__device__ inline void gyro_force3(const double* __restrict__ w, const double* __restrict__ iinv, double* __restrict__ o) {
+    double I[3], Iw[3];
+    for (int k = 0; k < 3; ++k) I[k] = iinv[k] > 0 ? 1.0 / iinv[k] : 0.0;
+    for (int k = 0; k < 3; ++k) Iw[k] = I[k] * w[k];
+    o[0] = -(w[1] * Iw[2] - w[2] * Iw[1]);
+    o[1] = -(w[2] * Iw[0] - w[0] * Iw[2]);
+    o[2] = -(w[0] * Iw[1] - w[1] * Iw[0]);
+}
+
 // Inverse mass of a body as [m^-1, Ixx^-1, Iyy^-1, Izz^-1]; zeros for a static side.
 __device__ inline double4 load_minv4(const double* __restrict__ minv, int b) {
     if (b < 0) return make_double4(0.0, 0.0, 0.0, 0.0);

@@ -15,8 +15,11 @@
 //                 res <= 1 (standard_loop, :371-377); every later launch returns immediately.
 //
 // Both sweep kernels are HBM-bound (about 0.5 flop per byte); there is no MFMA work.
+#include <algorithm>
 #include <cmath>
 #include <deque>
+#include <mutex>
+#include <string>
 
 #include "cdx_pj_device.hpp"
 
@@ -1361,5 +1364,427 @@ static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_o
     const int st = run_sweeps_graphed(ctx, prm->max_iter, 8, true, launch_iter, iters_out);
     if (st) return st;
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CDX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free projected Jacobi (cdx_pj_solve_noalloc): the standard loop of cdx_pj_solve on
+// caller-owned device arrays.  Every buffer the solve needs beyond its inputs and outputs (padded
+// body state, the contact rows in component-major order, R, b, p, the side-impulse records, the
+// incidence lists, residual partials, control words) lives in one file-scope device array,
+// `scratch`, laid out for the maximum sizes below; the calls allocate nothing.  Each GPU holds its
+// own copy for the life of the process; calls on one GPU share it, so calls that overlap on one
+// GPU (two host threads, or two streams) race.
+// ---------------------------------------------------------------------------------------------
+namespace cdx {
+
+constexpr int kNaMaxBodies = CDX_NOALLOC_MAX_BODIES;
+constexpr int kNaMaxContacts = CDX_NOALLOC_MAX_CONTACTS;
+constexpr int kNaMaxDevices = 64;
+constexpr size_t kNaNB = kNaMaxBodies, kNaNC = kNaMaxContacts;
+constexpr size_t kNaPartials = (kNaNB * kLanesPerBody + kBlock - 1) / kBlock;
+// double region (offsets in doubles)
+constexpr size_t kNaX = 0;                          // solver state x, padded [nb][8]
+constexpr size_t kNaV = kNaX + kNaNB * kBodyStride;  // v_n
+constexpr size_t kNaF = kNaV + kNaNB * kBodyStride;  // f_ext
+constexpr size_t kNaG = kNaF + kNaNB * kBodyStride;  // f_gyro
+constexpr size_t kNaM = kNaG + kNaNB * kBodyStride;  // [m^-1, I^-1] per body
+constexpr size_t kNaJ = kNaM + kNaNB * kMinvStride;  // J[27][nc]
+constexpr size_t kNaMu = kNaJ + kJ * kNaNC;
+constexpr size_t kNaRest = kNaMu + kNaNC;
+constexpr size_t kNaVb = kNaRest + 3 * kNaNC;
+constexpr size_t kNaP = kNaVb + 3 * kNaNC;
+constexpr size_t kNaR = kNaP + 3 * kNaNC;
+constexpr size_t kNaBias = kNaR + 3 * kNaNC;
+constexpr size_t kNaImp = kNaBias + 3 * kNaNC;       // side-impulse records [nc][9]
+constexpr size_t kNaPart = kNaImp + 9 * kNaNC;       // residual partials
+constexpr size_t kNaRes = kNaPart + kNaPartials;     // [0] residual
+constexpr size_t kNaDoubles = kNaRes + 4;
+// int region (offsets in ints, after the doubles)
+constexpr size_t kNaDof = 0;
+constexpr size_t kNaVoff = kNaDof + kNaNB;           // [nb+1] stacked offsets
+constexpr size_t kNaIncPtr = kNaVoff + kNaNB + 1;    // [nb+1]
+constexpr size_t kNaCnt = kNaIncPtr + kNaNB + 1;     // [nb] incidence counts, then fill cursors
+constexpr size_t kNaIncIdx = kNaCnt + kNaNB;         // [2nc]
+constexpr size_t kNaIda = kNaIncIdx + 2 * kNaNC;
+constexpr size_t kNaIdb = kNaIda + kNaNC;
+constexpr size_t kNaCtrl = kNaIdb + kNaNC;           // [4] done, iterations, status, chunk counter
+constexpr size_t kNaFlags = kNaCtrl + 4;             // [0] input error, [1] rows, [2] nv, [4] done mirror
+constexpr size_t kNaInts = kNaFlags + 8;
+constexpr size_t kNaBytes = kNaDoubles * sizeof(double) + kNaInts * sizeof(int);
+
+static __device__ __align__(256) double scratch[(kNaBytes + sizeof(double) - 1) / sizeof(double)];
+
+struct NaBuf {
+    double *x, *v, *f, *g, *m, *J, *mu, *rest, *vb, *p, *R, *bias, *imp, *part, *res;
+    int *dof, *voff, *incptr, *cnt, *incidx, *ida, *idb, *ctrl, *flags;
+};
+
+static NaBuf na_buf(double* base) {
+    int* ib = reinterpret_cast<int*>(base + kNaDoubles);
+    return NaBuf{base + kNaX,    base + kNaV,    base + kNaF,    base + kNaG,   base + kNaM,    base + kNaJ,
+                 base + kNaMu,   base + kNaRest, base + kNaVb,   base + kNaP,   base + kNaR,    base + kNaBias,
+                 base + kNaImp,  base + kNaPart, base + kNaRes,  ib + kNaDof,   ib + kNaVoff,   ib + kNaIncPtr,
+                 ib + kNaCnt,    ib + kNaIncIdx, ib + kNaIda,    ib + kNaIdb,   ib + kNaCtrl,   ib + kNaFlags};
+}
+
+// Input errors are merged by atomicMax, so a call reports the same status as cdx_set_bodies /
+// cdx_set_contacts / cdx_pj_solve would on the same input: CDX_BAD_ARG before CDX_NONFINITE
+// before CDX_NOT_FACTORIZED.
+__device__ inline void na_flag(int* flags, int code) { atomicMax(&flags[0], code); }
+
+// Clears what must start at zero: control words, flags, incidence counts; residual = NaN.
+__global__ void k_na_clear(int nb, int* __restrict__ cnt, int* __restrict__ ctrl, int* __restrict__ flags,
+                           double* __restrict__ res) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) cnt[i] = 0;
+    if (i < 4) ctrl[i] = 0;
+    if (i < 8) flags[i] = 0;
+    if (i == 0) res[0] = __longlong_as_double(0x7ff8000000000000ll);
+}
+
+// Exclusive prefix sum of n values of `in` into out[0..n] by one workgroup of kNaScan threads
+// (each thread sums a contiguous segment, then a scan over the segment sums in LDS).
+constexpr int kNaScan = 1024;
+__device__ inline void na_block_scan(int n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ lds) {
+    const int t = threadIdx.x;
+    const int per = (n + kNaScan - 1) / kNaScan;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    int s = 0;
+    for (int i = i0; i < i1; ++i) s += in[i];
+    lds[t] = s;
+    __syncthreads();
+    for (int off = 1; off < kNaScan; off <<= 1) {
+        const int v = t >= off ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += v;
+        __syncthreads();
+    }
+    int run = lds[t] - s;  // exclusive prefix of this segment
+    for (int i = i0; i < i1; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == kNaScan - 1) out[n] = lds[t];
+}
+
+// dof per body (3 or 6, else CDX_BAD_ARG) and the stacked offsets voff; flags[2] = nv.
+__global__ void __launch_bounds__(kNaScan) k_na_offsets(int nb, const int* __restrict__ dof_in, int* __restrict__ dof,
+                                                        int* __restrict__ voff, int* __restrict__ flags) {
+    __shared__ int lds[kNaScan];
+    for (int b = threadIdx.x; b < nb; b += kNaScan) {
+        int d = dof_in[b];
+        if (d != 3 && d != 6) {
+            na_flag(flags, CDX_BAD_ARG);
+            d = 6;
+        }
+        dof[b] = d;
+    }
+    __syncthreads();
+    na_block_scan(nb, dof, voff, lds);
+    if (threadIdx.x == kNaScan - 1) flags[2] = voff[nb];
+}
+
+// Bodies into the padded layout (cdx_set_bodies): [v; omega], f_ext, f_gyro (computed as
+// -omega x (I omega) when not given), [m^-1, I^-1].
+__global__ void k_na_pack_bodies(int nb, const int* __restrict__ dof, const int* __restrict__ voff,
+                                 const double* __restrict__ minv, const double* __restrict__ v,
+                                 const double* __restrict__ fext, const double* __restrict__ fgyro,
+                                 double* __restrict__ xv, double* __restrict__ vv, double* __restrict__ fv,
+                                 double* __restrict__ gv, double* __restrict__ mv, int* __restrict__ flags) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int o = voff[b], d = dof[b];
+    double* X = xv + (size_t)b * kBodyStride;
+    double* V = vv + (size_t)b * kBodyStride;
+    double* F = fv + (size_t)b * kBodyStride;
+    double* G = gv + (size_t)b * kBodyStride;
+    double* M = mv + (size_t)b * kMinvStride;
+    if (minv[o] != minv[o + 1] || minv[o] != minv[o + 2]) na_flag(flags, CDX_BAD_ARG);
+    M[0] = minv[o];
+    M[1] = M[2] = M[3] = 0.0;
+    for (int k = 0; k < kBodyStride; ++k) X[k] = V[k] = F[k] = G[k] = 0.0;
+    for (int k = 0; k < d; ++k) {
+        const double mk = minv[o + k], vk = v[o + k], fk = fext[o + k], gk = fgyro ? fgyro[o + k] : 0.0;
+        if (!isfinite(mk) || !isfinite(vk) || !isfinite(fk) || !isfinite(gk)) na_flag(flags, CDX_NONFINITE);
+        if (!(mk > 0)) na_flag(flags, CDX_NOT_FACTORIZED);
+        X[k] = V[k] = vk;
+        F[k] = fk;
+        G[k] = gk;
+    }
+    if (d == 6) {
+        for (int k = 0; k < 3; ++k) M[1 + k] = minv[o + 3 + k];
+        if (!fgyro) gyro_force3(V + 3, M + 1, G + 3);
+    }
+}
+
+// Contacts into the component-major layout of cdx_set_contacts: static-static contacts carry no
+// rows (both ids -1), a static side and a point mass no angular Jacobian; mu, restitution, bias
+// and warm start only on the rows the contact has.  Counts rows and each body's incidences.
+__global__ void k_na_pack_contacts(int nc, int nb, const int* __restrict__ dof, const int* __restrict__ body_a,
+                                   const int* __restrict__ body_b, const double* __restrict__ mu,
+                                   const double* __restrict__ frame, const double* __restrict__ angA,
+                                   const double* __restrict__ angB, const double* __restrict__ rest,
+                                   const double* __restrict__ vbias, const double* __restrict__ warm,
+                                   int* __restrict__ ida, int* __restrict__ idb, double* __restrict__ J,
+                                   double* __restrict__ mus, double* __restrict__ rs, double* __restrict__ vbs,
+                                   double* __restrict__ ps, int* __restrict__ cnt, int* __restrict__ flags) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int a = body_a[c], b = body_b[c];
+    if (a < -1 || a >= nb || b < -1 || b >= nb || (a >= 0 && a == b)) {
+        na_flag(flags, CDX_BAD_ARG);
+        a = b = -1;
+    }
+    const double m = mu[c];
+    bool finite = isfinite(m);
+    const bool fr = m > 0;
+    const bool angAok = a >= 0 && dof[a] == 6, angBok = b >= 0 && dof[b] == 6;
+    for (int k = 0; k < 9; ++k) {
+        const double fk = frame[9 * (size_t)c + k], ak = angA[9 * (size_t)c + k], bk = angB[9 * (size_t)c + k];
+        finite = finite && isfinite(fk) && isfinite(ak) && isfinite(bk);
+        J[(size_t)k * nc + c] = fk;
+        J[(size_t)(9 + k) * nc + c] = angAok ? ak : 0.0;
+        J[(size_t)(18 + k) * nc + c] = angBok ? bk : 0.0;
+    }
+    const bool has_rows = a >= 0 || b >= 0;
+    for (int k = 0; k < 3; ++k) {
+        const double rk = rest[3 * (size_t)c + k], vk = vbias[3 * (size_t)c + k];
+        const double wk = warm ? warm[3 * (size_t)c + k] : 0.0;
+        finite = finite && isfinite(rk) && isfinite(vk) && isfinite(wk);
+        const bool row = has_rows && (k == 0 || fr);
+        rs[(size_t)k * nc + c] = row ? rk : 0.0;
+        vbs[(size_t)k * nc + c] = row ? vk : 0.0;
+        ps[(size_t)k * nc + c] = row ? wk : 0.0;
+    }
+    if (!finite) na_flag(flags, CDX_NONFINITE);
+    mus[c] = fr ? m : 0.0;
+    ida[c] = has_rows ? a : -1;
+    idb[c] = has_rows ? b : -1;
+    if (has_rows) atomicAdd(&flags[1], fr ? 3 : 1);
+    if (a >= 0) atomicAdd(&cnt[a], 1);
+    if (b >= 0) atomicAdd(&cnt[b], 1);
+}
+
+// incptr = exclusive prefix of the counts; the counts become fill cursors.
+__global__ void __launch_bounds__(kNaScan) k_na_incidence_ptr(int nb, int* __restrict__ cnt, int* __restrict__ incptr) {
+    __shared__ int lds[kNaScan];
+    na_block_scan(nb, cnt, incptr, lds);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kNaScan) cnt[b] = incptr[b];
+}
+
+__global__ void k_na_incidence_fill(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                    int* __restrict__ cursor, int* __restrict__ incidx) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ida[c], b = idb[c];
+    if (a >= 0) incidx[atomicAdd(&cursor[a], 1)] = 2 * c;
+    if (b >= 0) incidx[atomicAdd(&cursor[b], 1)] = 2 * c + 1;
+}
+
+// Each body's incidences into ascending order (= contact order, as build_incidence lists them),
+// so the body pass sums in the same fixed order as cdx_pj_solve: insertion sort for short lists,
+// heap sort beyond.
+__global__ void k_na_incidence_sort(int nb, const int* __restrict__ incptr, int* __restrict__ incidx) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int* e = incidx + incptr[b];
+    const int n = incptr[b + 1] - incptr[b];
+    if (n <= 32) {
+        for (int i = 1; i < n; ++i) {
+            const int v = e[i];
+            int j = i - 1;
+            while (j >= 0 && e[j] > v) {
+                e[j + 1] = e[j];
+                --j;
+            }
+            e[j + 1] = v;
+        }
+        return;
+    }
+    auto sift = [&](int root, int end) {
+        while (2 * root + 1 < end) {
+            int child = 2 * root + 1;
+            if (child + 1 < end && e[child] < e[child + 1]) ++child;
+            if (e[root] >= e[child]) return;
+            const int t = e[root];
+            e[root] = e[child];
+            e[child] = t;
+            root = child;
+        }
+    };
+    for (int i = n / 2 - 1; i >= 0; --i) sift(i, n);
+    for (int end = n - 1; end > 0; --end) {
+        const int t = e[0];
+        e[0] = e[end];
+        e[end] = t;
+        sift(0, end);
+    }
+}
+
+// Solved velocities back to the stacked layout, impulses to [nc][3] (cdx_get_velocities /
+// cdx_get_impulses).
+__global__ void k_na_unpack(int nb, int nc, const int* __restrict__ dof, const int* __restrict__ voff,
+                            const double* __restrict__ x, const double* __restrict__ p, double* __restrict__ v_out,
+                            double* __restrict__ imp_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb)
+        for (int k = 0; k < dof[i]; ++k) v_out[voff[i] + k] = x[(size_t)i * kBodyStride + k];
+    if (i < nc)
+        for (int k = 0; k < 3; ++k) imp_out[3 * (size_t)i + k] = p[(size_t)k * nc + i];
+}
+
+static thread_local std::string g_na_err;
+static std::mutex g_na_mutex;
+static double* g_na_scratch[kNaMaxDevices] = {};
+
+const char* noalloc_last_error() { return g_na_err.c_str(); }
+
+static int na_fail(int code, const std::string& msg) {
+    g_na_err = msg;
+    return code;
+}
+
+static int na_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return CDX_OK;
+    return na_fail(CDX_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define NA_HIP(expr) CDX_TRY(na_hip((expr), #expr))
+
+// Device address of this GPU's scratch array, looked up once per device with that device current.
+// The array's name is only a host-side handle: kernels and copies get this address.
+static int na_scratch_ptr(int dev, double** out) {
+    if (dev < 0 || dev >= kNaMaxDevices) return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return na_fail(CDX_HIP_ERROR, std::string("cdx_pj_solve_noalloc: hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(e));
+        g_na_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_na_scratch[dev];
+    return CDX_OK;
+}
+
+static int na_read_flags(hipStream_t s, const int* d_flags, int* h_flags) {
+    NA_HIP(hipMemcpyAsync(h_flags, d_flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    NA_HIP(hipStreamSynchronize(s));
+    return CDX_OK;
+}
+
+}  // namespace cdx
+
+extern "C" int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, double theta, const cdx_solver_params* prm,
+                                    double* v_out, double* imp_out, cdx_solve_info* info, void* stream) {
+    (void)theta;  // theta only enters joint rows, which this entry point does not take
+    g_na_err.clear();
+    if (info) {
+        info->iters = 0;
+        info->residual = std::nan("");
+        info->nrows = 0;
+        info->ncolors = 0;
+    }
+    if (!pb || !prm) return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: null problem or params");
+    const int nb = pb->nb, nc = pb->nc;
+    if (nb < 0 || nc < 0) return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: negative size");
+    if (nb > kNaMaxBodies || nc > kNaMaxContacts)
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: " + std::to_string(nb) + " bodies / " + std::to_string(nc) +
+                                        " contacts exceed the allocation-free maximum of " +
+                                        std::to_string(kNaMaxBodies) + " bodies / " + std::to_string(kNaMaxContacts) +
+                                        " contacts");
+    if (nb > 0 && (!pb->dof || !pb->minv || !pb->v || !pb->fext || !v_out))
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: null body array");
+    if (nc > 0 && (!pb->body_a || !pb->body_b || !pb->mu || !pb->frame || !pb->angA || !pb->angB || !pb->rest ||
+                   !pb->vbias || !imp_out))
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: null contact array");
+    if (nb == 0 && nc > 0) return na_fail(CDX_NO_STATE, "cdx_pj_solve_noalloc: contacts without bodies");
+    if (prm->max_iter < 1) return na_fail(CDX_BAD_ARG, "max_iter must be >= 1");
+    if (prm->check_every < 1) return na_fail(CDX_BAD_ARG, "check_every must be >= 1");
+    if (!std::isfinite(prm->alpha) || !std::isfinite(prm->tol_abs) || !std::isfinite(prm->tol_rel))
+        return na_fail(CDX_BAD_ARG, "non-finite solver parameter");
+    if (prm->accel != CDX_ACCEL_NONE || prm->block_precond)
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: standard loop with the row scaling only");
+    if (nb == 0) return CDX_OK;
+    int dev = 0;
+    NA_HIP(hipGetDevice(&dev));
+    double* base = nullptr;
+    CDX_TRY(na_scratch_ptr(dev, &base));
+    const NaBuf s = na_buf(base);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int n_clear = std::max(nb, 8);
+    hipLaunchKernelGGL(k_na_clear, dim3(grid_for(n_clear)), dim3(kBlock), 0, st, nb, s.cnt, s.ctrl, s.flags, s.res);
+    hipLaunchKernelGGL(k_na_offsets, dim3(1), dim3(kNaScan), 0, st, nb, pb->dof, s.dof, s.voff, s.flags);
+    NA_HIP(hipGetLastError());
+    int fl[4];
+    CDX_TRY(na_read_flags(st, s.flags, fl));
+    if (fl[0]) return na_fail(fl[0], "cdx_pj_solve_noalloc: dof must be 3 or 6");
+    hipLaunchKernelGGL(k_na_pack_bodies, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s.dof, s.voff, pb->minv, pb->v,
+                       pb->fext, pb->fgyro, s.x, s.v, s.f, s.g, s.m, s.flags);
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_na_pack_contacts, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, nb, s.dof, pb->body_a,
+                           pb->body_b, pb->mu, pb->frame, pb->angA, pb->angB, pb->rest, pb->vbias, pb->warm, s.ida, s.idb,
+                           s.J, s.mu, s.rest, s.vb, s.p, s.cnt, s.flags);
+    }
+    hipLaunchKernelGGL(k_na_incidence_ptr, dim3(1), dim3(kNaScan), 0, st, nb, s.cnt, s.incptr);
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_na_incidence_fill, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, s.ida, s.idb, s.cnt, s.incidx);
+        hipLaunchKernelGGL(k_na_incidence_sort, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s.incptr, s.incidx);
+    }
+    NA_HIP(hipGetLastError());
+    CDX_TRY(na_read_flags(st, s.flags, fl));
+    if (fl[0] == CDX_BAD_ARG)
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: bad input (anisotropic translational inverse mass, or a "
+                                    "contact body index out of range)");
+    if (fl[0] == CDX_NONFINITE) return na_fail(CDX_NONFINITE, "cdx_pj_solve_noalloc: non-finite input");
+    if (fl[0] == CDX_NOT_FACTORIZED)
+        return na_fail(CDX_NOT_FACTORIZED, "S matrix is not factorized: a dynamic dof has zero mass");
+    const int nrows = fl[1], nv = fl[2];
+    if (info) info->nrows = nrows;
+    const bool have_contacts = nrows > 0;
+    if (have_contacts)
+        hipLaunchKernelGGL(k_pj_setup_contacts, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, s.ida, s.idb, s.J, s.mu,
+                           s.rest, s.vb, s.m, s.v, prm->alpha, prm->warmstart, s.R, s.bias, s.p, s.imp, nullptr);
+    hipLaunchKernelGGL(k_pj_setup_bodies, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s.dof, s.m, s.v, s.f, s.g,
+                       s.incptr, s.incidx, s.imp, dt, have_contacts ? 1 : 0, s.x, nullptr, nullptr);
+    NA_HIP(hipGetLastError());
+    int iters = 0;
+    if (have_contacts) {
+        const int gb8 = grid_for(nb * kLanesPerBody), gc4 = grid_for(nc * kLanesPerContact);
+        const int ce = prm->check_every;
+        const int chunk = 4 * ce;  // sweeps between two polls of the done flag
+        int ctrl[4] = {0, 0, 0, 0};
+        for (int it = 0; it < prm->max_iter && !ctrl[0];) {
+            const int end = std::min(prm->max_iter, it + chunk);
+            for (; it < end; ++it) {
+                const int check = (it % ce) == 0;
+                // the bias array is always read: it holds exact zeros where a row has no bias,
+                // and p - (R g + 0) rounds as p - R g
+                hipLaunchKernelGGL(k_pj_contacts4<false>, dim3(gc4), dim3(kBlock), 0, st, nc, gc4, s.ida, s.idb, s.J,
+                                   s.R, s.bias, 1, s.mu, s.p, s.x, s.imp, s.ctrl);
+                hipLaunchKernelGGL(k_pj_bodies8, dim3(gb8), dim3(kBlock), 0, st, nb, s.dof, s.m, s.incptr, s.incidx,
+                                   s.imp, s.x, check, prm->tol_rel, prm->tol_abs, s.part, s.ctrl, nullptr, nullptr);
+                if (check)
+                    hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(kBlock), 0, st, gb8, s.part, nv, it, ce, 0,
+                                       s.ctrl, s.res, s.flags + 4);
+            }
+            NA_HIP(hipGetLastError());
+            NA_HIP(hipMemcpyAsync(ctrl, s.ctrl, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+            NA_HIP(hipStreamSynchronize(st));
+        }
+        iters = ctrl[0] ? ctrl[1] : prm->max_iter;
+    }
+    hipLaunchKernelGGL(k_na_unpack, dim3(grid_for(std::max(nb, nc))), dim3(kBlock), 0, st, nb, nc, s.dof, s.voff, s.x,
+                       s.p, v_out, nc > 0 ? imp_out : nullptr);
+    NA_HIP(hipGetLastError());
+    double r = std::nan("");
+    NA_HIP(hipMemcpyAsync(&r, s.res, sizeof(double), hipMemcpyDeviceToHost, st));
+    NA_HIP(hipStreamSynchronize(st));
+    if (info) {
+        info->iters = iters;
+        info->residual = r;
+    }
     return CDX_OK;
 }

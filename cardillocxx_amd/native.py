@@ -62,6 +62,7 @@ EXPORTED = [
     "cdx_set_recording",
     "cdx_pj_adjoint",
     "cdx_pj_adjoint_joints",
+    "cdx_pj_solve_noalloc",
 ]
 CDX_ADJ_IMPLICIT = 0
 CDX_ADJ_UNROLLED = 1
@@ -101,6 +102,13 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
+
+class DeviceProblem(C.Structure):
+    """cdx_device_problem: device pointers (as integers) in the layouts of cdx_set_bodies /
+    cdx_set_contacts."""
+    _fields_ = [("nb", C.c_int), ("nc", C.c_int)] + [(n, _vp) for n in (
+        "dof", "minv", "v", "fext", "fgyro", "body_a", "body_b", "mu", "frame", "angA", "angB", "rest", "vbias", "warm")]
+
 _lib = None
 _scene = None
 
@@ -134,6 +142,8 @@ def _declare(lib):
         "cdx_batch_pj_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _dp]),
         "cdx_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
         "cdx_batch_autotune": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(SolverParams), _ip, _ip, _dp]),
+        "cdx_pj_solve_noalloc": (C.c_int, [C.POINTER(DeviceProblem), C.c_double, C.c_double, C.POINTER(SolverParams),
+                                           _vp, _vp, C.POINTER(SolveInfo), _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -383,3 +393,17 @@ class Context:
         ms = C.c_double(0)
         self._check(self._l.cdx_kernel_time(self.h, name.encode(), C.byref(calls), C.byref(ms)), "cdx_kernel_time")
         return calls.value, ms.value
+
+
+def pj_solve_noalloc(nb, nc, ptrs, dt, theta, params, v_out, imp_out, stream=None):
+    """cdx_pj_solve_noalloc on device arrays given as integer device addresses: `ptrs` maps the
+    cdx_device_problem field names (dof, minv, v, fext, fgyro, body_a, ..., warm) to addresses (0 or
+    missing = NULL); v_out / imp_out are output addresses; `stream` a hipStream_t as an integer."""
+    L = lib()
+    pb = DeviceProblem(nb, nc, *[ptrs.get(n) or None for n, _ in DeviceProblem._fields_[2:]])
+    info = SolveInfo()
+    st = L.cdx_pj_solve_noalloc(C.byref(pb), dt, theta, C.byref(params), v_out or None, imp_out or None,
+                                C.byref(info), stream or None)
+    if st != CDX_OK:
+        raise CdxError(st, f"cdx_pj_solve_noalloc: {L.cdx_last_error(None).decode()}")
+    return info

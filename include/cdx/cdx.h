@@ -94,6 +94,8 @@ void cdx_default_params(cdx_solver_params* p);
 /* Context lifetime.  device: HIP ordinal. */
 int cdx_ctx_create(int device, cdx_ctx** out);
 void cdx_ctx_destroy(cdx_ctx* ctx);
+/* Message of the most recent failure on ctx; with ctx = NULL, of the calling thread's most recent
+ * context-free call (cdx_pj_solve_noalloc). */
 const char* cdx_last_error(const cdx_ctx* ctx);
 int cdx_synchronize(cdx_ctx* ctx);
 
@@ -249,6 +251,51 @@ int cdx_pj_adjoint_joints(cdx_ctx* ctx, double dt, const cdx_solver_params* prm,
  * one kernel name ("pj_contacts", "pj_bodies", "gs_color", ...) and resets it. */
 int cdx_set_profiling(cdx_ctx* ctx, int enable);
 int cdx_kernel_time(cdx_ctx* ctx, const char* kernel, int* calls, double* total_ms);
+
+/* ---- Allocation-free projected Jacobi.  No counterpart in the reference: this entry point is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc,
+ * code that must not grow the device heap).  It runs the standard loop of cdx_pj_solve on arrays
+ * the caller already holds on the device, with every buffer the solve needs beyond its inputs and
+ * outputs in one statically sized device array of the library (about 32 MiB per GPU, held from
+ * the library's first call on that GPU until the process ends): no hipMalloc, hipFree or other
+ * allocating call.  Limits: at most CDX_NOALLOC_MAX_BODIES bodies and CDX_NOALLOC_MAX_CONTACTS
+ * contacts (larger problems fail with CDX_BAD_ARG), standard loop with the row scaling only (no
+ * acceleration, block preconditioner or joint rows: CDX_BAD_ARG).  All calls on one GPU share the
+ * array, so calls that overlap on one GPU (two host threads, or two streams) race; it keeps the
+ * last call's contents, and each call clears what must start at zero.  The call synchronises
+ * `stream` to read the convergence flag every 4 x check_every sweeps, so it cannot itself be
+ * captured into a graph. */
+#define CDX_NOALLOC_MAX_BODIES 16384
+#define CDX_NOALLOC_MAX_CONTACTS 65536
+
+/* A contact problem on the device, in the layouts of cdx_set_bodies / cdx_set_contacts (device
+ * pointers, read only).  fgyro may be NULL (computed as -omega x (I omega)); warm may be NULL
+ * (no warm start). */
+typedef struct {
+    int nb, nc;
+    const int* dof;
+    const double* minv;
+    const double* v;
+    const double* fext;
+    const double* fgyro;
+    const int* body_a;
+    const int* body_b;
+    const double* mu;
+    const double* frame;
+    const double* angA;
+    const double* angB;
+    const double* rest;
+    const double* vbias;
+    const double* warm;
+} cdx_device_problem;
+
+/* cdx_pj_solve + cdx_get_velocities + cdx_get_impulses of `pb` in one call, on `stream` (a
+ * hipStream_t; NULL = the default stream) of the current device: v_out (nv = sum(dof) doubles) and
+ * imp_out (3 * nc doubles, canonical normal >= 0) are device pointers the caller owns.  The same
+ * statuses as the context path; the message of a failure is returned by cdx_last_error(NULL) on
+ * the calling thread. */
+int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, double theta, const cdx_solver_params* prm,
+                         double* v_out, double* imp_out, cdx_solve_info* info, void* stream);
 
 #ifdef __cplusplus
 }
