@@ -52,20 +52,22 @@ def log(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
-def committed_traffic(kernel, contacts):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if it was
-    taken on a problem with the same contact count (FETCH_SIZE/WRITE_SIZE passes scaled by a
-    device-copy calibration); None otherwise."""
-    path = os.path.join(ROOT, "profiles", "pj_contacts4_pile100k_hbm_traffic.txt")
+def committed_traffic(contacts):
+    """HBM bytes per projected-Jacobi sweep from the committed rocprofv3 counter summary (the sum
+    over the sweep's kernels of FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by a device-copy
+    calibration), if it was taken on a problem with the same contact count; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pj_sweep_pile100k_hbm_traffic.txt")
     try:
-        vals = {}
+        per_sweep, n = None, None
         for line in open(path):
             f = line.split()
-            if len(f) == 3 and f[0] in ("traffic_bytes_per_launch", "traffic_contacts") and f[1] == kernel:
-                vals[f[0]] = int(f[2])
-        if vals.get("traffic_contacts") == contacts:
-            return vals["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
-    except (OSError, ValueError, KeyError):
+            if len(f) == 2 and f[0] == "traffic_bytes_per_sweep":
+                per_sweep = int(f[1])
+            if len(f) == 2 and f[0] == "traffic_contacts":
+                n = int(f[1])
+        if per_sweep is not None and n == contacts:
+            return per_sweep, os.path.relpath(path, ROOT)
+    except (OSError, ValueError):
         pass
     return None, None
 
@@ -97,6 +99,26 @@ class Hip:
         if self.l.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
             raise RuntimeError("hipMalloc failed")
         return p
+
+    def event(self):
+        e = ctypes.c_void_p()
+        if self.l.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e
+
+    def record(self, e, stream):
+        if self.l.hipEventRecord(e, ctypes.c_void_p(stream or None)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, e0, e1):
+        self.l.hipEventSynchronize(e1)
+        ms = ctypes.c_float(0)
+        self.l.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        return float(ms.value)
+
+    def destroy(self, *events):
+        for e in events:
+            self.l.hipEventDestroy(e)
 
     def copy_once(self, mb):
         n = mb << 20
@@ -134,8 +156,26 @@ class Hip:
 # large single scene (headline)
 # --------------------------------------------------------------------------------------------
 
-def colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed):
-    """Coloured Gauss-Seidel on the pile already on the device, same tolerance (residual every sweep)."""
+def profiled_kernels(ctx, solve):
+    """Runs `solve` once with start/stop HIP events carried in every kernel dispatch on the solve's
+    stream (sweeps then launch one by one instead of as graph replays) and returns
+    ({kernel: (calls, total ms)}, the solve's result)."""
+    ctx.set_profiling(True)
+    ctx.kernel_stats()
+    for k in list(ctx.kernel_stats()):
+        ctx.kernel_time(k)
+    res = solve()
+    ctx.synchronize()
+    stats = ctx.kernel_stats()
+    for k in stats:
+        ctx.kernel_time(k)
+    ctx.set_profiling(False)
+    return stats, res
+
+
+def colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed, bytes_sweep):
+    """Coloured Gauss-Seidel on the pile already on the device, same tolerance (residual every
+    sweep): time to solution with graph-replayed sweeps, then one more solve with kernel events."""
     pg = native.default_params()
     pg.tol_abs = pg.tol_rel = args.tol
     pg.max_iter = args.gs_max_iter
@@ -149,14 +189,43 @@ def colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed):
     ig = ctx.impulses()
     ig[:, 1:] *= -1.0  # the Gauss-Seidel solver stores tangents with the opposite sign
     log(f"colored GS: {ginfo.iters} sweeps, {ginfo.ncolors} colours, {tg:.3f}s")
+    kern = None
+    if not args.no_kernel_events:
+        stats, pinfo = profiled_kernels(ctx, lambda: ctx.gs_colored_solve(dt, 1.0, pg, raise_on_divergence=False))
+        sw = max(pinfo.iters, 1)
+        calls, ms = stats.get("gs_color", (0, 0.0))
+        rc, rms = stats.get("gs_body_residual", (0, 0.0))
+        fc, fms = stats.get("gs_residual", (0, 0.0))
+        sweep_us = (ms + rms + fms) / sw * 1e3
+        kern = {"gs_color_avg_us": ms / max(calls, 1) * 1e3, "colour_launches_per_sweep": calls / sw,
+                "gs_body_residual_avg_us": rms / max(rc, 1) * 1e3, "gs_residual_avg_us": fms / max(fc, 1) * 1e3,
+                "kernel_us_per_sweep": sweep_us, "wall_us_per_sweep": tg / max(ginfo.iters, 1) * 1e6,
+                "achieved_GBs": bytes_sweep / (sweep_us * 1e-6) / 1e9,
+                "frac": bytes_sweep / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                "timing": "HIP events in every dispatch during one more solve of the same problem (direct launches)"}
+        log(f"colored GS kernels: {kern['colour_launches_per_sweep']:.1f} colour launches of "
+            f"{kern['gs_color_avg_us']:.2f} us per sweep, {sweep_us:.1f} us of kernels per sweep")
     return {"value": prob.nc * ginfo.iters / tg, "unit": "contact-updates/s", "sweeps": ginfo.iters,
             "colors": ginfo.ncolors, "residual": ginfo.residual, "seconds_to_solution": tg,
             "pj_seconds_to_solution": elapsed / args.steps, "complementarity": metrics.complementarity(prob, vg, ig),
             "pj_complementarity": comp_pj, "kinetic_energy": float(0.5 * (vg ** 2 / prob.minv).sum()),
-            "pj_kinetic_energy": float(0.5 * (v ** 2 / prob.minv).sum())}
+            "pj_kinetic_energy": float(0.5 * (v ** 2 / prob.minv).sum()), "kernels": kern}
 
 
-def pile_leg(args, native, scenes, metrics):
+def cpu_threads():
+    """Threads for the CPU baseline: the CPUs this process may run on, capped by OMP_NUM_THREADS
+    when the host sets it (a GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
+def pile_leg(args, native, scenes, metrics, hip):
     t0 = time.time()
     pile = scenes.sphere_pile(n=args.spheres, settle_steps=args.settle, seed=0)
     prob = pile.problem()
@@ -183,18 +252,25 @@ def pile_leg(args, native, scenes, metrics):
         info = ctx.pj_solve(dt, 1.0, p)
         log(f"warmup solve: {info.iters} sweeps, residual {info.residual:.3e}")
 
+    stream = ctx.stream()
+    e0, e1 = hip.event(), hip.event()
     ctx.synchronize()
     sweeps = 0
     residuals = []
     t1 = time.perf_counter()
+    hip.record(e0, stream)
     for _ in range(args.steps):
         info = ctx.pj_solve(dt, 1.0, p)
         sweeps += info.iters
         residuals.append(info.residual)
+    hip.record(e1, stream)
     ctx.synchronize()
     elapsed = time.perf_counter() - t1
+    ev_s = hip.elapsed_ms(e0, e1) * 1e-3
+    hip.destroy(e0, e1)
     cups = prob.nc * sweeps / elapsed
-    log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s, {cups:.3e} contact-updates/s")
+    log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s ({ev_s:.3f}s by stream events), "
+        f"{cups:.3e} contact-updates/s")
     td = time.perf_counter()
     v = ctx.velocities()
     imp = ctx.impulses()
@@ -202,41 +278,47 @@ def pile_leg(args, native, scenes, metrics):
     cone_ok = bool((imp[:, 0] >= 0).all() and (np.linalg.norm(imp[:, 1:], axis=1) <= prob.mu * imp[:, 0] * (1 + 1e-12) + 1e-300).all())
     comp_pj = metrics.complementarity(prob, v, imp)
 
-    # Kernel durations: one more step of the same workload with start/stop HIP events carried in
-    # every dispatch on the solve's stream (graph replay is off while events are recorded).
-    ctx.set_profiling(True)
-    for k in ("pj_contacts", "pj_bodies", "pj_residual"):
-        ctx.kernel_time(k)
-    tp0 = time.perf_counter()
-    pinfo = ctx.pj_solve(dt, 1.0, p)
-    ctx.synchronize()
-    tp = time.perf_counter() - tp0
-    ncall_c, ms_c = ctx.kernel_time("pj_contacts")
-    ncall_b, ms_b = ctx.kernel_time("pj_bodies")
-    ctx.set_profiling(False)
-    # launches enqueued after convergence return at once; count the sweeps that did work
-    psweeps = max(pinfo.iters, 1)
-    avg_c = ms_c / 1e3 / psweeps
-    avg_b = ms_b / 1e3 / psweeps
-    ach_c = bc / avg_c / 1e9
-    ach_sweep = (bc + bb) * sweeps / elapsed / 1e9
-    traffic, traffic_src = committed_traffic("k_pj_contacts4", prob.nc)
+    # the whole sweep over the timed region: algorithmic bytes of every sweep / stream-event time
+    bytes_sweep = bc + bb
+    ach_sweep = bytes_sweep * sweeps / ev_s / 1e9
+    # per-kernel durations: one more solve with start/stop events carried in every dispatch
+    kernels = None
+    if not args.no_kernel_events:
+        stats, pinfo = profiled_kernels(ctx, lambda: ctx.pj_solve(dt, 1.0, p))
+        # launches enqueued after convergence return at once; average over the sweeps that did work
+        psweeps = max(pinfo.iters, 1)
+        kernels = {}
+        for name, nbytes in (("pj_contacts", bc), ("pj_bodies", bb), ("pj_residual", 0)):
+            calls, ms = stats.get(name, (0, 0.0))
+            n_eff = psweeps if name != "pj_residual" else max(1, (psweeps + 9) // 10)
+            avg = ms / 1e3 / n_eff
+            kernels[name] = {"avg_us": avg * 1e6, "launches_per_sweep": n_eff / psweeps,
+                             "algorithmic_bytes": nbytes or None,
+                             "GBs": nbytes / avg / 1e9 if nbytes and avg > 0 else None,
+                             "frac": nbytes / avg / 1e9 / HBM_PEAK_GBS if nbytes and avg > 0 else None}
+        kern_sum = sum(k["avg_us"] * k["launches_per_sweep"] for k in kernels.values())
+        kernels["kernel_us_per_sweep"] = kern_sum
+        kernels["timing"] = ("start/stop HIP events carried in every kernel dispatch (hipExtLaunchKernelGGL) on the "
+                             "solve stream, one more solve of the same workload right after the timed ones")
+    traffic, traffic_src = committed_traffic(prob.nc)
 
     # accelerated loops on the same problem and tolerance: time to solution
-    accel = None if args.no_accel else accel_leg(native, ctx, prob, dt, args, metrics, elapsed / args.steps, sweeps / args.steps)
+    accel = None if args.no_accel else accel_leg(native, ctx, prob, dt, args, metrics, elapsed / args.steps,
+                                                 sweeps / args.steps, bytes_sweep)
 
     # coloured Gauss-Seidel on the same problem and tolerance (residual every sweep)
     if args.no_gs:
         gs = None
     else:
-        gs = colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed)
+        gs = colored_gs_leg(native, ctx, prob, dt, args, metrics, v, comp_pj, elapsed, bytes_sweep)
+    ctx.close()
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         prm = oracle.params(max_iter=args.cpu_sweeps, tol_abs=0, tol_rel=0, warmstart=0)
         tc = time.perf_counter()
         st, v_cpu, imp_cpu, res = oracle.pj_solve(prob, dt, prm, nthreads=threads)
@@ -245,15 +327,17 @@ def pile_leg(args, native, scenes, metrics):
         st2, _, _, res2 = oracle.pj_solve(prob, dt, prm, nthreads=-threads)
         tcpu2 = time.perf_counter() - tc
         cpu = {"value": prob.nc * res.iters / tcpu, "unit": "contact-updates/s", "cores": threads, "kind": "port",
-               "cpu": cpu_model(),
+               "cpu": cpu_model(), "host_cpus": os.cpu_count(),
                "sample": f"oracle PJ (C restatement of projected_jacobi.cpp: single-thread SpMV as the reference pins "
-                         f"Eigen to one thread, OpenMP projection on {threads} threads) on the same {prob.nc}-contact "
-                         f"problem, {res.iters} cold-start sweeps incl. W assembly, {tcpu:.1f}s",
+                         f"Eigen to one thread, OpenMP projection on {threads} threads = every CPU this process may use) "
+                         f"on the same {prob.nc}-contact problem, {res.iters} cold-start sweeps incl. W assembly, "
+                         f"{tcpu:.1f}s",
                "fully_threaded": {"value": prob.nc * res2.iters / tcpu2, "cores": threads,
                                   "sample": f"same sweeps with both SpMVs and the vector updates on {threads} OpenMP "
                                             f"threads (upper bound, not the reference's structure), {tcpu2:.1f}s"},
                "gpu_over_cpu": cups / (prob.nc * res.iters / tcpu)}
-        log(f"cpu baseline: {cpu['value']:.3e} (threaded {cpu['fully_threaded']['value']:.3e}) contact-updates/s")
+        log(f"cpu baseline: {cpu['value']:.3e} (threaded {cpu['fully_threaded']['value']:.3e}) contact-updates/s "
+            f"on {threads} threads")
 
     out = {
         "metric": "FP64 contact-updates/s at res<=1e-8 (projected Jacobi)",
@@ -277,23 +361,23 @@ def pile_leg(args, native, scenes, metrics):
                                       "value": prob.nc * sweeps / (elapsed + args.steps * (t_upload + t_download)),
                                       "note": "each step with its host bodies/contacts uploaded and velocities/impulses "
                                               "read back through the C ABI"}},
-        "roofline": {"bound": "hbm", "kernel": "pj_contacts", "achieved": ach_c, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_c / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "bytes_per_launch": bc, "avg_launch_us": avg_c * 1e6,
-                     "pj_bodies_avg_us": avg_b * 1e6, "pj_bodies_GBs": bb / avg_b / 1e9 if avg_b > 0 else None,
-                     "whole_sweep_GBs": ach_sweep, "whole_sweep_frac": ach_sweep / HBM_PEAK_GBS,
-                     "timing": "start/stop HIP events carried in every kernel dispatch (hipExtLaunchKernelGGL) on the "
-                               "solve stream, one profiled step of the same workload run right after the timed steps",
-                     "profiled_step_ms": tp * 1e3, "profiled_sweeps": pinfo.iters},
+        "roofline": {"bound": "hbm", "kernel": "pj_sweep", "achieved": ach_sweep, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_sweep / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "bytes_per_sweep": bytes_sweep, "avg_sweep_us": ev_s / sweeps * 1e6,
+                     "definition": "one projected-Jacobi sweep (contact pass + body pass, residual every 10th): "
+                                   "algorithmic bytes per sweep (328 B per contact with two dynamic sides, 256 B with a "
+                                   "static side, 128 B per body) x sweeps / time of the timed solves measured by HIP "
+                                   "events recorded on the solve stream around them (setup kernels and launch gaps "
+                                   "included)",
+                     "kernels": kernels},
         "cpu_baseline": cpu,
         "accelerated": accel,
         "colored_gs": gs,
     }
-    del ctx
     return out
 
 
-def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std):
+def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std, bytes_sweep):
     """Chebyshev (pj.chebyshev), Nesterov (pj.nesterov) and Anderson (pj.anderson, history
     pj.anderson_m) loops on the pile, cold start, to the same tolerance as the standard loop:
     sweeps and wall time of one solve each.  Anderson is capped at 3x the standard loop's sweeps:
@@ -319,6 +403,16 @@ def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std):
                      "converged": bool(info.residual <= 1.0), "residual": float(info.residual),
                      "speedup_vs_standard": t_std / t, "complementarity": metrics.complementarity(prob, v, imp)}
         log(f"{name}: {info.iters} sweeps, {t:.3f}s ({t_std / t:.2f}x the standard loop), residual {info.residual:.3e}")
+        if name == "chebyshev" and not args.no_kernel_events:
+            stats, pinfo = profiled_kernels(ctx, lambda: ctx.pj_solve(dt, 1.0, p))
+            sw = max(pinfo.iters, 1)
+            per = {k: stats.get(k, (0, 0.0))[1] / sw * 1e3 for k in ("pjc_contacts", "pjc_bodies", "pjc_residual")}
+            us = sum(per.values())
+            out[name]["kernels"] = {**{f"{k}_us_per_sweep": v for k, v in per.items()}, "kernel_us_per_sweep": us,
+                                    "achieved_GBs_standard_bytes": bytes_sweep / (us * 1e-6) / 1e9,
+                                    "timing": "HIP events in every dispatch, one more solve (direct launches)"}
+            log(f"chebyshev kernels: {us:.1f} us per sweep ({per['pjc_contacts']:.1f} contacts, "
+                f"{per['pjc_bodies']:.1f} bodies, {per['pjc_residual']:.1f} residual)")
     return out
 
 
@@ -444,6 +538,8 @@ def batched_leg(args, native, scenes):
         if s % 50 == 0 or s == steps - 1:
             log(f"batched step {s}: {nc} contacts (max {int(per.max()) if len(per) else 0}/scene), "
                 f"max sweeps {int(iters.max())}, solve {t_step_solve * 1e3:.2f} ms")
+    ctx.close()
+    one.close()
     out = {
         "metric": "batched env-steps/s (scenes x Moreau steps per second)",
         "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box, "
@@ -582,6 +678,8 @@ def chains_leg(args, native, scenes):
         t_dev += tc - tb - t_sample
         if s % 50 == 0 or s == steps - 1:
             log(f"chains step {s}: {nc} contacts, max sweeps {int(iters.max())}, solve {t_step * 1e3:.2f} ms")
+    ctx.close()
+    one.close()
     out = {
         "metric": "batched jointed env-steps/s (chains x Moreau steps per second)",
         "workload": f"{ns} chains of 32 capsules (r=0.02 m, length 0.1 m) joined by damped spherical joints "
@@ -635,6 +733,7 @@ def diff_leg(args, native, scenes):
         loss, gpos, gm = diff.com_loss(pos, mass, b.body_off, tgt)
         g = r.backward(gpos)
         t2 = time.perf_counter()
+        r.close()
         return r, loss, g, gm, t1 - t0, t2 - t1
 
     r, loss, g, gm, tf, tb = run(batch, target, True)
@@ -697,6 +796,7 @@ def chains_diff_leg(args, native, scenes):
         loss, gpos, _ = diff.com_loss(pos, b.mass, b.body_off, tgt)
         g = r.backward(gpos)
         t2 = time.perf_counter()
+        r.close()
         return r, loss, g, t1 - t0, t2 - t1
 
     r, loss, g, tf, tb = run(batch, jt, D0, target, True)
@@ -752,6 +852,8 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
+    ap.add_argument("--no-kernel-events", action="store_true",
+                    help="skip the extra solves timed with per-dispatch events (for profiler runs)")
     ap.add_argument("--pile-only", action="store_true", help="skip the batched and differentiable legs")
     ap.add_argument("--batch-scenes", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -795,7 +897,7 @@ def main():
         print(json.dumps({"jointed_chains": chains_leg(args, native, scenes),
                           "jointed_chains_differentiable": chains_diff_leg(args, native, scenes)}), flush=True)
         return
-    out = pile_leg(args, native, scenes, metrics)
+    out = pile_leg(args, native, scenes, metrics, hip)
     out["roofline"]["copy_GBs"] = copy_gbs
     out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
     if not args.pile_only:

@@ -46,10 +46,19 @@ def _run(cmd, cwd=None):
 
 
 def build_device(verbose: bool = True) -> str:
+    """Each source compiled on its own (in parallel), then one link into libcdx.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     out = os.path.join(PKG, "libcdx.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([hipcc, *HIP_FLAGS, "-o", out + ".tmp", *srcs])
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    jobs = [[hipcc, *cflags, "-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), max(1, min(8, os.cpu_count() or 1)))) as ex:
+        list(ex.map(_run, jobs))
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
     return out
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 
 #include "cdx_device.hpp"
 
@@ -223,6 +224,12 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
 }
 
 const char* cdx_last_error(const cdx_ctx* ctx) { return ctx ? ctx->err.c_str() : noalloc_last_error(); }
+
+int cdx_get_stream(cdx_ctx* ctx, void** stream) {
+    if (!ctx || !stream) return CDX_BAD_ARG;
+    *stream = static_cast<void*>(ctx->stream);
+    return CDX_OK;
+}
 
 int cdx_synchronize(cdx_ctx* ctx) {
     if (!ctx) return CDX_BAD_ARG;
@@ -489,6 +496,27 @@ int cdx_set_profiling(cdx_ctx* ctx, int enable) {
     if (!ctx) return CDX_BAD_ARG;
     if (!enable) harvest_events(ctx);
     ctx->profiling = enable != 0;
+    return CDX_OK;
+}
+
+int cdx_kernel_stat(cdx_ctx* ctx, int index, char* name_buf, int buf_len, int* calls, double* total_ms) {
+    if (!ctx || index < 0) return CDX_BAD_ARG;
+    harvest_events(ctx);
+    if (index >= (int)ctx->stats.size()) return CDX_BAD_ARG;
+    auto it = ctx->stats.begin();
+    std::advance(it, index);
+    if (name_buf && buf_len > 0) {
+        std::strncpy(name_buf, it->first.c_str(), (size_t)buf_len - 1);
+        name_buf[buf_len - 1] = '\0';
+    }
+    if (calls) *calls = it->second.calls;
+    if (total_ms) *total_ms = it->second.ms;
+    return CDX_OK;
+}
+
+int cdx_set_graphs(cdx_ctx* ctx, int enable) {
+    if (!ctx) return CDX_BAD_ARG;
+    ctx->graphs = enable != 0;
     return CDX_OK;
 }
 

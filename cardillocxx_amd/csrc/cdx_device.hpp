@@ -201,10 +201,20 @@ __device__ inline double row_D(const double d[3], const double aA[3], const doub
 
 // Runs the sweep loop of a solve: enqueues sweeps in chunks (aligned to the residual cadence) and
 // keeps at most `window` chunks in flight, polling the pinned done flag.  Enqueued sweeps after
-// convergence return on their first instruction.
+// convergence return on their first instruction.  On every exit path, errors included, the
+// stream is drained and the chunk events are destroyed before returning, so a caller may free
+// what the queued work uses (a graph exec, buffers) as soon as this returns.
 template <typename SweepFn>
 int run_sweeps(cdx_ctx* ctx, int max_iter, int chunk, int window, SweepFn&& sweep, int* iters_out) {
-    std::deque<hipEvent_t> inflight;
+    struct Drain {
+        cdx_ctx* ctx;
+        std::deque<hipEvent_t> inflight;
+        ~Drain() {
+            hipStreamSynchronize(ctx->stream);
+            for (hipEvent_t e : inflight) hipEventDestroy(e);
+        }
+    } drain{ctx, {}};
+    auto& inflight = drain.inflight;
     int it = 0;
     bool done = false;
     while (it < max_iter && !done) {
@@ -212,19 +222,16 @@ int run_sweeps(cdx_ctx* ctx, int max_iter, int chunk, int window, SweepFn&& swee
         for (; it < end; ++it) CDX_TRY(sweep(it));
         hipEvent_t ev;
         CDX_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        CDX_HIP(ctx, hipEventRecord(ev, ctx->stream));
         inflight.push_back(ev);
+        CDX_HIP(ctx, hipEventRecord(ev, ctx->stream));
         if ((int)inflight.size() >= window) {
             hipEvent_t e = inflight.front();
             inflight.pop_front();
-            CDX_HIP(ctx, hipEventSynchronize(e));
+            const hipError_t se = hipEventSynchronize(e);
             hipEventDestroy(e);
+            CDX_HIP(ctx, se);
             if (__atomic_load_n(&ctx->h_flag[0], __ATOMIC_ACQUIRE)) done = true;
         }
-    }
-    for (hipEvent_t e : inflight) {
-        hipEventSynchronize(e);
-        hipEventDestroy(e);
     }
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int ctrl[3] = {0, 0, 0};

@@ -1,14 +1,18 @@
 // cdx_adapter_demo.cpp -- C++ end-to-end driver of the adapter: the 1000-sphere drop
 // (8x8 jittered grid, pitch 0.12 m, r=0.05 m, m=1 kg, mu=0.5, 1x1x2 m box), Moreau steps with
 // CPU contact generation, warm-start matching, the device PJ solve behind SolverBase, and the
-// device position update.  Prints a per-step line and a final fingerprint in the shape of the
-// reference's CARDILLO_DUMP_STATE output (examples/example_main.hpp:56-69).
+// device position update.  Prints a per-step line, a final fingerprint line, the section breakdown
+// of TimingManager::printBreakdown (TimingManager.hpp:107-163; host sections plus the device
+// kernels booked under theirs), and, when CARDILLO_DUMP_STATE is set, the reference's
+// "[STATE-DUMP] t= numBodies= totalKE= posNormSum=" line on stderr (examples/example_main.hpp:56-69).
 //
 // usage: cdx_adapter_demo [steps=200] [sweeps=100] [solver=pj|gs]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iomanip>
+#include <iostream>
 #include <vector>
 
 #include "cdx_scene.h"
@@ -47,6 +51,10 @@ int main(int argc, char** argv) {
     cdx::GpuProjectedJacobiSolver pj(P, cfg);
     cdx::GpuColoredGaussSeidelSolver gsv(P, cfg);
     cdx::GpuSolver& solver = gs ? (cdx::GpuSolver&)gsv : (cdx::GpuSolver&)pj;
+    cdx::TimingManager timings;
+    solver.setTimings(&timings);
+    double t = 0.0;
+    cdx::TimingManager::Scope total = timings.scope("Total");
 
     std::vector<int> pa, pb;
     std::vector<double> ppt, pimp;
@@ -54,8 +62,12 @@ int main(int argc, char** argv) {
     std::vector<int> ia(cap), ib(cap);
     std::vector<double> cmu(cap), fr(9 * (size_t)cap), aA(9 * (size_t)cap), aB(9 * (size_t)cap), pt(3 * (size_t)cap), dep(cap);
     for (int s = 0; s < steps; ++s) {
-        const int nc = cdx_sphere_contacts(n, P.q.data(), radius.data(), mub.data(), 5, wn, wd, wmu, 0.0, cap, ia.data(),
-                                           ib.data(), cmu.data(), fr.data(), aA.data(), aB.data(), pt.data(), dep.data());
+        int nc = 0;
+        {
+            auto sc = timings.scope("CollisionNarrowphase");
+            nc = cdx_sphere_contacts(n, P.q.data(), radius.data(), mub.data(), 5, wn, wd, wmu, 0.0, cap, ia.data(), ib.data(),
+                                     cmu.data(), fr.data(), aA.data(), aB.data(), pt.data(), dep.data());
+        }
         if (nc < 0 || nc > cap) {
             std::fprintf(stderr, "contact generation failed\n");
             return 1;
@@ -69,27 +81,44 @@ int main(int argc, char** argv) {
         P.rest.assign(3 * (size_t)nc, 0.0);
         P.vbias.assign(3 * (size_t)nc, 0.0);
         P.warm.assign(3 * (size_t)nc, 0.0);
-        cdx_match_warmstart((int)pa.size(), pa.data(), pb.data(), ppt.data(), pimp.data(), nc, P.body_a.data(),
-                            P.body_b.data(), pt.data(), 0.02, P.warm.data());
+        {
+            auto sc = timings.scope("CollisionMatching");
+            cdx_match_warmstart((int)pa.size(), pa.data(), pb.data(), ppt.data(), pimp.data(), nc, P.body_a.data(),
+                                P.body_b.data(), pt.data(), 0.02, P.warm.data());
+        }
         std::vector<double> v = solver.solve(h, 1.0);
         pa = P.body_a;
         pb = P.body_b;
         ppt.assign(pt.begin(), pt.begin() + 3 * (size_t)nc);
         pimp = P.warm;
         // v_{n+1} and q_{n+1} = q_n + h B(q) v_{n+1} on the device, then back to the host state
-        if (cdx_moreau_update(solver.context(), h, 1.0) != CDX_OK || cdx_get_positions(solver.context(), P.q.data()) != CDX_OK) {
-            std::fprintf(stderr, "moreau update failed: %s\n", cdx_last_error(solver.context()));
-            return 1;
+        {
+            auto sc = timings.scope("Integration");
+            if (cdx_moreau_update(solver.context(), h, 1.0) != CDX_OK ||
+                cdx_get_positions(solver.context(), P.q.data()) != CDX_OK) {
+                std::fprintf(stderr, "moreau update failed: %s\n", cdx_last_error(solver.context()));
+                return 1;
+            }
+            timings.harvestDevice(solver.context());
         }
+        t += h;
         P.v = v;
         double ke = 0;
         for (int i = 0; i < 6 * n; ++i) ke += 0.5 * v[i] * v[i] / P.minv[i];
         if (s % 20 == 0 || s == steps - 1)
             std::printf("step %d contacts %d iters %d KE %.12e\n", s, nc, solver.lastIterations(), ke);
     }
+    { cdx::TimingManager::Scope done = std::move(total); }
     double ke = 0, pos = 0;
     for (int i = 0; i < 6 * n; ++i) ke += 0.5 * P.v[i] * P.v[i] / P.minv[i];
     for (int i = 0; i < n; ++i) pos += std::sqrt(P.q[7 * i] * P.q[7 * i] + P.q[7 * i + 1] * P.q[7 * i + 1] + P.q[7 * i + 2] * P.q[7 * i + 2]);
+    std::fflush(stdout);
+    timings.printBreakdown(std::cout);
+    std::cout << std::flush;
     std::printf("totalKE=%.15e posNormSum=%.15e numBodies=%d solver=%s\n", ke, pos, n, solver.name());
+    std::fflush(stdout);
+    if (std::getenv("CARDILLO_DUMP_STATE"))
+        std::cerr << std::setprecision(15) << "[STATE-DUMP] t=" << t << " numBodies=" << n << " totalKE=" << ke
+                  << " posNormSum=" << pos << std::endl;
     return 0;
 }

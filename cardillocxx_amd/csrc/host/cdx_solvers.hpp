@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "cdx/cdx.h"
+#include "cdx_timing.hpp"
 
 namespace cdx {
 
@@ -100,6 +101,14 @@ class GpuSolver : public SolverBase {
     const cdx_solve_info& lastInfo() const { return m_info; }
     cdx_ctx* context() { return m_ctx; }
 
+    // Section timings of every solve (the reference's DynamicsAssembler::timings()): the solve's
+    // wall time under name(), the upload under RebuildContactJacobians, the device kernels under
+    // their sections (kernel events on: sweeps are then launched one by one, not graph-replayed).
+    void setTimings(TimingManager* t) {
+        m_timings = t;
+        check(cdx_set_profiling(m_ctx, t ? 1 : 0), "cdx_set_profiling");
+    }
+
    protected:
     void check(int st, const char* what) const {
         if (st != CDX_OK)
@@ -108,6 +117,7 @@ class GpuSolver : public SolverBase {
     }
 
     void upload() {
+        TimingManager::Scope sc = timed("RebuildContactJacobians");
         ContactProblem& p = m_problem;
         const int nb = p.numBodies(), nc = p.numContacts();
         check(cdx_set_bodies(m_ctx, nb, p.dof.data(), p.minv.data(), p.v.data(), p.fext.data(),
@@ -120,7 +130,11 @@ class GpuSolver : public SolverBase {
               "cdx_set_contacts");
     }
 
+    // A scope on the timings when they are enabled (a throwaway manager otherwise).
+    TimingManager::Scope timed(const char* section) { return (m_timings ? *m_timings : m_idle).scope(section); }
+
     std::vector<double> finish() {
+        if (m_timings) m_timings->harvestDevice(m_ctx);
         m_last_iters = m_info.iters;
         std::vector<double> v(m_problem.numV());
         check(cdx_get_velocities(m_ctx, v.data()), "cdx_get_velocities");
@@ -133,12 +147,15 @@ class GpuSolver : public SolverBase {
     Config m_cfg;
     cdx_ctx* m_ctx = nullptr;
     cdx_solve_info m_info{};
+    TimingManager* m_timings = nullptr;
+    TimingManager m_idle;
 };
 
 class GpuProjectedJacobiSolver : public GpuSolver {
    public:
     using GpuSolver::GpuSolver;
     std::vector<double> solve(double dt, double theta) override {
+        TimingManager::Scope sc = timed(name());
         upload();
         const cdx_solver_params prm = m_cfg.params();
         check(cdx_pj_solve(m_ctx, dt, theta, &prm, &m_info), "cdx_pj_solve");
@@ -151,6 +168,7 @@ class GpuColoredGaussSeidelSolver : public GpuSolver {
    public:
     using GpuSolver::GpuSolver;
     std::vector<double> solve(double dt, double theta) override {
+        TimingManager::Scope sc = timed(name());
         upload();
         const cdx_solver_params prm = m_cfg.params();
         const int st = cdx_gs_colored_solve(m_ctx, dt, theta, &prm, &m_info);

@@ -41,6 +41,7 @@ class Rollout:
         self.steps = steps
         self.dt = dt
         self.k = max(1, checkpoint_every)
+        self._own_ctx = ctx is None
         self.ctx = ctx or native.Context(device)
         self.prm = native.default_params()
         self.prm.tol_abs = self.prm.tol_rel = tol
@@ -60,6 +61,12 @@ class Rollout:
         # backward recomputes nothing; otherwise it recomputes each segment from its checkpoint
         self.store_steps = store_steps
         self.records = None
+
+    def close(self):
+        """Releases the device context if this rollout created it."""
+        if self._own_ctx and self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
 
     # ---- model parameters -> solver inputs
     def _bodies(self, v, mass):

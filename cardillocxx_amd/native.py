@@ -63,6 +63,9 @@ EXPORTED = [
     "cdx_pj_adjoint",
     "cdx_pj_adjoint_joints",
     "cdx_pj_solve_noalloc",
+    "cdx_set_graphs",
+    "cdx_kernel_stat",
+    "cdx_get_stream",
 ]
 CDX_ADJ_IMPLICIT = 0
 CDX_ADJ_UNROLLED = 1
@@ -132,6 +135,9 @@ def _declare(lib):
         "cdx_moreau_predrift": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_moreau_update": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
+        "cdx_set_graphs": (C.c_int, [_vp, C.c_int]),
+        "cdx_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "cdx_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _ip, _dp]),
         "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
         "cdx_set_scenes": (C.c_int, [_vp, C.c_int, _ip, _ip]),
         "cdx_set_recording": (C.c_int, [_vp, C.c_int, C.c_int]),
@@ -387,6 +393,25 @@ class Context:
 
     def set_profiling(self, on: bool):
         self._check(self._l.cdx_set_profiling(self.h, 1 if on else 0), "cdx_set_profiling")
+
+    def kernel_stats(self):
+        """{kernel name: (calls, total ms)} of every kernel timed since profiling was enabled."""
+        out, buf = {}, C.create_string_buffer(64)
+        calls, ms = C.c_int(0), C.c_double(0)
+        i = 0
+        while self._l.cdx_kernel_stat(self.h, i, buf, 64, C.byref(calls), C.byref(ms)) == CDX_OK:
+            out[buf.value.decode()] = (calls.value, ms.value)
+            i += 1
+        return out
+
+    def stream(self):
+        """The context's hipStream_t as an integer."""
+        s = _vp()
+        self._check(self._l.cdx_get_stream(self.h, C.byref(s)), "cdx_get_stream")
+        return s.value or 0
+
+    def set_graphs(self, on: bool):
+        self._check(self._l.cdx_set_graphs(self.h, 1 if on else 0), "cdx_set_graphs")
 
     def kernel_time(self, name: str):
         calls = C.c_int(0)

@@ -98,6 +98,9 @@ void cdx_ctx_destroy(cdx_ctx* ctx);
  * context-free call (cdx_pj_solve_noalloc). */
 const char* cdx_last_error(const cdx_ctx* ctx);
 int cdx_synchronize(cdx_ctx* ctx);
+/* The context's HIP stream (a hipStream_t) in *stream: every kernel of the context runs on it, so
+ * a caller can order its own work, or record timing events, against the solve. */
+int cdx_get_stream(cdx_ctx* ctx, void** stream);
 
 /* Body state: stacks what DynamicsAssembler::refreshState builds
  * (dynamics_assembler.cpp:41-120, 469-513).  fgyro may be NULL, in which case it is
@@ -251,6 +254,15 @@ int cdx_pj_adjoint_joints(cdx_ctx* ctx, double dt, const cdx_solver_params* prm,
  * one kernel name ("pj_contacts", "pj_bodies", "gs_color", ...) and resets it. */
 int cdx_set_profiling(cdx_ctx* ctx, int enable);
 int cdx_kernel_time(cdx_ctx* ctx, const char* kernel, int* calls, double* total_ms);
+/* The index-th kernel with recorded time (in name order), without resetting it: its name (copied
+ * into name_buf, at most buf_len bytes with the terminating NUL), call count and total ms.
+ * CDX_BAD_ARG past the last one.  For a TimingManager-style breakdown over all kernels. */
+int cdx_kernel_stat(cdx_ctx* ctx, int index, char* name_buf, int buf_len, int* calls, double* total_ms);
+
+/* Replay of sweep chunks as HIP graphs (default on).  With enable = 0 every sweep's kernels are
+ * launched one by one (as with profiling on): the same kernels and results, more host launches.
+ * For profilers and tools that intercept kernel dispatches.  No counterpart in the reference. */
+int cdx_set_graphs(cdx_ctx* ctx, int enable);
 
 /* ---- Allocation-free projected Jacobi.  No counterpart in the reference: this entry point is for
  * callers that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc,

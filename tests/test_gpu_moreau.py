@@ -60,18 +60,44 @@ def test_sphere_drop_per_step_parity(solver):
 
 
 def test_cpp_adapter_drives_the_same_step():
-    """The C++ SolverBase adapter (csrc/host/cdx_solvers.hpp) runs the drop end to end."""
+    """The C++ SolverBase adapter (csrc/host/cdx_solvers.hpp) runs the drop end to end, prints the
+    TimingManager-style breakdown (TimingManager.hpp:107-163) and, with CARDILLO_DUMP_STATE set,
+    the reference's [STATE-DUMP] line on stderr (examples/example_main.hpp:56-69)."""
     exe = os.path.join(ROOT, "cardillocxx_amd", "cdx_adapter_demo")
-    out = subprocess.run([exe, "50", "100"], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, CARDILLO_DUMP_STATE="1")
+    out = subprocess.run([exe, "50", "100"], capture_output=True, text=True, timeout=120, env=env)
     assert out.returncode == 0, out.stderr
     lines = out.stdout.strip().splitlines()
     assert lines[-1].startswith("totalKE=") and "numBodies=1000" in lines[-1]
     # exactly 100 sweeps whenever the step has contacts; none (free flight) when it has none,
     # as projected_jacobi.cpp:460-467 returns the free solution without sweeping
-    steps = [l.split() for l in lines[:-1]]
-    assert steps and all(s[0] == "step" for s in steps), out.stdout
+    steps = [l.split() for l in lines if l.startswith("step ")]
+    assert steps, out.stdout
     for s in steps:
         assert int(s[5]) == (100 if int(s[3]) > 0 else 0), " ".join(s)
     assert any(int(s[3]) > 0 for s in steps)
     ke = float(lines[-1].split()[0].split("=")[1])
+    pos = float(lines[-1].split()[1].split("=")[1])
     assert np.isfinite(ke) and ke > 0
+    # breakdown: header, the sections the run went through, percentages summing to 100
+    i = lines.index("[Timing Breakdown - Exclusive]")
+    assert lines[i + 1].split() == ["Section", "Calls", "Excl(s)", "%", "Avg(us)"]
+    rows = {}
+    for l in lines[i + 3:]:
+        if l.startswith("---"):
+            break
+        f = l.split()
+        rows[f[0]] = (int(f[1]), float(f[2]), float(f[3]))
+    for sec in ("ProjectedJacobi", "ProjectedJacobiSweep", "ProjectedJacobiSetup", "RebuildContactJacobians",
+                "CollisionNarrowphase", "CollisionMatching", "Integration"):
+        assert sec in rows, (sec, rows)
+    assert rows["CollisionNarrowphase"][0] == 50 and rows["ProjectedJacobi"][0] == 50
+    assert abs(sum(r[2] for r in rows.values()) - 100.0) < 0.1
+    total = [l for l in lines if l.startswith("Total (inclusive)")]
+    assert total and float(total[0].split()[2]) > 0
+    dump = [l for l in out.stderr.splitlines() if l.startswith("[STATE-DUMP]")]
+    assert len(dump) == 1, out.stderr
+    kv = dict(f.split("=") for f in dump[0].split()[1:])
+    assert set(kv) == {"t", "numBodies", "totalKE", "posNormSum"}
+    assert kv["numBodies"] == "1000" and float(kv["t"]) == pytest.approx(0.05, rel=1e-12)
+    assert float(kv["totalKE"]) == pytest.approx(ke, rel=1e-14) and float(kv["posNormSum"]) == pytest.approx(pos, rel=1e-14)
