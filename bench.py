@@ -288,12 +288,18 @@ def pile_leg(args, native, scenes, metrics, hip):
         # launches enqueued after convergence return at once; average over the sweeps that did work
         psweeps = max(pinfo.iters, 1)
         kernels = {}
-        for name, nbytes in (("pj_contacts", bc), ("pj_bodies", bb), ("pj_residual", 0)):
+        # the sweep's kernels: the fused tile sweep and its boundary pass, or the separate contact
+        # and body passes; the residual reduction every 10th sweep
+        sweep_bytes = {"pj_fused": bc + bb, "pj_contacts": bc, "pj_bodies": bb}
+        for name in ("pj_fused", "pj_boundary", "pj_contacts", "pj_bodies", "pj_residual"):
             calls, ms = stats.get(name, (0, 0.0))
+            if not calls:
+                continue
             n_eff = psweeps if name != "pj_residual" else max(1, (psweeps + 9) // 10)
             avg = ms / 1e3 / n_eff
+            nbytes = sweep_bytes.get(name)
             kernels[name] = {"avg_us": avg * 1e6, "launches_per_sweep": n_eff / psweeps,
-                             "algorithmic_bytes": nbytes or None,
+                             "algorithmic_bytes": nbytes,
                              "GBs": nbytes / avg / 1e9 if nbytes and avg > 0 else None,
                              "frac": nbytes / avg / 1e9 / HBM_PEAK_GBS if nbytes and avg > 0 else None}
         kern_sum = sum(k["avg_us"] * k["launches_per_sweep"] for k in kernels.values())
