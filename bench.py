@@ -446,10 +446,12 @@ def batched_leg(args, native, scenes):
     ns, nb = b.nscenes, len(b.mass)
     log(f"batched: {ns} scenes, {nb} bodies, generated in {time.time() - t0:.1f}s")
     dt = 1e-3
+    accel = {"none": native.CDX_ACCEL_NONE, "chebyshev": native.CDX_ACCEL_CHEBYSHEV}[args.batch_accel]
     p = native.default_params()
     p.tol_abs = p.tol_rel = args.batch_tol
     p.max_iter = args.batch_max_iter
     p.warmstart = 1
+    p.accel = accel
     ctx = native.Context(0)
     ctx.set_bodies(bd["dof"], bd["minv"], bd["v"], bd["fext"], None, bd["q"])
     scene_of = np.repeat(np.arange(b.nscenes), np.diff(b.body_off))
@@ -460,6 +462,62 @@ def batched_leg(args, native, scenes):
     q = bd["q"].copy()
     v = bd["v"].copy()
     prev = None
+
+    def contacts_of(q, prev):
+        con = b.contacts(q=q.reshape(-1, 7))
+        nc = len(con["body_a"])
+        warm = scenes.match_warmstart(prev, con) if prev is not None and nc else np.zeros((nc, 3))
+        return con, warm
+
+    def upload(con, warm):
+        ctx.set_contacts(con["body_a"], con["body_b"], con["mu"], con["frame"], con["angA"], con["angB"], con["rest"],
+                         con["vbias"], warm)
+        ctx.set_scenes(b.body_off, con["contact_off"])
+
+    def advance(h, q_prev):
+        """Moreau update on the device; a scene whose state went non-finite (projected Jacobi is not
+        guaranteed to converge) is put back at rest where it was.  Returns q, v, impulses, resets."""
+        ctx.moreau_update(h, 1.0)
+        q, v, imp = ctx.positions(), ctx.velocities(), ctx.impulses()
+        bad_body = ~(np.isfinite(v.reshape(-1, 6)).all(axis=1) & np.isfinite(q.reshape(-1, 7)).all(axis=1))
+        nbad = 0
+        if bad_body.any():
+            bad = np.unique(scene_of[bad_body])
+            nbad = len(bad)
+            mask = np.isin(scene_of, bad)
+            q = q.reshape(-1, 7).copy()
+            v = v.reshape(-1, 6).copy()
+            q[mask] = q_prev.reshape(-1, 7)[mask]
+            v[mask] = 0.0
+            q, v = q.reshape(-1), v.reshape(-1)
+            imp = np.where(np.isfinite(imp), imp, 0.0)
+            ctx.set_bodies(bd["dof"], bd["minv"], v, bd["fext"], None, q)
+        return q, v, imp, nbad
+
+    # settle (input preparation, not timed): larger steps at a loose tolerance until the scenes
+    # lie in piles, so the timed steps run on the configured contact load
+    settle = None
+    if args.batch_settle > 0:
+        ts0 = time.perf_counter()
+        ps = native.default_params()
+        ps.tol_abs = ps.tol_rel = 1e-4
+        ps.max_iter = 1000
+        ps.warmstart = 1
+        ps.accel = native.CDX_ACCEL_CHEBYSHEV
+        resets = 0
+        for s in range(args.batch_settle):
+            con, warm = contacts_of(q, prev)
+            upload(con, warm)
+            ctx.batch_pj_solve(args.batch_settle_dt, 1.0, ps)
+            q, v, imp, nbad = advance(args.batch_settle_dt, q)
+            resets += nbad
+            prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
+        per = np.diff(con["contact_off"])
+        settle = {"steps": args.batch_settle, "h": args.batch_settle_dt, "tol": 1e-4, "max_iter": 1000,
+                  "loop": "chebyshev", "scene_resets": resets, "s": time.perf_counter() - ts0,
+                  "contacts_after": int(len(con["body_a"])), "max_contacts_per_scene_after": int(per.max())}
+        log(f"batched: settled {args.batch_settle} steps of {args.batch_settle_dt:g} s in {settle['s']:.1f}s, "
+            f"{settle['contacts_after']} contacts (max {settle['max_contacts_per_scene_after']}/scene)")
     t_cpu = t_dev = t_solve = 0.0
     cu = 0  # contact-updates (sum over scenes of contacts x sweeps)
     contacts_total = 0
@@ -469,16 +527,14 @@ def batched_leg(args, native, scenes):
     obo_solves = 0
     worst_dv = 0.0
     per_scene_max = 0
+    per_scene_all = []
     blown = 0  # scene-steps whose velocities or positions went non-finite (reset to rest)
     for s in range(steps):
         ta = time.perf_counter()
-        con = b.contacts(q=q.reshape(-1, 7))
+        con, warm = contacts_of(q, prev)
         nc = len(con["body_a"])
-        warm = scenes.match_warmstart(prev, con) if prev is not None and nc else np.zeros((nc, 3))
         tb = time.perf_counter()
-        ctx.set_contacts(con["body_a"], con["body_b"], con["mu"], con["frame"], con["angA"], con["angB"], con["rest"],
-                         con["vbias"], warm)
-        ctx.set_scenes(b.body_off, con["contact_off"])
+        upload(con, warm)
         if s == args.tune_step:
             tune = ctx.batch_autotune(dt, 1.0, p)
             log(f"batched: autotuned (block, groups) = ({tune[0]}, {tune[1]}), {tune[2]:.3f} ms at step {s}")
@@ -489,6 +545,7 @@ def batched_leg(args, native, scenes):
         t_solve += t_step_solve
         coff = con["contact_off"]
         per = np.diff(coff)
+        per_scene_all.append(per.copy())
         per_scene_max = max(per_scene_max, int(per.max()) if len(per) else 0)
         cu += int((per.astype(np.int64) * iters).sum())
         contacts_total += nc
@@ -517,26 +574,10 @@ def batched_leg(args, native, scenes):
             batch_time_sampled += t_step_solve
             obo_solves += len(pick)
             t_sample = time.perf_counter() - tsa
-        ctx.moreau_update(dt, 1.0)
-        q_prev = q
-        q = ctx.positions()
-        v = ctx.velocities()
-        imp = ctx.impulses()
-        bad_body = ~(np.isfinite(v.reshape(-1, 6)).all(axis=1) & np.isfinite(q.reshape(-1, 7)).all(axis=1))
-        if bad_body.any():
-            # a diverged scene (projected Jacobi is not guaranteed to converge): put it back at
-            # rest where it was and keep the other scenes going
-            bad = np.unique(scene_of[bad_body])
-            blown += len(bad)
-            mask = np.isin(scene_of, bad)
-            q = q.reshape(-1, 7).copy()
-            v = v.reshape(-1, 6).copy()
-            q[mask] = q_prev.reshape(-1, 7)[mask]
-            v[mask] = 0.0
-            q, v = q.reshape(-1), v.reshape(-1)
-            imp = np.where(np.isfinite(imp), imp, 0.0)
-            ctx.set_bodies(bd["dof"], bd["minv"], v, bd["fext"], None, q)
-            log(f"batched step {s}: scenes {bad.tolist()[:8]} went non-finite; reset to rest")
+        q, v, imp, nbad = advance(dt, q)
+        if nbad:
+            blown += nbad
+            log(f"batched step {s}: {nbad} scenes went non-finite; reset to rest")
         tc = time.perf_counter()
         prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         t_cpu += tb - ta
@@ -546,11 +587,19 @@ def batched_leg(args, native, scenes):
                 f"max sweeps {int(iters.max())}, solve {t_step_solve * 1e3:.2f} ms")
     ctx.close()
     one.close()
+    allper = np.concatenate(per_scene_all) if per_scene_all else np.zeros(1, int)
+    dist = {"p10": float(np.percentile(allper, 10)), "p50": float(np.percentile(allper, 50)),
+            "p90": float(np.percentile(allper, 90)), "p99": float(np.percentile(allper, 99)), "max": int(allper.max()),
+            "share_10_to_3000": float(((allper >= 10) & (allper <= 3000)).mean()),
+            "last_step_max": int(per_scene_all[-1].max()) if per_scene_all else 0}
     out = {
         "metric": "batched env-steps/s (scenes x Moreau steps per second)",
-        "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box, "
-                    f"h=1e-3 s, {steps} steps, warm-started PJ to tol {args.batch_tol:g} per scene (reference default), "
-                    f"one batched solve launch per step",
+        "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box"
+                    f"{f' and settled ({args.batch_settle} steps of {args.batch_settle_dt:g} s, untimed)' if settle else ''}, "
+                    f"h=1e-3 s, {steps} steps, warm-started PJ ({args.batch_accel} loop) to tol {args.batch_tol:g} "
+                    f"per scene (reference default), one batched solve launch per step",
+        "loop": args.batch_accel, "settle": settle,
+        "contacts_per_scene": dist,
         "steps": steps, "scenes": ns, "bodies": nb,
         "value": ns * steps / t_dev, "unit": "env-steps/s",
         "definition": "device path per step: contact upload, batched solve, position update, state download",
@@ -559,7 +608,8 @@ def batched_leg(args, native, scenes):
         "cpu_contact_generation_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
         "contact_updates_per_s": cu / t_solve, "mean_contacts_per_step": contacts_total / steps,
         "max_contacts_per_scene": per_scene_max, "scene_solves_at_iteration_cap": capped,
-        "scene_steps_reset_after_divergence": blown,
+        "capped_rate": capped / float(ns * steps), "scene_steps_reset_after_divergence": blown,
+        "reset_rate": blown / float(ns * steps),
         "launch": {"block": tune[0], "groups": tune[1],
                    "note": "the tuner's shape; with one scene per workgroup and every scene within 60 KiB of LDS the "
                            "solve keeps each scene in LDS and raises the block to one thread per contact and per body "
@@ -866,6 +916,11 @@ def main():
     ap.add_argument("--batch-max-iter", type=int, default=20_000)
     ap.add_argument("--batch-tol", type=float, default=1e-5, help="batched PJ tolerance (the reference default)")
     ap.add_argument("--tune-step", type=int, default=100)
+    ap.add_argument("--batch-accel", choices=("none", "chebyshev"), default="none",
+                    help="loop of the batched solve (pj.chebyshev); the one-by-one comparison uses the same")
+    ap.add_argument("--batch-settle", type=int, default=0, help="untimed settling steps before the batched steps")
+    ap.add_argument("--batch-settle-dt", type=float, default=4e-3)
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg and print it")
     ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
     ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")
     ap.add_argument("--no-chains", action="store_true", help="skip the jointed capsule-chain leg")
@@ -899,6 +954,9 @@ def main():
     copy_gbs = hip.copy_bandwidth()
     log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
 
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args, native, scenes)}), flush=True)
+        return
     if args.chains_only:
         print(json.dumps({"jointed_chains": chains_leg(args, native, scenes),
                           "jointed_chains_differentiable": chains_diff_leg(args, native, scenes)}), flush=True)

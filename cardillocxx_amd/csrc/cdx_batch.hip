@@ -166,6 +166,284 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     }
 }
 
+// ---- Chebyshev-accelerated standard loop per scene (chebyshev_loop with estimateSpectralRadius,
+// projected_jacobi.cpp:145-228): each scene's group estimates its own spectral radius (8
+// normalised power iterations of the linear sweep map from ones/sqrt(nv), rho = min(lambda,
+// 0.995), 0 when the norm vanishes) and runs its own w recurrence, residual every sweep.  As in
+// the grid-wide loop the extrapolation is fused into the next sweep: X holds the swept iterate,
+// XK = xk, XK1 the previous input; the passes form the input u = w (X - XK) + XK (u = X on the
+// first sweep and after a fallback), rotate XK <- XK1, XK1 <- u and sweep from u.  A converged
+// scene returns its sweep input (XK1, PK1); after max_iter sweeps the pending update.
+
+// linear contact pass of the power iteration: dp = -R (W v), no projection, no bias
+__device__ inline void cheb_lin_contact(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                        const double* __restrict__ J, const double* __restrict__ R,
+                                        const double* __restrict__ mu, const double* __restrict__ v,
+                                        double* __restrict__ imp) {
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const int nr = mu[c] > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6];
+    load_body(v, a, ua);
+    load_body(v, b, ub);
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = -(R[(size_t)k * nc + c] * row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub));
+    }
+    write_side_impulses9(imp, c, a, b, nr, j, dp);
+}
+
+__device__ inline void cheb_input(const double* __restrict__ X, const double* __restrict__ XK, int b, double w,
+                                  bool take, double u[6]) {
+    load_body(X, b, u);
+    if (take || b < 0) return;
+    double k6[6];
+    load_body(XK, b, k6);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = cheb_extrap(w, u[m], k6[m]);
+}
+
+// contact pass of a Chebyshev sweep (the grid-wide k_pjc_contacts4, one thread per contact)
+__device__ inline void pjc_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                         const double* __restrict__ J, const double* __restrict__ R,
+                                         const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
+                                         double* __restrict__ P, double* __restrict__ PK, double* __restrict__ PK1,
+                                         const double* __restrict__ X, const double* __restrict__ XK, double w,
+                                         bool take, double* __restrict__ imp) {
+    const int a = ida[c], b = idb[c];
+    if (a < 0 && b < 0) return;
+    const double muc = mu[c];
+    const int nr = muc > 0 ? 3 : 1;
+    ContactJ j;
+    load_J(J, nc, c, nr, j);
+    double ua[6], ub[6];
+    cheb_input(X, XK, a, w, take, ua);
+    cheb_input(X, XK, b, w, take, ub);
+    double pin[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const size_t r = (size_t)k * nc + c;
+        const double ps = P[r];
+        pin[k] = take ? ps : cheb_extrap(w, ps, PK[r]);
+        PK[r] = PK1[r];
+        PK1[r] = pin[k];
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        pn[k] = pin[k] - (R[r] * g + (use_bias ? bias[r] : 0.0));
+    }
+    pn[0] = fmax(pn[0], 0.0);
+    if (nr == 3) {
+        const double t1 = pn[1], t2 = pn[2];
+        const double tn2 = t1 * t1 + t2 * t2;
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (muc * pn[0]) / sqrt(tn2));
+            pn[1] = sc * t1;
+            pn[2] = sc * t2;
+        }
+    }
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = pn[k] - pin[k];
+        P[(size_t)k * nc + c] = pn[k];
+    }
+    write_side_impulses9(imp, c, a, b, nr, j, dp);
+}
+
+__global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pjc(
+    int ns, int groups, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
+    const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
+    const int* __restrict__ idb, const double* __restrict__ J, const double* __restrict__ mu,
+    const double* __restrict__ rest, const double* __restrict__ vb, double* __restrict__ R, double* __restrict__ bias,
+    double* __restrict__ p, double* __restrict__ imp, const int* __restrict__ dof, const double* __restrict__ minv,
+    const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
+    const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double* __restrict__ XK,
+    double* __restrict__ XK1, double* __restrict__ PK, double* __restrict__ PK1, double dt, double alpha, int use_bias,
+    int warmstart, int max_iter, double tol_rel, double tol_abs, int* __restrict__ iters_out,
+    double* __restrict__ res_out) {
+    __shared__ double wsum[kMaxBatchBlock / 64];
+    __shared__ double wbad[kMaxBatchBlock / 64];
+    __shared__ int gdone[kMaxBatchBlock / 64];
+    __shared__ int all_done;
+    const int gsize = blockDim.x / groups;
+    const int g = threadIdx.x / gsize, lt = threadIdx.x - g * gsize;
+    const int slot = blockIdx.x * groups + g;
+    const int s = slot < ns ? order[slot] : -1;
+    const int b0 = s >= 0 ? boff[s] : 0, b1 = s >= 0 ? boff[s + 1] : 0;
+    const int c0 = s >= 0 ? coff[s] : 0, c1 = s >= 0 ? coff[s + 1] : 0;
+    const bool rows = s >= 0 && srows[s] > 0;
+    const int wpg = gsize >> 6;
+    // the group's sum of one value per thread, in every thread of the group (two barriers)
+    auto group_sum2 = [&](double a, double b, double* sa, double* sb) {
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            b += __shfl_xor(b, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wsum[threadIdx.x >> 6] = a;
+            wbad[threadIdx.x >> 6] = b;
+        }
+        __syncthreads();
+        double ta = 0, tb = 0;
+        for (int w = 0; w < wpg; ++w) {
+            ta += wsum[g * wpg + w];
+            tb += wbad[g * wpg + w];
+        }
+        *sa = ta;
+        *sb = tb;
+        __syncthreads();
+    };
+    if (s >= 0)
+        for (int c = c0 + lt; c < c1; c += gsize)
+            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+    __syncthreads();
+    for (int b = b0 + lt; b < b1; b += gsize)
+        pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
+    // spectral radius, the power vector in XK1
+    const double inv = 1.0 / sqrt((double)(s >= 0 ? snv[s] : 1));
+    for (int i = lt; i < 6 * (b1 - b0); i += gsize) {
+        const int b = b0 + i / 6, l = i % 6;
+        XK1[(size_t)b * kBodyStride + l] = l < dof[b] ? inv : 0.0;
+    }
+    __syncthreads();
+    bool stopped = !rows;
+    double lambda = 0;
+    for (int pit = 0; pit < 8; ++pit) {
+        if (!stopped)
+            for (int c = c0 + lt; c < c1; c += gsize) cheb_lin_contact(c, nc, ida, idb, J, R, mu, XK1, imp);
+        __syncthreads();
+        double s2 = 0;
+        if (!stopped)
+            for (int b = b0 + lt; b < b1; b += gsize) {
+                double acc[6] = {0, 0, 0, 0, 0, 0};
+                for (int i = incptr[b]; i < incptr[b + 1]; ++i) gather_side9(imp, incidx[i], acc);
+                const double* mi = minv + (size_t)b * kMinvStride;
+                for (int l = 0; l < dof[b]; ++l) {
+                    const double mk = l < 3 ? mi[0] : mi[l - 2];
+                    const double M = mk > 0 ? 1.0 / mk : 0.0;
+                    double* vp = XK1 + (size_t)b * kBodyStride + l;
+                    const double un = *vp + acc[l] / M;
+                    *vp = un;
+                    s2 += un * un;
+                }
+            }
+        double n2 = 0, unused = 0;
+        group_sum2(s2, 0.0, &n2, &unused);
+        if (!stopped) {
+            const double n = sqrt(n2);
+            if (!(n > 1e-300)) {
+                stopped = true;
+                lambda = 0.0;
+            } else {
+                lambda = n;
+                for (int b = b0 + lt; b < b1; b += gsize)
+                    for (int l = 0; l < dof[b]; ++l) XK1[(size_t)b * kBodyStride + l] /= n;
+            }
+        }
+        __syncthreads();
+    }
+    const double rho = lambda > 0 ? fmin(lambda, 0.995) : 0.0;
+    const double rho2 = rho * rho;
+    // xk = xk1 = x0, pk = pk1 = p0
+    for (int i = lt; i < 6 * (b1 - b0); i += gsize) {
+        const size_t o = (size_t)(b0 + i / 6) * kBodyStride + i % 6;
+        XK[o] = XK1[o] = x[o];
+    }
+    for (int i = lt; i < 3 * (c1 - c0); i += gsize) {
+        const size_t r = (size_t)(i / (c1 - c0)) * nc + c0 + i % (c1 - c0);
+        PK[r] = PK1[r] = p[r];
+    }
+    if (lt == 0) gdone[g] = rows ? 0 : 1;
+    bool done = !rows, converged = false;
+    int iters = 0;
+    double res = NAN, w = 1.0;
+    bool take = true;
+    __syncthreads();
+    for (int it = 0; it < max_iter; ++it) {
+        if (!done)
+            for (int c = c0 + lt; c < c1; c += gsize)
+                pjc_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, PK, PK1, x, XK, w, take, imp);
+        __syncthreads();
+        double e2 = 0, bad = 0;
+        if (!done)
+            for (int b = b0 + lt; b < b1; b += gsize) {
+                double acc[6] = {0, 0, 0, 0, 0, 0};
+                const int i0 = incptr[b], i1 = incptr[b + 1];
+                for (int i = i0; i < i1; ++i) gather_side9(imp, incidx[i], acc);
+                const double* mi = minv + (size_t)b * kMinvStride;
+                for (int l = 0; l < dof[b]; ++l) {
+                    const size_t o = (size_t)b * kBodyStride + l;
+                    const double xs = x[o];
+                    const double u = take ? xs : cheb_extrap(w, xs, XK[o]);
+                    XK[o] = XK1[o];
+                    XK1[o] = u;
+                    const double mk = l < 3 ? mi[0] : mi[l - 2];
+                    const double M = mk > 0 ? 1.0 / mk : 0.0;
+                    const double un = i1 > i0 ? u + acc[l] / M : u;
+                    x[o] = un;
+                    const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                    const double e = (un - u) / q;
+                    e2 += e * e;
+                    bad += isfinite(un) ? 0.0 : 1.0;
+                }
+            }
+        double t2 = 0, nbad = 0;
+        group_sum2(e2, bad, &t2, &nbad);
+        if (!done) {
+            iters = it + 1;
+            res = 1.0 / sqrt((double)snv[s]) * sqrt(t2);
+            if (res <= 1.0) {
+                done = converged = true;
+            } else {
+                const double wn = cheb_next_w(it, rho2, w);
+                if (!isfinite(wn) || wn <= 0.0 || nbad != 0.0) {
+                    w = 1.0;
+                    take = true;
+                } else {
+                    w = wn;
+                    take = false;
+                }
+            }
+        }
+        if (lt == 0) gdone[g] = done ? 1 : 0;
+        if (threadIdx.x == 0) all_done = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int a = 1;
+            for (int k = 0; k < groups; ++k) a &= gdone[k];
+            all_done = a;
+        }
+        __syncthreads();
+        if (all_done) break;
+    }
+    // the converged sweep's input, or the pending update after max_iter sweeps
+    if (rows) {
+        for (int i = lt; i < 6 * (b1 - b0); i += gsize) {
+            const int b = b0 + i / 6, l = i % 6;
+            if (l >= dof[b]) continue;
+            const size_t o = (size_t)b * kBodyStride + l;
+            if (converged) x[o] = XK1[o];
+            else if (!take) x[o] = cheb_extrap(w, x[o], XK[o]);
+        }
+        for (int i = lt; i < 3 * (c1 - c0); i += gsize) {
+            const int k = i / (c1 - c0), c = c0 + i % (c1 - c0);
+            if ((ida[c] < 0 && idb[c] < 0) || k >= (mu[c] > 0 ? 3 : 1)) continue;
+            const size_t r = (size_t)k * nc + c;
+            if (converged) p[r] = PK1[r];
+            else if (!take) p[r] = cheb_extrap(w, p[r], PK[r]);
+        }
+    }
+    if (s >= 0 && lt == 0) {
+        iters_out[s] = rows ? iters : 0;
+        res_out[s] = res;
+    }
+}
+
 // ---- one scene per workgroup with its velocities and side-impulse records in LDS
 //
 // When every scene's bodies (6 doubles each), contact records (9 doubles each) and incidence
@@ -415,6 +693,19 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
     const int ns = ctx->nscenes;
     const int* d = ctx->d_scene_i;
     const int grid = (ns + groups - 1) / groups;
+    if (prm->accel == CDX_ACCEL_CHEBYSHEV) {
+        const size_t n8 = (size_t)ctx->nb * kBodyStride, n3 = 3 * (size_t)ctx->nc;
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_xk1, n8, &ctx->xk1_cap, "chebyshev xk1"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, n3, &ctx->pk_cap, "chebyshev pk"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_pk1, n3, &ctx->pk1_cap, "chebyshev pk1"));
+        CDX_LAUNCH(ctx, "batch_pjc", k_batch_pjc, grid, block, ns, groups, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2,
+                   d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R,
+                   ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,
+                   ctx->d_incptr, ctx->d_incidx, ctx->d_x, ctx->d_uprev, ctx->d_xk1, ctx->d_pk, ctx->d_pk1, dt,
+                   prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->tol_rel, prm->tol_abs,
+                   ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+        return hip_check(ctx, hipGetLastError(), "batch launch");
+    }
     size_t shm = 0;
     const int wave_lds = ctx->njrows > 0 ? batch_wave_lds(ctx, block, &shm) : 0;
     if (ctx->njrows > 0 && wave_lds < kMinWaveLds)
@@ -513,7 +804,10 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     CDX_TRY(check_solve_args(ctx, prm));
     const bool joints = ctx->njrows > 0;
     if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: theta must be positive with joint rows");
-    if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: only the standard loop is batched");
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_CHEBYSHEV)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve runs the standard or the Chebyshev loop");
+    if (prm->accel == CDX_ACCEL_CHEBYSHEV && ctx->njrows > 0)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched Chebyshev loop does not take joint rows");
     if (prm->block_precond)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve uses the row scaling only");
     if (ctx->nscenes < 1 || ctx->scene_boff.back() != ctx->nb || ctx->scene_coff.back() != ctx->nc)
@@ -536,7 +830,7 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         if (it[s] < 0) return set_err(ctx, CDX_NOT_FACTORIZED, "cdx_batch_pj_solve: S is not factorized (singular joint block)");
     ctx->solved = true;
     ctx->joints_solved = joints;
-    ctx->pj_last = !joints;  // same converged state as cdx_pj_solve: the implicit adjoint applies
+    ctx->pj_last = !joints;  // a converged state of the same fixed point: the implicit adjoint applies
     ctx->pjj_last = joints;
     ctx->last_theta = theta;
     ctx->record_valid = false;

@@ -523,11 +523,6 @@ __global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const doubl
 // fallback), 2 rho^2, 3 rho, 4 power iteration stopped, 5 lambda, 6 current norm.
 // ---------------------------------------------------------------------------------------------
 
-// w (x - xk) + xk with separate roundings, as the reference's vector expression
-__device__ inline double cheb_extrap(double w, double x, double xk) {
-#pragma clang fp contract(off)
-    return w * (x - xk) + xk;
-}
 
 __device__ inline void cheb_input6(const double* __restrict__ X, const double* __restrict__ XK, int b, double w, bool take,
                                    double u[6]) {

@@ -165,6 +165,20 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
     else store6(x + (size_t)bi * kBodyStride, out);
 }
 
+// w (x - xk) + xk with separate roundings, as the reference's vector expression (the Chebyshev
+// extrapolation, projected_jacobi.cpp:208-209)
+__device__ inline double cheb_extrap(double w, double x, double xk) {
+#pragma clang fp contract(off)
+    return w * (x - xk) + xk;
+}
+
+// The next Chebyshev weight after sweep `it` (projected_jacobi.cpp:202): 1 / (1 - rho^2 / 2) after
+// the first sweep, 1 / (1 - rho^2 / 4 w) after later ones.
+__device__ inline double cheb_next_w(int it, double rho2, double w) {
+#pragma clang fp contract(off)
+    return it == 0 ? 1.0 / (1.0 - rho2 * 0.5) : 1.0 / (1.0 - (rho2 * 0.25) * w);
+}
+
 // One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).
 // The bias is only read when the problem has one (`use_bias`: some restitution or velocity bias).
 __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,

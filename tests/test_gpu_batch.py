@@ -147,3 +147,69 @@ def test_scene_validation(ctx):
             ctx.set_scenes(boff, np.sort(moved))
     with pytest.raises(native.CdxError):
         ctx.batch_set_launch(256, 3)
+
+
+CHEB_CASES = [
+    dict(nb=20, nc=60, seed=2),
+    dict(nb=50, nc=200, seed=3, p_point=0.3),
+    dict(nb=200, nc=300, seed=4, rest=0.4),
+    dict(nb=3, nc=8, seed=6, p_static=0.3, p_point=1.0, warm=False),  # full column rank: rho < 0.995
+]
+
+
+@pytest.mark.parametrize("launch", [(256, 1), (512, 4)])
+@pytest.mark.parametrize("sweeps", [1, 2, 9, 64])
+def test_batch_chebyshev_fixed_sweeps_matches_oracle(ctx, launch, sweeps):
+    """Per-scene Chebyshev loop (chebyshev_loop with estimateSpectralRadius,
+    projected_jacobi.cpp:145-228): every scene's own spectral radius and w recurrence, equal to
+    the oracle's standalone Chebyshev solve of that scene after the same sweeps."""
+    probs = [scenes.random_problem(**c) for c in CHEB_CASES] + make_batch(12, seed=3)
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.accel = native.CDX_ACCEL_CHEBYSHEV
+    p.max_iter, p.tol_abs, p.tol_rel = sweeps, 0.0, 0.0
+    try:
+        ctx.batch_set_launch(*launch)
+        iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    finally:
+        ctx.batch_set_launch(256, 1)
+    v, imp = ctx.velocities(), ctx.impulses()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, max_iter=sweeps, tol_abs=0.0,
+                                                                          tol_rel=0.0))
+        assert st == 0 and iters[s] == r.iters, s
+        vs = v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()]
+        assert rel(vs, v_ref) <= TOL, s
+        if prob.nc:
+            assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL, s
+
+
+@pytest.mark.parametrize("tol", [1e-5, 1e-9])
+def test_batch_chebyshev_to_tolerance(ctx, tol):
+    """Scenes on which the Chebyshev loop converges: the same iteration count as the oracle and
+    the converged sweep's input returned, per scene."""
+    probs = [scenes.random_problem(**CHEB_CASES[0]), scenes.random_problem(**CHEB_CASES[3])]
+    probs += [scenes.sphere_pile(n=40 + 20 * k, settle_steps=60, box=(0.5, 0.5, 1.0), seed=k).problem() for k in range(4)]
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.accel = native.CDX_ACCEL_CHEBYSHEV
+    p.tol_abs = p.tol_rel = tol
+    iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    v, imp = ctx.velocities(), ctx.impulses()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(chebyshev=1, tol_abs=tol, tol_rel=tol))
+        assert st == 0 and iters[s] == r.iters and r.residual <= 1, s
+        vs = v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()]
+        assert rel(vs, v_ref) <= TOL, s
+        if prob.nc:
+            assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL, s
+
+
+def test_batch_refuses_other_accelerations(ctx):
+    probs = make_batch(3, seed=2)
+    upload_batch(ctx, probs)
+    p = native.default_params()
+    for mode in (native.CDX_ACCEL_NESTEROV, native.CDX_ACCEL_ANDERSON):
+        p.accel = mode
+        with pytest.raises(native.CdxError):
+            ctx.batch_pj_solve(1e-3, 1.0, p)
