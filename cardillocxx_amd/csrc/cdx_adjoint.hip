@@ -17,7 +17,9 @@
 //
 // Final stage: u_free_bar = x_bar_0, M^-1_bar from x = ... + M^-1 W^T p and R(M^-1), v_bar from
 // u_free (with the gyroscopic Jacobian) and the restitution term of c'', f_ext_bar = h M^-1 u_bar.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "cdx_pj_device.hpp"
 
@@ -631,6 +633,63 @@ __global__ void k_gm_sub(int n, const double* __restrict__ g, const double* __re
     if (r < n) out[r] = g[r] - w[r];
 }
 
+// ---- scene-segmented GMRES: the scenes of a batch are independent (B is block diagonal), and each
+// gets its own Krylov space, coefficients and convergence test, so every scene's adjoint equals the
+// one GMRES finds for that scene alone.  Scene s owns the contacts [coff[s], coff[s+1]); its
+// vector entries are rows k * nc + c of those contacts.
+
+// out[s * ostride + i] = (out +) <w, V_i> over scene s's entries, i < nvec; one workgroup per scene
+__global__ void __launch_bounds__(kBlock) k_gms_dots(int ns, const int* __restrict__ coff, int nc, int nvec,
+                                                     const double* __restrict__ V, size_t ld, const double* __restrict__ w,
+                                                     double* __restrict__ out, size_t ostride) {
+    __shared__ double red[kBlock / 64];
+    const int s = blockIdx.x;
+    if (s >= ns) return;
+    const int c0 = coff[s], c1 = coff[s + 1];
+    for (int i = 0; i < nvec; ++i) {
+        double d = 0;
+        for (int c = c0 + (int)threadIdx.x; c < c1; c += kBlock)
+            for (int k = 0; k < 3; ++k) {
+                const size_t r = (size_t)k * nc + c;
+                d += w[r] * V[i * ld + r];
+            }
+        d = block_sum(d, red);
+        if (threadIdx.x == 0) out[(size_t)s * ostride + i] = d;
+    }
+}
+
+// out = w / sqrt(nrm2[scene]) per scene (zero after a breakdown); beta[s * bstride] = sqrt(nrm2[s])
+__global__ void k_gms_normalize(int nc, const int* __restrict__ cscene, const double* __restrict__ w,
+                                const double* __restrict__ nrm2, size_t nstride, double* __restrict__ beta, size_t bstride,
+                                double* __restrict__ out, int ns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < ns) beta[(size_t)r * bstride] = sqrt(fmax(nrm2[(size_t)r * nstride], 0.0));
+    if (r >= 3 * nc) return;
+    const int c = r % nc;
+    const double b = sqrt(fmax(nrm2[(size_t)cscene[c] * nstride], 0.0));
+    out[r] = b > 1e-300 ? w[r] / b : 0.0;
+}
+
+// w += sign * sum_i h[scene * hstride + i] V_i
+__global__ void k_gms_axpy(int nc, const int* __restrict__ cscene, int nvec, const double* __restrict__ V, size_t ld,
+                           const double* __restrict__ h, size_t hstride, double sign, double* __restrict__ w) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * nc) return;
+    const double* hs = h + (size_t)cscene[r % nc] * hstride;
+    double acc = 0;
+    for (int i = 0; i < nvec; ++i) acc += hs[i] * V[i * ld + r];
+    w[r] += sign * acc;
+}
+
+// out[s * stride + i] += a[s * stride + i], i < n
+__global__ void k_gms_add(int ns, int n, const double* __restrict__ a, size_t astride, double* __restrict__ out,
+                          size_t ostride) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns * n) return;
+    const int s = t / n, i = t % n;
+    out[(size_t)s * ostride + i] += a[(size_t)s * astride + i];
+}
+
 // Jointed bodies after the chain solve r = S^-1 [x_bar; 0] (velocity part rv): dL/dv_n += M r_v
 // (r_v = M v + h f_ext + h f_gyro(v) + ...), the gyroscopic Jacobian, and the (1 - theta) G v term
 // of the joint rows' right-hand side; dL/df_ext = h r_v.
@@ -709,6 +768,351 @@ static int adj_alloc(cdx_ctx* ctx) {
     return CDX_OK;
 }
 
+// Restarted GMRES(kGmRestart) on B lambda = g_p, B = I - (I - A R) P^T the transposed linearised
+// sweep (lambda_0 = g_p from the setup in S): classical Gram-Schmidt with one reorthogonalisation
+// on the device, the small least-squares problem on the host at each restart.  With joint rows the
+// body pass hands the jointed bodies' share to the chain solve (jl: its row scratch).  Stops at a
+// relative residual <= aprm->tol, after aprm->max_iter applications, or after three restarts in a
+// row without a 1 % gain (a (nearly) inconsistent system: lambda is then the least-squares
+// solution of the last cycle).  t, I0: body and side-impulse scratch of one application.
+static int adjoint_gmres(cdx_ctx* ctx, const cdx_adjoint_params* aprm, bool joints, double* jl, double* S, double* I0,
+                         double* t, int* applies_out, double* rel_out, bool* converged_out) {
+    const int nb = ctx->nb, nc = ctx->nc;
+    const int gc = grid_for(nc), gb = grid_for(nb);
+    // restarted GMRES on B lambda = g_p (lambda_0 = g_p from the setup), classical Gram-Schmidt
+    // with one reorthogonalisation; the small least-squares problem is solved on the host
+    const int n = 3 * nc, m = kGmRestart;
+    const int gn = grid_for(n);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_gm, (size_t)(m + 3) * n + (size_t)(m + 1) * (m + 1) + 8, &ctx->gm_cap,
+                         "adjoint GMRES vectors"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_gm_part, (size_t)(m + 1) * gn, &ctx->gm_part_cap, "adjoint GMRES partials"));
+    double* V = ctx->d_gm;                        // m + 1 basis vectors
+    double* Wk = V + (size_t)(m + 1) * n;         // work vector
+    double* Xi = Wk + n;                          // P^T v of the last application
+    double* H = Xi + n;                           // (m + 1) x m, column-major, then scalars
+    double* sc = H + (size_t)(m + 1) * m;         // [0] |.|^2, [1] beta, [2..] second-pass dots
+    double* lamv = S + (size_t)A_LAM * nc;
+    const double* gvec = S + (size_t)A_GP * nc;
+    double* part = ctx->d_gm_part;
+    auto apply = [&](const double* v, double* w) -> int {
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, v, Xi,
+                   I0);
+        CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   I0, t, nullptr, joints ? ctx->d_jslot : nullptr, joints ? ctx->d_jtr : nullptr);
+        if (joints) CDX_TRY(joints_apply(ctx, t, jl, nullptr));
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_out, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, t, v, Xi, w);
+        return hip_check(ctx, hipGetLastError(), "adjoint GMRES apply");
+    };
+    auto dots = [&](int nvec, const double* Vb, const double* w, double* out, int accumulate) -> int {
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_dots, gn, kBlock, n, nvec, Vb, (size_t)n, w, part);
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_reduce, 1, kBlock, nvec, gn, part, out, accumulate);
+        return hip_check(ctx, hipGetLastError(), "adjoint GMRES dots");
+    };
+    auto host_scalar = [&](const double* d, double* h) -> int {
+        CDX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return CDX_OK;
+    };
+    // the restart cycle's graph (one per call: it holds this call's pointers)
+    const bool use_graph = ctx->graphs && !ctx->profiling;
+    hipGraphExec_t cycle = nullptr;
+    struct CycleGuard {
+        hipGraphExec_t* e;
+        ~CycleGuard() {
+            if (*e) hipGraphExecDestroy(*e);
+        }
+    } cycle_guard{&cycle};
+    double gnorm2 = 0;
+    CDX_TRY(dots(1, gvec, gvec, sc, 0));
+    CDX_TRY(host_scalar(sc, &gnorm2));
+    const double gnorm = std::sqrt(gnorm2);
+    int applies = 0, stalls = 0;
+    bool converged = gnorm == 0.0, stalled = false;
+    double rel = 0.0, rel_prev = INFINITY;
+    std::vector<double> Hh((size_t)(m + 1) * m), y(m);
+    while (!converged && !stalled && applies < aprm->max_iter) {
+        CDX_TRY(apply(lamv, Wk));
+        ++applies;
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_sub, gn, kBlock, n, gvec, Wk, Wk);
+        CDX_TRY(dots(1, Wk, Wk, sc, 0));
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, sc + 1, V);
+        double beta = 0;
+        CDX_TRY(host_scalar(sc + 1, &beta));
+        rel = beta / gnorm;
+        if (rel <= aprm->tol) {
+            converged = true;
+            break;
+        }
+        // three restarts in a row without a 1% gain: the system is (nearly) inconsistent -- contacts
+        // on a stick/slip or contact/separation boundary, where the projection is not differentiable
+        stalls = rel > 0.99 * rel_prev ? stalls + 1 : 0;
+        rel_prev = rel;
+        if (stalls >= 3) {
+            stalled = true;
+            break;
+        }
+        CDX_HIP(ctx, hipMemsetAsync(H, 0, sizeof(double) * (size_t)(m + 1) * m, ctx->stream));
+        // Arnoldi step k: w = B v_k, two Gram-Schmidt passes against v_0..v_k, v_{k+1} = w / |w|
+        auto arnoldi = [&](int k) -> int {
+            double* vk = V + (size_t)k * n;
+            CDX_TRY(apply(vk, Wk));
+            double* hk = H + (size_t)k * (m + 1);
+            CDX_TRY(dots(k + 1, V, Wk, hk, 0));
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, hk, -1.0, Wk);
+            CDX_TRY(dots(k + 1, V, Wk, sc + 2, 0));
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, sc + 2, -1.0, Wk);
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_add, 1, kBlock, k + 1, sc + 2, hk);
+            CDX_TRY(dots(1, Wk, Wk, sc, 0));
+            CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, hk + k + 1, V + (size_t)(k + 1) * n);
+            return hip_check(ctx, hipGetLastError(), "adjoint GMRES Arnoldi step");
+        };
+        int k = 0;
+        if (use_graph && aprm->max_iter - applies >= m) {
+            // a full cycle is the same launches every restart: captured once, replayed
+            if (!cycle) {
+                CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int cst = CDX_OK;
+                for (int kk = 0; kk < m && cst == CDX_OK; ++kk) cst = arnoldi(kk);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+                if (cst) {
+                    if (g) hipGraphDestroy(g);
+                    return cst;
+                }
+                CDX_HIP(ctx, ce);
+                const hipError_t ie = hipGraphInstantiate(&cycle, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                CDX_HIP(ctx, ie);
+            }
+            CDX_HIP(ctx, hipGraphLaunch(cycle, ctx->stream));
+            applies += m;
+            k = m;
+        } else {
+            for (; k < m && applies < aprm->max_iter; ++k) {
+                CDX_TRY(arnoldi(k));
+                ++applies;
+            }
+        }
+        // least squares min |beta e1 - H y| by Givens rotations
+        CDX_HIP(ctx, hipMemcpyAsync(Hh.data(), H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<double> gq(k + 1, 0.0), cs(k), sn(k);
+        gq[0] = beta;
+        for (int c = 0; c < k; ++c) {
+            double* col = &Hh[(size_t)c * (m + 1)];
+            for (int i = 0; i < c; ++i) {
+                const double a0 = col[i], a1 = col[i + 1];
+                col[i] = cs[i] * a0 + sn[i] * a1;
+                col[i + 1] = -sn[i] * a0 + cs[i] * a1;
+            }
+            const double r = std::hypot(col[c], col[c + 1]);
+            cs[c] = r > 0 ? col[c] / r : 1.0;
+            sn[c] = r > 0 ? col[c + 1] / r : 0.0;
+            col[c] = r;
+            col[c + 1] = 0.0;
+            const double g0 = gq[c];
+            gq[c] = cs[c] * g0;
+            gq[c + 1] = -sn[c] * g0;
+        }
+        for (int i = k - 1; i >= 0; --i) {
+            double s = gq[i];
+            for (int l = i + 1; l < k; ++l) s -= Hh[(size_t)l * (m + 1) + i] * y[l];
+            const double d = Hh[(size_t)i * (m + 1) + i];
+            y[i] = d != 0.0 ? s / d : 0.0;
+        }
+        CDX_HIP(ctx, hipMemcpyAsync(sc + 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k, V, (size_t)n, sc + 2, 1.0, lamv);
+        rel = std::fabs(gq[k]) / gnorm;
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *applies_out = applies;
+    *rel_out = rel;
+    *converged_out = converged;
+    return CDX_OK;
+}
+
+// adjoint_gmres with one Krylov space per scene of the context's batch (cdx_set_scenes): the
+// applications of B are shared, the dot products, coefficients, least-squares problems and
+// convergence / stall tests are per scene.  A scene that has converged or stalled keeps its lambda.
+// *rel_out is the worst scene's relative residual.
+static int adjoint_gmres_scenes(cdx_ctx* ctx, const cdx_adjoint_params* aprm, double* S, double* I0, double* t,
+                                int* applies_out, double* rel_out, bool* converged_out) {
+    const int nb = ctx->nb, nc = ctx->nc, ns = ctx->nscenes;
+    const int gc = grid_for(nc), gb = grid_for(nb);
+    const int n = 3 * nc, m = kGmRestart;
+    const int gn = grid_for(n);
+    const size_t hs = (size_t)(m + 1) * m;  // one scene's Hessenberg, column-major
+    const size_t ss = (size_t)(m + 4);      // one scene's scalars: [0] |.|^2, [1] beta, [2..] second-pass dots
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_gm, (size_t)(m + 3) * n + hs * ns + ss * ns + (size_t)(m + 1) * ns + (size_t)nc + 8,
+                         &ctx->gm_cap, "adjoint GMRES vectors"));
+    double* V = ctx->d_gm;
+    double* Wk = V + (size_t)(m + 1) * n;
+    double* Xi = Wk + n;
+    double* H = Xi + n;               // ns Hessenbergs
+    double* sc = H + hs * ns;         // ns scalar blocks
+    double* Y = sc + ss * ns;         // ns coefficient vectors
+    int* cscene = reinterpret_cast<int*>(Y + (size_t)(m + 1) * ns);  // scene of each contact
+    {
+        std::vector<int> h(nc);
+        for (int q = 0; q < ns; ++q)
+            for (int c = ctx->scene_coff[q]; c < ctx->scene_coff[q + 1]; ++c) h[c] = q;
+        if (nc) CDX_HIP(ctx, hipMemcpyAsync(cscene, h.data(), sizeof(int) * nc, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int* coff = ctx->d_scene_i + ns + ns + 1;  // scene table: [order ns | boff ns+1 | coff ns+1 | ...]
+    double* lamv = S + (size_t)A_LAM * nc;
+    const double* gvec = S + (size_t)A_GP * nc;
+    auto apply = [&](const double* v, double* w) -> int {
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, v, Xi,
+                   I0);
+        CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   I0, t, nullptr, nullptr, nullptr);
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_out, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, t, v, Xi, w);
+        return hip_check(ctx, hipGetLastError(), "adjoint GMRES apply");
+    };
+    auto sdots = [&](int nvec, const double* Vb, const double* w, double* out, size_t ostride) -> int {
+        CDX_LAUNCH(ctx, "adj_residual", k_gms_dots, ns, kBlock, ns, coff, nc, nvec, Vb, (size_t)n, w, out, ostride);
+        return hip_check(ctx, hipGetLastError(), "adjoint GMRES scene dots");
+    };
+    const int gnorm_grid = grid_for(std::max(n, ns));
+    std::vector<double> gn2(ns), hsc((size_t)ns * ss);
+    CDX_TRY(sdots(1, gvec, gvec, sc, ss));
+    CDX_HIP(ctx, hipMemcpyAsync(hsc.data(), sc, sizeof(double) * hsc.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> gnorm(ns), rel(ns, 0.0), rel_prev(ns, INFINITY);
+    std::vector<int> stalls(ns, 0);
+    std::vector<char> done(ns, 0), conv(ns, 0);
+    for (int q = 0; q < ns; ++q) {
+        gnorm[q] = std::sqrt(hsc[(size_t)q * ss]);
+        if (!(gnorm[q] > 0)) done[q] = conv[q] = 1;
+    }
+    const bool use_graph = ctx->graphs && !ctx->profiling;
+    hipGraphExec_t cycle = nullptr;
+    struct CycleGuard {
+        hipGraphExec_t* e;
+        ~CycleGuard() {
+            if (*e) hipGraphExecDestroy(*e);
+        }
+    } cycle_guard{&cycle};
+    int applies = 0;
+    std::vector<double> Hh(hs * ns), ys((size_t)(m + 1) * ns);
+    auto all_done = [&]() {
+        for (char d : done)
+            if (!d) return false;
+        return true;
+    };
+    while (!all_done() && applies < aprm->max_iter) {
+        CDX_TRY(apply(lamv, Wk));
+        ++applies;
+        CDX_LAUNCH(ctx, "adj_residual", k_gm_sub, gn, kBlock, n, gvec, Wk, Wk);
+        CDX_TRY(sdots(1, Wk, Wk, sc, ss));
+        CDX_LAUNCH(ctx, "adj_residual", k_gms_normalize, gnorm_grid, kBlock, nc, cscene, Wk, sc, ss, sc + 1, ss, V, ns);
+        CDX_HIP(ctx, hipMemcpyAsync(hsc.data(), sc, sizeof(double) * hsc.size(), hipMemcpyDeviceToHost, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int q = 0; q < ns; ++q) {
+            if (done[q]) continue;
+            rel[q] = hsc[(size_t)q * ss + 1] / gnorm[q];
+            if (rel[q] <= aprm->tol) {
+                done[q] = conv[q] = 1;
+                continue;
+            }
+            // three restarts in a row without a 1 % gain: (nearly) inconsistent -- keep the
+            // least-squares lambda of the last cycle
+            stalls[q] = rel[q] > 0.99 * rel_prev[q] ? stalls[q] + 1 : 0;
+            rel_prev[q] = rel[q];
+            if (stalls[q] >= 3) done[q] = 1;
+        }
+        if (all_done()) break;
+        CDX_HIP(ctx, hipMemsetAsync(H, 0, sizeof(double) * hs * ns, ctx->stream));
+        auto arnoldi = [&](int k) -> int {
+            double* vk = V + (size_t)k * n;
+            CDX_TRY(apply(vk, Wk));
+            double* hk = H + (size_t)k * (m + 1);  // scene q's column k at hk + q * hs
+            CDX_TRY(sdots(k + 1, V, Wk, hk, hs));
+            CDX_LAUNCH(ctx, "adj_residual", k_gms_axpy, gn, kBlock, nc, cscene, k + 1, V, (size_t)n, hk, hs, -1.0, Wk);
+            CDX_TRY(sdots(k + 1, V, Wk, sc + 2, ss));
+            CDX_LAUNCH(ctx, "adj_residual", k_gms_axpy, gn, kBlock, nc, cscene, k + 1, V, (size_t)n, sc + 2, ss, -1.0, Wk);
+            CDX_LAUNCH(ctx, "adj_residual", k_gms_add, grid_for(ns * (k + 1)), kBlock, ns, k + 1, sc + 2, ss, hk, hs);
+            CDX_TRY(sdots(1, Wk, Wk, sc, ss));
+            CDX_LAUNCH(ctx, "adj_residual", k_gms_normalize, gnorm_grid, kBlock, nc, cscene, Wk, sc, ss, hk + k + 1, hs,
+                       V + (size_t)(k + 1) * n, ns);
+            return hip_check(ctx, hipGetLastError(), "adjoint GMRES Arnoldi step");
+        };
+        int k = 0;
+        if (use_graph && aprm->max_iter - applies >= m) {
+            if (!cycle) {
+                CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int cst = CDX_OK;
+                for (int kk = 0; kk < m && cst == CDX_OK; ++kk) cst = arnoldi(kk);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+                if (cst) {
+                    if (g) hipGraphDestroy(g);
+                    return cst;
+                }
+                CDX_HIP(ctx, ce);
+                const hipError_t ie = hipGraphInstantiate(&cycle, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                CDX_HIP(ctx, ie);
+            }
+            CDX_HIP(ctx, hipGraphLaunch(cycle, ctx->stream));
+            applies += m;
+            k = m;
+        } else {
+            for (; k < m && applies < aprm->max_iter; ++k) {
+                CDX_TRY(arnoldi(k));
+                ++applies;
+            }
+        }
+        // per scene: least squares min |beta e1 - H y| by Givens rotations (zero update once done)
+        CDX_HIP(ctx, hipMemcpyAsync(Hh.data(), H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::fill(ys.begin(), ys.end(), 0.0);
+        for (int q = 0; q < ns; ++q) {
+            if (done[q]) continue;
+            double* Hq = &Hh[(size_t)q * hs];
+            double* y = &ys[(size_t)q * (m + 1)];
+            std::vector<double> gq(k + 1, 0.0), cs(k), sn(k);
+            gq[0] = hsc[(size_t)q * ss + 1];
+            for (int c = 0; c < k; ++c) {
+                double* col = Hq + (size_t)c * (m + 1);
+                for (int i = 0; i < c; ++i) {
+                    const double a0 = col[i], a1 = col[i + 1];
+                    col[i] = cs[i] * a0 + sn[i] * a1;
+                    col[i + 1] = -sn[i] * a0 + cs[i] * a1;
+                }
+                const double r = std::hypot(col[c], col[c + 1]);
+                cs[c] = r > 0 ? col[c] / r : 1.0;
+                sn[c] = r > 0 ? col[c + 1] / r : 0.0;
+                col[c] = r;
+                col[c + 1] = 0.0;
+                const double g0 = gq[c];
+                gq[c] = cs[c] * g0;
+                gq[c + 1] = -sn[c] * g0;
+            }
+            for (int i = k - 1; i >= 0; --i) {
+                double sum = gq[i];
+                for (int l = i + 1; l < k; ++l) sum -= Hq[(size_t)l * (m + 1) + i] * y[l];
+                const double d = Hq[(size_t)i * (m + 1) + i];
+                y[i] = d != 0.0 ? sum / d : 0.0;
+            }
+            rel[q] = std::fabs(gq[k]) / gnorm[q];
+        }
+        CDX_HIP(ctx, hipMemcpyAsync(Y, ys.data(), sizeof(double) * ys.size(), hipMemcpyHostToDevice, ctx->stream));
+        CDX_LAUNCH(ctx, "adj_residual", k_gms_axpy, gn, kBlock, nc, cscene, k, V, (size_t)n, Y, (size_t)(m + 1), 1.0, lamv);
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    double worst = 0;
+    bool all_conv = true;
+    for (int q = 0; q < ns; ++q) {
+        worst = std::max(worst, rel[q]);
+        all_conv = all_conv && conv[q];
+    }
+    *applies_out = applies;
+    *rel_out = worst;
+    *converged_out = all_conv;
+    return CDX_OK;
+}
+
 static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const cdx_adjoint_params* aprm,
                        const double* g_v, const double* g_p, double* gv_out, double* gminv_out, double* gfext_out,
                        double* gmu_out, double** dlp_out, double** dgmu_out, int* iters_out, double* res_out, int* mode_io) {
@@ -744,6 +1148,20 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
     const int gc = grid_for(nc), gb = grid_for(nb);
     const bool rows = nc > 0 && ctx->nrows > 0;
     int& mode = *mode_io;
+    if (rows && mode == 2) {
+        // the same linear system by restarted GMRES (the least-squares adjoint where it is
+        // inconsistent: impulses that are not unique, contacts on a stick/slip boundary)
+        CDX_LAUNCH(ctx, "adj_setup", k_adj_setup, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias,
+                   ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, ctx->d_x, ctx->d_p, lx, dlp, S, I0, 0);
+        bool converged = false;
+        if (ctx->nscenes >= 1 && !ctx->scene_coff.empty() && ctx->scene_coff.back() == nc)
+            CDX_TRY(adjoint_gmres_scenes(ctx, aprm, S, I0, t, iters_out, res_out, &converged));
+        else
+            CDX_TRY(adjoint_gmres(ctx, aprm, false, nullptr, S, I0, t, iters_out, res_out, &converged));
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S,
+                   S + (size_t)A_LAM * nc, S + (size_t)A_XI * nc, I0);
+        CDX_HIP(ctx, hipGetLastError());
+    }
     if (rows && mode == 0) {
         CDX_TRY(reset_control(ctx));
         CDX_LAUNCH(ctx, "adj_setup", k_adj_setup, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias,
@@ -767,11 +1185,8 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
         CDX_HIP(ctx, hipMemcpy(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
         CDX_TRY(read_residual(ctx, res_out));
         *iters_out = it_done;
-        if (!ctrl[0]) {  // not converged within max_iter
-            if (aprm->mode != CDX_ADJ_AUTO)
-                return set_err(ctx, CDX_DIVERGED, "cdx_pj_adjoint: implicit adjoint iteration did not converge");
-            mode = 1;
-        }
+        if (!ctrl[0])  // not converged within max_iter
+            return set_err(ctx, CDX_DIVERGED, "cdx_pj_adjoint: implicit adjoint iteration did not converge");
     }
     if (rows && mode == 1) {
         if (!ctx->record_valid)
@@ -800,11 +1215,12 @@ static int adjoint_run(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, co
         *iters_out = K;
         *res_out = 0.0;
     }
-    // final stage (shared by both modes)
-    const double* pv = mode == 0 ? ctx->d_p : ctx->d_rec_p;  // p* (implicit) or p_0 (unrolled)
-    const double* base = mode == 0 ? lx : xbar;
+    // final stage (shared by the modes; GMRES leaves the implicit state)
+    const bool implicit = mode != 1;
+    const double* pv = implicit ? ctx->d_p : ctx->d_rec_p;  // p* (implicit) or p_0 (unrolled)
+    const double* base = implicit ? lx : xbar;
     if (rows) {
-        CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, mode == 0 ? 1 : 0, prm->alpha, ctx->d_ida, ctx->d_idb,
+        CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, implicit ? 1 : 0, prm->alpha, ctx->d_ida, ctx->d_idb,
                    ctx->d_J, ctx->d_R, ctx->d_mu, ctx->d_rest, pv, S, I0, I1, I2, I3, dgmu);
     } else {
         CDX_HIP(ctx, hipMemsetAsync(I0, 0, sizeof(double) * 48 * (size_t)(nc ? nc : 1), ctx->stream));
@@ -881,152 +1297,10 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
         CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx, I0, t,
                    nullptr, ctx->d_jslot, ctx->d_jtr);
         CDX_TRY(joints_apply(ctx, t, jl, nullptr));
-        // restarted GMRES on B lambda = g_p (lambda_0 = g_p from the setup), classical Gram-Schmidt
-        // with one reorthogonalisation; the small least-squares problem is solved on the host
-        const int n = 3 * nc, m = kGmRestart;
-        const int gn = grid_for(n);
-        CDX_TRY(ensure_alloc(ctx, &ctx->d_gm, (size_t)(m + 3) * n + (size_t)(m + 1) * (m + 1) + 8, &ctx->gm_cap,
-                             "adjoint GMRES vectors"));
-        CDX_TRY(ensure_alloc(ctx, &ctx->d_gm_part, (size_t)(m + 1) * gn, &ctx->gm_part_cap, "adjoint GMRES partials"));
-        double* V = ctx->d_gm;                        // m + 1 basis vectors
-        double* Wk = V + (size_t)(m + 1) * n;         // work vector
-        double* Xi = Wk + n;                          // P^T v of the last application
-        double* H = Xi + n;                           // (m + 1) x m, column-major, then scalars
-        double* sc = H + (size_t)(m + 1) * m;         // [0] |.|^2, [1] beta, [2..] second-pass dots
-        double* lamv = S + (size_t)A_LAM * nc;
-        const double* gvec = S + (size_t)A_GP * nc;
-        double* part = ctx->d_gm_part;
-        auto apply = [&](const double* v, double* w) -> int {
-            CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, v, Xi,
-                       I0);
-            CDX_LAUNCH(ctx, "adj_body", k_adj_body_t, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                       I0, t, nullptr, ctx->d_jslot, ctx->d_jtr);
-            CDX_TRY(joints_apply(ctx, t, jl, nullptr));
-            CDX_LAUNCH(ctx, "adj_iter", k_gm_out, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, t, v, Xi, w);
-            return hip_check(ctx, hipGetLastError(), "adjoint GMRES apply");
-        };
-        auto dots = [&](int nvec, const double* Vb, const double* w, double* out, int accumulate) -> int {
-            CDX_LAUNCH(ctx, "adj_residual", k_gm_dots, gn, kBlock, n, nvec, Vb, (size_t)n, w, part);
-            CDX_LAUNCH(ctx, "adj_residual", k_gm_reduce, 1, kBlock, nvec, gn, part, out, accumulate);
-            return hip_check(ctx, hipGetLastError(), "adjoint GMRES dots");
-        };
-        auto host_scalar = [&](const double* d, double* h) -> int {
-            CDX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            return CDX_OK;
-        };
-        // the restart cycle's graph (one per call: it holds this call's pointers)
-        const bool use_graph = ctx->graphs && !ctx->profiling;
-        hipGraphExec_t cycle = nullptr;
-        struct CycleGuard {
-            hipGraphExec_t* e;
-            ~CycleGuard() {
-                if (*e) hipGraphExecDestroy(*e);
-            }
-        } cycle_guard{&cycle};
-        double gnorm2 = 0;
-        CDX_TRY(dots(1, gvec, gvec, sc, 0));
-        CDX_TRY(host_scalar(sc, &gnorm2));
-        const double gnorm = std::sqrt(gnorm2);
-        int applies = 0, stalls = 0;
-        bool converged = gnorm == 0.0, stalled = false;
-        double rel = 0.0, rel_prev = INFINITY;
-        std::vector<double> Hh((size_t)(m + 1) * m), y(m);
-        while (!converged && !stalled && applies < aprm->max_iter) {
-            CDX_TRY(apply(lamv, Wk));
-            ++applies;
-            CDX_LAUNCH(ctx, "adj_residual", k_gm_sub, gn, kBlock, n, gvec, Wk, Wk);
-            CDX_TRY(dots(1, Wk, Wk, sc, 0));
-            CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, sc + 1, V);
-            double beta = 0;
-            CDX_TRY(host_scalar(sc + 1, &beta));
-            rel = beta / gnorm;
-            if (rel <= aprm->tol) {
-                converged = true;
-                break;
-            }
-            // three restarts in a row without a 1% gain: the system is (nearly) inconsistent -- contacts
-            // on a stick/slip or contact/separation boundary, where the projection is not differentiable
-            stalls = rel > 0.99 * rel_prev ? stalls + 1 : 0;
-            rel_prev = rel;
-            if (stalls >= 3) {
-                stalled = true;
-                break;
-            }
-            CDX_HIP(ctx, hipMemsetAsync(H, 0, sizeof(double) * (size_t)(m + 1) * m, ctx->stream));
-            // Arnoldi step k: w = B v_k, two Gram-Schmidt passes against v_0..v_k, v_{k+1} = w / |w|
-            auto arnoldi = [&](int k) -> int {
-                double* vk = V + (size_t)k * n;
-                CDX_TRY(apply(vk, Wk));
-                double* hk = H + (size_t)k * (m + 1);
-                CDX_TRY(dots(k + 1, V, Wk, hk, 0));
-                CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, hk, -1.0, Wk);
-                CDX_TRY(dots(k + 1, V, Wk, sc + 2, 0));
-                CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k + 1, V, (size_t)n, sc + 2, -1.0, Wk);
-                CDX_LAUNCH(ctx, "adj_residual", k_gm_add, 1, kBlock, k + 1, sc + 2, hk);
-                CDX_TRY(dots(1, Wk, Wk, sc, 0));
-                CDX_LAUNCH(ctx, "adj_residual", k_gm_normalize, gn, kBlock, n, Wk, sc, hk + k + 1, V + (size_t)(k + 1) * n);
-                return hip_check(ctx, hipGetLastError(), "adjoint GMRES Arnoldi step");
-            };
-            int k = 0;
-            if (use_graph && aprm->max_iter - applies >= m) {
-                // a full cycle is the same launches every restart: captured once, replayed
-                if (!cycle) {
-                    CDX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                    int cst = CDX_OK;
-                    for (int kk = 0; kk < m && cst == CDX_OK; ++kk) cst = arnoldi(kk);
-                    hipGraph_t g = nullptr;
-                    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-                    if (cst) {
-                        if (g) hipGraphDestroy(g);
-                        return cst;
-                    }
-                    CDX_HIP(ctx, ce);
-                    const hipError_t ie = hipGraphInstantiate(&cycle, g, nullptr, nullptr, 0);
-                    hipGraphDestroy(g);
-                    CDX_HIP(ctx, ie);
-                }
-                CDX_HIP(ctx, hipGraphLaunch(cycle, ctx->stream));
-                applies += m;
-                k = m;
-            } else {
-                for (; k < m && applies < aprm->max_iter; ++k) {
-                    CDX_TRY(arnoldi(k));
-                    ++applies;
-                }
-            }
-            // least squares min |beta e1 - H y| by Givens rotations
-            CDX_HIP(ctx, hipMemcpyAsync(Hh.data(), H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, ctx->stream));
-            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            std::vector<double> gq(k + 1, 0.0), cs(k), sn(k);
-            gq[0] = beta;
-            for (int c = 0; c < k; ++c) {
-                double* col = &Hh[(size_t)c * (m + 1)];
-                for (int i = 0; i < c; ++i) {
-                    const double a0 = col[i], a1 = col[i + 1];
-                    col[i] = cs[i] * a0 + sn[i] * a1;
-                    col[i + 1] = -sn[i] * a0 + cs[i] * a1;
-                }
-                const double r = std::hypot(col[c], col[c + 1]);
-                cs[c] = r > 0 ? col[c] / r : 1.0;
-                sn[c] = r > 0 ? col[c + 1] / r : 0.0;
-                col[c] = r;
-                col[c + 1] = 0.0;
-                const double g0 = gq[c];
-                gq[c] = cs[c] * g0;
-                gq[c + 1] = -sn[c] * g0;
-            }
-            for (int i = k - 1; i >= 0; --i) {
-                double s = gq[i];
-                for (int l = i + 1; l < k; ++l) s -= Hh[(size_t)l * (m + 1) + i] * y[l];
-                const double d = Hh[(size_t)i * (m + 1) + i];
-                y[i] = d != 0.0 ? s / d : 0.0;
-            }
-            CDX_HIP(ctx, hipMemcpyAsync(sc + 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
-            CDX_LAUNCH(ctx, "adj_residual", k_gm_axpy, gn, kBlock, n, k, V, (size_t)n, sc + 2, 1.0, lamv);
-            rel = std::fabs(gq[k]) / gnorm;
-            CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        }
+        int applies = 0;
+        double rel = 0.0;
+        bool converged = false;
+        CDX_TRY(adjoint_gmres(ctx, aprm, true, jl, S, I0, t, &applies, &rel, &converged));
         *iters_out = applies;
         *res_out = rel;
         if (!converged && aprm->mode != CDX_ADJ_AUTO)
@@ -1034,8 +1308,8 @@ static int adjoint_joints_run(cdx_ctx* ctx, double dt, const cdx_solver_params* 
                            "cdx_pj_adjoint_joints: adjoint GMRES did not converge (relative residual " + std::to_string(rel) +
                                " after " + std::to_string(applies) + " applications)");
         // xi = P^T lambda for the final stage
-        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S, lamv,
-                   S + (size_t)A_XI * nc, I0);
+        CDX_LAUNCH(ctx, "adj_iter", k_gm_pt, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_mu, S,
+                   S + (size_t)A_LAM * nc, S + (size_t)A_XI * nc, I0);
         CDX_LAUNCH(ctx, "adj_final", k_adj_final_contacts, gc, kBlock, nc, 1, prm->alpha, ctx->d_ida, ctx->d_idb, ctx->d_J,
                    ctx->d_R, ctx->d_mu, ctx->d_rest, ctx->d_p, S, I0, I1, I2, I3, dgmu);
     } else {
@@ -1099,7 +1373,7 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
     double* dgmu = nullptr;
     int iters = 0;
     double res = std::nan("");
-    int mode = aprm->mode == CDX_ADJ_UNROLLED ? 1 : 0;
+    int mode = aprm->mode == CDX_ADJ_UNROLLED ? 1 : (aprm->mode == CDX_ADJ_AUTO ? 2 : 0);
     const int st = adjoint_run(ctx, dt, prm, aprm, g_v, g_p, gv_out, gminv_out, gfext_out, gmu_out, &dlp, &dgmu, &iters,
                                &res, &mode);
     if (dlp) hipFree(dlp);
@@ -1107,7 +1381,7 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
     if (info) {
         info->iters = iters;
         info->residual = res;
-        info->mode_used = mode == 0 ? CDX_ADJ_IMPLICIT : CDX_ADJ_UNROLLED;
+        info->mode_used = aprm->mode;
     }
     return st;
 }

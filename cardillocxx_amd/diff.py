@@ -54,7 +54,7 @@ class Rollout:
         # batched: every forward solve is one cdx_batch_pj_solve over the batch's scenes, each
         # scene converging on its own residual exactly as a standalone solve of that scene would
         # (the adjoints run on the restored state of each step, so every mode applies)
-        self.batched = batched and batch.nscenes > 1
+        self.batched = batched
         self.frozen = None  # optional per-step contact dicts to reuse (geometry held fixed)
         self.stats = dict(forward_solves=0, sweeps=0, adjoint_iters=0, unrolled_fallbacks=0)
         # store_steps: the forward keeps every step's record (contacts, q, v, impulses), so the
@@ -146,11 +146,12 @@ class Rollout:
 
     def _step_adjoint(self, rec, lv):
         """Adjoint of one step's solve given dL/dv_{n+1} = lv.  Implicit: restore the converged
-        state with one sweep from its impulses and iterate the adjoint.  AUTO: implicit with at
-        most `adjoint_max_iter` iterations, and where that does not converge (a contact set whose
-        impulses are not unique, e.g. a box face on four coplanar points) the unrolled fallback.
-        Unrolled: reverse mode needs every sweep, so the step is re-solved from a cold start to the
-        same tolerance with recording on.  Reverse mode holds the starting impulses constant, so a
+        state with one sweep from its impulses and iterate the adjoint.  AUTO: the same implicit
+        system by restarted GMRES, which also converges where the contact impulses are not unique
+        (a box face on four coplanar points) and returns the least-squares adjoint where the
+        system is inconsistent (its relative residual is tracked in the stats).  Unrolled:
+        reverse mode needs every sweep, so the step is re-solved from a cold start to the same
+        tolerance with recording on.  Reverse mode holds the starting impulses constant, so a
         warm-started solve that converges within a few sweeps would drop how its warm start
         depends on the parameters; from p_0 = 0 the derivative of the sweep map converges to that
         of the fixed point."""
@@ -159,13 +160,16 @@ class Rollout:
             prm = native.default_params()
             prm.max_iter, prm.tol_abs, prm.tol_rel, prm.warmstart = 1, 0.0, 0.0, 1
             self._solve(rec["q"], rec["v"], self.minv, self.fext, rec["fgyro"], rec["con"], rec["mu"], rec["imp"], prm)
-            try:
-                return self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=native.CDX_ADJ_IMPLICIT,
-                                           max_iter=self.adjoint_max_iter, tol=self.adjoint_tol)
-            except native.CdxError as e:
-                if mode == native.CDX_ADJ_IMPLICIT or e.status != native.CDX_DIVERGED:
-                    raise
-                self.stats["unrolled_fallbacks"] += 1
+            if self.batched:  # one Krylov space per scene in the GMRES adjoint
+                self.ctx.set_scenes(self.batch.body_off, rec["con"]["contact_off"])
+            g = self.ctx.pj_adjoint(self.dt, self.prm, lv, None, mode=mode, max_iter=self.adjoint_max_iter,
+                                    tol=self.adjoint_tol)
+            if mode == native.CDX_ADJ_AUTO and len(rec["con"]["body_a"]):
+                r = float(g["info"].residual)
+                self.stats["worst_adjoint_residual"] = max(self.stats.get("worst_adjoint_residual", 0.0), r)
+                if r > self.adjoint_tol:
+                    self.stats["least_squares_adjoints"] = self.stats.get("least_squares_adjoints", 0) + 1
+            return g
         prm = native.default_params()
         prm.tol_abs, prm.tol_rel, prm.max_iter, prm.warmstart = self.prm.tol_abs, self.prm.tol_rel, self.prm.max_iter, 0
         per_sweep = 8 * (8 * len(self.mass) + 3 * len(rec["con"]["body_a"]))

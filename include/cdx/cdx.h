@@ -197,8 +197,13 @@ int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
  *                      same contact/body passes as the forward sweep;
  *   CDX_ADJ_UNROLLED : reverse mode through the recorded forward sweeps (cdx_set_recording must
  *                      have been on during the solve);
- *   CDX_ADJ_AUTO     : implicit, falling back to unrolled when the adjoint iteration does not
- *                      converge within max_iter (requires recording). */
+ *   CDX_ADJ_AUTO     : the implicit system solved by restarted GMRES(30) from lambda_0 = g_p instead
+ *                      of the fixed-point iteration: it converges where that iteration drifts (a
+ *                      singular or near-singular system: impulses that are not unique, as on a box
+ *                      face resting on four coplanar contacts), and where the system is
+ *                      inconsistent (contacts on a stick/slip or contact/separation boundary) it
+ *                      returns the least-squares adjoint, with its relative residual in
+ *                      info->residual (GMRES stops after three restarts without a 1 % gain). */
 enum cdx_adjoint_mode { CDX_ADJ_IMPLICIT = 0, CDX_ADJ_UNROLLED = 1, CDX_ADJ_AUTO = 2 };
 
 typedef struct {
@@ -208,8 +213,8 @@ typedef struct {
 } cdx_adjoint_params;
 
 typedef struct {
-    int iters;       /* implicit iterations, or unrolled sweeps */
-    double residual; /* last relative change of the implicit iterate */
+    int iters;       /* implicit iterations, GMRES applications (CDX_ADJ_AUTO), or unrolled sweeps */
+    double residual; /* last relative change of the implicit iterate, or GMRES's relative residual */
     int mode_used;
 } cdx_adjoint_info;
 
