@@ -452,6 +452,7 @@ def batched_leg(args, native, scenes):
     p.max_iter = args.batch_max_iter
     p.warmstart = 1
     p.accel = accel
+    p.block_precond = 1 if args.batch_precond == "block" else 0
     ctx = native.Context(0)
     ctx.set_bodies(bd["dof"], bd["minv"], bd["v"], bd["fext"], None, bd["q"])
     scene_of = np.repeat(np.arange(b.nscenes), np.diff(b.body_off))
@@ -503,7 +504,7 @@ def batched_leg(args, native, scenes):
         ps.tol_abs = ps.tol_rel = 1e-4
         ps.max_iter = 1000
         ps.warmstart = 1
-        ps.accel = native.CDX_ACCEL_CHEBYSHEV
+        ps.block_precond = 1  # converges on the box manifolds where the row-scaled loops stagnate or diverge
         resets = 0
         for s in range(args.batch_settle):
             con, warm = contacts_of(q, prev)
@@ -514,7 +515,7 @@ def batched_leg(args, native, scenes):
             prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         per = np.diff(con["contact_off"])
         settle = {"steps": args.batch_settle, "h": args.batch_settle_dt, "tol": 1e-4, "max_iter": 1000,
-                  "loop": "chebyshev", "scene_resets": resets, "s": time.perf_counter() - ts0,
+                  "loop": "standard, block preconditioner", "scene_resets": resets, "s": time.perf_counter() - ts0,
                   "contacts_after": int(len(con["body_a"])), "max_contacts_per_scene_after": int(per.max())}
         log(f"batched: settled {args.batch_settle} steps of {args.batch_settle_dt:g} s in {settle['s']:.1f}s, "
             f"{settle['contacts_after']} contacts (max {settle['max_contacts_per_scene_after']}/scene)")
@@ -596,9 +597,9 @@ def batched_leg(args, native, scenes):
         "metric": "batched env-steps/s (scenes x Moreau steps per second)",
         "workload": f"{ns} scenes of 8-512 spheres/boxes (50/50, box edges U[0.05,0.2] m) dropped into a unit box"
                     f"{f' and settled ({args.batch_settle} steps of {args.batch_settle_dt:g} s, untimed)' if settle else ''}, "
-                    f"h=1e-3 s, {steps} steps, warm-started PJ ({args.batch_accel} loop) to tol {args.batch_tol:g} "
+                    f"h=1e-3 s, {steps} steps, warm-started PJ ({args.batch_accel} loop, {args.batch_precond} preconditioner) to tol {args.batch_tol:g} "
                     f"per scene (reference default), one batched solve launch per step",
-        "loop": args.batch_accel, "settle": settle,
+        "loop": args.batch_accel, "preconditioner": args.batch_precond, "settle": settle,
         "contacts_per_scene": dist,
         "steps": steps, "scenes": ns, "bodies": nb,
         "value": ns * steps / t_dev, "unit": "env-steps/s",
@@ -939,6 +940,9 @@ def main():
     ap.add_argument("--tune-step", type=int, default=100)
     ap.add_argument("--batch-accel", choices=("none", "chebyshev"), default="none",
                     help="loop of the batched solve (pj.chebyshev); the one-by-one comparison uses the same")
+    ap.add_argument("--batch-precond", choices=("row", "block"), default="block",
+                    help="batched PJ preconditioner: the row scaling alpha / D_ii (reference default) or the "
+                         "per-contact inverse Delassus blocks (pj.rdiag_true_delassus)")
     ap.add_argument("--batch-settle", type=int, default=0, help="untimed settling steps before the batched steps")
     ap.add_argument("--batch-settle-dt", type=float, default=4e-3)
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg and print it")

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
     const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
     int use_bias, int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
-    double* __restrict__ res_out, BatchJoints ja) {
+    double* __restrict__ res_out, double* __restrict__ Rb, BatchJoints ja) {
     __shared__ double wsum[kMaxBatchBlock / 64];
     __shared__ int gdone[kMaxBatchBlock / 64];
     __shared__ int gfail[kMaxBatchBlock / 64];
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     // whose static-static contacts get R = 0 and p = 0, as in the grid-wide setup)
     if (s >= 0)
         for (int c = c0 + lt; c < c1; c += gsize)
-            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp, Rb);
     __syncthreads();
     for (int b = b0 + lt; b < b1; b += gsize)
         pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x, ja.jslot, ja.tr);
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
     for (int it = 0; it < max_iter; ++it) {
         if (!done)
             for (int c = c0 + lt; c < c1; c += gsize)
-                pj_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, x, imp);
+                pj_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, x, imp, Rb);
         __syncthreads();
         const int check = (it % check_every) == 0;
         double e2 = 0;
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj(
 __device__ inline void cheb_lin_contact(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
                                         const double* __restrict__ J, const double* __restrict__ R,
                                         const double* __restrict__ mu, const double* __restrict__ v,
-                                        double* __restrict__ imp) {
+                                        double* __restrict__ imp, const double* __restrict__ Rb) {
     const int a = ida[c], b = idb[c];
     if (a < 0 && b < 0) return;
     const int nr = mu[c] > 0 ? 3 : 1;
@@ -192,7 +192,12 @@ __device__ inline void cheb_lin_contact(int c, int nc, const int* __restrict__ i
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (k >= nr) break;
-        dp[k] = -(R[(size_t)k * nc + c] * row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub));
+        const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
+        dp[k] = Rb ? g : -(R[(size_t)k * nc + c] * g);
+    }
+    if (Rb) {  // -Rb (W v)
+        const double zero[3] = {0, 0, 0};
+        block_apply_rows(c, nc, nr, Rb, zero, dp);
     }
     write_side_impulses9(imp, c, a, b, nr, j, dp);
 }
@@ -213,7 +218,7 @@ __device__ inline void pjc_contact_sweep(int c, int nc, const int* __restrict__ 
                                          const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
                                          double* __restrict__ P, double* __restrict__ PK, double* __restrict__ PK1,
                                          const double* __restrict__ X, const double* __restrict__ XK, double w,
-                                         bool take, double* __restrict__ imp) {
+                                         bool take, double* __restrict__ imp, const double* __restrict__ Rb) {
     const int a = ida[c], b = idb[c];
     if (a < 0 && b < 0) return;
     const double muc = mu[c];
@@ -233,8 +238,9 @@ __device__ inline void pjc_contact_sweep(int c, int nc, const int* __restrict__ 
         PK[r] = PK1[r];
         PK1[r] = pin[k];
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
-        pn[k] = pin[k] - (R[r] * g + (use_bias ? bias[r] : 0.0));
+        pn[k] = Rb ? g + (use_bias ? bias[r] : 0.0) : pin[k] - (R[r] * g + (use_bias ? bias[r] : 0.0));
     }
+    if (Rb) block_apply_rows(c, nc, nr, Rb, pin, pn);
     pn[0] = fmax(pn[0], 0.0);
     if (nr == 3) {
         const double t1 = pn[1], t2 = pn[2];
@@ -265,7 +271,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pjc(
     const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double* __restrict__ XK,
     double* __restrict__ XK1, double* __restrict__ PK, double* __restrict__ PK1, double dt, double alpha, int use_bias,
     int warmstart, int max_iter, double tol_rel, double tol_abs, int* __restrict__ iters_out,
-    double* __restrict__ res_out) {
+    double* __restrict__ res_out, double* __restrict__ Rb) {
     __shared__ double wsum[kMaxBatchBlock / 64];
     __shared__ double wbad[kMaxBatchBlock / 64];
     __shared__ int gdone[kMaxBatchBlock / 64];
@@ -300,7 +306,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pjc(
     };
     if (s >= 0)
         for (int c = c0 + lt; c < c1; c += gsize)
-            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+            pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp, Rb);
     __syncthreads();
     for (int b = b0 + lt; b < b1; b += gsize)
         pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
@@ -315,7 +321,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pjc(
     double lambda = 0;
     for (int pit = 0; pit < 8; ++pit) {
         if (!stopped)
-            for (int c = c0 + lt; c < c1; c += gsize) cheb_lin_contact(c, nc, ida, idb, J, R, mu, XK1, imp);
+            for (int c = c0 + lt; c < c1; c += gsize) cheb_lin_contact(c, nc, ida, idb, J, R, mu, XK1, imp, Rb);
         __syncthreads();
         double s2 = 0;
         if (!stopped)
@@ -367,7 +373,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pjc(
     for (int it = 0; it < max_iter; ++it) {
         if (!done)
             for (int c = c0 + lt; c < c1; c += gsize)
-                pjc_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, PK, PK1, x, XK, w, take, imp);
+                pjc_contact_sweep(c, nc, ida, idb, J, R, bias, use_bias, mu, p, PK, PK1, x, XK, w, take, imp, Rb);
         __syncthreads();
         double e2 = 0, bad = 0;
         if (!done)
@@ -477,7 +483,8 @@ __device__ inline void contact_sweep_lds(int c, int c0, int b0, int nc, const in
                                          const int* __restrict__ idb, const double* __restrict__ J,
                                          const double* __restrict__ R, const double* __restrict__ bias, int use_bias,
                                          const double* __restrict__ mu, double* __restrict__ p,
-                                         const double* __restrict__ xs, double* __restrict__ rs) {
+                                         const double* __restrict__ xs, double* __restrict__ rs,
+                                         const double* __restrict__ Rb) {
     const int a = ida[c], b = idb[c];
     if (a < 0 && b < 0) return;
     const double muc = mu[c];
@@ -493,8 +500,12 @@ __device__ inline void contact_sweep_lds(int c, int c0, int b0, int nc, const in
         if (k >= nr) break;
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
         pold[k] = p[(size_t)k * nc + c];
-        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+        if (Rb)
+            pn[k] = g + (use_bias ? bias[(size_t)k * nc + c] : 0.0);
+        else
+            pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
+    if (Rb) block_apply_rows(c, nc, nr, Rb, pold, pn);
     pn[0] = fmax(pn[0], 0.0);
     if (nr == 3) {
         const double t1 = pn[1], t2 = pn[2];
@@ -564,7 +575,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     const double* __restrict__ v, const double* __restrict__ fext, const double* __restrict__ fgyro,
     const int* __restrict__ incptr, const int* __restrict__ incidx, double* __restrict__ x, double dt, double alpha,
     int use_bias, int warmstart, int max_iter, int check_every, double tol_rel, double tol_abs, int* __restrict__ iters_out,
-    double* __restrict__ res_out) {
+    double* __restrict__ res_out, double* __restrict__ Rb) {
     extern __shared__ double sdyn[];
     __shared__ double wsum[kMaxBatchBlock / 64];
     if ((int)blockIdx.x >= ns) return;  // uniform over the workgroup
@@ -578,7 +589,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     int* ix = ip + lds_bodies + 1;                                     // [2 lds_contacts]
     // setup in global memory, as the grid-wide solve, then x0 into LDS
     for (int c = c0 + lt; c < c1; c += gsize)
-        pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp);
+        pj_setup_contact(c, nc, ida, idb, J, mu, rest, vb, minv, v, alpha, warmstart, R, bias, p, imp, Rb);
     __syncthreads();
     for (int b = b0 + lt; b < b1; b += gsize)
         pj_setup_body(b, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, rows ? 1 : 0, x);
@@ -593,7 +604,7 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     double res = NAN;
     for (int it = 0; it < max_iter && !done; ++it) {
         for (int c = c0 + lt; c < c1; c += gsize)
-            contact_sweep_lds(c, c0, b0, nc, ida, idb, J, R, bias, use_bias, mu, p, xs, rs);
+            contact_sweep_lds(c, c0, b0, nc, ida, idb, J, R, bias, use_bias, mu, p, xs, rs, Rb);
         __syncthreads();
         const int check = (it % check_every) == 0;
         double e2 = 0;
@@ -693,6 +704,11 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
     const int ns = ctx->nscenes;
     const int* d = ctx->d_scene_i;
     const int grid = (ns + groups - 1) / groups;
+    double* Rb = nullptr;  // block mode: the contacts' inverse Delassus blocks
+    if (prm->block_precond && ctx->nc > 0) {
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_Rb, 9 * (size_t)ctx->nc, &ctx->Rb_cap, "inverse Delassus blocks"));
+        Rb = ctx->d_Rb;
+    }
     if (prm->accel == CDX_ACCEL_CHEBYSHEV) {
         const size_t n8 = (size_t)ctx->nb * kBodyStride, n3 = 3 * (size_t)ctx->nc;
         CDX_TRY(ensure_alloc(ctx, &ctx->d_xk1, n8, &ctx->xk1_cap, "chebyshev xk1"));
@@ -703,7 +719,7 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
                    ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,
                    ctx->d_incptr, ctx->d_incidx, ctx->d_x, ctx->d_uprev, ctx->d_xk1, ctx->d_pk, ctx->d_pk1, dt,
                    prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->tol_rel, prm->tol_abs,
-                   ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+                   ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, Rb);
         return hip_check(ctx, hipGetLastError(), "batch launch");
     }
     size_t shm = 0;
@@ -715,7 +731,7 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
         ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv,         \
         ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha,                     \
         ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs,              \
-        ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, batch_joints(ctx, theta, wave_lds)
+        ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, Rb, batch_joints(ctx, theta, wave_lds)
     const size_t lds_bytes =
         sizeof(double) * (6 * (size_t)ctx->scene_max_bodies + 9 * (size_t)ctx->scene_max_contacts) +
         sizeof(int) * ((size_t)ctx->scene_max_bodies + 1 + 2 * (size_t)ctx->scene_max_contacts);
@@ -730,7 +746,7 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
                        ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof,
                        ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt,
                        prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every,
-                       prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res);
+                       prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, Rb);
         return hip_check(ctx, hipGetLastError(), "batch launch");
     }
     if (ctx->njrows > 0) {
@@ -808,8 +824,8 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve runs the standard or the Chebyshev loop");
     if (prm->accel == CDX_ACCEL_CHEBYSHEV && ctx->njrows > 0)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched Chebyshev loop does not take joint rows");
-    if (prm->block_precond)
-        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve uses the row scaling only");
+    if (prm->block_precond && ctx->njrows > 0)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the block preconditioner does not take joint rows");
     if (ctx->nscenes < 1 || ctx->scene_boff.back() != ctx->nb || ctx->scene_coff.back() != ctx->nc)
         return set_err(ctx, CDX_NO_STATE, "cdx_batch_pj_solve: scenes not set for the current bodies/contacts");
     ctx->solved = false;
@@ -830,7 +846,9 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         if (it[s] < 0) return set_err(ctx, CDX_NOT_FACTORIZED, "cdx_batch_pj_solve: S is not factorized (singular joint block)");
     ctx->solved = true;
     ctx->joints_solved = joints;
-    ctx->pj_last = !joints;  // a converged state of the same fixed point: the implicit adjoint applies
+    // a converged state of the same fixed point: the implicit adjoint applies (to the row-scaled
+    // solve, as after cdx_pj_solve)
+    ctx->pj_last = !joints && !prm->block_precond;
     ctx->pjj_last = joints;
     ctx->last_theta = theta;
     ctx->record_valid = false;
@@ -845,7 +863,10 @@ int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (!ctx) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
-    if (prm->block_precond) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve uses the row scaling only");
+    if (prm->block_precond && ctx->njrows > 0)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the block preconditioner does not take joint rows");
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_CHEBYSHEV)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve runs the standard or the Chebyshev loop");
     if (ctx->nscenes < 1) return set_err(ctx, CDX_NO_STATE, "cdx_batch_autotune: scenes not set");
     if (ctx->njrows > 0 && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: theta must be positive with joint rows");
     CDX_TRY(build_incidence(ctx));

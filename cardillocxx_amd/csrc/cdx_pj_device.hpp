@@ -179,13 +179,28 @@ __device__ inline double cheb_next_w(int it, double rho2, double w) {
     return it == 0 ? 1.0 / (1.0 - rho2 * 0.5) : 1.0 / (1.0 - (rho2 * 0.25) * w);
 }
 
+// p_k - sum_l Rb(k, l) t_l over the nr rows of contact c (block mode: the inverse Delassus block,
+// entry (k, l) at (3k + l) nc + c; BlockDiagonal::applyTo, block_diagonal.hpp:103-124), in place
+// of the scaled rows p_k - t_k.
+__device__ inline void block_apply_rows(int c, int nc, int nr, const double* __restrict__ Rb, const double pold[3],
+                                        double t[3]) {
+    const double t0 = t[0], t1 = t[1], t2 = t[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const double* r = Rb + (size_t)(3 * k) * nc + c;
+        t[k] = pold[k] - (nr == 1 ? r[0] * t0 : r[0] * t0 + r[nc] * t1 + r[2 * (size_t)nc] * t2);
+    }
+}
+
 // One contact of a PJ sweep (pj_sweep, projected_jacobi.cpp:121-138, with project(), :100-119).
 // The bias is only read when the problem has one (`use_bias`: some restitution or velocity bias).
+// With Rb (block mode) R is unused and the step is p - Rb (W x + b).
 __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
                                         const double* __restrict__ J, const double* __restrict__ R,
                                         const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
                                         double* __restrict__ p, const double* __restrict__ x,
-                                        double* __restrict__ imp) {
+                                        double* __restrict__ imp, const double* __restrict__ Rb = nullptr) {
     const int a = ida[c], b = idb[c];
     if (a < 0 && b < 0) return;
     const double muc = mu[c];
@@ -201,8 +216,12 @@ __device__ inline void pj_contact_sweep(int c, int nc, const int* __restrict__ i
         if (k >= nr) break;
         const double g = row_dot(j.d[k], j.aA[k], j.aB[k], ua, ub);
         pold[k] = p[(size_t)k * nc + c];
-        pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
+        if (Rb)
+            pn[k] = g + (use_bias ? bias[(size_t)k * nc + c] : 0.0);
+        else
+            pn[k] = pold[k] - (R[(size_t)k * nc + c] * g + (use_bias ? bias[(size_t)k * nc + c] : 0.0));
     }
+    if (Rb) block_apply_rows(c, nc, nr, Rb, pold, pn);
     // project() (projected_jacobi.cpp:100-119)
     pn[0] = fmax(pn[0], 0.0);
     if (nr == 3) {

@@ -77,7 +77,7 @@ typedef struct {
     int nesterov_restart_limit;     /* pj.nesterov_restart_limit (default 10000) */
     int block_precond; /* pj.rdiag_true_delassus (default 0): per-contact inverse Delassus blocks alpha (W_c M^-1 W_c^T)^-1
                           instead of the row scaling alpha / D_ii (projected_jacobi.cpp:71-98, 330-348); projected
-                          Jacobi only (ignored by Gauss-Seidel, refused by the batched solve and by cdx_pj_adjoint) */
+                          Jacobi only (ignored by Gauss-Seidel, refused with joint rows and by cdx_pj_adjoint) */
     int anderson_m;    /* pj.anderson_m (default 2): Anderson history window, 1..8 */
 } cdx_solver_params;
 
@@ -168,9 +168,11 @@ int cdx_moreau_update(cdx_ctx* ctx, double dt, double theta);
  * Must be called after cdx_set_bodies / cdx_set_contacts (which clear the partition). */
 int cdx_set_scenes(cdx_ctx* ctx, int nscenes, const int* body_off, const int* contact_off);
 
-/* Solves every scene with the projected-Jacobi standard loop in one persistent launch; each scene
- * has its own residual test and iteration count (iters[s], residuals[s]; either may be NULL).
- * Results are read with cdx_get_velocities / cdx_get_impulses like a single solve. */
+/* Solves every scene with the projected-Jacobi standard or Chebyshev loop (prm->accel NONE or
+ * CHEBYSHEV), row scaling or block preconditioner (prm->block_precond, not with joint rows), in
+ * one persistent launch; each scene has its own residual test, spectral radius and iteration count
+ * (iters[s], residuals[s]; either may be NULL).  Results are read with cdx_get_velocities /
+ * cdx_get_impulses like a single solve. */
 int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, int* iters,
                        double* residuals);
 

@@ -213,3 +213,66 @@ def test_batch_refuses_other_accelerations(ctx):
         p.accel = mode
         with pytest.raises(native.CdxError):
             ctx.batch_pj_solve(1e-3, 1.0, p)
+
+
+def block_batch():
+    """Random scenes, boxes resting on a plane (four-corner manifolds: coupled frictional blocks)
+    and a small sphere pile."""
+    probs = [scenes.random_problem(**c) for c in CHEB_CASES[:3]] + make_batch(8, seed=11)
+    plane = scenes.boxes_on_plane(nscenes=2, per_scene=16, seed=4)
+    probs += [plane.scene(s).problem()[0] for s in range(plane.nscenes)]
+    probs.append(scenes.sphere_pile(n=60, settle_steps=60, box=(0.5, 0.5, 1.0), seed=2).problem())
+    return probs
+
+
+@pytest.mark.parametrize("launch", [(256, 1), (512, 4)])
+@pytest.mark.parametrize("accel", ["none", "chebyshev"])
+@pytest.mark.parametrize("sweeps", [1, 7, 40])
+def test_batch_block_preconditioner_fixed_sweeps_matches_oracle(ctx, launch, accel, sweeps):
+    """Per-contact inverse Delassus blocks in the batched solve (pj.rdiag_true_delassus,
+    projected_jacobi.cpp:71-98, 330-348), standard and Chebyshev loops, LDS-resident (256, 1) and
+    global (512, 4) kernels: every scene equals the oracle's standalone block solve after the same
+    sweeps."""
+    probs = block_batch()
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.block_precond = 1
+    p.accel = native.CDX_ACCEL_CHEBYSHEV if accel == "chebyshev" else native.CDX_ACCEL_NONE
+    p.max_iter, p.tol_abs, p.tol_rel = sweeps, 0.0, 0.0
+    try:
+        ctx.batch_set_launch(*launch)
+        iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    finally:
+        ctx.batch_set_launch(256, 1)
+    v, imp = ctx.velocities(), ctx.impulses()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, chebyshev=int(accel == "chebyshev"),
+                                                                          max_iter=sweeps, tol_abs=0.0, tol_rel=0.0))
+        assert st == 0 and iters[s] == r.iters, s
+        vs = v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()]
+        assert rel(vs, v_ref) <= TOL, s
+        if prob.nc:
+            assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL, s
+
+
+@pytest.mark.parametrize("launch", [(256, 1), (128, 2)])
+def test_batch_block_preconditioner_to_tolerance(ctx, launch):
+    """Block mode to tol 1e-8 per scene: the oracle's iteration count, velocities and impulses."""
+    probs = block_batch()
+    cat, boff, coff = upload_batch(ctx, probs)
+    p = native.default_params()
+    p.block_precond = 1
+    p.tol_abs = p.tol_rel = 1e-8
+    try:
+        ctx.batch_set_launch(*launch)
+        iters, res = ctx.batch_pj_solve(1e-3, 1.0, p)
+    finally:
+        ctx.batch_set_launch(256, 1)
+    v, imp = ctx.velocities(), ctx.impulses()
+    for s, prob in enumerate(probs):
+        st, v_ref, imp_ref, r = oracle.pj_solve(prob, 1e-3, oracle.params(block_precond=1, tol_abs=1e-8, tol_rel=1e-8))
+        assert st == 0 and iters[s] == r.iters, s
+        vs = v[cat.dof[:boff[s]].sum():cat.dof[:boff[s + 1]].sum()]
+        assert rel(vs, v_ref) <= TOL, s
+        if prob.nc:
+            assert rel(imp[coff[s]:coff[s + 1]], imp_ref) <= TOL, s

@@ -1,7 +1,8 @@
 """GPU parity of the exact per-contact Delassus block preconditioner (pj.rdiag_true_delassus,
 projected_jacobi.cpp:71-98, 330-348) against the oracle: velocities and impulses to <= 1e-10
 relative after the same sweeps, in the standard, Nesterov and Chebyshev loops, and the error
-paths of the solvers that do not take it."""
+paths of the solvers that do not take it (the batched solve's block mode is covered with the
+batched tests)."""
 import numpy as np
 import pytest
 
@@ -108,10 +109,12 @@ def test_block_refused_where_not_supported(ctx):
     with pytest.raises(native.CdxError) as e:  # the adjoint differentiates the row-scaled solve only
         ctx.pj_adjoint(1e-3, params(block_precond=1), np.ones(prob.nv))
     assert e.value.status == native.CDX_NO_STATE
+    # a batched block solve leaves no state the adjoint takes either
     ctx.set_scenes([0, prob.nb], [0, prob.nc])
+    ctx.batch_pj_solve(1e-3, 1.0, params(block_precond=1, max_iter=20))
     with pytest.raises(native.CdxError) as e:
-        ctx.batch_pj_solve(1e-3, 1.0, params(block_precond=1))
-    assert e.value.status == native.CDX_BAD_ARG
+        ctx.pj_adjoint(1e-3, params(block_precond=1), np.ones(prob.nv))
+    assert e.value.status == native.CDX_NO_STATE
     # Gauss-Seidel has its own local solves: the flag does not change it (each solve warm-starts
     # from the contacts' impulses, so both start from the uploaded ones)
     out = []
