@@ -234,6 +234,9 @@ def pile_leg(args, native, scenes, metrics, hip):
     bc, bb = algorithmic_bytes(prob)
 
     ctx = native.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
+    ctx.set_range_sweep(args.range_mode, args.range_contacts, args.range_block)
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
     ctx.synchronize()
@@ -269,6 +272,10 @@ def pile_leg(args, native, scenes, metrics, hip):
     ev_s = hip.elapsed_ms(e0, e1) * 1e-3
     hip.destroy(e0, e1)
     cups = prob.nc * sweeps / elapsed
+    r_used, r_n, r_parts, r_per = ctx.range_sweep_info()
+    sweep_form = {"range_sweep": r_used, "ranges": r_n, "partials_per_sweep": r_parts, "contacts_per_range": r_per,
+                  "mode": args.range_mode, "block": args.range_block or 512}
+    log("sweep: " + (f"{r_n} contact ranges of {r_per}, {r_parts} partial sums" if r_used else "two passes"))
     log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s ({ev_s:.3f}s by stream events), "
         f"{cups:.3e} contact-updates/s")
     td = time.perf_counter()
@@ -288,10 +295,10 @@ def pile_leg(args, native, scenes, metrics, hip):
         # launches enqueued after convergence return at once; average over the sweeps that did work
         psweeps = max(pinfo.iters, 1)
         kernels = {}
-        # the sweep's kernels: the fused tile sweep and its boundary pass, or the separate contact
-        # and body passes; the residual reduction every 10th sweep
-        sweep_bytes = {"pj_fused": bc + bb, "pj_contacts": bc, "pj_bodies": bb}
-        for name in ("pj_fused", "pj_boundary", "pj_contacts", "pj_bodies", "pj_residual"):
+        # the sweep's kernels: the range pass and the body pass over its partial sums, or the two
+        # passes (contacts, bodies); the residual reduction every 10th sweep
+        sweep_bytes = {"pj_range": bc, "pj_bodies_parts": bb, "pj_contacts": bc, "pj_bodies": bb}
+        for name in ("pj_range", "pj_bodies_parts", "pj_contacts", "pj_bodies", "pj_residual"):
             calls, ms = stats.get(name, (0, 0.0))
             if not calls:
                 continue
@@ -369,7 +376,7 @@ def pile_leg(args, native, scenes, metrics, hip):
                                               "read back through the C ABI"}},
         "roofline": {"bound": "hbm", "kernel": "pj_sweep", "achieved": ach_sweep, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_sweep / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "bytes_per_sweep": bytes_sweep, "avg_sweep_us": ev_s / sweeps * 1e6,
+                     "bytes_per_sweep": bytes_sweep, "avg_sweep_us": ev_s / sweeps * 1e6, "sweep_form": sweep_form,
                      "definition": "one projected-Jacobi sweep (contact pass + body pass, residual every 10th): "
                                    "algorithmic bytes per sweep (328 B per contact with two dynamic sides, 256 B with a "
                                    "static side, 128 B per body) x sweeps / time of the timed solves measured by HIP "
@@ -927,7 +934,14 @@ def main():
     ap.add_argument("--no-gs", action="store_true", help="skip the coloured Gauss-Seidel solve of the pile")
     ap.add_argument("--no-accel", action="store_true", help="skip the Chebyshev / Nesterov / Anderson solves of the pile")
     ap.add_argument("--anderson-m", type=int, default=2, help="Anderson history window (pj.anderson_m)")
-    ap.add_argument("--cpu-sweeps", type=int, default=100, help="sweeps in the CPU baseline sample")
+    ap.add_argument("--cpu-sweeps", type=int, default=600, help="sweeps in the CPU baseline sample")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="pile leg: launch every sweep's kernels directly instead of replaying graph-captured chunks "
+                         "(profiler runs)")
+    ap.add_argument("--range-mode", type=int, default=1, choices=(0, 1, 2),
+                    help="range sweep of projected Jacobi: 0 two passes, 1 from 32,768 contacts, 2 always")
+    ap.add_argument("--range-contacts", type=int, default=0, help="contacts per range (0 = 128)")
+    ap.add_argument("--range-block", type=int, default=0, help="threads per range workgroup (0 = 512, or 256)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
     ap.add_argument("--no-kernel-events", action="store_true",

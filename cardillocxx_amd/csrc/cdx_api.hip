@@ -84,6 +84,9 @@ int build_incidence(cdx_ctx* ctx) {
     if (nc > 0)
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incidx, idx.data(), sizeof(int) * 2 * (size_t)nc, hipMemcpyHostToDevice, ctx->stream));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    idx.resize(2 * (size_t)nc);
+    ctx->h_incptr = std::move(ptr);
+    ctx->h_incidx = std::move(idx);
     ctx->incidence_valid = true;
     return CDX_OK;
 }
@@ -210,7 +213,8 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jbody_blk, ctx->d_jslot, ctx->d_jr, ctx->d_jrlam, ctx->d_jz, ctx->d_jlam, ctx->d_jL,
                     ctx->d_jDinv, ctx->d_jtr, ctx->d_sc_off, ctx->d_sc_list, ctx->d_adj_jl, ctx->d_jr_blk, ctx->d_jy, ctx->d_gm, ctx->d_gm_part, ctx->d_jr_info, ctx->d_jbody_info,
                     ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
-                    ctx->d_gs_pack, ctx->d_gs_ids};
+                    ctx->d_gs_pack, ctx->d_gs_ids, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc,
+                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);
@@ -324,6 +328,7 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
     }
     if (resized) {
         ctx->incidence_valid = false;
+        ctx->ranges_valid = false;
         ctx->coloring_valid = false;
     }
     ctx->nscenes = 0;
@@ -424,6 +429,7 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     }
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->incidence_valid = false;
+    ctx->ranges_valid = false;
     ctx->coloring_valid = false;
     ctx->nscenes = 0;
     ctx->solved = false;

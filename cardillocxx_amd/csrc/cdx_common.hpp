@@ -116,6 +116,7 @@ struct cdx_ctx {
     double* d_imp = nullptr;   // [12*nc]
     int* d_incptr = nullptr;   // [nb+1]
     int* d_incidx = nullptr;   // [2*nc]
+    std::vector<int> h_incptr, h_incidx;  // host copies of the incidence lists
     size_t contact_cap = 0;
     bool incidence_valid = false;
 
@@ -186,6 +187,21 @@ struct cdx_ctx {
 
     // ---- replay whole chunks of sweeps as HIP graphs (off while profiling or recording)
     bool graphs = true;
+
+    // ---- range sweep of projected Jacobi: one workgroup per contiguous contact range, per-body
+    // partial sums reduced in LDS; tables rebuilt at the first standard-loop solve after the
+    // contacts change
+    int range_mode = 1;         // 0 off, 1 from kRangeAutoContacts contacts, 2 for every problem
+    int range_per = 0;          // contacts per range (0: 128)
+    int range_block = 512;      // threads per range workgroup
+    bool ranges_valid = false;  // the tables match the current contacts and settings
+    bool ranges_ok = false;     // the tables fit the LDS budget
+    bool ranges_used = false;   // the last standard-loop solve swept ranges
+    int nranges = 0, npart = 0, range_per_used = 0, range_maxp = 0, range_maxl = 0;
+    int *d_rg_p = nullptr, *d_rg_l = nullptr, *d_pr_ptr = nullptr, *d_pr_out = nullptr, *d_pr_inc = nullptr;
+    int* d_bp_ptr = nullptr;   // [nb] partial sums per body
+    int* d_bp_ovf = nullptr;   // [nb] offset of a body's partials beyond its fixed slots
+    double* d_part = nullptr;  // partial sums: fixed slots per body, then the overflow area
 
     // ---- profiling
     bool profiling = false;

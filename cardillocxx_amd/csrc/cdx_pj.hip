@@ -242,6 +242,235 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Range sweep: per-body partial sums of each workgroup's contact range, reduced in LDS
+// ---------------------------------------------------------------------------------------------
+//
+// One workgroup takes a contiguous range of contacts.  It runs their contact pass (as
+// k_pj_contacts4) with the side-impulse records in LDS instead of device memory, and then, after
+// a workgroup barrier, sums for every body its range touches the records of its incidences in the
+// range, and writes that sum as one 6-double partial.  The body pass (k_pj_bodies_parts) adds a
+// body's partials, one per range that touches it, and updates x: together the Jacobi sweep of
+// projected_jacobi.cpp:121-138.  With contacts in the order of their lower body (the contact
+// generator's order) a range touches few bodies outside its own span, so a body has about two
+// partials where the separate passes move one record per incidence (eight per body in a pile).
+// Partials are stored body by body, ranges in ascending order, so every sum has a fixed order
+// and results repeat bit for bit.
+constexpr int kRangeMaxContacts = 768;     // LDS records: 9 doubles per contact, 55,296 bytes
+constexpr int kRangeAutoContacts = 32768;  // range mode 1 from this many contacts
+constexpr int kRangeLdsBytes = 65536;      // LDS of one range workgroup: records and tables
+
+__host__ __device__ constexpr int pad64(int n) { return (n + 63) & ~63; }
+
+// Copies n ints from src to the LDS array dst by LDS DMA (global_load_lds_dword: lane i of a wave
+// writes dst + 4 i bytes), one 64-int chunk per wave and step; a lane past the end loads the last
+// element again into padding (dst holds pad64(n) ints).  Completed by s_waitcnt + barrier.
+__device__ __forceinline__ void dma_ints(const int* __restrict__ src, int n, int* dst) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, W = blockDim.x >> 6;
+    for (int ch = w; ch * 64 < n; ch += W) {
+        const int i = min(ch * 64 + lane, n - 1);
+        __builtin_amdgcn_global_load_lds(src + i, dst + ch * 64, 4, 0, 0);
+    }
+}
+
+template <bool Block>
+__global__ void __launch_bounds__(512, 8) k_pj_range(
+    int nranges, int nc, int per, int maxs, int maxp, const int* __restrict__ rg_p, const int* __restrict__ rg_l,
+    const int* __restrict__ pr_ptr, const int* __restrict__ pr_out, const int* __restrict__ pr_inc,
+    const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
+    const double* __restrict__ R, const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
+    double* __restrict__ p, const double* __restrict__ x, double* __restrict__ part, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    // LDS: the records (9 doubles per contact of the range: lin | angA | angB), then the range's
+    // tables, copied by DMA while the contact rounds run: the incidence offsets of its pairs
+    // (body, range), their partial slots, and the incidences (contact in the range << 1 | side)
+    extern __shared__ double lrec[];
+    const int g = xcd_tile(nranges);
+    const int s0 = g * per, s1 = min(nc, s0 + per);
+    const int q0 = rg_p[g], q1 = rg_p[g + 1], l0 = rg_l[g], l1 = rg_l[g + 1];
+    int* Lptr = reinterpret_cast<int*>(lrec + 9 * (size_t)maxs);
+    int* Lout = Lptr + pad64(maxp + 1);
+    int* Linc = Lout + pad64(maxp);
+    dma_ints(pr_ptr + q0, q1 - q0 + 1, Lptr);
+    dma_ints(pr_out + q0, q1 - q0, Lout);
+    dma_ints(pr_inc + l0, l1 - l0, Linc);
+
+    // Contact rounds, 4 lanes per contact as k_pj_contacts4.  The next round's body ids are loaded
+    // during the current round, and lanes 0-2 load their rows and twists whatever the contact's
+    // friction (mu arrives with them), so a round waits for one memory latency, not two.
+    const int Q = blockDim.x >> 2;
+    int an = -1, bn = -1;
+    {
+        const int c = s0 + (threadIdx.x >> 2);
+        if (c < s1) {
+            an = ida[c];
+            bn = idb[c];
+        }
+    }
+#pragma unroll 1
+    for (int base = s0; base < s1; base += Q) {
+        const int c = base + (threadIdx.x >> 2);
+        const bool live = c < s1;
+        const int a = an, b = bn;
+        if (c + Q < s1) {
+            an = ida[c + Q];
+            bn = idb[c + Q];
+        }
+        // The lane's row and the column stride pass through empty asm statements, so the compiler
+        // recomputes the 27 row addresses in each round instead of keeping them all in registers
+        // across the loop (which would spill at 64 VGPRs).
+        int k = threadIdx.x & 3, ncl = nc;
+        asm volatile("" : "+v"(k));
+        asm volatile("" : "+s"(ncl));
+        const bool dyn = live && (a >= 0 || b >= 0);
+        const bool ld = dyn && k < 3;
+        const double muc = live ? mu[c] : 0.0;
+        double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
+        double gk = 0, pold = 0, rk = 0, bk = 0, pt = 0;
+        if (ld) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const double* Jr = J + (size_t)(3 * k) * ncl + c;
+                d[m] = Jr[(size_t)m * ncl];
+                aA[m] = Jr[(size_t)(9 + m) * ncl];
+                aB[m] = Jr[(size_t)(18 + m) * ncl];
+            }
+            double ua[6], ub[6];
+            load_body(x, a, ua);
+            load_body(x, b, ub);
+            gk = row_dot(d, aA, aB, ua, ub);
+            pold = p[(size_t)k * ncl + c];
+            if constexpr (!Block) rk = R[(size_t)k * ncl + c];
+            if (use_bias) bk = bias[(size_t)k * ncl + c];
+        }
+        const int nr = muc > 0 ? 3 : 1;
+        const bool row = dyn && k < nr;
+        if (row) {
+            if constexpr (Block)
+                pt = gk + bk;
+            else
+                pt = pold - (rk * gk + bk);
+        } else {
+            pold = 0;
+        }
+        if constexpr (Block) pt = pold - block_apply4(pt, k, c, ncl, nr, row, R);
+        const double pnew = project4(pt, k, muc);
+        double dp = 0;
+        if (row) {
+            dp = pnew - pold;
+            p[(size_t)k * ncl + c] = pnew;
+        }
+        double v9[9];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            v9[m] = d[m] * dp;
+            v9[3 + m] = aA[m] * dp;
+            v9[6 + m] = aB[m] * dp;
+        }
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v9[m] = quad_sum(v9[m]);
+        if (dyn && k < 3) {  // lane 0 writes lin, lane 1 angA, lane 2 angB (zero for a static side)
+            double* l = lrec + 9 * (c - s0) + 3 * k;
+            l[0] = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
+            l[1] = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
+            l[2] = k == 0 ? v9[2] : (k == 1 ? v9[5] : v9[8]);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's table DMA has landed
+    __syncthreads();                // ... and every wave's records and DMA
+    // One partial per (body, range) pair, 8 lanes per pair as k_pj_bodies8's gather.
+    const int l = threadIdx.x & 7;
+    const int npairs = q1 - q0;
+#pragma unroll 1
+    for (int base = 0; base < npairs; base += blockDim.x >> 3) {
+        const int j = base + (threadIdx.x >> 3);
+        const bool live = j < npairs;
+        int i0 = 0, i1 = 0, out = 0;
+        if (live) {
+            i0 = Lptr[j] - l0;
+            i1 = Lptr[j + 1] - l0;
+            out = Lout[j];
+        }
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = i0 + l; i < i1; i += 8) {
+            const int e = Linc[i];
+            const double* r = lrec + 9 * (e >> 1);
+            if (e & 1) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    acc[m] += r[m];
+                    acc[3 + m] += r[6 + m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    acc[m] -= r[m];
+                    acc[3 + m] -= r[3 + m];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc[m] = oct_sum(acc[m]);
+        if (live && l < 6) {
+            double a = acc[0];
+#pragma unroll
+            for (int m = 1; m < 6; ++m)
+                if (l == m) a = acc[m];
+            part[(size_t)out * 6 + l] = a;
+        }
+    }
+}
+
+// Body pass over the partials.  Body b's first kPartSlots partials sit in slots b kPartSlots + j
+// and any further ones in an overflow area (ovf[b] + j - kPartSlots), so a body's count, twist,
+// inverse mass and first partials are all loaded at once.  Lane l < 6 of a body's 8 adds
+// component l of its partials in order and updates x (x += (W^T dp) / M), with the residual
+// partials every check sweep.
+constexpr int kPartSlots = 3;
+
+__global__ void __launch_bounds__(kBlock) k_pj_bodies_parts(int nb, const int* __restrict__ dof,
+                                                            const double* __restrict__ minv,
+                                                            const int* __restrict__ pcnt, const int* __restrict__ povf,
+                                                            const double* __restrict__ part, double* __restrict__ x,
+                                                            int check, double tol_rel, double tol_abs,
+                                                            double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    double e2 = 0;
+    if (bi < nb && l < 6) {
+        const int nd = dof[bi], n = pcnt[bi];
+        const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+        const double u = x[(size_t)bi * kBodyStride + l];
+        double ps[kPartSlots];
+#pragma unroll
+        for (int j = 0; j < kPartSlots; ++j) ps[j] = part[((size_t)bi * kPartSlots + j) * 6 + l];
+        if (l < nd) {
+            double a = 0;
+#pragma unroll
+            for (int j = 0; j < kPartSlots; ++j)
+                if (j < n) a += ps[j];
+            if (n > kPartSlots) {
+                const size_t o = (size_t)nb * kPartSlots + povf[bi];
+                for (int j = kPartSlots; j < n; ++j) a += part[(o + j - kPartSlots) * 6 + l];
+            }
+            const double M = mk > 0 ? 1.0 / mk : 0.0;
+            const double un = u + a / M;
+            if (n > 0) x[(size_t)bi * kBodyStride + l] = un;
+            if (check) {
+                const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+                const double ee = (un - u) / q;
+                e2 = ee * ee;
+            }
+        }
+    }
+    if (check) {
+        const double s = block_sum(e2, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
 // mode 0: PJ (break when res <= 1); mode 1: GS (break when res < 1, NaN/Inf -> diverged).
 // it < 0: the sweep index comes from the device chunk counter ctrl[3] (graph-replayed chunks of
 // `cadence` sweeps whose residual check is the chunk's first sweep); the counter advances on
@@ -1123,6 +1352,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         info->ncolors = 0;
     }
     ctx->solved = false;
+    ctx->ranges_used = false;
     ctx->pj_last = false;
     ctx->pjj_last = false;
     ctx->last_theta = theta;
@@ -1214,8 +1444,151 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     return CDX_OK;
 }
 
+// LDS of one range workgroup: 9 doubles per contact of the range, then the int tables
+// (k_pj_range).
+static size_t range_lds_bytes(int maxs, int maxp, int maxl) {
+    return sizeof(double) * 9 * (size_t)maxs + sizeof(int) * (size_t)(pad64(maxp + 1) + pad64(maxp) + pad64(maxl));
+}
+
+template <typename T>
+static int upload_exact(cdx_ctx* ctx, T** dst, const std::vector<T>& src, const char* what) {
+    if (*dst) hipFree(*dst);
+    *dst = nullptr;
+    CDX_TRY(ensure_alloc(ctx, dst, src.size(), nullptr, what));
+    if (!src.empty())
+        CDX_HIP(ctx, hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, ctx->stream));
+    return CDX_OK;
+}
+
+// Tables of the range sweep (k_pj_range, k_pj_bodies_parts) for the current contacts: range g
+// holds contacts [g per, (g + 1) per).  A (body, range) pair exists for every range holding an
+// incidence of the body; a body's partials are numbered in ascending range order, the first
+// kPartSlots in the body's fixed slots and the rest in the overflow area.
+// Per range, its pairs in body order (rg_p), each with its partial slot (pr_out) and its
+// incidences in contact order (pr_ptr, pr_inc: contact in the range << 1 | side).
+static int build_ranges(cdx_ctx* ctx) {
+    ctx->ranges_valid = true;
+    ctx->ranges_ok = false;
+    const int nb = ctx->nb, nc = ctx->nc;
+    if (nb <= 0 || nc <= 0) return CDX_OK;
+    const int per = std::max(16, std::min(ctx->range_per > 0 ? ctx->range_per : 128, kRangeMaxContacts));
+    const int ng = (nc + per - 1) / per;
+    const std::vector<int>& ip = ctx->h_incptr;
+    const std::vector<int>& ix = ctx->h_incidx;
+    // pairs and incidences per range; partial slots per body
+    std::vector<int> npr(ng + 1, 0), nin(ng + 1, 0), bp(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        int last = -1;
+        for (int i = ip[b]; i < ip[b + 1]; ++i) {
+            const int g = (ix[i] >> 1) / per;  // incidences run in contact order: ranges ascend
+            if (g != last) {
+                npr[g + 1]++;
+                bp[b + 1]++;
+                last = g;
+            }
+            nin[g + 1]++;
+        }
+    }
+    for (int g = 0; g < ng; ++g) {
+        npr[g + 1] += npr[g];
+        nin[g + 1] += nin[g];
+    }
+    // partial slots: the first kPartSlots of body b at b kPartSlots + j, the rest in the overflow
+    // area after nb kPartSlots slots
+    std::vector<int> pcnt(nb), povf(nb, 0);
+    int novf = 0;
+    for (int b = 0; b < nb; ++b) {
+        pcnt[b] = bp[b + 1];
+        povf[b] = novf;
+        novf += std::max(0, pcnt[b] - kPartSlots);
+    }
+    const int npairs = npr[ng], ninc = nin[ng];
+    int maxp = 0, maxl = 0;
+    for (int g = 0; g < ng; ++g) {
+        maxp = std::max(maxp, npr[g + 1] - npr[g]);
+        maxl = std::max(maxl, nin[g + 1] - nin[g]);
+    }
+    const size_t shm = range_lds_bytes(std::min(per, nc), maxp, maxl);
+    if (shm > (size_t)kRangeLdsBytes) return CDX_OK;  // tables beyond the LDS budget: separate passes
+    std::vector<int> pr_ptr(npairs + 1), pr_out(npairs), pr_inc(std::max(ninc, 1));
+    std::vector<int> fp(npr.begin(), npr.end() - 1), fl(nin.begin(), nin.end() - 1);
+    for (int b = 0; b < nb; ++b) {
+        int last = -1, k = 0;
+        for (int i = ip[b]; i < ip[b + 1]; ++i) {
+            const int e = ix[i], c = e >> 1, g = c / per;
+            if (g != last) {
+                const int q = fp[g]++;
+                pr_out[q] = k < kPartSlots ? b * kPartSlots + k : nb * kPartSlots + povf[b] + k - kPartSlots;
+                ++k;
+                pr_ptr[q] = fl[g];
+                last = g;
+            }
+            pr_inc[fl[g]++] = ((c - g * per) << 1) | (e & 1);
+        }
+    }
+    pr_ptr[npairs] = ninc;
+    CDX_TRY(upload_exact(ctx, &ctx->d_rg_p, npr, "range pairs"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_rg_l, nin, "range incidences"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_pr_ptr, pr_ptr, "pair incidence offsets"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_pr_out, npairs ? pr_out : std::vector<int>{0}, "pair partial slots"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_pr_inc, pr_inc, "pair incidences"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_bp_ptr, pcnt, "body partial counts"));
+    CDX_TRY(upload_exact(ctx, &ctx->d_bp_ovf, povf, "body partial overflow offsets"));
+    if (ctx->d_part) hipFree(ctx->d_part);
+    ctx->d_part = nullptr;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_part, 6 * ((size_t)nb * kPartSlots + novf), nullptr, "partial sums"));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->nranges = ng;
+    ctx->npart = npairs;
+    ctx->range_per_used = per;
+    ctx->range_maxp = maxp;
+    ctx->range_maxl = maxl;
+    ctx->ranges_ok = true;
+    return CDX_OK;
+}
+
+// Standard loop with the range sweep: per sweep one k_pj_range launch and one k_pj_bodies_parts
+// launch, the residual every check_every sweeps; chunks between two checks replayed as graphs.
+static int run_range_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out) {
+    const int nb = ctx->nb, nc = ctx->nc, ng = ctx->nranges, per = ctx->range_per_used;
+    const int check_every = prm->check_every;
+    const bool block = prm->block_precond != 0;
+    const int gb8 = grid_for(nb * kLanesPerBody);
+    const int nthr = ctx->range_block;
+    const int maxs = std::min(per, nc);
+    const size_t shm = range_lds_bytes(maxs, ctx->range_maxp, ctx->range_maxl);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
+    ctx->ranges_used = true;
+    auto launch_sweep = [&](int it, int check) -> int {
+        if (block)
+            CDX_LAUNCH_SHM(ctx, "pj_range", k_pj_range<true>, ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,
+                           ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,
+                           ctx->d_J, ctx->d_Rb, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,
+                           ctx->d_part, ctx->d_ctrl);
+        else
+            CDX_LAUNCH_SHM(ctx, "pj_range", k_pj_range<false>, ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,
+                           ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,
+                           ctx->d_J, ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,
+                           ctx->d_part, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_bodies_parts", k_pj_bodies_parts, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_bp_ptr,
+                   ctx->d_bp_ovf, ctx->d_part, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial,
+                   ctx->d_ctrl);
+        if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));
+        return hip_check(ctx, hipGetLastError(), "pj range sweep launch");
+    };
+    return run_sweeps_graphed(ctx, prm->max_iter, check_every, true,
+                              [&](int it, int pos) { return launch_sweep(it, pos == 0); }, iters_out);
+}
+
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out) {
     const int nb = ctx->nb, nc = ctx->nc;
+    // range sweeps: standard loop without joint rows or recording, on large enough problems
+    const bool want_ranges = ctx->range_mode == 2 || (ctx->range_mode == 1 && nc >= kRangeAutoContacts);
+    if (want_ranges && ctx->njrows == 0 && !ctx->record) {
+        if (!ctx->ranges_valid) CDX_TRY(build_ranges(ctx));
+        if (ctx->ranges_ok) return run_range_sweeps(ctx, prm, iters_out);
+    }
+    ctx->ranges_used = false;
     const int check_every = prm->check_every;
     const bool block = prm->block_precond != 0;
     const int gb8 = grid_for(nb * kLanesPerBody);
@@ -1781,5 +2154,29 @@ extern "C" int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, dou
         info->iters = iters;
         info->residual = r;
     }
+    return CDX_OK;
+}
+
+extern "C" int cdx_set_range_sweep(cdx_ctx* ctx, int mode, int contacts_per_range, int block) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (mode < 0 || mode > 2) return set_err(ctx, CDX_BAD_ARG, "cdx_set_range_sweep: mode must be 0, 1 or 2");
+    if (contacts_per_range != 0 && (contacts_per_range < 16 || contacts_per_range > kRangeMaxContacts))
+        return set_err(ctx, CDX_BAD_ARG, "cdx_set_range_sweep: contacts_per_range must be 0 or in 16..768");
+    if (block != 0 && block != 256 && block != 512)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_set_range_sweep: block must be 0, 256 or 512");
+    const int nblock = block ? block : 512;
+    if (contacts_per_range != ctx->range_per) ctx->ranges_valid = false;
+    ctx->range_mode = mode;
+    ctx->range_per = contacts_per_range;
+    ctx->range_block = nblock;
+    return CDX_OK;
+}
+
+extern "C" int cdx_range_sweep_info(cdx_ctx* ctx, int* used, int* nranges, int* npartials, int* contacts_per_range) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (used) *used = ctx->ranges_used ? 1 : 0;
+    if (nranges) *nranges = ctx->ranges_ok ? ctx->nranges : 0;
+    if (npartials) *npartials = ctx->ranges_ok ? ctx->npart : 0;
+    if (contacts_per_range) *contacts_per_range = ctx->ranges_ok ? ctx->range_per_used : 0;
     return CDX_OK;
 }

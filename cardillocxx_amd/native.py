@@ -64,6 +64,8 @@ EXPORTED = [
     "cdx_pj_adjoint_joints",
     "cdx_pj_solve_noalloc",
     "cdx_set_graphs",
+    "cdx_set_range_sweep",
+    "cdx_range_sweep_info",
     "cdx_kernel_stat",
     "cdx_get_stream",
 ]
@@ -136,6 +138,8 @@ def _declare(lib):
         "cdx_moreau_update": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
         "cdx_set_graphs": (C.c_int, [_vp, C.c_int]),
+        "cdx_set_range_sweep": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+        "cdx_range_sweep_info": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
         "cdx_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
         "cdx_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _ip, _dp]),
         "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
@@ -412,6 +416,16 @@ class Context:
 
     def set_graphs(self, on: bool):
         self._check(self._l.cdx_set_graphs(self.h, 1 if on else 0), "cdx_set_graphs")
+
+    def set_range_sweep(self, mode: int, contacts_per_range: int = 0, block: int = 0):
+        """Range sweep of projected Jacobi: mode 0 off, 1 from 32,768 contacts (default), 2 always."""
+        self._check(self._l.cdx_set_range_sweep(self.h, mode, contacts_per_range, block), "cdx_set_range_sweep")
+
+    def range_sweep_info(self):
+        """(used by the last solve, ranges, partial sums per sweep, contacts per range)."""
+        v = [C.c_int(0) for _ in range(4)]
+        self._check(self._l.cdx_range_sweep_info(self.h, *[C.byref(x) for x in v]), "cdx_range_sweep_info")
+        return bool(v[0].value), v[1].value, v[2].value, v[3].value
 
     def kernel_time(self, name: str):
         calls = C.c_int(0)

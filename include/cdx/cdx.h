@@ -271,6 +271,24 @@ int cdx_kernel_stat(cdx_ctx* ctx, int index, char* name_buf, int buf_len, int* c
  * For profilers and tools that intercept kernel dispatches.  No counterpart in the reference. */
 int cdx_set_graphs(cdx_ctx* ctx, int enable);
 
+/* Range sweep of the standard loop (cdx_pj_solve without acceleration, joint rows or sweep
+ * recording; row scaling or block preconditioner).  One workgroup takes a contiguous range of
+ * contacts: it runs their contact pass with the side impulses in LDS and writes, for each body its
+ * range touches, the sum of those side impulses as one partial; a body pass adds each body's
+ * partials and updates the velocities.  The same sweep as the two-pass form (pj_sweep,
+ * projected_jacobi.cpp:121-138), with per-body sums in another fixed order: results equal to
+ * rounding, bit-reproducible run to run.  It moves the least data when contacts come in the order
+ * of their lower body and body indices follow space; any order is correct.
+ * mode: 0 = off (two passes), 1 = on from 32,768 contacts (default), 2 = on for every problem.
+ * contacts_per_range: 0 (= 128) or 16..768.  block: threads per range workgroup, 0 (= 512), 256
+ * or 512.  When a range's tables exceed the workgroup's LDS the solve uses the two-pass form.
+ * No counterpart in the reference. */
+int cdx_set_range_sweep(cdx_ctx* ctx, int mode, int contacts_per_range, int block);
+/* Whether the last cdx_pj_solve swept ranges, and the range tables of the current contacts:
+ * ranges, partial sums per sweep (one per body and range touching it), contacts per range (0
+ * until a range solve has built them). */
+int cdx_range_sweep_info(cdx_ctx* ctx, int* used, int* nranges, int* npartials, int* contacts_per_range);
+
 /* ---- Allocation-free projected Jacobi.  No counterpart in the reference: this entry point is for
  * callers that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc,
  * code that must not grow the device heap).  It runs the standard loop of cdx_pj_solve on arrays
