@@ -934,7 +934,7 @@ def main():
     ap.add_argument("--no-gs", action="store_true", help="skip the coloured Gauss-Seidel solve of the pile")
     ap.add_argument("--no-accel", action="store_true", help="skip the Chebyshev / Nesterov / Anderson solves of the pile")
     ap.add_argument("--anderson-m", type=int, default=2, help="Anderson history window (pj.anderson_m)")
-    ap.add_argument("--cpu-sweeps", type=int, default=600, help="sweeps in the CPU baseline sample")
+    ap.add_argument("--cpu-sweeps", type=int, default=900, help="sweeps in the CPU baseline sample")
     ap.add_argument("--no-graphs", action="store_true",
                     help="pile leg: launch every sweep's kernels directly instead of replaying graph-captured chunks "
                          "(profiler runs)")

@@ -370,6 +370,135 @@ __global__ void k_gs_finish_contacts(int nc, double* __restrict__ lam) {
     lam[c] = -lam[c];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Nesterov-accelerated coloured Gauss-Seidel (the pj_nesterov branch of
+// ProjectedGaussSeidelSolver::solve, projected_gauss_seidel.cpp:196-262).
+//
+// Buffers: X = u_corr (the sweep works in place), XK = u_k, UY = the sweep input u_y (kept for the
+// residual and the restart test), lambda in colour order (the sweep works in place), LK = lambda_k.
+// Each iteration: extrapolate (lambda_y, u_y) = k1 + beta (k1 - k) with beta from the previous
+// iteration (0 on a restart, with momentum off, or before the first sweep), rotate k <- k1, run
+// the colour launches, then reduce the residual between u_y and u_k1 and the restart dot
+// (u_y - u_k1) . (u_k1 - u_k) and decide beta (the same rules and nest[] state as k_pjn_finalize).
+// ---------------------------------------------------------------------------------------------
+
+// Threads [0, nb): bodies; [nb, nb + ns): contacts in colour order.
+__global__ void k_gsn_extrapolate(int nb, int ns, double* __restrict__ x, double* __restrict__ xk,
+                                  double* __restrict__ uy, double* __restrict__ lam, double* __restrict__ lamk,
+                                  const double* __restrict__ nest, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    const double beta = nest[0];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb) {
+        double u[6], uk[6], y[6];
+        load_body(x, t, u);
+        load_body(xk, t, uk);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) y[k] = beta == 0.0 ? u[k] : u[k] + beta * (u[k] - uk[k]);
+        store6(xk + (size_t)t * kBodyStride, u);
+        store6(x + (size_t)t * kBodyStride, y);
+        store6(uy + (size_t)t * kBodyStride, y);
+    } else if (t < nb + ns) {
+        const int i = t - nb;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double l = lam[(size_t)k * ns + i], lk = lamk[(size_t)k * ns + i];
+            lamk[(size_t)k * ns + i] = l;
+            lam[(size_t)k * ns + i] = beta == 0.0 ? l : l + beta * (l - lk);
+        }
+    }
+}
+
+// Residual partials between u_y and u_k1 (projected_gauss_seidel.cpp:124-131) and the restart dot
+// (u_y - u_k1) . (u_k1 - u_k) over the body's dofs.
+__global__ void __launch_bounds__(kBlock) k_gsn_residual(int nb, const int* __restrict__ dof, const double* __restrict__ uf,
+                                                         const double* __restrict__ uk1, const double* __restrict__ uy,
+                                                         const double* __restrict__ uk, double tol_rel, double tol_abs,
+                                                         double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    double e2 = 0, dt = 0;
+    if (b < nb) {
+        double f[6], u1[6], u0[6], k0[6];
+        load_body(uf, b, f);
+        load_body(uk1, b, u1);
+        load_body(uy, b, u0);
+        load_body(uk, b, k0);
+        for (int k = 0; k < dof[b]; ++k) {
+            const double d = u0[k] - u1[k];
+            const double v0 = f[k] - u1[k], v1 = f[k] - u0[k];
+            const double q = fmax(fabs(v0), fabs(v1)) * tol_rel + tol_abs;
+            const double e = d / q;
+            e2 += e * e;
+            dt += (u0[k] - u1[k]) * (u1[k] - k0[k]);
+        }
+    }
+    const double s = block_sum(e2, red);
+    const double d = block_sum(dt, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = d;
+    }
+}
+
+// nest[]: 0 beta, 1 thk, 2 err_prev, 3 restarts, 4 momentum_disabled.  A non-finite residual ends
+// the solve with CDX_DIVERGED (check_residual throws before the restart rules, :133-138); the loop
+// stops when res <= 1 (:224).  it < 0: the sweep index comes from the chunk counter ctrl[3].
+__global__ void __launch_bounds__(kBlock) k_gsn_finalize(int nparts, const double* __restrict__ partial, int nv, int it,
+                                                         double beta_threshold, int restart_limit, int* __restrict__ ctrl,
+                                                         double* __restrict__ res, int* __restrict__ hflag,
+                                                         double* __restrict__ nest) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    double s = 0, d = 0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        s += partial[i];
+        d += partial[nparts + i];
+    }
+    s = block_sum(s, red);
+    d = block_sum(d, red);
+    if (threadIdx.x != 0) return;
+    if (it < 0) it = ctrl[3];
+    ctrl[3] = ctrl[3] + 1;
+    const double err = 1.0 / sqrt((double)nv) * sqrt(s);
+    res[0] = err;
+    if (isnan(err) || isinf(err) || err <= 1.0) {
+        ctrl[1] = it + 1;
+        ctrl[2] = (isnan(err) || isinf(err)) ? CDX_DIVERGED : 0;
+        ctrl[0] = 1;
+        hflag[0] = 1;
+        return;
+    }
+    double thk = nest[1];
+    const double err_prev = nest[2];
+    double restarts = nest[3];
+    int disabled = nest[4] != 0.0;
+    const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+    const double betak1 = (thk - 1.0) / thk1;
+    bool restart = false;
+    if (err_prev < INFINITY && err > 1.05 * err_prev) restart = true;
+    else if (betak1 > beta_threshold) restart = true;
+    else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = true;
+    else if (d > 0.0) restart = true;
+    double beta = 0.0;
+    if (restart) {
+        thk = 1.0;
+        restarts += 1.0;
+        if (restarts >= (double)restart_limit) disabled = 1;
+    } else if (disabled) {
+        thk = 1.0;
+    } else {
+        beta = betak1;
+        thk = thk1;
+    }
+    nest[0] = beta;
+    nest[1] = thk;
+    nest[2] = err;
+    nest[3] = restarts;
+    nest[4] = disabled ? 1.0 : 0.0;
+}
+
 }  // namespace cdx
 
 using namespace cdx;
@@ -383,7 +512,9 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     hipSetDevice(ctx->device);
     CDX_TRY(check_solve_args(ctx, prm));
     if (ctx->njrows > 0) return set_err(ctx, CDX_BAD_ARG, "cdx_gs_colored_solve: joint rows are solved by cdx_pj_solve only");
-    if (prm->accel != CDX_ACCEL_NONE) return set_err(ctx, CDX_BAD_ARG, "cdx_gs_colored_solve: acceleration mode not supported");
+    if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_NESTEROV)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_gs_colored_solve: only the Nesterov acceleration is offered (pj.nesterov)");
+    const bool nesterov = prm->accel == CDX_ACCEL_NESTEROV;
     if (info) {
         info->iters = 0;
         info->residual = std::nan("");
@@ -446,7 +577,41 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     // 8 sweeps replayed as graphs.  (A single cooperative launch with grid barriers between the
     // colours measured twice as slow per sweep: each device-scope barrier writes back and
     // invalidates the L2 of every XCD, so every colour re-fetched its data.)
-    const int st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
+    int st = CDX_OK;
+    if (!nesterov) {
+        st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, sweep, &iters);
+    } else {
+        // (lambda_k, u_k) = (lambda_y, u_y) = the warm-started start; beta 0, thk 1, err_prev inf
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_xk1, (size_t)nb * kBodyStride, &ctx->xk1_cap, "gs nesterov u_k"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)std::max(ns, 1), &ctx->pk_cap, "gs nesterov lambda_k"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_nest, 8, nullptr, "nesterov scalars"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)gb, &ctx->partial2_cap, "gs nesterov partials"));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_xk1, ctx->d_x, sizeof(double) * (size_t)nb * kBodyStride, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        if (ns > 0)
+            CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk, lam_ns, sizeof(double) * 3 * (size_t)ns, hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+        const double n0[8] = {0.0, 1.0, INFINITY, 0.0, 0.0, 0.0, 0.0, 0.0};
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_nest, n0, sizeof(n0), hipMemcpyHostToDevice, ctx->stream));
+        auto nsweep = [&](int it, int) -> int {
+            CDX_LAUNCH(ctx, "gsn_extrapolate", k_gsn_extrapolate, grid_for(nb + ns), kBlock, nb, ns, ctx->d_x, ctx->d_xk1,
+                       ctx->d_uprev, lam_ns, ctx->d_pk, ctx->d_nest, ctx->d_ctrl);
+            for (int c = 0; c < ctx->ncolors; ++c) {
+                const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
+                if (n == 0) continue;
+                CDX_LAUNCH(ctx, "gs_color", k_gs_color4, grid_for(n * kLanesPerContact), kBlock, n, ctx->color_ptr[c], ns,
+                           ctx->d_gs_ids, ctx->d_gs_pack, ctx->d_minv, prm->relaxation, prm->alpha, lam_ns, ctx->d_x,
+                           ctx->d_ctrl);
+            }
+            CDX_LAUNCH(ctx, "gsn_residual", k_gsn_residual, gb, kBlock, nb, ctx->d_dof, ctx->d_ufree, ctx->d_x, ctx->d_uprev,
+                       ctx->d_xk1, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl);
+            CDX_LAUNCH(ctx, "gs_residual", k_gsn_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it,
+                       prm->nesterov_beta_threshold, prm->nesterov_restart_limit, ctx->d_ctrl, ctx->d_res,
+                       ctx->h_flag_dev, ctx->d_nest);
+            return hip_check(ctx, hipGetLastError(), "gs nesterov sweep launch");
+        };
+        st = run_sweeps_graphed(ctx, prm->max_iter, kGsChunk, true, nsweep, &iters);
+    }
     if (ns > 0)
         CDX_LAUNCH(ctx, "gs_unpack", k_gs_unpack, grid_for(ns), kBlock, ns, nc, ctx->d_color_blocks, ctx->d_gs_pack,
                    ctx->d_p);

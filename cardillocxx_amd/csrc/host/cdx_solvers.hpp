@@ -170,7 +170,9 @@ class GpuColoredGaussSeidelSolver : public GpuSolver {
     std::vector<double> solve(double dt, double theta) override {
         TimingManager::Scope sc = timed(name());
         upload();
-        const cdx_solver_params prm = m_cfg.params();
+        cdx_solver_params prm = m_cfg.params();
+        // ProjectedGaussSeidelSolver reads pj.nesterov only (projected_gauss_seidel.cpp:196)
+        prm.accel = m_cfg.pj_nesterov ? CDX_ACCEL_NESTEROV : CDX_ACCEL_NONE;
         const int st = cdx_gs_colored_solve(m_ctx, dt, theta, &prm, &m_info);
         if (st == CDX_DIVERGED) throw std::runtime_error("Projected Gauss-Seidel diverged");
         check(st, "cdx_gs_colored_solve");

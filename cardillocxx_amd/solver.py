@@ -113,7 +113,10 @@ class ProjectedJacobiSolver(_DeviceSolver):
 class ColoredGaussSeidelSolver(_DeviceSolver):
     def solve(self, dt: float, theta: float) -> np.ndarray:
         self._upload()
-        return self._finish(self.ctx.gs_colored_solve(dt, theta, self.cfg.params()))
+        p = self.cfg.params()
+        # ProjectedGaussSeidelSolver reads pj.nesterov only (projected_gauss_seidel.cpp:196)
+        p.accel = native.CDX_ACCEL_NESTEROV if self.cfg.pj_nesterov else native.CDX_ACCEL_NONE
+        return self._finish(self.ctx.gs_colored_solve(dt, theta, p))
 
     def name(self) -> str:
         return "ColoredGaussSeidel"
@@ -151,6 +154,7 @@ class MoreauIntegrator:
         if self.solver == "pj":
             info = self.ctx.pj_solve(dt, theta, prm)
         else:
+            prm.accel = native.CDX_ACCEL_NESTEROV if self.cfg.pj_nesterov else native.CDX_ACCEL_NONE
             info = self.ctx.gs_colored_solve(dt, theta, prm)
         con["imp"] = self.ctx.impulses()
         self.prev = con

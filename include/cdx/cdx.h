@@ -142,7 +142,10 @@ int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params*
 /* Graph-coloured projected Gauss-Seidel: the PGS block update
  * (projected_gauss_seidel.cpp:144-169) run colour by colour as in
  * CondensedSolver's coloredSweep (condensed_solver.cpp:220-251), Welsh-Powell colouring
- * (graph_coloring.cpp:8-41).  Residual every sweep, CDX_DIVERGED on NaN/Inf. */
+ * (graph_coloring.cpp:8-41).  Residual every sweep, CDX_DIVERGED on NaN/Inf.  prm->accel:
+ * CDX_ACCEL_NONE, or CDX_ACCEL_NESTEROV for the reference's Nesterov outer loop over (lambda,
+ * u_corr) with its restart rules (pj.nesterov, projected_gauss_seidel.cpp:196-262); the other
+ * modes are refused (CDX_BAD_ARG): the reference's PGS reads pj.nesterov only. */
 int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info);
 
 /* Outputs of the last solve.  v_out: nv stacked velocities (the SolverBase::solve return

@@ -1083,6 +1083,31 @@ static void pgs_project(double* lam, double* dl, double mu, int off, int n) {
  * with PgsAssembler::ufree/rhs/DinvDiag (pgs_assembler.cpp:12-82,220-288) for contact rows.
  * `colored` visits the blocks colour class by colour class (condensed_solver.cpp:220-251)
  * instead of in row order; the block update is unchanged. */
+/* pgs_sweep (projected_gauss_seidel.cpp:144-169): the blocks in `visit` order (row order, or
+ * colour by colour), each with r = rhs - W_j u_corr (C = 0 for contact rows), dl = (D^-1 r) *
+ * relaxation * alpha, lambda += dl, the projection with dl corrected, u_corr += M^-1 W_j^T dl. */
+static void pgs_sweep(const orc_W* W, const orc_bodies* B, const orc_params* prm, int nblk, const int* boff,
+                      const int* visit, const double* Dinv, const double* rhs, double* lam, double* dl, double* uc) {
+    for (int v = 0; v < nblk; ++v) {
+        const int off = boff[visit[v]];
+        const int bs = off < W->nnfc ? 1 : 3;
+        for (int k = 0; k < bs; ++k) {
+            const int r = off + k;
+            double s = 0;
+            for (int q = W->rowptr[r]; q < W->rowptr[r + 1]; ++q) s += W->val[q] * uc[W->col[q]];
+            const double rr = rhs[r] - s;
+            dl[r] = (Dinv[r] * rr) * prm->relaxation;
+        }
+        for (int k = 0; k < bs; ++k) dl[off + k] *= prm->alpha;
+        for (int k = 0; k < bs; ++k) lam[off + k] += dl[off + k];
+        pgs_project(lam, dl, W->mu[off], off, bs);
+        for (int k = 0; k < bs; ++k) {
+            const int r = off + k;
+            for (int q = W->rowptr[r]; q < W->rowptr[r + 1]; ++q) uc[W->col[q]] += (B->minv[W->col[q]] * W->val[q]) * dl[r];
+        }
+    }
+}
+
 static int pgs_core(const orc_bodies* B, const orc_contacts* C, double dt, const orc_params* prm, int colored,
                     double* v_out, double* imp_out, orc_result* res) {
     orc_W W;
@@ -1160,32 +1185,91 @@ static int pgs_core(const orc_bodies* B, const orc_contacts* C, double dt, const
         free(ba); free(bb); free(col);
     }
 
-    for (int iter = 0; iter < prm->max_iter; ++iter) {
-        /* pgs_sweep (:144-169) */
-        for (int v = 0; v < nblk; ++v) {
-            const int off = boff[visit[v]];
-            const int bs = off < W.nnfc ? 1 : 3;
-            for (int k = 0; k < bs; ++k) {
-                const int r = off + k;
-                double s = 0;
-                for (int q = W.rowptr[r]; q < W.rowptr[r + 1]; ++q) s += W.val[q] * uc[W.col[q]];
-                const double rr = rhs[r] - s; /* C = 0 for contact rows */
-                dl[r] = (Dinv[r] * rr) * prm->relaxation;
-            }
-            for (int k = 0; k < bs; ++k) dl[off + k] *= prm->alpha;
-            for (int k = 0; k < bs; ++k) lam[off + k] += dl[off + k];
-            pgs_project(lam, dl, W.mu[off], off, bs);
-            for (int k = 0; k < bs; ++k) {
-                const int r = off + k;
-                for (int q = W.rowptr[r]; q < W.rowptr[r + 1]; ++q) uc[W.col[q]] += (B->minv[W.col[q]] * W.val[q]) * dl[r];
-            }
+    if (!prm->nesterov) {
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            pgs_sweep(&W, B, prm, nblk, boff, visit, Dinv, rhs, lam, dl, uc);
+            res->iters = iter + 1;
+            const double e = gs_residual(nv, uf, uc, uk, prm->tol_rel, prm->tol_abs);
+            res->residual = e;
+            if (isnan(e) || isinf(e)) { st = ORC_DIVERGED; break; } /* :133-138 */
+            if (e < 1.0) break;
+            memcpy(uk, uc, sizeof(double) * (size_t)nv);
         }
-        res->iters = iter + 1;
-        const double e = gs_residual(nv, uf, uc, uk, prm->tol_rel, prm->tol_abs);
-        res->residual = e;
-        if (isnan(e) || isinf(e)) { st = ORC_DIVERGED; break; } /* :133-138 */
-        if (e < 1.0) break;
-        memcpy(uk, uc, sizeof(double) * (size_t)nv);
+    } else {
+        /* Nesterov outer loop (:196-262) on (lambda, u_corr); the divergence test of
+         * check_residual (:133-138) precedes the restart rules, so a non-finite residual ends the
+         * solve.  The ORC_MUT_NES_* macros plant the same mistakes as in the PJ loop. */
+        double* lk = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* ly = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* lk1 = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* uy = (double*)malloc(sizeof(double) * (size_t)nv);
+        double* uk1 = (double*)malloc(sizeof(double) * (size_t)nv);
+        memcpy(lk, lam, sizeof(double) * nr); memcpy(ly, lam, sizeof(double) * nr);
+        memcpy(uy, uc, sizeof(double) * nv); /* uk holds u_corr already */
+        double thk = 1.0, err_prev = INFINITY;
+        int restarts = 0, momentum_disabled = 0;
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(lam, ly, sizeof(double) * nr);
+            memcpy(uc, uy, sizeof(double) * nv);
+            pgs_sweep(&W, B, prm, nblk, boff, visit, Dinv, rhs, lam, dl, uc);
+            memcpy(lk1, lam, sizeof(double) * nr);
+            memcpy(uk1, uc, sizeof(double) * nv);
+            res->iters = iter + 1;
+            const double err = gs_residual(nv, uf, uy, uk1, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (isnan(err) || isinf(err)) { st = ORC_DIVERGED; break; }
+            if (err <= 1) break;
+#if defined(ORC_MUT_NES_THETA)
+            const double thk1 = 0.5 * (1.0 + sqrt(thk * thk + 1.0));
+#else
+            const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+#endif
+#if defined(ORC_MUT_NES_BETA)
+            const double betak1 = thk / thk1;
+#else
+            const double betak1 = (thk - 1.0) / thk1;
+#endif
+            int restart = 0;
+#if defined(ORC_MUT_NES_GROWTH)
+            if (err_prev < INFINITY && err > 1.5 * err_prev) restart = 1;
+#else
+            if (err_prev < INFINITY && err > 1.05 * err_prev) restart = 1;
+#endif
+            else if (betak1 > prm->nesterov_beta_threshold) restart = 1;
+            else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = 1;
+            else {
+                double dot = 0;
+                for (int j = 0; j < nv; ++j) dot += (uy[j] - uk1[j]) * (uk1[j] - uk[j]);
+#if defined(ORC_MUT_NES_DIRECTION)
+                if (dot < 0.0) restart = 1;
+#else
+                if (dot > 0.0) restart = 1;
+#endif
+            }
+            if (restart) {
+                memcpy(ly, lk1, sizeof(double) * nr); memcpy(uy, uk1, sizeof(double) * nv);
+                thk = 1.0;
+#if defined(ORC_MUT_NES_LIMIT)
+                if (++restarts > prm->nesterov_restart_limit) momentum_disabled = 1;
+#else
+                if (++restarts >= prm->nesterov_restart_limit) momentum_disabled = 1;
+#endif
+            } else if (momentum_disabled) {
+                memcpy(ly, lk1, sizeof(double) * nr); memcpy(uy, uk1, sizeof(double) * nv);
+                thk = 1.0;
+            } else {
+                for (int r = 0; r < nr; ++r) ly[r] = lk1[r] + betak1 * (lk1[r] - lk[r]);
+#if defined(ORC_MUT_NES_ANCHOR)
+                for (int j = 0; j < nv; ++j) uy[j] = uk1[j] + betak1 * (uk1[j] - uy[j]);
+#else
+                for (int j = 0; j < nv; ++j) uy[j] = uk1[j] + betak1 * (uk1[j] - uk[j]);
+#endif
+                thk = thk1;
+            }
+            memcpy(lk, lk1, sizeof(double) * nr); memcpy(uk, uk1, sizeof(double) * nv);
+            err_prev = err;
+        }
+        free(lk); free(ly); free(lk1); free(uy); free(uk1);
     }
     for (int ci = 0; ci < C->nc; ++ci) {
         double* o = imp_out + 3 * (size_t)ci;

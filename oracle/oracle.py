@@ -177,12 +177,12 @@ def pj_solve(prob, dt, prm, nthreads=1, L=None):
     return _solve("orc_pj_solve", prob, dt, prm, nthreads, L)
 
 
-def pgs_solve(prob, dt, prm):
-    return _solve("orc_pgs_solve", prob, dt, prm)
+def pgs_solve(prob, dt, prm, L=None):
+    return _solve("orc_pgs_solve", prob, dt, prm, L=L)
 
 
-def colored_gs_solve(prob, dt, prm):
-    return _solve("orc_colored_gs_solve", prob, dt, prm)
+def colored_gs_solve(prob, dt, prm, L=None):
+    return _solve("orc_colored_gs_solve", prob, dt, prm, L=L)
 
 
 def color_blocks(nb, body_a, body_b):
