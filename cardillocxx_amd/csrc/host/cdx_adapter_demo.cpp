@@ -5,23 +5,31 @@
 // of TimingManager::printBreakdown (TimingManager.hpp:107-163; host sections plus the device
 // kernels booked under theirs), and, when CARDILLO_DUMP_STATE is set, the reference's
 // "[STATE-DUMP] t= numBodies= totalKE= posNormSum=" line on stderr (examples/example_main.hpp:56-69).
+// With a VTK directory it writes the body and contact frames every `vtk_every` steps (VtkWriter,
+// src/io/vtk_writer.hpp:39-112) under the section OutputWrite.
 //
-// usage: cdx_adapter_demo [steps=200] [sweeps=100] [solver=pj|gs]
+// usage: cdx_adapter_demo [steps=200] [sweeps=100] [solver=pj|gs] [vtk_dir] [vtk_every=10]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <iomanip>
 #include <iostream>
 #include <vector>
 
 #include "cdx_scene.h"
 #include "cdx_solvers.hpp"
+#include "cdx_vtk.hpp"
 
 int main(int argc, char** argv) {
     const int steps = argc > 1 ? std::atoi(argv[1]) : 200;
     const int sweeps = argc > 2 ? std::atoi(argv[2]) : 100;
     const bool gs = argc > 3 && std::strcmp(argv[3], "gs") == 0;
+    const std::string vtk_dir = argc > 4 ? argv[4] : "";
+    const int vtk_every = argc > 5 ? std::atoi(argv[5]) : 10;
+    if (!vtk_dir.empty()) std::filesystem::create_directories(vtk_dir);
+    cdx::VtkWriter vtk(vtk_dir, "spheres", vtk_every);
     const int n = 1000;
     const double r = 0.05, m = 1.0, mu = 0.5, h = 1e-3;
     std::vector<double> q(7 * (size_t)n);
@@ -103,6 +111,13 @@ int main(int argc, char** argv) {
         }
         t += h;
         P.v = v;
+        if (!vtk_dir.empty()) {
+            auto sc = timings.scope("OutputWrite");
+            if (!vtk.maybeWrite(s, t, P, radius, ppt)) {
+                std::fprintf(stderr, "vtk write failed in %s\n", vtk_dir.c_str());
+                return 1;
+            }
+        }
         double ke = 0;
         for (int i = 0; i < 6 * n; ++i) ke += 0.5 * v[i] * v[i] / P.minv[i];
         if (s % 20 == 0 || s == steps - 1)
