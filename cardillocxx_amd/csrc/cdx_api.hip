@@ -214,7 +214,7 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jDinv, ctx->d_jtr, ctx->d_sc_off, ctx->d_sc_list, ctx->d_adj_jl, ctx->d_jr_blk, ctx->d_jy, ctx->d_gm, ctx->d_gm_part, ctx->d_jr_info, ctx->d_jbody_info,
                     ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
                     ctx->d_gs_pack, ctx->d_gs_ids, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc,
-                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part};
+                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);

@@ -181,6 +181,8 @@ struct cdx_ctx {
     double* d_pk1 = nullptr;       // Chebyshev: impulse sweep input [3*nc]
     size_t pk1_cap = 0;
     double* d_cheb = nullptr;      // Chebyshev scalars: w, take-swept, rho^2, rho, stopped, lambda, norm
+    double* d_accj = nullptr;      // accelerated loops with joint rows: K, K1, Y (twists, Lambda, impulses)
+    size_t accj_cap = 0;
     int* h_flag = nullptr;         // pinned mirror of the done flag
     int* h_flag_dev = nullptr;
     bool solved = false;

@@ -1055,6 +1055,140 @@ __global__ void k_pjc_finish_contacts(int nc, const int* __restrict__ ida, const
 }
 
 // ---------------------------------------------------------------------------------------------
+// Nesterov and Chebyshev loops with joint rows (nesterov_loop, chebyshev_loop and
+// estimateSpectralRadius, projected_jacobi.cpp:155-228,379-449, over x = [v; Lambda]).
+//
+// The sweep is the standard one (contact pass, body pass, chain solve of the jointed bodies and
+// Lambda); these kernels keep the loops' state around it.  The state has three segments: the
+// padded twists (nb x 8), Lambda (one per joint row) and the impulses (3 nc).  C is the working
+// copy the sweep updates in place (d_x, d_jlam, d_p), K and K1 the loops' older iterates.
+//   Nesterov: after sweep i, C = x_{k+1}; the update forms y = C + beta (C - K) (y = C when beta
+//   is 0: a restart, momentum off, or the first sweep), K <- C, C <- y, and keeps y's twists
+//   for the next residual.
+//   Chebyshev: K = x_k, K1 = x_{k+1} (the sweep input), C = sweep(K1); the update forms
+//   x_{k+2} = w (C - K) + K (C after a fallback), K <- K1, K1 <- x_{k+2}, C <- x_{k+2}.
+// The residual and the restart direction test run over the velocities, the extrapolations over
+// everything, as in the reference.
+// ---------------------------------------------------------------------------------------------
+
+// Residual partials between C and the sweep input IN over the velocities, and per mode the restart
+// dot (IN - C) . (C - K) (0) or the count of non-finite entries of C and of the joint impulses
+// (1: Chebyshev's fallback test on the whole of x).
+__global__ void __launch_bounds__(kBlock) k_accj_residual(int mode, int nb, const int* __restrict__ dof,
+                                                          const double* __restrict__ C, const double* __restrict__ IN,
+                                                          const double* __restrict__ K, int nj,
+                                                          const double* __restrict__ LC, double tol_rel, double tol_abs,
+                                                          double* __restrict__ partial, const int* __restrict__ ctrl) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    double e2 = 0, aux = 0;
+    if (b < nb) {
+        double c[6], y[6], k6[6];
+        load_body(C, b, c);
+        load_body(IN, b, y);
+        load_body(K, b, k6);
+        for (int m = 0; m < dof[b]; ++m) {
+            const double q = fmax(fabs(c[m]), fabs(y[m])) * tol_rel + tol_abs;
+            const double e = (c[m] - y[m]) / q;
+            e2 += e * e;
+            if (mode == 0) aux += (y[m] - c[m]) * (c[m] - k6[m]);
+            else if (!isfinite(c[m])) aux += 1.0;
+        }
+    }
+    if (mode == 1)
+        for (int r = b; r < nj; r += gridDim.x * kBlock)
+            if (!isfinite(LC[r])) aux += 1.0;
+    const double s = block_sum(e2, red);
+    const double d = block_sum(aux, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = d;
+    }
+}
+
+// The loop update over the three segments (lengths n0, n1, n2).  mode 0 (Nesterov): s = nest
+// (s[0] = beta), A = the kept sweep input (twists only, A0).  mode 1 (Chebyshev): s = cheb
+// (s[0] = w, s[1] = take the swept iterate), A = K1.
+__global__ void k_accj_update(int mode, int n0, int n1, int n2, double* __restrict__ C0, double* __restrict__ C1,
+                              double* __restrict__ C2, double* __restrict__ K0, double* __restrict__ K1,
+                              double* __restrict__ K2, double* __restrict__ A0, double* __restrict__ A1,
+                              double* __restrict__ A2, const double* __restrict__ s, const int* __restrict__ ctrl) {
+#pragma clang fp contract(off)
+    if (ctrl[0]) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double *c, *k, *a;
+    if (t < n0) {
+        c = C0; k = K0; a = A0;
+    } else if ((t -= n0) < n1) {
+        c = C1; k = K1; a = A1;
+    } else if ((t -= n1) < n2) {
+        c = C2; k = K2; a = A2;
+    } else {
+        return;
+    }
+    const double cv = c[t], kv = k[t];
+    if (mode == 0) {
+        const double beta = s[0];
+        const double y = beta == 0.0 ? cv : cv + beta * (cv - kv);
+        k[t] = cv;
+        c[t] = y;
+        if (a) a[t] = y;
+    } else {
+        const double n = s[1] != 0.0 ? cv : cheb_extrap(s[0], cv, kv);
+        k[t] = a[t];
+        a[t] = n;
+        c[t] = n;
+    }
+}
+
+// After the loop: Nesterov returns x_{k+1} (K after an update, so when the loop ran out of
+// sweeps), Chebyshev the sweep input K1.
+__global__ void k_accj_finish(int mode, int n0, int n1, int n2, double* __restrict__ C0, double* __restrict__ C1,
+                              double* __restrict__ C2, const double* __restrict__ S0, const double* __restrict__ S1,
+                              const double* __restrict__ S2, const int* __restrict__ ctrl) {
+    if (mode == 0 && ctrl[0]) return;  // converged: the swept iterate is in place
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n0) C0[t] = S0[t];
+    else if ((t -= n0) < n1) C1[t] = S1[t];
+    else if ((t -= n1) < n2) C2[t] = S2[t];
+}
+
+// Power iteration with joint rows: the vector over the twists and Lambda.
+__global__ void k_accj_fill(int nb, const int* __restrict__ dof, int nj, double inv, double* __restrict__ vx,
+                            double* __restrict__ vl) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb * kBodyStride) {
+        const int bi = t / kBodyStride, l = t % kBodyStride;
+        vx[t] = l < dof[bi] ? inv : 0.0;
+    } else if (t - nb * kBodyStride < nj) {
+        vl[t - nb * kBodyStride] = inv;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_accj_sq(int nb, const int* __restrict__ dof, const double* __restrict__ vx,
+                                                    int nj, const double* __restrict__ vl, double* __restrict__ partial,
+                                                    const double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    double s2 = 0;
+    if (b < nb)
+        for (int m = 0; m < dof[b]; ++m) s2 += vx[(size_t)b * kBodyStride + m] * vx[(size_t)b * kBodyStride + m];
+    for (int r = b; r < nj; r += gridDim.x * kBlock) s2 += vl[r] * vl[r];
+    const double s = block_sum(s2, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_accj_scale(int nb, int nj, double* __restrict__ vx, double* __restrict__ vl,
+                             const double* __restrict__ cheb) {
+    if (cheb[4] != 0.0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb * kBodyStride) vx[t] = vx[t] / cheb[6];
+    else if (t - nb * kBodyStride < nj) vl[t - nb * kBodyStride] = vl[t - nb * kBodyStride] / cheb[6];
+}
+
+// ---------------------------------------------------------------------------------------------
 // Anderson mixing (anderson_loop, projected_jacobi.cpp:224-310).
 //
 // The mixed vector z = [x; p] over the entries the reference has: x (b, k) for k < dof[b] (entry
@@ -1326,6 +1460,7 @@ using namespace cdx;
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
 static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
 static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
+static int run_accel_joints(cdx_ctx* ctx, const cdx_solver_params* prm, int mode, int* iters_out);
 
 extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info) {
     // theta only enters the joint rows of S (PjAssembler::buildAndFactorS / rhs)
@@ -1339,8 +1474,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: anderson_m must be in 1..8");
     const bool joints = ctx->njrows > 0;
     const bool block = prm->block_precond != 0;
-    if (joints && prm->accel != CDX_ACCEL_NONE)
-        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the accelerated loop does not take joint rows");
+    if (joints && prm->accel == CDX_ACCEL_ANDERSON)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the Anderson loop does not take joint rows");
     if (joints && block)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the block preconditioner does not take joint rows");
     if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: theta must be positive with joint rows");
@@ -1397,7 +1532,9 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb, &ctx->partial_cap, "residual partials"));
     int iters = 0;
     int st = CDX_OK;
-    if (prm->accel == CDX_ACCEL_NESTEROV) {
+    if (joints && (prm->accel == CDX_ACCEL_NESTEROV || prm->accel == CDX_ACCEL_CHEBYSHEV)) {
+        st = run_accel_joints(ctx, prm, prm->accel == CDX_ACCEL_NESTEROV ? 0 : 1, &iters);
+    } else if (prm->accel == CDX_ACCEL_NESTEROV) {
         // xk = x0, pk = p0, beta = 0, thk = 1, err_prev = inf, restarts = 0, momentum on
         CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)nc, &ctx->pk_cap, "nesterov pk"));
         CDX_TRY(ensure_alloc(ctx, &ctx->d_nest, 8, nullptr, "nesterov scalars"));
@@ -1629,6 +1766,84 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
     // one chunk = the sweeps between two residual checks
     return run_sweeps_graphed(ctx, prm->max_iter, check_every, true,
                               [&](int it, int pos) { return launch_sweep(it, pos == 0); }, iters_out);
+}
+
+// Nesterov (mode 0) or Chebyshev (mode 1) loop with joint rows over x = [v; Lambda]: the
+// standard sweep (contact pass, body pass, chain solve) around the k_accj_* state kernels.
+// Chebyshev's spectral radius: 8 power iterations of v -> v + S^-1 [W^T dp; 0], dp = -R W v_head,
+// over the twists and Lambda, from ones / sqrt(nv + joint rows).
+static int run_accel_joints(cdx_ctx* ctx, const cdx_solver_params* prm, int mode, int* iters_out) {
+    const int nb = ctx->nb, nc = ctx->nc, nj = ctx->njrows;
+    const int n0 = nb * kBodyStride, n1 = nj, n2 = 3 * nc, m1 = std::max(n1, 1);
+    const int gb = grid_for(nb), gb8 = grid_for(nb * kLanesPerBody), gc4 = grid_for(nc * kLanesPerContact);
+    const int gn = grid_for(n0 + n1 + n2), gv = grid_for(n0 + n1);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_accj, 3 * (size_t)n0 + 3 * (size_t)m1 + 2 * (size_t)n2, &ctx->accj_cap,
+                         "accelerated loop state"));
+    double* Kx = ctx->d_accj;
+    double* K1x = Kx + n0;
+    double* Yx = K1x + n0;  // Nesterov: the sweep input's twists; Chebyshev: the power-iteration vector
+    double* Kl = Yx + n0;
+    double* K1l = Kl + m1;
+    double* Vl = K1l + m1;
+    double* Kp = Vl + m1;
+    double* K1p = Kp + n2;
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)gb, &ctx->partial2_cap, "accelerated loop partials"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)(gb8 + ctx->nchains), &ctx->partial_cap, "residual partials"));
+    const size_t bx = sizeof(double) * (size_t)n0, bl = sizeof(double) * (size_t)n1, bp = sizeof(double) * (size_t)n2;
+    for (double* d : {Kx, K1x, Yx}) CDX_HIP(ctx, hipMemcpyAsync(d, ctx->d_x, bx, hipMemcpyDeviceToDevice, ctx->stream));
+    if (n1)
+        for (double* d : {Kl, K1l}) CDX_HIP(ctx, hipMemcpyAsync(d, ctx->d_jlam, bl, hipMemcpyDeviceToDevice, ctx->stream));
+    for (double* d : {Kp, K1p}) CDX_HIP(ctx, hipMemcpyAsync(d, ctx->d_p, bp, hipMemcpyDeviceToDevice, ctx->stream));
+    double* st_scalars = nullptr;
+    if (mode == 0) {
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_nest, 8, nullptr, "nesterov scalars"));
+        const double n0s[8] = {0.0, 1.0, INFINITY, 0.0, 0.0, 0.0, 0.0, 0.0};
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_nest, n0s, sizeof(n0s), hipMemcpyHostToDevice, ctx->stream));
+        st_scalars = ctx->d_nest;
+    } else {
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_cheb, 8, nullptr, "chebyshev scalars"));
+        const double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_cheb, z, sizeof(z), hipMemcpyHostToDevice, ctx->stream));
+        st_scalars = ctx->d_cheb;
+        CDX_LAUNCH(ctx, "cheb_power", k_accj_fill, gv, kBlock, nb, ctx->d_dof, nj, 1.0 / std::sqrt((double)(ctx->nv + nj)), Yx,
+                   Vl);
+        for (int k = 0; k < 8; ++k) {
+            CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                       ctx->d_R, ctx->d_mu, Yx, ctx->d_imp, ctx->d_cheb, nullptr);
+            CDX_LAUNCH(ctx, "cheb_power", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
+                       ctx->d_incidx, ctx->d_imp, Yx, 0, 0.0, 0.0, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
+            CDX_TRY(joints_apply(ctx, Yx, Vl, ctx->d_ctrl));
+            CDX_LAUNCH(ctx, "cheb_power", k_accj_sq, gb, kBlock, nb, ctx->d_dof, Yx, nj, Vl, ctx->d_partial2, ctx->d_cheb);
+            CDX_LAUNCH(ctx, "cheb_power", k_cheb_norm, 1, kBlock, gb, ctx->d_partial2, ctx->d_cheb);
+            CDX_LAUNCH(ctx, "cheb_power", k_accj_scale, gv, kBlock, nb, nj, Yx, Vl, ctx->d_cheb);
+        }
+        CDX_LAUNCH(ctx, "cheb_power", k_cheb_start, 1, 1, ctx->d_cheb);
+    }
+    auto sweep = [&](int it) -> int {
+        CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<false>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
+                   ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
+        CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
+                   ctx->d_imp, ctx->d_x, 0, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
+        CDX_TRY(joints_solve(ctx, 0, 0, prm->tol_rel, prm->tol_abs, nullptr));
+        CDX_LAUNCH(ctx, "accj_residual", k_accj_residual, gb, kBlock, mode, nb, ctx->d_dof, ctx->d_x, mode == 0 ? Yx : K1x,
+                   Kx, nj, ctx->d_jlam, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl);
+        if (mode == 0)
+            CDX_LAUNCH(ctx, "pj_residual", k_pjn_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it,
+                       prm->nesterov_beta_threshold, prm->nesterov_restart_limit, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev,
+                       ctx->d_nest);
+        else
+            CDX_LAUNCH(ctx, "pjc_residual", k_pjc_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it, ctx->d_ctrl,
+                       ctx->d_res, ctx->h_flag_dev, ctx->d_cheb);
+        CDX_LAUNCH(ctx, "accj_update", k_accj_update, gn, kBlock, mode, n0, n1, n2, ctx->d_x, ctx->d_jlam, ctx->d_p, Kx, Kl,
+                   Kp, mode == 0 ? Yx : K1x, mode == 0 ? nullptr : K1l, mode == 0 ? nullptr : K1p, st_scalars, ctx->d_ctrl);
+        return hip_check(ctx, hipGetLastError(), "pj accelerated joint sweep launch");
+    };
+    const int st = run_sweeps(ctx, prm->max_iter, 10, 4, sweep, iters_out);
+    if (st) return st;
+    CDX_LAUNCH(ctx, "accj_finish", k_accj_finish, gn, kBlock, mode, n0, n1, n2, ctx->d_x, ctx->d_jlam, ctx->d_p,
+               mode == 0 ? Kx : K1x, mode == 0 ? Kl : K1l, mode == 0 ? Kp : K1p, ctx->d_ctrl);
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return hip_check(ctx, hipGetLastError(), "pj accelerated joint finish");
 }
 
 // chebyshev_loop with estimateSpectralRadius (projected_jacobi.cpp:155-228, 470-476): buffers

@@ -136,7 +136,9 @@ int cdx_get_joint_impulses(cdx_ctx* ctx, double* lam_out);
 /* Projected Jacobi: replaces ProjectedJacobiSolver::solve (projected_jacobi.cpp:451-500).
  * Results stay on the device: fetch with cdx_get_velocities / cdx_get_impulses.
  * The new impulses also become the next warm start (storeImpulse, warmstart.hpp:41-56).
- * With joint rows (cdx_set_joints) x = [v; Lambda] solves the extended system. */
+ * With joint rows (cdx_set_joints) x = [v; Lambda] solves the extended system, in the standard,
+ * Nesterov and Chebyshev loops (extrapolating the whole of x, testing the residual on v, as the
+ * reference does); Anderson and the block preconditioner are refused with joint rows. */
 int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_params* prm, cdx_solve_info* info);
 
 /* Graph-coloured projected Gauss-Seidel: the PGS block update

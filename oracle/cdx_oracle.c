@@ -915,7 +915,9 @@ static double joint_row_dot(const orc_joints* jt, const int* voff, const int* do
 
 int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
                         const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
-    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->nesterov || prm->chebyshev || prm->anderson || prm->block_precond)
+    /* the loops of ProjectedJacobiSolver::solve (:469-476) over x = [v; Lambda]: Nesterov before
+     * Chebyshev; Anderson and the block preconditioner are not restated with joint rows */
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->block_precond || (prm->anderson && !prm->nesterov && !prm->chebyshev))
         return ORC_BAD_ARG;
     orc_W W;
     int st = orc_build_W(B, C, &W);
@@ -1006,26 +1008,140 @@ int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_jo
     lu_solve(&lu, rhs, x);
     orc_W RW = W;
     RW.val = RWv;
-    /* standard_loop with pj_sweep (:121-138): x += S^-1 [W^T dp; 0] */
-    for (int iter = 0; iter < prm->max_iter; ++iter) {
-        memcpy(xp, x, sizeof(double) * (size_t)nv);
-        memcpy(pp, p, sizeof(double) * (size_t)nr);
-        spmv(&RW, x, dl);
-        for (int r = 0; r < nr; ++r) dl[r] += bias[r];
-        for (int r = 0; r < nr; ++r) p[r] -= dl[r];
-        pj_project(&W, p, 1);
-        for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];
-        memset(d, 0, sizeof(double) * (size_t)N);
-        spmv_t_acc(&W, dl, d);
-        lu_solve(&lu, d, dx);
-        for (int k = 0; k < N; ++k) x[k] += dx[k];
-        res->iters = iter + 1;
-        if (iter % 10 == 0) {
-            const double e = pj_residual(nv, x, xp, prm->tol_rel, prm->tol_abs);
-            res->residual = e;
-            if (e <= 1) break;
+    /* pj_sweep (:121-138) on the extended state: p from the velocity part, x += S^-1 [W^T dp; 0] */
+#define JSWEEP()                                                        \
+    do {                                                                \
+        memcpy(pp, p, sizeof(double) * (size_t)nr);                     \
+        spmv(&RW, x, dl);                                               \
+        for (int r = 0; r < nr; ++r) dl[r] += bias[r];                  \
+        for (int r = 0; r < nr; ++r) p[r] -= dl[r];                     \
+        pj_project(&W, p, 1);                                           \
+        for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];              \
+        memset(d, 0, sizeof(double) * (size_t)N);                       \
+        spmv_t_acc(&W, dl, d);                                          \
+        lu_solve(&lu, d, dx);                                           \
+        for (int k = 0; k < N; ++k) x[k] += dx[k];                      \
+    } while (0)
+    if (prm->nesterov) {
+        /* nesterov_loop (:379-449) with x = [v; Lambda]: extrapolation over the whole of x, the
+         * residual and the restart direction test over the velocities */
+        double* xk = (double*)malloc(sizeof(double) * (size_t)N);
+        double* pk = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xy = (double*)malloc(sizeof(double) * (size_t)N);
+        double* py = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xk1 = (double*)malloc(sizeof(double) * (size_t)N);
+        double* pk1 = (double*)malloc(sizeof(double) * (size_t)nr);
+        memcpy(xk, x, sizeof(double) * N); memcpy(xy, x, sizeof(double) * N); memcpy(xk1, x, sizeof(double) * N);
+        memcpy(pk, p, sizeof(double) * nr); memcpy(py, p, sizeof(double) * nr); memcpy(pk1, p, sizeof(double) * nr);
+        double thk = 1.0, err_prev = INFINITY;
+        int restarts = 0, momentum_disabled = 0;
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(x, xy, sizeof(double) * N);
+            memcpy(p, py, sizeof(double) * nr);
+            JSWEEP();
+            memcpy(xk1, x, sizeof(double) * N);
+            memcpy(pk1, p, sizeof(double) * nr);
+            res->iters = iter + 1;
+            const double err = pj_residual(nv, xk1, xy, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (err <= 1) break;
+            const double thk1 = 0.5 * (1.0 + sqrt(4.0 * thk * thk + 1.0));
+            const double betak1 = (thk - 1.0) / thk1;
+            int restart = 0;
+            if (!isfinite(err)) restart = 1;
+            else if (err_prev < INFINITY && err > 1.05 * err_prev) restart = 1;
+            else if (betak1 > prm->nesterov_beta_threshold) restart = 1;
+            else if (!isfinite(betak1) || betak1 < 0.0 || betak1 > 1.0) restart = 1;
+            else {
+                double dot = 0;
+                for (int j = 0; j < nv; ++j) dot += (xy[j] - xk1[j]) * (xk1[j] - xk[j]);
+                if (dot > 0.0) restart = 1;
+            }
+            if (restart || momentum_disabled) {
+                memcpy(xy, xk1, sizeof(double) * N); memcpy(py, pk1, sizeof(double) * nr);
+                thk = 1.0;
+                if (restart && ++restarts >= prm->nesterov_restart_limit) momentum_disabled = 1;
+            } else {
+                for (int k = 0; k < N; ++k) xy[k] = xk1[k] + betak1 * (xk1[k] - xk[k]);
+                for (int r = 0; r < nr; ++r) py[r] = pk1[r] + betak1 * (pk1[r] - pk[r]);
+                thk = thk1;
+            }
+            memcpy(xk, xk1, sizeof(double) * N); memcpy(pk, pk1, sizeof(double) * nr);
+            err_prev = err;
+        }
+        memcpy(x, xk1, sizeof(double) * N);
+        memcpy(p, pk1, sizeof(double) * nr);
+        free(xk); free(pk); free(xy); free(py); free(xk1); free(pk1);
+    } else if (prm->chebyshev) {
+        /* estimateSpectralRadius (:155-178) of v -> v - S^-1 [W^T R W v_head; 0] over the full
+         * length of x, then chebyshev_loop (:180-228) with x = [v; Lambda] */
+        double* v = (double*)malloc(sizeof(double) * ((size_t)N + 1));
+        const double inv = 1.0 / sqrt((double)N);
+        for (int k = 0; k < N; ++k) v[k] = inv;
+        double lambda = 0.0;
+        for (int it = 0; it < 8; ++it) {
+            spmv(&RW, v, dl);
+            memset(d, 0, sizeof(double) * (size_t)N);
+            spmv_t_acc(&W, dl, d);
+            lu_solve(&lu, d, dx);
+            double n2 = 0.0;
+            for (int k = 0; k < N; ++k) {
+                v[k] = v[k] - dx[k];
+                n2 += v[k] * v[k];
+            }
+            const double n = sqrt(n2);
+            if (!(n > 1e-300)) { lambda = 0.0; break; }
+            lambda = n;
+            for (int k = 0; k < N; ++k) v[k] = v[k] / n;
+        }
+        free(v);
+        const double rho = dmin(lambda, 0.995), rho2 = rho * rho;
+        res->rho = rho;
+        double* xk = (double*)malloc(sizeof(double) * (size_t)N);
+        double* pk = (double*)malloc(sizeof(double) * (size_t)nr);
+        double* xk1 = (double*)malloc(sizeof(double) * (size_t)N);
+        double* pk1 = (double*)malloc(sizeof(double) * (size_t)nr);
+        memcpy(xk, x, sizeof(double) * N); memcpy(xk1, x, sizeof(double) * N);
+        memcpy(pk, p, sizeof(double) * nr); memcpy(pk1, p, sizeof(double) * nr);
+        double omega = 1.0;
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(x, xk1, sizeof(double) * N);
+            memcpy(p, pk1, sizeof(double) * nr);
+            JSWEEP();
+            res->iters = iter + 1;
+            const double err = pj_residual(nv, x, xk1, prm->tol_rel, prm->tol_abs);
+            res->residual = err;
+            if (err <= 1) break;
+            double omega_next = (iter == 0) ? (1.0 / (1.0 - rho2 * 0.5)) : (1.0 / (1.0 - (rho2 * 0.25) * omega));
+            int finite = 1;
+            for (int k = 0; k < N && finite; ++k) finite = isfinite(x[k]);
+            if (!isfinite(omega_next) || omega_next <= 0.0 || !finite) {
+                memcpy(xk, xk1, sizeof(double) * N); memcpy(pk, pk1, sizeof(double) * nr);
+                memcpy(xk1, x, sizeof(double) * N); memcpy(pk1, p, sizeof(double) * nr);
+                omega = 1.0;
+            } else {
+                for (int k = 0; k < N; ++k) { const double t = omega_next * (x[k] - xk[k]) + xk[k]; xk[k] = xk1[k]; xk1[k] = t; }
+                for (int r = 0; r < nr; ++r) { const double t = omega_next * (p[r] - pk[r]) + pk[r]; pk[r] = pk1[r]; pk1[r] = t; }
+                omega = omega_next;
+            }
+        }
+        memcpy(x, xk1, sizeof(double) * N);
+        memcpy(p, pk1, sizeof(double) * nr);
+        free(xk); free(pk); free(xk1); free(pk1);
+    } else {
+        /* standard_loop (:371-377) */
+        for (int iter = 0; iter < prm->max_iter; ++iter) {
+            memcpy(xp, x, sizeof(double) * (size_t)nv);
+            JSWEEP();
+            res->iters = iter + 1;
+            if (iter % 10 == 0) {
+                const double e = pj_residual(nv, x, xp, prm->tol_rel, prm->tol_abs);
+                res->residual = e;
+                if (e <= 1) break;
+            }
         }
     }
+#undef JSWEEP
     for (int ci = 0; ci < C->nc; ++ci) {
         double* o = imp_out + 3 * (size_t)ci;
         o[0] = o[1] = o[2] = 0;

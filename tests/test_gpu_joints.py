@@ -169,9 +169,9 @@ def test_joint_errors(ctx):
     ctx.set_joints(rows)
     with pytest.raises(native.CdxError):
         ctx.gs_colored_solve(1e-3, 1.0, params())
-    for accel in (native.CDX_ACCEL_NESTEROV, native.CDX_ACCEL_CHEBYSHEV):
+    for kw in (dict(accel=native.CDX_ACCEL_ANDERSON), dict(block_precond=1)):
         with pytest.raises(native.CdxError):
-            ctx.pj_solve(1e-3, 1.0, params(accel=accel))
+            ctx.pj_solve(1e-3, 1.0, params(**kw))
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
     info = ctx.pj_solve(1e-3, 1.0, params())
