@@ -1362,6 +1362,8 @@ int cdx_pj_adjoint(cdx_ctx* ctx, double dt, const cdx_solver_params* prm, const 
     if (!ctx || !prm || !aprm || !g_v) return CDX_BAD_ARG;
     hipSetDevice(ctx->device);
     if (!ctx->solved || !ctx->pj_last) return set_err(ctx, CDX_NO_STATE, "cdx_pj_adjoint: needs a preceding cdx_pj_solve");
+    if (ctx->gyro_last || prm->implicit_gyro)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_adjoint: differentiates the explicit-gyroscopy solve only");
     if (aprm->mode < 0 || aprm->mode > 2 || aprm->max_iter < 1) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_adjoint: bad params");
     if (info) {
         info->iters = 0;

@@ -176,6 +176,7 @@ void cdx_default_params(cdx_solver_params* p) {
     p->nesterov_restart_limit = 10000;
     p->block_precond = 0;
     p->anderson_m = 2;
+    p->implicit_gyro = 0;
 }
 
 int cdx_ctx_create(int device, cdx_ctx** out) {
@@ -214,7 +215,7 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jDinv, ctx->d_jtr, ctx->d_sc_off, ctx->d_sc_list, ctx->d_adj_jl, ctx->d_jr_blk, ctx->d_jy, ctx->d_gm, ctx->d_gm_part, ctx->d_jr_info, ctx->d_jbody_info,
                     ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
                     ctx->d_gs_pack, ctx->d_gs_ids, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc,
-                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj};
+                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj, ctx->d_sinv};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);

@@ -822,6 +822,8 @@ int cdx_batch_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
     if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: theta must be positive with joint rows");
     if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_CHEBYSHEV)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve runs the standard or the Chebyshev loop");
+    if (prm->implicit_gyro)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched solve takes explicit gyroscopy only");
     if (prm->accel == CDX_ACCEL_CHEBYSHEV && ctx->njrows > 0)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_pj_solve: the batched Chebyshev loop does not take joint rows");
     if (prm->block_precond && ctx->njrows > 0)
@@ -867,6 +869,8 @@ int cdx_batch_autotune(cdx_ctx* ctx, double dt, double theta, const cdx_solver_p
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the block preconditioner does not take joint rows");
     if (prm->accel != CDX_ACCEL_NONE && prm->accel != CDX_ACCEL_CHEBYSHEV)
         return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve runs the standard or the Chebyshev loop");
+    if (prm->implicit_gyro)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: the batched solve takes explicit gyroscopy only");
     if (ctx->nscenes < 1) return set_err(ctx, CDX_NO_STATE, "cdx_batch_autotune: scenes not set");
     if (ctx->njrows > 0 && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_batch_autotune: theta must be positive with joint rows");
     CDX_TRY(build_incidence(ctx));

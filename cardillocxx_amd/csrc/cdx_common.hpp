@@ -145,6 +145,7 @@ struct cdx_ctx {
     // ---- differentiation
     bool pj_last = false;  // the last solve was cdx_pj_solve (adjoint available)
     bool pjj_last = false;  // the last solve was a standard-loop joint solve (cdx_pj_adjoint_joints available)
+    bool gyro_last = false;  // the last PJ solve ran with implicit gyroscopy (no adjoint)
     double last_theta = 1.0;
     bool record = false, record_valid = false;
     int record_max = 0, recorded = 0;
@@ -183,6 +184,8 @@ struct cdx_ctx {
     double* d_cheb = nullptr;      // Chebyshev scalars: w, take-swept, rho^2, rho, stopped, lambda, norm
     double* d_accj = nullptr;      // accelerated loops with joint rows: K, K1, Y (twists, Lambda, impulses)
     size_t accj_cap = 0;
+    double* d_sinv = nullptr;      // implicit gyroscopy: inverse rotational blocks of S [nb*9]
+    size_t sinv_cap = 0;
     int* h_flag = nullptr;         // pinned mirror of the done flag
     int* h_flag_dev = nullptr;
     bool solved = false;

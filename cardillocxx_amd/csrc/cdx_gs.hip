@@ -94,7 +94,8 @@ static int build_coloring(cdx_ctx* ctx) {
     return CDX_OK;
 }
 
-// u_free = v + M^-1 (dt f_ext) + M^-1 (dt f_gyro) (pgs_assembler.cpp:75-82).
+// u_free = v + M^-1 (dt f_ext) + M^-1 (dt f_gyro) (pgs_assembler.cpp:75-82); fgyro = nullptr with
+// implicit gyroscopy, whose only effect on the Gauss-Seidel solve is to drop that term (:33-36).
 __global__ void k_gs_ufree(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
                            const double* __restrict__ v, const double* __restrict__ fext,
                            const double* __restrict__ fgyro, double dt, double* __restrict__ uf) {
@@ -102,13 +103,13 @@ __global__ void k_gs_ufree(int nb, const int* __restrict__ dof, const double* __
     if (b >= nb) return;
     const double* vv = v + (size_t)b * kBodyStride;
     const double* fe = fext + (size_t)b * kBodyStride;
-    const double* fg = fgyro + (size_t)b * kBodyStride;
+    const double* fg = fgyro ? fgyro + (size_t)b * kBodyStride : nullptr;
     const double* mi = minv + (size_t)b * kMinvStride;
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < dof[b]; ++k) {
         const double m = k < 3 ? mi[0] : mi[k - 2];
         double u = vv[k] + m * (dt * fe[k]);
-        u += m * (dt * fg[k]);
+        if (fg) u += m * (dt * fg[k]);
         o[k] = u;
     }
     store6(uf + (size_t)b * kBodyStride, o);
@@ -532,8 +533,8 @@ extern "C" int cdx_gs_colored_solve(cdx_ctx* ctx, double dt, double theta, const
     CDX_TRY(build_incidence(ctx));
     CDX_TRY(reset_control(ctx));
     const int gb = grid_for(nb);
-    CDX_LAUNCH(ctx, "gs_ufree", k_gs_ufree, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro, dt,
-               ctx->d_ufree);
+    CDX_LAUNCH(ctx, "gs_ufree", k_gs_ufree, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext,
+               prm->implicit_gyro ? nullptr : ctx->d_fgyro, dt, ctx->d_ufree);
     if (ctx->nrows == 0) {  // numLambda == 0: return u_free (projected_gauss_seidel.cpp:73)
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_x, ctx->d_ufree, sizeof(double) * (size_t)nb * kBodyStride, hipMemcpyDeviceToDevice,
                                     ctx->stream));

@@ -50,10 +50,28 @@ __global__ void k_pj_setup_bodies(int nb, const int* __restrict__ dof, const dou
                                   const double* __restrict__ fgyro, const int* __restrict__ incptr,
                                   const int* __restrict__ incidx, const double* __restrict__ imp, double dt,
                                   int have_contacts, double* __restrict__ x, const int* __restrict__ jslot,
-                                  double* __restrict__ tr) {
+                                  double* __restrict__ tr, const double* __restrict__ sinv) {
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= nb) return;
-    pj_setup_body(bi, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, have_contacts, x, jslot, tr);
+    pj_setup_body(bi, dof, minv, v, fext, fgyro, incptr, incidx, imp, dt, have_contacts, x, jslot, tr, sinv);
+}
+
+// Implicit gyroscopy: the inverse rotational block of S of every body (zero for bodies without 6
+// DOFs), one body per thread; a singular or non-finite block stops the solve as not factorized
+// (the reference's SparseLU of S fails).
+__global__ void k_pj_gyro_blocks(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
+                                 const double* __restrict__ v, double dt, double* __restrict__ sinv,
+                                 int* __restrict__ ctrl) {
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= nb) return;
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (dof[bi] == 6 &&
+        !gyro_block_inverse(minv + (size_t)bi * kMinvStride, v + (size_t)bi * kBodyStride + 3, dt, K)) {
+        ctrl[2] = CDX_NOT_FACTORIZED;
+        ctrl[0] = 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sinv[(size_t)bi * 9 + k] = K[k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -191,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
                                                        const double* __restrict__ imp9, double* __restrict__ x, int check,
                                                        double tol_rel, double tol_abs, double* __restrict__ partial,
                                                        const int* __restrict__ ctrl, const int* __restrict__ jslot,
-                                                       double* __restrict__ tr) {
+                                                       double* __restrict__ tr, const double* __restrict__ sinv) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     // XCD-aware tiles, as the contact pass: a contiguous range of bodies per XCD gathers the
@@ -227,7 +245,11 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies8(int nb, const int* __rest
             if (l < nd) tr[(size_t)bi * 6 + l] = a;  // W^T dp of a jointed body: the chain solve applies it
         } else if (l < nd) {
             const double M = mk > 0 ? 1.0 / mk : 0.0;
-            const double un = u + a / M;
+            // S^-1: a / M, or a row of the inverse rotational block (implicit gyroscopy)
+            const double d = sinv && nd == 6 && l >= 3
+                                 ? gyro_row(sinv + (size_t)bi * 9 + 3 * (l - 3), acc[3], acc[4], acc[5])
+                                 : a / M;
+            const double un = u + d;
             if (i1 > i0) x[(size_t)bi * kBodyStride + l] = un;
             if (check) {
                 const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
@@ -433,7 +455,8 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies_parts(int nb, const int* _
                                                             const int* __restrict__ pcnt, const int* __restrict__ povf,
                                                             const double* __restrict__ part, double* __restrict__ x,
                                                             int check, double tol_rel, double tol_abs,
-                                                            double* __restrict__ partial, const int* __restrict__ ctrl) {
+                                                            double* __restrict__ partial, const int* __restrict__ ctrl,
+                                                            const double* __restrict__ sinv) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
@@ -456,7 +479,20 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies_parts(int nb, const int* _
                 for (int j = kPartSlots; j < n; ++j) a += part[(o + j - kPartSlots) * 6 + l];
             }
             const double M = mk > 0 ? 1.0 / mk : 0.0;
-            const double un = u + a / M;
+            double d = a / M;
+            if (sinv && nd == 6 && l >= 3) {
+                // implicit gyroscopy: a row of the inverse rotational block; the lane sums the three
+                // rotational components itself, in the order every lane sums its own
+                double a3[3] = {0, 0, 0};
+                for (int j = 0; j < n; ++j) {
+                    const size_t o = j < kPartSlots ? ((size_t)bi * kPartSlots + j) * 6
+                                                    : ((size_t)nb * kPartSlots + povf[bi] + j - kPartSlots) * 6;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) a3[k] += part[o + 3 + k];
+                }
+                d = gyro_row(sinv + (size_t)bi * 9 + 3 * (l - 3), a3[0], a3[1], a3[2]);
+            }
+            const double un = u + d;
             if (n > 0) x[(size_t)bi * kBodyStride + l] = un;
             if (check) {
                 const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
@@ -626,7 +662,8 @@ __global__ void __launch_bounds__(kBlock) k_pjn_bodies(int nb, const int* __rest
                                                        const int* __restrict__ incidx, const double* __restrict__ imp,
                                                        double* __restrict__ x, double* __restrict__ xk, double tol_rel,
                                                        double tol_abs, double* __restrict__ partial,
-                                                       const int* __restrict__ ctrl, const double* __restrict__ nest) {
+                                                       const int* __restrict__ ctrl, const double* __restrict__ nest,
+                                                       const double* __restrict__ sinv) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     const double beta = nest[0];
@@ -656,6 +693,11 @@ __global__ void __launch_bounds__(kBlock) k_pjn_bodies(int nb, const int* __rest
         for (int k = 0; k < 6; ++k) {
             const double M = k < 3 ? Ml : Ma[k - 3];
             un[k] = (k < nd) ? y[k] + acc[k] / M : y[k];
+        }
+        if (sinv && nd == 6) {  // implicit gyroscopy: the inverse rotational block
+            const double* K = sinv + (size_t)bi * 9;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) un[3 + k] = y[3 + k] + gyro_row(K + 3 * k, acc[3], acc[4], acc[5]);
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -836,11 +878,20 @@ __device__ __forceinline__ double lane_mass(const double* __restrict__ minv, int
     return mk > 0 ? 1.0 / mk : 0.0;
 }
 
+// Component l of S^-1 acc for a body of nd DOFs: acc_l / M_l, or with implicit gyroscopy (sinv) a
+// row of a rigid body's inverse rotational block.
+__device__ __forceinline__ double lane_sinv(const double acc[6], int l, int nd, int bi, const double* __restrict__ minv,
+                                            const double* __restrict__ sinv) {
+    if (sinv && nd == 6 && l >= 3) return gyro_row(sinv + (size_t)bi * 9 + 3 * (l - 3), acc[3], acc[4], acc[5]);
+    return lane_value(acc, l) / lane_mass(minv, bi, l);
+}
+
 // linear body pass: v <- v + W^T dp / M, per-block sums of the new v^2
 __global__ void __launch_bounds__(kBlock) k_cheb_lin_bodies8(int nb, const int* __restrict__ dof, const double* __restrict__ minv,
                                                              const int* __restrict__ incptr, const int* __restrict__ incidx,
                                                              const double* __restrict__ imp9, double* __restrict__ v,
-                                                             double* __restrict__ partial, const double* __restrict__ cheb) {
+                                                             double* __restrict__ partial, const double* __restrict__ cheb,
+                                                             const double* __restrict__ sinv) {
     if (cheb[4] != 0.0) return;
     __shared__ double red[kBlock / 64];
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
@@ -850,9 +901,10 @@ __global__ void __launch_bounds__(kBlock) k_cheb_lin_bodies8(int nb, const int* 
     bool touched = false;
     gather_body8(bi, l, live, incptr, incidx, imp9, acc, &touched);
     double s2 = 0;
-    if (live && l < dof[bi]) {
+    const int nd = live ? dof[bi] : 0;
+    if (live && l < nd) {
         double* vp = v + (size_t)bi * kBodyStride + l;
-        const double un = *vp + lane_value(acc, l) / lane_mass(minv, bi, l);
+        const double un = *vp + lane_sinv(acc, l, nd, bi, minv, sinv);
         *vp = un;
         s2 = un * un;
     }
@@ -956,7 +1008,8 @@ __global__ void __launch_bounds__(kBlock) k_pjc_bodies8(int nb, const int* __res
                                                         const double* __restrict__ imp9, double* __restrict__ X,
                                                         double* __restrict__ XK, double* __restrict__ XK1, double tol_rel,
                                                         double tol_abs, double* __restrict__ partial,
-                                                        const int* __restrict__ ctrl, const double* __restrict__ cheb) {
+                                                        const int* __restrict__ ctrl, const double* __restrict__ cheb,
+                                                        const double* __restrict__ sinv) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     const double w = cheb[0];
@@ -968,13 +1021,14 @@ __global__ void __launch_bounds__(kBlock) k_pjc_bodies8(int nb, const int* __res
     bool touched = false;
     gather_body8(bi, l, live, incptr, incidx, imp9, acc, &touched);
     double e2 = 0, bad = 0;
-    if (live && l < dof[bi]) {
+    const int nd = live ? dof[bi] : 0;
+    if (live && l < nd) {
         const size_t o = (size_t)bi * kBodyStride + l;
         const double xs = X[o];
         const double u = take ? xs : cheb_extrap(w, xs, XK[o]);
         XK[o] = XK1[o];
         XK1[o] = u;
-        const double un = touched ? u + lane_value(acc, l) / lane_mass(minv, bi, l) : u;
+        const double un = touched ? u + lane_sinv(acc, l, nd, bi, minv, sinv) : u;
         X[o] = un;
         const double q = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
         const double e = (un - u) / q;
@@ -1458,6 +1512,11 @@ __global__ void __launch_bounds__(kBlock) k_and_commit(int n, int nb, int nc, co
 using namespace cdx;
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
+
+// The inverse rotational blocks of S of the current solve with implicit gyroscopy, else nullptr.
+static const double* gyro_sinv(const cdx_ctx* ctx, const cdx_solver_params* prm) {
+    return prm->implicit_gyro ? ctx->d_sinv : nullptr;
+}
 static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
 static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_out);
 static int run_accel_joints(cdx_ctx* ctx, const cdx_solver_params* prm, int mode, int* iters_out);
@@ -1479,7 +1538,11 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     if (joints && block)
         return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: the block preconditioner does not take joint rows");
     if (joints && !(theta > 0)) return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: theta must be positive with joint rows");
+    if (joints && prm->implicit_gyro)
+        return set_err(ctx, CDX_BAD_ARG, "cdx_pj_solve: implicit gyroscopy does not take joint rows");
+    const bool gyro = prm->implicit_gyro != 0;
     ctx->joints_solved = false;
+    ctx->gyro_last = false;
     if (info) {
         info->iters = 0;
         info->residual = std::nan("");
@@ -1506,9 +1569,18 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         CDX_LAUNCH(ctx, "pj_setup_contacts", k_pj_setup_contacts, grid_for(nc), kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
                    ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_minv, ctx->d_v, prm->alpha, prm->warmstart, ctx->d_R,
                    ctx->d_bias, ctx->d_p, ctx->d_imp, block ? ctx->d_Rb : nullptr);
+    if (gyro) {  // S's rotational blocks; a singular one ends the solve before the loop
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_sinv, 9 * (size_t)nb, &ctx->sinv_cap, "gyroscopic blocks"));
+        CDX_LAUNCH(ctx, "pj_gyro_blocks", k_pj_gyro_blocks, grid_for(nb), kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v, dt,
+                   ctx->d_sinv, ctx->d_ctrl);
+        int ctrl[3];
+        CDX_HIP(ctx, hipMemcpyAsync(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, ctx->stream));
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctrl[0]) return set_err(ctx, CDX_NOT_FACTORIZED, "cdx_pj_solve: S is not factorized (singular gyroscopic block)");
+    }
     CDX_LAUNCH(ctx, "pj_setup_bodies", k_pj_setup_bodies, grid_for(nb), kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_v,
                ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_imp, dt, have_contacts ? 1 : 0, ctx->d_x,
-               joints ? ctx->d_jslot : nullptr, joints ? ctx->d_jtr : nullptr);
+               joints ? ctx->d_jslot : nullptr, joints ? ctx->d_jtr : nullptr, gyro_sinv(ctx, prm));
     CDX_HIP(ctx, hipGetLastError());
     if (joints) {  // x0 = S^-1 (rhs + [W^T p0; 0]) for the jointed bodies, and Lambda_0
         CDX_TRY(joints_factor(ctx, dt, theta));
@@ -1525,7 +1597,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         ctx->pj_last = !joints && !block;
         ctx->pjj_last = joints;
         ctx->joints_solved = joints;
-        ctx->record_valid = ctx->record && !joints && !block;
+        ctx->gyro_last = gyro;
+        ctx->record_valid = ctx->record && !joints && !block && !gyro;
         return CDX_OK;
     }
     const int gb = grid_for(nb), gc = grid_for(nc);
@@ -1550,7 +1623,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
                        ctx->d_nest, block ? ctx->d_Rb : nullptr);
             CDX_LAUNCH(ctx, "pj_bodies", k_pjn_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                        ctx->d_imp, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
-                       ctx->d_nest);
+                       ctx->d_nest, gyro_sinv(ctx, prm));
             CDX_LAUNCH(ctx, "pj_residual", k_pjn_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it,
                        prm->nesterov_beta_threshold, prm->nesterov_restart_limit, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev,
                        ctx->d_nest);
@@ -1577,7 +1650,8 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     ctx->pj_last = !joints && !block;  // the adjoint differentiates the row-scaled, diagonal-S solve only
     ctx->pjj_last = joints && prm->accel == CDX_ACCEL_NONE;  // joint rows refuse block mode and acceleration
     ctx->recorded = iters;
-    ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && !block && iters <= ctx->record_max;
+    ctx->gyro_last = gyro;
+    ctx->record_valid = ctx->record && prm->accel == CDX_ACCEL_NONE && !block && !gyro && iters <= ctx->record_max;
     return CDX_OK;
 }
 
@@ -1709,7 +1783,7 @@ static int run_range_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int* ite
                            ctx->d_part, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies_parts", k_pj_bodies_parts, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_bp_ptr,
                    ctx->d_bp_ovf, ctx->d_part, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial,
-                   ctx->d_ctrl);
+                   ctx->d_ctrl, gyro_sinv(ctx, prm));
         if (check) CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, check_every));
         return hip_check(ctx, hipGetLastError(), "pj range sweep launch");
     };
@@ -1748,7 +1822,7 @@ static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, 
                        ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl,
-                   gj ? ctx->d_jslot : nullptr, gj ? ctx->d_jtr : nullptr);
+                   gj ? ctx->d_jslot : nullptr, gj ? ctx->d_jtr : nullptr, gyro_sinv(ctx, prm));
         if (gj) CDX_TRY(joints_solve(ctx, 0, check, prm->tol_rel, prm->tol_abs, ctx->d_partial + gb8));
         if (check) CDX_TRY(launch_residual_finalize(ctx, gb8 + gj, it, 0, check_every));
         return hip_check(ctx, hipGetLastError(), "pj sweep launch");
@@ -1811,7 +1885,8 @@ static int run_accel_joints(cdx_ctx* ctx, const cdx_solver_params* prm, int mode
             CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
                        ctx->d_R, ctx->d_mu, Yx, ctx->d_imp, ctx->d_cheb, nullptr);
             CDX_LAUNCH(ctx, "cheb_power", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
-                       ctx->d_incidx, ctx->d_imp, Yx, 0, 0.0, 0.0, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
+                       ctx->d_incidx, ctx->d_imp, Yx, 0, 0.0, 0.0, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr,
+                       nullptr);
             CDX_TRY(joints_apply(ctx, Yx, Vl, ctx->d_ctrl));
             CDX_LAUNCH(ctx, "cheb_power", k_accj_sq, gb, kBlock, nb, ctx->d_dof, Yx, nj, Vl, ctx->d_partial2, ctx->d_cheb);
             CDX_LAUNCH(ctx, "cheb_power", k_cheb_norm, 1, kBlock, gb, ctx->d_partial2, ctx->d_cheb);
@@ -1823,7 +1898,8 @@ static int run_accel_joints(cdx_ctx* ctx, const cdx_solver_params* prm, int mode
         CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<false>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
                    ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                   ctx->d_imp, ctx->d_x, 0, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr);
+                   ctx->d_imp, ctx->d_x, 0, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, ctx->d_jslot, ctx->d_jtr,
+                   nullptr);
         CDX_TRY(joints_solve(ctx, 0, 0, prm->tol_rel, prm->tol_abs, nullptr));
         CDX_LAUNCH(ctx, "accj_residual", k_accj_residual, gb, kBlock, mode, nb, ctx->d_dof, ctx->d_x, mode == 0 ? Yx : K1x,
                    Kx, nj, ctx->d_jlam, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl);
@@ -1869,7 +1945,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
                    ctx->d_mu, ctx->d_xk1, ctx->d_imp, ctx->d_cheb, Rb);
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_lin_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
-                   ctx->d_incidx, ctx->d_imp, ctx->d_xk1, ctx->d_partial, ctx->d_cheb);
+                   ctx->d_incidx, ctx->d_imp, ctx->d_xk1, ctx->d_partial, ctx->d_cheb, gyro_sinv(ctx, prm));
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_norm, 1, kBlock, gb8, ctx->d_partial, ctx->d_cheb);
         CDX_LAUNCH(ctx, "cheb_power", k_cheb_scale, gb6, kBlock, nb, ctx->d_dof, ctx->d_xk1, ctx->d_cheb);
     }
@@ -1886,7 +1962,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
                    ctx->d_imp, ctx->d_ctrl, ctx->d_cheb, Rb);
         CDX_LAUNCH(ctx, "pjc_bodies", k_pjc_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
                    ctx->d_imp, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
-                   ctx->d_cheb);
+                   ctx->d_cheb, gyro_sinv(ctx, prm));
         CDX_LAUNCH(ctx, "pjc_residual", k_pjc_finalize, 1, kBlock, gb8, ctx->d_partial2, ctx->nv, it, ctx->d_ctrl, ctx->d_res,
                    ctx->h_flag_dev, ctx->d_cheb);
         return hip_check(ctx, hipGetLastError(), "pj chebyshev sweep launch");
@@ -1932,7 +2008,8 @@ static int run_anderson(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_o
             CDX_LAUNCH(ctx, "pj_contacts", k_pj_contacts4<false>, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J,
                        ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x, ctx->d_imp, ctx->d_ctrl);
         CDX_LAUNCH(ctx, "pj_bodies", k_pj_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                   ctx->d_imp, ctx->d_x, 1, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, nullptr, nullptr);
+                   ctx->d_imp, ctx->d_x, 1, prm->tol_rel, prm->tol_abs, ctx->d_partial, ctx->d_ctrl, nullptr, nullptr,
+                   gyro_sinv(ctx, prm));
         CDX_TRY(launch_residual_finalize(ctx, gb8, it, 0, 1));
         CDX_LAUNCH(ctx, "and_push", k_and_push, gn, kBlock, n, nb, nc, ctx->d_dof, ctx->d_ida, ctx->d_idb, ctx->d_mu,
                    ctx->d_x, ctx->d_p, ring, m, ctx->d_and_st, ctx->d_and_part, ctx->d_ctrl);
@@ -2288,8 +2365,8 @@ extern "C" int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, dou
     if (prm->check_every < 1) return na_fail(CDX_BAD_ARG, "check_every must be >= 1");
     if (!std::isfinite(prm->alpha) || !std::isfinite(prm->tol_abs) || !std::isfinite(prm->tol_rel))
         return na_fail(CDX_BAD_ARG, "non-finite solver parameter");
-    if (prm->accel != CDX_ACCEL_NONE || prm->block_precond)
-        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: standard loop with the row scaling only");
+    if (prm->accel != CDX_ACCEL_NONE || prm->block_precond || prm->implicit_gyro)
+        return na_fail(CDX_BAD_ARG, "cdx_pj_solve_noalloc: standard loop with the row scaling and explicit gyroscopy only");
     if (nb == 0) return CDX_OK;
     int dev = 0;
     NA_HIP(hipGetDevice(&dev));
@@ -2331,7 +2408,7 @@ extern "C" int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, dou
         hipLaunchKernelGGL(k_pj_setup_contacts, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, s.ida, s.idb, s.J, s.mu,
                            s.rest, s.vb, s.m, s.v, prm->alpha, prm->warmstart, s.R, s.bias, s.p, s.imp, nullptr);
     hipLaunchKernelGGL(k_pj_setup_bodies, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s.dof, s.m, s.v, s.f, s.g,
-                       s.incptr, s.incidx, s.imp, dt, have_contacts ? 1 : 0, s.x, nullptr, nullptr);
+                       s.incptr, s.incidx, s.imp, dt, have_contacts ? 1 : 0, s.x, nullptr, nullptr, nullptr);
     NA_HIP(hipGetLastError());
     int iters = 0;
     if (have_contacts) {
@@ -2348,7 +2425,7 @@ extern "C" int cdx_pj_solve_noalloc(const cdx_device_problem* pb, double dt, dou
                 hipLaunchKernelGGL(k_pj_contacts4<false>, dim3(gc4), dim3(kBlock), 0, st, nc, gc4, s.ida, s.idb, s.J,
                                    s.R, s.bias, 1, s.mu, s.p, s.x, s.imp, s.ctrl);
                 hipLaunchKernelGGL(k_pj_bodies8, dim3(gb8), dim3(kBlock), 0, st, nb, s.dof, s.m, s.incptr, s.incidx,
-                                   s.imp, s.x, check, prm->tol_rel, prm->tol_abs, s.part, s.ctrl, nullptr, nullptr);
+                                   s.imp, s.x, check, prm->tol_rel, prm->tol_abs, s.part, s.ctrl, nullptr, nullptr, nullptr);
                 if (check)
                     hipLaunchKernelGGL(k_residual_finalize, dim3(1), dim3(kBlock), 0, st, gb8, s.part, nv, it, ce, 0,
                                        s.ctrl, s.res, s.flags + 4);

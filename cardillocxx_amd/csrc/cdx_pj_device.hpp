@@ -136,12 +136,57 @@ __device__ inline void pj_setup_contact(int c, int nc, const int* __restrict__ i
 }
 
 // Per-body x0 (see k_pj_setup_bodies).
+// Implicit gyroscopy (moreau.implicit_gyroscopy, pj_assembler.cpp:32-61): the inverse of the
+// rotational block diag(I) - dt G_rot(w) of S for a rigid body, G_rot = 1/2 ([I w]_x - [w]_x diag(I)),
+// I = 1 / minv[1..3] of the body, w = v[3..5]; inverted by its adjugate into K (row-major).
+// Returns false for a singular or non-finite block (S not factorized).
+__device__ inline bool gyro_block_inverse(const double* __restrict__ mi, const double* __restrict__ w, double dt,
+                                          double K[9]) {
+    double I[3], Iw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        I[k] = mi[1 + k] > 0 ? 1.0 / mi[1 + k] : 0.0;
+        Iw[k] = I[k] * w[k];
+    }
+    const double sw[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    const double sIw[9] = {0, -Iw[2], Iw[1], Iw[2], 0, -Iw[0], -Iw[1], Iw[0], 0};
+    double S[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double corr = -dt * (0.5 * (sIw[3 * r + c] - sw[3 * r + c] * I[c]));
+            S[3 * r + c] = r == c ? I[r] + corr : corr;
+        }
+    const double c0 = S[4] * S[8] - S[5] * S[7], c1 = S[5] * S[6] - S[3] * S[8], c2 = S[3] * S[7] - S[4] * S[6];
+    const double det = S[0] * c0 + S[1] * c1 + S[2] * c2;
+    if (!(det != 0.0) || !isfinite(det)) return false;
+    K[0] = c0 / det;
+    K[1] = (S[2] * S[7] - S[1] * S[8]) / det;
+    K[2] = (S[1] * S[5] - S[2] * S[4]) / det;
+    K[3] = c1 / det;
+    K[4] = (S[0] * S[8] - S[2] * S[6]) / det;
+    K[5] = (S[2] * S[3] - S[0] * S[5]) / det;
+    K[6] = c2 / det;
+    K[7] = (S[1] * S[6] - S[0] * S[7]) / det;
+    K[8] = (S[0] * S[4] - S[1] * S[3]) / det;
+    return true;
+}
+
+// Row r of a rigid body's inverse rotational block times (a3, a4, a5): the rotational part of
+// S^-1 a with implicit gyroscopy (the other DOFs take a_k / M_k).
+__device__ __forceinline__ double gyro_row(const double* __restrict__ K, double a3, double a4, double a5) {
+    return K[0] * a3 + K[1] * a4 + K[2] * a5;
+}
+
+// sinv: with implicit gyroscopy the bodies' inverse rotational blocks (9 doubles per body, zero for
+// bodies without 6 DOFs), and dt f_gyro leaves the right-hand side; nullptr otherwise.
 __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const double* __restrict__ minv,
                                      const double* __restrict__ v, const double* __restrict__ fext,
                                      const double* __restrict__ fgyro, const int* __restrict__ incptr,
                                      const int* __restrict__ incidx, const double* __restrict__ imp, double dt,
                                      int have_contacts, double* __restrict__ x, const int* __restrict__ jslot = nullptr,
-                                     double* __restrict__ tr = nullptr) {
+                                     double* __restrict__ tr = nullptr, const double* __restrict__ sinv = nullptr) {
     const int nd = dof[bi];
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (have_contacts)
@@ -150,15 +195,23 @@ __device__ inline void pj_setup_body(int bi, const int* __restrict__ dof, const 
     const double* fe = fext + (size_t)bi * kBodyStride;
     const double* fg = fgyro + (size_t)bi * kBodyStride;
     const double* mi = minv + (size_t)bi * kMinvStride;
-    double out[6] = {0, 0, 0, 0, 0, 0};
+    double out[6] = {0, 0, 0, 0, 0, 0}, rhs[6] = {0, 0, 0, 0, 0, 0};
     const bool jointed = jslot && jslot[bi] >= 0;
-    for (int k = 0; k < nd; ++k) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (k >= nd) break;
         const double mk = k < 3 ? mi[0] : mi[k - 2];
         const double M = mk > 0 ? 1.0 / mk : 0.0;
         double r = M * vv[k] + dt * fe[k];
-        r += dt * fg[k];
+        if (!sinv) r += dt * fg[k];
         r += acc[k];
+        rhs[k] = r;
         out[k] = jointed ? r : r / M;
+    }
+    if (sinv && nd == 6 && !jointed) {  // the rotational block of S^-1
+        const double* K = sinv + (size_t)bi * 9;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[3 + k] = gyro_row(K + 3 * k, rhs[3], rhs[4], rhs[5]);
     }
     // a jointed body's right-hand side goes to the chain solve, which writes its x0
     if (jointed) store6(tr + (size_t)bi * 6, out);

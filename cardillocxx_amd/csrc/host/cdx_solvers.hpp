@@ -37,6 +37,7 @@ struct Config {  // hot-path keys of cardillo::config::Config (config.hpp:46-113
     bool pj_rdiag_true_delassus = false;
     bool pj_warmstart = true;
     double moreau_theta = 1.0;
+    bool moreau_implicit_gyroscopy = false;  // config.hpp:233
 
     cdx_solver_params params() const {
         cdx_solver_params p;
@@ -54,6 +55,7 @@ struct Config {  // hot-path keys of cardillo::config::Config (config.hpp:46-113
         p.block_precond = pj_rdiag_true_delassus ? 1 : 0;
         p.nesterov_beta_threshold = pj_nesterov_beta_threshold;
         p.nesterov_restart_limit = pj_nesterov_restart_limit;
+        p.implicit_gyro = moreau_implicit_gyroscopy ? 1 : 0;
         return p;
     }
 };

@@ -88,6 +88,7 @@ class SolverParams(C.Structure):
         ("nesterov_restart_limit", C.c_int),
         ("block_precond", C.c_int),
         ("anderson_m", C.c_int),
+        ("implicit_gyro", C.c_int),
     ]
 
 

@@ -36,6 +36,7 @@ class Config:
     pj_rdiag_true_delassus: bool = False  # pj.rdiag_true_delassus
     pj_warmstart: bool = True
     moreau_theta: float = 1.0
+    moreau_implicit_gyroscopy: bool = False  # moreau.implicit_gyroscopy (config.hpp:233)
     collision_match_max_dist: float = 0.02
     pj_check_every: int = 10  # residual cadence of the PJ standard loop (projected_jacobi.cpp:375)
 
@@ -55,6 +56,7 @@ class Config:
         p.block_precond = 1 if self.pj_rdiag_true_delassus else 0
         p.nesterov_beta_threshold = self.pj_nesterov_beta_threshold
         p.nesterov_restart_limit = self.pj_nesterov_restart_limit
+        p.implicit_gyro = 1 if self.moreau_implicit_gyroscopy else 0
         return p
 
 

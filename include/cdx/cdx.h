@@ -79,6 +79,12 @@ typedef struct {
                           instead of the row scaling alpha / D_ii (projected_jacobi.cpp:71-98, 330-348); projected
                           Jacobi only (ignored by Gauss-Seidel, refused with joint rows and by cdx_pj_adjoint) */
     int anderson_m;    /* pj.anderson_m (default 2): Anderson history window, 1..8 */
+    int implicit_gyro; /* moreau.implicit_gyroscopy (default 0, config.hpp:233): projected Jacobi solves with
+                          the rotational block diag(I) - dt G_rot(w_n) of S per rigid body, G_rot =
+                          1/2 ([I w]_x - [w]_x diag(I)), and without dt f_gyro on the right-hand side
+                          (pj_assembler.cpp:32-61,150-153); Gauss-Seidel only drops dt f_gyro from u_free
+                          (pgs_assembler.cpp:33-36,75-82).  Refused with joint rows, by the batched solve
+                          and by cdx_pj_solve_noalloc; cdx_pj_adjoint refuses to follow such a solve. */
 } cdx_solver_params;
 
 typedef struct {

@@ -378,10 +378,88 @@ static void fill_mdiag(int nv, const double* minv, double* M) {
     for (int j = 0; j < nv; ++j) M[j] = minv[j] > 0 ? 1.0 / minv[j] : 0.0;
 }
 
+/* S of the contact-only system: diag(M), or with implicit gyroscopy (blk != NULL) the inverse
+ * rotational block of every rigid body in blk (9 doubles per body, row-major). */
+typedef struct {
+    int nb;
+    const int* dof;
+    const int* voff;
+    const double* M;
+    const double* blk;
+} orc_sinv;
+
+/* The inverse rotational blocks of S with implicit gyroscopy (PjAssembler::buildAndFactorS,
+ * pj_assembler.cpp:32-61): a rigid body (6 DOFs) with inertia I = 1 / minv[3..5] and angular
+ * velocity w = v[3..5] has S_rot = diag(I) + corr, corr = -dt G_rot, G_rot = 1/2 ([I w]_x -
+ * [w]_x diag(I)), [a]_x b = a x b; inverted here by its adjugate (the reference factors the whole
+ * S by SparseLU).  Returns ORC_NOT_FACTORIZED for a singular or non-finite block.  The
+ * ORC_MUT_GYRO_* macros each plant one deliberate mistake for the mutation tests. */
+static int gyro_blocks(const orc_bodies* B, const int* voff, double dt, double* blk) {
+    for (int b = 0; b < B->nb; ++b) {
+        double* K = blk + 9 * (size_t)b;
+        for (int k = 0; k < 9; ++k) K[k] = 0.0;
+        if (B->dof[b] != 6) continue;
+        const int o = voff[b];
+        double I[3], w[3], Iw[3];
+        for (int k = 0; k < 3; ++k) {
+            I[k] = B->minv[o + 3 + k] > 0 ? 1.0 / B->minv[o + 3 + k] : 0.0;
+            w[k] = B->v[o + 3 + k];
+            Iw[k] = I[k] * w[k];
+        }
+        const double sw[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+        const double sIw[9] = {0, -Iw[2], Iw[1], Iw[2], 0, -Iw[0], -Iw[1], Iw[0], 0};
+        double S[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+#if defined(ORC_MUT_GYRO_T)
+                const double g = 0.5 * (sIw[3 * c + r] - sw[3 * c + r] * I[r]);
+#elif defined(ORC_MUT_GYRO_HALF)
+                const double g = sIw[3 * r + c] - sw[3 * r + c] * I[c];
+#else
+                const double g = 0.5 * (sIw[3 * r + c] - sw[3 * r + c] * I[c]);
+#endif
+#if defined(ORC_MUT_GYRO_SIGN)
+                const double corr = dt * g;
+#else
+                const double corr = -dt * g;
+#endif
+                S[3 * r + c] = r == c ? I[r] + corr : corr;
+            }
+        const double c0 = S[4] * S[8] - S[5] * S[7], c1 = S[5] * S[6] - S[3] * S[8], c2 = S[3] * S[7] - S[4] * S[6];
+        const double det = S[0] * c0 + S[1] * c1 + S[2] * c2;
+        if (!(det != 0.0) || !isfinite(det)) return ORC_NOT_FACTORIZED;
+        K[0] = c0 / det;
+        K[1] = (S[2] * S[7] - S[1] * S[8]) / det;
+        K[2] = (S[1] * S[5] - S[2] * S[4]) / det;
+        K[3] = c1 / det;
+        K[4] = (S[0] * S[8] - S[2] * S[6]) / det;
+        K[5] = (S[2] * S[3] - S[0] * S[5]) / det;
+        K[6] = c2 / det;
+        K[7] = (S[1] * S[6] - S[0] * S[7]) / det;
+        K[8] = (S[0] * S[4] - S[1] * S[3]) / det;
+    }
+    return ORC_OK;
+}
+
+/* y <- S^-1 y (solveS, pj_assembler.cpp:173-182) */
+static void sinv_inplace(const orc_sinv* s, double* y) {
+    for (int b = 0; b < s->nb; ++b) {
+        const int o = s->voff[b], nd = s->dof[b];
+        if (s->blk && nd == 6) {
+            const double* K = s->blk + 9 * (size_t)b;
+            for (int j = o; j < o + 3; ++j) y[j] = y[j] / s->M[j];
+            const double y3 = y[o + 3], y4 = y[o + 4], y5 = y[o + 5];
+            for (int r = 0; r < 3; ++r) y[o + 3 + r] = K[3 * r] * y3 + K[3 * r + 1] * y4 + K[3 * r + 2] * y5;
+        } else {
+            for (int j = o; j < o + nd; ++j) y[j] = y[j] / s->M[j];
+        }
+    }
+}
+
 /* Spectral radius of the linear sweep map v -> v - S^-1 W^T R W v, R the row scaling or R_block
  * (estimateSpectralRadius, projected_jacobi.cpp:155-178): 8 power iterations from the normalised ones vector, clamped to
- * 0.995; here S = diag(M). */
-static double estimate_rho(const orc_W* W, const orc_W* RW, const double* Rb, const double* M, int nv) {
+ * 0.995. */
+static double estimate_rho(const orc_W* W, const orc_W* RW, const double* Rb, const orc_sinv* SI, int nv) {
     if (nv <= 0) return 0.0;
     double* v = (double*)malloc(sizeof(double) * (size_t)nv);
     double* r = (double*)malloc(sizeof(double) * (size_t)nv);
@@ -394,9 +472,10 @@ static double estimate_rho(const orc_W* W, const orc_W* RW, const double* Rb, co
         if (Rb) apply_rblock(W, Rb, d);
         memset(r, 0, sizeof(double) * (size_t)nv);
         spmv_t_acc(W, d, r);
+        sinv_inplace(SI, r);
         double n2 = 0.0;
         for (int j = 0; j < nv; ++j) {
-            v[j] = v[j] - r[j] / M[j];
+            v[j] = v[j] - r[j];
             n2 += v[j] * v[j];
         }
         const double n = sqrt(n2);
@@ -532,16 +611,28 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     fill_mdiag(nv, B->minv, M);
     for (int j = 0; j < nv; ++j)
         if (M[j] == 0.0) { free(M); orc_free_W(&W); return ORC_NOT_FACTORIZED; } /* singular diagonal S */
+    double* blk = NULL;
+    if (prm->implicit_gyro) {
+        blk = (double*)malloc(sizeof(double) * 9 * (size_t)(B->nb + 1));
+        if (gyro_blocks(B, W.voff, dt, blk)) { free(blk); free(M); orc_free_W(&W); return ORC_NOT_FACTORIZED; }
+    }
+    const orc_sinv SI = {B->nb, B->dof, W.voff, M, blk};
 
-    /* rhs_free = M v + dt f_ext (+ dt f_gyro) (pj_assembler.cpp:152-153) */
+    /* rhs_free = M v + dt f_ext (+ dt f_gyro unless implicit) (pj_assembler.cpp:152-153) */
     double* x = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
     for (int j = 0; j < nv; ++j) x[j] = M[j] * B->v[j] + dt * B->fext[j];
+#if defined(ORC_MUT_GYRO_RHS)
     for (int j = 0; j < nv; ++j) x[j] += dt * B->fgyro[j];
+#else
+    if (!prm->implicit_gyro)
+        for (int j = 0; j < nv; ++j) x[j] += dt * B->fgyro[j];
+#endif
 
     if (nr == 0) { /* no contacts: free solution (:460-467) */
-        for (int j = 0; j < nv; ++j) v_out[j] = x[j] / M[j];
+        memcpy(v_out, x, sizeof(double) * (size_t)nv);
+        sinv_inplace(&SI, v_out);
         for (int c = 0; c < C->nc; ++c) { imp_out[3 * c] = imp_out[3 * c + 1] = imp_out[3 * c + 2] = 0; }
-        free(x); free(M); orc_free_W(&W);
+        free(x); free(blk); free(M); orc_free_W(&W);
         return ORC_OK;
     }
 
@@ -581,7 +672,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         }
     /* x0 = S^-1 (rhs_free + W^T p) (:361-363) */
     spmv_t_acc(&W, p, x);
-    for (int j = 0; j < nv; ++j) x[j] = x[j] / M[j];
+    sinv_inplace(&SI, x);
 
     orc_W RW = W;
     RW.val = RWv;
@@ -606,6 +697,11 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
             _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
             for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];              \
             spmv_t_mt(&W, &WT, dl, rhs, nthreads);                          \
+            if (blk) {                                                      \
+                sinv_inplace(&SI, rhs);                                     \
+                for (int j = 0; j < nv; ++j) x[j] += rhs[j];                \
+                break;                                                      \
+            }                                                               \
             _Pragma("omp parallel for schedule(static) num_threads(nthreads)") \
             for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];             \
             break;                                                          \
@@ -620,7 +716,8 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         for (int r = 0; r < nr; ++r) dl[r] = p[r] - pp[r];                  \
         memset(rhs, 0, sizeof(double) * (size_t)nv);                        \
         spmv_t_acc(&W, dl, rhs);                                            \
-        for (int j = 0; j < nv; ++j) x[j] += rhs[j] / M[j];                 \
+        sinv_inplace(&SI, rhs);                                             \
+        for (int j = 0; j < nv; ++j) x[j] += rhs[j];                        \
     } while (0)
 
     if (!prm->nesterov && !prm->chebyshev && prm->anderson) {
@@ -700,7 +797,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
     } else if (!prm->nesterov && prm->chebyshev) {
         /* chebyshev_loop (:180-228): residual every sweep against the sweep's input; on
          * convergence the loop returns that input (x = xk1), not the swept iterate */
-        const double rho = estimate_rho(&W, &RW, Rb, M, nv);
+        const double rho = estimate_rho(&W, &RW, Rb, &SI, nv);
         const double rho2 = rho * rho;
         res->rho = rho;
         double* xk = (double*)malloc(sizeof(double) * (size_t)nv);
@@ -839,7 +936,7 @@ int orc_pj_solve(const orc_bodies* B, const orc_contacts* C, double dt, double t
         if (W.row_size[ci] > 1) { o[1] = p[base + 1]; o[2] = p[base + 2]; }
     }
     memcpy(v_out, x, sizeof(double) * (size_t)nv);
-    free(R); free(Rb); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(M);
+    free(R); free(Rb); free(bias); free(p); free(pp); free(dl); free(xp); free(rhs); free(RWv); free(x); free(blk); free(M);
     free(WT.ptr); free(WT.row); free(WT.ent);
     orc_free_W(&W);
     return ORC_OK;
@@ -916,8 +1013,10 @@ static double joint_row_dot(const orc_joints* jt, const int* voff, const int* do
 int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_joints* jt, double dt, double theta,
                         const orc_params* prm, double* v_out, double* imp_out, double* lam_out, orc_result* res) {
     /* the loops of ProjectedJacobiSolver::solve (:469-476) over x = [v; Lambda]: Nesterov before
-     * Chebyshev; Anderson and the block preconditioner are not restated with joint rows */
-    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->block_precond || (prm->anderson && !prm->nesterov && !prm->chebyshev))
+     * Chebyshev; Anderson, the block preconditioner and implicit gyroscopy are not restated with
+     * joint rows */
+    if (!jt || jt->ns < 0 || jt->nd < 0 || prm->block_precond || (prm->anderson && !prm->nesterov && !prm->chebyshev) ||
+        prm->implicit_gyro)
         return ORC_BAD_ARG;
     orc_W W;
     int st = orc_build_W(B, C, &W);
@@ -1158,10 +1257,12 @@ int orc_pj_solve_joints(const orc_bodies* B, const orc_contacts* C, const orc_jo
     return ORC_OK;
 }
 
-/* u_free = v + Minv .* (dt f_ext) (+ Minv .* (dt f_gyro)) (pgs_assembler.cpp:75-82) */
-static void u_free_of(int nv, const orc_bodies* B, double dt, double* uf) {
+/* u_free = v + Minv .* (dt f_ext) (+ Minv .* (dt f_gyro) unless moreau.implicit_gyroscopy, which
+ * the PGS assembler does not otherwise model: pgs_assembler.cpp:33-36,75-82) */
+static void u_free_of(int nv, const orc_bodies* B, double dt, int implicit_gyro, double* uf) {
     for (int j = 0; j < nv; ++j) uf[j] = B->v[j] + B->minv[j] * (dt * B->fext[j]);
-    for (int j = 0; j < nv; ++j) uf[j] += B->minv[j] * (dt * B->fgyro[j]);
+    if (!implicit_gyro)
+        for (int j = 0; j < nv; ++j) uf[j] += B->minv[j] * (dt * B->fgyro[j]);
 }
 
 /* Residual on velocities u_free - u_corr (projected_gauss_seidel.cpp:124-131) */
@@ -1232,7 +1333,7 @@ static int pgs_core(const orc_bodies* B, const orc_contacts* C, double dt, const
     const int nv = W.nv, nr = W.nrows;
     res->iters = 0; res->residual = NAN; res->nrows = nr; res->ncolors = 0; res->rho = NAN;
     double* uf = (double*)malloc(sizeof(double) * (size_t)(nv + 1));
-    u_free_of(nv, B, dt, uf);
+    u_free_of(nv, B, dt, prm->implicit_gyro, uf);
     if (nr == 0) {
         memcpy(v_out, uf, sizeof(double) * (size_t)nv);
         for (int c = 0; c < C->nc; ++c) imp_out[3 * c] = imp_out[3 * c + 1] = imp_out[3 * c + 2] = 0;

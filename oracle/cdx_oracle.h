@@ -68,6 +68,9 @@ typedef struct {
     int block_precond; /* pj.rdiag_true_delassus: exact per-contact Delassus blocks instead of the row scaling */
     int anderson;      /* pj.anderson (ignored when nesterov or chebyshev is set, projected_jacobi.cpp:470-472) */
     int anderson_m;    /* pj.anderson_m: history window (>= 1) */
+    int implicit_gyro; /* moreau.implicit_gyroscopy: PJ solves with S_rot = diag(I) - dt G_rot per rigid
+                          body and leaves dt f_gyro out of the right-hand side; PGS only leaves it out
+                          (refused by orc_pj_solve_joints) */
 } orc_params;
 
 typedef struct {

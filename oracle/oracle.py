@@ -33,7 +33,8 @@ class Params(C.Structure):
     _fields_ = [("alpha", C.c_double), ("relaxation", C.c_double), ("max_iter", C.c_int), ("tol_abs", C.c_double),
                 ("tol_rel", C.c_double), ("warmstart", C.c_int), ("nesterov", C.c_int),
                 ("nesterov_beta_threshold", C.c_double), ("nesterov_restart_limit", C.c_int), ("chebyshev", C.c_int),
-                ("block_precond", C.c_int), ("anderson", C.c_int), ("anderson_m", C.c_int)]
+                ("block_precond", C.c_int), ("anderson", C.c_int), ("anderson_m", C.c_int),
+                ("implicit_gyro", C.c_int)]
 
 
 class Joints(C.Structure):
@@ -133,9 +134,9 @@ def lstsq_colpiv(A, b):
 
 def params(alpha=0.3, relaxation=1.0, max_iter=200000, tol_abs=1e-5, tol_rel=1e-5, warmstart=1, nesterov=0,
            nesterov_beta_threshold=1.0, nesterov_restart_limit=10000, chebyshev=0, block_precond=0, anderson=0,
-           anderson_m=2):
+           anderson_m=2, implicit_gyro=0):
     return Params(alpha, relaxation, max_iter, tol_abs, tol_rel, warmstart, nesterov, nesterov_beta_threshold,
-                  nesterov_restart_limit, chebyshev, block_precond, anderson, anderson_m)
+                  nesterov_restart_limit, chebyshev, block_precond, anderson, anderson_m, implicit_gyro)
 
 
 def _solve(name, prob, dt, prm, nthreads=None, L=None):
