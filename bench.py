@@ -237,6 +237,7 @@ def pile_leg(args, native, scenes, metrics, hip):
     if args.no_graphs:
         ctx.set_graphs(False)
     ctx.set_range_sweep(args.range_mode, args.range_contacts, args.range_block)
+    ctx.set_geometry_rows(0 if args.no_geometry_rows else 1)
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
     ctx.synchronize()
@@ -244,6 +245,7 @@ def pile_leg(args, native, scenes, metrics, hip):
     tu = time.perf_counter()
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
+    ctx.set_contact_points(prob.point)  # the geometry form of the rows, where they reproduce bit for bit
     ctx.synchronize()
     t_upload = time.perf_counter() - tu
     p = native.default_params()
@@ -273,9 +275,13 @@ def pile_leg(args, native, scenes, metrics, hip):
     hip.destroy(e0, e1)
     cups = prob.nc * sweeps / elapsed
     r_used, r_n, r_parts, r_per = ctx.range_sweep_info()
+    g_ok, g_bad, g_used = ctx.geometry_rows_info()
     sweep_form = {"range_sweep": r_used, "ranges": r_n, "partials_per_sweep": r_parts, "contacts_per_range": r_per,
-                  "mode": args.range_mode, "block": args.range_block or 512}
-    log("sweep: " + (f"{r_n} contact ranges of {r_per}, {r_parts} partial sums" if r_used else "two passes"))
+                  "mode": args.range_mode, "block": args.range_block or 512,
+                  "geometry_rows": {"used": g_used, "usable": g_ok, "mismatching_contacts": g_bad}}
+    log("sweep: " + (f"{r_n} contact ranges of {r_per}, {r_parts} partial sums" if r_used else "two passes")
+        + (", angular rows rebuilt from the contact points" if g_used else
+           f", angular rows read (points usable: {g_ok}, mismatching contacts: {g_bad})"))
     log(f"timed: {args.steps} solves, {sweeps} sweeps, {elapsed:.3f}s ({ev_s:.3f}s by stream events), "
         f"{cups:.3e} contact-updates/s")
     td = time.perf_counter()
@@ -942,6 +948,8 @@ def main():
                     help="range sweep of projected Jacobi: 0 two passes, 1 from 32,768 contacts, 2 always")
     ap.add_argument("--range-contacts", type=int, default=0, help="contacts per range (0 = 128)")
     ap.add_argument("--range-block", type=int, default=0, help="threads per range workgroup (0 = 512, or 256)")
+    ap.add_argument("--no-geometry-rows", action="store_true",
+                    help="range sweep reads the angular rows instead of rebuilding them from the contact points")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run one device copy of this size (counter calibration)")
     ap.add_argument("--no-kernel-events", action="store_true",

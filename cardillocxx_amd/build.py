@@ -65,7 +65,10 @@ def build_device(verbose: bool = True) -> str:
 
 def build_scene() -> str:
     out = os.path.join(PKG, "libcdx_scene.so")
-    _run(["g++", *HOST_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(CSRC, "host", "cdx_scene.cpp")])
+    # no contraction: the angular rows it writes are rebuilt bit for bit on the device from the
+    # contact points (cdx_set_contact_points), with the same operations
+    _run(["g++", *HOST_FLAGS, "-ffp-contract=off", "-shared", "-o", out + ".tmp",
+          os.path.join(CSRC, "host", "cdx_scene.cpp")])
     os.replace(out + ".tmp", out)
     return out
 

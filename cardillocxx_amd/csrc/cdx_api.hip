@@ -215,7 +215,8 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jDinv, ctx->d_jtr, ctx->d_sc_off, ctx->d_sc_list, ctx->d_adj_jl, ctx->d_jr_blk, ctx->d_jy, ctx->d_gm, ctx->d_gm_part, ctx->d_jr_info, ctx->d_jbody_info,
                     ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
                     ctx->d_gs_pack, ctx->d_gs_ids, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc,
-                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj, ctx->d_sinv};
+                    ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj, ctx->d_sinv, ctx->d_geo, ctx->d_geo_pt,
+                    ctx->d_geo_bad};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);
@@ -336,6 +337,8 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
     ctx->njrows = ctx->nchains = ctx->njbodies = 0;  // joint rows depend on the body state: set them again
     ctx->joints_solved = false;
     ctx->solved = false;
+    ctx->geo_ok = false;  // the lever arms depend on the positions
+    ctx->geo_bad = -1;
     return CDX_OK;
 }
 
@@ -434,6 +437,8 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     ctx->coloring_valid = false;
     ctx->nscenes = 0;
     ctx->solved = false;
+    ctx->geo_ok = false;
+    ctx->geo_bad = -1;
     return build_incidence(ctx);
 }
 
@@ -482,6 +487,10 @@ int cdx_moreau_predrift(cdx_ctx* ctx, double dt, double theta) {
     if (h != 0.0 && ctx->nb > 0)
         CDX_LAUNCH(ctx, "moreau_drift", k_position_update, grid_for(ctx->nb), kBlock, ctx->nb, ctx->d_dof, ctx->d_q, ctx->d_v,
                    ctx->d_v, h, 0);
+    if (h != 0.0) {
+        ctx->geo_ok = false;
+        ctx->geo_bad = -1;
+    }
     CDX_HIP(ctx, hipGetLastError());
     return CDX_OK;
 }
@@ -495,6 +504,8 @@ int cdx_moreau_update(cdx_ctx* ctx, double dt, double theta) {
                    ctx->d_x, theta * dt, 1);
         CDX_LAUNCH(ctx, "gyro", k_gyro, grid_for(ctx->nb), kBlock, ctx->nb, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fgyro);
     }
+    ctx->geo_ok = false;
+    ctx->geo_bad = -1;
     CDX_HIP(ctx, hipGetLastError());
     return CDX_OK;
 }

@@ -186,6 +186,15 @@ struct cdx_ctx {
     size_t accj_cap = 0;
     double* d_sinv = nullptr;      // implicit gyroscopy: inverse rotational blocks of S [nb*9]
     size_t sinv_cap = 0;
+    // geometry form of the contact rows (cdx_set_contact_points): lever arms [6*nc], points [3*nc]
+    double* d_geo = nullptr;
+    double* d_geo_pt = nullptr;
+    int* d_geo_bad = nullptr;
+    size_t geo_cap = 0, geo_pt_cap = 0, geo_bad_cap = 0;
+    bool geo_ok = false;    // every rebuilt angular row equals the current one bit for bit
+    int geo_mode = 1;       // 0: never use the geometry form
+    bool geo_used = false;  // the last standard-loop solve rebuilt the rows
+    int geo_bad = -1;       // contacts whose rebuilt rows differ (-1: no points for these contacts)
     int* h_flag = nullptr;         // pinned mirror of the done flag
     int* h_flag_dev = nullptr;
     bool solved = false;

@@ -295,17 +295,120 @@ __device__ __forceinline__ void dma_ints(const int* __restrict__ src, int n, int
     }
 }
 
-template <bool Block>
-__global__ void __launch_bounds__(512, 8) k_pj_range(
+// ---------------------------------------------------------------------------------------------
+// Geometry form of the contact rows
+// ---------------------------------------------------------------------------------------------
+//
+// A rigid side's angular row k is r_body x (R^T d_k) (buildContactRowByDof, contact_jacobian.hpp:
+// 17-23, with makeContact's body-frame lever arm r_body = R^T (p - x_com)), R the body's rotation.
+// Given the world contact points, k_geo_pack derives each side's r_body from the body's position and
+// quaternion and rebuilds every angular row; where all of them equal the rows passed to
+// cdx_set_contacts bit for bit, the range sweep rebuilds them from (r_body, quaternion, frame) in
+// every sweep instead of reading them: 48 bytes of lever arms per contact in place of 144 bytes of
+// angular rows, the quaternions being gathered with the twists.  The arithmetic is the host contact
+// generator's, operation by operation and without contraction, so that the rebuilt rows are the
+// same numbers.
+
+// R^T v for the unit quaternion (qx, qy, qz, qw) (Eigen's coefficient order), matrix entries first.
+__device__ inline void geo_rot_t(const double* __restrict__ q, const double v[3], double out[3]) {
+#pragma clang fp contract(off)
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    const double r00 = 1 - 2 * (y * y + z * z), r01 = 2 * (x * y - z * w), r02 = 2 * (x * z + y * w);
+    const double r10 = 2 * (x * y + z * w), r11 = 1 - 2 * (x * x + z * z), r12 = 2 * (y * z - x * w);
+    const double r20 = 2 * (x * z - y * w), r21 = 2 * (y * z + x * w), r22 = 1 - 2 * (x * x + y * y);
+    out[0] = r00 * v[0] + r10 * v[1] + r20 * v[2];
+    out[1] = r01 * v[0] + r11 * v[1] + r21 * v[2];
+    out[2] = r02 * v[0] + r12 * v[1] + r22 * v[2];
+}
+
+__device__ inline void geo_cross(const double a[3], const double b[3], double out[3]) {
+#pragma clang fp contract(off)
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// The angular row r x (R^T d) of one side (q: the body's quaternion).
+__device__ inline void geo_row(const double* __restrict__ q, const double r[3], const double d[3], double out[3]) {
+    double t[3];
+    geo_rot_t(q, d, t);
+    geo_cross(r, t, out);
+}
+
+// One thread per contact: the lever arms r_body = R^T (p - x_com) of its rigid sides into G
+// (component-major, rA then rB; NaN for a static or point-mass side, which has no angular row) and
+// a count of the contacts whose rebuilt angular rows differ from J's in any bit, per workgroup.
+__global__ void __launch_bounds__(kBlock) k_geo_pack(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                                     const int* __restrict__ dof, const double* __restrict__ q,
+                                                     const double* __restrict__ J, const double* __restrict__ point,
+                                                     double* __restrict__ G, int* __restrict__ bad_part) {
+    __shared__ double red[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double bad = 0;
+    if (c < nc) {
+        for (int side = 0; side < 2; ++side) {
+            const int b = side == 0 ? ida[c] : idb[c];
+            double r[3] = {NAN, NAN, NAN};
+            if (b >= 0 && dof[b] == 6) {
+                const double* qb = q + (size_t)b * kBodyStride;
+                double rel[3];
+                {
+#pragma clang fp contract(off)
+                    rel[0] = point[3 * (size_t)c] - qb[0];
+                    rel[1] = point[3 * (size_t)c + 1] - qb[1];
+                    rel[2] = point[3 * (size_t)c + 2] - qb[2];
+                }
+                geo_rot_t(qb + 3, rel, r);
+                for (int k = 0; k < 3; ++k) {
+                    double d[3], a[3];
+                    for (int m = 0; m < 3; ++m) d[m] = J[(size_t)(3 * k + m) * nc + c];
+                    geo_row(qb + 3, r, d, a);
+                    for (int m = 0; m < 3; ++m)
+                        if (__double_as_longlong(a[m]) != __double_as_longlong(J[(size_t)(9 + 9 * side + 3 * k + m) * nc + c]))
+                            bad = 1;
+                }
+            }
+            for (int m = 0; m < 3; ++m) G[(size_t)(3 * side + m) * nc + c] = r[m];
+        }
+    }
+    const double s = block_sum(bad, red);
+    if (threadIdx.x == 0) bad_part[blockIdx.x] = (int)s;
+}
+
+// project() (projected_jacobi.cpp:100-119) of one contact's trial values z (nr = 1 or 3 rows).
+__device__ __forceinline__ void project3(double z[3], int nr, double muc) {
+    z[0] = fmax(z[0], 0.0);
+    if (nr == 3) {
+        const double tn2 = z[1] * z[1] + z[2] * z[2];
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (muc * z[0]) / sqrt(tn2));
+            z[1] = sc * z[1];
+            z[2] = sc * z[2];
+        }
+    }
+}
+
+// One lane per contact and one lane per (body, range) pair: the pass issues few instructions per
+// contact (no cross-lane sums), which is what bounds it -- with four lanes per contact the DPP
+// exchanges of the records and of the projection took most of its instruction issue.
+//
+// Block = true: R holds the inverse Delassus blocks and the bias is unscaled
+// (pj.rdiag_true_delassus): dp = Pi(p - Rb (W x + b)) - p.  Geo = true: the angular rows are
+// rebuilt from the lever arms G and the quaternions q (the geometry form above) instead of being
+// read from J.
+template <bool Block, bool Geo>
+__global__ void __launch_bounds__(256) k_pj_range(
     int nranges, int nc, int per, int maxs, int maxp, const int* __restrict__ rg_p, const int* __restrict__ rg_l,
     const int* __restrict__ pr_ptr, const int* __restrict__ pr_out, const int* __restrict__ pr_inc,
     const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
     const double* __restrict__ R, const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
-    double* __restrict__ p, const double* __restrict__ x, double* __restrict__ part, const int* __restrict__ ctrl) {
+    double* __restrict__ p, const double* __restrict__ x, double* __restrict__ part, const int* __restrict__ ctrl,
+    const double* __restrict__ G, const double* __restrict__ q) {
     if (ctrl[0]) return;
-    // LDS: the records (9 doubles per contact of the range: lin | angA | angB), then the range's
-    // tables, copied by DMA while the contact rounds run: the incidence offsets of its pairs
-    // (body, range), their partial slots, and the incidences (contact in the range << 1 | side)
+    // LDS: the records (9 doubles per contact of the range: lin | angA | angB, summed over its
+    // rows), then the range's tables, copied by DMA while the contacts are swept: the incidence
+    // offsets of its pairs (body, range), their partial slots, and the incidences (contact in the
+    // range << 1 | side)
     extern __shared__ double lrec[];
     const int g = xcd_tile(nranges);
     const int s0 = g * per, s1 = min(nc, s0 + per);
@@ -317,104 +420,92 @@ __global__ void __launch_bounds__(512, 8) k_pj_range(
     dma_ints(pr_out + q0, q1 - q0, Lout);
     dma_ints(pr_inc + l0, l1 - l0, Linc);
 
-    // Contact rounds, 4 lanes per contact as k_pj_contacts4.  The next round's body ids are loaded
-    // during the current round, and lanes 0-2 load their rows and twists whatever the contact's
-    // friction (mu arrives with them), so a round waits for one memory latency, not two.
-    const int Q = blockDim.x >> 2;
-    int an = -1, bn = -1;
-    {
-        const int c = s0 + (threadIdx.x >> 2);
-        if (c < s1) {
-            an = ida[c];
-            bn = idb[c];
-        }
-    }
 #pragma unroll 1
-    for (int base = s0; base < s1; base += Q) {
-        const int c = base + (threadIdx.x >> 2);
-        const bool live = c < s1;
-        const int a = an, b = bn;
-        if (c + Q < s1) {
-            an = ida[c + Q];
-            bn = idb[c + Q];
-        }
-        // The lane's row and the column stride pass through empty asm statements, so the compiler
-        // recomputes the 27 row addresses in each round instead of keeping them all in registers
-        // across the loop (which would spill at 64 VGPRs).
-        int k = threadIdx.x & 3, ncl = nc;
-        asm volatile("" : "+v"(k));
-        asm volatile("" : "+s"(ncl));
-        const bool dyn = live && (a >= 0 || b >= 0);
-        const bool ld = dyn && k < 3;
-        const double muc = live ? mu[c] : 0.0;
-        double d[3] = {0, 0, 0}, aA[3] = {0, 0, 0}, aB[3] = {0, 0, 0};
-        double gk = 0, pold = 0, rk = 0, bk = 0, pt = 0;
-        if (ld) {
+    for (int c = s0 + threadIdx.x; c < s1; c += blockDim.x) {
+        const int a = ida[c], b = idb[c];
+        const double muc = mu[c];
+        const int nr = muc > 0 ? 3 : 1;
+        double rec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (a >= 0 || b >= 0) {
+            double d[3][3], aA[3][3], aB[3][3];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const double* Jr = J + (size_t)(3 * k) * ncl + c;
-                d[m] = Jr[(size_t)m * ncl];
-                aA[m] = Jr[(size_t)(9 + m) * ncl];
-                aB[m] = Jr[(size_t)(18 + m) * ncl];
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    d[k][m] = J[(size_t)(3 * k + m) * nc + c];
+                    if constexpr (!Geo) {
+                        aA[k][m] = J[(size_t)(9 + 3 * k + m) * nc + c];
+                        aB[k][m] = J[(size_t)(18 + 3 * k + m) * nc + c];
+                    } else {
+                        aA[k][m] = 0.0;
+                        aB[k][m] = 0.0;
+                    }
+                }
+            if constexpr (Geo) {
+                double rA[3], rB[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    rA[m] = G[(size_t)m * nc + c];
+                    rB[m] = G[(size_t)(3 + m) * nc + c];
+                }
+                if (rA[0] == rA[0])
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) geo_row(q + (size_t)a * kBodyStride + 3, rA, d[k], aA[k]);
+                if (rB[0] == rB[0])
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) geo_row(q + (size_t)b * kBodyStride + 3, rB, d[k], aB[k]);
             }
             double ua[6], ub[6];
             load_body(x, a, ua);
             load_body(x, b, ub);
-            gk = row_dot(d, aA, aB, ua, ub);
-            pold = p[(size_t)k * ncl + c];
-            if constexpr (!Block) rk = R[(size_t)k * ncl + c];
-            if (use_bias) bk = bias[(size_t)k * ncl + c];
-        }
-        const int nr = muc > 0 ? 3 : 1;
-        const bool row = dyn && k < nr;
-        if (row) {
-            if constexpr (Block)
-                pt = gk + bk;
-            else
-                pt = pold - (rk * gk + bk);
-        } else {
-            pold = 0;
-        }
-        if constexpr (Block) pt = pold - block_apply4(pt, k, c, ncl, nr, row, R);
-        const double pnew = project4(pt, k, muc);
-        double dp = 0;
-        if (row) {
-            dp = pnew - pold;
-            p[(size_t)k * ncl + c] = pnew;
-        }
-        double v9[9];
+            double pold[3] = {0, 0, 0}, z[3] = {0, 0, 0};
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            v9[m] = d[m] * dp;
-            v9[3 + m] = aA[m] * dp;
-            v9[6 + m] = aB[m] * dp;
-        }
+            for (int k = 0; k < 3; ++k) {
+                if (k >= nr) break;
+                pold[k] = p[(size_t)k * nc + c];
+                const double gk = row_dot(d[k], aA[k], aB[k], ua, ub);
+                const double bk = use_bias ? bias[(size_t)k * nc + c] : 0.0;
+                if constexpr (Block)
+                    z[k] = gk + bk;
+                else
+                    z[k] = pold[k] - (R[(size_t)k * nc + c] * gk + bk);
+            }
+            if constexpr (Block) {  // p - R_block t, in row order (BlockDiagonal::applyTo)
+                const double t0 = z[0], t1 = z[1], t2 = z[2];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) v9[m] = quad_sum(v9[m]);
-        if (dyn && k < 3) {  // lane 0 writes lin, lane 1 angA, lane 2 angB (zero for a static side)
-            double* l = lrec + 9 * (c - s0) + 3 * k;
-            l[0] = k == 0 ? v9[0] : (k == 1 ? v9[3] : v9[6]);
-            l[1] = k == 0 ? v9[1] : (k == 1 ? v9[4] : v9[7]);
-            l[2] = k == 0 ? v9[2] : (k == 1 ? v9[5] : v9[8]);
+                for (int k = 0; k < 3; ++k) {
+                    if (k >= nr) break;
+                    const double* r = R + (size_t)(3 * k) * nc + c;
+                    z[k] = pold[k] - (nr == 1 ? r[0] * t0 : r[0] * t0 + r[nc] * t1 + r[2 * (size_t)nc] * t2);
+                }
+            }
+            project3(z, nr, muc);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= nr) break;
+                p[(size_t)k * nc + c] = z[k];
+                const double dp = z[k] - pold[k];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    rec[m] += d[k][m] * dp;
+                    rec[3 + m] += aA[k][m] * dp;  // zero for a static side
+                    rec[6 + m] += aB[k][m] * dp;
+                }
+            }
         }
+        double* l = lrec + 9 * (size_t)(c - s0);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) l[m] = rec[m];
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's table DMA has landed
     __syncthreads();                // ... and every wave's records and DMA
-    // One partial per (body, range) pair, 8 lanes per pair as k_pj_bodies8's gather.
-    const int l = threadIdx.x & 7;
+    // One partial per (body, range) pair: the pair's incidences in order, one lane per pair.
     const int npairs = q1 - q0;
 #pragma unroll 1
-    for (int base = 0; base < npairs; base += blockDim.x >> 3) {
-        const int j = base + (threadIdx.x >> 3);
-        const bool live = j < npairs;
-        int i0 = 0, i1 = 0, out = 0;
-        if (live) {
-            i0 = Lptr[j] - l0;
-            i1 = Lptr[j + 1] - l0;
-            out = Lout[j];
-        }
+    for (int j = threadIdx.x; j < npairs; j += blockDim.x) {
+        const int i0 = Lptr[j] - l0, i1 = Lptr[j + 1] - l0, out = Lout[j];
         double acc[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = i0 + l; i < i1; i += 8) {
+        for (int i = i0; i < i1; ++i) {
             const int e = Linc[i];
             const double* r = lrec + 9 * (e >> 1);
             if (e & 1) {
@@ -431,15 +522,10 @@ __global__ void __launch_bounds__(512, 8) k_pj_range(
                 }
             }
         }
-#pragma unroll
-        for (int m = 0; m < 6; ++m) acc[m] = oct_sum(acc[m]);
-        if (live && l < 6) {
-            double a = acc[0];
-#pragma unroll
-            for (int m = 1; m < 6; ++m)
-                if (l == m) a = acc[m];
-            part[(size_t)out * 6 + l] = a;
-        }
+        double2* o = reinterpret_cast<double2*>(part + (size_t)out * 6);
+        o[0] = make_double2(acc[0], acc[1]);
+        o[1] = make_double2(acc[2], acc[3]);
+        o[2] = make_double2(acc[4], acc[5]);
     }
 }
 
@@ -1551,6 +1637,7 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
     }
     ctx->solved = false;
     ctx->ranges_used = false;
+    ctx->geo_used = false;
     ctx->pj_last = false;
     ctx->pjj_last = false;
     ctx->last_theta = theta;
@@ -1765,22 +1852,30 @@ static int run_range_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int* ite
     const int check_every = prm->check_every;
     const bool block = prm->block_precond != 0;
     const int gb8 = grid_for(nb * kLanesPerBody);
-    const int nthr = ctx->range_block;
+    // one lane per contact of a range (and per pair afterwards): up to 256 threads, looping over
+    // longer ranges
+    const int nthr = std::max(64, std::min({ctx->range_block, 256, pad64(std::min(per, nc))}));
     const int maxs = std::min(per, nc);
     const size_t shm = range_lds_bytes(maxs, ctx->range_maxp, ctx->range_maxl);
     CDX_TRY(ensure_alloc(ctx, &ctx->d_partial, (size_t)gb8, &ctx->partial_cap, "residual partials"));
     ctx->ranges_used = true;
+    const bool geo = ctx->geo_ok && ctx->geo_mode;
+    ctx->geo_used = geo;
     auto launch_sweep = [&](int it, int check) -> int {
-        if (block)
-            CDX_LAUNCH_SHM(ctx, "pj_range", k_pj_range<true>, ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,
-                           ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,
-                           ctx->d_J, ctx->d_Rb, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,
-                           ctx->d_part, ctx->d_ctrl);
+#define CDX_RANGE_LAUNCH(BLK, GEO)                                                                                    \
+    CDX_LAUNCH_SHM(ctx, "pj_range", (k_pj_range<BLK, GEO>), ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,         \
+                   ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,     \
+                   ctx->d_J, BLK ? ctx->d_Rb : ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p,     \
+                   ctx->d_x, ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q)
+        if (block && geo)
+            CDX_RANGE_LAUNCH(true, true);
+        else if (block)
+            CDX_RANGE_LAUNCH(true, false);
+        else if (geo)
+            CDX_RANGE_LAUNCH(false, true);
         else
-            CDX_LAUNCH_SHM(ctx, "pj_range", k_pj_range<false>, ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,
-                           ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,
-                           ctx->d_J, ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,
-                           ctx->d_part, ctx->d_ctrl);
+            CDX_RANGE_LAUNCH(false, false);
+#undef CDX_RANGE_LAUNCH
         CDX_LAUNCH(ctx, "pj_bodies_parts", k_pj_bodies_parts, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_bp_ptr,
                    ctx->d_bp_ovf, ctx->d_part, ctx->d_x, check, prm->tol_rel, prm->tol_abs, ctx->d_partial,
                    ctx->d_ctrl, gyro_sinv(ctx, prm));
@@ -2470,5 +2565,52 @@ extern "C" int cdx_range_sweep_info(cdx_ctx* ctx, int* used, int* nranges, int* 
     if (nranges) *nranges = ctx->ranges_ok ? ctx->nranges : 0;
     if (npartials) *npartials = ctx->ranges_ok ? ctx->npart : 0;
     if (contacts_per_range) *contacts_per_range = ctx->ranges_ok ? ctx->range_per_used : 0;
+    return CDX_OK;
+}
+
+// Geometry form of the contact rows (k_geo_pack, k_pj_range<*, true>).
+extern "C" int cdx_set_contact_points(cdx_ctx* ctx, const double* point) {
+    if (!ctx) return CDX_BAD_ARG;
+    hipSetDevice(ctx->device);
+    const int nc = ctx->nc;
+    ctx->geo_ok = false;
+    ctx->geo_bad = -1;
+    if (nc > 0 && !point) return set_err(ctx, CDX_BAD_ARG, "cdx_set_contact_points: null array");
+    if (nc > 0 && !ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_set_contact_points: no positions were set");
+    for (size_t i = 0; i < 3 * (size_t)nc; ++i)
+        if (!std::isfinite(point[i])) return set_err(ctx, CDX_NONFINITE, "cdx_set_contact_points: non-finite point");
+    if (nc == 0) {
+        ctx->geo_bad = 0;
+        return CDX_OK;
+    }
+    const int gc = grid_for(nc);
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_geo, 6 * (size_t)nc, &ctx->geo_cap, "contact lever arms"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_geo_pt, 3 * (size_t)nc, &ctx->geo_pt_cap, "contact points"));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_geo_bad, (size_t)gc, &ctx->geo_bad_cap, "contact point check"));
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_geo_pt, point, sizeof(double) * 3 * (size_t)nc, hipMemcpyHostToDevice, ctx->stream));
+    CDX_LAUNCH(ctx, "geo_pack", k_geo_pack, gc, kBlock, nc, ctx->d_ida, ctx->d_idb, ctx->d_dof, ctx->d_q, ctx->d_J,
+               ctx->d_geo_pt, ctx->d_geo, ctx->d_geo_bad);
+    std::vector<int> bad(gc);
+    CDX_HIP(ctx, hipMemcpyAsync(bad.data(), ctx->d_geo_bad, sizeof(int) * (size_t)gc, hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    long long nbad = 0;
+    for (int v : bad) nbad += v;
+    ctx->geo_bad = (int)nbad;
+    ctx->geo_ok = nbad == 0;
+    return CDX_OK;
+}
+
+extern "C" int cdx_set_geometry_rows(cdx_ctx* ctx, int mode) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (mode != 0 && mode != 1) return set_err(ctx, CDX_BAD_ARG, "cdx_set_geometry_rows: mode must be 0 or 1");
+    ctx->geo_mode = mode;
+    return CDX_OK;
+}
+
+extern "C" int cdx_geometry_rows_info(cdx_ctx* ctx, int* usable, int* mismatches, int* used) {
+    if (!ctx) return CDX_BAD_ARG;
+    if (usable) *usable = ctx->geo_ok ? 1 : 0;
+    if (mismatches) *mismatches = ctx->geo_bad;
+    if (used) *used = ctx->geo_used ? 1 : 0;
     return CDX_OK;
 }

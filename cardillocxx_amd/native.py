@@ -66,6 +66,9 @@ EXPORTED = [
     "cdx_set_graphs",
     "cdx_set_range_sweep",
     "cdx_range_sweep_info",
+    "cdx_set_contact_points",
+    "cdx_set_geometry_rows",
+    "cdx_geometry_rows_info",
     "cdx_kernel_stat",
     "cdx_get_stream",
 ]
@@ -141,6 +144,9 @@ def _declare(lib):
         "cdx_set_graphs": (C.c_int, [_vp, C.c_int]),
         "cdx_set_range_sweep": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
         "cdx_range_sweep_info": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
+        "cdx_set_contact_points": (C.c_int, [_vp, _dp]),
+        "cdx_set_geometry_rows": (C.c_int, [_vp, C.c_int]),
+        "cdx_geometry_rows_info": (C.c_int, [_vp, _ip, _ip, _ip]),
         "cdx_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
         "cdx_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _ip, _dp]),
         "cdx_kernel_time": (C.c_int, [_vp, C.c_char_p, _ip, _dp]),
@@ -421,6 +427,20 @@ class Context:
     def set_range_sweep(self, mode: int, contacts_per_range: int = 0, block: int = 0):
         """Range sweep of projected Jacobi: mode 0 off, 1 from 32,768 contacts (default), 2 always."""
         self._check(self._l.cdx_set_range_sweep(self.h, mode, contacts_per_range, block), "cdx_set_range_sweep")
+
+    def set_contact_points(self, point):
+        """World contact points [nc, 3] for the geometry form of the rows (cdx_set_contact_points)."""
+        pt = np.ascontiguousarray(point, dtype=np.float64).reshape(-1)
+        self._check(self._l.cdx_set_contact_points(self.h, dptr(pt)), "cdx_set_contact_points")
+
+    def set_geometry_rows(self, mode: int):
+        self._check(self._l.cdx_set_geometry_rows(self.h, mode), "cdx_set_geometry_rows")
+
+    def geometry_rows_info(self):
+        """(usable, contacts whose rebuilt rows differ or -1, used by the last solve)."""
+        v = [C.c_int(0) for _ in range(3)]
+        self._check(self._l.cdx_geometry_rows_info(self.h, *[C.byref(x) for x in v]), "cdx_geometry_rows_info")
+        return bool(v[0].value), v[1].value, bool(v[2].value)
 
     def range_sweep_info(self):
         """(used by the last solve, ranges, partial sums per sweep, contacts per range)."""

@@ -300,6 +300,27 @@ int cdx_set_range_sweep(cdx_ctx* ctx, int mode, int contacts_per_range, int bloc
  * until a range solve has built them). */
 int cdx_range_sweep_info(cdx_ctx* ctx, int* used, int* nranges, int* npartials, int* contacts_per_range);
 
+/* World contact points of the current contacts (3 doubles per contact, the reference Contact's
+ * point), for the geometry form of the rows: a rigid side's angular row k is r_body x (R^T d_k),
+ * r_body = R^T (p - x_com) (makeContact + buildContactRowByDof, contact_jacobian.hpp:17-23;
+ * rebuildW_, dynamics_assembler.cpp:211-332).  The library derives every rigid side's r_body from
+ * the point and the body's current position and quaternion, rebuilds the angular rows from them
+ * with the contact generator's arithmetic (R from the quaternion entry by entry, then R^T d, then
+ * the cross product, no contraction) and compares them with the rows passed to cdx_set_contacts.
+ * When every one matches bit for bit, the range sweep rebuilds the rows in each sweep from the
+ * lever arms and the quaternions instead of reading them (48 instead of 144 bytes per contact);
+ * the results are identical.  Otherwise the sweep reads the rows as always.  Call after
+ * cdx_set_contacts; cdx_set_bodies, cdx_set_contacts and the Moreau position updates discard the
+ * points.  No counterpart in the reference (which builds W from these points each step). */
+int cdx_set_contact_points(cdx_ctx* ctx, const double* point);
+/* mode 1 (default): the range sweep uses the geometry form when the points match; 0: never.
+ * No counterpart in the reference. */
+int cdx_set_geometry_rows(cdx_ctx* ctx, int mode);
+/* usable: the current points reproduce every angular row; mismatches: contacts whose rebuilt rows
+ * differ in any bit (-1 when no points were given for the current state); used: the last
+ * cdx_pj_solve swept with the geometry form. */
+int cdx_geometry_rows_info(cdx_ctx* ctx, int* usable, int* mismatches, int* used);
+
 /* ---- Allocation-free projected Jacobi.  No counterpart in the reference: this entry point is for
  * callers that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc,
  * code that must not grow the device heap).  It runs the standard loop of cdx_pj_solve on arrays

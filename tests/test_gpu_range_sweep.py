@@ -73,7 +73,7 @@ CASES = [
     dict(nb=1000, nc=1500, seed=6, warm=False),
     dict(nb=3000, nc=9000, seed=7),
 ]
-SHAPES = [(16, 256), (256, 512), (640, 256)]  # (contacts per range, threads): 640/256 is 10 rounds
+SHAPES = [(16, 256), (256, 512), (512, 256)]  # (contacts per range, threads): 512/256 is 8 rounds
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"nb{c['nb']}_nc{c['nc']}_s{c['seed']}")
