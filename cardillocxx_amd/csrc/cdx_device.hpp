@@ -181,8 +181,10 @@ __device__ inline double4 load_minv4(const double* __restrict__ minv, int b) {
 
 // D = sum_j W_kj^2 M^-1_j of one row (rdiag_sparse, projected_jacobi.cpp:57-69): summed side by
 // side in column order (lower body index first), as the row-major compressed W stores the row.
+// Without contraction: the same operations as the oracle's delassus_diag.
 __device__ inline double row_D(const double d[3], const double aA[3], const double aB[3], double4 mA, double4 mB, int a,
                                int b) {
+#pragma clang fp contract(off)
     double DA = 0, DB = 0;
     DA += d[0] * d[0] * mA.x;
     DA += d[1] * d[1] * mA.x;

@@ -305,9 +305,9 @@ __device__ __forceinline__ void dma_ints(const int* __restrict__ src, int n, int
 // quaternion and rebuilds every angular row; where all of them equal the rows passed to
 // cdx_set_contacts bit for bit, the range sweep rebuilds them from (r_body, quaternion, frame) in
 // every sweep instead of reading them: 48 bytes of lever arms per contact in place of 144 bytes of
-// angular rows, the quaternions being gathered with the twists.  The arithmetic is the host contact
-// generator's, operation by operation and without contraction, so that the rebuilt rows are the
-// same numbers.
+// angular rows, the quaternions being gathered with the twists; the second tangent is rebuilt as
+// n x t1 where that holds bit for bit too.  The arithmetic is the host contact generator's,
+// operation by operation and without contraction, so that the rebuilt rows are the same numbers.
 
 // R^T v for the unit quaternion (qx, qy, qz, qw) (Eigen's coefficient order), matrix entries first.
 __device__ inline void geo_rot_t(const double* __restrict__ q, const double v[3], double out[3]) {
@@ -337,7 +337,8 @@ __device__ inline void geo_row(const double* __restrict__ q, const double r[3], 
 
 // One thread per contact: the lever arms r_body = R^T (p - x_com) of its rigid sides into G
 // (component-major, rA then rB; NaN for a static or point-mass side, which has no angular row) and
-// a count of the contacts whose rebuilt angular rows differ from J's in any bit, per workgroup.
+// a count of the contacts whose rebuilt angular rows or second tangent n x t1 differ from J's in any
+// bit, per workgroup.
 __global__ void __launch_bounds__(kBlock) k_geo_pack(int nc, const int* __restrict__ ida, const int* __restrict__ idb,
                                                      const int* __restrict__ dof, const double* __restrict__ q,
                                                      const double* __restrict__ J, const double* __restrict__ point,
@@ -359,6 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_geo_pack(int nc, const int* __restri
                     rel[2] = point[3 * (size_t)c + 2] - qb[2];
                 }
                 geo_rot_t(qb + 3, rel, r);
+#pragma unroll 1
                 for (int k = 0; k < 3; ++k) {
                     double d[3], a[3];
                     for (int m = 0; m < 3; ++m) d[m] = J[(size_t)(3 * k + m) * nc + c];
@@ -370,6 +372,15 @@ __global__ void __launch_bounds__(kBlock) k_geo_pack(int nc, const int* __restri
             }
             for (int m = 0; m < 3; ++m) G[(size_t)(3 * side + m) * nc + c] = r[m];
         }
+        // the second tangent is rebuilt as n x t1
+        double n[3], t1[3], t2[3];
+        for (int m = 0; m < 3; ++m) {
+            n[m] = J[(size_t)m * nc + c];
+            t1[m] = J[(size_t)(3 + m) * nc + c];
+        }
+        geo_cross(n, t1, t2);
+        for (int m = 0; m < 3; ++m)
+            if (__double_as_longlong(t2[m]) != __double_as_longlong(J[(size_t)(6 + m) * nc + c])) bad = 1;
     }
     const double s = block_sum(bad, red);
     if (threadIdx.x == 0) bad_part[blockIdx.x] = (int)s;
@@ -394,10 +405,10 @@ __device__ __forceinline__ void project3(double z[3], int nr, double muc) {
 //
 // Block = true: R holds the inverse Delassus blocks and the bias is unscaled
 // (pj.rdiag_true_delassus): dp = Pi(p - Rb (W x + b)) - p.  Geo = true: the angular rows are
-// rebuilt from the lever arms G and the quaternions q (the geometry form above) instead of being
-// read from J.
+// rebuilt from the lever arms G and the quaternions q (the geometry form above) and the second
+// tangent as n x t1 instead of being read from J.
 template <bool Block, bool Geo>
-__global__ void __launch_bounds__(256) k_pj_range(
+__global__ void __launch_bounds__(256, 4) k_pj_range(
     int nranges, int nc, int per, int maxs, int maxp, const int* __restrict__ rg_p, const int* __restrict__ rg_l,
     const int* __restrict__ pr_ptr, const int* __restrict__ pr_out, const int* __restrict__ pr_inc,
     const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
@@ -432,7 +443,7 @@ __global__ void __launch_bounds__(256) k_pj_range(
             for (int k = 0; k < 3; ++k)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    d[k][m] = J[(size_t)(3 * k + m) * nc + c];
+                    if (!Geo || k < 2) d[k][m] = J[(size_t)(3 * k + m) * nc + c];
                     if constexpr (!Geo) {
                         aA[k][m] = J[(size_t)(9 + 3 * k + m) * nc + c];
                         aB[k][m] = J[(size_t)(18 + 3 * k + m) * nc + c];
@@ -442,6 +453,7 @@ __global__ void __launch_bounds__(256) k_pj_range(
                     }
                 }
             if constexpr (Geo) {
+                geo_cross(d[0], d[1], d[2]);  // t2 = n x t1 (checked bit for bit by k_geo_pack)
                 double rA[3], rB[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {

@@ -100,6 +100,11 @@ void fill_contact(const RawContact& rc, const double* q, const double* mu_body, 
     mu[c] = std::min(std::max(0.0, ma), std::max(0.0, mb));  // combineFrictionMu "min" (:94-106)
     double nn[3] = {rc.n.x, rc.n.y, rc.n.z}, t1[3], t2[3];
     cdx_tangent_frame(nn, t1, t2);
+    // the normal row is the unit normal the frame was built from (n / |n|, as cdx_tangent_frame
+    // normalises it; at most an ulp from the narrowphase's), so that t2 = n x t1 holds exactly
+    const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    if (len > 2.220446049250313e-16)
+        for (double& x : nn) x /= len;
     const V3 d[3] = {{nn[0], nn[1], nn[2]}, {t1[0], t1[1], t1[2]}, {t2[0], t2[1], t2[2]}};
     for (int k = 0; k < 3; ++k) {
         frame[9 * (size_t)c + 3 * k + 0] = d[k].x;

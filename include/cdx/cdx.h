@@ -307,9 +307,10 @@ int cdx_range_sweep_info(cdx_ctx* ctx, int* used, int* nranges, int* npartials, 
  * the point and the body's current position and quaternion, rebuilds the angular rows from them
  * with the contact generator's arithmetic (R from the quaternion entry by entry, then R^T d, then
  * the cross product, no contraction) and compares them with the rows passed to cdx_set_contacts.
- * When every one matches bit for bit, the range sweep rebuilds the rows in each sweep from the
- * lever arms and the quaternions instead of reading them (48 instead of 144 bytes per contact);
- * the results are identical.  Otherwise the sweep reads the rows as always.  Call after
+ * It also checks that each frame's second tangent is n x t1.  When every one matches bit for bit,
+ * the range sweep rebuilds the rows in each sweep from the lever arms, the quaternions and the
+ * normal and first tangent instead of reading them (96 instead of 216 bytes per contact); the
+ * results are identical.  Otherwise the sweep reads the rows as always.  Call after
  * cdx_set_contacts; cdx_set_bodies, cdx_set_contacts and the Moreau position updates discard the
  * points.  No counterpart in the reference (which builds W from these points each step). */
 int cdx_set_contact_points(cdx_ctx* ctx, const double* point);

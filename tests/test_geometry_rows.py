@@ -4,7 +4,8 @@ IEEE operations in a fixed order and no contraction, so that the device can rebu
 bit from the contact points (cdx_set_contact_points, the geometry form of the range sweep).  Here,
 on the CPU: the rows of a sphere pile and of a mixed sphere/box scene with random orientations equal
 a numpy evaluation of the same operations (numpy ufuncs round every operation and never contract),
-and a planted change of the operation order breaks that equality.  The device side of the claim
+the second tangent is n x t1 bit for bit, and a planted change of the operation order breaks that
+equality.  The device side of the claim
 (k_geo_pack finds no mismatching contact) is in tests/test_gpu_geometry_rows.py."""
 import numpy as np
 import pytest
@@ -68,6 +69,10 @@ def pile_scene():
 def test_rows_are_the_plain_ieee_evaluation(build):
     q, dof, con = build()
     assert len(con["body_a"]) > 50
+    f = con["frame"]
+    # the normal row is the unit normal the frame was built from, so t2 = n x t1 exactly
+    np.testing.assert_array_equal(cross(f[:, 0:3], f[:, 3:6]).view(np.int64), f[:, 6:9].view(np.int64))
+    np.testing.assert_allclose(np.linalg.norm(f[:, 0:3], axis=1), 1.0, rtol=0, atol=4e-16)
     checked = 0
     for side, sel, ang in rebuilt_rows(q, dof, con):
         given = (con["angA"] if side == 0 else con["angB"])[sel]

@@ -85,7 +85,8 @@ def test_velocity_force_and_friction_gradients(ctx):
             p = prob.copy()
             p.mu[c] = x
             return gv @ forward(p, rows)
-        check(out["mu"][c], fd(Lmu, prob.mu[c], 1e-6), 1e-4, 1e-8)
+        # central differences with h = 1e-6 of solves converged to 1e-13 carry ~1e-7 of noise
+        check(out["mu"][c], fd(Lmu, prob.mu[c], 1e-6), 1e-4, 1e-7)
 
 
 def test_theta_and_spring_compliance(ctx):
