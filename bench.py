@@ -9,11 +9,14 @@ tol_abs = tol_rel = 1e-8 (checked every 10 sweeps), everything resident in HBM.
 contact-updates/s = contacts x sweeps / time.  Timed with chunks of sweeps replayed as HIP graphs
 (the production path).
 
-`roofline`: the dominant kernel (pj_contacts).  Its average duration comes from start/stop HIP
-events carried in every dispatch on the solve's own stream during one more step of the same
-workload right after the timed ones; `traffic` is the HBM bytes per launch from the committed
-rocprofv3 FETCH_SIZE/WRITE_SIZE summary when it was taken on this workload; `copy_GBs` is a
-device-to-device copy measured in the same run (a second denominator).
+`roofline`: the whole sweep (range pass + body pass + the residual every 10th sweep): the
+algorithmic bytes of every sweep over the timed solves' stream-event time; `kernels` gives each
+kernel's average duration from start/stop HIP events carried in every dispatch on the solve's own
+stream during one more step of the same workload right after the timed ones; `traffic` is the HBM
+bytes per sweep from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when it was taken on
+this workload and sweep form; `copy_GBs` is a device-to-device copy measured in the same run (a
+second denominator).  The pile's contact points go to the library (cdx_set_contact_points), so the
+range pass rebuilds the angular rows instead of reading them (--no-geometry-rows: read them).
 
 `cpu_baseline`: the CPU oracle (a C restatement of the reference PJ sweep: single-thread SpMV as
 the reference pins Eigen to one thread, OpenMP projection) on a bounded sample of the same
@@ -319,7 +322,8 @@ def pile_leg(args, native, scenes, metrics, hip):
         kernels["kernel_us_per_sweep"] = kern_sum
         kernels["timing"] = ("start/stop HIP events carried in every kernel dispatch (hipExtLaunchKernelGGL) on the "
                              "solve stream, one more solve of the same workload right after the timed ones")
-    traffic, traffic_src = committed_traffic(prob.nc)
+    # the committed counters were taken with the geometry form of the rows
+    traffic, traffic_src = committed_traffic(prob.nc) if g_used else (None, None)
 
     # accelerated loops on the same problem and tolerance: time to solution
     accel = None if args.no_accel else accel_leg(native, ctx, prob, dt, args, metrics, elapsed / args.steps,
