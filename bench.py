@@ -429,13 +429,15 @@ def accel_leg(native, ctx, prob, dt, args, metrics, t_std, sweeps_std, bytes_swe
         if name == "chebyshev" and not args.no_kernel_events:
             stats, pinfo = profiled_kernels(ctx, lambda: ctx.pj_solve(dt, 1.0, p))
             sw = max(pinfo.iters, 1)
-            per = {k: stats.get(k, (0, 0.0))[1] / sw * 1e3 for k in ("pjc_contacts", "pjc_bodies", "pjc_residual")}
+            # the range sweep's kernels (pjc_range, pjc_bodies_parts) or the two passes'
+            per = {k: stats.get(k, (0, 0.0))[1] / sw * 1e3
+                   for k in ("pjc_range", "pjc_bodies_parts", "pjc_contacts", "pjc_bodies", "pjc_residual")}
+            per = {k: v for k, v in per.items() if v > 0 or k == "pjc_residual"}
             us = sum(per.values())
             out[name]["kernels"] = {**{f"{k}_us_per_sweep": v for k, v in per.items()}, "kernel_us_per_sweep": us,
                                     "achieved_GBs_standard_bytes": bytes_sweep / (us * 1e-6) / 1e9,
                                     "timing": "HIP events in every dispatch, one more solve (direct launches)"}
-            log(f"chebyshev kernels: {us:.1f} us per sweep ({per['pjc_contacts']:.1f} contacts, "
-                f"{per['pjc_bodies']:.1f} bodies, {per['pjc_residual']:.1f} residual)")
+            log(f"chebyshev kernels: {us:.1f} us per sweep (" + ", ".join(f"{k} {v:.1f}" for k, v in per.items()) + ")")
     return out
 
 

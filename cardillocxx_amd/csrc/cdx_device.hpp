@@ -41,6 +41,21 @@ __device__ __forceinline__ double oct_sum(double v) {
     return v + half_mirror(v);
 }
 
+// Sum of v[i] over i = threadIdx.x + j kBlock (j = 0, 1, ...) with eight loads in flight per step
+// (a one-workgroup reduction over thousands of partials otherwise waits on one load at a time); the
+// accumulators meet in a fixed order, so the sum repeats bit for bit.
+__device__ inline double strided_sum(const double* __restrict__ v, int n) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int i = threadIdx.x;
+    for (; i + 7 * kBlock < n; i += 8 * kBlock)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += v[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+        if (i + u * kBlock < n) a[u] += v[i + u * kBlock];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __device__ inline void load_body(const double* __restrict__ x, int b, double u[6]) {
     if (b < 0) {
 #pragma unroll

@@ -452,11 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_gsn_finalize(int nparts, const doubl
                                                          double* __restrict__ nest) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
-    double s = 0, d = 0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) {
-        s += partial[i];
-        d += partial[nparts + i];
-    }
+    double s = strided_sum(partial, nparts), d = strided_sum(partial + nparts, nparts);
     s = block_sum(s, red);
     d = block_sum(d, red);
     if (threadIdx.x != 0) return;

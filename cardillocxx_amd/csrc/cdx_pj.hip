@@ -406,16 +406,28 @@ __device__ __forceinline__ void project3(double z[3], int nr, double muc) {
 // Block = true: R holds the inverse Delassus blocks and the bias is unscaled
 // (pj.rdiag_true_delassus): dp = Pi(p - Rb (W x + b)) - p.  Geo = true: the angular rows are
 // rebuilt from the lever arms G and the quaternions q (the geometry form above) and the second
-// tangent as n x t1 instead of being read from J.
-template <bool Block, bool Geo>
+// tangent as n x t1 instead of being read from J.  Cheb = true (row scaling only): a sweep of the
+// Chebyshev loop (chebyshev_loop, projected_jacobi.cpp:180-228) -- its input is the extrapolation
+// w (x - xk) + xk of the twists and w (p - pk) + pk of the impulses (or x, p after a reset:
+// cheb[1]), formed here as k_pjc_contacts4 forms it, with the impulse history rotated
+// (pk <- pk1, pk1 <- input); the twists' history is rotated by k_pjc_bodies_parts.
+template <bool Block, bool Geo, bool Cheb>
 __global__ void __launch_bounds__(256, 4) k_pj_range(
     int nranges, int nc, int per, int maxs, int maxp, const int* __restrict__ rg_p, const int* __restrict__ rg_l,
     const int* __restrict__ pr_ptr, const int* __restrict__ pr_out, const int* __restrict__ pr_inc,
     const int* __restrict__ ida, const int* __restrict__ idb, const double* __restrict__ J,
     const double* __restrict__ R, const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
     double* __restrict__ p, const double* __restrict__ x, double* __restrict__ part, const int* __restrict__ ctrl,
-    const double* __restrict__ G, const double* __restrict__ q) {
+    const double* __restrict__ G, const double* __restrict__ q, double* __restrict__ PK, double* __restrict__ PK1,
+    const double* __restrict__ XK, const double* __restrict__ cheb) {
+    static_assert(!(Block && Cheb), "the range form of the Chebyshev loop takes the row scaling");
     if (ctrl[0]) return;
+    double w = 1.0;
+    bool take = true;
+    if constexpr (Cheb) {
+        w = cheb[0];
+        take = cheb[1] != 0.0;
+    }
     // LDS: the records (9 doubles per contact of the range: lin | angA | angB, summed over its
     // rows), then the range's tables, copied by DMA while the contacts are swept: the incidence
     // offsets of its pairs (body, range), their partial slots, and the incidences (contact in the
@@ -468,13 +480,24 @@ __global__ void __launch_bounds__(256, 4) k_pj_range(
                     for (int k = 0; k < 3; ++k) geo_row(q + (size_t)b * kBodyStride + 3, rB, d[k], aB[k]);
             }
             double ua[6], ub[6];
-            load_body(x, a, ua);
-            load_body(x, b, ub);
+            if constexpr (Cheb) {
+                cheb_input6(x, XK, a, w, take, ua);
+                cheb_input6(x, XK, b, w, take, ub);
+            } else {
+                load_body(x, a, ua);
+                load_body(x, b, ub);
+            }
             double pold[3] = {0, 0, 0}, z[3] = {0, 0, 0};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if (k >= nr) break;
                 pold[k] = p[(size_t)k * nc + c];
+                if constexpr (Cheb) {  // the sweep's input impulse, and the history rotated
+                    const size_t r = (size_t)k * nc + c;
+                    pold[k] = take ? pold[k] : cheb_extrap(w, pold[k], PK[r]);
+                    PK[r] = PK1[r];
+                    PK1[r] = pold[k];
+                }
                 const double gk = row_dot(d[k], aA[k], aB[k], ua, ub);
                 const double bk = use_bias ? bias[(size_t)k * nc + c] : 0.0;
                 if constexpr (Block)
@@ -605,6 +628,72 @@ __global__ void __launch_bounds__(kBlock) k_pj_bodies_parts(int nb, const int* _
     }
 }
 
+// Body pass of a Chebyshev range sweep (as k_pjc_bodies8 over the partials): input
+// u = w (X - XK) + XK (or X), XK <- XK1, XK1 <- u, X <- u + S^-1 (sum of the body's partials); per-block
+// sums of the residual terms (every sweep) and of non-finite entries of X.
+__global__ void __launch_bounds__(kBlock) k_pjc_bodies_parts(int nb, const int* __restrict__ dof,
+                                                             const double* __restrict__ minv, const int* __restrict__ pcnt,
+                                                             const int* __restrict__ povf, const double* __restrict__ part,
+                                                             double* __restrict__ X, double* __restrict__ XK,
+                                                             double* __restrict__ XK1, double tol_rel, double tol_abs,
+                                                             double* __restrict__ partial, const int* __restrict__ ctrl,
+                                                             const double* __restrict__ cheb,
+                                                             const double* __restrict__ sinv) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const double w = cheb[0];
+    const bool take = cheb[1] != 0.0;
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    double e2 = 0, bad = 0;
+    if (bi < nb && l < 6) {
+        const int nd = dof[bi], n = pcnt[bi];
+        const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+        const size_t o = (size_t)bi * kBodyStride + l;
+        const double xs = X[o];
+        double ps[kPartSlots];
+#pragma unroll
+        for (int j = 0; j < kPartSlots; ++j) ps[j] = part[((size_t)bi * kPartSlots + j) * 6 + l];
+        if (l < nd) {
+            const double u = take ? xs : cheb_extrap(w, xs, XK[o]);
+            XK[o] = XK1[o];
+            XK1[o] = u;
+            double a = 0;
+#pragma unroll
+            for (int j = 0; j < kPartSlots; ++j)
+                if (j < n) a += ps[j];
+            if (n > kPartSlots) {
+                const size_t ov = (size_t)nb * kPartSlots + povf[bi];
+                for (int j = kPartSlots; j < n; ++j) a += part[(ov + j - kPartSlots) * 6 + l];
+            }
+            const double M = mk > 0 ? 1.0 / mk : 0.0;
+            double d = a / M;
+            if (sinv && nd == 6 && l >= 3) {  // implicit gyroscopy, as k_pj_bodies_parts
+                double a3[3] = {0, 0, 0};
+                for (int j = 0; j < n; ++j) {
+                    const size_t oo = j < kPartSlots ? ((size_t)bi * kPartSlots + j) * 6
+                                                     : ((size_t)nb * kPartSlots + povf[bi] + j - kPartSlots) * 6;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) a3[k] += part[oo + 3 + k];
+                }
+                d = gyro_row(sinv + (size_t)bi * 9 + 3 * (l - 3), a3[0], a3[1], a3[2]);
+            }
+            const double un = n > 0 ? u + d : u;
+            X[o] = un;
+            const double qq = fmax(fabs(un), fabs(u)) * tol_rel + tol_abs;
+            const double e = (un - u) / qq;
+            e2 = e * e;
+            bad = isfinite(un) ? 0.0 : 1.0;
+        }
+    }
+    const double s = block_sum(e2, red);
+    const double nf = block_sum(bad, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = nf;
+    }
+}
+
 // mode 0: PJ (break when res <= 1); mode 1: GS (break when res < 1, NaN/Inf -> diverged).
 // it < 0: the sweep index comes from the device chunk counter ctrl[3] (graph-replayed chunks of
 // `cadence` sweeps whose residual check is the chunk's first sweep); the counter advances on
@@ -615,7 +704,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_finalize(int nparts, const 
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
     double s = 0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) s += partial[i];
+    s = strided_sum(partial, nparts);
     s = block_sum(s, red);
     if (threadIdx.x == 0) {
         if (it < 0) it = ctrl[3] * cadence;
@@ -822,11 +911,7 @@ __global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const doubl
                                                          double* __restrict__ nest) {
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
-    double s = 0, d = 0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) {
-        s += partial[i];
-        d += partial[nparts + i];
-    }
+    double s = strided_sum(partial, nparts), d = strided_sum(partial + nparts, nparts);
     s = block_sum(s, red);
     d = block_sum(d, red);
     if (threadIdx.x != 0) return;
@@ -892,16 +977,6 @@ __global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const doubl
 // fallback), 2 rho^2, 3 rho, 4 power iteration stopped, 5 lambda, 6 current norm.
 // ---------------------------------------------------------------------------------------------
 
-
-__device__ inline void cheb_input6(const double* __restrict__ X, const double* __restrict__ XK, int b, double w, bool take,
-                                   double u[6]) {
-    load_body(X, b, u);
-    if (take || b < 0) return;
-    double k6[6];
-    load_body(XK, b, k6);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) u[m] = cheb_extrap(w, u[m], k6[m]);
-}
 
 __global__ void k_cheb_fill(int nb, const int* __restrict__ dof, double inv, double* __restrict__ v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1014,7 +1089,7 @@ __global__ void __launch_bounds__(kBlock) k_cheb_norm(int nparts, const double* 
     if (cheb[4] != 0.0) return;
     __shared__ double red[kBlock / 64];
     double s = 0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) s += partial[i];
+    s = strided_sum(partial, nparts);
     s = block_sum(s, red);
     if (threadIdx.x == 0) {
         const double n = sqrt(s);
@@ -1150,11 +1225,7 @@ __global__ void __launch_bounds__(kBlock) k_pjc_finalize(int nparts, const doubl
 #pragma clang fp contract(off)
     if (ctrl[0]) return;
     __shared__ double red[kBlock / 64];
-    double s = 0, nf = 0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) {
-        s += partial[i];
-        nf += partial[nparts + i];
-    }
+    double s = strided_sum(partial, nparts), nf = strided_sum(partial + nparts, nparts);
     s = block_sum(s, red);
     nf = block_sum(nf, red);
     if (threadIdx.x != 0) return;
@@ -1875,10 +1946,10 @@ static int run_range_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int* ite
     ctx->geo_used = geo;
     auto launch_sweep = [&](int it, int check) -> int {
 #define CDX_RANGE_LAUNCH(BLK, GEO)                                                                                    \
-    CDX_LAUNCH_SHM(ctx, "pj_range", (k_pj_range<BLK, GEO>), ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,         \
+    CDX_LAUNCH_SHM(ctx, "pj_range", (k_pj_range<BLK, GEO, false>), ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,  \
                    ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,     \
                    ctx->d_J, BLK ? ctx->d_Rb : ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p,     \
-                   ctx->d_x, ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q)
+                   ctx->d_x, ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q, nullptr, nullptr, nullptr, nullptr)
         if (block && geo)
             CDX_RANGE_LAUNCH(true, true);
         else if (block)
@@ -2063,13 +2134,41 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     CDX_HIP(ctx, hipMemcpyAsync(ctx->d_xk1, ctx->d_x, sizeof(double) * n8, hipMemcpyDeviceToDevice, ctx->stream));
     CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
     CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk1, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
+    // the range sweep (one lane per contact, per-body partials) on large problems, row scaling only
+    bool ranges = false;
+    if (!Rb && (ctx->range_mode == 2 || (ctx->range_mode == 1 && nc >= kRangeAutoContacts))) {
+        if (!ctx->ranges_valid) CDX_TRY(build_ranges(ctx));
+        ranges = ctx->ranges_ok;
+    }
+    ctx->ranges_used = ranges;
+    const bool geo = ranges && ctx->geo_ok && ctx->geo_mode;
+    ctx->geo_used = geo;
+    const int ng = ctx->nranges, per = ctx->range_per_used;
+    const int nthr = ranges ? std::max(64, std::min({ctx->range_block, 256, pad64(std::min(per, nc))})) : 0;
+    const size_t shm = ranges ? range_lds_bytes(std::min(per, nc), ctx->range_maxp, ctx->range_maxl) : 0;
     auto launch_sweep = [&](int it, int) -> int {
-        CDX_LAUNCH(ctx, "pjc_contacts", k_pjc_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                   ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_x, ctx->d_uprev,
-                   ctx->d_imp, ctx->d_ctrl, ctx->d_cheb, Rb);
-        CDX_LAUNCH(ctx, "pjc_bodies", k_pjc_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                   ctx->d_imp, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
-                   ctx->d_cheb, gyro_sinv(ctx, prm));
+        if (ranges) {
+#define CDX_CHEB_RANGE(GEO)                                                                                             \
+    CDX_LAUNCH_SHM(ctx, "pjc_range", (k_pj_range<false, GEO, true>), ng, nthr, shm, ng, nc, per, std::min(per, nc),      \
+                   ctx->range_maxp, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida,  \
+                   ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,   \
+                   ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q, ctx->d_pk, ctx->d_pk1, ctx->d_uprev, ctx->d_cheb)
+            if (geo)
+                CDX_CHEB_RANGE(true);
+            else
+                CDX_CHEB_RANGE(false);
+#undef CDX_CHEB_RANGE
+            CDX_LAUNCH(ctx, "pjc_bodies_parts", k_pjc_bodies_parts, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_bp_ptr,
+                       ctx->d_bp_ovf, ctx->d_part, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs,
+                       ctx->d_partial2, ctx->d_ctrl, ctx->d_cheb, gyro_sinv(ctx, prm));
+        } else {
+            CDX_LAUNCH(ctx, "pjc_contacts", k_pjc_contacts4, gc4, kBlock, nc, gc4, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                       ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_pk1, ctx->d_x,
+                       ctx->d_uprev, ctx->d_imp, ctx->d_ctrl, ctx->d_cheb, Rb);
+            CDX_LAUNCH(ctx, "pjc_bodies", k_pjc_bodies8, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
+                       ctx->d_incidx, ctx->d_imp, ctx->d_x, ctx->d_uprev, ctx->d_xk1, prm->tol_rel, prm->tol_abs,
+                       ctx->d_partial2, ctx->d_ctrl, ctx->d_cheb, gyro_sinv(ctx, prm));
+        }
         CDX_LAUNCH(ctx, "pjc_residual", k_pjc_finalize, 1, kBlock, gb8, ctx->d_partial2, ctx->nv, it, ctx->d_ctrl, ctx->d_res,
                    ctx->h_flag_dev, ctx->d_cheb);
         return hip_check(ctx, hipGetLastError(), "pj chebyshev sweep launch");

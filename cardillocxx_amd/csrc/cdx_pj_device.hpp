@@ -225,6 +225,17 @@ __device__ inline double cheb_extrap(double w, double x, double xk) {
     return w * (x - xk) + xk;
 }
 
+// Body b's twist as the input of a Chebyshev sweep: X, or w (X - XK) + XK (zeros for a static side).
+__device__ inline void cheb_input6(const double* __restrict__ X, const double* __restrict__ XK, int b, double w, bool take,
+                                   double u[6]) {
+    load_body(X, b, u);
+    if (take || b < 0) return;
+    double k6[6];
+    load_body(XK, b, k6);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = cheb_extrap(w, u[m], k6[m]);
+}
+
 // The next Chebyshev weight after sweep `it` (projected_jacobi.cpp:202): 1 / (1 - rho^2 / 2) after
 // the first sweep, 1 / (1 - rho^2 / 4 w) after later ones.
 __device__ inline double cheb_next_w(int it, double rho2, double w) {

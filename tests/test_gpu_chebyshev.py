@@ -115,3 +115,40 @@ def test_chebyshev_deterministic_and_faster(ctx):
     ctx.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
     ctx.set_contacts(*prob.contact_arrays())
     assert ctx.pj_solve(1e-3, 1.0, p).iters > 2 * info.iters
+
+
+@pytest.fixture
+def rctx():
+    """A context that sweeps contact ranges whatever the problem size (cdx_set_range_sweep mode 2)."""
+    c = native.Context(0)
+    c.set_range_sweep(2)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"nb{c['nb']}_nc{c['nc']}_s{c['seed']}")
+@pytest.mark.parametrize("sweeps", [1, 2, 9, 64])
+def test_chebyshev_range_sweep_fixed_sweeps(rctx, case, sweeps):
+    """The Chebyshev loop on the range sweep (k_pj_range<*, *, true> + k_pjc_bodies_parts)."""
+    prob = scenes.random_problem(**case)
+    (v, imp, info), (v_ref, imp_ref, res) = both(rctx, prob, max_iter=sweeps, tol_abs=0.0, tol_rel=0.0)
+    assert rctx.range_sweep_info()[0]
+    assert info.iters == res.iters == sweeps
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+@pytest.mark.parametrize("case", CONVERGING, ids=lambda c: f"nb{c['nb']}_nc{c['nc']}_s{c['seed']}")
+@pytest.mark.parametrize("tol", [1e-5, 1e-9])
+def test_chebyshev_range_sweep_to_tolerance(rctx, case, tol):
+    prob = scenes.random_problem(**case)
+    (v, imp, info), (v_ref, imp_ref, res) = both(rctx, prob, tol_abs=tol, tol_rel=tol)
+    assert rctx.range_sweep_info()[0]
+    assert info.iters == res.iters and res.residual <= 1
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+def test_chebyshev_block_mode_keeps_the_two_passes(rctx):
+    prob = scenes.random_problem(nb=50, nc=200, seed=3, p_point=0.3)
+    (v, imp, info), (v_ref, imp_ref, res) = both(rctx, prob, max_iter=9, tol_abs=0.0, tol_rel=0.0, block_precond=1)
+    assert not rctx.range_sweep_info()[0]
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL

@@ -4,8 +4,9 @@ quaternions and finds them equal, bit for bit, to the rows the host contact gene
 sphere pile with spun orientations, a sphere/box scene); the range sweep that rebuilds the rows in
 every sweep then gives exactly the bits of the range sweep that reads them, after fixed sweeps and
 to tolerance, row-scaled and block, and matches the oracle's pj_sweep (projected_jacobi.cpp:121-138)
-to <= 1e-10.  A row changed in its last bit is found (the sweep then reads the rows), and new
-positions, new contacts or a Moreau drift discard the points."""
+to <= 1e-10; likewise the Chebyshev loop's range sweep.  A row changed in its last bit is found
+(the sweep then reads the rows), and new positions, new contacts or a Moreau drift discard the
+points."""
 import numpy as np
 import pytest
 
@@ -69,6 +70,19 @@ def solve(c, prob, points, **kw):
     upload(c, prob, points)
     info = c.pj_solve(5e-4, 1.0, params(**kw))
     return c.velocities(), c.impulses(), info, c.geometry_rows_info()
+
+
+def test_chebyshev_loop_same_bits(gctx):
+    """The Chebyshev loop's range sweep with the rebuilt rows against the same loop reading them."""
+    prob = pile_problem(n=2000, seed=6)
+    for k, tol in ((1, 1e-15), (9, 1e-15), (20000, 1e-9)):
+        kw = dict(max_iter=k, tol_abs=tol, tol_rel=tol, accel=native.CDX_ACCEL_CHEBYSHEV)
+        v1, i1, inf1, g1 = solve(gctx, prob, True, **kw)
+        v0, i0, inf0, g0 = solve(gctx, prob, False, **kw)
+        assert g1 == (True, 0, True) and g0[2] is False and gctx.range_sweep_info()[0]
+        assert inf1.iters == inf0.iters
+        np.testing.assert_array_equal(v1, v0)
+        np.testing.assert_array_equal(i1, i0)
 
 
 @pytest.mark.parametrize("build", [pile_problem, mixed_problem], ids=["pile", "spheres_boxes"])
