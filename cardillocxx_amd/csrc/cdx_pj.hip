@@ -386,6 +386,19 @@ __global__ void __launch_bounds__(kBlock) k_geo_pack(int nc, const int* __restri
     if (threadIdx.x == 0) bad_part[blockIdx.x] = (int)s;
 }
 
+// The Nesterov extrapolation y = u + beta (u - uk) (u itself while beta is 0), as nesterov_loop
+// forms it (projected_jacobi.cpp:379-449).
+__device__ inline double extrapolate1(double u, double uk, double beta) {
+    return beta == 0.0 ? u : u + beta * (u - uk);
+}
+__device__ inline void extrapolate6(const double u[6], const double uk[6], double beta, double out[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = extrapolate1(u[k], uk[k], beta);
+}
+
+// Loops the range pass serves: the standard loop, a Chebyshev sweep, a Nesterov sweep.
+constexpr int kLoopStd = 0, kLoopCheb = 1, kLoopNest = 2;
+
 // project() (projected_jacobi.cpp:100-119) of one contact's trial values z (nr = 1 or 3 rows).
 __device__ __forceinline__ void project3(double z[3], int nr, double muc) {
     z[0] = fmax(z[0], 0.0);
@@ -406,12 +419,15 @@ __device__ __forceinline__ void project3(double z[3], int nr, double muc) {
 // Block = true: R holds the inverse Delassus blocks and the bias is unscaled
 // (pj.rdiag_true_delassus): dp = Pi(p - Rb (W x + b)) - p.  Geo = true: the angular rows are
 // rebuilt from the lever arms G and the quaternions q (the geometry form above) and the second
-// tangent as n x t1 instead of being read from J.  Cheb = true (row scaling only): a sweep of the
-// Chebyshev loop (chebyshev_loop, projected_jacobi.cpp:180-228) -- its input is the extrapolation
-// w (x - xk) + xk of the twists and w (p - pk) + pk of the impulses (or x, p after a reset:
-// cheb[1]), formed here as k_pjc_contacts4 forms it, with the impulse history rotated
-// (pk <- pk1, pk1 <- input); the twists' history is rotated by k_pjc_bodies_parts.
-template <bool Block, bool Geo, bool Cheb>
+// tangent as n x t1 instead of being read from J.  Loop = kLoopCheb (row scaling only): a sweep of
+// the Chebyshev loop (chebyshev_loop, projected_jacobi.cpp:180-228) -- its input is the
+// extrapolation w (x - xk) + xk of the twists and w (p - pk) + pk of the impulses (or x, p after a
+// reset: scal[1]), formed here as k_pjc_contacts4 forms it, with the impulse history rotated
+// (pk <- pk1, pk1 <- input); the twists' history is rotated by k_pjc_bodies_parts.  Loop =
+// kLoopNest (row scaling only): a sweep of the Nesterov loop from y = x + beta (x - xk),
+// py = p + beta (p - pk) (beta = scal[0]), with pk <- p, as k_pjn_contacts; k_pjn_bodies_parts
+// rotates the twists.
+template <bool Block, bool Geo, int Loop>
 __global__ void __launch_bounds__(256, 4) k_pj_range(
     int nranges, int nc, int per, int maxs, int maxp, const int* __restrict__ rg_p, const int* __restrict__ rg_l,
     const int* __restrict__ pr_ptr, const int* __restrict__ pr_out, const int* __restrict__ pr_inc,
@@ -419,15 +435,16 @@ __global__ void __launch_bounds__(256, 4) k_pj_range(
     const double* __restrict__ R, const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
     double* __restrict__ p, const double* __restrict__ x, double* __restrict__ part, const int* __restrict__ ctrl,
     const double* __restrict__ G, const double* __restrict__ q, double* __restrict__ PK, double* __restrict__ PK1,
-    const double* __restrict__ XK, const double* __restrict__ cheb) {
-    static_assert(!(Block && Cheb), "the range form of the Chebyshev loop takes the row scaling");
+    const double* __restrict__ XK, const double* __restrict__ scal) {
+    static_assert(!Block || Loop == kLoopStd, "the range form of the accelerated loops takes the row scaling");
     if (ctrl[0]) return;
-    double w = 1.0;
+    double w = 1.0, beta = 0.0;
     bool take = true;
-    if constexpr (Cheb) {
-        w = cheb[0];
-        take = cheb[1] != 0.0;
+    if constexpr (Loop == kLoopCheb) {
+        w = scal[0];
+        take = scal[1] != 0.0;
     }
+    if constexpr (Loop == kLoopNest) beta = scal[0];
     // LDS: the records (9 doubles per contact of the range: lin | angA | angB, summed over its
     // rows), then the range's tables, copied by DMA while the contacts are swept: the incidence
     // offsets of its pairs (body, range), their partial slots, and the incidences (contact in the
@@ -480,9 +497,17 @@ __global__ void __launch_bounds__(256, 4) k_pj_range(
                     for (int k = 0; k < 3; ++k) geo_row(q + (size_t)b * kBodyStride + 3, rB, d[k], aB[k]);
             }
             double ua[6], ub[6];
-            if constexpr (Cheb) {
+            if constexpr (Loop == kLoopCheb) {
                 cheb_input6(x, XK, a, w, take, ua);
                 cheb_input6(x, XK, b, w, take, ub);
+            } else if constexpr (Loop == kLoopNest) {
+                double t0[6], t1[6];
+                load_body(x, a, t0);
+                load_body(XK, a, t1);
+                extrapolate6(t0, t1, beta, ua);
+                load_body(x, b, t0);
+                load_body(XK, b, t1);
+                extrapolate6(t0, t1, beta, ub);
             } else {
                 load_body(x, a, ua);
                 load_body(x, b, ub);
@@ -492,11 +517,17 @@ __global__ void __launch_bounds__(256, 4) k_pj_range(
             for (int k = 0; k < 3; ++k) {
                 if (k >= nr) break;
                 pold[k] = p[(size_t)k * nc + c];
-                if constexpr (Cheb) {  // the sweep's input impulse, and the history rotated
+                if constexpr (Loop == kLoopCheb) {  // the sweep's input impulse, and the history rotated
                     const size_t r = (size_t)k * nc + c;
                     pold[k] = take ? pold[k] : cheb_extrap(w, pold[k], PK[r]);
                     PK[r] = PK1[r];
                     PK1[r] = pold[k];
+                }
+                if constexpr (Loop == kLoopNest) {
+                    const size_t r = (size_t)k * nc + c;
+                    const double pc = pold[k];
+                    pold[k] = extrapolate1(pc, PK[r], beta);
+                    PK[r] = pc;
                 }
                 const double gk = row_dot(d[k], aA[k], aB[k], ua, ub);
                 const double bk = use_bias ? bias[(size_t)k * nc + c] : 0.0;
@@ -694,6 +725,69 @@ __global__ void __launch_bounds__(kBlock) k_pjc_bodies_parts(int nb, const int* 
     }
 }
 
+// Body pass of a Nesterov range sweep (as k_pjn_bodies over the partials), one lane per DOF:
+// y = X + beta (X - XK), X <- y + S^-1 (sum of the body's partials), XK <- the old X; per-block sums
+// of the residual terms between the new X and y and of the restart dot (y - X_new) . (X_new - X_old).
+__global__ void __launch_bounds__(kBlock) k_pjn_bodies_parts(int nb, const int* __restrict__ dof,
+                                                             const double* __restrict__ minv, const int* __restrict__ pcnt,
+                                                             const int* __restrict__ povf, const double* __restrict__ part,
+                                                             double* __restrict__ X, double* __restrict__ XK,
+                                                             double tol_rel, double tol_abs, double* __restrict__ partial,
+                                                             const int* __restrict__ ctrl, const double* __restrict__ nest,
+                                                             const double* __restrict__ sinv) {
+    if (ctrl[0]) return;
+    __shared__ double red[kBlock / 64];
+    const double beta = nest[0];
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int bi = tid / kLanesPerBody, l = tid % kLanesPerBody;
+    double e2 = 0, dt = 0;
+    if (bi < nb && l < 6) {
+        const int nd = dof[bi], n = pcnt[bi];
+        const double mk = l < 3 ? minv[(size_t)bi * kMinvStride] : minv[(size_t)bi * kMinvStride + l - 2];
+        const size_t o = (size_t)bi * kBodyStride + l;
+        const double u = X[o], uk = XK[o];
+        double ps[kPartSlots];
+#pragma unroll
+        for (int j = 0; j < kPartSlots; ++j) ps[j] = part[((size_t)bi * kPartSlots + j) * 6 + l];
+        if (l < nd) {
+            const double y = extrapolate1(u, uk, beta);
+            double a = 0;
+#pragma unroll
+            for (int j = 0; j < kPartSlots; ++j)
+                if (j < n) a += ps[j];
+            if (n > kPartSlots) {
+                const size_t ov = (size_t)nb * kPartSlots + povf[bi];
+                for (int j = kPartSlots; j < n; ++j) a += part[(ov + j - kPartSlots) * 6 + l];
+            }
+            const double M = mk > 0 ? 1.0 / mk : 0.0;
+            double d = a / M;
+            if (sinv && nd == 6 && l >= 3) {  // implicit gyroscopy, as k_pj_bodies_parts
+                double a3[3] = {0, 0, 0};
+                for (int j = 0; j < n; ++j) {
+                    const size_t oo = j < kPartSlots ? ((size_t)bi * kPartSlots + j) * 6
+                                                     : ((size_t)nb * kPartSlots + povf[bi] + j - kPartSlots) * 6;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) a3[k] += part[oo + 3 + k];
+                }
+                d = gyro_row(sinv + (size_t)bi * 9 + 3 * (l - 3), a3[0], a3[1], a3[2]);
+            }
+            const double un = y + d;
+            const double qq = fmax(fabs(un), fabs(y)) * tol_rel + tol_abs;
+            const double e = (un - y) / qq;
+            e2 = e * e;
+            dt = (y - un) * (un - u);
+            XK[o] = u;
+            X[o] = un;
+        }
+    }
+    const double s = block_sum(e2, red);
+    const double d = block_sum(dt, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s;
+        partial[gridDim.x + blockIdx.x] = d;
+    }
+}
+
 // mode 0: PJ (break when res <= 1); mode 1: GS (break when res < 1, NaN/Inf -> diverged).
 // it < 0: the sweep index comes from the device chunk counter ctrl[3] (graph-replayed chunks of
 // `cadence` sweeps whose residual check is the chunk's first sweep); the counter advances on
@@ -775,10 +869,6 @@ int check_solve_args(cdx_ctx* ctx, const cdx_solver_params* prm) {
 // ---------------------------------------------------------------------------------------------
 
 // nest[]: 0 beta, 1 thk, 2 err_prev, 3 restarts, 4 momentum_disabled
-__device__ inline void extrapolate6(const double u[6], const double uk[6], double beta, double out[6]) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) out[k] = beta == 0.0 ? u[k] : u[k] + beta * (u[k] - uk[k]);
-}
 
 __global__ void __launch_bounds__(kBlock) k_pjn_contacts(int nc, int nblocks, const int* __restrict__ ida,
                                                          const int* __restrict__ idb, const double* __restrict__ J,
@@ -915,6 +1005,8 @@ __global__ void __launch_bounds__(kBlock) k_pjn_finalize(int nparts, const doubl
     s = block_sum(s, red);
     d = block_sum(d, red);
     if (threadIdx.x != 0) return;
+    if (it < 0) it = ctrl[3];  // graph-replayed chunk: the sweep index from the device counter
+    ctrl[3] = ctrl[3] + 1;
     const double err = 1.0 / sqrt((double)nv) * sqrt(s);
     res[0] = err;
     if (err <= 1.0) {
@@ -1681,6 +1773,8 @@ __global__ void __launch_bounds__(kBlock) k_and_commit(int n, int nb, int nc, co
 using namespace cdx;
 
 static int run_plain_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int gb, int gc, int* iters_out);
+static size_t range_lds_bytes(int maxs, int maxp, int maxl);
+static int build_ranges(cdx_ctx* ctx);
 
 // The inverse rotational blocks of S of the current solve with implicit gyroscopy, else nullptr.
 static const double* gyro_sinv(const cdx_ctx* ctx, const cdx_solver_params* prm) {
@@ -1779,27 +1873,57 @@ extern "C" int cdx_pj_solve(cdx_ctx* ctx, double dt, double theta, const cdx_sol
         st = run_accel_joints(ctx, prm, prm->accel == CDX_ACCEL_NESTEROV ? 0 : 1, &iters);
     } else if (prm->accel == CDX_ACCEL_NESTEROV) {
         // xk = x0, pk = p0, beta = 0, thk = 1, err_prev = inf, restarts = 0, momentum on
+        // the range sweep (one lane per contact, per-body partials) on large problems, row scaling only
+        bool ranges = false;
+        if (!block && (ctx->range_mode == 2 || (ctx->range_mode == 1 && nc >= kRangeAutoContacts))) {
+            if (!ctx->ranges_valid) CDX_TRY(build_ranges(ctx));
+            ranges = ctx->ranges_ok;
+        }
+        ctx->ranges_used = ranges;
+        const bool geo = ranges && ctx->geo_ok && ctx->geo_mode;
+        ctx->geo_used = geo;
+        const int gb8 = grid_for(nb * kLanesPerBody), ng = ctx->nranges, per = ctx->range_per_used;
+        const int nthr = ranges ? std::max(64, std::min({ctx->range_block, 256, pad64(std::min(per, nc))})) : 0;
+        const size_t shm = ranges ? range_lds_bytes(std::min(per, nc), ctx->range_maxp, ctx->range_maxl) : 0;
+        const int nparts = ranges ? gb8 : gb;
         CDX_TRY(ensure_alloc(ctx, &ctx->d_pk, 3 * (size_t)nc, &ctx->pk_cap, "nesterov pk"));
         CDX_TRY(ensure_alloc(ctx, &ctx->d_nest, 8, nullptr, "nesterov scalars"));
-        CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)gb, &ctx->partial2_cap, "nesterov partials"));
+        CDX_TRY(ensure_alloc(ctx, &ctx->d_partial2, 2 * (size_t)nparts, &ctx->partial2_cap, "nesterov partials"));
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_uprev, ctx->d_x, sizeof(double) * (size_t)nb * kBodyStride, hipMemcpyDeviceToDevice,
                                     ctx->stream));
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_pk, ctx->d_p, sizeof(double) * 3 * (size_t)nc, hipMemcpyDeviceToDevice, ctx->stream));
         const double n0[8] = {0.0, 1.0, INFINITY, 0.0, 0.0, 0.0, 0.0, 0.0};
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_nest, n0, sizeof(n0), hipMemcpyHostToDevice, ctx->stream));
-        auto nsweep = [&](int it) -> int {
-            CDX_LAUNCH(ctx, "pj_contacts", k_pjn_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
-                       ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_x, ctx->d_uprev, ctx->d_imp, ctx->d_ctrl,
-                       ctx->d_nest, block ? ctx->d_Rb : nullptr);
-            CDX_LAUNCH(ctx, "pj_bodies", k_pjn_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr, ctx->d_incidx,
-                       ctx->d_imp, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs, ctx->d_partial2, ctx->d_ctrl,
-                       ctx->d_nest, gyro_sinv(ctx, prm));
-            CDX_LAUNCH(ctx, "pj_residual", k_pjn_finalize, 1, kBlock, gb, ctx->d_partial2, ctx->nv, it,
+        // one sweep; it < 0 inside a captured chunk (the finalize kernel reads the sweep counter)
+        auto nsweep = [&](int it, int) -> int {
+            if (ranges) {
+#define CDX_NEST_RANGE(GEO)                                                                                             \
+    CDX_LAUNCH_SHM(ctx, "pjn_range", (k_pj_range<false, GEO, kLoopNest>), ng, nthr, shm, ng, nc, per, std::min(per, nc), \
+                   ctx->range_maxp, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida,  \
+                   ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,   \
+                   ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q, ctx->d_pk, nullptr, ctx->d_uprev, ctx->d_nest)
+                if (geo)
+                    CDX_NEST_RANGE(true);
+                else
+                    CDX_NEST_RANGE(false);
+#undef CDX_NEST_RANGE
+                CDX_LAUNCH(ctx, "pjn_bodies_parts", k_pjn_bodies_parts, gb8, kBlock, nb, ctx->d_dof, ctx->d_minv,
+                           ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs,
+                           ctx->d_partial2, ctx->d_ctrl, ctx->d_nest, gyro_sinv(ctx, prm));
+            } else {
+                CDX_LAUNCH(ctx, "pj_contacts", k_pjn_contacts, gc, kBlock, nc, gc, ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_R,
+                           ctx->d_bias, ctx->d_mu, ctx->d_p, ctx->d_pk, ctx->d_x, ctx->d_uprev, ctx->d_imp, ctx->d_ctrl,
+                           ctx->d_nest, block ? ctx->d_Rb : nullptr);
+                CDX_LAUNCH(ctx, "pj_bodies", k_pjn_bodies, gb, kBlock, nb, ctx->d_dof, ctx->d_minv, ctx->d_incptr,
+                           ctx->d_incidx, ctx->d_imp, ctx->d_x, ctx->d_uprev, prm->tol_rel, prm->tol_abs, ctx->d_partial2,
+                           ctx->d_ctrl, ctx->d_nest, gyro_sinv(ctx, prm));
+            }
+            CDX_LAUNCH(ctx, "pj_residual", k_pjn_finalize, 1, kBlock, nparts, ctx->d_partial2, ctx->nv, it,
                        prm->nesterov_beta_threshold, prm->nesterov_restart_limit, ctx->d_ctrl, ctx->d_res, ctx->h_flag_dev,
                        ctx->d_nest);
             return hip_check(ctx, hipGetLastError(), "pj nesterov sweep launch");
         };
-        st = run_sweeps(ctx, prm->max_iter, 10, 4, nsweep, &iters);
+        st = run_sweeps_graphed(ctx, prm->max_iter, 8, true, nsweep, &iters);
     } else if (prm->accel == CDX_ACCEL_CHEBYSHEV) {
         st = run_chebyshev(ctx, prm, &iters);
     } else if (prm->accel == CDX_ACCEL_ANDERSON) {
@@ -1946,7 +2070,7 @@ static int run_range_sweeps(cdx_ctx* ctx, const cdx_solver_params* prm, int* ite
     ctx->geo_used = geo;
     auto launch_sweep = [&](int it, int check) -> int {
 #define CDX_RANGE_LAUNCH(BLK, GEO)                                                                                    \
-    CDX_LAUNCH_SHM(ctx, "pj_range", (k_pj_range<BLK, GEO, false>), ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp,  \
+    CDX_LAUNCH_SHM(ctx, "pj_range", (k_pj_range<BLK, GEO, kLoopStd>), ng, nthr, shm, ng, nc, per, maxs, ctx->range_maxp, \
                    ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida, ctx->d_idb,     \
                    ctx->d_J, BLK ? ctx->d_Rb : ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p,     \
                    ctx->d_x, ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q, nullptr, nullptr, nullptr, nullptr)
@@ -2149,7 +2273,7 @@ static int run_chebyshev(cdx_ctx* ctx, const cdx_solver_params* prm, int* iters_
     auto launch_sweep = [&](int it, int) -> int {
         if (ranges) {
 #define CDX_CHEB_RANGE(GEO)                                                                                             \
-    CDX_LAUNCH_SHM(ctx, "pjc_range", (k_pj_range<false, GEO, true>), ng, nthr, shm, ng, nc, per, std::min(per, nc),      \
+    CDX_LAUNCH_SHM(ctx, "pjc_range", (k_pj_range<false, GEO, kLoopCheb>), ng, nthr, shm, ng, nc, per, std::min(per, nc), \
                    ctx->range_maxp, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc, ctx->d_ida,  \
                    ctx->d_idb, ctx->d_J, ctx->d_R, ctx->d_bias, ctx->any_bias ? 1 : 0, ctx->d_mu, ctx->d_p, ctx->d_x,   \
                    ctx->d_part, ctx->d_ctrl, ctx->d_geo, ctx->d_q, ctx->d_pk, ctx->d_pk1, ctx->d_uprev, ctx->d_cheb)

@@ -4,7 +4,7 @@ quaternions and finds them equal, bit for bit, to the rows the host contact gene
 sphere pile with spun orientations, a sphere/box scene); the range sweep that rebuilds the rows in
 every sweep then gives exactly the bits of the range sweep that reads them, after fixed sweeps and
 to tolerance, row-scaled and block, and matches the oracle's pj_sweep (projected_jacobi.cpp:121-138)
-to <= 1e-10; likewise the Chebyshev loop's range sweep.  A row changed in its last bit is found
+to <= 1e-10; likewise the Chebyshev and Nesterov loops' range sweeps.  A row changed in its last bit is found
 (the sweep then reads the rows), and new positions, new contacts or a Moreau drift discard the
 points."""
 import numpy as np
@@ -72,11 +72,13 @@ def solve(c, prob, points, **kw):
     return c.velocities(), c.impulses(), info, c.geometry_rows_info()
 
 
-def test_chebyshev_loop_same_bits(gctx):
-    """The Chebyshev loop's range sweep with the rebuilt rows against the same loop reading them."""
+@pytest.mark.parametrize("accel", [native.CDX_ACCEL_CHEBYSHEV, native.CDX_ACCEL_NESTEROV], ids=["chebyshev", "nesterov"])
+def test_accelerated_loops_same_bits(gctx, accel):
+    """The Chebyshev and Nesterov loops' range sweeps with the rebuilt rows against the same loops
+    reading them."""
     prob = pile_problem(n=2000, seed=6)
     for k, tol in ((1, 1e-15), (9, 1e-15), (20000, 1e-9)):
-        kw = dict(max_iter=k, tol_abs=tol, tol_rel=tol, accel=native.CDX_ACCEL_CHEBYSHEV)
+        kw = dict(max_iter=k, tol_abs=tol, tol_rel=tol, accel=accel)
         v1, i1, inf1, g1 = solve(gctx, prob, True, **kw)
         v0, i0, inf0, g0 = solve(gctx, prob, False, **kw)
         assert g1 == (True, 0, True) and g0[2] is False and gctx.range_sweep_info()[0]

@@ -183,3 +183,28 @@ def test_argument_checks(rctx):
     for args in ((3,), (1, 15), (1, 769), (1, 0, 128)):
         with pytest.raises(native.CdxError):
             rctx.set_range_sweep(*args)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_nesterov_loop_on_ranges(rctx, seed):
+    """The Nesterov loop on the range sweep (k_pj_range<*, *, kLoopNest> + k_pjn_bodies_parts):
+    restarts on residual growth, the beta threshold and the direction test, momentum off after the
+    restart limit (projected_jacobi.cpp:379-449), against the oracle."""
+    prob = scenes.random_problem(nb=120, nc=180, seed=30 + seed)
+    for kw in (dict(max_iter=40, tol_abs=0.0, tol_rel=0.0), dict(tol_abs=1e-8, tol_rel=1e-8),
+               dict(tol_abs=1e-8, tol_rel=1e-8, nesterov_restart_limit=3),
+               dict(tol_abs=1e-8, tol_rel=1e-8, nesterov_beta_threshold=0.5)):
+        st, v_ref, imp_ref, res = oracle.pj_solve(prob, 1e-3, oracle.params(nesterov=1, **kw))
+        v, imp, info, (used, *_r) = solve(rctx, prob, accel=native.CDX_ACCEL_NESTEROV, **kw)
+        assert used
+        assert info.iters == res.iters
+        assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
+
+
+def test_nesterov_block_mode_keeps_the_two_passes(rctx):
+    prob = scenes.random_problem(nb=120, nc=180, seed=30)
+    kw = dict(max_iter=25, tol_abs=0.0, tol_rel=0.0, block_precond=1)
+    st, v_ref, imp_ref, res = oracle.pj_solve(prob, 1e-3, oracle.params(nesterov=1, **kw))
+    v, imp, info, (used, *_r) = solve(rctx, prob, accel=native.CDX_ACCEL_NESTEROV, **kw)
+    assert not used and info.iters == res.iters == 25
+    assert rel(v, v_ref) <= TOL and rel(imp, imp_ref) <= TOL
