@@ -494,13 +494,19 @@ def batched_leg(args, native, scenes):
                          con["vbias"], warm)
         ctx.set_scenes(b.body_off, con["contact_off"])
 
-    def advance(h, q_prev):
+    def advance(h, q_prev, need_v):
         """Moreau update on the device; a scene whose state went non-finite (projected Jacobi is not
-        guaranteed to converge) is put back at rest where it was.  Returns q, v, impulses, resets."""
+        guaranteed to converge) is put back at rest where it was.  The positions (the next contact
+        generation) and impulses (the next warm start) come down every step, the velocities only
+        when the next step compares scenes one by one or a scene is reset (None otherwise); the
+        finiteness test runs on the device.  Returns q, v, impulses, resets."""
         ctx.moreau_update(h, 1.0)
-        q, v, imp = ctx.positions(), ctx.velocities(), ctx.impulses()
-        bad_body = ~(np.isfinite(v.reshape(-1, 6)).all(axis=1) & np.isfinite(q.reshape(-1, 7)).all(axis=1))
+        q, imp = ctx.positions(), ctx.impulses()
         nbad = 0
+        if ctx.nonfinite_bodies() == 0:
+            return q, (ctx.velocities() if need_v else None), imp, nbad
+        v = ctx.velocities()
+        bad_body = ~(np.isfinite(v.reshape(-1, 6)).all(axis=1) & np.isfinite(q.reshape(-1, 7)).all(axis=1))
         if bad_body.any():
             bad = np.unique(scene_of[bad_body])
             nbad = len(bad)
@@ -529,7 +535,7 @@ def batched_leg(args, native, scenes):
             con, warm = contacts_of(q, prev)
             upload(con, warm)
             ctx.batch_pj_solve(args.batch_settle_dt, 1.0, ps)
-            q, v, imp, nbad = advance(args.batch_settle_dt, q)
+            q, v, imp, nbad = advance(args.batch_settle_dt, q, s == args.batch_settle - 1 and 0 in sample_steps)
             resets += nbad
             prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         per = np.diff(con["contact_off"])
@@ -594,7 +600,7 @@ def batched_leg(args, native, scenes):
             batch_time_sampled += t_step_solve
             obo_solves += len(pick)
             t_sample = time.perf_counter() - tsa
-        q, v, imp, nbad = advance(dt, q)
+        q, v, imp, nbad = advance(dt, q, s + 1 in sample_steps)
         if nbad:
             blown += nbad
             log(f"batched step {s}: {nbad} scenes went non-finite; reset to rest")
@@ -744,11 +750,12 @@ def chains_leg(args, native, scenes):
             obo_solves += len(pick)
             t_sample = time.perf_counter() - tsa
         ctx.moreau_update(dt, 1.0)
-        q, v, imp = ctx.positions(), ctx.velocities(), ctx.impulses()
+        if ctx.nonfinite_bodies():
+            raise RuntimeError(f"chains: non-finite state at step {s}")
+        q, imp = ctx.positions(), ctx.impulses()
+        v = ctx.velocities() if s + 1 in sample_steps else None  # the one-by-one comparison's inputs
         lam = lam_new
         tc = time.perf_counter()
-        if not (np.isfinite(q).all() and np.isfinite(v).all()):
-            raise RuntimeError(f"chains: non-finite state at step {s}")
         prev = dict(body_a=con["body_a"], body_b=con["body_b"], point=con["point"], imp=imp)
         t_cpu += tb - ta
         t_dev += tc - tb - t_sample

@@ -107,6 +107,44 @@ __global__ void k_gyro(int nb, const int* __restrict__ dof, const double* __rest
     gyro_force3(v + (size_t)b * kBodyStride + 3, minv + (size_t)b * kMinvStride + 1, o + 3);
 }
 
+// Packs a per-body padded array (kBodyStride doubles per body) into the stacked layout: body b's
+// first n entries (n = its DOFs, or 7 / 3 position entries with `positions`) at off[b].
+__global__ void k_pack_stacked(int nb, const int* __restrict__ dof, const int* __restrict__ off, int positions,
+                               const double* __restrict__ src, double* __restrict__ dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * kBodyStride) return;
+    const int b = t / kBodyStride, k = t % kBodyStride;
+    const int n = positions ? (dof[b] == 6 ? 7 : 3) : dof[b];
+    if (k < n) dst[off[b] + k] = src[t];
+}
+
+// Component-major impulses [3][nc] to contact-major [nc][3].
+__global__ void k_pack_impulses(int nc, const double* __restrict__ p, double* __restrict__ dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nc) return;
+    const int c = t / 3, k = t % 3;
+    dst[t] = p[(size_t)k * nc + c];
+}
+
+// Bodies with a non-finite twist or pose entry, one workgroup sum each.
+__global__ void __launch_bounds__(kBlock) k_nonfinite_bodies(int nb, const int* __restrict__ dof,
+                                                            const double* __restrict__ x, const double* __restrict__ q,
+                                                            int have_q, double* __restrict__ partial) {
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    double bad = 0;
+    if (b < nb) {
+        const int nd = dof[b], nq = nd == 6 ? 7 : 3;
+        bool ok = true;
+        for (int k = 0; k < nd; ++k) ok = ok && isfinite(x[(size_t)b * kBodyStride + k]);
+        if (have_q)
+            for (int k = 0; k < nq; ++k) ok = ok && isfinite(q[(size_t)b * kBodyStride + k]);
+        bad = ok ? 0.0 : 1.0;
+    }
+    const double s = block_sum(bad, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // Explicit position update (integration_base.cpp:6-30) with alignQuaternionTo
 // (math_helper.hpp:11-17).  When `vsrc` differs from v, the solved velocities are first
 // written back as the body velocities (writeVelocityToSystem, dynamics_assembler.cpp:145-188).
@@ -216,10 +254,11 @@ void cdx_ctx_destroy(cdx_ctx* ctx) {
                     ctx->d_jl_off, ctx->d_jw, ctx->d_and, ctx->d_and_gam, ctx->d_and_part, ctx->d_and_st,
                     ctx->d_gs_pack, ctx->d_gs_ids, ctx->d_rg_p, ctx->d_rg_l, ctx->d_pr_ptr, ctx->d_pr_out, ctx->d_pr_inc,
                     ctx->d_bp_ptr, ctx->d_bp_ovf, ctx->d_part, ctx->d_accj, ctx->d_sinv, ctx->d_geo, ctx->d_geo_pt,
-                    ctx->d_geo_bad};
+                    ctx->d_geo_bad, ctx->d_voff, ctx->d_qoff, ctx->d_stage};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (ctx->h_flag) hipHostFree(ctx->h_flag);
+    if (ctx->h_stage) hipHostFree(ctx->h_stage);
     for (auto& p : ctx->pending) {
         hipEventDestroy(p.start);
         hipEventDestroy(p.stop);
@@ -311,6 +350,11 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
         if (ctx->d_incptr) hipFree(ctx->d_incptr);
         ctx->d_incptr = nullptr;
         CDX_TRY(ensure_alloc(ctx, &ctx->d_incptr, (size_t)nb + 1, nullptr, "incidence ptr"));
+        for (int** p : {&ctx->d_voff, &ctx->d_qoff}) {
+            if (*p) hipFree(*p);
+            *p = nullptr;
+            CDX_TRY(ensure_alloc(ctx, p, (size_t)nb + 1, nullptr, "stacked offsets"));
+        }
         cap = nb;
     }
     ctx->body_cap = cap;
@@ -321,6 +365,10 @@ int cdx_set_bodies(cdx_ctx* ctx, int nb, const int* dof, const double* minv, con
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_q, hq.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_minv, hm.data(), (size_t)nb * kMinvStride * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_dof, dof, (size_t)nb * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_voff, ctx->voff.data(), (size_t)(nb + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_qoff, ctx->qoff.data(), (size_t)(nb + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                    ctx->stream));
         if (fgyro) {
             CDX_HIP(ctx, hipMemcpyAsync(ctx->d_fgyro, hg.data(), n8 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         } else {
@@ -442,41 +490,77 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     return build_incidence(ctx);
 }
 
+// Grows the device and pinned host staging buffers of the downloads to n doubles.
+static int ensure_stage(cdx_ctx* ctx, size_t n) {
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_stage, n, &ctx->stage_cap, "download staging"));
+    if (ctx->h_stage_cap < n) {
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_stage) hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_cap = 0;
+        if (hipHostMalloc((void**)&ctx->h_stage, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            ctx->h_stage = nullptr;
+            return set_err(ctx, CDX_HIP_ERROR, "download staging: pinned host allocation failed");
+        }
+        ctx->h_stage_cap = n;
+    }
+    return CDX_OK;
+}
+
+// The stacked array packed on the device goes to the caller through the pinned staging buffer:
+// one DMA copy at the link's rate and one host copy, where a pageable copy of the padded array
+// and a repack on the host took several times as long.
+static int download_stage(cdx_ctx* ctx, size_t n, double* out) {
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->h_stage, ctx->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (out) std::memcpy(out, ctx->h_stage, n * sizeof(double));
+    return CDX_OK;
+}
+
 int cdx_get_velocities(cdx_ctx* ctx, double* v_out) {
     if (!ctx || (!v_out && ctx->nv > 0)) return CDX_BAD_ARG;
     if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_velocities: no solve yet");
-    std::vector<double> hx((size_t)ctx->nb * kBodyStride);
-    if (ctx->nb)
-        CDX_HIP(ctx, hipMemcpyAsync(hx.data(), ctx->d_x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int b = 0; b < ctx->nb; ++b)
-        for (int k = 0; k < ctx->dof[b]; ++k) v_out[ctx->voff[b] + k] = hx[(size_t)b * kBodyStride + k];
-    return CDX_OK;
+    if (ctx->nv == 0) return CDX_OK;
+    CDX_TRY(ensure_stage(ctx, (size_t)ctx->nv));
+    CDX_LAUNCH(ctx, "pack_velocities", k_pack_stacked, grid_for(ctx->nb * kBodyStride), kBlock, ctx->nb, ctx->d_dof,
+               ctx->d_voff, 0, ctx->d_x, ctx->d_stage);
+    return download_stage(ctx, (size_t)ctx->nv, v_out);
 }
 
 int cdx_get_impulses(cdx_ctx* ctx, double* imp_out) {
     if (!ctx || (!imp_out && ctx->nc > 0)) return CDX_BAD_ARG;
     if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_get_impulses: no solve yet");
     const int nc = ctx->nc;
-    std::vector<double> hp(3 * (size_t)nc);
-    if (nc) CDX_HIP(ctx, hipMemcpyAsync(hp.data(), ctx->d_p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int c = 0; c < nc; ++c)
-        for (int k = 0; k < 3; ++k) imp_out[3 * (size_t)c + k] = hp[(size_t)k * nc + c];
-    return CDX_OK;
+    if (nc == 0) return CDX_OK;
+    CDX_TRY(ensure_stage(ctx, 3 * (size_t)nc));
+    CDX_LAUNCH(ctx, "pack_impulses", k_pack_impulses, grid_for(3 * nc), kBlock, nc, ctx->d_p, ctx->d_stage);
+    return download_stage(ctx, 3 * (size_t)nc, imp_out);
 }
 
 int cdx_get_positions(cdx_ctx* ctx, double* q_out) {
     if (!ctx || (!q_out && ctx->nb > 0)) return CDX_BAD_ARG;
     if (!ctx->have_q) return set_err(ctx, CDX_NO_STATE, "cdx_get_positions: no positions were set");
-    std::vector<double> hq((size_t)ctx->nb * kBodyStride);
-    if (ctx->nb)
-        CDX_HIP(ctx, hipMemcpyAsync(hq.data(), ctx->d_q, hq.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int b = 0; b < ctx->nb; ++b) {
-        const int nq = ctx->dof[b] == 6 ? 7 : 3;
-        for (int k = 0; k < nq; ++k) q_out[ctx->qoff[b] + k] = hq[(size_t)b * kBodyStride + k];
-    }
+    const size_t nq = (size_t)ctx->qoff[ctx->nb];
+    if (nq == 0) return CDX_OK;
+    CDX_TRY(ensure_stage(ctx, nq));
+    CDX_LAUNCH(ctx, "pack_positions", k_pack_stacked, grid_for(ctx->nb * kBodyStride), kBlock, ctx->nb, ctx->d_dof,
+               ctx->d_qoff, 1, ctx->d_q, ctx->d_stage);
+    return download_stage(ctx, nq, q_out);
+}
+
+int cdx_nonfinite_bodies(cdx_ctx* ctx, int* count) {
+    if (!ctx || !count) return CDX_BAD_ARG;
+    if (!ctx->solved) return set_err(ctx, CDX_NO_STATE, "cdx_nonfinite_bodies: no solve yet");
+    *count = 0;
+    if (ctx->nb == 0) return CDX_OK;
+    const int grid = (ctx->nb + kBlock - 1) / kBlock;
+    CDX_TRY(ensure_stage(ctx, (size_t)grid));
+    CDX_LAUNCH(ctx, "nonfinite_bodies", k_nonfinite_bodies, grid, kBlock, ctx->nb, ctx->d_dof, ctx->d_x, ctx->d_q,
+               ctx->have_q ? 1 : 0, ctx->d_stage);
+    CDX_TRY(download_stage(ctx, (size_t)grid, nullptr));
+    double n = 0;
+    for (int k = 0; k < grid; ++k) n += ctx->h_stage[k];
+    *count = (int)n;
     return CDX_OK;
 }
 

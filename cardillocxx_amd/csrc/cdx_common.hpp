@@ -57,6 +57,12 @@ struct cdx_ctx {
     double* d_fext = nullptr;    // [nb*8]
     double* d_fgyro = nullptr;   // [nb*8]
     double* d_q = nullptr;       // [nb*8]
+    int* d_voff = nullptr;       // [nb+1] stacked velocity offsets (downloads)
+    int* d_qoff = nullptr;       // [nb+1] stacked position offsets
+    double* d_stage = nullptr;   // downloads: the stacked output packed on the device
+    size_t stage_cap = 0;
+    double* h_stage = nullptr;   // ... and its pinned host copy
+    size_t h_stage_cap = 0;
     double* d_uprev = nullptr;   // [nb*8] previous iterate (GS residual)
     double* d_ufree = nullptr;   // [nb*8]
     int* d_dof = nullptr;        // [nb]

@@ -51,6 +51,7 @@ EXPORTED = [
     "cdx_get_velocities",
     "cdx_get_impulses",
     "cdx_get_positions",
+    "cdx_nonfinite_bodies",
     "cdx_moreau_predrift",
     "cdx_moreau_update",
     "cdx_set_profiling",
@@ -138,6 +139,7 @@ def _declare(lib):
         "cdx_get_velocities": (C.c_int, [_vp, _dp]),
         "cdx_get_impulses": (C.c_int, [_vp, _dp]),
         "cdx_get_positions": (C.c_int, [_vp, _dp]),
+        "cdx_nonfinite_bodies": (C.c_int, [_vp, _ip]),
         "cdx_moreau_predrift": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_moreau_update": (C.c_int, [_vp, C.c_double, C.c_double]),
         "cdx_set_profiling": (C.c_int, [_vp, C.c_int]),
@@ -392,6 +394,12 @@ class Context:
         out = np.empty(self.nq, dtype=np.float64)
         self._check(self._l.cdx_get_positions(self.h, dptr(out)), "cdx_get_positions")
         return out
+
+    def nonfinite_bodies(self):
+        """Bodies whose solved twist or pose holds a NaN or Inf, counted on the device."""
+        n = C.c_int(0)
+        self._check(self._l.cdx_nonfinite_bodies(self.h, C.byref(n)), "cdx_nonfinite_bodies")
+        return n.value
 
     def moreau_predrift(self, dt, theta):
         self._check(self._l.cdx_moreau_predrift(self.h, dt, theta), "cdx_moreau_predrift")

@@ -163,6 +163,13 @@ int cdx_get_velocities(cdx_ctx* ctx, double* v_out);
 int cdx_get_impulses(cdx_ctx* ctx, double* imp_out);
 int cdx_get_positions(cdx_ctx* ctx, double* q_out);
 
+/* Number of bodies whose solved twist (cdx_get_velocities) or pose (cdx_get_positions, when
+ * positions were set) holds a NaN or Inf entry, counted on the device so that a caller can test a
+ * step without downloading the state.  No counterpart in the reference, whose PGS reports a
+ * non-finite residual (projected_gauss_seidel.cpp:133-138) and whose projected Jacobi returns the
+ * state as it is; for batched callers that put a scene that blew up back at rest. */
+int cdx_nonfinite_bodies(cdx_ctx* ctx, int* count);
+
 /* Moreau theta-method around the solve (moreau.cpp:6-13, integration_base.cpp:6-30):
  *   cdx_moreau_predrift : q += (1-theta) h B(q) v_n
  *   cdx_moreau_update   : v_n <- solved velocities (writeVelocityToSystem,
