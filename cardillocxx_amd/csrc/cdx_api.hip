@@ -63,6 +63,37 @@ int harvest_events(cdx_ctx* ctx) {
 
 // Body -> incident contact-side CSR, sides listed in contact order so the per-body sum has a
 // fixed order.  Entry value = 2*c + side.
+// Grows the device and pinned host staging buffers of the uploads and downloads to n doubles.
+int ensure_stage(cdx_ctx* ctx, size_t n) {
+    // grown by half again at least: a contact count that creeps up step by step would otherwise
+    // pin and map a new host buffer (tens of ms for 100 MB) on every step
+    if (ctx->stage_cap < n) CDX_TRY(ensure_alloc(ctx, &ctx->d_stage, std::max(n, ctx->stage_cap + ctx->stage_cap / 2),
+                                                 &ctx->stage_cap, "staging"));
+    if (ctx->h_stage_cap < n) {
+        const size_t want = std::max(n, ctx->h_stage_cap + ctx->h_stage_cap / 2);
+        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_stage) hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_cap = 0;
+        if (hipHostMalloc((void**)&ctx->h_stage, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            ctx->h_stage = nullptr;
+            return set_err(ctx, CDX_HIP_ERROR, "staging: pinned host allocation failed");
+        }
+        ctx->h_stage_cap = want;
+    }
+    return CDX_OK;
+}
+
+// The stacked array packed on the device goes to the caller through the pinned staging buffer:
+// one DMA copy at the link's rate and one host copy, where a pageable copy of the padded array
+// and a repack on the host took several times as long.
+static int download_stage(cdx_ctx* ctx, size_t n, double* out) {
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->h_stage, ctx->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (out) std::memcpy(out, ctx->h_stage, n * sizeof(double));
+    return CDX_OK;
+}
+
 int build_incidence(cdx_ctx* ctx) {
     if (ctx->incidence_valid) return CDX_OK;
     const int nb = ctx->nb, nc = ctx->nc;
@@ -79,10 +110,16 @@ int build_incidence(cdx_ctx* ctx) {
         if (a >= 0) idx[fill[a]++] = 2 * c;
         if (b >= 0) idx[fill[b]++] = 2 * c + 1;
     }
-    // d_incptr is sized with the body capacity, d_incidx with the contact capacity.
-    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incptr, ptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
+    // d_incptr is sized with the body capacity, d_incidx with the contact capacity; both go
+    // through the pinned staging buffer
+    const size_t nptr = (size_t)nb + 1, nidx = 2 * (size_t)nc;
+    CDX_TRY(ensure_stage(ctx, (nptr + nidx + 1) / 2));
+    int* hi = reinterpret_cast<int*>(ctx->h_stage);
+    std::memcpy(hi, ptr.data(), nptr * sizeof(int));
+    std::memcpy(hi + nptr, idx.data(), nidx * sizeof(int));
+    CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incptr, hi, sizeof(int) * nptr, hipMemcpyHostToDevice, ctx->stream));
     if (nc > 0)
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incidx, idx.data(), sizeof(int) * 2 * (size_t)nc, hipMemcpyHostToDevice, ctx->stream));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_incidx, hi + nptr, sizeof(int) * nidx, hipMemcpyHostToDevice, ctx->stream));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     idx.resize(2 * (size_t)nc);
     ctx->h_incptr = std::move(ptr);
@@ -105,6 +142,41 @@ __global__ void k_gyro(int nb, const int* __restrict__ dof, const double* __rest
     for (int k = 0; k < kBodyStride; ++k) o[k] = 0.0;
     if (dof[b] != 6) return;
     gyro_force3(v + (size_t)b * kBodyStride + 3, minv + (size_t)b * kMinvStride + 1, o + 3);
+}
+
+// The caller's contact-major contact arrays (staged as frame | angA | angB [nc][9], rest | vbias |
+// warm [nc][3], mu [nc], then body_a | body_b) to the component-major device arrays, with
+// rebuildW_'s masks (dynamics_assembler.cpp:211-332): no angular rows on a static side or a point
+// mass (buildContactRowByDof, dof 3), no tangent rows or friction without mu > 0, nothing on a
+// static-static contact (:259-260).
+__global__ void k_contacts_soa(int nc, const int* __restrict__ dof, const double* __restrict__ raw,
+                               const int* __restrict__ ids, int* __restrict__ ida, int* __restrict__ idb,
+                               double* __restrict__ J, double* __restrict__ mu_o, double* __restrict__ rest_o,
+                               double* __restrict__ vb_o, double* __restrict__ p_o) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const size_t n = (size_t)nc;
+    const double *frame = raw, *angA = raw + 9 * n, *angB = raw + 18 * n, *rest = raw + 27 * n,
+                 *vbias = raw + 30 * n, *warm = raw + 33 * n, *mu = raw + 36 * n;
+    int a = ids[c], b = ids[n + c];
+    const bool has_rows = a >= 0 || b >= 0;
+    if (!has_rows) a = b = -1;
+    ida[c] = a;
+    idb[c] = b;
+    const bool rotA = a >= 0 && dof[a] != 3, rotB = b >= 0 && dof[b] != 3;
+    for (int k = 0; k < 9; ++k) {
+        J[k * n + c] = frame[9 * (size_t)c + k];
+        J[(9 + k) * n + c] = rotA ? angA[9 * (size_t)c + k] : 0.0;
+        J[(18 + k) * n + c] = rotB ? angB[9 * (size_t)c + k] : 0.0;
+    }
+    const bool fr = mu[c] > 0;
+    mu_o[c] = fr ? mu[c] : 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const bool row = has_rows && (k == 0 || fr);
+        rest_o[k * n + c] = row ? rest[3 * (size_t)c + k] : 0.0;
+        vb_o[k * n + c] = row ? vbias[3 * (size_t)c + k] : 0.0;
+        p_o[k * n + c] = row ? warm[3 * (size_t)c + k] : 0.0;
+    }
 }
 
 // Packs a per-body padded array (kBodyStride doubles per body) into the stacked layout: body b's
@@ -410,32 +482,20 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
                 return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite contact data");
         if (a >= 0 || b >= 0) nrows += mu[c] > 0 ? 3 : 1;
     }
-    // Component-major SoA staging.  Static-static contacts carry no rows (rebuildW_, :259-260):
-    // both ids become -1 and the kernels skip them.
-    std::vector<double> J((size_t)kJ * nc), m(nc), r(3 * (size_t)nc), vb(3 * (size_t)nc), w(3 * (size_t)nc);
-    std::vector<int> ia(body_a, body_a + nc), ib(body_b, body_b + nc);
     bool any_rest = false, any_vbias = false;
+    std::vector<int> ia(body_a, body_a + nc), ib(body_b, body_b + nc);
+    std::vector<double> m(nc);
     for (int c = 0; c < nc; ++c) {
-        for (int k = 0; k < 9; ++k) {
-            J[(size_t)k * nc + c] = frame[9 * (size_t)c + k];
-            J[(size_t)(9 + k) * nc + c] = ia[c] >= 0 ? angA[9 * (size_t)c + k] : 0.0;
-            J[(size_t)(18 + k) * nc + c] = ib[c] >= 0 ? angB[9 * (size_t)c + k] : 0.0;
-        }
-        // point masses have no angular rows (buildContactRowByDof, dof 3)
-        if (ia[c] >= 0 && ctx->dof[ia[c]] == 3)
-            for (int k = 0; k < 9; ++k) J[(size_t)(9 + k) * nc + c] = 0.0;
-        if (ib[c] >= 0 && ctx->dof[ib[c]] == 3)
-            for (int k = 0; k < 9; ++k) J[(size_t)(18 + k) * nc + c] = 0.0;
         const bool fr = mu[c] > 0;
         m[c] = fr ? mu[c] : 0.0;
-        const bool has_rows = ia[c] >= 0 || ib[c] >= 0;
+        if (ia[c] < 0 && ib[c] < 0) {
+            ia[c] = ib[c] = -1;
+            continue;
+        }
         for (int k = 0; k < 3; ++k) {
-            const bool row = has_rows && (k == 0 || fr);
-            r[(size_t)k * nc + c] = row ? rest[3 * (size_t)c + k] : 0.0;
-            if (row && rest[3 * (size_t)c + k] != 0.0) any_rest = true;
-            vb[(size_t)k * nc + c] = row ? vbias[3 * (size_t)c + k] : 0.0;
-            if (row && vbias[3 * (size_t)c + k] != 0.0) any_vbias = true;
-            w[(size_t)k * nc + c] = row ? warm[3 * (size_t)c + k] : 0.0;
+            if (k > 0 && !fr) break;
+            any_rest = any_rest || rest[3 * (size_t)c + k] != 0.0;
+            any_vbias = any_vbias || vbias[3 * (size_t)c + k] != 0.0;
         }
     }
     if ((size_t)nc > ctx->contact_cap || !ctx->d_ida) {
@@ -461,8 +521,6 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
         CDX_TRY(ensure_alloc(ctx, &ctx->d_incidx, 2 * (size_t)nc, nullptr, "incidence"));
         ctx->contact_cap = nc;
     }
-    for (int c = 0; c < nc; ++c)
-        if (ia[c] < 0 && ib[c] < 0) ia[c] = ib[c] = -1;
     ctx->nc = nc;
     ctx->nrows = nrows;
     ctx->any_rest = any_rest;
@@ -471,13 +529,26 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     ctx->h_b = ib;
     ctx->h_mu.assign(m.begin(), m.end());
     if (nc > 0) {
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_ida, ia.data(), sizeof(int) * nc, hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_idb, ib.data(), sizeof(int) * nc, hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_J, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_mu, m.data(), sizeof(double) * nc, hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_rest, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_vb, vb.data(), sizeof(double) * vb.size(), hipMemcpyHostToDevice, ctx->stream));
-        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, ctx->stream));
+        // the caller's arrays as they are, through pinned staging in one copy; the device
+        // transposes and masks them (k_contacts_soa)
+        const size_t n = (size_t)nc, nd = 37 * n;
+        CDX_TRY(ensure_stage(ctx, nd + n));
+        double* h = ctx->h_stage;
+        const struct {
+            const double* src;
+            size_t len;
+        } parts[] = {{frame, 9 * n}, {angA, 9 * n}, {angB, 9 * n}, {rest, 3 * n}, {vbias, 3 * n}, {warm, 3 * n}, {mu, n}};
+        for (const auto& part : parts) {
+            std::memcpy(h, part.src, part.len * sizeof(double));
+            h += part.len;
+        }
+        int* hi = reinterpret_cast<int*>(h);
+        std::memcpy(hi, body_a, n * sizeof(int));
+        std::memcpy(hi + n, body_b, n * sizeof(int));
+        CDX_HIP(ctx, hipMemcpyAsync(ctx->d_stage, ctx->h_stage, (nd + n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CDX_LAUNCH(ctx, "contacts_soa", k_contacts_soa, grid_for(nc), kBlock, nc, ctx->d_dof, ctx->d_stage,
+                   reinterpret_cast<const int*>(ctx->d_stage + nd), ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu,
+                   ctx->d_rest, ctx->d_vb, ctx->d_p);
     }
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->incidence_valid = false;
@@ -488,33 +559,6 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     ctx->geo_ok = false;
     ctx->geo_bad = -1;
     return build_incidence(ctx);
-}
-
-// Grows the device and pinned host staging buffers of the downloads to n doubles.
-static int ensure_stage(cdx_ctx* ctx, size_t n) {
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_stage, n, &ctx->stage_cap, "download staging"));
-    if (ctx->h_stage_cap < n) {
-        CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->h_stage) hipHostFree(ctx->h_stage);
-        ctx->h_stage = nullptr;
-        ctx->h_stage_cap = 0;
-        if (hipHostMalloc((void**)&ctx->h_stage, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-            ctx->h_stage = nullptr;
-            return set_err(ctx, CDX_HIP_ERROR, "download staging: pinned host allocation failed");
-        }
-        ctx->h_stage_cap = n;
-    }
-    return CDX_OK;
-}
-
-// The stacked array packed on the device goes to the caller through the pinned staging buffer:
-// one DMA copy at the link's rate and one host copy, where a pageable copy of the padded array
-// and a repack on the host took several times as long.
-static int download_stage(cdx_ctx* ctx, size_t n, double* out) {
-    CDX_HIP(ctx, hipMemcpyAsync(ctx->h_stage, ctx->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (out) std::memcpy(out, ctx->h_stage, n * sizeof(double));
-    return CDX_OK;
 }
 
 int cdx_get_velocities(cdx_ctx* ctx, double* v_out) {

@@ -240,6 +240,9 @@ void prof_events(cdx_ctx* ctx, hipEvent_t* start, hipEvent_t* stop);
 void prof_push(cdx_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t stop);
 int harvest_events(cdx_ctx* ctx);
 int build_incidence(cdx_ctx* ctx);
+// Grows the device (d_stage) and pinned host (h_stage) staging buffers to n doubles each; the
+// host buffer is idle again once the stream has been synchronised after its last copy.
+int ensure_stage(cdx_ctx* ctx, size_t n);
 int grid_for(int n);
 int reset_control(cdx_ctx* ctx);
 int read_residual(cdx_ctx* ctx, double* r);

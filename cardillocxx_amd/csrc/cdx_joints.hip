@@ -111,11 +111,13 @@ using namespace cdx;
 // reading one coefficient of 64 rows touches 4 cache lines instead of 64.
 static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, const double* wa, const double* wb,
                                const double* comp, const double* vsrc, const double* gerr, const double* lam_prev) {
-    const size_t R = jorig.size();
-    std::vector<double> jr(R * kJRowStride);
+    const size_t R = jorig.size(), n = R * kJRowStride;
+    // gathered straight into the pinned staging buffer: one DMA copy at the link's rate
+    CDX_TRY(ensure_stage(ctx, n > 0 ? n : 1));
+    double* jr = ctx->h_stage;
     for (size_t i = 0; i < R; ++i) {
         const size_t r = (size_t)jorig[i];
-        double* o = jr.data() + i * kJRowStride;
+        double* o = jr + i * kJRowStride;
         for (int k = 0; k < 6; ++k) o[k] = wa[6 * r + k];
         for (int k = 0; k < 6; ++k) o[6 + k] = wb[6 * r + k];
         o[12] = comp[r];
@@ -123,8 +125,9 @@ static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, cons
         o[14] = gerr[r];
         o[15] = lam_prev[r];
     }
-    CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, jr.size(), &ctx->jr_cap, "joint rows"));
-    CDX_HIP(ctx, hipMemcpy(ctx->d_jr, jr.data(), sizeof(double) * jr.size(), hipMemcpyHostToDevice));
+    CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, n, &ctx->jr_cap, "joint rows"));
+    if (n > 0) CDX_HIP(ctx, hipMemcpyAsync(ctx->d_jr, jr, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jw, 12 * R, &ctx->jw_cap, "joint rows"));
     CDX_LAUNCH(ctx, "joint_by_coef", k_joint_by_coef, grid_for((int)(12 * R)), kBlock, (int)R, ctx->d_jr, ctx->d_jw);
     return hip_check(ctx, hipGetLastError(), "joint coefficient launch");
