@@ -628,7 +628,9 @@ def batched_leg(args, native, scenes):
         "contacts_per_scene": dist,
         "steps": steps, "scenes": ns, "bodies": nb,
         "value": ns * steps / t_dev, "unit": "env-steps/s",
-        "definition": "device path per step: contact upload, batched solve, position update, state download",
+        "definition": "device path per step: contact upload, batched solve, position update, finiteness test on the "
+                      "device, positions and impulses downloaded (velocities too when a scene is reset or the next "
+                      "step compares scenes one by one)",
         "solve_only_env_steps_per_s": ns * steps / t_solve,
         "end_to_end_env_steps_per_s": ns * steps / (t_cpu + t_dev),
         "cpu_contact_generation_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
@@ -772,7 +774,9 @@ def chains_leg(args, native, scenes):
         "steps": steps, "scenes": ns, "bodies": nb, "joints": int(len(jt["body_a"])),
         "joint_rows": int(rows["ns"] + rows["nd"]),
         "value": ns * steps / t_dev, "unit": "env-steps/s",
-        "definition": "device path per step: contact and joint-row upload, batched solve, position update, state download",
+        "definition": "device path per step: contact and joint-row upload, batched solve, position update, finiteness "
+                      "test on the device, positions, impulses and joint impulses downloaded (velocities when the next "
+                      "step compares chains one by one)",
         "solve_only_env_steps_per_s": ns * steps / t_solve,
         "end_to_end_env_steps_per_s": ns * steps / (t_cpu + t_dev),
         "cpu_contacts_and_joint_rows_s": t_cpu, "device_s": t_dev, "solve_s": t_solve,
