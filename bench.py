@@ -458,6 +458,15 @@ def _scene_problem(bd, con, warm, v, q, b0, b1, c0, c1):
     return bodies, contacts
 
 
+def settled_args(args):
+    """The batched leg on the configured contact load: the same scenes settled into piles first
+    (untimed, 4e-3 s steps at a loose tolerance; about 1.5M contacts, up to ~4,800 per scene),
+    then a few timed steps, one step compared scene by scene, the default launch shape."""
+    a = argparse.Namespace(**vars(args))
+    a.batch_settle, a.batch_steps, a.obo_steps, a.tune_step = args.settled_settle, args.settled_steps, 1, -1
+    return a
+
+
 def batched_leg(args, native, scenes):
     t0 = time.time()
     b = scenes.batched_drop(nscenes=args.batch_scenes, seed=0)
@@ -985,6 +994,9 @@ def main():
     ap.add_argument("--batch-settle", type=int, default=0, help="untimed settling steps before the batched steps")
     ap.add_argument("--batch-settle-dt", type=float, default=4e-3)
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg and print it")
+    ap.add_argument("--settled-steps", type=int, default=20,
+                    help="timed steps of the batched leg again on the settled piles (0: skip)")
+    ap.add_argument("--settled-settle", type=int, default=400, help="untimed settling steps before them")
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiable legs and print them")
     ap.add_argument("--obo-steps", type=int, default=4, help="steps at which scenes are also solved one by one")
     ap.add_argument("--obo-scenes", type=int, default=128, help="scenes solved one by one per sampled step")
@@ -1026,7 +1038,10 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args, native, scenes)}), flush=True)
+        out = {"batched": batched_leg(args, native, scenes)}
+        if args.settled_steps > 0:
+            out["batched_settled"] = batched_leg(settled_args(args), native, scenes)
+        print(json.dumps(out), flush=True)
         return
     if args.chains_only:
         print(json.dumps({"jointed_chains": chains_leg(args, native, scenes),
@@ -1037,6 +1052,8 @@ def main():
     out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
     if not args.pile_only:
         out["batched"] = batched_leg(args, native, scenes)
+        if args.settled_steps > 0:
+            out["batched_settled"] = batched_leg(settled_args(args), native, scenes)
         out["differentiable"] = diff_leg(args, native, scenes)
         if not args.no_chains:
             out["jointed_chains"] = chains_leg(args, native, scenes)
