@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
+#include <thread>
 
 #include "cdx_device.hpp"
 
@@ -64,6 +65,41 @@ int harvest_events(cdx_ctx* ctx) {
 // Body -> incident contact-side CSR, sides listed in contact order so the per-body sum has a
 // fixed order.  Entry value = 2*c + side.
 // Grows the device and pinned host staging buffers of the uploads and downloads to n doubles.
+// Host threads for a pass over n contacts: one below 32k, at most 8.
+static int host_threads(int n) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1, std::min({8, hw, n / 32768}));
+}
+
+// fn(t, c0, c1) over T contiguous chunks of [0, n), chunk t in thread t (chunk 0 on the caller's).
+template <class F>
+static void parallel_ranges(int n, int T, F fn) {
+    auto lo = [&](int t) { return (int)((long long)n * t / T); };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(fn, t, lo(t), lo(t + 1));
+    fn(0, 0, lo(1));
+    for (auto& x : th) x.join();
+}
+
+// 0 when contact c passes cdx_set_contacts's checks, else 1 (body index), 2 (mu), 3 (frame or
+// angular rows), 4 (restitution, bias or warm start), in the order they are tested.
+static int contact_fault(int nb, int c, const int* body_a, const int* body_b, const double* mu, const double* frame,
+                         const double* angA, const double* angB, const double* rest, const double* vbias,
+                         const double* warm) {
+    const int a = body_a[c], b = body_b[c];
+    if (a < -1 || a >= nb || b < -1 || b >= nb || (a >= 0 && a == b)) return 1;
+    if (!std::isfinite(mu[c])) return 2;
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(frame[9 * (size_t)c + k]) || !std::isfinite(angA[9 * (size_t)c + k]) ||
+            !std::isfinite(angB[9 * (size_t)c + k]))
+            return 3;
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(rest[3 * (size_t)c + k]) || !std::isfinite(vbias[3 * (size_t)c + k]) ||
+            !std::isfinite(warm[3 * (size_t)c + k]))
+            return 4;
+    return 0;
+}
+
 int ensure_stage(cdx_ctx* ctx, size_t n) {
     // grown by half again at least: a contact count that creeps up step by step would otherwise
     // pin and map a new host buffer (tens of ms for 100 MB) on every step
@@ -468,35 +504,49 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     if (nc < 0 || (nc > 0 && (!body_a || !body_b || !mu || !frame || !angA || !angB || !rest || !vbias || !warm)))
         return set_err(ctx, CDX_BAD_ARG, "cdx_set_contacts: null array");
     hipSetDevice(ctx->device);
-    int nrows = 0;
-    for (int c = 0; c < nc; ++c) {
-        const int a = body_a[c], b = body_b[c];
-        if (a < -1 || a >= ctx->nb || b < -1 || b >= ctx->nb || (a >= 0 && a == b))
-            return set_err(ctx, CDX_BAD_ARG, "cdx_set_contacts: body index out of range (bodies must be set first)");
-        if (!std::isfinite(mu[c])) return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite mu");
-        for (int k = 0; k < 9; ++k)
-            if (!std::isfinite(frame[9 * (size_t)c + k]) || !std::isfinite(angA[9 * (size_t)c + k]) || !std::isfinite(angB[9 * (size_t)c + k]))
-                return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite contact frame");
-        for (int k = 0; k < 3; ++k)
-            if (!std::isfinite(rest[3 * (size_t)c + k]) || !std::isfinite(vbias[3 * (size_t)c + k]) || !std::isfinite(warm[3 * (size_t)c + k]))
-                return set_err(ctx, CDX_NONFINITE, "cdx_set_contacts: non-finite contact data");
-        if (a >= 0 || b >= 0) nrows += mu[c] > 0 ? 3 : 1;
-    }
-    bool any_rest = false, any_vbias = false;
-    std::vector<int> ia(body_a, body_a + nc), ib(body_b, body_b + nc);
+    // the checks, the row count and the host copies of the ids and friction, over contiguous
+    // chunks in host threads; the first failing contact (the lowest chunk's first) names the error
+    const int T = host_threads(nc);
+    struct Scan {
+        int bad = -1, fault = 0, nrows = 0;
+        bool rest = false, vbias = false;
+    };
+    std::vector<Scan> sc(T);
+    std::vector<int> ia(nc), ib(nc);
     std::vector<double> m(nc);
-    for (int c = 0; c < nc; ++c) {
-        const bool fr = mu[c] > 0;
-        m[c] = fr ? mu[c] : 0.0;
-        if (ia[c] < 0 && ib[c] < 0) {
-            ia[c] = ib[c] = -1;
-            continue;
+    parallel_ranges(nc, T, [&](int t, int c0, int c1) {
+        Scan& r = sc[t];
+        for (int c = c0; c < c1; ++c) {
+            const int f = contact_fault(ctx->nb, c, body_a, body_b, mu, frame, angA, angB, rest, vbias, warm);
+            if (f) {
+                r.bad = c;
+                r.fault = f;
+                return;
+            }
+            const bool fr = mu[c] > 0, rows = body_a[c] >= 0 || body_b[c] >= 0;
+            m[c] = fr ? mu[c] : 0.0;
+            ia[c] = rows ? body_a[c] : -1;
+            ib[c] = rows ? body_b[c] : -1;
+            if (!rows) continue;
+            r.nrows += fr ? 3 : 1;
+            for (int k = 0; k < (fr ? 3 : 1); ++k) {
+                r.rest = r.rest || rest[3 * (size_t)c + k] != 0.0;
+                r.vbias = r.vbias || vbias[3 * (size_t)c + k] != 0.0;
+            }
         }
-        for (int k = 0; k < 3; ++k) {
-            if (k > 0 && !fr) break;
-            any_rest = any_rest || rest[3 * (size_t)c + k] != 0.0;
-            any_vbias = any_vbias || vbias[3 * (size_t)c + k] != 0.0;
+    });
+    int nrows = 0;
+    bool any_rest = false, any_vbias = false;
+    for (const Scan& r : sc) {
+        if (r.bad >= 0) {
+            static const char* const msg[] = {"", "cdx_set_contacts: body index out of range (bodies must be set first)",
+                                              "cdx_set_contacts: non-finite mu", "cdx_set_contacts: non-finite contact frame",
+                                              "cdx_set_contacts: non-finite contact data"};
+            return set_err(ctx, r.fault == 1 ? CDX_BAD_ARG : CDX_NONFINITE, msg[r.fault]);
         }
+        nrows += r.nrows;
+        any_rest = any_rest || r.rest;
+        any_vbias = any_vbias || r.vbias;
     }
     if ((size_t)nc > ctx->contact_cap || !ctx->d_ida) {
         int** ibufs[] = {&ctx->d_ida, &ctx->d_idb};
@@ -525,26 +575,28 @@ int cdx_set_contacts(cdx_ctx* ctx, int nc, const int* body_a, const int* body_b,
     ctx->nrows = nrows;
     ctx->any_rest = any_rest;
     ctx->any_bias = any_rest || any_vbias;
-    ctx->h_a = ia;
-    ctx->h_b = ib;
-    ctx->h_mu.assign(m.begin(), m.end());
+    ctx->h_a = std::move(ia);
+    ctx->h_b = std::move(ib);
+    ctx->h_mu = std::move(m);
     if (nc > 0) {
         // the caller's arrays as they are, through pinned staging in one copy; the device
         // transposes and masks them (k_contacts_soa)
         const size_t n = (size_t)nc, nd = 37 * n;
         CDX_TRY(ensure_stage(ctx, nd + n));
-        double* h = ctx->h_stage;
         const struct {
             const double* src;
-            size_t len;
-        } parts[] = {{frame, 9 * n}, {angA, 9 * n}, {angB, 9 * n}, {rest, 3 * n}, {vbias, 3 * n}, {warm, 3 * n}, {mu, n}};
-        for (const auto& part : parts) {
-            std::memcpy(h, part.src, part.len * sizeof(double));
-            h += part.len;
-        }
-        int* hi = reinterpret_cast<int*>(h);
-        std::memcpy(hi, body_a, n * sizeof(int));
-        std::memcpy(hi + n, body_b, n * sizeof(int));
+            size_t per;  // doubles per contact
+        } parts[] = {{frame, 9}, {angA, 9}, {angB, 9}, {rest, 3}, {vbias, 3}, {warm, 3}, {mu, 1}};
+        int* hi = reinterpret_cast<int*>(ctx->h_stage + nd);
+        parallel_ranges(nc, T, [&](int, int c0, int c1) {
+            double* h = ctx->h_stage;
+            for (const auto& part : parts) {
+                std::memcpy(h + part.per * c0, part.src + part.per * c0, part.per * (size_t)(c1 - c0) * sizeof(double));
+                h += part.per * n;
+            }
+            std::memcpy(hi + c0, body_a + c0, (size_t)(c1 - c0) * sizeof(int));
+            std::memcpy(hi + n + c0, body_b + c0, (size_t)(c1 - c0) * sizeof(int));
+        });
         CDX_HIP(ctx, hipMemcpyAsync(ctx->d_stage, ctx->h_stage, (nd + n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         CDX_LAUNCH(ctx, "contacts_soa", k_contacts_soa, grid_for(nc), kBlock, nc, ctx->d_dof, ctx->d_stage,
                    reinterpret_cast<const int*>(ctx->d_stage + nd), ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu,
