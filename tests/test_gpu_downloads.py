@@ -1,5 +1,7 @@
-"""GPU: the state downloads and the device finiteness count.  cdx_get_velocities /
-cdx_get_positions / cdx_get_impulses pack the stacked layout (sum(dof) twists, 7 or 3 pose entries
+"""GPU: the state transfers and the device finiteness count.  cdx_set_contacts checks the
+contacts over contiguous chunks in host threads (from 32,768 contacts) and stages them in pinned
+memory; the first failing contact still names the error, whichever chunk it lies in.
+cdx_get_velocities / cdx_get_positions / cdx_get_impulses pack the stacked layout (sum(dof) twists, 7 or 3 pose entries
 per body, [nc][3] impulses) on the device and copy it through a pinned staging buffer that grows
 with the problem; cdx_nonfinite_bodies counts the bodies whose solved twist or pose holds a NaN or
 Inf.  Mixed rigid bodies and point masses; a context that solves a large, a small and a large
@@ -84,5 +86,41 @@ def test_nonfinite_count_matches_the_downloaded_state():
         bad = per_body_bad(prob.dof, v, q)
         assert 0 < bad.sum() < len(bad)
         assert c.nonfinite_bodies() == int(bad.sum())
+    finally:
+        c.close()
+
+
+def test_threaded_contact_checks_name_the_first_failure():
+    # ten copies of a 7,000-contact problem side by side (70,000 contacts: two host threads' chunks)
+    prob = scenes.concat_problems([problem(2000, 7000, 7)] * 10)[0]
+    assert prob.nc == 70000
+    c = native.Context(0)
+    try:
+        c.set_bodies(prob.dof, prob.minv, prob.v, prob.fext, prob.fgyro, prob.q)
+        base = [np.array(a, copy=True) for a in prob.contact_arrays()]
+        cases = [
+            # (contact, array index, entry, value) planted in order; the lowest contact's check fails
+            ([(60000, 3, 2, np.nan), (65000, 0, None, prob.nb + 5)], "non-finite contact frame"),
+            ([(100, 0, None, prob.nb + 5), (50000, 2, None, np.inf)], "body index out of range"),
+            ([(69999, 8, 1, np.inf), (40000, 2, None, np.nan)], "non-finite mu"),
+            ([(1234, 6, 0, np.inf)], "non-finite contact data"),
+        ]
+        for plants, msg in cases:
+            arrs = [a.copy() for a in base]
+            for cc, k, e, val in plants:
+                if e is None:
+                    arrs[k][cc] = val
+                else:
+                    arrs[k][cc, e] = val
+            with pytest.raises(native.CdxError) as err:
+                c.set_contacts(*arrs)
+            assert msg in str(err.value)
+        # the rejected calls leave a context that takes the valid contacts and solves them
+        c.set_contacts(*base)
+        c.pj_solve(1e-3, 1.0, params(max_iter=30, tol_abs=1e-15, tol_rel=1e-15))
+        st, ov, oimp, res = oracle.pj_solve(prob, 1e-3, oracle.params(max_iter=30, tol_abs=1e-15, tol_rel=1e-15))
+        assert st == 0
+        assert np.abs(c.velocities() - ov).max() / np.abs(ov).max() <= 1e-10
+        assert np.abs(c.impulses() - oimp).max() / np.abs(oimp).max() <= 1e-10
     finally:
         c.close()
