@@ -528,8 +528,7 @@ __device__ inline void contact_sweep_lds(int c, int c0, int b0, int nc, const in
 
 // pj_body_sweep (no joints) on the LDS copies: the scene's incidence lists with local offsets
 // (ip, body bi at ip[bi - b0]) and local entries e - 2 c0 (ix), which read rs at 9 (c - c0).
-__device__ inline double body_sweep_lds(int bi, int b0, const int* __restrict__ dof,
-                                        const double* __restrict__ minv, const int* __restrict__ ip,
+__device__ inline double body_sweep_lds(int bi, int b0, const double4 m4, const int nd, const int* __restrict__ ip,
                                         const int* __restrict__ ix, const double* __restrict__ rs,
                                         double* __restrict__ xs, int check, double tol_rel, double tol_abs) {
     double e2 = 0;
@@ -538,8 +537,6 @@ __device__ inline double body_sweep_lds(int bi, int b0, const int* __restrict__ 
     for (int i = i0; i < i1; ++i) gather_side9(rs, ix[i], acc);
     double u[6];
     lds_body6(xs, bi - b0, u);
-    const double4 m4 = *reinterpret_cast<const double4*>(minv + (size_t)bi * kMinvStride);
-    const int nd = dof[bi];
     double un[6];
     const double Ml = m4.x > 0 ? 1.0 / m4.x : 0.0;
     const double Ma[3] = {m4.y > 0 ? 1.0 / m4.y : 0.0, m4.z > 0 ? 1.0 / m4.z : 0.0, m4.w > 0 ? 1.0 / m4.w : 0.0};
@@ -566,6 +563,89 @@ __device__ inline double body_sweep_lds(int bi, int b0, const int* __restrict__ 
     return e2;
 }
 
+// A contact's sweep inputs held in registers through a scene's sweeps (one contact per lane): its
+// rows, scaling or inverse Delassus block, bias and impulses, loaded once after the setup.
+struct ContactReg {
+    int a, b, nr;
+    double muc;
+    ContactJ j;
+    double R[3], bias[3], p[3], rb[9];
+};
+
+__device__ inline void load_contact_reg(int c, int nc, const int* __restrict__ ida, const int* __restrict__ idb,
+                                        const double* __restrict__ J, const double* __restrict__ R,
+                                        const double* __restrict__ bias, int use_bias, const double* __restrict__ mu,
+                                        const double* __restrict__ p, const double* __restrict__ Rb, ContactReg& r) {
+    r.a = ida[c];
+    r.b = idb[c];
+    r.muc = mu[c];
+    r.nr = r.muc > 0 ? 3 : 1;
+    if (r.a < 0 && r.b < 0) return;
+    load_J(J, nc, c, r.nr, r.j);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= r.nr) break;
+        r.p[k] = p[(size_t)k * nc + c];
+        r.R[k] = Rb ? 0.0 : R[(size_t)k * nc + c];
+        r.bias[k] = use_bias ? bias[(size_t)k * nc + c] : 0.0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) r.rb[3 * k + l] = Rb ? Rb[(size_t)(3 * k + l) * nc + c] : 0.0;
+    }
+}
+
+// contact_sweep_lds on the registers of load_contact_reg: the same arithmetic, the impulses kept
+// in r.p (written back after the last sweep).
+__device__ inline void contact_sweep_reg(ContactReg& r, int cl, int b0, bool block, const double* __restrict__ xs,
+                                         double* __restrict__ rs) {
+    if (r.a < 0 && r.b < 0) return;
+    const int nr = r.nr;
+    double ua[6], ub[6];
+    lds_body6(xs, r.a < 0 ? -1 : r.a - b0, ua);
+    lds_body6(xs, r.b < 0 ? -1 : r.b - b0, ub);
+    double pold[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        const double g = row_dot(r.j.d[k], r.j.aA[k], r.j.aB[k], ua, ub);
+        pold[k] = r.p[k];
+        if (block)
+            pn[k] = g + r.bias[k];
+        else
+            pn[k] = pold[k] - (r.R[k] * g + r.bias[k]);
+    }
+    if (block) {
+        const double t0 = pn[0], t1 = pn[1], t2 = pn[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nr) break;
+            const double* q = r.rb + 3 * k;
+            pn[k] = pold[k] - (nr == 1 ? q[0] * t0 : q[0] * t0 + q[1] * t1 + q[2] * t2);
+        }
+    }
+    pn[0] = fmax(pn[0], 0.0);
+    if (nr == 3) {
+        const double t1 = pn[1], t2 = pn[2];
+        const double tn2 = t1 * t1 + t2 * t2;
+        if (tn2 > 0) {
+            const double sc = fmin(1.0, (r.muc * pn[0]) / sqrt(tn2));
+            pn[1] = sc * t1;
+            pn[2] = sc * t2;
+        }
+    }
+    double dp[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= nr) break;
+        dp[k] = pn[k] - pold[k];
+        r.p[k] = pn[k];
+    }
+    write_side_impulses9(rs, cl, r.a, r.b, nr, r.j, dp);
+}
+
+// Resident: every scene has at most blockDim contacts and bodies, so each lane owns at most one
+// of each and keeps its contact's sweep inputs and its body's masses in registers through the
+// sweeps (a stagnating scene's sweep is then LDS traffic and barriers only).
+template <bool Resident>
 __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     int ns, int lds_bodies, int lds_contacts, const int* __restrict__ order, const int* __restrict__ boff, const int* __restrict__ coff,
     const int* __restrict__ snv, const int* __restrict__ srows, int nc, const int* __restrict__ ida,
@@ -602,14 +682,32 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
     bool done = !rows;  // the same in every thread: one scene per workgroup
     int iters = 0;
     double res = NAN;
+    ContactReg cr;
+    double4 bm4 = make_double4(0, 0, 0, 0);
+    int bnd = 0;
+    const bool own_c = Resident && c0 + lt < c1, own_b = Resident && b0 + lt < b1;
+    if (own_c) load_contact_reg(c0 + lt, nc, ida, idb, J, R, bias, use_bias, mu, p, Rb, cr);
+    if (own_b) {
+        bm4 = *reinterpret_cast<const double4*>(minv + (size_t)(b0 + lt) * kMinvStride);
+        bnd = dof[b0 + lt];
+    }
     for (int it = 0; it < max_iter && !done; ++it) {
-        for (int c = c0 + lt; c < c1; c += gsize)
-            contact_sweep_lds(c, c0, b0, nc, ida, idb, J, R, bias, use_bias, mu, p, xs, rs, Rb);
+        if constexpr (Resident) {
+            if (own_c) contact_sweep_reg(cr, lt, b0, Rb != nullptr, xs, rs);
+        } else {
+            for (int c = c0 + lt; c < c1; c += gsize)
+                contact_sweep_lds(c, c0, b0, nc, ida, idb, J, R, bias, use_bias, mu, p, xs, rs, Rb);
+        }
         __syncthreads();
         const int check = (it % check_every) == 0;
         double e2 = 0;
-        for (int b = b0 + lt; b < b1; b += gsize)
-            e2 += body_sweep_lds(b, b0, dof, minv, ip, ix, rs, xs, check, tol_rel, tol_abs);
+        if constexpr (Resident) {
+            if (own_b) e2 = body_sweep_lds(b0 + lt, b0, bm4, bnd, ip, ix, rs, xs, check, tol_rel, tol_abs);
+        } else {
+            for (int b = b0 + lt; b < b1; b += gsize)
+                e2 += body_sweep_lds(b, b0, *reinterpret_cast<const double4*>(minv + (size_t)b * kMinvStride), dof[b],
+                                     ip, ix, rs, xs, check, tol_rel, tol_abs);
+        }
         if (check) {
             for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
             if ((lt & 63) == 0) wsum[lt >> 6] = e2;
@@ -624,6 +722,10 @@ __global__ void __launch_bounds__(kMaxBatchBlock) k_batch_pj_lds(
         }
     }
     for (int i = lt; i < 6 * (b1 - b0); i += gsize) x[(size_t)(b0 + i / 6) * kBodyStride + i % 6] = xs[i];
+    if (own_c && (cr.a >= 0 || cr.b >= 0))
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (k < cr.nr) p[(size_t)k * nc + c0 + lt] = cr.p[k];
     if (lt == 0) {
         iters_out[s] = rows ? iters : 0;
         res_out[s] = res;
@@ -740,13 +842,19 @@ static int launch_batch(cdx_ctx* ctx, double dt, double theta, const cdx_solver_
         // stagnating scene's passes take one round each: its sweep latency sets the step time
         const int need = (std::max(ctx->scene_max_bodies, ctx->scene_max_contacts) + 63) / 64 * 64;
         const int lds_block = std::min(kMaxBatchBlock, std::max(block, need));
-        CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds, ns, lds_block, lds_bytes, ns, ctx->scene_max_bodies,
-                       ctx->scene_max_contacts, d, d + ns,
-                       d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc, ctx->d_ida, ctx->d_idb, ctx->d_J,
-                       ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias, ctx->d_p, ctx->d_imp, ctx->d_dof,
-                       ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro, ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt,
-                       prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart, prm->max_iter, prm->check_every,
-                       prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2, ctx->d_scene_res, Rb);
+#define CDX_LDS_LAUNCH(RES)                                                                                          \
+    CDX_LAUNCH_SHM(ctx, "batch_pj", k_batch_pj_lds<RES>, ns, lds_block, lds_bytes, ns, ctx->scene_max_bodies,          \
+                   ctx->scene_max_contacts, d, d + ns, d + 2 * ns + 1, d + 3 * ns + 2, d + 4 * ns + 2, ctx->nc,       \
+                   ctx->d_ida, ctx->d_idb, ctx->d_J, ctx->d_mu, ctx->d_rest, ctx->d_vb, ctx->d_R, ctx->d_bias,       \
+                   ctx->d_p, ctx->d_imp, ctx->d_dof, ctx->d_minv, ctx->d_v, ctx->d_fext, ctx->d_fgyro,                \
+                   ctx->d_incptr, ctx->d_incidx, ctx->d_x, dt, prm->alpha, ctx->any_bias ? 1 : 0, prm->warmstart,     \
+                   prm->max_iter, prm->check_every, prm->tol_rel, prm->tol_abs, ctx->d_scene_i + 5 * ns + 2,          \
+                   ctx->d_scene_res, Rb)
+        if (lds_block >= need)
+            CDX_LDS_LAUNCH(true);
+        else
+            CDX_LDS_LAUNCH(false);
+#undef CDX_LDS_LAUNCH
         return hip_check(ctx, hipGetLastError(), "batch launch");
     }
     if (ctx->njrows > 0) {
