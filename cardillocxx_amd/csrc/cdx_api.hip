@@ -7,7 +7,6 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
-#include <thread>
 
 #include "cdx_device.hpp"
 
@@ -65,22 +64,6 @@ int harvest_events(cdx_ctx* ctx) {
 // Body -> incident contact-side CSR, sides listed in contact order so the per-body sum has a
 // fixed order.  Entry value = 2*c + side.
 // Grows the device and pinned host staging buffers of the uploads and downloads to n doubles.
-// Host threads for a pass over n contacts: one below 32k, at most 8.
-static int host_threads(int n) {
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    return std::max(1, std::min({8, hw, n / 32768}));
-}
-
-// fn(t, c0, c1) over T contiguous chunks of [0, n), chunk t in thread t (chunk 0 on the caller's).
-template <class F>
-static void parallel_ranges(int n, int T, F fn) {
-    auto lo = [&](int t) { return (int)((long long)n * t / T); };
-    std::vector<std::thread> th;
-    for (int t = 1; t < T; ++t) th.emplace_back(fn, t, lo(t), lo(t + 1));
-    fn(0, 0, lo(1));
-    for (auto& x : th) x.join();
-}
-
 // 0 when contact c passes cdx_set_contacts's checks, else 1 (body index), 2 (mu), 3 (frame or
 // angular rows), 4 (restitution, bias or warm start), in the order they are tested.
 static int contact_fault(int nb, int c, const int* body_a, const int* body_b, const double* mu, const double* frame,

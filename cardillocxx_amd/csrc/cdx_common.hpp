@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cdx/cdx.h"
@@ -240,6 +241,22 @@ void prof_events(cdx_ctx* ctx, hipEvent_t* start, hipEvent_t* stop);
 void prof_push(cdx_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t stop);
 int harvest_events(cdx_ctx* ctx);
 int build_incidence(cdx_ctx* ctx);
+// Host threads for a pass over n items (contacts, joint rows): one below 32k, at most 8.
+inline int host_threads(int n) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1, std::min({8, hw, n / 32768}));
+}
+
+// fn(t, c0, c1) over T contiguous chunks of [0, n), chunk t in thread t (chunk 0 on the caller's).
+template <class F>
+void parallel_ranges(int n, int T, F fn) {
+    auto lo = [&](int t) { return (int)((long long)n * t / T); };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(fn, t, lo(t), lo(t + 1));
+    fn(0, 0, lo(1));
+    for (auto& x : th) x.join();
+}
+
 // Grows the device (d_stage) and pinned host (h_stage) staging buffers to n doubles each; the
 // host buffer is idle again once the stream has been synchronised after its last copy.
 int ensure_stage(cdx_ctx* ctx, size_t n);

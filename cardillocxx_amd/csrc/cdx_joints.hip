@@ -115,16 +115,18 @@ static int upload_joint_values(cdx_ctx* ctx, const std::vector<int>& jorig, cons
     // gathered straight into the pinned staging buffer: one DMA copy at the link's rate
     CDX_TRY(ensure_stage(ctx, n > 0 ? n : 1));
     double* jr = ctx->h_stage;
-    for (size_t i = 0; i < R; ++i) {
-        const size_t r = (size_t)jorig[i];
-        double* o = jr + i * kJRowStride;
-        for (int k = 0; k < 6; ++k) o[k] = wa[6 * r + k];
-        for (int k = 0; k < 6; ++k) o[6 + k] = wb[6 * r + k];
-        o[12] = comp[r];
-        o[13] = vsrc[r];
-        o[14] = gerr[r];
-        o[15] = lam_prev[r];
-    }
+    parallel_ranges((int)R, host_threads((int)R), [&](int, int i0, int i1) {
+        for (size_t i = (size_t)i0; i < (size_t)i1; ++i) {
+            const size_t r = (size_t)jorig[i];
+            double* o = jr + i * kJRowStride;
+            for (int k = 0; k < 6; ++k) o[k] = wa[6 * r + k];
+            for (int k = 0; k < 6; ++k) o[6 + k] = wb[6 * r + k];
+            o[12] = comp[r];
+            o[13] = vsrc[r];
+            o[14] = gerr[r];
+            o[15] = lam_prev[r];
+        }
+    });
     CDX_TRY(ensure_alloc(ctx, &ctx->d_jr, n, &ctx->jr_cap, "joint rows"));
     if (n > 0) CDX_HIP(ctx, hipMemcpyAsync(ctx->d_jr, jr, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
