@@ -109,7 +109,10 @@ int ensure_stage(cdx_ctx* ctx, size_t n) {
 static int download_stage(cdx_ctx* ctx, size_t n, double* out) {
     CDX_HIP(ctx, hipMemcpyAsync(ctx->h_stage, ctx->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     CDX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (out) std::memcpy(out, ctx->h_stage, n * sizeof(double));
+    if (out)
+        parallel_ranges((int)n, host_threads((int)(n / 8)), [&](int, int i0, int i1) {
+            std::memcpy(out + i0, ctx->h_stage + i0, (size_t)(i1 - i0) * sizeof(double));
+        });
     return CDX_OK;
 }
 
